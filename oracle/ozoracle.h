@@ -1,0 +1,134 @@
+/*
+ * ozoracle.h -- CPU oracle for the Ozaki-scheme BLAS path (TEST INFRASTRUCTURE ONLY).
+ *
+ * This is a plain-C restatement of the reference OzBLAS 1.6a CPU algorithm (the src/cpu sources) for
+ * binary64 (the <double,double,double> instantiations), plus independent exact and double-double
+ * checkers.  It exists so that tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+ * can check the HIP library against it.  Nothing in ozblas_amd/ may link, load or call it.
+ *
+ * Every routine cites the reference file:line it restates.  Memory layout of the reference's
+ * bump-allocated workspace is emulated byte-for-byte (ozo_handle.work), because a few reference
+ * paths read workspace slots they did not write in the same call (vector remainder slot,
+ * src/cpu/ozblas_split.cpp:96 vs :280); emulating the arena keeps those reads identical.
+ *
+ * Pinning (what checks each part; see tests/test_oracle_*.py):
+ *  - ozo_gemm / ozo_dot / ozo_gemv under Fsum and NearSum: the reference's own accuracy logs
+ *    script/a100/_d{gemm,dot,gemv}_*.dat (max relative error in hex and the count of entries
+ *    different from RN(exact)) regenerated from the harness inputs (testing/testing_common.cpp:718-763).
+ *  - ozo_nearsum: equals RN-to-nearest-even of the exact sum (exact big accumulator, and Python
+ *    fractions on small cases), incl. ties, cancellation, power-of-two boundaries.
+ *  - splits: sum of slices == input exactly; slice bit widths; tau rule == glibc ceil(log2).
+ *  - generators: drand48 / rand restatements == glibc drand48 / rand (ctypes).
+ *  - ozo_csrmv / ozo_cg under NearSum: closed form (correctly rounded SpMV/DOT + IEEE fma/div/sqrt).
+ *  - ozo_csrmv / ozo_cg under Fsum and the splitShift tuning of ozo_csrmv_splitA: parity unpinned
+ *    beyond the invariants above (the reference ships no CSR/CG output for a matrix available here).
+ */
+#ifndef OZORACLE_H
+#define OZORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+	/* option flags: include/ozblas.h:21-28, defaults src/cpu/ozblas.cpp:31-49 */
+	int32_t nSplitMax;
+	int32_t splitMode;
+	int32_t fastMode;
+	int32_t reproMode;
+	int32_t sumMode;
+	int32_t useBatchedGemmFlag;
+	int32_t splitEpsMode;
+	int32_t precxFlag;
+	/* runtime state: include/ozblas.h:15-17,44-47 */
+	uint64_t workSizeBytes;   /* capacity used by the fit loop (src/cpu/ozblas_gemm.cpp:59-97) */
+	uint64_t memAddr;
+	int32_t nSplitA_;
+	int32_t splitShift;
+	uint64_t memMaskSplitA;
+	/* emulated workspace arena (lazily zero-filled mapping of workSizeBytes) */
+	char *work;
+	uint64_t workCap;
+	/* exec info: include/ozblas.h:31-41 */
+	float nSplitA, nSplitB, nSplitC;
+	int32_t mbk, nbk;
+	/* CG: include/ozblas.h:49-53 */
+	int32_t trueresFlag;
+	int32_t verbose;
+	int32_t cg_numiter;
+	double *cg_verbose1;
+	double *cg_verbose2;
+	/* 0: fastMode drops one level, as the shipped src/cpu/ozblas_gemm.cpp:163-164 does;
+	   1: legacy rule maxlevel = min(nA'-1, nB'-1) over the slice counts nA', nB' without the
+	      reproMode=0 remainder slot -- the rule of src/cuda/cuozblas_gemm.cu:138 (whose split has no
+	      remainder slot) and of the line kept commented in src/cpu/ozblas_gemm.cpp_:170.  The
+	      shipped fastmode=1 logs (script/a100 _fast1_ logs, byte-identical for the CPU and CUDA
+	      builds) are reproduced only under this rule. */
+	int32_t fastModeRule;
+} ozo_handle;
+
+/* handle: src/cpu/ozblas.cpp:22-71 (defaults), with the arena emulating devWork */
+void ozo_create(ozo_handle *h, uint64_t workSizeBytes);
+void ozo_destroy(ozo_handle *h);
+ozo_handle *ozo_handle_new(uint64_t workSizeBytes);
+void ozo_handle_free(ozo_handle *h);
+
+/* building blocks */
+int32_t ozo_rho(int32_t dim, int32_t splitEpsMode);          /* src/cpu/ozblas_split.cpp:20-34 */
+int32_t ozo_tau(double max);                                 /* (short)ceil(log2(max)) :86,158 */
+int32_t ozo_pitch(int32_t n);                                /* src/cpu/ozblas_aux.cpp:264-267 */
+double ozo_nearsum(int32_t n, double *vec, int32_t ld);      /* src/cpu/ozblas_sum.cpp:82-112 */
+/* dense/vector split of a k x n column-major block into slices (src/cpu/ozblas_split.cpp:255-289);
+   returns the slice count, slices at split[lds*n*s] */
+int32_t ozo_split(ozo_handle *h, int32_t m, int32_t n, const double *in, int32_t ldi,
+                  double *out, int32_t ldo, double *split, int32_t lds, double *max);
+
+/* routines (binary64) */
+int32_t ozo_gemm(ozo_handle *h, char transA, char transB, int32_t m, int32_t n, int32_t k,
+                 double alpha, const double *A, int32_t lda, const double *B, int32_t ldb,
+                 double beta, double *C, int32_t ldc);               /* src/cpu/ozblas_gemm.cpp:3-315 */
+double ozo_dot(ozo_handle *h, int32_t n, const double *x, const double *y); /* ozblas_dot.cpp:4-39 */
+double ozo_nrm2(ozo_handle *h, int32_t n, const double *x);                /* ozblas_nrm2.cpp:4-31 */
+int32_t ozo_gemv(ozo_handle *h, char trans, int32_t m, int32_t n, double alpha, const double *A,
+                 int32_t lda, const double *x, double beta, double *y);     /* ozblas_gemv.cpp:4-29 */
+int32_t ozo_axpy(ozo_handle *h, int32_t n, double alpha, const double *x, double *y); /* ozblas_axpy.cpp */
+int32_t ozo_csrmv(ozo_handle *h, int32_t m, int32_t n, int32_t nnz, double alpha, const double *val,
+                  const int32_t *colind, const int32_t *rowptr, const double *x, double beta,
+                  double *y);                                       /* ozblas_csrmv.cpp:4-181 */
+double *ozo_csrmv_splitA(ozo_handle *h, int32_t m, int32_t n, int32_t nnz, const double *val,
+                         const int32_t *rowptr);                    /* ozblas_csrmv.cpp:192-271 */
+int32_t ozo_cg(ozo_handle *h, int32_t n, int32_t nnz, const double *val, const int32_t *colind,
+               const int32_t *rowptr, const double *b, double *x, int32_t maxiter, double tol);
+                                                                    /* ozblas_cg.cpp:24-128 */
+
+/* independent checkers (not restatements) */
+double ozo_exact_dot_rn(int64_t n, const double *x, int64_t incx, const double *y, int64_t incy);
+void ozo_exact_gemm_rn(char transA, char transB, int32_t m, int32_t n, int32_t k,
+                       const double *A, int32_t lda, const double *B, int32_t ldb,
+                       double *C, int32_t ldc, int32_t nthreads);
+/* RN(exact (op(A) op(B))[rows[e], cols[e]]) for a list of entries */
+void ozo_exact_gemm_rn_entries(char transA, char transB, int32_t k, const double *A, int32_t lda,
+                               const double *B, int32_t ldb, int64_t nent, const int32_t *rows,
+                               const int32_t *cols, double *out);
+void ozo_dd_dot(int64_t n, const double *x, int64_t incx, const double *y, int64_t incy,
+                double *hi, double *lo);
+double ozo_exact_sum_rn(int64_t n, const double *v);
+
+/* generators: testing/testing_common.cpp:664-806 */
+void ozo_srand48(int64_t seed);
+double ozo_drand48(void);
+void ozo_srand(uint32_t seed);
+int32_t ozo_rand(void);
+void ozo_init_mat(int32_t m, int32_t n, int32_t ld, double *mat, double phi, int32_t mode,
+                  int32_t seed);
+/* 3-D 7-point Laplacian on g^3 (+shift on the diagonal), CSR, rows lexicographic (i fastest) */
+int64_t ozo_laplace3d_nnz(int32_t g);
+void ozo_laplace3d(int32_t g, double shift, int32_t *rowptr, int32_t *colind, double *val);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
