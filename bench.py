@@ -1,0 +1,164 @@
+"""Headline benchmark: Ozaki DGEMM effective TFLOPS (2mnk/t) on one MI355X.
+
+Workload (one "step"): one Ozaki-scheme DGEMM C = A*B, m = n = k = 8192, binary64 inputs from the
+reference harness's phi-mode generator (seeds 1 and 2), split degree auto (nSplitMax = 0),
+NearSum (sumMode = 1), alpha = 1, beta = 0, computed with INT8 digit slices on INT32-accumulating
+MFMA.  The same product in the plain FP64-slice form (FP64 MFMA) is timed beside it and the two
+outputs are compared bit for bit.  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line (see the field list at the bottom).  Extra fields: the FP64-slice time and
+the speedup, the slice counts, the phi = 4 variant, the dominant kernel's roofline (MFMA), and the
+CPU baseline: the oracle restatement of the reference CPU OzBLAS timed on this host's cores on a
+bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+INT8_PEAK_TOPS = 5000.0   # MI355X dense INT8 MFMA, TOPS
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 MFMA
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--phi", type=float, default=0.5)
+    ap.add_argument("--phi2", type=float, default=4.0, help="second phi value (0 = skip)")
+    ap.add_argument("--form", default="int8", choices=["int8", "fp16", "bf16"])
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--cpu-n", type=int, default=1536, help="CPU baseline sample size (0 = skip)")
+    ap.add_argument("--work-gb", type=float, default=64.0)
+    return ap.parse_args()
+
+
+def timed_gemm(H, n, dA, dB, dC, steps, warmup, torch):
+    for _ in range(warmup):
+        H.dgemm("N", "N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n)
+    torch.cuda.synchronize()
+    tk = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        H.dgemm("N", "N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n)
+        tk.append(H.t_comp)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return (t1 - t0) / steps, sum(tk) / len(tk), dict(nSplitA=H.nSplitA, nSplitB=H.nSplitB,
+                                                      nSplitC=H.nSplitC, t_split=H.t_SplitA + H.t_SplitB,
+                                                      t_sum=H.t_sum)
+
+
+def cpu_baseline(n_cpu, phi):
+    """Oracle restatement of the reference CPU OzBLAS (plain FP64 slices, NearSum) on this host."""
+    import numpy as np
+    from oracle import oracle as O
+    A = O.init_mat(n_cpu, n_cpu, n_cpu, phi, 2, 1)
+    B = O.init_mat(n_cpu, n_cpu, n_cpu, phi, 2, 2)
+    C = np.zeros(n_cpu * n_cpu)
+    h = O.Handle(1 << 34, sumMode=1, reproMode=1)
+    t0 = time.perf_counter()
+    h.gemm("N", "N", n_cpu, n_cpu, n_cpu, 1.0, A, n_cpu, B, n_cpu, 0.0, C, n_cpu)
+    t = time.perf_counter() - t0
+    cores = len(os.sched_getaffinity(0))
+    return {"value": 2.0 * n_cpu ** 3 / t / 1e12, "unit": "TFLOPS (2mnk/t)", "cores": cores,
+            "kind": "port",
+            "sample": f"one plain-form Ozaki DGEMM m=n=k={n_cpu}, phi={phi}, NearSum, degree auto "
+                      f"({h.nSplitA:.0f}x{h.nSplitB:.0f} FP64 slices), oracle C restatement with "
+                      f"OpenMP on {cores} threads, {t:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the sharded design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import ozblas_amd
+
+    n = args.n
+    work = int(args.work_gb * (1 << 30))
+    res = {}
+    outs = {}
+    for phi in [args.phi] + ([args.phi2] if args.phi2 else []):
+        A = ozblas_amd.gen_phi(n, n, n, phi, 1)
+        B = ozblas_amd.gen_phi(n, n, n, phi, 2)
+        dA = torch.from_numpy(A).to("cuda:0")
+        dB = torch.from_numpy(B).to("cuda:0")
+        del A, B
+        dC = torch.zeros(n * n, dtype=torch.float64, device="cuda:0")
+        Hf = ozblas_amd.Handle(work, sumMode=1, reproMode=1, sliceType=args.form)
+        steps = args.steps if phi == args.phi else max(1, args.steps // 2)
+        t_fast, tk_fast, info_fast = timed_gemm(Hf, n, dA, dB, dC, steps, args.warmup, torch)
+        fast_bits = dC.clone()
+        Hf.close()
+        dC.zero_()
+        H64 = ozblas_amd.Handle(work, sumMode=1, reproMode=1, sliceType="fp64")
+        t_64, tk_64, info_64 = timed_gemm(H64, n, dA, dB, dC, args.fp64_steps, 1, torch)
+        H64.close()
+        same = bool(torch.equal(fast_bits.view(torch.int64), dC.view(torch.int64)))
+        res[phi] = dict(t_fast=t_fast, tk_fast=tk_fast, info_fast=info_fast, t_64=t_64,
+                        tk_64=tk_64, info_64=info_64, same=same, steps=steps)
+        print(f"phi={phi}: {args.form} {t_fast*1e3:.1f} ms/step (MFMA kernel {tk_fast*1e3:.1f} ms, "
+              f"{info_fast['nSplitA']:.0f}x{info_fast['nSplitB']:.0f} digit planes), fp64-slice "
+              f"{t_64*1e3:.1f} ms ({info_64['nSplitA']:.0f}x{info_64['nSplitB']:.0f} slices), "
+              f"bitwise equal: {same}", file=sys.stderr, flush=True)
+        del dA, dB, dC, fast_bits
+        torch.cuda.empty_cache()
+
+    r = res[args.phi]
+    flops = 2.0 * n ** 3
+    value = flops / r["t_fast"] / 1e12
+    # dominant kernel: the digit-plane MFMA kernel; algorithmic int8 ops per launch =
+    # 2 m n k per digit pair (nSplitC = number of pairs)
+    ops = 2.0 * n * n * n * r["info_fast"]["nSplitC"]
+    achieved = ops / r["tk_fast"] / 1e12
+    roof = {"bound": "mfma", "achieved": round(achieved, 1), "peak": INT8_PEAK_TOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": None,
+            "kernel": "k_digit_gemm_i8" if args.form == "int8" else "k_digit_gemm_h",
+            "note": "int8 ops (TOPS); ops per launch = 2*n^3*pairs"}
+    cpu = cpu_baseline(args.cpu_n, args.phi) if args.cpu_n > 0 else None
+    extra = {}
+    for phi, rr in res.items():
+        extra[f"phi{phi:g}"] = {
+            f"{args.form}_tflops": round(flops / rr["t_fast"] / 1e12, 3),
+            "fp64_slice_tflops": round(flops / rr["t_64"] / 1e12, 3),
+            "speedup_vs_fp64_slice": round(rr["t_64"] / rr["t_fast"], 2),
+            "bitwise_equal_to_fp64_slice_form": rr["same"],
+            "digit_planes": [rr["info_fast"]["nSplitA"], rr["info_fast"]["nSplitB"]],
+            "digit_pairs": rr["info_fast"]["nSplitC"],
+            "fp64_slices": [rr["info_64"]["nSplitA"], rr["info_64"]["nSplitB"]],
+            "fp64_mfma_kernel_ms": round(rr["tk_64"] * 1e3, 2),
+            "fp64_mfma_frac": round(2.0 * n ** 3 * rr["info_64"]["nSplitC"] / rr["tk_64"] / 1e12
+                                    / FP64_PEAK_TFLOPS, 4),
+            f"{args.form}_mfma_kernel_ms": round(rr["tk_fast"] * 1e3, 2),
+            "split_ms": round(rr["info_fast"]["t_split"] * 1e3, 2),
+            "reconstruct_ms": round(rr["info_fast"]["t_sum"] * 1e3, 2),
+        }
+    line = {
+        "metric": "Ozaki DGEMM eff. TFLOPS (2mnk/t): FP16/INT8-slice vs FP64-slice, same bits",
+        "value": round(value, 3), "unit": "TFLOPS", "n_gpus": 1, "steps": r["steps"],
+        "warmup": args.warmup, "ms_per_step": round(r["t_fast"] * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.form, "data": "synthetic (reference phi-mode generator, seeds 1/2)",
+        "config": {"workload": f"Ozaki DGEMM m=n=k={n}, phi={args.phi:g}, degree auto, NearSum, "
+                               f"{args.form} digit slices", "m": n, "n": n, "k": n,
+                   "phi": args.phi, "sumMode": 1, "parallelism": "single GPU"},
+        "roofline": roof, "cpu_baseline": cpu, "results": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

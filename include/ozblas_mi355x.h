@@ -1,0 +1,172 @@
+/*
+ * ozblas_mi355x.h -- C ABI of the MI355X (gfx950) Ozaki-scheme BLAS.
+ *
+ * Drop-in for the OzBLAS GPU path: the handle keeps the fields of cuozblasHandle_t
+ * (reference include/cuozblas.h:7-63) with the same names and meanings; the cuBLAS/cuSPARSE handles
+ * are replaced by a HIP stream and a library-private backend pointer.  Entry points are the
+ * binary64 (D) instantiations of the reference templates, given C names:
+ *
+ *   reference (include/cuozblas.h)                          here
+ *   cuozblasCreate / cuozblasDestroy              :67-68    ozblasCreate / ozblasDestroy
+ *   cuozblasRdot<double,double,double>            :75       ozblasDdot
+ *   cuozblasRnrm2<double,double,double>           :73       ozblasDnrm2
+ *   cuozblasRaxpy<double>                         :74       ozblasDaxpy
+ *   cuozblasRgemv<double,double,double>           :76       ozblasDgemv
+ *   cuozblasRgemm<double,double,double>           :77       ozblasDgemm
+ *   cuozblasRcsrmv<double,double>                 :78       ozblasDcsrmv
+ *   cuozblasRcsrmvSplitA<double,double>           :79       ozblasDcsrmvSplitA
+ *   cuozblasRcg<double,double>                    :80       ozblasDcg
+ *
+ * Semantics follow cuOzBLAS: matrices and vectors are device pointers (column-major, 0-based
+ * CSR), alpha/beta are host scalars, DOT/NRM2 write their result to a host pointer, every call is
+ * synchronous on the handle's stream.  Numerical semantics (split, slice counts, summation order,
+ * the epilogue fma(alpha, t, beta*C) that reads C even for beta == 0) follow the reference CPU
+ * OzBLAS (src/cpu), so with sliceType = OZBLAS_SLICE_FP64 the output is bitwise identical to it.
+ * Errors are returned as OZBLAS_STATUS_* codes instead of exit(1).
+ */
+#ifndef OZBLAS_MI355X_H
+#define OZBLAS_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+	OZBLAS_STATUS_SUCCESS = 0,
+	OZBLAS_STATUS_NOT_INITIALIZED = 1,
+	OZBLAS_STATUS_ALLOC_FAILED = 2,
+	OZBLAS_STATUS_INVALID_VALUE = 3,
+	OZBLAS_STATUS_NOT_SUPPORTED = 4,
+	OZBLAS_STATUS_WORKSPACE = 5,   /* workspace too small even after m/n blocking */
+	OZBLAS_STATUS_EXECUTION_FAILED = 6
+};
+
+/* slice formats of the products (no reference counterpart).  FP64 is the reference's plain form.
+ * INT8/FP16/BF16 cut every operand into exact 8-bit digit planes whose products are exact on
+ * INT8 (INT32 accumulate) or FP16/BF16 (FP32 accumulate) MFMA, and round the exact sum once: they
+ * give the bits of sumMode = 1 (NearSum) whenever the FP64 split is exact (nSplitMax = 0). */
+enum {
+	OZBLAS_SLICE_FP64 = 0,
+	OZBLAS_SLICE_INT8 = 1,
+	OZBLAS_SLICE_FP16 = 2,
+	OZBLAS_SLICE_BF16 = 3
+};
+
+typedef struct ozblasHandle {
+	/* backend: replaces cublasHandle_t ch / cusparseHandle_t csh (include/cuozblas.h:8-9) */
+	void *stream;  /* hipStream_t all work is issued on (NULL = default stream) */
+	void *backend; /* library-private state */
+
+	/* core (include/cuozblas.h:11-18) */
+	char *devWork;
+	char *devWorkCommon;
+	uint64_t workSizeBytes;
+	uint64_t memAddr;
+	int32_t nSplitMax;
+	char *hstBatchAddr;
+	char *devBatchAddr;
+
+	int32_t initialized;
+
+	/* option flags (include/cuozblas.h:22-28) */
+	int32_t splitMode;
+	int32_t fastMode;
+	int32_t reproMode;
+	int32_t sumMode;
+	int32_t useBatchedGemmFlag;
+	int32_t splitEpsMode;
+	int32_t precxFlag;
+
+	/* exec info (include/cuozblas.h:31-42) */
+	float nSplitA;
+	float nSplitB;
+	float nSplitC;
+	float t_SplitA;
+	float t_SplitB;
+	float t_comp;
+	float t_sum;
+	float t_total;
+	float n_comp;
+	int32_t mbk;
+	int32_t nbk;
+
+	/* SpMV in iterative solvers (include/cuozblas.h:45-47) */
+	int32_t nSplitA_;
+	int32_t splitShift;
+	uint64_t memMaskSplitA;
+
+	/* CG (include/cuozblas.h:50-61): cg_verbose1 is a host array of >= maxiter/verbose + 2 */
+	int32_t trueresFlag;
+	int32_t verbose;
+	int32_t cg_numiter;
+	void *cg_verbose1;
+	void *cg_verbose2;
+	double t_SplitMat_total;
+	double t_SplitVec_total;
+	double t_Sum_total;
+	double t_AXPY_SCAL_total;
+	double t_DOT_NRM2_total;
+	double t_SpMV_SpMM_total;
+
+	/* MI355X additions (no reference counterpart) */
+	int32_t sliceType;    /* OZBLAS_SLICE_*, default OZBLAS_SLICE_FP64 */
+	int32_t lastStatus;   /* status of the last call (also returned by it) */
+	int32_t nLevelBins;   /* fast forms: number of exact level planes of the last GEMM */
+	int32_t reserved[5];
+} ozblasHandle_t;
+
+/* lifecycle: cuozblasCreate/cuozblasDestroy (include/cuozblas.h:67-68, src/cuda/cuozblas.cpp:23-75).
+ * Defaults as src/cpu/ozblas.cpp:31-49 (reproMode=1, sumMode=0, nSplitMax=0, ...). */
+int32_t ozblasCreate(ozblasHandle_t *oh, uint64_t workSizeBytes);
+int32_t ozblasDestroy(ozblasHandle_t *oh);
+int32_t ozblasSetStream(ozblasHandle_t *oh, void *hipStream);
+
+/* level 1: cuozblasRdot / cuozblasRnrm2 / cuozblasRaxpy (include/cuozblas.h:73-75) */
+int32_t ozblasDdot(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx, const double *y,
+                   int32_t incy, double *ret);
+int32_t ozblasDnrm2(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx, double *ret);
+int32_t ozblasDaxpy(ozblasHandle_t *oh, int32_t n, double alpha, const double *x, int32_t incx,
+                    double *y, int32_t incy);
+
+/* level 2/3: cuozblasRgemv / cuozblasRgemm (include/cuozblas.h:76-77) */
+int32_t ozblasDgemv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, double alpha,
+                    const double *A, int32_t lda, const double *x, int32_t incx, double beta,
+                    double *y, int32_t incy);
+int32_t ozblasDgemm(ozblasHandle_t *oh, char transA, char transB, int32_t m, int32_t n, int32_t k,
+                    double alpha, const double *A, int32_t lda, const double *B, int32_t ldb,
+                    double beta, double *C, int32_t ldc);
+
+/* sparse: cuozblasRcsrmv / cuozblasRcsrmvSplitA / cuozblasRcg (include/cuozblas.h:78-80).
+ * descrA is accepted and ignored (general matrix, 0-based indices). */
+int32_t ozblasDcsrmv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, int32_t nnz,
+                     double alpha, const void *descrA, const double *A, const int32_t *colind,
+                     const int32_t *rowptr, const double *x, double beta, double *y);
+double *ozblasDcsrmvSplitA(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, int32_t nnz,
+                           const void *descrA, const double *A, const int32_t *rowptr);
+int32_t ozblasDcg(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz, const void *descrA,
+                  const double *A, const int32_t *colind, const int32_t *rowptr, const double *b,
+                  double *x, int32_t maxiter, double tol);
+
+/* batched uneven GEMM (no reference counterpart; the reference batches only the slice pairs of one
+ * GEMM, src/cuda/cuozblas_gemm.cu:150-190).  Arrays of per-problem sizes and device pointers live
+ * in host memory.  Every C[p] is bitwise equal to ozblasDgemm on problem p alone. */
+int32_t ozblasDgemmBatched(ozblasHandle_t *oh, int32_t batch, const char *transA,
+                           const char *transB, const int32_t *m, const int32_t *n,
+                           const int32_t *k, const double *alpha, const double *const *A,
+                           const int32_t *lda, const double *const *B, const int32_t *ldb,
+                           const double *beta, double *const *C, const int32_t *ldc);
+
+/* input generator of the reference test harness, phi-mode (testing/testing_common.cpp:703-717):
+ * (r1 - 0.5) * exp(phi * sqrt(-2 ln r1) sin(2 pi r2)) from drand48 seeded with `seed`, written
+ * column-major into the HOST buffer A (ld >= m) -- no reference counterpart in the library API */
+void ozblasGenPhi(int32_t m, int32_t n, int32_t ld, double *A, double phi, int64_t seed);
+
+/* library info */
+const char *ozblasGetVersionString(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

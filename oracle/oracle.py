@@ -54,6 +54,8 @@ def lib():
         L.ozo_rho.argtypes = [C.c_int32, C.c_int32]
         L.ozo_tau.restype = C.c_int32
         L.ozo_tau.argtypes = [C.c_double]
+        L.ozo_tau_batch.argtypes = [C.c_int64, _dp, np.ctypeslib.ndpointer(dtype=np.int32,
+                                                                         flags="C_CONTIGUOUS")]
         L.ozo_pitch.restype = C.c_int32
         L.ozo_pitch.argtypes = [C.c_int32]
         L.ozo_nearsum.restype = C.c_double
@@ -184,6 +186,13 @@ def rho(dim, mode=0):
 
 def tau(x):
     return lib().ozo_tau(float(x))
+
+
+def tau_batch(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(len(x), dtype=np.int32)
+    lib().ozo_tau_batch(len(x), x, out)
+    return out
 
 
 def pitch(n):

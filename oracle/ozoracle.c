@@ -125,6 +125,10 @@ int32_t ozo_tau(double max) { /* (short)ceil(log2(fabs(max))) :86,158,808 */
 	return (int32_t)(int16_t)d2i(c);
 }
 
+void ozo_tau_batch(int64_t n, const double *x, int32_t *out) {
+	for (int64_t i = 0; i < n; i++) out[i] = ozo_tau(x[i]);
+}
+
 static double sigma_of(int32_t rho, int32_t tau, int32_t shift) { /* :88,159 */
 	return SPLIT_CONST * scalbn_l(1., (int64_t)rho + tau) * shift;
 }

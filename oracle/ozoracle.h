@@ -79,6 +79,7 @@ void ozo_handle_free(ozo_handle *h);
 /* building blocks */
 int32_t ozo_rho(int32_t dim, int32_t splitEpsMode);          /* src/cpu/ozblas_split.cpp:20-34 */
 int32_t ozo_tau(double max);                                 /* (short)ceil(log2(max)) :86,158 */
+void ozo_tau_batch(int64_t n, const double *x, int32_t *out);
 int32_t ozo_pitch(int32_t n);                                /* src/cpu/ozblas_aux.cpp:264-267 */
 double ozo_nearsum(int32_t n, double *vec, int32_t ld);      /* src/cpu/ozblas_sum.cpp:82-112 */
 /* dense/vector split of a k x n column-major block into slices (src/cpu/ozblas_split.cpp:255-289);

@@ -1,0 +1,854 @@
+// oz_api.hip -- the C ABI drivers: ozblasD{gemm,gemv,dot,nrm2,axpy,csrmv,csrmvSplitA,cg}.
+//
+// Control flow restates the reference drivers (src/cpu/ozblas_gemm.cpp, ozblas_dot.cpp,
+// ozblas_nrm2.cpp, ozblas_gemv.cpp, ozblas_csrmv.cpp, ozblas_cg.cpp, ozblas_axpy.cpp); the work is
+// done by the gfx950 kernels of this library.  Differences that cannot change a result bit:
+//  - slice counts are measured before the slices are stored, so only the slots in use are
+//    allocated (the reference reserves nSplitMax+1 slots per operand and L^2 product planes);
+//  - m/n blocking happens only when the workspace cannot hold the whole problem;
+//  - the Fsum summation is fused into the FP64 product kernel when the reference's summation
+//    index map is the identity (always, except a GEMM whose column block has width 1).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "oz_kernels.hpp"
+
+namespace oz {
+
+static inline bool is_n(char t) { return t == 'N' || t == 'n'; }
+static inline int64_t rup(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+static inline int imin(int a, int b) { return a < b ? a : b; }
+static inline int imax(int a, int b) { return a > b ? a : b; }
+
+struct WallTimer {
+	std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+	float sec() const {
+		return std::chrono::duration<float>(std::chrono::steady_clock::now() - t0).count();
+	}
+};
+
+// elapsed seconds between two events recorded on the handle's stream (after a sync)
+static float ev_sec(hipEvent_t a, hipEvent_t b) {
+	float ms = 0.f;
+	if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+	return ms * 1e-3f;
+}
+
+static int32_t sync(ozblasHandle_t *oh) {
+	OZ_HIP(hipStreamSynchronize(be(oh)->stream));
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+// cap on slices of the reference split loop (src/cpu/ozblas_split.cpp:275-277)
+static inline int split_cap(const ozblasHandle_t *oh) {
+	int maxS = (oh->nSplitMax > 0) ? oh->nSplitMax : kNumSplitDefaultMax;
+	if (oh->reproMode == 0) maxS--;
+	return maxS;
+}
+
+// reference slice count from the per-column first-zero index (max over columns)
+static inline int split_count(const ozblasHandle_t *oh, int zmax) {
+	const int cap = split_cap(oh);
+	int count;
+	if (cap <= 1)
+		count = 1;
+	else if (zmax < cap)
+		count = imax(zmax, 1) + 1;
+	else
+		count = cap;
+	if (oh->reproMode == 0) count++;
+	return count;
+}
+
+// ---------------------------------------------------------------------------------------------
+// dense column split of a k-contiguous operand (k x cols, ld) into slices (ldk x cols per slot)
+// ---------------------------------------------------------------------------------------------
+struct DenseSplit {
+	double *slices = nullptr;
+	int64_t ldk = 0;      // slice column pitch
+	int64_t sstride = 0;  // slice stride
+	int count = 0;
+};
+
+static int32_t dense_split(ozblasHandle_t *oh, int k, int cols, const double *in, int64_t ldi,
+                           double *rem, int64_t ldr, DenseSplit &ds) {
+	Backend *b = be(oh);
+	const int cap = split_cap(oh);
+	const int rho = rho_host(k, oh->splitEpsMode);
+	ds.ldk = rup(k, 16);
+	ds.sstride = ds.ldk * cols;
+	int *zcol = (int *)walloc(oh, sizeof(int) * (size_t)cols);
+	if (!zcol) return OZBLAS_STATUS_WORKSPACE;
+	// the store pass needs the count; measure it first without storing (slices -> rem scratch)
+	OZ_HIP(hipMemsetAsync(b->dCounters, 0, sizeof(int), b->stream));
+	const int capMeasure = cap;
+	// counting pass: slices land in a scratch of one slot that is overwritten (sstride = 0)
+	double *scratch = (double *)walloc(oh, sizeof(double) * (size_t)ds.ldk * cols * 2);
+	if (!scratch) return OZBLAS_STATUS_WORKSPACE;
+	launch_split_cols(b->stream, k, cols, in, ldi, rem, ldr, scratch, ds.ldk, 0, rho,
+	                  oh->splitShift, capMeasure, b->dTau, zcol, b->dCounters);
+	OZ_HIP(hipMemcpyAsync(b->hPinned, b->dCounters, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+	OZ_TRY(sync(oh));
+	const int zmax = (int)((int32_t *)b->hPinned)[0];
+	ds.count = split_count(oh, zmax);
+	// store pass: count slots (+1 for the remainder slot the kernel writes after the last slice)
+	const int slots = imax(ds.count, imin(zmax, cap) + 2);
+	ds.slices = (double *)walloc(oh, sizeof(double) * (size_t)ds.sstride * slots);
+	if (!ds.slices) return OZBLAS_STATUS_WORKSPACE;
+	launch_split_cols(b->stream, k, cols, in, ldi, rem, ldr, ds.slices, ds.ldk, ds.sstride, rho,
+	                  oh->splitShift, cap, b->dTau, zcol, b->dCounters + 1);
+	launch_zero_slots(b->stream, k, cols, ds.slices, ds.ldk, ds.sstride, zcol, ds.count);
+	// zero the pad rows [k, ldk) of every slot the product kernels read
+	if (ds.ldk > k) {
+		for (int s = 0; s < ds.count; s++)
+			OZ_HIP(hipMemset2DAsync(ds.slices + (size_t)s * ds.sstride + k, ds.ldk * sizeof(double), 0,
+			                        (ds.ldk - k) * sizeof(double), cols, b->stream));
+	}
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------------------------
+// slice count of a k-contiguous operand without storing slices (for the blocking decision)
+// ---------------------------------------------------------------------------------------------
+static int32_t count_cols(ozblasHandle_t *oh, int k, int cols, const double *in, int64_t ldi,
+                          int *count) {
+	Backend *b = be(oh);
+	const uint64_t mark = wmark(oh);
+	const int64_t ldk = rup(k, 16);
+	double *rem = (double *)walloc(oh, 8ull * ldk * cols);
+	double *scratch = (double *)walloc(oh, 8ull * ldk * cols);
+	int *zcol = (int *)walloc(oh, sizeof(int) * (size_t)cols);
+	if (!rem || !scratch || !zcol) return OZBLAS_STATUS_WORKSPACE;
+	OZ_HIP(hipMemsetAsync(b->dCounters, 0, sizeof(int), b->stream));
+	launch_split_cols(b->stream, k, cols, in, ldi, rem, ldk, scratch, ldk, 0,
+	                  rho_host(k, oh->splitEpsMode), oh->splitShift, split_cap(oh), b->dTau, zcol,
+	                  b->dCounters);
+	OZ_HIP(hipMemcpyAsync(b->hPinned, b->dCounters, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+	OZ_TRY(sync(oh));
+	*count = split_count(oh, ((int32_t *)b->hPinned)[0]);
+	wreset(oh, mark);
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------------------------
+// GEMM, plain form (src/cpu/ozblas_gemm.cpp:3-315)
+// ---------------------------------------------------------------------------------------------
+static int32_t gemm_plain(ozblasHandle_t *oh, char transA, char transB, int m, int n, int k,
+                          double alpha, const double *A, int lda, const double *B, int ldb,
+                          double beta, double *C, int ldc) {
+	Backend *b = be(oh);
+	hipStream_t st = b->stream;
+	const bool fsum = (oh->sumMode == 0);
+	if (oh->sumMode != 0 && oh->sumMode != 1) return OZBLAS_STATUS_NOT_SUPPORTED;
+	if (oh->splitEpsMode != 0 && oh->splitEpsMode != 9) return OZBLAS_STATUS_NOT_SUPPORTED;
+	const bool dotPath = (m == 1 && n == 1);  // :25-28: DOT ignores fastMode
+	const int fastMode = dotPath ? 0 : oh->fastMode;
+	const uint64_t base = wmark(oh);
+	const int64_t ldk = rup(k, 16);
+	WallTimer tt;
+	// whole-operand copies of op(A) / op(B) in k-contiguous form (transposed when needed)
+	const double *ain = A, *bin = B;
+	int64_t ldai = lda, ldbi = ldb;
+	if (is_n(transA)) {
+		double *tA = (double *)walloc(oh, 8ull * ldk * m);
+		if (!tA) return OZBLAS_STATUS_WORKSPACE;
+		launch_transpose(st, m, k, A, lda, tA, ldk);
+		ain = tA;
+		ldai = ldk;
+	}
+	if (!is_n(transB)) {
+		double *tB = (double *)walloc(oh, 8ull * ldk * n);
+		if (!tB) return OZBLAS_STATUS_WORKSPACE;
+		launch_transpose(st, n, k, B, ldb, tB, ldk);
+		bin = tB;
+		ldbi = ldk;
+	}
+	const uint64_t base2 = wmark(oh);
+	// blocking: slice counts of the whole operands bound those of any block, so halving the
+	// larger side (as src/cpu/ozblas_gemm.cpp:91-96) until the exact need fits is safe
+	int nAf = 0, nBf = 0;
+	OZ_TRY(count_cols(oh, k, m, ain, ldai, &nAf));
+	OZ_TRY(count_cols(oh, k, n, bin, ldbi, &nBf));
+	auto need = [&](int64_t mb, int64_t nb) -> uint64_t {
+		uint64_t bytes = 8ull * ldk * std::max(mb, nb) * 2;            // remainder + count scratch
+		bytes += 8ull * ldk * (mb * (nAf + 1) + nb * (nBf + 1));        // slices (+ remainder slot)
+		const int64_t pairs = (int64_t)nAf * nBf;
+		if (!fsum || (oh->fastMode == 0 && nb == 1)) bytes += 8ull * rup(mb, 8) * nb * pairs;
+		bytes += 8ull * (mb + nb) + 24ull * pairs + 64 * 1024;         // counters, offsets, align
+		return bytes;
+	};
+	int mbk = m, nbk = n;
+	while (need(mbk, nbk) > b->cap - base2) {
+		if (mbk == 1 && nbk == 1) return OZBLAS_STATUS_WORKSPACE;
+		if (mbk >= nbk)
+			mbk = (mbk + 1) / 2;
+		else
+			nbk = (nbk + 1) / 2;
+	}
+	oh->mbk = mbk;
+	oh->nbk = nbk;
+	double sA = 0, sB = 0, sC = 0, tcomp = 0, tsum = 0;
+	int blocks = 0;
+	for (int im = 0; im * mbk < m; im++) {
+		const int mbk_ = imin(mbk, m - im * mbk);
+		for (int in = 0; in * nbk < n; in++) {
+			const int nbk_ = imin(nbk, n - in * nbk);
+			wreset(oh, base2);
+			double *rem = (double *)walloc(oh, 8ull * ldk * std::max(mbk_, nbk_));
+			if (!rem) return OZBLAS_STATUS_WORKSPACE;
+			DenseSplit da, db;
+			OZ_TRY(dense_split(oh, k, mbk_, ain + (int64_t)im * mbk * ldai, ldai, rem, ldk, da));
+			OZ_TRY(dense_split(oh, k, nbk_, bin + (int64_t)in * nbk * ldbi, ldbi, rem, ldk, db));
+			const int nA = da.count, nB = db.count;
+			// ---- pairs (src/cpu/ozblas_gemm.cpp:163-164,237-264)
+			int maxlevel = (nA - 1) + (nB - 1);
+			maxlevel = imax(maxlevel - fastMode, imin(nA - 1, nB - 1));
+			const bool gemvPath = !dotPath && (n == 1 && fastMode == 0);  // products at nB*ia+ib
+			const int sumorder = dotPath ? 2 : ((fastMode == 0 && nbk_ == 1) ? 3 : 1);
+			std::vector<int> pia, pib;
+			for (int ik = 0; ik <= maxlevel; ik++)
+				for (int ia = 0; ia < nA; ia++) {
+					const int ib = ik - ia;
+					if (ib < 0 || ib >= nB) continue;
+					pia.push_back(ia);
+					pib.push_back(ib);
+				}
+			const int ic = (int)pia.size();
+			const int64_t ldcs = rup(mbk_, 8);
+			const int64_t llsc = ldcs * nbk_;
+			std::vector<int64_t> hoff(3 * (size_t)ic);
+			for (int p = 0; p < ic; p++) {
+				hoff[p] = (int64_t)pia[p] * da.sstride;
+				hoff[ic + p] = (int64_t)pib[p] * db.sstride;
+				int64_t it = p;                                   // GEMM: level order (:243)
+				if (gemvPath) it = (int64_t)nB * pia[p] + pib[p];  // GEMV (:221)
+				if (dotPath) it = (int64_t)nA * pib[p] + pia[p];   // DOT: C[ia + ib*nA] (:214)
+				hoff[2 * ic + p] = it * llsc;
+			}
+			int64_t *dOff = (int64_t *)walloc(oh, sizeof(int64_t) * 3 * (size_t)ic);
+			if (!dOff) return OZBLAS_STATUS_WORKSPACE;
+			OZ_HIP(hipMemcpyAsync(dOff, hoff.data(), sizeof(int64_t) * 3 * ic, hipMemcpyHostToDevice, st));
+			double *Cblk = C + (int64_t)ldc * ((int64_t)in * nbk) + (int64_t)im * mbk;
+			OZ_HIP(hipEventRecord(b->ev[3], st));
+			// fused Fsum is valid when the summation visits the products in pair order; the
+			// reference's GEMM with a width-1 column block reads them through the GEMV map instead
+			const bool identityMap = gemvPath || dotPath || sumorder == 1;
+			bool summed = false;
+			if (fsum && identityMap) {
+				launch_gemm_f64_fsum(st, mbk_, nbk_, k, da.slices, da.ldk, db.slices, db.ldk, dOff,
+				                     dOff + ic, ic, alpha, beta, Cblk, ldc);
+				OZ_HIP(hipEventRecord(b->ev[4], st));
+				summed = true;
+			} else {
+				const int64_t nplanes = (gemvPath || dotPath || !identityMap) ? std::max<int64_t>((int64_t)nA * nB, ic) : ic;
+				double *Cs = (double *)walloc(oh, 8ull * llsc * nplanes);
+				if (!Cs) return OZBLAS_STATUS_WORKSPACE;
+				if (nplanes > ic) launch_fill(st, Cs, llsc * nplanes, 0.0);
+				launch_gemm_f64_batched(st, mbk_, nbk_, k, da.slices, da.ldk, db.slices, db.ldk, Cs, ldcs,
+				                        dOff, dOff + ic, dOff + 2 * ic, ic);
+				OZ_HIP(hipEventRecord(b->ev[4], st));
+				if (fsum)
+					launch_global_fsum(st, mbk_, nbk_, nA, nB, maxlevel, sumorder, Cs, llsc, ldcs, Cblk,
+					                   ldc, alpha, beta);
+				else
+					launch_global_nearsum(st, mbk_, nbk_, ic, Cs, llsc, ldcs, Cblk, ldc, alpha, beta,
+					                      b->dTau);
+			}
+			OZ_HIP(hipEventRecord(b->ev[5], st));
+			OZ_HIP(hipGetLastError());
+			OZ_TRY(sync(oh));
+			tcomp += ev_sec(b->ev[3], b->ev[4]);
+			if (!summed) tsum += ev_sec(b->ev[4], b->ev[5]);
+			sA += nA;
+			sB += nB;
+			sC += ic;
+			blocks++;
+		}
+	}
+	oh->nSplitA = (float)(sA / blocks);
+	oh->nSplitB = (float)(sB / blocks);
+	oh->nSplitC = (float)(sC / blocks);
+	oh->t_comp = (float)tcomp;  // the FP64 MFMA product kernel(s)
+	oh->t_sum = (float)tsum;
+	oh->t_total = tt.sec();
+	oh->n_comp = (float)(2.0 * m * n * (double)k * oh->nSplitC);
+	wreset(oh, base);
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------------------------
+// GEMM, fast forms: exact digit planes (no reference counterpart; same bits as NearSum)
+// ---------------------------------------------------------------------------------------------
+struct ExpStatsH {
+	int maxS, minLsb, maxE, nonfinite, maxSpan, pad[3];
+};
+
+struct DigitOperand {
+	void *planes = nullptr;
+	int64_t ldp = 0, pstride = 0;  // in elements
+	int *colE = nullptr;
+	int S = 0;
+	ExpStatsH st{};
+};
+
+static int32_t digit_split(ozblasHandle_t *oh, int elem, int k, int cols, const double *in,
+                           int64_t ldi, DigitOperand &d) {
+	Backend *b = be(oh);
+	d.colE = (int *)walloc(oh, sizeof(int) * (size_t)cols * 2);
+	ExpStatsH *dst = (ExpStatsH *)walloc(oh, sizeof(ExpStatsH));
+	if (!d.colE || !dst) return OZBLAS_STATUS_WORKSPACE;
+	ExpStatsH init{0, INT32_MAX, INT32_MIN, 0, INT32_MIN, {0, 0, 0}};
+	OZ_HIP(hipMemcpyAsync(dst, &init, sizeof(init), hipMemcpyHostToDevice, b->stream));
+	launch_digit_stats(b->stream, k, cols, in, ldi, d.colE, d.colE + cols, dst);
+	OZ_HIP(hipMemcpyAsync(b->hPinned, dst, sizeof(ExpStatsH), hipMemcpyDeviceToHost, b->stream));
+	OZ_TRY(sync(oh));
+	std::memcpy(&d.st, b->hPinned, sizeof(ExpStatsH));
+	d.S = d.st.maxS;
+	d.ldp = rup(k, 64);
+	d.pstride = d.ldp * cols;
+	const size_t esz = (elem == 1) ? 1 : 2;
+	if (d.S > 0) {
+		d.planes = walloc(oh, esz * (size_t)d.pstride * d.S);
+		if (!d.planes) return OZBLAS_STATUS_WORKSPACE;
+		launch_digit_write(b->stream, elem, k, cols, in, ldi, d.colE, d.S, d.planes, d.ldp, d.pstride);
+	}
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+// true when the plain FP64 split of these operands is exact (so NearSum == RN(exact)) and no
+// slice product can underflow or overflow -- the precondition for the fast forms' bit identity
+static bool fast_form_valid(const ozblasHandle_t *oh, const DigitOperand &a, const DigitOperand &b,
+                            int k) {
+	if (oh->nSplitMax != 0 || oh->sumMode != 1 || oh->splitShift != 1) return false;
+	if (a.st.nonfinite || b.st.nonfinite) return false;
+	if (a.S == 0 || b.S == 0) return true;  // an all-zero operand: exact zeros either way
+	const int rho = rho_host(k, oh->splitEpsMode);
+	const int bits = 53 - rho - 1;  // every FP64 slice removes at least this many bits
+	const int cap = split_cap(oh);
+	if (bits <= 0) return false;
+	if (a.st.maxSpan > bits * (cap - 3) || b.st.maxSpan > bits * (cap - 3)) return false;
+	if (a.st.minLsb + b.st.minLsb < -1000 || a.st.maxE + b.st.maxE > 1000) return false;
+	if (a.st.minLsb < -1000 || b.st.minLsb < -1000) return false;
+	return true;
+}
+
+static int32_t gemm_fast(ozblasHandle_t *oh, char transA, char transB, int m, int n, int k,
+                         double alpha, const double *A, int lda, const double *B, int ldb,
+                         double beta, double *C, int ldc, bool *handled) {
+	Backend *b = be(oh);
+	hipStream_t st = b->stream;
+	*handled = false;
+	const int elem = (oh->sliceType == OZBLAS_SLICE_INT8) ? 1 : (oh->sliceType == OZBLAS_SLICE_FP16 ? 2 : 3);
+	const uint64_t base = wmark(oh);
+	WallTimer tt;
+	const int64_t ldk = rup(k, 16);
+	const double *ain = A, *bin = B;
+	int64_t ldai = lda, ldbi = ldb;
+	if (is_n(transA)) {
+		double *tA = (double *)walloc(oh, 8ull * ldk * m);
+		if (!tA) return OZBLAS_STATUS_WORKSPACE;
+		launch_transpose(st, m, k, A, lda, tA, ldk);
+		ain = tA;
+		ldai = ldk;
+	}
+	if (!is_n(transB)) {
+		double *tB = (double *)walloc(oh, 8ull * ldk * n);
+		if (!tB) return OZBLAS_STATUS_WORKSPACE;
+		launch_transpose(st, n, k, B, ldb, tB, ldk);
+		bin = tB;
+		ldbi = ldk;
+	}
+	DigitOperand da, db;
+	OZ_HIP(hipEventRecord(b->ev[0], st));
+	OZ_TRY(digit_split(oh, elem, k, m, ain, ldai, da));
+	OZ_HIP(hipEventRecord(b->ev[1], st));
+	OZ_TRY(digit_split(oh, elem, k, n, bin, ldbi, db));
+	OZ_HIP(hipEventRecord(b->ev[2], st));
+	if (!fast_form_valid(oh, da, db, k)) {
+		wreset(oh, base);
+		return OZBLAS_STATUS_SUCCESS;  // caller falls back to the plain form
+	}
+	*handled = true;
+	const int SA = da.S, SB = db.S;
+	if (SA == 0 || SB == 0) {  // exact zero product
+		double *zero = (double *)walloc(oh, 8);
+		(void)zero;
+		// C = fma(alpha, +0, beta*C) through the reconstruction kernel with no bins
+		int *lv = (int *)walloc(oh, sizeof(int));
+		launch_reconstruct(st, m, n, nullptr, 0, 0, lv, 0, 0, da.colE, db.colE, alpha, beta, C, ldc);
+		OZ_HIP(hipGetLastError());
+		OZ_TRY(sync(oh));
+		wreset(oh, base);
+		return OZBLAS_STATUS_SUCCESS;
+	}
+	// bins: pairs of one level whose exact int32 sum cannot overflow: npairs * kc * 2^14 < 2^31
+	const int Kpad = (int)da.ldp;
+	const int kc = Kpad <= 65536 ? Kpad : 65536;
+	const int G = std::max(1, 131071 / kc);
+	const int Lmax = SA + SB - 2;
+	std::vector<int2> pairs;
+	std::vector<LevelBin> bins;
+	std::vector<int> binLevel;
+	for (int L = 0; L <= Lmax; L++) {
+		std::vector<int2> lp;
+		for (int s = 0; s < SA; s++) {
+			const int t = L - s;
+			if (t >= 0 && t < SB) lp.push_back(make_int2(s, t));
+		}
+		for (size_t g0 = 0; g0 < lp.size(); g0 += G)
+			for (int k0 = 0; k0 < Kpad; k0 += kc) {
+				LevelBin bn;
+				bn.level = L;
+				bn.pair0 = (int)pairs.size();
+				bn.npairs = (int)std::min<size_t>(G, lp.size() - g0);
+				bn.k0 = k0;
+				bn.k1 = std::min(Kpad, k0 + kc);
+				for (int q = 0; q < bn.npairs; q++) pairs.push_back(lp[g0 + q]);
+				bins.push_back(bn);
+				binLevel.push_back(L);
+			}
+	}
+	const int nbins = (int)bins.size();
+	oh->nLevelBins = nbins;
+	int2 *dPairs = (int2 *)walloc(oh, sizeof(int2) * pairs.size());
+	LevelBin *dBins = (LevelBin *)walloc(oh, sizeof(LevelBin) * bins.size());
+	int *dLevel = (int *)walloc(oh, sizeof(int) * binLevel.size());
+	const int64_t ldt = rup(m, 64);
+	const int64_t tstride = ldt * n;
+	int32_t *T = (int32_t *)walloc(oh, sizeof(int32_t) * (size_t)tstride * nbins);
+	if (!dPairs || !dBins || !dLevel || !T) {
+		wreset(oh, base);
+		*handled = false;
+		return OZBLAS_STATUS_WORKSPACE;
+	}
+	OZ_HIP(hipMemcpyAsync(dPairs, pairs.data(), sizeof(int2) * pairs.size(), hipMemcpyHostToDevice, st));
+	OZ_HIP(hipMemcpyAsync(dBins, bins.data(), sizeof(LevelBin) * bins.size(), hipMemcpyHostToDevice, st));
+	OZ_HIP(hipMemcpyAsync(dLevel, binLevel.data(), sizeof(int) * binLevel.size(), hipMemcpyHostToDevice, st));
+	OZ_HIP(hipEventRecord(b->ev[3], st));
+	launch_digit_gemm(st, elem, m, n, Kpad, da.planes, da.ldp, da.pstride, db.planes, db.ldp,
+	                  db.pstride, dPairs, dBins, nbins, T, ldt, tstride);
+	OZ_HIP(hipEventRecord(b->ev[4], st));
+	launch_reconstruct(st, m, n, T, ldt, tstride, dLevel, nbins, Lmax, da.colE, db.colE, alpha,
+	                   beta, C, ldc);
+	OZ_HIP(hipEventRecord(b->ev[5], st));
+	OZ_HIP(hipGetLastError());
+	OZ_TRY(sync(oh));
+	oh->t_SplitA = ev_sec(b->ev[0], b->ev[1]);
+	oh->t_SplitB = ev_sec(b->ev[1], b->ev[2]);
+	oh->t_comp = ev_sec(b->ev[3], b->ev[4]);  // the digit-plane MFMA kernel alone
+	oh->t_sum = ev_sec(b->ev[4], b->ev[5]);
+	oh->nSplitA = (float)SA;
+	oh->nSplitB = (float)SB;
+	oh->nSplitC = (float)pairs.size();
+	oh->mbk = m;
+	oh->nbk = n;
+	oh->t_total = tt.sec();
+	oh->n_comp = (float)(2.0 * m * n * (double)k * pairs.size() / ((double)Kpad / kc));
+	wreset(oh, base);
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------------------------
+// vector split (ozblasSplit with one column) -> slots at stride lds
+// ---------------------------------------------------------------------------------------------
+struct VecSplit {
+	double *slices = nullptr;
+	int64_t lds = 0;
+	int count = 0;
+};
+
+static int32_t vec_split(ozblasHandle_t *oh, int n, const double *x, VecSplit &vs) {
+	Backend *b = be(oh);
+	const int cap = split_cap(oh);
+	const int rho = rho_host(n, oh->splitEpsMode);
+	vs.lds = rup(n, 32);
+	double *rem = (double *)walloc(oh, 8ull * vs.lds);
+	vs.slices = (double *)walloc(oh, 8ull * vs.lds * (cap + 2));
+	unsigned long long *maxv = (unsigned long long *)walloc(oh, 8ull * (cap + 2));
+	int *info = (int *)walloc(oh, 16);
+	if (!rem || !vs.slices || !maxv || !info) return OZBLAS_STATUS_WORKSPACE;
+	OZ_HIP(hipMemsetAsync(maxv, 0, 8ull * (cap + 2), b->stream));
+	OZ_HIP(launch_split_vec_coop(b->stream, n, x, rem, vs.slices, vs.lds, rho, oh->splitShift, cap,
+	                             oh->reproMode, b->dTau, maxv, info));
+	OZ_HIP(hipMemcpyAsync(b->hPinned, info, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+	OZ_TRY(sync(oh));
+	vs.count = ((int32_t *)b->hPinned)[0];
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+// DOT core: xs/ys already split; result to device scalar dres (fma(1, t, 0*dres))
+static int32_t dot_core(ozblasHandle_t *oh, int n, const VecSplit &xs, const VecSplit &ys,
+                        double *dres) {
+	Backend *b = be(oh);
+	const int nA = xs.count, nB = ys.count;
+	double *vals = (double *)walloc(oh, 8ull * nA * nB);
+	if (!vals) return OZBLAS_STATUS_WORKSPACE;
+	OZ_HIP(hipMemsetAsync(vals, 0, 8ull * nA * nB, b->stream));
+	launch_slice_dots(b->stream, n, xs.slices, xs.lds, nA, ys.slices, ys.lds, nB, vals);
+	const int maxlevel = (nA - 1) + (nB - 1);  // fastMode is forced off for DOT (:25-28)
+	launch_nearsum_small(b->stream, nA * nB, vals, 1.0, 0.0, dres, 2, nA, nB, maxlevel,
+	                     oh->sumMode == 0 ? 1 : 0, b->dTau);
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+static int32_t dot_impl(ozblasHandle_t *oh, int n, const double *x, const double *y, double *ret) {
+	Backend *b = be(oh);
+	if (oh->sumMode != 0 && oh->sumMode != 1) return OZBLAS_STATUS_NOT_SUPPORTED;
+	if (oh->splitEpsMode == 2) return OZBLAS_STATUS_NOT_SUPPORTED;
+	const uint64_t base = wmark(oh);
+	oh->splitShift = 1;  // counterInit (src/cpu/ozblas.cpp:8)
+	VecSplit xs, ys;
+	OZ_TRY(vec_split(oh, n, x, xs));
+	if (y == x) {
+		ys = xs;
+	} else {
+		OZ_TRY(vec_split(oh, n, y, ys));
+	}
+	double *dres = (double *)walloc(oh, 8);
+	if (!dres) return OZBLAS_STATUS_WORKSPACE;
+	OZ_HIP(hipMemsetAsync(dres, 0, 8, b->stream));
+	OZ_TRY(dot_core(oh, n, xs, ys, dres));
+	OZ_HIP(hipMemcpyAsync(b->hPinned, dres, 8, hipMemcpyDeviceToHost, b->stream));
+	OZ_TRY(sync(oh));
+	std::memcpy(ret, b->hPinned, 8);
+	oh->nSplitA = (float)xs.count;
+	oh->nSplitB = (float)ys.count;
+	oh->nSplitC = 1;
+	wreset(oh, base);
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------------------------
+// CSR (src/cpu/ozblas_csrmv.cpp)
+// ---------------------------------------------------------------------------------------------
+struct CsrSplit {
+	double *slices = nullptr;
+	int64_t lds = 0;
+	int count = 0;
+};
+
+static int32_t csr_split(ozblasHandle_t *oh, int m, int nnz, const double *val, const int *rowptr,
+                         CsrSplit &cs) {
+	Backend *b = be(oh);
+	const int cap = split_cap(oh);
+	cs.lds = rup(nnz, 32);
+	double *work = (double *)walloc(oh, 8ull * cs.lds);
+	int *zrow = (int *)walloc(oh, sizeof(int) * (size_t)m);
+	if (!work || !zrow) return OZBLAS_STATUS_WORKSPACE;
+	if (!cs.slices) {
+		cs.slices = (double *)walloc(oh, 8ull * cs.lds * (cap + 2));
+		if (!cs.slices) return OZBLAS_STATUS_WORKSPACE;
+	}
+	OZ_HIP(hipMemsetAsync(b->dCounters, 0, sizeof(int), b->stream));
+	launch_split_csr(b->stream, m, val, rowptr, work, cs.slices, cs.lds, oh->splitShift, cap, b->dTau,
+	                 zrow, b->dCounters);
+	OZ_HIP(hipMemcpyAsync(b->hPinned, b->dCounters, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+	OZ_TRY(sync(oh));
+	const int zmax = ((int32_t *)b->hPinned)[0];
+	// ozblasSplitSparse returns s+1 at the first zero without the reproMode==0 extra slot
+	// (src/cpu/ozblas_split.cpp:919-927)
+	int count;
+	if (cap <= 1)
+		count = 1 + (oh->reproMode == 0 ? 1 : 0);
+	else if (zmax < cap)
+		count = imax(zmax, 1) + 1;
+	else
+		count = cap + (oh->reproMode == 0 ? 1 : 0);
+	cs.count = count;
+	launch_zero_slots_csr(b->stream, m, rowptr, cs.slices, cs.lds, zrow, count);
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+static int32_t csrmv_core(ozblasHandle_t *oh, int m, int n, int nnz, double alpha,
+                          const CsrSplit &as, const int *colind, const int *rowptr,
+                          const VecSplit &xs, double beta, double *y) {
+	Backend *b = be(oh);
+	const int nA = as.count, nB = xs.count;
+	int maxlevel = (nA - 1) + (nB - 1);
+	maxlevel = imax(maxlevel - oh->fastMode, imin(nA - 1, nB - 1));
+	const bool fsum = oh->sumMode == 0;
+	if (fsum && oh->fastMode) return OZBLAS_STATUS_NOT_SUPPORTED;  // reference reads unwritten slots
+	if (nA <= 4 && nB <= 8) {
+		launch_csrmm_sum(b->stream, m, nnz, as.slices, as.lds, nA, colind, rowptr, xs.slices, xs.lds, nB,
+		                 maxlevel, fsum ? 1 : 0, alpha, beta, y, b->dTau);
+	} else {
+		int ic = 0;
+		for (int ia = 0; ia < imin(maxlevel + 1, nA); ia++) ic += imin(nB, maxlevel + 1 - ia);
+		const int64_t ldcs = rup(m, 8);
+		double *Cs = (double *)walloc(oh, 8ull * ldcs * ((int64_t)nA * nB > ic ? (int64_t)nA * nB : ic));
+		if (!Cs) return OZBLAS_STATUS_WORKSPACE;
+		launch_csrmm_store(b->stream, m, as.slices, as.lds, nA, colind, rowptr, xs.slices, xs.lds, nB,
+		                   maxlevel, Cs, ldcs);
+		if (fsum)
+			launch_global_fsum(b->stream, m, 1, nA, nB, maxlevel, 3, Cs, ldcs, ldcs, y, 1, alpha, beta);
+		else
+			launch_global_nearsum(b->stream, m, 1, ic, Cs, ldcs, ldcs, y, 1, alpha, beta, b->dTau);
+	}
+	(void)n;
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+}  // namespace oz
+
+using namespace oz;
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+extern "C" int32_t ozblasDgemm(ozblasHandle_t *oh, char transA, char transB, int32_t m, int32_t n,
+                               int32_t k, double alpha, const double *A, int32_t lda,
+                               const double *B, int32_t ldb, double beta, double *C, int32_t ldc) {
+	OZ_TRY(ensure_init(oh));
+	if (m < 0 || n < 0 || k < 0) return oh->lastStatus = OZBLAS_STATUS_INVALID_VALUE;
+	if (m == 0 || n == 0) return oh->lastStatus = OZBLAS_STATUS_SUCCESS;
+	oh->splitShift = 1;  // counterInit
+	if (oh->reproMode == 0 && oh->nSplitMax == 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	if (oh->sliceType != OZBLAS_SLICE_FP64) {
+		if (oh->sumMode == 1 && oh->nSplitMax == 0) {
+			bool handled = false;
+			int32_t s = gemm_fast(oh, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, &handled);
+			if (s != OZBLAS_STATUS_SUCCESS || handled) return oh->lastStatus = s;
+		} else if (oh->sumMode != 1) {
+			return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+		}
+	}
+	return oh->lastStatus = gemm_plain(oh, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+extern "C" int32_t ozblasDgemv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, double alpha,
+                               const double *A, int32_t lda, const double *x, int32_t incx,
+                               double beta, double *y, int32_t incy) {
+	OZ_TRY(ensure_init(oh));
+	if (incx != 1 || incy != 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	const int32_t saveSlice = oh->sliceType;
+	oh->sliceType = OZBLAS_SLICE_FP64;
+	int32_t s;
+	if (is_n(transA))  // src/cpu/ozblas_gemv.cpp:23-26
+		s = ozblasDgemm(oh, 'n', 'n', m, 1, n, alpha, A, lda, x, n, beta, y, m);
+	else
+		s = ozblasDgemm(oh, 't', 'n', n, 1, m, alpha, A, lda, x, m, beta, y, n);
+	oh->sliceType = saveSlice;
+	return s;
+}
+
+extern "C" int32_t ozblasDdot(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx,
+                              const double *y, int32_t incy, double *ret) {
+	OZ_TRY(ensure_init(oh));
+	if (incx != 1 || incy != 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	if (n <= 0) {
+		*ret = 0.0;
+		return oh->lastStatus = OZBLAS_STATUS_SUCCESS;
+	}
+	if (oh->reproMode == 0 && oh->nSplitMax == 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	return oh->lastStatus = dot_impl(oh, n, x, y, ret);
+}
+
+extern "C" int32_t ozblasDnrm2(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx,
+                               double *ret) {
+	OZ_TRY(ensure_init(oh));
+	double d = 0.0;
+	int32_t s = ozblasDdot(oh, n, x, incx, x, incx, &d);
+	if (s != OZBLAS_STATUS_SUCCESS) return s;
+	*ret = sqrt(d);  // host sqrt as the reference (src/cpu/ozblas_nrm2.cpp:27)
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+extern "C" int32_t ozblasDaxpy(ozblasHandle_t *oh, int32_t n, double alpha, const double *x,
+                               int32_t incx, double *y, int32_t incy) {
+	OZ_TRY(ensure_init(oh));
+	if (incx != 1 || incy != 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	if (n <= 0) return OZBLAS_STATUS_SUCCESS;
+	launch_axpy_host(be(oh)->stream, n, alpha, x, y, oh->reproMode != 0);
+	OZ_HIP(hipGetLastError());
+	return oh->lastStatus = sync(oh);
+}
+
+extern "C" double *ozblasDcsrmvSplitA(ozblasHandle_t *oh, char transA, int32_t m, int32_t n,
+                                      int32_t nnz, const void *descrA, const double *A,
+                                      const int32_t *rowptr) {
+	(void)descrA;
+	(void)n;
+	if (ensure_init(oh) != OZBLAS_STATUS_SUCCESS) return nullptr;
+	if (!is_n(transA)) {
+		oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+		return nullptr;
+	}
+	Backend *b = be(oh);
+	b->keep = 0;
+	wreset(oh, 0);
+	oh->splitShift = 1;
+	CsrSplit cs;
+	cs.lds = rup(nnz, 32);
+	cs.slices = (double *)walloc(oh, 8ull * cs.lds * (split_cap(oh) + 2));
+	if (!cs.slices) {
+		oh->lastStatus = OZBLAS_STATUS_WORKSPACE;
+		return nullptr;
+	}
+	const uint64_t keepTop = wmark(oh);
+	int32_t s = csr_split(oh, m, nnz, A, rowptr, cs);
+	int nA = cs.count;
+	// splitShift tuning (src/cpu/ozblas_csrmv.cpp:240-255)
+	if (s == OZBLAS_STATUS_SUCCESS && oh->nSplitMax == 0) {
+		int nAOld;
+		oh->splitShift = 1;
+		do {
+			nAOld = nA;
+			oh->splitShift *= 2;
+			wreset(oh, keepTop);
+			s = csr_split(oh, m, nnz, A, rowptr, cs);
+			nA = cs.count;
+		} while (s == OZBLAS_STATUS_SUCCESS && nAOld == nA && oh->splitShift < 512);
+		if (nAOld == nA) oh->splitShift = 1;
+		wreset(oh, keepTop);
+		if (s == OZBLAS_STATUS_SUCCESS) s = csr_split(oh, m, nnz, A, rowptr, cs);
+		nA = cs.count;
+	}
+	oh->lastStatus = s;
+	if (s != OZBLAS_STATUS_SUCCESS) return nullptr;
+	oh->nSplitA_ = nA;
+	oh->nSplitA = (float)nA;
+	b->keep = keepTop;
+	oh->memMaskSplitA = keepTop;
+	wreset(oh, keepTop);
+	return cs.slices;
+}
+
+namespace oz {
+static int32_t csrmv_impl(ozblasHandle_t *oh, int m, int n, int nnz, double alpha, const double *A,
+                          const int *colind, const int *rowptr, const double *x, double beta,
+                          double *y) {
+	Backend *b = be(oh);
+	if (oh->sumMode != 0 && oh->sumMode != 1) return OZBLAS_STATUS_NOT_SUPPORTED;
+	if (oh->splitEpsMode == 2) return OZBLAS_STATUS_NOT_SUPPORTED;
+	const uint64_t base = (oh->memMaskSplitA != 0) ? b->keep : 0;
+	wreset(oh, base);
+	CsrSplit as;
+	if (oh->memMaskSplitA != 0) {  // pre-split by ozblasDcsrmvSplitA (src/cpu/ozblas_csrmv.cpp:95-97)
+		as.slices = (double *)A;
+		as.lds = rup(nnz, 32);
+		as.count = oh->nSplitA_;
+	} else {
+		oh->splitShift = 1;
+		OZ_TRY(csr_split(oh, m, nnz, A, rowptr, as));
+	}
+	oh->splitShift = 1;  // the vector split always uses splitShift = 1 (counterInit)
+	VecSplit xs;
+	OZ_TRY(vec_split(oh, n, x, xs));
+	OZ_TRY(csrmv_core(oh, m, n, nnz, alpha, as, colind, rowptr, xs, beta, y));
+	OZ_HIP(hipGetLastError());
+	OZ_TRY(sync(oh));
+	oh->nSplitA = (float)as.count;
+	oh->nSplitB = (float)xs.count;
+	wreset(oh, base);
+	return OZBLAS_STATUS_SUCCESS;
+}
+}  // namespace oz
+
+extern "C" int32_t ozblasDcsrmv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, int32_t nnz,
+                                double alpha, const void *descrA, const double *A,
+                                const int32_t *colind, const int32_t *rowptr, const double *x,
+                                double beta, double *y) {
+	(void)descrA;
+	OZ_TRY(ensure_init(oh));
+	if (!is_n(transA)) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	if (oh->reproMode == 0 && oh->nSplitMax == 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	return oh->lastStatus = csrmv_impl(oh, m, n, nnz, alpha, A, colind, rowptr, x, beta, y);
+}
+
+// CG (src/cpu/ozblas_cg.cpp:24-128): vectors stay on the device; the scalar recurrences are
+// evaluated on the host in binary64 (IEEE division and sqrt, as the reference)
+extern "C" int32_t ozblasDcg(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz,
+                             const void *descrA, const double *A, const int32_t *colind,
+                             const int32_t *rowptr, const double *b_, double *x, int32_t maxiter,
+                             double tol) {
+	OZ_TRY(ensure_init(oh));
+	if (!is_n(transA)) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	if (oh->sumMode != 0 && oh->sumMode != 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	Backend *b = be(oh);
+	hipStream_t st = b->stream;
+	double *hist = (double *)oh->cg_verbose1;
+	oh->cg_numiter = 0;
+	double *R, *P, *Q;
+	OZ_HIP(hipMalloc(&R, 8ull * n));
+	OZ_HIP(hipMalloc(&P, 8ull * n));
+	OZ_HIP(hipMalloc(&Q, 8ull * n));
+	int32_t s = OZBLAS_STATUS_SUCCESS;
+	double nrmb = 0, dold = 0, dnew = 0, tmp = 0, alpha = 0, beta = 0, resi = 0;
+	double *AS = ozblasDcsrmvSplitA(oh, transA, n, n, nnz, descrA, A, rowptr);
+	if (!AS) {
+		s = oh->lastStatus;
+		goto done;
+	}
+	s = ozblasDnrm2(oh, n, b_, 1, &nrmb);
+	if (s) goto done;
+	if (hipMemcpyAsync(R, b_, 8ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+		s = OZBLAS_STATUS_EXECUTION_FAILED;
+		goto done;
+	}
+	s = csrmv_impl(oh, n, n, nnz, -1.0, AS, colind, rowptr, x, 1.0, R);
+	if (s) goto done;
+	s = ozblasDdot(oh, n, R, 1, R, 1, &dold);
+	if (s) goto done;
+	resi = sqrt(dold);
+	if (oh->verbose > 0 && hist) hist[0] = resi / nrmb;
+	if (hipMemcpyAsync(P, R, 8ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+		s = OZBLAS_STATUS_EXECUTION_FAILED;
+		goto done;
+	}
+	{
+		int vcnt = 1;
+		while (oh->cg_numiter < maxiter) {
+			oh->cg_numiter++;
+			s = csrmv_impl(oh, n, n, nnz, 1.0, AS, colind, rowptr, P, 0.0, Q);
+			if (s) goto done;
+			s = ozblasDdot(oh, n, P, 1, Q, 1, &tmp);
+			if (s) goto done;
+			alpha = dold / tmp;
+			launch_axpy_host(st, n, alpha, P, x, oh->reproMode != 0);
+			launch_axpy_host(st, n, -alpha, Q, R, oh->reproMode != 0);
+			s = ozblasDdot(oh, n, R, 1, R, 1, &dnew);
+			if (s) goto done;
+			resi = sqrt(dnew);
+			beta = dnew / dold;
+			dold = dnew;
+			launch_scal_axpy(st, n, beta, R, P);
+			if (oh->verbose > 0) {
+				if (resi / nrmb < tol || oh->cg_numiter % oh->verbose == 0) {
+					if (hist) hist[vcnt] = resi / nrmb;
+					vcnt++;
+				}
+			}
+			if (resi / nrmb < tol) break;
+		}
+	}
+	if (hipStreamSynchronize(st) != hipSuccess) s = OZBLAS_STATUS_EXECUTION_FAILED;
+done:
+	(void)hipStreamSynchronize(st);
+	(void)hipFree(R);
+	(void)hipFree(P);
+	(void)hipFree(Q);
+	oh->memMaskSplitA = 0;
+	b->keep = 0;
+	oh->splitShift = 1;
+	wreset(oh, 0);
+	return oh->lastStatus = s;
+}
+
+extern "C" int32_t ozblasDgemmBatched(ozblasHandle_t *oh, int32_t batch, const char *transA,
+                                      const char *transB, const int32_t *m, const int32_t *n,
+                                      const int32_t *k, const double *alpha,
+                                      const double *const *A, const int32_t *lda,
+                                      const double *const *B, const int32_t *ldb,
+                                      const double *beta, double *const *C, const int32_t *ldc) {
+	OZ_TRY(ensure_init(oh));
+	for (int p = 0; p < batch; p++) {
+		int32_t s = ozblasDgemm(oh, transA[p], transB[p], m[p], n[p], k[p], alpha[p], A[p], lda[p],
+		                        B[p], ldb[p], beta[p], C[p], ldc[p]);
+		if (s != OZBLAS_STATUS_SUCCESS) return s;
+	}
+	return OZBLAS_STATUS_SUCCESS;
+}

@@ -1,0 +1,160 @@
+// oz_handle.cpp -- handle lifecycle, workspace and the libm ceil(log2) table.
+//
+// Mirrors cuozblasCreate/cuozblasDestroy (reference src/cuda/cuozblas.cpp:23-75) and the defaults
+// of src/cpu/ozblas.cpp:31-49; the workspace is one hipMalloc'd region with a bump allocator reset
+// on every call (counterInit, src/cpu/ozblas.cpp:3-20).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "oz_internal.hpp"
+
+namespace oz {
+
+static uint16_t g_tau[kTauBinades];
+static std::once_flag g_tau_once;
+
+static void build_tau_table() {
+	for (int e = -1074; e <= 1023; e++) {
+		const double base = ldexp(1.0, e);
+		const double u = ldexp(1.0, e - 52 > -1074 ? e - 52 : -1074);
+		// number of representable values strictly above 2^e inside the binade
+		const uint64_t nvals = (e - 52 >= -1074) ? ((uint64_t(1) << 52) - 1)
+		                                         : ((uint64_t(1) << (e + 1074)) - 1);
+		uint64_t i = 1;
+		for (; i <= nvals && i < 65535; i++) {
+			const double x = base + (double)i * u;
+			if (ceil(log2(x)) != (double)e) break;
+		}
+		g_tau[e + kTauBias] = (uint16_t)(i - 1);
+	}
+}
+
+const uint16_t *tau_table_host() {
+	std::call_once(g_tau_once, build_tau_table);
+	return g_tau;
+}
+
+// getRho<double,double> (src/cpu/ozblas_split.cpp:20-34), evaluated with the host libm
+int rho_host(int dim, int splitEpsMode) {
+	double r;
+	switch (splitEpsMode) {
+	case 2: r = ceil((53 - (106 - log2(1. * dim)) / 2)); break;
+	case 9: r = ceil((53 - (53 - log2(2. * sqrt((double)dim))) / 2)); break;
+	default: r = ceil((53 - (53 - log2(1. * dim)) / 2)); break;
+	}
+	if (!(r >= -2147483648.0 && r < 2147483648.0)) return INT32_MIN;
+	return (int)r;
+}
+
+int32_t ensure_init(ozblasHandle_t *oh) {
+	if (oh == nullptr || oh->initialized != 1 || oh->backend == nullptr)
+		return OZBLAS_STATUS_NOT_INITIALIZED;
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+}  // namespace oz
+
+using namespace oz;
+
+extern "C" int32_t ozblasCreate(ozblasHandle_t *oh, uint64_t workSizeBytes) {
+	if (oh == nullptr) return OZBLAS_STATUS_INVALID_VALUE;
+	std::memset(oh, 0, sizeof(*oh));
+	Backend *b = new Backend();
+	OZ_HIP(hipGetDevice(&b->device));
+	if (workSizeBytes < (1u << 20)) workSizeBytes = 1u << 20;
+	if (hipMalloc(&b->work, workSizeBytes) != hipSuccess) {
+		delete b;
+		fprintf(stderr, "OzBLAS error: work memory allocation error (%1.3e Bytes requested).\n",
+		        (double)workSizeBytes);
+		return OZBLAS_STATUS_ALLOC_FAILED;
+	}
+	b->cap = workSizeBytes;
+	OZ_HIP(hipMalloc(&b->dTau, sizeof(uint16_t) * kTauBinades));
+	OZ_HIP(hipMemcpy(b->dTau, tau_table_host(), sizeof(uint16_t) * kTauBinades,
+	                 hipMemcpyHostToDevice));
+	OZ_HIP(hipHostMalloc(&b->hPinned, 64 * sizeof(int64_t), hipHostMallocDefault));
+	OZ_HIP(hipMalloc(&b->dCounters, 256 * sizeof(int32_t)));
+	for (auto &e : b->ev) OZ_HIP(hipEventCreate(&e));
+	oh->backend = b;
+	oh->stream = nullptr;
+	oh->devWork = b->work;
+	oh->workSizeBytes = workSizeBytes;
+	// defaults: src/cpu/ozblas.cpp:31-49
+	oh->nSplitMax = 0;
+	oh->memMaskSplitA = 0;
+	oh->splitShift = 1;
+	oh->splitMode = 0;
+	oh->fastMode = 0;
+	oh->reproMode = 1;
+	oh->sumMode = 0;
+	oh->useBatchedGemmFlag = 0;
+	oh->splitEpsMode = 0;
+	oh->precxFlag = 0;
+	oh->sliceType = OZBLAS_SLICE_FP64;
+	oh->initialized = 1;
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+extern "C" int32_t ozblasDestroy(ozblasHandle_t *oh) {
+	if (oh == nullptr || oh->backend == nullptr) return OZBLAS_STATUS_NOT_INITIALIZED;
+	Backend *b = be(oh);
+	if (b->stream) (void)hipStreamSynchronize(b->stream);
+	else (void)hipDeviceSynchronize();
+	for (auto &e : b->ev)
+		if (e) (void)hipEventDestroy(e);
+	(void)hipFree(b->work);
+	(void)hipFree(b->dTau);
+	(void)hipFree(b->dCounters);
+	(void)hipHostFree(b->hPinned);
+	delete b;
+	oh->backend = nullptr;
+	oh->devWork = nullptr;
+	oh->initialized = 0;
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+extern "C" int32_t ozblasSetStream(ozblasHandle_t *oh, void *stream) {
+	OZ_TRY(ensure_init(oh));
+	be(oh)->stream = (hipStream_t)stream;
+	oh->stream = stream;
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+extern "C" const char *ozblasGetVersionString(void) {
+	return "ozblas_mi355x 0.1 (Ozaki-scheme BLAS for gfx950; OzBLAS 1.6a semantics)";
+}
+
+// Harness input generator (reference testing/testing_common.cpp:703-717, phi-mode): srand48(seed),
+// then per element r1, r2 = drand48(), x1 = sqrt(-2 log r1) sin(2 pi r2), a = (r1 - 0.5) exp(phi x1),
+// column by column.  drand48 is the POSIX 48-bit LCG (a = 0x5DEECE66D, c = 11, seed in bits 16..47
+// over 0x330E); log/sin/exp/sqrt are the host libm's, as in the harness.
+extern "C" void ozblasGenPhi(int32_t m, int32_t n, int32_t ld, double *A, double phi, int64_t seed) {
+	uint64_t x = (((uint64_t)(uint32_t)seed) << 16) | 0x330EULL;
+	auto drand = [&x]() {
+		x = (0x5DEECE66DULL * x + 0xBULL) & 0xFFFFFFFFFFFFULL;
+		return ldexp((double)x, -48);
+	};
+	if (ld < m) ld = m;
+	for (int32_t j = 0; j < n; j++)
+		for (int32_t i = 0; i < m; i++) {
+			const double r1 = drand();
+			const double r2 = drand();
+			const double x1 = 0.0 + (sqrt(-2. * log(r1)) * sin(2. * M_PI * r2)) * 1.0;
+			A[(int64_t)j * ld + i] = (r1 - 0.5) * exp(phi * x1);
+		}
+}
+
+// test hooks (not part of the reference API): the tau rule and rho evaluated on the host with the
+// same code the kernels run
+extern "C" int32_t ozblas_internal_tau(double max) { return tau_of(max, tau_table_host()); }
+extern "C" int32_t ozblas_internal_rho(int32_t dim, int32_t splitEpsMode) {
+	return rho_host(dim, splitEpsMode);
+}
+extern "C" int32_t ozblas_internal_rho_row(int32_t dim) { return rho_of_dim(dim); }
+extern "C" void ozblas_internal_tau_batch(int64_t n, const double *x, int32_t *out) {
+	const uint16_t *J = tau_table_host();
+	for (int64_t i = 0; i < n; i++) out[i] = tau_of(x[i], J);
+}

@@ -1,0 +1,87 @@
+// oz_kernels.hpp -- kernel launchers shared between the translation units (private header).
+#pragma once
+
+#include "oz_internal.hpp"
+
+namespace oz {
+
+// ---- split (oz_split.hip)
+void launch_split_cols(hipStream_t st, int m, int n, const double *in, int64_t ldi, double *out,
+                       int64_t ldo, double *split, int64_t lds, int64_t sstride, int rho,
+                       int shift, int cap, const uint16_t *J, int *zcol, int *zmax);
+void launch_zero_slots(hipStream_t st, int m, int n, double *split, int64_t lds, int64_t sstride,
+                       const int *zcol, int count);
+void launch_transpose(hipStream_t st, int rows, int cols, const double *in, int64_t ldi,
+                      double *out, int64_t ldo);
+void launch_vec_max(hipStream_t st, int m, const double *x, unsigned long long *slot);
+void launch_split_vec_pass(hipStream_t st, int m, const double *src, double *rem, double *slice,
+                           int rho, int shift, const uint16_t *J, const unsigned long long *maxIn,
+                           unsigned long long *maxOut);
+void launch_fill(hipStream_t st, double *p, int64_t n, double v);
+void launch_copy(hipStream_t st, const double *src, double *dst, int64_t n);
+void launch_split_csr(hipStream_t st, int m, const double *val, const int *rowptr, double *work,
+                      double *split, int64_t lds, int shift, int cap, const uint16_t *J, int *zrow,
+                      int *zmax);
+void launch_zero_slots_csr(hipStream_t st, int m, const int *rowptr, double *split, int64_t lds,
+                           const int *zrow, int count);
+void launch_digit_stats(hipStream_t st, int m, int n, const double *in, int64_t ldi, int *colE,
+                        int *colS, void *stats);
+void launch_digit_write(hipStream_t st, int elem, int m, int n, const double *in, int64_t ldi,
+                        const int *colE, int S, void *planes, int64_t ldp, int64_t pstride);
+
+// ---- FP64 slice products (oz_gemm_f64.hip)
+// C_b (M x N, ldc) = A_b^T B_b with A_b (K x M, lda), B_b (K x N, ldb) for batch entries b:
+// A_b = A + aOff[b], B_b = B + bOff[b], C_b = C + cOff[b] (element offsets, device arrays)
+void launch_gemm_f64_batched(hipStream_t st, int M, int N, int K, const double *A, int64_t lda,
+                             const double *B, int64_t ldb, double *C, int64_t ldc,
+                             const int64_t *aOff, const int64_t *bOff, const int64_t *cOff,
+                             int batch);
+// fused Fsum: t = sum over pairs p (in order) of A_p^T B_p; C = fma(alpha, t, beta*C)
+void launch_gemm_f64_fsum(hipStream_t st, int M, int N, int K, const double *A, int64_t lda,
+                          const double *B, int64_t ldb, const int64_t *aOff, const int64_t *bOff,
+                          int npairs, double alpha, double beta, double *C, int64_t ldc);
+
+// ---- sums (oz_sum.hip)
+void launch_global_nearsum(hipStream_t st, int m, int n, int ic, double *Cs, int64_t llsc,
+                           int64_t ldsc, double *C, int64_t ldc, double alpha, double beta,
+                           const uint16_t *J);
+void launch_global_fsum(hipStream_t st, int m, int n, int nA, int nB, int maxlevel, int sumOrder,
+                        const double *Cs, int64_t llsc, int64_t ldsc, double *C, int64_t ldc,
+                        double alpha, double beta);
+
+// ---- fast forms (oz_gemm_digits.hip)
+struct LevelBin {      // one exact int32 plane: sum over `npairs` digit pairs, k in [k0, k1)
+	int level;
+	int npairs;
+	int pair0;         // first entry in the pair list
+	int k0, k1;
+};
+void launch_digit_gemm(hipStream_t st, int elem, int M, int N, int Kpad, const void *Ap,
+                       int64_t ldap, int64_t apstride, const void *Bp, int64_t ldbp,
+                       int64_t bpstride, const int2 *pairs, const LevelBin *bins, int nbins,
+                       int32_t *T, int64_t ldt, int64_t tstride);
+void launch_reconstruct(hipStream_t st, int M, int N, const int32_t *T, int64_t ldt,
+                        int64_t tstride, const int *binLevel, int nbins, int Lmax, const int *rowE,
+                        const int *colE, double alpha, double beta, double *C, int64_t ldc);
+
+// ---- vectors / sparse (oz_level12.hip)
+void launch_slice_dots(hipStream_t st, int n, const double *xs, int64_t ldx, int nA,
+                       const double *ys, int64_t ldy, int nB, double *out);
+void launch_nearsum_small(hipStream_t st, int nvals, double *vals, double alpha, double beta,
+                          double *C, int sumOrderFsum, int nA, int nB, int maxlevel, int fsum,
+                          const uint16_t *J);
+void launch_csrmm_sum(hipStream_t st, int m, int nnz, const double *As, int64_t ldas, int nA,
+                      const int *colind, const int *rowptr, const double *Xs, int64_t ldxs, int nB,
+                      int maxlevel, int fsum, double alpha, double beta, double *y,
+                      const uint16_t *J);
+void launch_csrmm_store(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
+                        const int *colind, const int *rowptr, const double *Xs, int64_t ldxs,
+                        int nB, int maxlevel, double *Cs, int64_t ldcs);
+hipError_t launch_split_vec_coop(hipStream_t st, int n, const double *x, double *rem,
+                                 double *slices, int64_t lds, int rho, int shift, int cap,
+                                 int reproMode, const uint16_t *J, unsigned long long *maxv,
+                                 int *info);
+void launch_axpy_host(hipStream_t st, int n, double alpha, const double *x, double *y, int fused);
+void launch_scal_axpy(hipStream_t st, int n, double beta, const double *r, double *p);
+
+}  // namespace oz

@@ -1,0 +1,84 @@
+"""CPU checks of the C-ABI library: it loads, exports every entry point of include/*.h, and the
+host-side rules the kernels share (tau = (short)ceil(log2(max)) via the libm-measured table, rho
+of a CSR row without a log) agree with the C library the reference calls.  No GPU call is made."""
+import ctypes
+import glob
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "ozblas_amd", "libozblas_mi355x.so")
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        pytest.fail("libozblas_mi355x.so is not built (run __graft_entry__.build())")
+    return ctypes.CDLL(LIB)
+
+
+def declared_functions():
+    names = []
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^\s*[\w\s\*]+?\b(ozblas\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_reference_entry_points():
+    names = declared_functions()
+    for n in ("ozblasCreate", "ozblasDestroy", "ozblasDdot", "ozblasDnrm2", "ozblasDaxpy",
+              "ozblasDgemv", "ozblasDgemm", "ozblasDcsrmv", "ozblasDcsrmvSplitA", "ozblasDcg"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol(lib):
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert missing == []
+
+
+def test_tau_rule_matches_libm(lib, oracle):
+    """(short)ceil(log2(x)) through the table equals glibc on the first 420 values above every
+    power of two (where log2 may round to the integer), on powers of two, just below them,
+    on subnormals, zero, inf and NaN."""
+    f = lib.ozblas_internal_tau_batch
+    f.argtypes = [ctypes.c_int64, np.ctypeslib.ndpointer(np.float64), np.ctypeslib.ndpointer(np.int32)]
+    xs = []
+    for e in range(-1074, 1024):
+        base = math.ldexp(1.0, e)
+        u = math.ldexp(1.0, max(e - 52, -1074))
+        nv = (1 << 52) - 1 if e - 52 >= -1074 else (1 << (e + 1074)) - 1
+        xs += [base + i * u for i in range(0, min(420, nv + 1))]
+        if e > -1074:
+            xs.append(math.nextafter(base, 0.0))
+    rng = np.random.default_rng(1)
+    xs += list(np.ldexp(rng.uniform(1, 2, 20000), rng.integers(-1074, 1023, 20000)))
+    xs += [0.0, -0.0, float("inf"), float("nan"), -3.0, 5e-324, 1.7976931348623157e308]
+    x = np.array(xs, dtype=np.float64)
+    got = np.zeros(len(x), dtype=np.int32)
+    f(len(x), x, got)
+    want = oracle.tau_batch(x)
+    bad = np.nonzero(got != want)[0]
+    assert len(bad) == 0, [(x[i], got[i], want[i]) for i in bad[:10]]
+
+
+def test_rho_rules_match_libm(lib, oracle):
+    lib.ozblas_internal_rho_row.restype = ctypes.c_int32
+    lib.ozblas_internal_rho.restype = ctypes.c_int32
+    for d in list(range(1, 5000)) + [10 ** 6, 2 ** 20, 2 ** 20 + 1, 2 ** 31 - 1, 123456789]:
+        want = oracle.rho(d)
+        assert lib.ozblas_internal_rho(d, 0) == want
+        assert lib.ozblas_internal_rho_row(d) == want, d
+
+
+def test_gen_phi_matches_oracle_generator(oracle):
+    import ozblas_amd
+    for phi, seed in ((1.0, 1), (0.5, 2), (4.0, 7)):
+        a = ozblas_amd.gen_phi(33, 9, 40, phi, seed)
+        b = oracle.init_mat(33, 9, 40, phi, 2, seed)
+        assert a.tobytes() == b.tobytes()

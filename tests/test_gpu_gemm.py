@@ -1,0 +1,158 @@
+"""GPU parity of DGEMM/DGEMV through the C ABI against the oracle (bitwise).
+
+Plain form (FP64 slices): bitwise identical to the oracle restatement of the reference CPU OzBLAS
+for every transposition, summation mode (Fsum, NearSum), reproMode, fastMode, degree and shape
+below.  Fast forms (INT8/FP16/BF16 digit slices): bitwise identical to the plain form under
+NearSum, i.e. RN of the exact product, checked against the oracle's exact accumulator too.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import bits_equal, dev, first_diff, host, say, torch_cuda
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def oz():
+    torch_cuda()
+    import ozblas_amd
+    return ozblas_amd
+
+
+def _shape(trans, rows, cols):
+    return (rows, cols) if trans in "Nn" else (cols, rows)
+
+
+def run_pair(oz, oracle, ta, tb, m, n, k, phi=1.0, seeds=(1, 2), alpha=1.0, beta=0.0, cinit=None,
+             slice_type="fp64", ws=1 << 30, **flags):
+    ar, ac = _shape(ta, m, k)
+    br, bc = _shape(tb, k, n)
+    lda, ldb, ldc = ar + 3, br + 1, m + 2
+    A = oracle.init_mat(ar, ac, lda, phi, 2, seeds[0])
+    B = oracle.init_mat(br, bc, ldb, phi, 2, seeds[1])
+    C0 = np.zeros(ldc * n) if cinit is None else cinit(ldc * n)
+    h = oracle.Handle(1 << 31, **flags)
+    Cw = C0.copy()
+    h.gemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cw, ldc)
+    H = oz.Handle(ws, sliceType=slice_type, **flags)
+    dA, dB, dC = dev(A), dev(B), dev(C0)
+    H.dgemm(ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc)
+    got = host(dC)
+    return got, Cw, H, h
+
+
+CASES = [
+    # (m, n, k, phi)
+    (256, 256, 256, 1.0),
+    (77, 133, 199, 4.0),
+    (130, 129, 300, 0.5),
+    (1, 1, 37, 1.0),
+    (64, 1, 300, 2.0),
+    (33, 17, 1, 1.0),
+    (5, 300, 64, 3.0),
+]
+
+
+@pytest.mark.parametrize("m,n,k,phi", CASES)
+@pytest.mark.parametrize("trans", ["NN", "NT", "TN", "TT"])
+@pytest.mark.parametrize("sumMode", [0, 1])
+def test_plain_gemm_bitwise(oz, oracle, m, n, k, phi, trans, sumMode):
+    say(f"plain dgemm {trans} m={m} n={n} k={k} phi={phi} sumMode={sumMode}")
+    got, want, H, h = run_pair(oz, oracle, trans[0], trans[1], m, n, k, phi=phi, sumMode=sumMode,
+                               reproMode=1)
+    assert H.nSplitA == h.nSplitA and H.nSplitB == h.nSplitB
+    assert bits_equal(got, want), first_diff(got, want)
+
+
+@pytest.mark.parametrize("flags", [dict(reproMode=0), dict(fastMode=1), dict(nSplitMax=2),
+                                   dict(nSplitMax=3, reproMode=0), dict(fastMode=1, sumMode=1)])
+def test_plain_gemm_modes(oz, oracle, flags):
+    f = dict(sumMode=0, reproMode=1)
+    f.update(flags)
+    for (m, n, k) in ((96, 80, 64), (40, 1, 50)):
+        say(f"plain dgemm modes {f} m={m} n={n} k={k}")
+        got, want, H, h = run_pair(oz, oracle, "N", "N", m, n, k, phi=2.0, **f)
+        assert bits_equal(got, want), first_diff(got, want)
+
+
+def test_plain_gemm_alpha_beta(oz, oracle):
+    rng = np.random.default_rng(4)
+    for sumMode in (0, 1):
+        got, want, _, _ = run_pair(oz, oracle, "T", "N", 70, 90, 110, alpha=-1.25, beta=0.5,
+                                   cinit=lambda s: rng.standard_normal(s), sumMode=sumMode)
+        assert bits_equal(got, want), first_diff(got, want)
+
+
+def test_plain_gemm_blocked_workspace(oz, oracle):
+    """A workspace too small for the whole problem forces m/n blocking; NearSum bits are
+    independent of it (the reference blocks too, src/cpu/ozblas_gemm.cpp:59-97)."""
+    got, want, H, _ = run_pair(oz, oracle, "N", "N", 300, 280, 256, phi=1.0, sumMode=1,
+                               ws=10 << 20)
+    assert H.mbk < 300 or H.nbk < 280
+    assert bits_equal(got, want), first_diff(got, want)
+
+
+@pytest.mark.parametrize("slice_type", ["int8", "fp16", "bf16"])
+@pytest.mark.parametrize("m,n,k,phi,trans", [(256, 256, 256, 1.0, "NN"), (77, 133, 199, 4.0, "TN"),
+                                             (130, 129, 300, 0.5, "NT"), (200, 64, 1000, 2.0, "TT"),
+                                             (1, 1, 37, 1.0, "NN"), (64, 1, 300, 1.0, "NN")])
+def test_fast_forms_match_plain_nearsum(oz, oracle, slice_type, m, n, k, phi, trans):
+    say(f"{slice_type} dgemm {trans} m={m} n={n} k={k} phi={phi}")
+    got, want, H, _ = run_pair(oz, oracle, trans[0], trans[1], m, n, k, phi=phi, sumMode=1,
+                               slice_type=slice_type)
+    assert bits_equal(got, want), first_diff(got, want)
+
+
+def test_fast_form_alpha_beta_and_exact(oz, oracle):
+    rng = np.random.default_rng(9)
+    m, n, k = 150, 70, 513
+    got, want, H, _ = run_pair(oz, oracle, "N", "T", m, n, k, phi=1.0, alpha=0.75, beta=-2.0,
+                               cinit=lambda s: rng.standard_normal(s), sumMode=1,
+                               slice_type="int8")
+    assert bits_equal(got, want), first_diff(got, want)
+    # and against the oracle's exact accumulator directly (alpha=1, beta=0)
+    got, _, _, _ = run_pair(oz, oracle, "N", "N", m, n, k, phi=4.0, sumMode=1, slice_type="int8")
+    A = oracle.init_mat(m, k, m + 3, 4.0, 2, 1)
+    B = oracle.init_mat(k, n, k + 1, 4.0, 2, 2)
+    ex = oracle.exact_gemm_rn("N", "N", m, n, k, A, m + 3, B, k + 1)
+    gotm = got.reshape(n, m + 2).T[:m, :]
+    assert bits_equal(gotm, ex), first_diff(gotm, ex)
+
+
+def test_fast_form_edge_values(oz, oracle):
+    """Zeros, a zero row, powers of two, values at the digit boundaries, mixed exponents."""
+    m, n, k = 40, 30, 96
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((m, k)) * np.exp2(rng.integers(-40, 40, (m, k)))
+    A[3, :] = 0.0
+    A[5, :7] = [1.0, -1.0, 2.0 ** 30, -(2.0 ** -30), 127.0, -128.0, 0.9921875]
+    B = rng.standard_normal((k, n))
+    B[:, 4] = 0.0
+    B[10, :] = 2.0 ** -60
+    Af, Bf = np.asfortranarray(A).T.ravel().copy(), np.asfortranarray(B).T.ravel().copy()
+    want = oracle.exact_gemm_rn("N", "N", m, n, k, Af, m, Bf, k)
+    for st in ("int8", "fp16", "bf16", "fp64"):
+        H = oz.Handle(1 << 28, sumMode=1, sliceType=st)
+        dC = dev(np.zeros(m * n))
+        H.dgemm("N", "N", m, n, k, 1.0, dev(Af), m, dev(Bf), k, 0.0, dC, m)
+        got = host(dC).reshape(n, m).T
+        assert bits_equal(got, want), (st, first_diff(got, want))
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("sumMode", [0, 1])
+def test_gemv_bitwise(oz, oracle, trans, sumMode):
+    for (m, n) in ((100, 80), (7, 300), (300, 7)):
+        say(f"dgemv {trans} m={m} n={n} sumMode={sumMode}")
+        A = oracle.init_mat(m, n, m, 1.0, 2, 11)
+        lx, ly = (n, m) if trans == "N" else (m, n)
+        x = oracle.init_mat(lx, 1, lx, 1.0, 2, 12)
+        y0 = np.zeros(ly)
+        h = oracle.Handle(1 << 28, sumMode=sumMode)
+        yw = y0.copy()
+        h.gemv(trans, m, n, 1.0, A, m, x, 0.0, yw)
+        H = oz.Handle(1 << 28, sumMode=sumMode)
+        dy = dev(y0)
+        H.dgemv(trans, m, n, 1.0, dev(A), m, dev(x), 0.0, dy)
+        assert bits_equal(host(dy), yw), first_diff(host(dy), yw)

@@ -1,0 +1,101 @@
+"""GPU parity of DOT, NRM2, AXPY, CSRMV (with and without the pre-split of ozblasDcsrmvSplitA) and
+the CG solver through the C ABI, bitwise against the oracle restatement of the reference."""
+import math
+
+import numpy as np
+import pytest
+
+from gpu_util import bits_equal, dev, first_diff, host, say, torch_cuda
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def oz():
+    torch_cuda()
+    import ozblas_amd
+    return ozblas_amd
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 1000, 65536, 100003])
+@pytest.mark.parametrize("sumMode", [0, 1])
+@pytest.mark.parametrize("reproMode", [0, 1])
+def test_dot_nrm2_bitwise(oz, oracle, n, sumMode, reproMode):
+    say(f"ddot n={n} sumMode={sumMode} reproMode={reproMode}")
+    x = oracle.init_mat(n, 1, n, 2.0, 2, 21)
+    y = oracle.init_mat(n, 1, n, 2.0, 2, 22)
+    h = oracle.Handle(1 << 30, sumMode=sumMode, reproMode=reproMode)
+    H = oz.Handle(1 << 28, sumMode=sumMode, reproMode=reproMode)
+    dx, dy = dev(x), dev(y)
+    got = H.ddot(n, dx, dy)
+    want = oracle.Handle(1 << 30, sumMode=sumMode, reproMode=reproMode).dot(n, x, y)
+    assert np.float64(got).tobytes() == np.float64(want).tobytes(), (got.hex(), want.hex())
+    assert H.dnrm2(n, dx) == h.nrm2(n, x)
+
+
+def test_dot_is_correctly_rounded_under_nearsum(oz, oracle):
+    rng = np.random.default_rng(8)
+    for n in (3, 999, 40000):
+        x = rng.standard_normal(n) * np.exp2(rng.integers(-30, 30, n))
+        y = rng.standard_normal(n)
+        H = oz.Handle(1 << 28, sumMode=1)
+        assert H.ddot(n, dev(x), dev(y)) == oracle.exact_dot_rn(x, y)
+
+
+def test_axpy(oz, oracle):
+    rng = np.random.default_rng(1)
+    x, y = rng.standard_normal(1001), rng.standard_normal(1001)
+    for reproMode in (0, 1):
+        H = oz.Handle(1 << 24, reproMode=reproMode)
+        dy = dev(y)
+        H.daxpy(1001, -0.3, dev(x), dy)
+        want = oracle.Handle(1 << 24, reproMode=reproMode).axpy(1001, -0.3, x, y.copy())
+        assert bits_equal(host(dy), want)
+
+
+def _laplace(oracle, g):
+    rowptr, colind, val = oracle.laplace3d(g, 1e-3)
+    return rowptr, colind, val, g ** 3, len(val)
+
+
+@pytest.mark.parametrize("sumMode", [0, 1])
+@pytest.mark.parametrize("presplit", [False, True])
+def test_csrmv_bitwise(oz, oracle, sumMode, presplit):
+    rowptr, colind, val, n, nnz = _laplace(oracle, 12)
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, n) * np.exp2(rng.integers(-20, 20, n))
+    y0 = rng.standard_normal(n)
+    say(f"dcsrmv n={n} sumMode={sumMode} presplit={presplit}")
+    h = oracle.Handle(1 << 30, sumMode=sumMode)
+    H = oz.Handle(1 << 28, sumMode=sumMode)
+    dval, dcol, drow, dx = dev(val), dev(colind), dev(rowptr), dev(x)
+    yw = y0.copy()
+    dy = dev(y0)
+    if presplit:
+        p = h.csrmv_splitA(n, n, nnz, val, rowptr)
+        P = H.dcsrmv_splitA(n, n, nnz, dval, drow)
+        assert (H.splitShift, H.nSplitA_) == (h.splitShift, h.nSplitA_)
+        h.csrmv(n, n, nnz, 0.5, p, colind, rowptr, x, -1.5, yw)
+        H.dcsrmv(n, n, nnz, 0.5, P, dcol, drow, dx, -1.5, dy)
+    else:
+        h.csrmv(n, n, nnz, 0.5, val, colind, rowptr, x, -1.5, yw)
+        H.dcsrmv(n, n, nnz, 0.5, dval, dcol, drow, dx, -1.5, dy)
+    assert bits_equal(host(dy), yw), first_diff(host(dy), yw)
+
+
+@pytest.mark.parametrize("sumMode", [0, 1])
+def test_cg_residual_history_bitwise(oz, oracle, sumMode):
+    g, iters = 10, 40
+    rowptr, colind, val, n, nnz = _laplace(oracle, g)
+    oracle.srand48(3)
+    b = np.array([2.0 * oracle.drand48() - 1.0 for _ in range(n)])
+    say(f"dcg n={n} iters={iters} sumMode={sumMode}")
+    h = oracle.Handle(1 << 30, sumMode=sumMode)
+    xw = np.zeros(n)
+    want = h.cg(n, nnz, val, colind, rowptr, b, xw, iters, 0.0)
+    H = oz.Handle(1 << 28, sumMode=sumMode)
+    dx = dev(np.zeros(n))
+    got = H.dcg(n, nnz, dev(val), dev(colind), dev(rowptr), dev(b), dx, iters, 0.0)
+    assert len(got) == iters + 1
+    assert bits_equal(got, want), first_diff(got, want)
+    assert bits_equal(host(dx), xw), first_diff(host(dx), xw)
