@@ -52,53 +52,103 @@ __device__ inline void store_stage(const Stage &s, char *sA, char *sB) {
 }
 
 // ---- INT8 ----------------------------------------------------------------------------------
-// acc[mi][nj] covers rows (A side) 32*mi.., cols (B side) 32*nj.. of the wave's 64x64 region.
+// Workgroup = 2 x 2 waves; wave tile = TM x TN MFMA tiles of 32 x 32 (block tile 64TM x 64TN).
+// Operands stream through a double-buffered LDS stage of 64 k-bytes; one barrier per stage.  The
+// (pair, k) loop of a bin is flattened so the software pipeline does not drain between pairs.
 // The MFMA is issued as B-fragment x A-fragment so that lane l%32 indexes the A row: stores of the
-// column-major result planes are then contiguous across lanes.
+// column-major result planes are contiguous across lanes.
+template <int TM, int TN>
 __global__ __launch_bounds__(256) void k_digit_gemm_i8(int M, int N, const int8_t *A, int64_t ldap,
                                                        int64_t apstride, const int8_t *B,
                                                        int64_t ldbp, int64_t bpstride,
                                                        const int2 *pairs, const LevelBin *bins,
                                                        int32_t *T, int64_t ldt, int64_t tstride,
                                                        int tilesM) {
-	__shared__ __attribute__((aligned(16))) char sA[BM * LDSROW];
-	__shared__ __attribute__((aligned(16))) char sB[BN * LDSROW];
+	constexpr int TBM = 64 * TM, TBN = 64 * TN;
+	extern __shared__ __attribute__((aligned(16))) char smem[];
+	char *sA = smem;                              // [2][TBM][LDSROW]
+	char *sB = smem + 2 * TBM * LDSROW;           // [2][TBN][LDSROW]
 	const LevelBin bin = bins[blockIdx.y];
-	const int i0 = (blockIdx.x % tilesM) * BM, j0 = (blockIdx.x / tilesM) * BN;
-	const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-	const int wm = (w % 2) * 64, wn = (w / 2) * 64;
-	i16v acc[2][2];
-	for (int a = 0; a < 2; a++)
-		for (int b = 0; b < 2; b++)
+	const int i0 = (blockIdx.x % tilesM) * TBM, j0 = (blockIdx.x / tilesM) * TBN;
+	const int t = threadIdx.x, w = t / 64, l = t % 64;
+	const int wm = (w % 2) * 32 * TM, wn = (w / 2) * 32 * TN;
+	i16v acc[TM][TN];
+#pragma unroll
+	for (int a = 0; a < TM; a++)
+#pragma unroll
+		for (int b = 0; b < TN; b++)
+#pragma unroll
 			for (int r = 0; r < 16; r++) acc[a][b][r] = 0;
-	for (int p = 0; p < bin.npairs; p++) {
+	const int kst = (bin.k1 - bin.k0) / BKB;
+	const int total = bin.npairs * kst;
+	uint4 ra[TM], rb[TN];
+	auto gload = [&](int step) {
+		const int p = step / kst, kb = bin.k0 + (step % kst) * BKB;
 		const int2 st = pairs[bin.pair0 + p];
 		const char *Ap = (const char *)(A + (int64_t)st.x * apstride);
 		const char *Bp = (const char *)(B + (int64_t)st.y * bpstride);
-		Stage stg;
-		load_stage(stg, Ap, ldap, Bp, ldbp, i0, j0, M, N, bin.k0);
-		for (int kb = bin.k0; kb < bin.k1; kb += BKB) {
-			__syncthreads();
-			store_stage(stg, sA, sB);
-			__syncthreads();
-			if (kb + BKB < bin.k1) load_stage(stg, Ap, ldap, Bp, ldbp, i0, j0, M, N, kb + BKB);
 #pragma unroll
-			for (int kk = 0; kk < 2; kk++) {
-				l2v af[2], bf[2];
-				for (int mi = 0; mi < 2; mi++)
-					af[mi] = *(const l2v *)(sA + (wm + 32 * mi + l % 32) * LDSROW + 32 * kk + 16 * (l / 32));
-				for (int nj = 0; nj < 2; nj++)
-					bf[nj] = *(const l2v *)(sB + (wn + 32 * nj + l % 32) * LDSROW + 32 * kk + 16 * (l / 32));
-				for (int mi = 0; mi < 2; mi++)
-					for (int nj = 0; nj < 2; nj++)
-						acc[mi][nj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bf[nj], af[mi], acc[mi][nj], 0, 0, 0);
-			}
+		for (int h = 0; h < TM; h++) {
+			const int c = t + 256 * h, row = c / 4, ch = c % 4;
+			const int ia = i0 + row;
+			ra[h] = (ia < M) ? *(const uint4 *)(Ap + (int64_t)ia * ldap + kb + 16 * ch)
+			                 : make_uint4(0, 0, 0, 0);
 		}
+#pragma unroll
+		for (int h = 0; h < TN; h++) {
+			const int c = t + 256 * h, row = c / 4, ch = c % 4;
+			const int jb = j0 + row;
+			rb[h] = (jb < N) ? *(const uint4 *)(Bp + (int64_t)jb * ldbp + kb + 16 * ch)
+			                 : make_uint4(0, 0, 0, 0);
+		}
+	};
+	auto lstore = [&](int buf) {
+		char *a = sA + buf * TBM * LDSROW, *b = sB + buf * TBN * LDSROW;
+#pragma unroll
+		for (int h = 0; h < TM; h++) {
+			const int c = t + 256 * h;
+			*(uint4 *)(a + (c / 4) * LDSROW + 16 * (c % 4)) = ra[h];
+		}
+#pragma unroll
+		for (int h = 0; h < TN; h++) {
+			const int c = t + 256 * h;
+			*(uint4 *)(b + (c / 4) * LDSROW + 16 * (c % 4)) = rb[h];
+		}
+	};
+	if (total > 0) {
+		gload(0);
+		lstore(0);
+	}
+	__syncthreads();
+	for (int step = 0; step < total; step++) {
+		const int cur = step & 1;
+		if (step + 1 < total) gload(step + 1);
+		const char *a = sA + cur * TBM * LDSROW, *b = sB + cur * TBN * LDSROW;
+#pragma unroll
+		for (int kk = 0; kk < 2; kk++) {
+			l2v af[TM], bf[TN];
+#pragma unroll
+			for (int mi = 0; mi < TM; mi++)
+				af[mi] = *(const l2v *)(a + (wm + 32 * mi + l % 32) * LDSROW + 32 * kk + 16 * (l / 32));
+#pragma unroll
+			for (int nj = 0; nj < TN; nj++)
+				bf[nj] = *(const l2v *)(b + (wn + 32 * nj + l % 32) * LDSROW + 32 * kk + 16 * (l / 32));
+#pragma unroll
+			for (int mi = 0; mi < TM; mi++)
+#pragma unroll
+				for (int nj = 0; nj < TN; nj++)
+					acc[mi][nj] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bf[nj], af[mi], acc[mi][nj], 0, 0, 0);
+		}
+		if (step + 1 < total) lstore(cur ^ 1);
+		__syncthreads();
 	}
 	// D[p][q]: p = B-side (column j) = 8*(r/4) + 4*(l/32) + r%4, q = A-side (row i) = l%32
 	int32_t *Tb = T + (int64_t)blockIdx.y * tstride;
-	for (int mi = 0; mi < 2; mi++)
-		for (int nj = 0; nj < 2; nj++)
+#pragma unroll
+	for (int mi = 0; mi < TM; mi++)
+#pragma unroll
+		for (int nj = 0; nj < TN; nj++)
+#pragma unroll
 			for (int r = 0; r < 16; r++) {
 				const int i = i0 + wm + 32 * mi + l % 32;
 				const int j = j0 + wn + 32 * nj + 8 * (r / 4) + 4 * (l / 32) + r % 4;
@@ -192,9 +242,27 @@ void launch_digit_gemm(hipStream_t st, int elem, int M, int N, int Kpad, const v
 	const int tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
 	dim3 grid(tilesM * tilesN, nbins);
 	if (elem == 1) {
-		hipLaunchKernelGGL(k_digit_gemm_i8, grid, dim3(256), 0, st, M, N, (const int8_t *)Ap, ldap,
-		                   apstride, (const int8_t *)Bp, ldbp, bpstride, pairs, bins, T, ldt,
-		                   tstride, tilesM);
+		if (M >= 512 && N >= 512) {
+			constexpr int TM = 4, TN = 4;
+			const int tm = (M + 64 * TM - 1) / (64 * TM), tn = (N + 64 * TN - 1) / (64 * TN);
+			const size_t lds = 2 * (64 * TM + 64 * TN) * LDSROW;
+			static bool attr = false;
+			if (!attr) {  // more than 64 KB of dynamic LDS (gfx950 has 160 KB per CU)
+				(void)hipFuncSetAttribute((const void *)k_digit_gemm_i8<TM, TN>,
+				                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+				attr = true;
+			}
+			hipLaunchKernelGGL((k_digit_gemm_i8<TM, TN>), dim3(tm * tn, nbins), dim3(256), lds, st, M, N,
+			                   (const int8_t *)Ap, ldap, apstride, (const int8_t *)Bp, ldbp, bpstride,
+			                   pairs, bins, T, ldt, tstride, tm);
+		} else {
+			constexpr int TM = 2, TN = 2;
+			const int tm = (M + 64 * TM - 1) / (64 * TM), tn = (N + 64 * TN - 1) / (64 * TN);
+			const size_t lds = 2 * (64 * TM + 64 * TN) * LDSROW;
+			hipLaunchKernelGGL((k_digit_gemm_i8<TM, TN>), dim3(tm * tn, nbins), dim3(256), lds, st, M, N,
+			                   (const int8_t *)Ap, ldap, apstride, (const int8_t *)Bp, ldbp, bpstride,
+			                   pairs, bins, T, ldt, tstride, tm);
+		}
 	} else if (elem == 2) {
 		hipLaunchKernelGGL(k_digit_gemm_h<h8v>, grid, dim3(256), 0, st, M, N, (const char *)Ap,
 		                   2 * ldap, 2 * apstride, (const char *)Bp, 2 * ldbp, 2 * bpstride, pairs,
