@@ -308,8 +308,13 @@ static int32_t digit_split(ozblasHandle_t *oh, int elem, int k, int cols, const 
 	OZ_TRY(sync(oh));
 	std::memcpy(&d.st, b->hPinned, sizeof(ExpStatsH));
 	d.S = d.st.maxS;
-	d.ldp = rup(k, 64);
-	d.pstride = d.ldp * cols;
+	if (elem == 1) {  // tiled INT8 layout: k padded to 128 bytes, columns to 256 (zero digits)
+		d.ldp = rup(k, 128);
+		d.pstride = d.ldp * rup(cols, 256);
+	} else {
+		d.ldp = rup(k, 64);
+		d.pstride = d.ldp * cols;
+	}
 	const size_t esz = (elem == 1) ? 1 : 2;
 	if (d.S > 0) {
 		d.planes = walloc(oh, esz * (size_t)d.pstride * d.S);
@@ -414,13 +419,20 @@ static int32_t gemm_fast(ozblasHandle_t *oh, char transA, char transB, int m, in
 	}
 	const int nbins = (int)bins.size();
 	oh->nLevelBins = nbins;
+	// launch order: bins with the most pairs first, so the short ones fill the tail
+	std::vector<int> order(nbins);
+	for (int q = 0; q < nbins; q++) order[q] = q;
+	std::stable_sort(order.begin(), order.end(),
+	                 [&](int x, int y) { return bins[x].npairs > bins[y].npairs; });
 	int2 *dPairs = (int2 *)walloc(oh, sizeof(int2) * pairs.size());
 	LevelBin *dBins = (LevelBin *)walloc(oh, sizeof(LevelBin) * bins.size());
 	int *dLevel = (int *)walloc(oh, sizeof(int) * binLevel.size());
-	const int64_t ldt = rup(m, 64);
-	const int64_t tstride = ldt * n;
+	int *dOrder = (int *)walloc(oh, sizeof(int) * order.size());
+	// INT8 result planes cover whole 256 x 256 tiles (the kernel stores without bounds checks)
+	const int64_t ldt = (elem == 1) ? rup(m, 256) : rup(m, 64);
+	const int64_t tstride = ldt * ((elem == 1) ? rup(n, 256) : n);
 	int32_t *T = (int32_t *)walloc(oh, sizeof(int32_t) * (size_t)tstride * nbins);
-	if (!dPairs || !dBins || !dLevel || !T) {
+	if (!dPairs || !dBins || !dLevel || !dOrder || !T) {
 		wreset(oh, base);
 		*handled = false;
 		return OZBLAS_STATUS_WORKSPACE;
@@ -428,9 +440,10 @@ static int32_t gemm_fast(ozblasHandle_t *oh, char transA, char transB, int m, in
 	OZ_HIP(hipMemcpyAsync(dPairs, pairs.data(), sizeof(int2) * pairs.size(), hipMemcpyHostToDevice, st));
 	OZ_HIP(hipMemcpyAsync(dBins, bins.data(), sizeof(LevelBin) * bins.size(), hipMemcpyHostToDevice, st));
 	OZ_HIP(hipMemcpyAsync(dLevel, binLevel.data(), sizeof(int) * binLevel.size(), hipMemcpyHostToDevice, st));
+	OZ_HIP(hipMemcpyAsync(dOrder, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, st));
 	OZ_HIP(hipEventRecord(b->ev[3], st));
 	launch_digit_gemm(st, elem, m, n, Kpad, da.planes, da.ldp, da.pstride, db.planes, db.ldp,
-	                  db.pstride, dPairs, dBins, nbins, T, ldt, tstride);
+	                  db.pstride, dPairs, dBins, nbins, dOrder, T, ldt, tstride);
 	OZ_HIP(hipEventRecord(b->ev[4], st));
 	launch_reconstruct(st, m, n, T, ldt, tstride, dLevel, nbins, Lmax, da.colE, db.colE, alpha,
 	                   beta, C, ldc);

@@ -59,7 +59,7 @@ struct LevelBin {      // one exact int32 plane: sum over `npairs` digit pairs, 
 void launch_digit_gemm(hipStream_t st, int elem, int M, int N, int Kpad, const void *Ap,
                        int64_t ldap, int64_t apstride, const void *Bp, int64_t ldbp,
                        int64_t bpstride, const int2 *pairs, const LevelBin *bins, int nbins,
-                       int32_t *T, int64_t ldt, int64_t tstride);
+                       const int *binOrder, int32_t *T, int64_t ldt, int64_t tstride);
 void launch_reconstruct(hipStream_t st, int M, int N, const int32_t *T, int64_t ldt,
                         int64_t tstride, const int *binLevel, int nbins, int Lmax, const int *rowE,
                         const int *colE, double alpha, double beta, double *C, int64_t ldc);

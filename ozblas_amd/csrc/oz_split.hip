@@ -361,6 +361,36 @@ __global__ __launch_bounds__(256) void k_digit_write(int m, int n, const double 
 	}
 }
 
+// INT8 planes in the tiled layout of the MFMA kernel: element (column j, k index q) of plane s at
+//   s * pstride + (j / 16) * 16 * ldp + (q / 64) * 1024 + (j % 16) * 64 + q % 64.
+// One block writes one 16 x 64 tile of every plane (1 KiB per plane, 4 bytes per thread), padding
+// columns >= n and k >= m with zero digits.  Grid: (ldp / 64, padded columns / 16).
+__global__ __launch_bounds__(256) void k_digit_write_i8t(int m, int n, const double *in, int64_t ldi,
+                                                         const int *colE, int S, int8_t *planes,
+                                                         int64_t ldp, int64_t pstride) {
+	const int t = threadIdx.x;
+	const int j = blockIdx.y * 16 + t / 16;
+	const int q0 = blockIdx.x * 64 + 4 * (t % 16);
+	Digits d[4];
+	if (j < n) {
+		const int E = colE[j];
+		const double *x = in + (int64_t)j * ldi;
+#pragma unroll
+		for (int e = 0; e < 4; e++) d[e] = make_digits(q0 + e < m ? x[q0 + e] : 0.0, E, S);
+	} else {
+#pragma unroll
+		for (int e = 0; e < 4; e++) d[e] = make_digits(0.0, 0, S);
+	}
+	int8_t *dst = planes + (int64_t)blockIdx.y * 16 * ldp + (int64_t)blockIdx.x * 1024 + 4 * t;
+	for (int s = 0; s < S; s++) {
+		uint32_t word = 0;
+#pragma unroll
+		for (int e = 0; e < 4; e++)
+			word |= (uint32_t)(uint8_t)digit_at(d[e], (S - 1 - s) - d[e].base) << (8 * e);
+		*(uint32_t *)(dst + (int64_t)s * pstride) = word;
+	}
+}
+
 // zero the padding rows [m, ldp) of every plane/column so MFMA tiles can read them
 template <typename T>
 __global__ void k_digit_pad(int m, int n, int S, T *planes, int64_t ldp, int64_t pstride) {
@@ -461,8 +491,10 @@ void launch_digit_write_t(hipStream_t st, int m, int n, const double *in, int64_
 
 void launch_digit_write(hipStream_t st, int elem, int m, int n, const double *in, int64_t ldi,
                         const int *colE, int S, void *planes, int64_t ldp, int64_t pstride) {
-	if (elem == 1)
-		launch_digit_write_t<int8_t>(st, m, n, in, ldi, colE, S, planes, ldp, pstride);
+	if (elem == 1) {  // tiled layout; ldp % 128 == 0 and pstride = ldp * (n rounded up to 256)
+		hipLaunchKernelGGL(k_digit_write_i8t, dim3((unsigned)(ldp / 64), (unsigned)(pstride / ldp / 16)),
+		                   dim3(256), 0, st, m, n, in, ldi, colE, S, (int8_t *)planes, ldp, pstride);
+	}
 	else if (elem == 2)
 		launch_digit_write_t<_Float16>(st, m, n, in, ldi, colE, S, planes, ldp, pstride);
 	else
