@@ -114,7 +114,10 @@ typedef struct ozblasHandle {
 	int32_t sliceType;    /* OZBLAS_SLICE_*, default OZBLAS_SLICE_FP64 */
 	int32_t lastStatus;   /* status of the last call (also returned by it) */
 	int32_t nLevelBins;   /* fast forms: number of exact level planes of the last GEMM */
-	int32_t reserved[5];
+	int32_t batchTile;    /* ozblasDgemmBatched: output tile edge picked by the autotuner (128, 256) */
+	int32_t batchReplay;  /* ozblasDgemmBatched: 1 when the call replayed the cached hipGraph */
+	int32_t batchGrouped; /* ozblasDgemmBatched: problems run in the grouped launch */
+	int32_t reserved[2];
 } ozblasHandle_t;
 
 /* lifecycle: cuozblasCreate/cuozblasDestroy (include/cuozblas.h:67-68, src/cuda/cuozblas.cpp:23-75).

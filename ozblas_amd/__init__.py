@@ -48,7 +48,8 @@ class OzblasHandleT(C.Structure):
         ("t_Sum_total", C.c_double), ("t_AXPY_SCAL_total", C.c_double),
         ("t_DOT_NRM2_total", C.c_double), ("t_SpMV_SpMM_total", C.c_double),
         ("sliceType", C.c_int32), ("lastStatus", C.c_int32), ("nLevelBins", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("batchTile", C.c_int32), ("batchReplay", C.c_int32), ("batchGrouped", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 

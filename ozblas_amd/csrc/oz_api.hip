@@ -43,13 +43,6 @@ static int32_t sync(ozblasHandle_t *oh) {
 	return OZBLAS_STATUS_SUCCESS;
 }
 
-// cap on slices of the reference split loop (src/cpu/ozblas_split.cpp:275-277)
-static inline int split_cap(const ozblasHandle_t *oh) {
-	int maxS = (oh->nSplitMax > 0) ? oh->nSplitMax : kNumSplitDefaultMax;
-	if (oh->reproMode == 0) maxS--;
-	return maxS;
-}
-
 // reference slice count from the per-column first-zero index (max over columns)
 static inline int split_count(const ozblasHandle_t *oh, int zmax) {
 	const int cap = split_cap(oh);
@@ -283,10 +276,6 @@ static int32_t gemm_plain(ozblasHandle_t *oh, char transA, char transB, int m, i
 // ---------------------------------------------------------------------------------------------
 // GEMM, fast forms: exact digit planes (no reference counterpart; same bits as NearSum)
 // ---------------------------------------------------------------------------------------------
-struct ExpStatsH {
-	int maxS, minLsb, maxE, nonfinite, maxSpan, pad[3];
-};
-
 struct DigitOperand {
 	void *planes = nullptr;
 	int64_t ldp = 0, pstride = 0;  // in elements
@@ -324,21 +313,9 @@ static int32_t digit_split(ozblasHandle_t *oh, int elem, int k, int cols, const 
 	return OZBLAS_STATUS_SUCCESS;
 }
 
-// true when the plain FP64 split of these operands is exact (so NearSum == RN(exact)) and no
-// slice product can underflow or overflow -- the precondition for the fast forms' bit identity
 static bool fast_form_valid(const ozblasHandle_t *oh, const DigitOperand &a, const DigitOperand &b,
                             int k) {
-	if (oh->nSplitMax != 0 || oh->sumMode != 1 || oh->splitShift != 1) return false;
-	if (a.st.nonfinite || b.st.nonfinite) return false;
-	if (a.S == 0 || b.S == 0) return true;  // an all-zero operand: exact zeros either way
-	const int rho = rho_host(k, oh->splitEpsMode);
-	const int bits = 53 - rho - 1;  // every FP64 slice removes at least this many bits
-	const int cap = split_cap(oh);
-	if (bits <= 0) return false;
-	if (a.st.maxSpan > bits * (cap - 3) || b.st.maxSpan > bits * (cap - 3)) return false;
-	if (a.st.minLsb + b.st.minLsb < -1000 || a.st.maxE + b.st.maxE > 1000) return false;
-	if (a.st.minLsb < -1000 || b.st.minLsb < -1000) return false;
-	return true;
+	return fast_valid(oh, a.st, b.st, k);
 }
 
 static int32_t gemm_fast(ozblasHandle_t *oh, char transA, char transB, int m, int n, int k,
@@ -849,19 +826,4 @@ done:
 	oh->splitShift = 1;
 	wreset(oh, 0);
 	return oh->lastStatus = s;
-}
-
-extern "C" int32_t ozblasDgemmBatched(ozblasHandle_t *oh, int32_t batch, const char *transA,
-                                      const char *transB, const int32_t *m, const int32_t *n,
-                                      const int32_t *k, const double *alpha,
-                                      const double *const *A, const int32_t *lda,
-                                      const double *const *B, const int32_t *ldb,
-                                      const double *beta, double *const *C, const int32_t *ldc) {
-	OZ_TRY(ensure_init(oh));
-	for (int p = 0; p < batch; p++) {
-		int32_t s = ozblasDgemm(oh, transA[p], transB[p], m[p], n[p], k[p], alpha[p], A[p], lda[p],
-		                        B[p], ldb[p], beta[p], C[p], ldc[p]);
-		if (s != OZBLAS_STATUS_SUCCESS) return s;
-	}
-	return OZBLAS_STATUS_SUCCESS;
 }

@@ -103,6 +103,8 @@ extern "C" int32_t ozblasDestroy(ozblasHandle_t *oh) {
 	Backend *b = be(oh);
 	if (b->stream) (void)hipStreamSynchronize(b->stream);
 	else (void)hipDeviceSynchronize();
+	batch_plan_free(b->batch);
+	b->batch = nullptr;
 	for (auto &e : b->ev)
 		if (e) (void)hipEventDestroy(e);
 	(void)hipFree(b->work);

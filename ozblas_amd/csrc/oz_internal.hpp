@@ -48,7 +48,10 @@ struct Backend {
 	int64_t *hPinned = nullptr;  // pinned host scratch (64 x 8 bytes)
 	int32_t *dCounters = nullptr;  // device scratch counters (256 x 4 bytes)
 	hipEvent_t ev[8] = {};
+	void *batch = nullptr;  // cached plan of ozblasDgemmBatched (oz_batched.hip)
 };
+
+void batch_plan_free(void *plan);
 
 inline Backend *be(ozblasHandle_t *oh) { return reinterpret_cast<Backend *>(oh->backend); }
 
@@ -131,6 +134,38 @@ __host__ __device__ inline int rho_of_dim(int dim) {
 
 // host helpers (oz_handle.cpp)
 int rho_host(int dim, int splitEpsMode);
+
+// cap on slices of the reference split loop (src/cpu/ozblas_split.cpp:275-277)
+inline int split_cap(const ozblasHandle_t *oh) {
+	int maxS = (oh->nSplitMax > 0) ? oh->nSplitMax : kNumSplitDefaultMax;
+	if (oh->reproMode == 0) maxS--;
+	return maxS;
+}
+
+// host image of the digit statistics of one operand (ExpStats of oz_digits.hpp)
+struct ExpStatsH {
+	int maxS, minLsb, maxE, nonfinite, maxSpan, pad[3];
+};
+
+// True when the plain FP64 split of these operands (digit statistics a of A, b of B, inner
+// dimension k) is exact, so NearSum returns RN(exact), and no slice product can underflow or
+// overflow: the precondition for the fast forms' bit identity with the plain form.
+inline bool fast_valid(const ozblasHandle_t *oh, const ExpStatsH &a, const ExpStatsH &b, int k) {
+	if (oh->nSplitMax != 0 || oh->sumMode != 1 || oh->splitShift != 1) return false;
+	if (a.nonfinite || b.nonfinite) return false;
+	if (a.maxS == 0 || b.maxS == 0) return true;  // an all-zero operand: exact zeros either way
+	const int rho = rho_host(k, oh->splitEpsMode);
+	const int bits = 53 - rho - 1;  // every FP64 slice removes at least this many bits
+	const int cap = split_cap(oh);
+	if (bits <= 0) return false;
+	if (a.maxSpan > bits * (cap - 3) || b.maxSpan > bits * (cap - 3)) return false;
+	if (a.minLsb + b.minLsb < -1000 || a.maxE + b.maxE > 1000) return false;
+	if (a.minLsb < -1000 || b.minLsb < -1000) return false;
+	// the exact reconstruction holds 8 Lmax + 41 bits in at most 9 words (k_reconstruct<9>)
+	if (8 * (a.maxS + b.maxS - 2) + 41 > 9 * 64) return false;
+	return true;
+}
+
 const uint16_t *tau_table_host();
 int32_t ensure_init(ozblasHandle_t *oh);
 

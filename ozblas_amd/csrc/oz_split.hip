@@ -10,22 +10,9 @@
 // Vectors share one tau across all elements, so their split is one grid-wide pass per slice.
 //
 // Fast forms (INT8/FP16/BF16): exact balanced radix-256 digit planes, see k_digit_*.
-#include "oz_kernels.hpp"
+#include "oz_digits.hpp"
 
 namespace oz {
-
-template <int BS>
-__device__ inline double block_max(double v, double *red) {
-	// wavefront max via cross-lane shuffles, then across the waves through LDS
-	for (int o = 32; o > 0; o >>= 1) v = std_max(v, __shfl_xor(v, o, 64));
-	const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-	__syncthreads();
-	if (l == 0) red[w] = v;
-	__syncthreads();
-	double r = red[0];
-	for (int i = 1; i < BS / 64; i++) r = std_max(r, red[i]);
-	return r;
-}
 
 // ---------------------------------------------------------------------------------------------
 // dense: one workgroup per column of a k-contiguous operand (column j = in + j*ldi, length m)
@@ -208,143 +195,17 @@ __global__ void k_zero_slots_csr(int m, const int *rowptr, double *split, int64_
 // balanced base-256 representation of the integer x / 2^(E_j - 8S).  S is the smallest count for
 // which every element is an integer multiple of 2^(E_j - 8S); the operand uses S = max_j S_j.
 // ---------------------------------------------------------------------------------------------
-struct ExpStats {  // global reductions written by k_digit_stats
-	int maxS;        // max digit count over columns
-	int minLsb;      // min exponent of the lowest set bit over all nonzero elements
-	int maxE;        // max column scale exponent
-	int nonfinite;   // any Inf/NaN
-	int maxSpan;     // max (top exponent - lowest bit) over columns
-	int pad[3];
-};
-
-// exponent of the lowest set bit of a finite nonzero double
-__device__ inline int lsb_exp(double x) {
-	const uint64_t b = d2u(x);
-	const int E = (int)((b >> 52) & 0x7ff);
-	uint64_t f = b & ((uint64_t(1) << 52) - 1);
-	if (E == 0) return -1074 + __builtin_ctzll(f);
-	f |= (uint64_t(1) << 52);
-	return (E - 1075) + __builtin_ctzll(f);
-}
-
-template <int BS>
-__device__ inline int block_max_i(int v, int *red) {
-	for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-	const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-	__syncthreads();
-	if (l == 0) red[w] = v;
-	__syncthreads();
-	int r = red[0];
-	for (int i = 1; i < BS / 64; i++) r = max(r, red[i]);
-	__syncthreads();
-	return r;
-}
-
 template <int BS>
 __global__ __launch_bounds__(BS) void k_digit_stats(int m, int n, const double *in, int64_t ldi,
                                                     int *colE, int *colS, ExpStats *st) {
 	__shared__ int red[BS / 64];
 	__shared__ double redd[BS / 64];
-	for (int j = blockIdx.x; j < n; j += gridDim.x) {
-		const double *x = in + (int64_t)j * ldi;
-		int nlo = INT32_MIN, bad = 0;  // nlo = -(min lsb)
-		double mx = 0.0;
-		for (int i = threadIdx.x; i < m; i += BS) {
-			const double v = x[i];
-			if (!isfinite(v)) {
-				bad = 1;
-				continue;
-			}
-			if (v == 0.0) continue;
-			mx = std_max(mx, fabs(v));
-			nlo = max(nlo, -lsb_exp(v));
-		}
-		mx = block_max<BS>(mx, redd);
-		nlo = block_max_i<BS>(nlo, red);
-		bad = block_max_i<BS>(bad, red);
-		if (threadIdx.x == 0) {
-			int E = 0, S = 0;
-			if (mx > 0.0) {
-				int hi;
-				frexp(mx, &hi);  // mx in [2^(hi-1), 2^hi)
-				// smallest E with mx <= 127 * 2^(E-8): hi+1 when mx <= (127/128) 2^hi, else hi+2
-				E = (mx <= ldexp(127.0, hi - 7)) ? hi + 1 : hi + 2;
-				const int lo = -nlo;
-				S = (E - lo + 7) / 8;  // 8S >= E - lo
-				atomicMin(&st->minLsb, lo);
-				atomicMax(&st->maxSpan, hi - lo);
-			}
-			colE[j] = E;
-			colS[j] = S;
-			atomicMax(&st->maxS, S);
-			atomicMax(&st->maxE, E);
-			if (bad) atomicOr(&st->nonfinite, 1);
-		}
-	}
+	for (int j = blockIdx.x; j < n; j += gridDim.x)
+		digit_col_stats<BS>(in + (int64_t)j * ldi, m, colE + j, colS + j, st, red, redd);
 }
 
-// digit plane writer: plane s of column j at planes + s*pstride + j*ldp (k-contiguous, bytes or
-// halfs).  Each thread handles 4 consecutive elements per step.
-template <typename T>
-__device__ inline T digit_cast(int d);
-template <>
-__device__ inline int8_t digit_cast<int8_t>(int d) { return (int8_t)d; }
-template <>
-__device__ inline _Float16 digit_cast<_Float16>(int d) { return (_Float16)(float)d; }
-template <>
-__device__ inline __bf16 digit_cast<__bf16>(int d) { return (__bf16)(float)d; }
-
-// X = M + 0x80808080808080808080 (80-bit, two words) for the element's aligned integer M;
-// digit q of M (from the bottom) = byte q of X - 128
-struct Digits {
-	uint64_t lo, hi;
-	int base;  // plane-index offset: digit q lands in plane S-1-(q+base)
-	bool any;
-};
-
-__device__ inline Digits make_digits(double v, int E, int S) {
-	Digits d{0, 0, 0, false};
-	if (v == 0.0 || !isfinite(v)) return d;
-	const uint64_t b = d2u(v);
-	const int Ex = (int)((b >> 52) & 0x7ff);
-	uint64_t f = b & ((uint64_t(1) << 52) - 1);
-	int e0;  // v = +-f * 2^e0
-	if (Ex == 0) {
-		e0 = -1074;
-	} else {
-		f |= (uint64_t(1) << 52);
-		e0 = Ex - 1075;
-	}
-	const int bottom = E - 8 * S;        // exponent of the unit of the lowest plane
-	const int sh = e0 - bottom;          // >= 0 by the choice of S (or f has trailing zeros)
-	int q0, bits;
-	if (sh >= 0) {
-		q0 = sh / 8;
-		bits = sh % 8;
-	} else {  // e0 below the grid: trailing zeros of f cover it (checked by the stats pass)
-		f >>= -sh;
-		q0 = 0;
-		bits = 0;
-	}
-	const int64_t M = (int64_t)(f << bits) * ((b >> 63) ? -1 : 1);  // |M| < 2^61
-	// 128-bit X = M + K, K = 0x80 in bytes 0..9
-	const uint64_t Klo = 0x8080808080808080ULL, Khi = 0x8080ULL;
-	const uint64_t mlo = (uint64_t)M, mhi = (M < 0) ? ~0ULL : 0ULL;
-	const uint64_t lo = mlo + Klo;
-	const uint64_t carry = lo < mlo ? 1 : 0;
-	d.lo = lo;
-	d.hi = mhi + Khi + carry;
-	d.base = q0;
-	d.any = true;
-	return d;
-}
-
-__device__ inline int digit_at(const Digits &d, int q) {  // q relative to the element's base
-	if (!d.any || q < 0 || q >= 10) return 0;
-	const uint64_t w = (q < 8) ? (d.lo >> (8 * q)) : (d.hi >> (8 * (q - 8)));
-	return (int)(w & 0xff) - 128;
-}
-
+// digit plane writer for 2-byte digits: plane s of column j at planes + s*pstride + j*ldp
+// (k-contiguous).
 template <typename T>
 __global__ __launch_bounds__(256) void k_digit_write(int m, int n, const double *in, int64_t ldi,
                                                      const int *colE, int S, T *planes,
@@ -361,34 +222,11 @@ __global__ __launch_bounds__(256) void k_digit_write(int m, int n, const double 
 	}
 }
 
-// INT8 planes in the tiled layout of the MFMA kernel: element (column j, k index q) of plane s at
-//   s * pstride + (j / 16) * 16 * ldp + (q / 64) * 1024 + (j % 16) * 64 + q % 64.
-// One block writes one 16 x 64 tile of every plane (1 KiB per plane, 4 bytes per thread), padding
-// columns >= n and k >= m with zero digits.  Grid: (ldp / 64, padded columns / 16).
+// INT8 planes in the tiled layout (digit_write_tile_i8).  Grid: (ldp / 64, padded columns / 16).
 __global__ __launch_bounds__(256) void k_digit_write_i8t(int m, int n, const double *in, int64_t ldi,
                                                          const int *colE, int S, int8_t *planes,
                                                          int64_t ldp, int64_t pstride) {
-	const int t = threadIdx.x;
-	const int j = blockIdx.y * 16 + t / 16;
-	const int q0 = blockIdx.x * 64 + 4 * (t % 16);
-	Digits d[4];
-	if (j < n) {
-		const int E = colE[j];
-		const double *x = in + (int64_t)j * ldi;
-#pragma unroll
-		for (int e = 0; e < 4; e++) d[e] = make_digits(q0 + e < m ? x[q0 + e] : 0.0, E, S);
-	} else {
-#pragma unroll
-		for (int e = 0; e < 4; e++) d[e] = make_digits(0.0, 0, S);
-	}
-	int8_t *dst = planes + (int64_t)blockIdx.y * 16 * ldp + (int64_t)blockIdx.x * 1024 + 4 * t;
-	for (int s = 0; s < S; s++) {
-		uint32_t word = 0;
-#pragma unroll
-		for (int e = 0; e < 4; e++)
-			word |= (uint32_t)(uint8_t)digit_at(d[e], (S - 1 - s) - d[e].base) << (8 * e);
-		*(uint32_t *)(dst + (int64_t)s * pstride) = word;
-	}
+	digit_write_tile_i8(m, n, in, ldi, colE, S, planes, ldp, pstride, blockIdx.x, blockIdx.y);
 }
 
 // zero the padding rows [m, ldp) of every plane/column so MFMA tiles can read them
