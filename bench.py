@@ -9,7 +9,13 @@ outputs are compared bit for bit.  Inputs are resident in HBM before the timed r
 Prints one JSON line (see the field list at the bottom).  Extra fields: the FP64-slice time and
 the speedup, the slice counts, the phi = 4 variant, the dominant kernel's roofline (MFMA), and the
 CPU baseline: the oracle restatement of the reference CPU OzBLAS timed on this host's cores on a
-bounded sample.
+bounded sample.  Under "results" two more workloads are reported beside the headline:
+  cg_laplace100: reproducible CG (Ozaki DOT/NRM2/CSRMV, NearSum) on the 3-D 7-point Laplacian of
+    a 100^3 grid (n = 10^6), 500 iterations, residual history checked bit for bit against the
+    committed golden history;
+  batched256: 256 uneven Ozaki DGEMMs (m, n, k uniform in [64, 1024] from srand48(4), phi = 2,
+    seeds 1000 + 2p / 1001 + 2p) in the grouped INT8 launch replayed from its hipGraph, against
+    the same problems one ozblasDgemm call at a time.
 """
 from __future__ import annotations
 
@@ -36,7 +42,9 @@ def parse():
     ap.add_argument("--phi2", type=float, default=4.0, help="second phi value (0 = skip)")
     ap.add_argument("--form", default="int8", choices=["int8", "fp16", "bf16"])
     ap.add_argument("--fp64-steps", type=int, default=2)
-    ap.add_argument("--cpu-n", type=int, default=1536, help="CPU baseline sample size (0 = skip)")
+    ap.add_argument("--cpu-n", type=int, default=2560, help="CPU baseline sample size (0 = skip)")
+    ap.add_argument("--cg-iters", type=int, default=500, help="CG iterations on 100^3 (0 = skip)")
+    ap.add_argument("--batch", type=int, default=256, help="batched problems (0 = skip)")
     ap.add_argument("--work-gb", type=float, default=64.0)
     return ap.parse_args()
 
@@ -76,6 +84,120 @@ def cpu_baseline(n_cpu, phi):
                       f"OpenMP on {cores} threads, {t:.2f} s"}
 
 
+def drand48_stream(seed):
+    """POSIX drand48 after srand48(seed): X <- (0x5DEECE66D X + 0xB) mod 2^48, value X / 2^48."""
+    x = ((seed & 0xFFFFFFFF) << 16) | 0x330E
+    while True:
+        x = (0x5DEECE66D * x + 0xB) % (1 << 48)
+        yield x / float(1 << 48)
+
+
+def laplace3d(g, shift):
+    """3-D 7-point Laplacian on a g^3 grid (lexicographic, i fastest), diagonal 6 + shift, CSR
+    with int32 indices and sorted columns."""
+    import numpy as np
+    n = g ** 3
+    idx = np.arange(n, dtype=np.int64)
+    i, j, k = idx % g, (idx // g) % g, idx // (g * g)
+    cols, vals = [], []
+    for (di, dj, dk, v) in ((0, 0, -1, -1.0), (0, -1, 0, -1.0), (-1, 0, 0, -1.0), (0, 0, 0, 6.0 + shift),
+                            (1, 0, 0, -1.0), (0, 1, 0, -1.0), (0, 0, 1, -1.0)):
+        ok = (i + di >= 0) & (i + di < g) & (j + dj >= 0) & (j + dj < g) & (k + dk >= 0) & (k + dk < g)
+        cols.append(np.where(ok, idx + di + dj * g + dk * g * g, -1))
+        vals.append(np.full(n, v))
+    C = np.stack(cols, 1)
+    V = np.stack(vals, 1)
+    keep = C >= 0
+    rowptr = np.zeros(n + 1, dtype=np.int32)
+    rowptr[1:] = np.cumsum(keep.sum(1))
+    return rowptr, C[keep].astype(np.int32), V[keep]
+
+
+def bench_cg(iters, torch, oz):
+    """Config: reproducible CG on the 100^3 Laplacian (n = 10^6), NearSum, `iters` iterations."""
+    import hashlib
+    import numpy as np
+    g = 100
+    rowptr, colind, val = laplace3d(g, 1e-3)
+    n, nnz = g ** 3, len(val)
+    r = drand48_stream(3)
+    b = np.array([2.0 * next(r) - 1.0 for _ in range(n)])
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+    dval, dcol, drow, db = d(val), d(colind), d(rowptr), d(b)
+    H = oz.Handle(4 << 30, sumMode=1)
+    dx = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = H.dcg(n, nnz, dval, dcol, drow, db, dx, iters, 0.0)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    out = {"iterations": iters, "seconds": round(t, 3), "ms_per_iteration": round(t / iters * 1e3, 3),
+           "final_residual": float(hist[-1]), "nSplitA_": int(H.nSplitA_), "splitShift": int(H.splitShift),
+           # reference accounting of one iteration (testing/testing_solver.cpp:146-149)
+           "ref_bytes_per_iteration": 12 * nnz + 140 * n,
+           "ref_GBps": round((12 * nnz + 140 * n) / (t / iters) / 1e9, 1)}
+    fix = os.path.join(ROOT, "tests", "golden", "cg_laplace100.json")
+    if os.path.exists(fix) and iters == 500:
+        case = json.load(open(fix))["cases"]["1"]
+        out["history_bitwise_equal_to_cpu"] = [float(v).hex() for v in hist] == \
+            [float.fromhex(v).hex() for v in case["history_hex"]]
+        out["x_bitwise_equal_to_cpu"] = hashlib.sha256(
+            dx.cpu().numpy().tobytes()).hexdigest() == case["x_sha256"]
+        out["cpu_oracle_seconds_500_iterations"] = case["cpu_seconds"]
+    H.close()
+    return out
+
+
+def bench_batched(count, torch, oz):
+    """Config: `count` uneven Ozaki DGEMMs, m, n, k = 64 + floor(961 drand48()) (srand48(4)),
+    phi = 2 inputs (seeds 1000 + 2p, 1001 + 2p), NearSum, INT8 digit form."""
+    r = drand48_stream(4)
+    sizes = [tuple(64 + int(961 * next(r)) for _ in range(3)) for _ in range(count)]
+    As = [torch.from_numpy(oz.gen_phi(m, k, m, 2.0, 1000 + 2 * p)).to("cuda:0") for p, (m, n, k) in enumerate(sizes)]
+    Bs = [torch.from_numpy(oz.gen_phi(k, n, k, 2.0, 1001 + 2 * p)).to("cuda:0") for p, (m, n, k) in enumerate(sizes)]
+    Cs = [torch.zeros(m * n, dtype=torch.float64, device="cuda:0") for m, n, k in sizes]
+    flops = sum(2.0 * m * n * k for m, n, k in sizes)
+    ms, ns, ks = [s[0] for s in sizes], [s[1] for s in sizes], [s[2] for s in sizes]
+    H = oz.Handle(1 << 30, sumMode=1, sliceType="int8")
+    call = lambda: H.dgemm_batched(["N"] * count, ["N"] * count, ms, ns, ks, [1.0] * count, As, ms,
+                                   Bs, ks, [0.0] * count, Cs, ms)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    call()  # plan, autotune, capture, run
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    ts = []
+    for _ in range(5):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        call()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    tb = sorted(ts)[len(ts) // 2]
+    out = {"problems": count, "sum_2mnk": flops, "tile": int(H.batchTile), "replayed": int(H.batchReplay),
+           "grouped_problems": int(H.batchGrouped), "digit_pairs": int(H.nSplitC),
+           "first_call_s": round(t_first, 3), "ms_per_batch": round(tb * 1e3, 3),
+           "eff_tflops": round(flops / tb / 1e12, 3),
+           "grouped_mfma_kernel_ms": round(H.t_comp * 1e3, 3),
+           "grouped_mfma_tops": round(H.n_comp / H.t_comp / 1e12, 1) if H.t_comp > 0 else None}
+    Hs = oz.Handle(8 << 30, sumMode=1, sliceType="int8")
+    Cl = [torch.zeros_like(c) for c in Cs]
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for p, (m, n, k) in enumerate(sizes):
+            Hs.dgemm("N", "N", m, n, k, 1.0, As[p], m, Bs[p], k, 0.0, Cl[p], m)
+        torch.cuda.synchronize()
+        tl = time.perf_counter() - t0
+    out["loop_ms"] = round(tl * 1e3, 3)
+    out["speedup_vs_loop"] = round(tl / tb, 2)
+    out["bitwise_equal_to_individual_calls"] = all(
+        torch.equal(a.view(torch.int64), c.view(torch.int64)) for a, c in zip(Cs, Cl))
+    H.close()
+    Hs.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -104,10 +226,15 @@ def main():
         fast_bits = dC.clone()
         Hf.close()
         dC.zero_()
-        H64 = ozblas_amd.Handle(work, sumMode=1, reproMode=1, sliceType="fp64")
-        t_64, tk_64, info_64 = timed_gemm(H64, n, dA, dB, dC, args.fp64_steps, 1, torch)
-        H64.close()
-        same = bool(torch.equal(fast_bits.view(torch.int64), dC.view(torch.int64)))
+        if args.fp64_steps > 0:
+            H64 = ozblas_amd.Handle(work, sumMode=1, reproMode=1, sliceType="fp64")
+            t_64, tk_64, info_64 = timed_gemm(H64, n, dA, dB, dC, args.fp64_steps, 1, torch)
+            H64.close()
+            same = bool(torch.equal(fast_bits.view(torch.int64), dC.view(torch.int64)))
+        else:
+            t_64 = tk_64 = float("nan")
+            info_64 = dict(nSplitA=0, nSplitB=0, nSplitC=0)
+            same = None
         res[phi] = dict(t_fast=t_fast, tk_fast=tk_fast, info_fast=info_fast, t_64=t_64,
                         tk_64=tk_64, info_64=info_64, same=same, steps=steps)
         print(f"phi={phi}: {args.form} {t_fast*1e3:.1f} ms/step (MFMA kernel {tk_fast*1e3:.1f} ms, "
@@ -131,6 +258,16 @@ def main():
     cpu = cpu_baseline(args.cpu_n, args.phi) if args.cpu_n > 0 else None
     extra = {}
     for phi, rr in res.items():
+        if rr["same"] is None:  # FP64-slice form not run (--fp64-steps 0)
+            extra[f"phi{phi:g}"] = {
+                f"{args.form}_tflops": round(flops / rr["t_fast"] / 1e12, 3),
+                "digit_planes": [rr["info_fast"]["nSplitA"], rr["info_fast"]["nSplitB"]],
+                "digit_pairs": rr["info_fast"]["nSplitC"],
+                f"{args.form}_mfma_kernel_ms": round(rr["tk_fast"] * 1e3, 2),
+                "split_ms": round(rr["info_fast"]["t_split"] * 1e3, 2),
+                "reconstruct_ms": round(rr["info_fast"]["t_sum"] * 1e3, 2),
+            }
+            continue
         extra[f"phi{phi:g}"] = {
             f"{args.form}_tflops": round(flops / rr["t_fast"] / 1e12, 3),
             "fp64_slice_tflops": round(flops / rr["t_64"] / 1e12, 3),
@@ -146,6 +283,10 @@ def main():
             "split_ms": round(rr["info_fast"]["t_split"] * 1e3, 2),
             "reconstruct_ms": round(rr["info_fast"]["t_sum"] * 1e3, 2),
         }
+    if args.cg_iters > 0:
+        extra["cg_laplace100"] = bench_cg(args.cg_iters, torch, ozblas_amd)
+    if args.batch > 0:
+        extra[f"batched{args.batch}"] = bench_batched(args.batch, torch, ozblas_amd)
     line = {
         "metric": "Ozaki DGEMM eff. TFLOPS (2mnk/t): FP16/INT8-slice vs FP64-slice, same bits",
         "value": round(value, 3), "unit": "TFLOPS", "n_gpus": 1, "steps": r["steps"],

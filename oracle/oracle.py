@@ -32,7 +32,7 @@ class OzoHandle(C.Structure):
         ("mbk", C.c_int32), ("nbk", C.c_int32),
         ("trueresFlag", C.c_int32), ("verbose", C.c_int32), ("cg_numiter", C.c_int32),
         ("cg_verbose1", C.c_void_p), ("cg_verbose2", C.c_void_p),
-        ("fastModeRule", C.c_int32),
+        ("fastModeRule", C.c_int32), ("dot2Chunks", C.c_int32),
     ]
 
 
@@ -50,6 +50,11 @@ def lib():
         L.ozo_handle_new.restype = H
         L.ozo_handle_new.argtypes = [C.c_uint64]
         L.ozo_handle_free.argtypes = [H]
+        dp = C.POINTER(C.c_double)
+        L.ozo_dd_dot2i.argtypes = [C.c_double, C.c_double, dp, dp]
+        L.ozo_dd_quad_add.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, dp, dp]
+        L.ozo_dd_quad_mul.argtypes = [C.c_double, C.c_double, C.c_double, dp, dp]
+        L.ozo_dot2_chunked.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, dp, dp]
         L.ozo_rho.restype = C.c_int32
         L.ozo_rho.argtypes = [C.c_int32, C.c_int32]
         L.ozo_tau.restype = C.c_int32
@@ -280,3 +285,30 @@ def laplace3d(g, shift):
     val = np.zeros(nnz, dtype=np.float64)
     lib().ozo_laplace3d(g, float(shift), rowptr, colind, val)
     return rowptr, colind, val
+
+
+# Dot2 building blocks (splitEpsMode = 2) ------------------------------------------------------
+def dd_dot2i(a, b, c=(0.0, 0.0)):
+    x, y = C.c_double(c[0]), C.c_double(c[1])
+    lib().ozo_dd_dot2i(float(a), float(b), C.byref(x), C.byref(y))
+    return x.value, y.value
+
+
+def dd_quad_add(a, b):
+    x, y = C.c_double(), C.c_double()
+    lib().ozo_dd_quad_add(a[0], a[1], b[0], b[1], C.byref(x), C.byref(y))
+    return x.value, y.value
+
+
+def dd_quad_mul(a, b):
+    x, y = C.c_double(), C.c_double()
+    lib().ozo_dd_quad_mul(float(a), b[0], b[1], C.byref(x), C.byref(y))
+    return x.value, y.value
+
+
+def dot2_chunked(x, y, T):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    hi, lo = C.c_double(), C.c_double()
+    lib().ozo_dot2_chunked(len(x), x.ctypes.data, y.ctypes.data, int(T), C.byref(hi), C.byref(lo))
+    return hi.value, lo.value

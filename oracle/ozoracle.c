@@ -58,6 +58,7 @@ void ozo_create(ozo_handle *h, uint64_t workSizeBytes) {
 	h->useBatchedGemmFlag = 0;
 	h->splitEpsMode = 0;
 	h->precxFlag = 0;
+	h->dot2Chunks = 1024;
 	/* lazily-zeroed pages: the reference's aligned_alloc of a multi-GB workspace is served by
 	   fresh (zero) pages too, so untouched slots read as +0 in both */
 	void *p = mmap(NULL, workSizeBytes + 4096, PROT_READ | PROT_WRITE,
@@ -436,6 +437,119 @@ static void csrmm_n(int32_t m, int32_t n, double alpha, const double *val, const
 		}
 }
 
+/* ---- Dot2: error-free transformations, scalar forms of src/cpu/eft.h:8-182 ---------------- */
+typedef struct {
+	double x, y;
+} dd_t;
+
+static void two_sum(double a, double b, double *s, double *e) { /* eft.h TwoSum */
+	double v;
+	*s = a + b;
+	v = *s - a;
+	*e = (a - (*s - v)) + (b - v);
+}
+static void quick_two_sum(double a, double b, double *s, double *e) { /* eft.h QuickTwoSum */
+	*s = a + b;
+	*e = b - (*s - a);
+}
+static void two_prod(double a, double b, double *p, double *e) { /* eft.h TwoProdFMA */
+	*p = a * b;
+	*e = fma(a, b, -*p);
+}
+static void dot2i(double a, double b, dd_t *c) { /* eft.h Dot2i */
+	double h, r, q;
+	two_prod(a, b, &h, &r);
+	two_sum(c->x, h, &c->x, &q);
+	c->y = c->y + (q + r);
+}
+static dd_t quad_add(dd_t a, dd_t b) { /* eft.h QuadAdd */
+	dd_t t, c;
+	two_sum(a.x, b.x, &t.x, &t.y);
+	t.y = t.y + (a.y + b.y);
+	quick_two_sum(t.x, t.y, &c.x, &c.y);
+	return c;
+}
+static dd_t quad_mul_dd(double a, dd_t b) { /* eft.h QuadMul(double, double2) */
+	dd_t t, c;
+	two_prod(a, b.x, &t.x, &t.y);
+	t.y = fma(a, b.y, t.y);
+	quick_two_sum(t.x, t.y, &c.x, &c.y);
+	return c;
+}
+
+/* libgomp's static partition of [0, n) over T threads: thread t gets q (+1 for t < r) iterations */
+static void static_chunk(int64_t n, int32_t T, int32_t t, int64_t *lo, int64_t *hi) {
+	const int64_t q = n / T, r = n % T;
+	*lo = t * q + (t < r ? t : r);
+	*hi = *lo + q + (t < r ? 1 : 0);
+}
+
+/* dgemm_tn_skinny_x2 (src/cpu/ozblas_blas_wrapper.cpp:109-152), alpha = 1, beta = 0, flag = 1:
+   C[nn*ldc + mm] = hi, C[nn*ldc + mm + n*ldc] = lo of the Dot2 sum of column mm of A and column nn
+   of B; T per-thread partials combined in thread order (see ozo_handle.dot2Chunks) */
+static void gemm_tn_x2(int32_t m, int32_t n, int32_t k, const double *A, int32_t lda, const double *B,
+                       int32_t ldb, double *C, int32_t ldc, int32_t T) {
+	if (T < 1) T = 1;
+	for (int32_t nn = 0; nn < n; nn++)
+		for (int32_t mm = 0; mm < m; mm++) {
+			dd_t G = {0., 0.};
+			for (int32_t t = 0; t < T; t++) {
+				int64_t lo, hi;
+				static_chunk(k, T, t, &lo, &hi);
+				dd_t L = {0., 0.};
+				for (int64_t kk = lo; kk < hi; kk++)
+					dot2i(A[(int64_t)mm * lda + kk], B[(int64_t)nn * ldb + kk], &L);
+				G = quad_add(G, L);
+			}
+			G = quad_mul_dd(1., G);
+			C[(int64_t)nn * ldc + mm] = G.x;
+			C[(int64_t)nn * ldc + mm + (int64_t)n * ldc] = G.y;
+		}
+}
+
+/* csrmm_n_x2 (src/cpu/ozblas_blas_wrapper.cpp:262-295; the _l2.._l5 forms compute the same
+   sequence per row), alpha = 1, beta = 0, flag = 1: per row and x slice j, Dot2 over the row's
+   nonzeros in order; C[j*ldc + row] = hi, C[j*ldc + row + ldc*n] = lo */
+static void csrmm_n_x2(int32_t m, int32_t n, const double *val, const int32_t *colind,
+                       const int32_t *rowptr, const double *B, int32_t ldb, double *C, int32_t ldc) {
+#pragma omp parallel for schedule(static)
+	for (int32_t r = 0; r < m; r++)
+		for (int32_t j = 0; j < n; j++) {
+			dd_t Tr = {0., 0.};
+			for (int32_t i = rowptr[r]; i < rowptr[r + 1]; i++)
+				dot2i(val[i], B[(int64_t)j * ldb + colind[i]], &Tr);
+			Tr = quad_mul_dd(1., Tr);
+			C[(int64_t)j * ldc + r] = Tr.x;
+			C[(int64_t)j * ldc + r + (int64_t)ldc * n] = Tr.y;
+		}
+}
+
+/* test hooks for the EFT primitives */
+void ozo_dd_dot2i(double a, double b, double *cx, double *cy) {
+	dd_t c = {*cx, *cy};
+	dot2i(a, b, &c);
+	*cx = c.x;
+	*cy = c.y;
+}
+void ozo_dd_quad_add(double ax, double ay, double bx, double by, double *cx, double *cy) {
+	dd_t a = {ax, ay}, b = {bx, by};
+	const dd_t c = quad_add(a, b);
+	*cx = c.x;
+	*cy = c.y;
+}
+void ozo_dd_quad_mul(double a, double bx, double by, double *cx, double *cy) {
+	dd_t b = {bx, by};
+	const dd_t c = quad_mul_dd(a, b);
+	*cx = c.x;
+	*cy = c.y;
+}
+void ozo_dot2_chunked(int32_t n, const double *x, const double *y, int32_t T, double *hi, double *lo) {
+	double C2[2];
+	gemm_tn_x2(1, 1, n, x, n, y, n, C2, 1, T);
+	*hi = C2[0];
+	*lo = C2[1];
+}
+
 /* ---------------------------------------------------------------------------------------------
  * drivers
  * ------------------------------------------------------------------------------------------- */
@@ -548,11 +662,11 @@ int32_t ozo_gemm(ozo_handle *h, char transA, char transB, int32_t m, int32_t n, 
 			if (h->fastModeRule == 1 && h->fastMode) /* legacy rule, see ozo_handle.fastModeRule */
 				maxlevel = imin(nSplitA - 1 - (h->reproMode == 0), nSplitB - 1 - (h->reproMode == 0));
 			const int sumOk = (h->sumMode < 2 || h->sumMode == 30);
-			if (n == 1 && m == 1 && h->splitEpsMode == 2 && h->fastMode == 0 && sumOk) {
-				fprintf(stderr, "ozoracle: Dot2 (splitEpsMode=2) is not restated\n");
-				return -2;
-			}
-			if (n == 1 && m == 1 && h->fastMode == 0 && sumOk) { /* DOT :209-215 */
+			const int dot2 = (n == 1 && m == 1 && h->splitEpsMode == 2 && h->fastMode == 0 && sumOk);
+			if (dot2) { /* Dot2 DOT :166-173: (hi, lo) of every slice pair */
+				gemm_tn_x2(nSplitA, nSplitB, k, AS, ldas, BS, ldbs, CS, nSplitA, h->dot2Chunks);
+				ic++;
+			} else if (n == 1 && m == 1 && h->fastMode == 0 && sumOk) { /* DOT :209-215 */
 				gemm_tn(nSplitA, nSplitB, k, AS, ldas, BS, ldbs, CS, nSplitA);
 				ic++;
 			} else if (n == 1 && h->fastMode == 0 && sumOk) { /* GEMV :216-225 */
@@ -578,8 +692,10 @@ int32_t ozo_gemm(ozo_handle *h, char transA, char transB, int32_t m, int32_t n, 
 				double *Cblk = &C[(int64_t)ldc * (in * nbk) + (int64_t)im * mbk];
 				if (m == 1 && n == 1) {
 					const int32_t sumorder = (h->fastMode == 0) ? 2 : 1;
-					global_sum(h, 1, 1, nSplitA, nSplitB, CS, 1, 1, Cblk, ldc, alpha, beta, maxlevel,
-					           sumorder);
+					/* Dot2: the lo parts count as nSplitB more B slices (:281-285) */
+					if (h->splitEpsMode == 2) maxlevel = (nSplitA - 1) + (nSplitB * 2 - 1);
+					global_sum(h, 1, 1, nSplitA, nSplitB * ((h->splitEpsMode == 2) ? 2 : 1), CS, 1, 1,
+					           Cblk, ldc, alpha, beta, maxlevel, sumorder);
 				} else {
 					const int32_t sumorder = (h->fastMode == 0 && nbk_ == 1) ? 3 : 1;
 					global_sum(h, mbk_, nbk_, nSplitA, nSplitB, CS, (int64_t)ldcs * nbk_, ldcs, Cblk,
@@ -694,23 +810,26 @@ int32_t ozo_csrmv(ozo_handle *h, int32_t m, int32_t n, int32_t nnz, double alpha
 	const int32_t nSplitB = ozo_split(h, n, 1, x, n, WP(h, oBT, double), n, BS, ldbs, WP(h, oBmax, double));
 	int32_t maxlevel = (nSplitA - 1) + (nSplitB - 1);
 	maxlevel = imax(maxlevel - h->fastMode, imin(nSplitA - 1, nSplitB - 1));
-	if (h->splitEpsMode == 2) {
-		fprintf(stderr, "ozoracle: Dot2 (splitEpsMode=2) is not restated\n");
-		return -2;
-	}
 	double *pC = CS;
 	int32_t ic = 0;
 	for (int32_t ia = 0; ia < imin(maxlevel + 1, nSplitA); ia++) {
 		const int32_t numB = imin(nSplitB, maxlevel + 1 - ia);
-		csrmm_n(m, numB, 1., AS + (int64_t)ldas * ia, colind, rowptr, BS, ldbs, 0., pC, ldcs);
-		pC += (int64_t)ldcs * numB;
+		if (h->splitEpsMode == 2) { /* :139-142 */
+			csrmm_n_x2(m, numB, AS + (int64_t)ldas * ia, colind, rowptr, BS, ldbs, pC, ldcs);
+			pC += (int64_t)ldcs * numB * 2;
+		} else {
+			csrmm_n(m, numB, 1., AS + (int64_t)ldas * ia, colind, rowptr, BS, ldbs, 0., pC, ldcs);
+			pC += (int64_t)ldcs * numB;
+		}
 		ic += numB;
 	}
 	h->nSplitA += nSplitA;
 	h->nSplitB += nSplitB;
 	h->nSplitC += ic;
 	if (h->sumMode == 3) return -3;
-	return global_sum(h, m, 1, nSplitA, nSplitB, CS, ldcs, ldcs, y, 1, alpha, beta, maxlevel, 3);
+	if (h->splitEpsMode == 2) maxlevel = (nSplitA - 1) + (nSplitB * 2 - 1); /* :159-162 */
+	return global_sum(h, m, 1, nSplitA, nSplitB * ((h->splitEpsMode == 2) ? 2 : 1), CS, ldcs, ldcs,
+	                  y, 1, alpha, beta, maxlevel, 3);
 }
 
 double *ozo_csrmv_splitA(ozo_handle *h, int32_t m, int32_t n, int32_t nnz, const double *val,

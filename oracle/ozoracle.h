@@ -68,6 +68,12 @@ typedef struct {
 	      shipped fastmode=1 logs (script/a100 _fast1_ logs, byte-identical for the CPU and CUDA
 	      builds) are reproduced only under this rule. */
 	int32_t fastModeRule;
+	/* Dot2 DOT (splitEpsMode = 2): the reference's dgemm_tn_skinny_x2 sums each slice dot in
+	   per-thread partials (OpenMP static schedule over k) combined by QuadAdd inside a critical
+	   section (src/cpu/ozblas_blas_wrapper.cpp:109-152), so its bits depend on the thread count and
+	   the order threads enter the section.  dot2Chunks fixes one such execution: that many
+	   threads, libgomp's static partition, partials combined in thread order. */
+	int32_t dot2Chunks;
 } ozo_handle;
 
 /* handle: src/cpu/ozblas.cpp:22-71 (defaults), with the arena emulating devWork */
@@ -77,6 +83,12 @@ ozo_handle *ozo_handle_new(uint64_t workSizeBytes);
 void ozo_handle_free(ozo_handle *h);
 
 /* building blocks */
+/* Dot2 (splitEpsMode = 2) building blocks, src/cpu/eft.h: c += a*b (Dot2i), c = a + b (QuadAdd),
+   c = a * b (QuadMul), and one chunked Dot2 dot product (T libgomp static chunks in order) */
+void ozo_dd_dot2i(double a, double b, double *cx, double *cy);
+void ozo_dd_quad_add(double ax, double ay, double bx, double by, double *cx, double *cy);
+void ozo_dd_quad_mul(double a, double bx, double by, double *cx, double *cy);
+void ozo_dot2_chunked(int32_t n, const double *x, const double *y, int32_t T, double *hi, double *lo);
 int32_t ozo_rho(int32_t dim, int32_t splitEpsMode);          /* src/cpu/ozblas_split.cpp:20-34 */
 int32_t ozo_tau(double max);                                 /* (short)ceil(log2(max)) :86,158 */
 void ozo_tau_batch(int64_t n, const double *x, int32_t *out);

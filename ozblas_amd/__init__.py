@@ -84,6 +84,7 @@ _gen_phi = _sig("ozblasGenPhi", None, _i, _i, _i, _p, _d, C.c_int64)
 internal_tau = _sig("ozblas_internal_tau", _i, _d)
 internal_rho = _sig("ozblas_internal_rho", _i, _i, _i)
 internal_rho_row = _sig("ozblas_internal_rho_row", _i, _i)
+internal_rho_row_mode = _sig("ozblas_internal_rho_row_mode", _i, _i, _i)
 
 
 class OzblasError(RuntimeError):

@@ -475,6 +475,16 @@ static int32_t dot_core(ozblasHandle_t *oh, int n, const VecSplit &xs, const Vec
                         double *dres) {
 	Backend *b = be(oh);
 	const int nA = xs.count, nB = ys.count;
+	if (oh->splitEpsMode == 2) {  // Dot2: (hi, lo) of every slice pair, as 2 nB B slices (:166-173)
+		double *vals = (double *)walloc(oh, 16ull * nA * nB);
+		void *part = walloc(oh, dot2_partials_bytes(nA, nB));
+		if (!vals || !part) return OZBLAS_STATUS_WORKSPACE;
+		launch_dot2_slices(b->stream, n, xs.slices, xs.lds, nA, ys.slices, ys.lds, nB, part, vals);
+		const int maxlevel = (nA - 1) + (2 * nB - 1);  // :281-285
+		launch_nearsum_small(b->stream, 2 * nA * nB, vals, 1.0, 0.0, dres, 2, nA, 2 * nB, maxlevel,
+		                     oh->sumMode == 0 ? 1 : 0, b->dTau);
+		return OZBLAS_STATUS_SUCCESS;
+	}
 	double *vals = (double *)walloc(oh, 8ull * nA * nB);
 	if (!vals) return OZBLAS_STATUS_WORKSPACE;
 	OZ_HIP(hipMemsetAsync(vals, 0, 8ull * nA * nB, b->stream));
@@ -488,7 +498,10 @@ static int32_t dot_core(ozblasHandle_t *oh, int n, const VecSplit &xs, const Vec
 static int32_t dot_impl(ozblasHandle_t *oh, int n, const double *x, const double *y, double *ret) {
 	Backend *b = be(oh);
 	if (oh->sumMode != 0 && oh->sumMode != 1) return OZBLAS_STATUS_NOT_SUPPORTED;
-	if (oh->splitEpsMode == 2) return OZBLAS_STATUS_NOT_SUPPORTED;
+	if (n <= 0) {  // empty vectors: the sum is +0 (fma(1, +0, 0 * ret) with ret = +0)
+		*ret = 0.0;
+		return OZBLAS_STATUS_SUCCESS;
+	}
 	const uint64_t base = wmark(oh);
 	oh->splitShift = 1;  // counterInit (src/cpu/ozblas.cpp:8)
 	VecSplit xs, ys;
@@ -535,7 +548,7 @@ static int32_t csr_split(ozblasHandle_t *oh, int m, int nnz, const double *val, 
 	}
 	OZ_HIP(hipMemsetAsync(b->dCounters, 0, sizeof(int), b->stream));
 	launch_split_csr(b->stream, m, val, rowptr, work, cs.slices, cs.lds, oh->splitShift, cap, b->dTau,
-	                 zrow, b->dCounters);
+	                 zrow, b->dCounters, oh->splitEpsMode);
 	OZ_HIP(hipMemcpyAsync(b->hPinned, b->dCounters, sizeof(int), hipMemcpyDeviceToHost, b->stream));
 	OZ_TRY(sync(oh));
 	const int zmax = ((int32_t *)b->hPinned)[0];
@@ -562,6 +575,15 @@ static int32_t csrmv_core(ozblasHandle_t *oh, int m, int n, int nnz, double alph
 	maxlevel = imax(maxlevel - oh->fastMode, imin(nA - 1, nB - 1));
 	const bool fsum = oh->sumMode == 0;
 	if (fsum && oh->fastMode) return OZBLAS_STATUS_NOT_SUPPORTED;  // reference reads unwritten slots
+	if (oh->splitEpsMode == 2) {  // Dot2 slice products (:139-142, 159-162)
+		// with fastMode the reference's 2 numB-column blocks no longer match its summation index
+		if (oh->fastMode || nA > 4 || nB > 4) return OZBLAS_STATUS_NOT_SUPPORTED;
+		launch_csrmm_sum_x2(b->stream, m, as.slices, as.lds, nA, colind, rowptr, xs.slices, xs.lds, nB,
+		                    fsum ? 1 : 0, alpha, beta, y, b->dTau);
+		(void)n;
+		(void)maxlevel;
+		return OZBLAS_STATUS_SUCCESS;
+	}
 	if (nA <= 4 && nB <= 8) {
 		launch_csrmm_sum(b->stream, m, nnz, as.slices, as.lds, nA, colind, rowptr, xs.slices, xs.lds, nB,
 		                 maxlevel, fsum ? 1 : 0, alpha, beta, y, b->dTau);
@@ -597,6 +619,12 @@ extern "C" int32_t ozblasDgemm(ozblasHandle_t *oh, char transA, char transB, int
 	if (m == 0 || n == 0) return oh->lastStatus = OZBLAS_STATUS_SUCCESS;
 	oh->splitShift = 1;  // counterInit
 	if (oh->reproMode == 0 && oh->nSplitMax == 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	if (k == 0) {  // no slices to multiply: the reference's sum is +0
+		launch_gemm_k0(be(oh)->stream, m, n, alpha, beta, C, ldc);
+		OZ_HIP(hipGetLastError());
+		OZ_TRY(sync(oh));
+		return oh->lastStatus = OZBLAS_STATUS_SUCCESS;
+	}
 	if (oh->sliceType != OZBLAS_SLICE_FP64) {
 		if (oh->sumMode == 1 && oh->nSplitMax == 0) {
 			bool handled = false;

@@ -168,11 +168,12 @@ __global__ __launch_bounds__(256) void k_write_grouped(const GWrite *g, const in
 // the counter until the list is exhausted.  Items are sorted by decreasing cost.
 template <int TM, int TN>
 __global__ __launch_bounds__(256, 1) void k_gemm_grouped(const GProb *P, const GItem *items,
-                                                         int nitems, int *counter) {
+                                                         int nitems, int *counter, const int *flag) {
 	constexpr int TBM = 64 * TM, TBN = 64 * TN;
 	__shared__ __attribute__((aligned(16))) char stage0[(TBM + TBN) * I8_BK];
 	__shared__ __attribute__((aligned(16))) char stage1[(TBM + TBN) * I8_BK];
 	__shared__ int next;
+	if (*flag) return;  // the plan check failed: the call will be planned again
 	for (;;) {
 		if (threadIdx.x == 0) next = atomicAdd(counter, 1);
 		__syncthreads();  // also: the previous item's LDS traffic is complete
@@ -204,7 +205,9 @@ __global__ __launch_bounds__(256, 1) void k_gemm_grouped(const GProb *P, const G
 }
 
 template <int NW>
-__global__ __launch_bounds__(256) void k_recon_grouped(const GRec *g, const int *start, int n) {
+__global__ __launch_bounds__(256) void k_recon_grouped(const GRec *g, const int *start, int n,
+                                                       const int *flag) {
+	if (*flag) return;  // C stays untouched when the plan check failed
 	const int o = owner(start, n, blockIdx.x);
 	const GRec r = g[o];
 	const int64_t e = (int64_t)(blockIdx.x - start[o]) * 256 + threadIdx.x;
@@ -315,24 +318,24 @@ static int32_t launch_execute(BatchPlan *p, hipStream_t st) {
 	if (p->nItems[c] > 0) {
 		if (c == 0)
 			hipLaunchKernelGGL((k_gemm_grouped<2, 2>), dim3(p->grid[0]), dim3(256), 0, st, p->dProb[0],
-			                   p->dItems[0], p->nItems[0], p->dCounter);
+			                   p->dItems[0], p->nItems[0], p->dCounter, p->dFlag);
 		else
 			hipLaunchKernelGGL((k_gemm_grouped<4, 4>), dim3(p->grid[1]), dim3(256), 0, st, p->dProb[1],
-			                   p->dItems[1], p->nItems[1], p->dCounter);
+			                   p->dItems[1], p->nItems[1], p->dCounter, p->dFlag);
 	}
 	if (p->nRecBlocks > 0) {
 		if (p->NW <= 3)
 			hipLaunchKernelGGL(k_recon_grouped<3>, dim3(p->nRecBlocks), dim3(256), 0, st, p->dRec,
-			                   p->dRecStart, p->nRec);
+			                   p->dRecStart, p->nRec, p->dFlag);
 		else if (p->NW <= 4)
 			hipLaunchKernelGGL(k_recon_grouped<4>, dim3(p->nRecBlocks), dim3(256), 0, st, p->dRec,
-			                   p->dRecStart, p->nRec);
+			                   p->dRecStart, p->nRec, p->dFlag);
 		else if (p->NW <= 6)
 			hipLaunchKernelGGL(k_recon_grouped<6>, dim3(p->nRecBlocks), dim3(256), 0, st, p->dRec,
-			                   p->dRecStart, p->nRec);
+			                   p->dRecStart, p->nRec, p->dFlag);
 		else
 			hipLaunchKernelGGL(k_recon_grouped<9>, dim3(p->nRecBlocks), dim3(256), 0, st, p->dRec,
-			                   p->dRecStart, p->nRec);
+			                   p->dRecStart, p->nRec, p->dFlag);
 	}
 	OZ_HIP(hipGetLastError());
 	return OZBLAS_STATUS_SUCCESS;
@@ -709,10 +712,10 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 			OZ_HIP(hipEventRecord(b->ev[6], st));
 			if (c == 0)
 				hipLaunchKernelGGL((k_gemm_grouped<2, 2>), dim3(p->grid[0]), dim3(256), 0, st, p->dProb[0],
-				                   p->dItems[0], p->nItems[0], p->dCounter);
+				                   p->dItems[0], p->nItems[0], p->dCounter, p->dFlag);
 			else
 				hipLaunchKernelGGL((k_gemm_grouped<4, 4>), dim3(p->grid[1]), dim3(256), 0, st, p->dProb[1],
-				                   p->dItems[1], p->nItems[1], p->dCounter);
+				                   p->dItems[1], p->nItems[1], p->dCounter, p->dFlag);
 			OZ_HIP(hipEventRecord(b->ev[7], st));
 			OZ_HIP(hipEventSynchronize(b->ev[7]));
 			OZ_HIP(hipEventElapsedTime(&p->tuneMs[c], b->ev[6], b->ev[7]));

@@ -389,7 +389,7 @@ struct BigInt {
 };
 
 template <int NW>
-__device__ inline void big_add_shifted(BigInt<NW> &a, int64_t v, int sh) {
+__host__ __device__ inline void big_add_shifted(BigInt<NW> &a, int64_t v, int sh) {
 	// a += v * 2^sh (two's complement, NW 64-bit words)
 	const int wi = sh >> 6, b = sh & 63;
 	const uint64_t ext = (v < 0) ? ~0ULL : 0ULL;
@@ -417,7 +417,7 @@ __device__ inline void big_add_shifted(BigInt<NW> &a, int64_t v, int sh) {
 }
 
 template <int NW>
-__device__ inline uint64_t big_word(const BigInt<NW> &a, int q) {
+__host__ __device__ inline uint64_t big_word(const BigInt<NW> &a, int q) {
 	uint64_t r = 0;
 #pragma unroll
 	for (int i = 0; i < NW; i++)
@@ -427,7 +427,7 @@ __device__ inline uint64_t big_word(const BigInt<NW> &a, int q) {
 
 // RN-even of N * 2^e0 (N non-negative, NW words)
 template <int NW>
-__device__ inline double big_round(const BigInt<NW> &a, int e0) {
+__host__ __device__ inline double big_round(const BigInt<NW> &a, int e0) {
 	int top = -1;
 #pragma unroll
 	for (int q = 0; q < NW; q++)
@@ -459,19 +459,32 @@ __device__ inline double big_round(const BigInt<NW> &a, int e0) {
 }
 
 // Exact value of one output element: N = sum_b T_b 2^(8 (Lmax - L_b)) over the bins' int32 planes
-// (element at Te[b * tstride]), rounded once to nearest-even binary64 as N * 2^e0, then
-// fma(alpha, RN(N 2^e0), beta * c).  NW 64-bit words must hold 8 Lmax + 41 bits.
+// (element at Te[b * tstride], bins in ascending level order), rounded once to nearest-even
+// binary64 as N * 2^e0, then fma(alpha, RN(N 2^e0), beta * c).  NW 64-bit words must hold
+// 8 Lmax + 41 bits.  Levels are first folded four at a time into an int64 (|T| < 2^31, so up to
+// 128 terms of weight <= 2^24 stay below 2^63) and each fold is added to N once.
 template <int NW>
-__device__ inline double recon_elem(const int32_t *Te, int64_t tstride, const int *binLevel,
-                                   int nbins, int Lmax, int e0, double alpha, double beta,
-                                   double c) {
+__host__ __device__ inline double recon_elem(const int32_t *Te, int64_t tstride, const int *binLevel,
+                                            int nbins, int Lmax, int e0, double alpha, double beta,
+                                            double c) {
 	BigInt<NW> a;
 #pragma unroll
 	for (int q = 0; q < NW; q++) a.w[q] = 0;
+	int64_t fold = 0;
+	int group = -1, top = 0, terms = 0;
 	for (int b = 0; b < nbins; b++) {
-		const int32_t v = Te[(int64_t)b * tstride];
-		if (v != 0) big_add_shifted<NW>(a, (int64_t)v, 8 * (Lmax - binLevel[b]));
+		const int L = binLevel[b];
+		if (L / 4 != group || terms == 128) {
+			if (fold != 0) big_add_shifted<NW>(a, fold, 8 * (Lmax - top));
+			fold = 0;
+			terms = 0;
+			group = L / 4;
+			top = (4 * group + 3 < Lmax) ? 4 * group + 3 : Lmax;  // unit of the fold: 2^(8 (Lmax - top))
+		}
+		fold += (int64_t)Te[(int64_t)b * tstride] * ((int64_t)1 << (8 * (top - L)));
+		terms++;
 	}
+	if (fold != 0) big_add_shifted<NW>(a, fold, 8 * (Lmax - top));
 	const bool neg = (a.w[NW - 1] >> 63) != 0;
 	if (neg) {  // two's complement negate
 		uint64_t carry = 1;

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <mutex>
 
+#include "oz_digits.hpp"
 #include "oz_internal.hpp"
 
 namespace oz {
@@ -156,7 +157,18 @@ extern "C" int32_t ozblas_internal_rho(int32_t dim, int32_t splitEpsMode) {
 	return rho_host(dim, splitEpsMode);
 }
 extern "C" int32_t ozblas_internal_rho_row(int32_t dim) { return rho_of_dim(dim); }
+extern "C" int32_t ozblas_internal_rho_row_mode(int32_t dim, int32_t mode) { return rho_dev(dim, mode); }
 extern "C" void ozblas_internal_tau_batch(int64_t n, const double *x, int32_t *out) {
 	const uint16_t *J = tau_table_host();
 	for (int64_t i = 0; i < n; i++) out[i] = tau_of(x[i], J);
+}
+
+// test hook: the exact reconstruction of one element (host build of recon_elem)
+extern "C" double ozblas_internal_recon(int32_t nbins, const int32_t *T, const int32_t *levels,
+                                        int32_t Lmax, int32_t e0, double alpha, double beta, double c) {
+	const int nw = (8 * Lmax + 32 + 8 + 1 + 63) / 64;
+	if (nw <= 3) return oz::recon_elem<3>(T, 1, levels, nbins, Lmax, e0, alpha, beta, c);
+	if (nw <= 4) return oz::recon_elem<4>(T, 1, levels, nbins, Lmax, e0, alpha, beta, c);
+	if (nw <= 6) return oz::recon_elem<6>(T, 1, levels, nbins, Lmax, e0, alpha, beta, c);
+	return oz::recon_elem<9>(T, 1, levels, nbins, Lmax, e0, alpha, beta, c);
 }

@@ -132,6 +132,18 @@ __host__ __device__ inline int rho_of_dim(int dim) {
 	return (p & 1) ? (55 + p) / 2 : (54 + p) / 2;
 }
 
+// rho of getRho for every splitEpsMode, from integers only (the per-row split of a CSR matrix runs
+// it on the device): mode 2 (Dot2) ceil(53 - (106 - log2 d) / 2) = ceil(log2(d) / 2)
+// = ceil(q / 2); mode 9 ceil(53 - (53 - log2(2 sqrt d)) / 2) = 27 + ceil(log2(d) / 4)
+// = 27 + ceil(q / 4); q = ceil(log2 d).  d = 0 gives INT_MIN as for mode 0.
+__host__ __device__ inline int rho_dev(int dim, int mode) {
+	if (mode != 2 && mode != 9) return rho_of_dim(dim);
+	if (dim <= 0) return INT32_MIN;
+	const int p = 31 - __builtin_clz((unsigned)dim);
+	const int q = ((dim & (dim - 1)) == 0) ? p : p + 1;
+	return (mode == 2) ? (q + 1) / 2 : 27 + (q + 3) / 4;
+}
+
 // host helpers (oz_handle.cpp)
 int rho_host(int dim, int splitEpsMode);
 
@@ -152,6 +164,7 @@ struct ExpStatsH {
 // overflow: the precondition for the fast forms' bit identity with the plain form.
 inline bool fast_valid(const ozblasHandle_t *oh, const ExpStatsH &a, const ExpStatsH &b, int k) {
 	if (oh->nSplitMax != 0 || oh->sumMode != 1 || oh->splitShift != 1) return false;
+	if (oh->splitEpsMode == 2) return false;  // Dot2 slices: the plain form's products are inexact
 	if (a.nonfinite || b.nonfinite) return false;
 	if (a.maxS == 0 || b.maxS == 0) return true;  // an all-zero operand: exact zeros either way
 	const int rho = rho_host(k, oh->splitEpsMode);

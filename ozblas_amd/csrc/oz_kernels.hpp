@@ -21,7 +21,7 @@ void launch_fill(hipStream_t st, double *p, int64_t n, double v);
 void launch_copy(hipStream_t st, const double *src, double *dst, int64_t n);
 void launch_split_csr(hipStream_t st, int m, const double *val, const int *rowptr, double *work,
                       double *split, int64_t lds, int shift, int cap, const uint16_t *J, int *zrow,
-                      int *zmax);
+                      int *zmax, int epsMode);
 void launch_zero_slots_csr(hipStream_t st, int m, const int *rowptr, double *split, int64_t lds,
                            const int *zrow, int count);
 void launch_digit_stats(hipStream_t st, int m, int n, const double *in, int64_t ldi, int *colE,
@@ -67,6 +67,13 @@ void launch_reconstruct(hipStream_t st, int M, int N, const int32_t *T, int64_t 
 // ---- vectors / sparse (oz_level12.hip)
 void launch_slice_dots(hipStream_t st, int n, const double *xs, int64_t ldx, int nA,
                        const double *ys, int64_t ldy, int nB, double *out);
+void launch_gemm_k0(hipStream_t st, int m, int n, double alpha, double beta, double *C, int64_t ldc);
+size_t dot2_partials_bytes(int nA, int nB);
+void launch_dot2_slices(hipStream_t st, int n, const double *xs, int64_t ldx, int nA,
+                        const double *ys, int64_t ldy, int nB, void *part, double *vals);
+void launch_csrmm_sum_x2(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
+                         const int *colind, const int *rowptr, const double *Xs, int64_t ldxs, int nB,
+                         int fsum, double alpha, double beta, double *y, const uint16_t *J);
 void launch_nearsum_small(hipStream_t st, int nvals, double *vals, double alpha, double beta,
                           double *C, int sumOrderFsum, int nA, int nB, int maxlevel, int fsum,
                           const uint16_t *J);

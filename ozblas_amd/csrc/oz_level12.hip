@@ -183,6 +183,106 @@ void launch_slice_dots(hipStream_t st, int n, const double *xs, int64_t ldx, int
 	                   nB, out);
 }
 
+// GEMM with k = 0: every slice product is zero, so the reference's sum is +0 and the epilogue
+// leaves C = fma(alpha, +0, beta * C)
+__global__ void k_gemm_k0(int m, int n, double alpha, double beta, double *C, int64_t ldc) {
+	const int64_t total = (int64_t)m * n;
+	for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+	     e += (int64_t)gridDim.x * blockDim.x) {
+		double *c = C + (e / m) * ldc + e % m;
+		*c = fma(alpha, 0.0, beta * *c);
+	}
+}
+
+void launch_gemm_k0(hipStream_t st, int m, int n, double alpha, double beta, double *C, int64_t ldc) {
+	int64_t g = ((int64_t)m * n + 255) / 256;
+	if (g > 4096) g = 4096;
+	if (g < 1) g = 1;
+	hipLaunchKernelGGL(k_gemm_k0, dim3((unsigned)g), dim3(256), 0, st, m, n, alpha, beta, C, ldc);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Dot2 mode (splitEpsMode = 2): error-free transformations of src/cpu/eft.h (scalar forms) and the
+// double-double slice dots.  Compiled without contraction, so every expression rounds as written.
+// ---------------------------------------------------------------------------------------------
+struct dd {
+	double x, y;
+};
+__device__ inline void two_sum(double a, double b, double &s, double &e) {  // TwoSum
+	s = a + b;
+	const double v = s - a;
+	e = (a - (s - v)) + (b - v);
+}
+__device__ inline void quick_two_sum(double a, double b, double &s, double &e) {  // QuickTwoSum
+	s = a + b;
+	e = b - (s - a);
+}
+__device__ inline void dot2i(double a, double b, dd &c) {  // Dot2i: TwoProdFMA + TwoSum
+	const double h = a * b;
+	const double r = fma(a, b, -h);
+	double q;
+	two_sum(c.x, h, c.x, q);
+	c.y = c.y + (q + r);
+}
+__device__ inline dd quad_add(dd a, dd b) {  // QuadAdd
+	dd t, c;
+	two_sum(a.x, b.x, t.x, t.y);
+	t.y = t.y + (a.y + b.y);
+	quick_two_sum(t.x, t.y, c.x, c.y);
+	return c;
+}
+__device__ inline dd quad_mul(double a, dd b) {  // QuadMul(double, double2)
+	dd t, c;
+	t.x = a * b.x;
+	t.y = fma(a, b.x, -t.x);
+	t.y = fma(a, b.y, t.y);
+	quick_two_sum(t.x, t.y, c.x, c.y);
+	return c;
+}
+
+// Dot2 DOT (dgemm_tn_skinny_x2, src/cpu/ozblas_blas_wrapper.cpp:109-152): the reference sums each
+// slice dot in OpenMP per-thread partials over a static schedule of k, combined by QuadAdd in a
+// critical section.  This is that computation for kDot2Chunks threads entering the section in
+// thread order: partial t covers libgomp's static chunk t of [0, n).
+constexpr int kDot2Chunks = 1024;
+
+__global__ __launch_bounds__(256) void k_dot2_partials(int n, const double *xs, int64_t ldx, int nA,
+                                                       const double *ys, int64_t ldy, dd *part) {
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;  // chunk
+	const int pr = blockIdx.y;                            // pair: ia + nA * ib
+	if (t >= kDot2Chunks) return;
+	const int ia = pr % nA, ib = pr / nA;
+	const int64_t q = n / kDot2Chunks, r = n % kDot2Chunks;
+	const int64_t lo = t * q + (t < r ? t : r), hi = lo + q + (t < r ? 1 : 0);
+	const double *x = xs + (int64_t)ia * ldx, *y = ys + (int64_t)ib * ldy;
+	dd L{0.0, 0.0};
+	for (int64_t k = lo; k < hi; k++) dot2i(x[k], y[k], L);
+	part[(int64_t)pr * kDot2Chunks + t] = L;
+}
+
+// one thread per pair: QuadAdd of the partials in chunk order, QuadMul(1, .), then hi to
+// vals[nA * ib + ia] and lo to vals[nA * (nB + ib) + ia] (C and C + n*ldc of the reference)
+__global__ void k_dot2_combine(int nA, int nB, const dd *part, double *vals) {
+	const int pr = blockIdx.x * blockDim.x + threadIdx.x;
+	if (pr >= nA * nB) return;
+	const int ia = pr % nA, ib = pr / nA;
+	dd G{0.0, 0.0};
+	for (int t = 0; t < kDot2Chunks; t++) G = quad_add(G, part[(int64_t)pr * kDot2Chunks + t]);
+	G = quad_mul(1.0, G);
+	vals[nA * ib + ia] = G.x;
+	vals[nA * (nB + ib) + ia] = G.y;
+}
+
+size_t dot2_partials_bytes(int nA, int nB) { return sizeof(dd) * (size_t)nA * nB * kDot2Chunks; }
+
+void launch_dot2_slices(hipStream_t st, int n, const double *xs, int64_t ldx, int nA,
+                        const double *ys, int64_t ldy, int nB, void *part, double *vals) {
+	hipLaunchKernelGGL(k_dot2_partials, dim3((kDot2Chunks + 255) / 256, nA * nB), dim3(256), 0, st, n,
+	                   xs, ldx, nA, ys, ldy, (dd *)part);
+	hipLaunchKernelGGL(k_dot2_combine, dim3((nA * nB + 63) / 64), dim3(64), 0, st, nA, nB,
+	                   (const dd *)part, vals);
+}
+
 // DOT / NRM2 summation of the nA x nB slice dots (sumOrder 2, src/cpu/ozblas_gemm.cpp:281-288),
 // one thread; C[0] = fma(alpha, t, beta * C[0])
 __global__ void k_dot_sum(int nvals, double *vals, double alpha, double beta, double *C, int nA,
@@ -288,6 +388,91 @@ void launch_csrmm_sum(hipStream_t st, int m, int nnz, const double *As, int64_t 
 	else
 		hipLaunchKernelGGL((k_csrmm_sum<4, 8>), g, b, 0, st, m, As, ldas, nA, colind, rowptr, Xs,
 		                   ldxs, nB, maxlevel, fsum, alpha, beta, y, J);
+}
+
+// Dot2 CSRMV (csrmm_n_x2, src/cpu/ozblas_blas_wrapper.cpp:262-295, then the global sum over
+// 2 nB "slices" of src/cpu/ozblas_csrmv.cpp:159-162): per row and slice pair, Dot2 over the row's
+// nonzeros in order and QuadMul(1, .); w[ia][ib'] = hi of pair (ia, ib') for ib' < nB and lo of
+// pair (ia, ib' - nB) otherwise; NearSum over all w, or Fsum in (level ia + ib', ia) order.
+template <int NAM, int NBM>
+__global__ __launch_bounds__(256) void k_csrmm_sum_x2(int m, const double *As, int64_t ldas, int nA,
+                                                      const int *colind, const int *rowptr,
+                                                      const double *Xs, int64_t ldxs, int nB,
+                                                      int fsum, double alpha, double beta, double *y,
+                                                      const uint16_t *J) {
+	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= m) return;
+	const int b = rowptr[r], e = rowptr[r + 1];
+	dd v[NAM][NBM];
+#pragma unroll
+	for (int ia = 0; ia < NAM; ia++)
+#pragma unroll
+		for (int ib = 0; ib < NBM; ib++) v[ia][ib] = dd{0.0, 0.0};
+	for (int jj = b; jj < e; jj++) {
+		const int c = colind[jj];
+		double a[NAM], x[NBM];
+#pragma unroll
+		for (int ia = 0; ia < NAM; ia++) a[ia] = (ia < nA) ? As[(int64_t)ia * ldas + jj] : 0.0;
+#pragma unroll
+		for (int ib = 0; ib < NBM; ib++) x[ib] = (ib < nB) ? Xs[(int64_t)ib * ldxs + c] : 0.0;
+#pragma unroll
+		for (int ia = 0; ia < NAM; ia++)
+#pragma unroll
+			for (int ib = 0; ib < NBM; ib++) dot2i(a[ia], x[ib], v[ia][ib]);
+	}
+	double fx[NAM][NBM], fy[NAM][NBM];  // renormalised (hi, lo) of every pair: QuadMul(1, .)
+#pragma unroll
+	for (int ia = 0; ia < NAM; ia++)
+#pragma unroll
+		for (int ib = 0; ib < NBM; ib++) {
+			const dd f = quad_mul(1.0, v[ia][ib]);
+			fx[ia][ib] = f.x;
+			fy[ia][ib] = f.y;
+		}
+	double t;
+	if (fsum) {
+		t = 0.;
+		for (int ik = 0; ik <= (nA - 1) + (2 * nB - 1); ik++)
+			for (int ia = 0; ia < nA; ia++) {
+				const int ibp = ik - ia;
+				if (ibp < 0 || ibp >= 2 * nB) continue;
+				const bool hiPart = ibp < nB;
+				const int ib = hiPart ? ibp : ibp - nB;
+				double w = 0.0;
+#pragma unroll
+				for (int qa = 0; qa < NAM; qa++)
+#pragma unroll
+					for (int qb = 0; qb < NBM; qb++)
+						if (qa == ia && qb == ib) w = hiPart ? fx[qa][qb] : fy[qa][qb];
+				t = t + w;
+			}
+	} else {
+		RegVec<NAM * 2 * NBM> u;  // the 2 nA nB values, compacted to the front
+#pragma unroll
+		for (int q = 0; q < NAM * 2 * NBM; q++) u[q] = 0.0;
+		int cnt = 0;
+#pragma unroll
+		for (int ia = 0; ia < NAM; ia++)
+#pragma unroll
+			for (int ib = 0; ib < NBM; ib++) {
+				const bool ok = ia < nA && ib < nB;
+#pragma unroll
+				for (int q = 0; q < NAM * 2 * NBM; q++) {
+					if (ok && q == cnt) u[q] = fx[ia][ib];
+					if (ok && q == cnt + 1) u[q] = fy[ia][ib];
+				}
+				cnt += ok ? 2 : 0;
+			}
+		t = nearsum(cnt, u, J);
+	}
+	y[r] = fma(alpha, t, beta * y[r]);
+}
+
+void launch_csrmm_sum_x2(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
+                         const int *colind, const int *rowptr, const double *Xs, int64_t ldxs, int nB,
+                         int fsum, double alpha, double beta, double *y, const uint16_t *J) {
+	hipLaunchKernelGGL((k_csrmm_sum_x2<4, 4>), dim3((m + 255) / 256), dim3(256), 0, st, m, As, ldas,
+	                   nA, colind, rowptr, Xs, ldxs, nB, fsum, alpha, beta, y, J);
 }
 
 // general case: store the products where blasRcsrmm puts them (column ic of CSplit, ld ldcs)

@@ -145,11 +145,11 @@ __global__ void k_copy(const double *src, double *dst, int64_t n) {
 // ---------------------------------------------------------------------------------------------
 __global__ void k_split_csr(int m, const double *val, const int *rowptr, double *work,
                             double *split, int64_t lds, int shift, int cap, const uint16_t *J,
-                            int *zrow, int *zmax) {
+                            int *zrow, int *zmax, int epsMode) {
 	const int r = blockIdx.x * blockDim.x + threadIdx.x;
 	if (r >= m) return;
 	const int b = rowptr[r], e = rowptr[r + 1];
-	const int rho = rho_of_dim(e - b);
+	const int rho = rho_dev(e - b, epsMode);
 	double mx = 0.0;
 	for (int i = b; i < e; i++) mx = std_max(mx, fabs(val[i]));
 	int s = 0;
@@ -296,9 +296,9 @@ void launch_copy(hipStream_t st, const double *src, double *dst, int64_t n) {
 
 void launch_split_csr(hipStream_t st, int m, const double *val, const int *rowptr, double *work,
                       double *split, int64_t lds, int shift, int cap, const uint16_t *J, int *zrow,
-                      int *zmax) {
+                      int *zmax, int epsMode) {
 	hipLaunchKernelGGL(k_split_csr, dim3((m + 255) / 256), dim3(256), 0, st, m, val, rowptr, work,
-	                   split, lds, shift, cap, J, zrow, zmax);
+	                   split, lds, shift, cap, J, zrow, zmax, epsMode);
 }
 
 void launch_zero_slots_csr(hipStream_t st, int m, const int *rowptr, double *split, int64_t lds,

@@ -76,9 +76,61 @@ def test_rho_rules_match_libm(lib, oracle):
         assert lib.ozblas_internal_rho_row(d) == want, d
 
 
+def test_rho_modes_match_libm(lib, oracle):
+    """getRho for splitEpsMode 2 (Dot2) and 9: the host form (glibc log2/sqrt) and the integer
+    form the device uses for CSR rows agree with the oracle's libm evaluation."""
+    lib.ozblas_internal_rho.restype = ctypes.c_int32
+    lib.ozblas_internal_rho_row_mode.restype = ctypes.c_int32
+    dims = list(range(0, 5000)) + [2 ** p + o for p in range(12, 31) for o in (-1, 0, 1)]
+    dims += [10 ** 6, 123456789, 2 ** 31 - 1]
+    for mode in (0, 2, 9):
+        for d in dims:
+            want = oracle.rho(d, mode)
+            assert lib.ozblas_internal_rho(d, mode) == want, (mode, d)
+            assert lib.ozblas_internal_rho_row_mode(d, mode) == want, (mode, d)
+
+
 def test_gen_phi_matches_oracle_generator(oracle):
     import ozblas_amd
     for phi, seed in ((1.0, 1), (0.5, 2), (4.0, 7)):
         a = ozblas_amd.gen_phi(33, 9, 40, phi, seed)
         b = oracle.init_mat(33, 9, 40, phi, 2, seed)
         assert a.tobytes() == b.tobytes()
+
+
+def test_exact_reconstruction_matches_big_integers(lib):
+    """recon_elem (the exact reconstruction of the fast forms, host build through a hook):
+    fma(alpha, RN(N 2^e0), beta c) with N = sum_b T_b 256^(Lmax - L_b), against Python integers and
+    the correctly rounded float(Fraction) -- random level structures (repeated levels, more than
+    128 bins of one level), extreme int32 values, subnormal and huge results."""
+    from fractions import Fraction as F
+    import math
+    f = lib.ozblas_internal_recon
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                  ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    rng = np.random.default_rng(12)
+    for trial in range(3000):
+        Lmax = int(rng.integers(0, 66))
+        levels = []
+        for L in range(Lmax + 1):
+            reps = int(rng.integers(0, 3)) if trial % 50 else int(rng.integers(0, 300))
+            levels += [L] * reps
+        if not levels:
+            levels = [0]
+        T = rng.integers(-2 ** 31, 2 ** 31, len(levels)).astype(np.int64)
+        if trial % 7 == 0:
+            T[:] = rng.choice([-2 ** 31, 2 ** 31 - 1, 0, 1, -1], len(levels))
+        T = T.astype(np.int32)
+        lv = np.array(levels, dtype=np.int32)
+        N = sum(int(t) * 256 ** (Lmax - int(L)) for t, L in zip(T, lv))
+        bits = max(1, abs(N).bit_length())
+        e0 = int(rng.integers(-1074 - bits - 5, 1000 - bits))
+        alpha, beta, c = ((1.0, 0.0, 0.0) if trial % 3 else
+                          (float(rng.standard_normal()), float(rng.standard_normal()), float(rng.standard_normal())))
+        got = f(len(lv), T.ctypes.data, lv.ctypes.data, Lmax, e0, alpha, beta, c)
+        t = float(F(N) * F(2) ** e0)
+        want = float(F(alpha) * F(t) + F(beta * c))
+        if want == 0.0:
+            want = alpha * t + beta * c  # signed zero as fma gives it
+        assert got == want and math.copysign(1, got) == math.copysign(1, want), (trial, Lmax, e0, got, want)

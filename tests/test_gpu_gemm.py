@@ -168,3 +168,17 @@ def test_gemv_bitwise(oz, oracle, trans, sumMode):
         dy = dev(y0)
         H.dgemv(trans, m, n, 1.0, dev(A), m, dev(x), 0.0, dy)
         assert bits_equal(host(dy), yw), first_diff(host(dy), yw)
+
+
+def test_plain_gemm_k0_and_empty(oz, oracle):
+    """k = 0: no slices, C = fma(alpha, +0, beta * C) as the reference computes it; m = 0 / n = 0:
+    C untouched."""
+    for sumMode in (0, 1):
+        for (alpha, beta) in ((1.0, 0.0), (0.5, -2.0)):
+            got, want, _, _ = run_pair(oz, oracle, "N", "N", 9, 11, 0, alpha=alpha, beta=beta,
+                                       cinit=lambda n: np.linspace(-3, 3, n), sumMode=sumMode)
+            assert bits_equal(got, want), first_diff(got, want)
+    H = oz.Handle(1 << 24, sumMode=1)
+    C = dev(np.full(12, 7.0))
+    H.dgemm("N", "N", 0, 3, 4, 1.0, dev(np.ones(12)), 1, dev(np.ones(12)), 4, 0.0, C, 1)
+    assert bits_equal(host(C), np.full(12, 7.0))

@@ -99,3 +99,74 @@ def test_cg_residual_history_bitwise(oz, oracle, sumMode):
     assert len(got) == iters + 1
     assert bits_equal(got, want), first_diff(got, want)
     assert bits_equal(host(dx), xw), first_diff(host(dx), xw)
+
+
+# ---- Dot2 mode (splitEpsMode = 2): wide slices, double-double slice products ------------------
+@pytest.mark.parametrize("n", [1, 7, 1000, 65536, 100003])
+@pytest.mark.parametrize("sumMode", [0, 1])
+def test_dot2_dot_nrm2_bitwise(oz, oracle, n, sumMode):
+    """The DOT slice dots are the reference's OpenMP Dot2 partial sums for 1024 threads combined
+    in thread order (oracle dot2Chunks = 1024, the GPU's fixed chunking)."""
+    say(f"ddot Dot2 n={n} sumMode={sumMode}")
+    x = oracle.init_mat(n, 1, n, 2.0, 2, 31)
+    y = oracle.init_mat(n, 1, n, 2.0, 2, 32)
+    h = oracle.Handle(1 << 30, sumMode=sumMode, splitEpsMode=2)
+    assert h.dot2Chunks == 1024
+    H = oz.Handle(1 << 28, sumMode=sumMode, splitEpsMode=2)
+    dx, dy = dev(x), dev(y)
+    got = H.ddot(n, dx, dy)
+    want = h.dot(n, x, y)
+    assert np.float64(got).tobytes() == np.float64(want).tobytes(), (got.hex(), want.hex())
+    assert (H.nSplitA, H.nSplitB) == (h.nSplitA, h.nSplitB)
+    assert H.dnrm2(n, dx) == oracle.Handle(1 << 30, sumMode=sumMode, splitEpsMode=2).nrm2(n, x)
+
+
+@pytest.mark.parametrize("sumMode", [0, 1])
+@pytest.mark.parametrize("presplit", [False, True])
+def test_dot2_csrmv_bitwise(oz, oracle, sumMode, presplit):
+    rowptr, colind, val, n, nnz = _laplace(oracle, 12)
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, n) * np.exp2(rng.integers(-20, 20, n))
+    y0 = rng.standard_normal(n)
+    say(f"dcsrmv Dot2 n={n} sumMode={sumMode} presplit={presplit}")
+    h = oracle.Handle(1 << 30, sumMode=sumMode, splitEpsMode=2)
+    H = oz.Handle(1 << 28, sumMode=sumMode, splitEpsMode=2)
+    dval, dcol, drow, dx = dev(val), dev(colind), dev(rowptr), dev(x)
+    yw = y0.copy()
+    dy = dev(y0)
+    if presplit:
+        p = h.csrmv_splitA(n, n, nnz, val, rowptr)
+        P = H.dcsrmv_splitA(n, n, nnz, dval, drow)
+        assert (H.splitShift, H.nSplitA_) == (h.splitShift, h.nSplitA_)
+        h.csrmv(n, n, nnz, 0.5, p, colind, rowptr, x, -1.5, yw)
+        H.dcsrmv(n, n, nnz, 0.5, P, dcol, drow, dx, -1.5, dy)
+    else:
+        h.csrmv(n, n, nnz, 0.5, val, colind, rowptr, x, -1.5, yw)
+        H.dcsrmv(n, n, nnz, 0.5, dval, dcol, drow, dx, -1.5, dy)
+    assert bits_equal(host(dy), yw), first_diff(host(dy), yw)
+
+
+@pytest.mark.parametrize("sumMode", [0, 1])
+def test_dot2_cg_residual_history_bitwise(oz, oracle, sumMode):
+    g, iters = 10, 40
+    rowptr, colind, val, n, nnz = _laplace(oracle, g)
+    oracle.srand48(3)
+    b = np.array([2.0 * oracle.drand48() - 1.0 for _ in range(n)])
+    say(f"dcg Dot2 n={n} iters={iters} sumMode={sumMode}")
+    h = oracle.Handle(1 << 30, sumMode=sumMode, splitEpsMode=2)
+    xw = np.zeros(n)
+    want = h.cg(n, nnz, val, colind, rowptr, b, xw, iters, 0.0)
+    H = oz.Handle(1 << 28, sumMode=sumMode, splitEpsMode=2)
+    dx = dev(np.zeros(n))
+    got = H.dcg(n, nnz, dev(val), dev(colind), dev(rowptr), dev(b), dx, iters, 0.0)
+    assert bits_equal(got, want), first_diff(got, want)
+    assert bits_equal(host(dx), xw), first_diff(host(dx), xw)
+
+
+def test_dot2_gemm_not_supported(oz):
+    """GEMM / GEMV with Dot2 slices would need inexact slice products (the reference's result then
+    depends on the BLAS's summation order): refused with NOT_SUPPORTED."""
+    H = oz.Handle(1 << 24, sumMode=1, splitEpsMode=2)
+    A, B, C = dev(np.ones(16)), dev(np.ones(16)), dev(np.zeros(16))
+    with pytest.raises(oz.OzblasError):
+        H.dgemm("N", "N", 4, 4, 4, 1.0, A, 4, B, 4, 0.0, C, 4)
