@@ -117,7 +117,10 @@ typedef struct ozblasHandle {
 	int32_t batchTile;    /* ozblasDgemmBatched: output tile edge picked by the autotuner (128, 256) */
 	int32_t batchReplay;  /* ozblasDgemmBatched: 1 when the call replayed the cached hipGraph */
 	int32_t batchGrouped; /* ozblasDgemmBatched: problems run in the grouped launch */
-	int32_t reserved[2];
+	int32_t digitLevels;  /* INT8 form: digit levels computed before the certified rounding check;
+	                         0 = automatic (default), -1 = all levels, n > 0 = the first n levels */
+	int32_t fallbackTiles; /* INT8 form: output tiles of the last GEMM that needed the levels after
+	                          the check */
 } ozblasHandle_t;
 
 /* lifecycle: cuozblasCreate/cuozblasDestroy (include/cuozblas.h:67-68, src/cuda/cuozblas.cpp:23-75).

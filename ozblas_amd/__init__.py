@@ -49,7 +49,7 @@ class OzblasHandleT(C.Structure):
         ("t_DOT_NRM2_total", C.c_double), ("t_SpMV_SpMM_total", C.c_double),
         ("sliceType", C.c_int32), ("lastStatus", C.c_int32), ("nLevelBins", C.c_int32),
         ("batchTile", C.c_int32), ("batchReplay", C.c_int32), ("batchGrouped", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("digitLevels", C.c_int32), ("fallbackTiles", C.c_int32),
     ]
 
 

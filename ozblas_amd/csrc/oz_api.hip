@@ -15,7 +15,7 @@
 #include <cstring>
 #include <vector>
 
-#include "oz_kernels.hpp"
+#include "oz_digits.hpp"
 
 namespace oz {
 
@@ -396,11 +396,35 @@ static int32_t gemm_fast(ozblasHandle_t *oh, char transA, char transB, int m, in
 	}
 	const int nbins = (int)bins.size();
 	oh->nLevelBins = nbins;
-	// launch order: bins with the most pairs first, so the short ones fill the tail
+	// Certified early termination (INT8): pass 1 computes the levels <= L1; the reconstruction
+	// bounds the rest by R = sum_{L > L1} npairs_L * k * 2^14 * 2^(8 (Lmax - L)) (every digit
+	// product is at most 2^14 in magnitude) and keeps the elements whose rounding of N1 +- R agrees;
+	// pass 2 computes the remaining levels only for the tiles holding the other elements.
+	int L1 = Lmax;
+	if (elem == 1 && oh->digitLevels > 0) L1 = imin(Lmax, oh->digitLevels - 1);
+	if (elem == 1 && oh->digitLevels == 0) {
+		// automatic: the first level with R / 2^(8 Lmax) <= 2^-62, i.e. 10 bits below the binary64
+		// precision of an element of size 2^(8 Lmax) units (the product of the two scales)
+		std::vector<double> np(Lmax + 1, 0.0);
+		for (const LevelBin &bn : bins) np[bn.level] += bn.npairs * (double)(bn.k1 - bn.k0) / Kpad;
+		for (L1 = 0; L1 < Lmax; L1++) {
+			double r = 0.0;
+			for (int Lv = L1 + 1; Lv <= Lmax; Lv++) r += np[Lv] * (double)k * 16384.0 * std::ldexp(1.0, -8 * Lv);
+			if (r <= std::ldexp(1.0, -62)) break;
+		}
+		int64_t p1 = 0;
+		for (const LevelBin &bn : bins)
+			if (bn.level <= L1) p1 += bn.npairs;
+		if (L1 >= Lmax - 1 || p1 > 0.9 * (double)pairs.size()) L1 = Lmax;  // too little to skip
+	}
+	int nb1 = 0;  // bins are in level order: pass 1 is a prefix
+	while (nb1 < nbins && bins[nb1].level <= L1) nb1++;
+	// launch order within each pass: bins with the most pairs first, so the short ones fill the tail
 	std::vector<int> order(nbins);
 	for (int q = 0; q < nbins; q++) order[q] = q;
-	std::stable_sort(order.begin(), order.end(),
-	                 [&](int x, int y) { return bins[x].npairs > bins[y].npairs; });
+	auto byPairs = [&](int x, int y) { return bins[x].npairs > bins[y].npairs; };
+	std::stable_sort(order.begin(), order.begin() + nb1, byPairs);
+	std::stable_sort(order.begin() + nb1, order.end(), byPairs);
 	int2 *dPairs = (int2 *)walloc(oh, sizeof(int2) * pairs.size());
 	LevelBin *dBins = (LevelBin *)walloc(oh, sizeof(LevelBin) * bins.size());
 	int *dLevel = (int *)walloc(oh, sizeof(int) * binLevel.size());
@@ -409,7 +433,11 @@ static int32_t gemm_fast(ozblasHandle_t *oh, char transA, char transB, int m, in
 	const int64_t ldt = (elem == 1) ? rup(m, 256) : rup(m, 64);
 	const int64_t tstride = ldt * ((elem == 1) ? rup(n, 256) : n);
 	int32_t *T = (int32_t *)walloc(oh, sizeof(int32_t) * (size_t)tstride * nbins);
-	if (!dPairs || !dBins || !dLevel || !dOrder || !T) {
+	const int tile = digit_gemm_tile(elem, m, n);
+	const int tilesM = (m + tile - 1) / tile, tilesN = (n + tile - 1) / tile;
+	uint64_t *dR = (uint64_t *)walloc(oh, 8 * 16);
+	int *dMark = (int *)walloc(oh, sizeof(int) * (size_t)tilesM * tilesN * 2);
+	if (!dPairs || !dBins || !dLevel || !dOrder || !T || !dR || !dMark) {
 		wreset(oh, base);
 		*handled = false;
 		return OZBLAS_STATUS_WORKSPACE;
@@ -418,26 +446,59 @@ static int32_t gemm_fast(ozblasHandle_t *oh, char transA, char transB, int m, in
 	OZ_HIP(hipMemcpyAsync(dBins, bins.data(), sizeof(LevelBin) * bins.size(), hipMemcpyHostToDevice, st));
 	OZ_HIP(hipMemcpyAsync(dLevel, binLevel.data(), sizeof(int) * binLevel.size(), hipMemcpyHostToDevice, st));
 	OZ_HIP(hipMemcpyAsync(dOrder, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, st));
+	int64_t pairs1 = 0;
+	for (int q = 0; q < nb1; q++) pairs1 += bins[q].npairs;
 	OZ_HIP(hipEventRecord(b->ev[3], st));
 	launch_digit_gemm(st, elem, m, n, Kpad, da.planes, da.ldp, da.pstride, db.planes, db.ldp,
-	                  db.pstride, dPairs, dBins, nbins, dOrder, T, ldt, tstride);
+	                  db.pstride, dPairs, dBins, nb1, dOrder, T, ldt, tstride);
 	OZ_HIP(hipEventRecord(b->ev[4], st));
-	launch_reconstruct(st, m, n, T, ldt, tstride, dLevel, nbins, Lmax, da.colE, db.colE, alpha,
-	                   beta, C, ldc);
+	int fallback = 0;
+	if (nb1 == nbins) {
+		launch_reconstruct(st, m, n, T, ldt, tstride, dLevel, nbins, Lmax, da.colE, db.colE, alpha,
+		                   beta, C, ldc);
+	} else {
+		BigInt<9> R;
+		big_zero<9>(R);
+		for (const LevelBin &bn : bins)
+			if (bn.level > L1)
+				big_add_shifted<9>(R, (int64_t)bn.npairs * (int64_t)imax(0, imin(k, bn.k1) - bn.k0) * 16384,
+				                   8 * (Lmax - bn.level));
+		OZ_HIP(hipMemcpyAsync(dR, R.w, sizeof(R.w), hipMemcpyHostToDevice, st));
+		int *dList = dMark + tilesM * tilesN;
+		OZ_HIP(hipMemsetAsync(dMark, 0, sizeof(int) * (size_t)tilesM * tilesN, st));
+		launch_reconstruct_check(st, m, n, T, ldt, tstride, dLevel, nb1, Lmax, da.colE, db.colE, alpha,
+		                         beta, C, ldc, dR, tile, tilesM, dMark);
+		std::vector<int> mark((size_t)tilesM * tilesN);
+		OZ_HIP(hipMemcpyAsync(mark.data(), dMark, sizeof(int) * mark.size(), hipMemcpyDeviceToHost, st));
+		OZ_TRY(sync(oh));
+		std::vector<int> list;
+		for (int q = 0; q < (int)mark.size(); q++)
+			if (mark[q]) list.push_back(q);
+		fallback = (int)list.size();
+		if (fallback > 0) {
+			OZ_HIP(hipMemcpyAsync(dList, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, st));
+			launch_digit_gemm(st, elem, m, n, Kpad, da.planes, da.ldp, da.pstride, db.planes, db.ldp,
+			                  db.pstride, dPairs, dBins, nbins - nb1, dOrder + nb1, T, ldt, tstride, dList,
+			                  fallback);
+			launch_reconstruct_fix(st, m, n, T, ldt, tstride, dLevel, nb1, nbins, Lmax, da.colE, db.colE,
+			                       alpha, beta, C, ldc, dR, tile, tilesM, dList, fallback);
+		}
+	}
 	OZ_HIP(hipEventRecord(b->ev[5], st));
 	OZ_HIP(hipGetLastError());
 	OZ_TRY(sync(oh));
+	oh->fallbackTiles = fallback;
 	oh->t_SplitA = ev_sec(b->ev[0], b->ev[1]);
 	oh->t_SplitB = ev_sec(b->ev[1], b->ev[2]);
 	oh->t_comp = ev_sec(b->ev[3], b->ev[4]);  // the digit-plane MFMA kernel alone
 	oh->t_sum = ev_sec(b->ev[4], b->ev[5]);
 	oh->nSplitA = (float)SA;
 	oh->nSplitB = (float)SB;
-	oh->nSplitC = (float)pairs.size();
+	oh->nSplitC = (float)pairs1;  // pairs of the first pass (the kernel timed in t_comp)
 	oh->mbk = m;
 	oh->nbk = n;
 	oh->t_total = tt.sec();
-	oh->n_comp = (float)(2.0 * m * n * (double)k * pairs.size() / ((double)Kpad / kc));
+	oh->n_comp = (float)(2.0 * m * n * (double)k * pairs1 / ((double)Kpad / kc));
 	wreset(oh, base);
 	return OZBLAS_STATUS_SUCCESS;
 }
@@ -741,7 +802,6 @@ static int32_t csrmv_impl(ozblasHandle_t *oh, int m, int n, int nnz, double alph
                           double *y) {
 	Backend *b = be(oh);
 	if (oh->sumMode != 0 && oh->sumMode != 1) return OZBLAS_STATUS_NOT_SUPPORTED;
-	if (oh->splitEpsMode == 2) return OZBLAS_STATUS_NOT_SUPPORTED;
 	const uint64_t base = (oh->memMaskSplitA != 0) ? b->keep : 0;
 	wreset(oh, base);
 	CsrSplit as;

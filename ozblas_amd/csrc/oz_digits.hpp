@@ -458,21 +458,22 @@ __host__ __device__ inline double big_round(const BigInt<NW> &a, int e0) {
 	return ldexp((double)mant, r + e0);
 }
 
-// Exact value of one output element: N = sum_b T_b 2^(8 (Lmax - L_b)) over the bins' int32 planes
-// (element at Te[b * tstride], bins in ascending level order), rounded once to nearest-even
-// binary64 as N * 2^e0, then fma(alpha, RN(N 2^e0), beta * c).  NW 64-bit words must hold
-// 8 Lmax + 41 bits.  Levels are first folded four at a time into an int64 (|T| < 2^31, so up to
-// 128 terms of weight <= 2^24 stay below 2^63) and each fold is added to N once.
+// Exact level sums of the fast forms.  N = sum_b T_b 2^(8 (Lmax - L_b)) over int32 planes (element
+// at Te[b * tstride], bins in ascending level order) is accumulated in an NW-word two's-complement
+// integer.  Levels are first folded four at a time into an int64 (|T| < 2^31, so up to 128 terms
+// of weight <= 2^24 stay below 2^63) and each fold is added to N once.
 template <int NW>
-__host__ __device__ inline double recon_elem(const int32_t *Te, int64_t tstride, const int *binLevel,
-                                            int nbins, int Lmax, int e0, double alpha, double beta,
-                                            double c) {
-	BigInt<NW> a;
+__host__ __device__ inline void big_zero(BigInt<NW> &a) {
 #pragma unroll
 	for (int q = 0; q < NW; q++) a.w[q] = 0;
+}
+
+template <int NW>
+__host__ __device__ inline void recon_accumulate(BigInt<NW> &a, const int32_t *Te, int64_t tstride,
+                                                 const int *binLevel, int b0, int b1, int Lmax) {
 	int64_t fold = 0;
 	int group = -1, top = 0, terms = 0;
-	for (int b = 0; b < nbins; b++) {
+	for (int b = b0; b < b1; b++) {
 		const int L = binLevel[b];
 		if (L / 4 != group || terms == 128) {
 			if (fold != 0) big_add_shifted<NW>(a, fold, 8 * (Lmax - top));
@@ -485,6 +486,27 @@ __host__ __device__ inline double recon_elem(const int32_t *Te, int64_t tstride,
 		terms++;
 	}
 	if (fold != 0) big_add_shifted<NW>(a, fold, 8 * (Lmax - top));
+}
+
+// a += sgn * r (NW-word two's complement)
+template <int NW>
+__host__ __device__ inline void big_add_signed(BigInt<NW> &a, const uint64_t *r, int sgn) {
+	uint64_t carry = (sgn < 0) ? 1 : 0;  // a + ~r + 1 for subtraction
+#pragma unroll
+	for (int q = 0; q < NW; q++) {
+		const uint64_t add = (sgn < 0) ? ~r[q] : r[q];
+		const uint64_t s1 = a.w[q] + add;
+		const uint64_t c1 = s1 < add ? 1 : 0;
+		const uint64_t s2 = s1 + carry;
+		const uint64_t c2 = s2 < carry ? 1 : 0;
+		a.w[q] = s2;
+		carry = c1 | c2;
+	}
+}
+
+// RN-even of N * 2^e0 for a signed N
+template <int NW>
+__host__ __device__ inline double big_finish(BigInt<NW> a, int e0) {
 	const bool neg = (a.w[NW - 1] >> 63) != 0;
 	if (neg) {  // two's complement negate
 		uint64_t carry = 1;
@@ -495,9 +517,33 @@ __host__ __device__ inline double recon_elem(const int32_t *Te, int64_t tstride,
 			a.w[q] = v;
 		}
 	}
-	double t = big_round<NW>(a, e0);
-	if (neg) t = -t;
-	return fma(alpha, t, beta * c);
+	const double t = big_round<NW>(a, e0);
+	return neg ? -t : t;
+}
+
+// Exact value of one output element: fma(alpha, RN(N 2^e0), beta * c) over all the bins.
+template <int NW>
+__host__ __device__ inline double recon_elem(const int32_t *Te, int64_t tstride, const int *binLevel,
+                                            int nbins, int Lmax, int e0, double alpha, double beta,
+                                            double c) {
+	BigInt<NW> a;
+	big_zero<NW>(a);
+	recon_accumulate<NW>(a, Te, tstride, binLevel, 0, nbins, Lmax);
+	return fma(alpha, big_finish<NW>(a, e0), beta * c);
+}
+
+// Certified rounding from the leading levels: with N1 the sum over the computed bins and R a
+// bound on |N - N1| (the levels not computed), RN(N 2^e0) is known when RN((N1 - R) 2^e0) and
+// RN((N1 + R) 2^e0) agree, RN being monotone.  Returns true and sets *t in that case.
+template <int NW>
+__host__ __device__ inline bool recon_certify(const BigInt<NW> &n1, const uint64_t *R, int e0,
+                                             double *t) {
+	BigInt<NW> lo = n1, hi = n1;
+	big_add_signed<NW>(lo, R, -1);
+	big_add_signed<NW>(hi, R, +1);
+	const double tl = big_finish<NW>(lo, e0), th = big_finish<NW>(hi, e0);
+	*t = tl;
+	return __builtin_bit_cast(uint64_t, tl) == __builtin_bit_cast(uint64_t, th);
 }
 
 }  // namespace oz

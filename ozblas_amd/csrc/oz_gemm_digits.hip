@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256, 1) void k_digit_gemm_i8(const int8_t *A, int64
                                                           int64_t ldp, const int2 *pairs,
                                                           const LevelBin *bins, const int *binOrder,
                                                           int32_t *T, int64_t ldt, int64_t tstride,
-                                                          int tilesM) {
+                                                          int tilesM, const int *tileList) {
 	constexpr int TBM = 64 * TM, TBN = 64 * TN;
 	__shared__ __attribute__((aligned(16))) char stage0[(TBM + TBN) * I8_BK];
 	__shared__ __attribute__((aligned(16))) char stage1[(TBM + TBN) * I8_BK];
@@ -88,8 +88,9 @@ __global__ __launch_bounds__(256, 1) void k_digit_gemm_i8(const int8_t *A, int64
 	g.npairs = bin.npairs;
 	g.k0 = bin.k0;
 	g.k1 = bin.k1;
-	g.i0 = (blockIdx.x % tilesM) * TBM;
-	g.j0 = (blockIdx.x / tilesM) * TBN;
+	const int tile = tileList ? tileList[blockIdx.x] : (int)blockIdx.x;  // a tile subset: fallback pass
+	g.i0 = (tile % tilesM) * TBM;
+	g.j0 = (tile / tilesM) * TBN;
 	g.T = T + (int64_t)binIdx * tstride;
 	g.ldt = ldt;
 	i8_tile<TM, TN>(g, stage0, stage1);
@@ -174,10 +175,16 @@ __global__ __launch_bounds__(256) void k_digit_gemm_h(int M, int N, const char *
 			}
 }
 
+int digit_gemm_tile(int elem, int M, int N) {
+	if (elem != 1) return BM;
+	return (M >= 512 && N >= 512) ? 256 : 128;
+}
+
 void launch_digit_gemm(hipStream_t st, int elem, int M, int N, int Kpad, const void *Ap,
                        int64_t ldap, int64_t apstride, const void *Bp, int64_t ldbp,
                        int64_t bpstride, const int2 *pairs, const LevelBin *bins, int nbins,
-                       const int *binOrder, int32_t *T, int64_t ldt, int64_t tstride) {
+                       const int *binOrder, int32_t *T, int64_t ldt, int64_t tstride,
+                       const int *tileList, int ntiles) {
 	(void)Kpad;
 	const int tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
 	dim3 grid(tilesM * tilesN, nbins);
@@ -186,15 +193,15 @@ void launch_digit_gemm(hipStream_t st, int elem, int M, int N, int Kpad, const v
 		if (M >= 512 && N >= 512) {
 			constexpr int TM = 4, TN = 4;
 			const int tm = (M + 64 * TM - 1) / (64 * TM), tn = (N + 64 * TN - 1) / (64 * TN);
-			hipLaunchKernelGGL((k_digit_gemm_i8<TM, TN>), dim3(tm * tn, nbins), dim3(256), 0, st,
-			                   (const int8_t *)Ap, apstride, (const int8_t *)Bp, bpstride, ldap,
-			                   pairs, bins, binOrder, T, ldt, tstride, tm);
+			hipLaunchKernelGGL((k_digit_gemm_i8<TM, TN>), dim3(tileList ? ntiles : tm * tn, nbins), dim3(256),
+			                   0, st, (const int8_t *)Ap, apstride, (const int8_t *)Bp, bpstride, ldap,
+			                   pairs, bins, binOrder, T, ldt, tstride, tm, tileList);
 		} else {
 			constexpr int TM = 2, TN = 2;
 			const int tm = (M + 64 * TM - 1) / (64 * TM), tn = (N + 64 * TN - 1) / (64 * TN);
-			hipLaunchKernelGGL((k_digit_gemm_i8<TM, TN>), dim3(tm * tn, nbins), dim3(256), 0, st,
-			                   (const int8_t *)Ap, apstride, (const int8_t *)Bp, bpstride, ldap,
-			                   pairs, bins, binOrder, T, ldt, tstride, tm);
+			hipLaunchKernelGGL((k_digit_gemm_i8<TM, TN>), dim3(tileList ? ntiles : tm * tn, nbins), dim3(256),
+			                   0, st, (const int8_t *)Ap, apstride, (const int8_t *)Bp, bpstride, ldap,
+			                   pairs, bins, binOrder, T, ldt, tstride, tm, tileList);
 		}
 	} else if (elem == 2) {
 		hipLaunchKernelGGL(k_digit_gemm_h<h8v>, grid, dim3(256), 0, st, M, N, (const char *)Ap,
@@ -244,6 +251,101 @@ void launch_reconstruct(hipStream_t st, int M, int N, const int32_t *T, int64_t 
 	else
 		hipLaunchKernelGGL(k_reconstruct<9>, dim3((unsigned)g), dim3(256), 0, st, M, N, T, ldt,
 		                   tstride, binLevel, nbins, Lmax, rowE, colE, alpha, beta, C, ldc);
+}
+
+// Certified early termination (pass 1): N1 over the computed bins [0, nb1); where the rounding
+// of N1 +- R agrees the element is final, otherwise its tile is marked for the remaining levels
+// and C is left untouched.
+template <int NW>
+__global__ __launch_bounds__(256) void k_reconstruct_check(int M, int N, const int32_t *T, int64_t ldt,
+                                                           int64_t tstride, const int *binLevel,
+                                                           int nb1, int Lmax, const int *rowE,
+                                                           const int *colE, double alpha, double beta,
+                                                           double *C, int64_t ldc, const uint64_t *R,
+                                                           int tile, int tilesM, int *tileMark) {
+	uint64_t r[NW];
+#pragma unroll
+	for (int q = 0; q < NW; q++) r[q] = R[q];
+	const int64_t total = (int64_t)M * N;
+	for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+	     e += (int64_t)gridDim.x * blockDim.x) {
+		const int i = (int)(e % M), j = (int)(e / M);
+		BigInt<NW> a;
+		big_zero<NW>(a);
+		recon_accumulate<NW>(a, T + (int64_t)j * ldt + i, tstride, binLevel, 0, nb1, Lmax);
+		double t;
+		if (recon_certify<NW>(a, r, rowE[i] + colE[j] - 16 - 8 * Lmax, &t)) {
+			double *c = C + (int64_t)j * ldc + i;
+			*c = fma(alpha, t, beta * *c);
+		} else {
+			tileMark[(i / tile) + (j / tile) * tilesM] = 1;
+		}
+	}
+}
+
+// Pass 2: every element of the listed tiles whose pass-1 check failed gets the exact sum over all
+// the bins [0, nbins) (the remaining ones were computed for these tiles only).
+template <int NW>
+__global__ __launch_bounds__(256) void k_reconstruct_fix(int M, int N, const int32_t *T, int64_t ldt,
+                                                         int64_t tstride, const int *binLevel,
+                                                         int nb1, int nbins, int Lmax,
+                                                         const int *rowE, const int *colE,
+                                                         double alpha, double beta, double *C,
+                                                         int64_t ldc, const uint64_t *R, int tile,
+                                                         int tilesM, const int *tileList) {
+	const int perTile = tile * tile / 256;  // workgroups per tile
+	const int tl = tileList[blockIdx.x / perTile];
+	const int local = (blockIdx.x % perTile) * 256 + threadIdx.x;
+	const int i = (tl % tilesM) * tile + local % tile, j = (tl / tilesM) * tile + local / tile;
+	if (i >= M || j >= N) return;
+	uint64_t r[NW];
+#pragma unroll
+	for (int q = 0; q < NW; q++) r[q] = R[q];
+	const int32_t *Te = T + (int64_t)j * ldt + i;
+	const int e0 = rowE[i] + colE[j] - 16 - 8 * Lmax;
+	BigInt<NW> a;
+	big_zero<NW>(a);
+	recon_accumulate<NW>(a, Te, tstride, binLevel, 0, nb1, Lmax);
+	double t;
+	if (recon_certify<NW>(a, r, e0, &t)) return;  // final since pass 1
+	recon_accumulate<NW>(a, Te, tstride, binLevel, nb1, nbins, Lmax);
+	double *c = C + (int64_t)j * ldc + i;
+	*c = fma(alpha, big_finish<NW>(a, e0), beta * *c);
+}
+
+int recon_words(int Lmax) { return (8 * Lmax + 32 + 8 + 1 + 63) / 64; }
+
+#define OZ_RECON_NW(NWV, ...)                                     \
+	do {                                                          \
+		const int nw_ = (NWV);                                    \
+		if (nw_ <= 3) { constexpr int NW = 3; __VA_ARGS__; }      \
+		else if (nw_ <= 4) { constexpr int NW = 4; __VA_ARGS__; } \
+		else if (nw_ <= 6) { constexpr int NW = 6; __VA_ARGS__; } \
+		else { constexpr int NW = 9; __VA_ARGS__; }               \
+	} while (0)
+
+void launch_reconstruct_check(hipStream_t st, int M, int N, const int32_t *T, int64_t ldt,
+                              int64_t tstride, const int *binLevel, int nb1, int Lmax,
+                              const int *rowE, const int *colE, double alpha, double beta, double *C,
+                              int64_t ldc, const uint64_t *R, int tile, int tilesM, int *tileMark) {
+	int64_t g = ((int64_t)M * N + 255) / 256;
+	if (g > 16384) g = 16384;
+	OZ_RECON_NW(recon_words(Lmax),
+	            hipLaunchKernelGGL(k_reconstruct_check<NW>, dim3((unsigned)g), dim3(256), 0, st, M, N, T,
+	                               ldt, tstride, binLevel, nb1, Lmax, rowE, colE, alpha, beta, C, ldc, R,
+	                               tile, tilesM, tileMark));
+}
+
+void launch_reconstruct_fix(hipStream_t st, int M, int N, const int32_t *T, int64_t ldt,
+                            int64_t tstride, const int *binLevel, int nb1, int nbins, int Lmax,
+                            const int *rowE, const int *colE, double alpha, double beta, double *C,
+                            int64_t ldc, const uint64_t *R, int tile, int tilesM, const int *tileList,
+                            int ntiles) {
+	const int perTile = tile * tile / 256;
+	OZ_RECON_NW(recon_words(Lmax),
+	            hipLaunchKernelGGL(k_reconstruct_fix<NW>, dim3((unsigned)(ntiles * perTile)), dim3(256), 0,
+	                               st, M, N, T, ldt, tstride, binLevel, nb1, nbins, Lmax, rowE, colE,
+	                               alpha, beta, C, ldc, R, tile, tilesM, tileList));
 }
 
 }  // namespace oz

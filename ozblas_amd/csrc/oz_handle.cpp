@@ -172,3 +172,13 @@ extern "C" double ozblas_internal_recon(int32_t nbins, const int32_t *T, const i
 	if (nw <= 6) return oz::recon_elem<6>(T, 1, levels, nbins, Lmax, e0, alpha, beta, c);
 	return oz::recon_elem<9>(T, 1, levels, nbins, Lmax, e0, alpha, beta, c);
 }
+
+// test hook: the certified rounding of one element from its leading levels: returns 1 and *t when
+// RN((N1 +- R) 2^e0) agree, else 0 (N1 from nb1 bins, R given as 9 little-endian words)
+extern "C" int32_t ozblas_internal_recon_certify(int32_t nb1, const int32_t *T, const int32_t *levels,
+                                                 int32_t Lmax, int32_t e0, const uint64_t *R, double *t) {
+	oz::BigInt<9> a;
+	oz::big_zero<9>(a);
+	oz::recon_accumulate<9>(a, T, 1, levels, 0, nb1, Lmax);
+	return oz::recon_certify<9>(a, R, e0, t) ? 1 : 0;
+}

@@ -59,7 +59,19 @@ struct LevelBin {      // one exact int32 plane: sum over `npairs` digit pairs, 
 void launch_digit_gemm(hipStream_t st, int elem, int M, int N, int Kpad, const void *Ap,
                        int64_t ldap, int64_t apstride, const void *Bp, int64_t ldbp,
                        int64_t bpstride, const int2 *pairs, const LevelBin *bins, int nbins,
-                       const int *binOrder, int32_t *T, int64_t ldt, int64_t tstride);
+                       const int *binOrder, int32_t *T, int64_t ldt, int64_t tstride,
+                       const int *tileList = nullptr, int ntiles = 0);
+int digit_gemm_tile(int elem, int M, int N);
+int recon_words(int Lmax);
+void launch_reconstruct_check(hipStream_t st, int M, int N, const int32_t *T, int64_t ldt,
+                              int64_t tstride, const int *binLevel, int nb1, int Lmax,
+                              const int *rowE, const int *colE, double alpha, double beta, double *C,
+                              int64_t ldc, const uint64_t *R, int tile, int tilesM, int *tileMark);
+void launch_reconstruct_fix(hipStream_t st, int M, int N, const int32_t *T, int64_t ldt,
+                            int64_t tstride, const int *binLevel, int nb1, int nbins, int Lmax,
+                            const int *rowE, const int *colE, double alpha, double beta, double *C,
+                            int64_t ldc, const uint64_t *R, int tile, int tilesM, const int *tileList,
+                            int ntiles);
 void launch_reconstruct(hipStream_t st, int M, int N, const int32_t *T, int64_t ldt,
                         int64_t tstride, const int *binLevel, int nbins, int Lmax, const int *rowE,
                         const int *colE, double alpha, double beta, double *C, int64_t ldc);

@@ -134,3 +134,42 @@ def test_exact_reconstruction_matches_big_integers(lib):
         if want == 0.0:
             want = alpha * t + beta * c  # signed zero as fma gives it
         assert got == want and math.copysign(1, got) == math.copysign(1, want), (trial, Lmax, e0, got, want)
+
+
+
+def test_certified_rounding_from_leading_levels(lib):
+    """recon_certify: when it accepts, every N in [N1 - R, N1 + R] rounds to the returned value;
+    when it refuses, the two ends round differently -- checked with Python integers and exact
+    rounding on random N1 / R straddling and avoiding rounding boundaries."""
+    import random
+    from fractions import Fraction as F
+    f = lib.ozblas_internal_recon_certify
+    f.restype = ctypes.c_int32
+    f.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                  ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+    rng = np.random.default_rng(21)
+    accepted = refused = 0
+    for trial in range(2000):
+        Lmax = int(rng.integers(2, 40))
+        nb1 = int(rng.integers(1, Lmax))
+        lv = np.arange(nb1, dtype=np.int32)
+        T = rng.integers(-2 ** 31, 2 ** 31, nb1).astype(np.int32)
+        N1 = sum(int(t) * 256 ** (Lmax - int(L)) for t, L in zip(T, lv))
+        Rint = int(rng.integers(0, 2 ** 62)) << int(rng.integers(0, max(1, 8 * (Lmax - nb1))))
+        words = np.array([(Rint >> (64 * q)) & (2 ** 64 - 1) for q in range(9)], dtype=np.uint64)
+        bits = max(1, abs(N1).bit_length())
+        e0 = int(rng.integers(-1074 - bits, 900 - bits))
+        out = ctypes.c_double()
+        ok = f(nb1, T.ctypes.data, lv.ctypes.data, Lmax, e0, words.ctypes.data, ctypes.byref(out))
+        lo = float(F(N1 - Rint) * F(2) ** e0)
+        hi = float(F(N1 + Rint) * F(2) ** e0)
+        if ok:
+            accepted += 1
+            assert lo == hi == out.value
+            pick = random.Random(trial).randint(-Rint, Rint)
+            for N in (N1 - Rint, N1, N1 + Rint, N1 + pick):
+                assert float(F(N) * F(2) ** e0) == out.value
+        else:
+            refused += 1
+            assert lo != hi or (lo == hi == 0.0 and np.signbit(lo) != np.signbit(hi))
+    assert accepted > 100 and refused > 100

@@ -132,10 +132,13 @@ def test_batched_new_operands_replan(oz, oracle):
 
 
 def test_batched_other_forms_loop(oz, oracle):
-    """Fsum (sumMode 0) and the FP64 slice form run problem by problem, as ozblasDgemm."""
+    """The FP64 slice form (Fsum and NearSum) runs problem by problem, as ozblasDgemm; a form that
+    ozblasDgemm refuses (digit slices with Fsum) is refused by the batch too."""
     sizes = sizes_of(5, 5, 8, 120)
     probs = make_batch(oz, oracle, sizes, ["NT"] * 5)
-    for flags in (dict(sumMode=0, sliceType="int8"), dict(sumMode=1, sliceType="fp64")):
+    with pytest.raises(oz.OzblasError):
+        run_batched(oz.Handle(1 << 30, sumMode=0, sliceType="int8"), probs)
+    for flags in (dict(sumMode=0, sliceType="fp64"), dict(sumMode=1, sliceType="fp64")):
         H1 = oz.Handle(1 << 30, **flags)
         _, _, dC = run_batched(H1, probs)
         H2 = oz.Handle(1 << 30, **flags)

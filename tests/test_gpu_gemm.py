@@ -182,3 +182,39 @@ def test_plain_gemm_k0_and_empty(oz, oracle):
     C = dev(np.full(12, 7.0))
     H.dgemm("N", "N", 0, 3, 4, 1.0, dev(np.ones(12)), 1, dev(np.ones(12)), 4, 0.0, C, 1)
     assert bits_equal(host(C), np.full(12, 7.0))
+
+
+@pytest.mark.parametrize("levels", [1, 2, 4, 0, -1])
+@pytest.mark.parametrize("m,n,k,ta,tb", [(600, 520, 700, "N", "N"), (200, 130, 3000, "T", "N")])
+def test_int8_certified_levels_bitwise(oz, oracle, levels, m, n, k, ta, tb):
+    """INT8 form with certified early termination: the first `levels` digit levels, then the
+    rounding check and the remaining levels for the tiles that need them (levels 0: automatic,
+    -1: every level in one pass) -- bitwise equal to the plain form in every case."""
+    say(f"int8 certified levels={levels} {ta}{tb} m={m} n={n} k={k}")
+    want, _, _, _ = run_pair(oz, oracle, ta, tb, m, n, k, phi=1.0, sumMode=1, slice_type="fp64")
+    got, _, H, _ = run_pair(oz, oracle, ta, tb, m, n, k, phi=1.0, sumMode=1, slice_type="int8",
+                            digitLevels=levels)
+    assert bits_equal(got, want), first_diff(got, want)
+    if levels in (1, 2):
+        assert H.fallbackTiles > 0  # the check cannot settle most elements from so few levels
+
+
+def test_int8_certified_cancellation(oz, oracle):
+    """Exact cancellation (every C entry is 0 or tiny): the check cannot settle the sign from the
+    leading levels, so every tile takes the second pass; the result is still RN(exact)."""
+    m, n, k = 300, 260, 512
+    A = oracle.init_mat(m, k, m, 1.0, 2, 41).reshape(k, m).T.copy()
+    B = oracle.init_mat(k, n, k, 1.0, 2, 42).reshape(n, k).T.copy()
+    A[:, k // 2:] = A[:, : k // 2]
+    B[k // 2:, :] = -B[: k // 2, :]
+    B[k - 1, :] += np.ldexp(1.0, -70)  # a tiny exact remainder in part of the products
+    Af = np.asfortranarray(A).T.ravel().copy()
+    Bf = np.asfortranarray(B).T.ravel().copy()
+    ex = np.ascontiguousarray(np.asarray(oracle.exact_gemm_rn("N", "N", m, n, k, Af, m, Bf, k)).T).ravel()
+    for levels in (0, 6):
+        H = oz.Handle(1 << 30, sumMode=1, sliceType="int8", digitLevels=levels)
+        dC = dev(np.zeros(m * n))
+        H.dgemm("N", "N", m, n, k, 1.0, dev(Af), m, dev(Bf), k, 0.0, dC, m)
+        assert bits_equal(host(dC), ex), first_diff(host(dC), ex)
+        if levels == 6:
+            assert H.fallbackTiles == ((m + 127) // 128) * ((n + 127) // 128)  # every tile
