@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=2560, help="CPU baseline sample size (0 = skip)")
     ap.add_argument("--cg-iters", type=int, default=500, help="CG iterations on 100^3 (0 = skip)")
     ap.add_argument("--batch", type=int, default=256, help="batched problems (0 = skip)")
+    ap.add_argument("--level12", type=int, default=1, help="DGEMV / DDOT / DCSRMV rates (0 = skip)")
     ap.add_argument("--work-gb", type=float, default=64.0)
     return ap.parse_args()
 
@@ -144,6 +145,59 @@ def bench_cg(iters, torch, oz):
         out["x_bitwise_equal_to_cpu"] = hashlib.sha256(
             dx.cpu().numpy().tobytes()).hexdigest() == case["x_sha256"]
         out["cpu_oracle_seconds_500_iterations"] = case["cpu_seconds"]
+    H.close()
+    return out
+
+
+def bench_level12(torch, oz):
+    """Level 1/2 routines at the sizes of the CG and GEMM configs, NearSum, with the reference
+    harness's byte counts (testing/testing_level1b.cpp:2, testing_level2.cpp:2,
+    testing_level2s.cpp:2): DGEMV (2m + n + mn) 8, DDOT 2n 8, DCSRMV (nnz + 2m + n) 8 + (nnz + m + 1) 4."""
+    import numpy as np
+    out = {}
+
+    def timeit(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    m = n = 8192
+    A = torch.from_numpy(oz.gen_phi(m, n, m, 0.5, 1)).to("cuda:0")
+    x = torch.from_numpy(oz.gen_phi(n, 1, n, 0.5, 2)).to("cuda:0")
+    y = torch.zeros(m, dtype=torch.float64, device="cuda:0")
+    H = oz.Handle(8 << 30, sumMode=1)
+    t = timeit(lambda: H.dgemv("N", m, n, 1.0, A, m, x, 0.0, y), 5)
+    out["dgemv_8192"] = {"ms": round(t * 1e3, 3), "slices": [float(H.nSplitA), float(H.nSplitB)],
+                         "ref_GBps": round((2.0 * m + n + m * n) * 8 / t / 1e9, 1)}
+    H.close()
+    del A
+    nv = 10 ** 6
+    xv = torch.from_numpy(oz.gen_phi(nv, 1, nv, 2.0, 21)).to("cuda:0")
+    yv = torch.from_numpy(oz.gen_phi(nv, 1, nv, 2.0, 22)).to("cuda:0")
+    for mode in (0, 2):
+        H = oz.Handle(1 << 30, sumMode=1, splitEpsMode=mode)
+        t = timeit(lambda: H.ddot(nv, xv, yv), 20)
+        out[f"ddot_1e6{'_dot2' if mode == 2 else ''}"] = {
+            "ms": round(t * 1e3, 3), "slices": [float(H.nSplitA), float(H.nSplitB)],
+            "ref_GBps": round(2.0 * nv * 8 / t / 1e9, 1)}
+        H.close()
+    g = 100
+    rowptr, colind, val = laplace3d(g, 1e-3)
+    nr, nnz = g ** 3, len(val)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+    dval, dcol, drow = d(val), d(colind), d(rowptr)
+    xs = torch.from_numpy(oz.gen_phi(nr, 1, nr, 1.0, 5)).to("cuda:0")
+    ys = torch.zeros(nr, dtype=torch.float64, device="cuda:0")
+    H = oz.Handle(4 << 30, sumMode=1)
+    P = H.dcsrmv_splitA(nr, nr, nnz, dval, drow)
+    t = timeit(lambda: H.dcsrmv(nr, nr, nnz, 1.0, P, dcol, drow, xs, 0.0, ys), 20)
+    out["dcsrmv_laplace100_presplit"] = {
+        "ms": round(t * 1e3, 3), "slices": [float(H.nSplitA), float(H.nSplitB)],
+        "ref_GBps": round(((nnz + 2.0 * nr + nr) * 8 + (nnz + nr + 1.0) * 4) / t / 1e9, 1)}
     H.close()
     return out
 
@@ -287,6 +341,8 @@ def main():
         extra["cg_laplace100"] = bench_cg(args.cg_iters, torch, ozblas_amd)
     if args.batch > 0:
         extra[f"batched{args.batch}"] = bench_batched(args.batch, torch, ozblas_amd)
+    if args.level12:
+        extra["level12"] = bench_level12(torch, ozblas_amd)
     line = {
         "metric": "Ozaki DGEMM eff. TFLOPS (2mnk/t): FP16/INT8-slice vs FP64-slice, same bits",
         "value": round(value, 3), "unit": "TFLOPS", "n_gpus": 1, "steps": r["steps"],
