@@ -121,6 +121,12 @@ typedef struct ozblasHandle {
 	                         0 = automatic (default), -1 = all levels, n > 0 = the first n levels */
 	int32_t fallbackTiles; /* INT8 form: output tiles of the last GEMM that needed the levels after
 	                          the check */
+	int32_t cgDeviceSlices; /* CG: 0 = iterate device-resident (default); -1 = iterate on the
+	                           synchronous path; n > 0 = give up the device path (and redo the
+	                           iterations synchronously) once a search direction has more than n
+	                           slices */
+	int32_t cgDevicePath;   /* last CG: 1 iterated device-resident, 2 gave up and redid the iterations
+	                           synchronously, 0 iterated synchronously */
 } ozblasHandle_t;
 
 /* lifecycle: cuozblasCreate/cuozblasDestroy (include/cuozblas.h:67-68, src/cuda/cuozblas.cpp:23-75).

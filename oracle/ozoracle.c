@@ -875,7 +875,9 @@ int32_t ozo_cg(ozo_handle *h, int32_t n, int32_t nnz, const double *val, const i
 	double alpha, beta, resi, nrmb, tmp, dnew, dold;
 	double *R = (double *)malloc(sizeof(double) * (size_t)n);
 	double *P = (double *)malloc(sizeof(double) * (size_t)n);
-	double *Q = (double *)malloc(sizeof(double) * (size_t)n);
+	/* q is read through 0 * q by the first product (beta = 0): start it at +0, as the fresh
+	   zero pages of the reference's new TYPE1[dimN] of any sizeable n */
+	double *Q = (double *)calloc((size_t)n, sizeof(double));
 	h->cg_numiter = 0;
 	double *AS = ozo_csrmv_splitA(h, n, n, nnz, val, rowptr);
 	nrmb = ozo_nrm2(h, n, b);

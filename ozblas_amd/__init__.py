@@ -50,6 +50,7 @@ class OzblasHandleT(C.Structure):
         ("sliceType", C.c_int32), ("lastStatus", C.c_int32), ("nLevelBins", C.c_int32),
         ("batchTile", C.c_int32), ("batchReplay", C.c_int32), ("batchGrouped", C.c_int32),
         ("digitLevels", C.c_int32), ("fallbackTiles", C.c_int32),
+        ("cgDeviceSlices", C.c_int32), ("cgDevicePath", C.c_int32),
     ]
 
 

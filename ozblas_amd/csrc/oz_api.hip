@@ -509,23 +509,31 @@ static int32_t gemm_fast(ozblasHandle_t *oh, char transA, char transB, int m, in
 struct VecSplit {
 	double *slices = nullptr;
 	int64_t lds = 0;
-	int count = 0;
+	int count = 0;     // host copy of the slice count (valid when read back)
+	int *info = nullptr;  // device: info[0] slice count, info[1] first all-zero remainder
+	unsigned long long *maxv = nullptr;  // device: remainder maxima per slice (cap + 2)
 };
 
-static int32_t vec_split(ozblasHandle_t *oh, int n, const double *x, VecSplit &vs) {
+// Splits x; with readCount the slice count is read back to the host (one synchronisation),
+// otherwise it stays on the device in vs.info for kernels that read it there.
+static int32_t vec_split(ozblasHandle_t *oh, int n, const double *x, VecSplit &vs,
+                         bool readCount = true) {
 	Backend *b = be(oh);
 	const int cap = split_cap(oh);
 	const int rho = rho_host(n, oh->splitEpsMode);
 	vs.lds = rup(n, 32);
-	double *rem = (double *)walloc(oh, 8ull * vs.lds);
 	vs.slices = (double *)walloc(oh, 8ull * vs.lds * (cap + 2));
-	unsigned long long *maxv = (unsigned long long *)walloc(oh, 8ull * (cap + 2));
-	int *info = (int *)walloc(oh, 16);
-	if (!rem || !vs.slices || !maxv || !info) return OZBLAS_STATUS_WORKSPACE;
-	OZ_HIP(hipMemsetAsync(maxv, 0, 8ull * (cap + 2), b->stream));
-	OZ_HIP(launch_split_vec_coop(b->stream, n, x, rem, vs.slices, vs.lds, rho, oh->splitShift, cap,
-	                             oh->reproMode, b->dTau, maxv, info));
-	OZ_HIP(hipMemcpyAsync(b->hPinned, info, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+	vs.maxv = (unsigned long long *)walloc(oh, 8ull * (cap + 2));
+	vs.info = (int *)walloc(oh, 16);
+	void *scratch = walloc(oh, vsplit_scratch_bytes());
+	VsPlan *plan = (VsPlan *)walloc(oh, sizeof(VsPlan));
+	if (!vs.slices || !vs.maxv || !vs.info || !scratch || !plan) return OZBLAS_STATUS_WORKSPACE;
+	OZ_HIP(hipMemsetAsync(scratch, 0, vsplit_scratch_bytes(), b->stream));
+	OZ_HIP(hipMemsetAsync(vs.maxv, 0, 8ull * (cap + 2), b->stream));
+	OZ_HIP(launch_vsplit(b->stream, n, x, vs.slices, vs.lds, rho, oh->splitShift, cap, oh->reproMode,
+	                     b->dTau, vs.maxv, vs.info, scratch, plan));
+	if (!readCount) return OZBLAS_STATUS_SUCCESS;
+	OZ_HIP(hipMemcpyAsync(b->hPinned, vs.info, sizeof(int), hipMemcpyDeviceToHost, b->stream));
 	OZ_TRY(sync(oh));
 	vs.count = ((int32_t *)b->hPinned)[0];
 	return OZBLAS_STATUS_SUCCESS;
@@ -565,20 +573,42 @@ static int32_t dot_impl(ozblasHandle_t *oh, int n, const double *x, const double
 	}
 	const uint64_t base = wmark(oh);
 	oh->splitShift = 1;  // counterInit (src/cpu/ozblas.cpp:8)
+	const bool dot2 = (oh->splitEpsMode == 2);
 	VecSplit xs, ys;
-	OZ_TRY(vec_split(oh, n, x, xs));
+	OZ_TRY(vec_split(oh, n, x, xs, dot2));
 	if (y == x) {
 		ys = xs;
 	} else {
-		OZ_TRY(vec_split(oh, n, y, ys));
+		OZ_TRY(vec_split(oh, n, y, ys, dot2));
 	}
 	double *dres = (double *)walloc(oh, 8);
 	if (!dres) return OZBLAS_STATUS_WORKSPACE;
 	OZ_HIP(hipMemsetAsync(dres, 0, 8, b->stream));
-	OZ_TRY(dot_core(oh, n, xs, ys, dres));
+	if (dot2) {
+		OZ_TRY(dot_core(oh, n, xs, ys, dres));
+	} else {
+		// slice counts stay on the device: one launch for the pair dots and their sum
+		const int cap = split_cap(oh);
+		double *vals = (double *)walloc(oh, 8ull * (cap + 1) * (cap + 1));
+		unsigned *ctr = (unsigned *)walloc(oh, 8);
+		if (!vals || !ctr) return OZBLAS_STATUS_WORKSPACE;
+		OZ_HIP(hipMemsetAsync(vals, 0, 8ull * (cap + 1) * (cap + 1), b->stream));
+		OZ_HIP(hipMemsetAsync(ctr, 0, 8, b->stream));
+		launch_dots_dev(b->stream, n, xs.slices, xs.lds, xs.info, ys.slices, ys.lds, ys.info, vals, ctr,
+		                xs.maxv, ys.maxv, cap + 2, b->dTau, oh->sumMode == 0 ? 1 : 0, nullptr, 0, 0,
+		                dres);
+	}
 	OZ_HIP(hipMemcpyAsync(b->hPinned, dres, 8, hipMemcpyDeviceToHost, b->stream));
+	if (!dot2) {
+		OZ_HIP(hipMemcpyAsync((int32_t *)b->hPinned + 2, xs.info, 4, hipMemcpyDeviceToHost, b->stream));
+		OZ_HIP(hipMemcpyAsync((int32_t *)b->hPinned + 3, ys.info, 4, hipMemcpyDeviceToHost, b->stream));
+	}
 	OZ_TRY(sync(oh));
 	std::memcpy(ret, b->hPinned, 8);
+	if (!dot2) {
+		xs.count = ((int32_t *)b->hPinned)[2];
+		ys.count = ((int32_t *)b->hPinned)[3];
+	}
 	oh->nSplitA = (float)xs.count;
 	oh->nSplitB = (float)ys.count;
 	oh->nSplitC = 1;
@@ -837,8 +867,119 @@ extern "C" int32_t ozblasDcsrmv(ozblasHandle_t *oh, char transA, int32_t m, int3
 	return oh->lastStatus = csrmv_impl(oh, m, n, nnz, alpha, A, colind, rowptr, x, beta, y);
 }
 
-// CG (src/cpu/ozblas_cg.cpp:24-128): vectors stay on the device; the scalar recurrences are
-// evaluated on the host in binary64 (IEEE division and sqrt, as the reference)
+// CG (src/cpu/ozblas_cg.cpp:24-128).  The set-up (pre-split of A, |b|, r0 = b - A x0, <r0,r0>)
+// runs through the synchronous entry points; the iterations then run device-resident: every
+// iteration is enqueued without waiting (cg_device_loop).  When a vector has more slices than the
+// device CSR product holds, or the options leave the device path, the iterations run on the
+// synchronous path with the scalar recurrences on the host (cg_host_loop); both give the same bits.
+namespace oz {
+
+struct CgVecs {
+	double *R = nullptr, *P = nullptr, *Q = nullptr, *X0 = nullptr;
+};
+
+static int32_t cg_host_loop(ozblasHandle_t *oh, int n, int nnz, const double *AS, const int *colind,
+                            const int *rowptr, double *x, int maxiter, double tol, double nrmb,
+                            double dold, const CgVecs &v, double *hist) {
+	hipStream_t st = be(oh)->stream;
+	double tmp, alpha, dnew, resi, beta;
+	int vcnt = 1;
+	while (oh->cg_numiter < maxiter) {
+		oh->cg_numiter++;
+		OZ_TRY(csrmv_impl(oh, n, n, nnz, 1.0, AS, colind, rowptr, v.P, 0.0, v.Q));
+		OZ_TRY(ozblasDdot(oh, n, v.P, 1, v.Q, 1, &tmp));
+		alpha = dold / tmp;
+		launch_axpy_host(st, n, alpha, v.P, x, oh->reproMode != 0);
+		launch_axpy_host(st, n, -alpha, v.Q, v.R, oh->reproMode != 0);
+		OZ_TRY(ozblasDdot(oh, n, v.R, 1, v.R, 1, &dnew));
+		resi = sqrt(dnew);
+		beta = dnew / dold;
+		dold = dnew;
+		launch_scal_axpy(st, n, beta, v.R, v.P);
+		if (oh->verbose > 0) {
+			if (resi / nrmb < tol || oh->cg_numiter % oh->verbose == 0) {
+				if (hist) hist[vcnt] = resi / nrmb;
+				vcnt++;
+			}
+		}
+		if (resi / nrmb < tol) break;
+	}
+	OZ_HIP(hipStreamSynchronize(st));
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+// Device-resident iterations; *overflow = 1 when the host path must redo them.
+static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *AS, const int *colind,
+                              const int *rowptr, double *x, int maxiter, double tol, double nrmb,
+                              double dold, const CgVecs &v, double *hist, int *overflow) {
+	Backend *b = be(oh);
+	hipStream_t st = b->stream;
+	const int cap = split_cap(oh);
+	const int rho = rho_host(n, oh->splitEpsMode);
+	const int64_t lds = rup(n, 32);
+	const int fsum = oh->sumMode == 0 ? 1 : 0;
+	const int nhist = (oh->verbose > 0) ? maxiter / oh->verbose + 2 : 1;
+	double *slP = (double *)walloc(oh, 8ull * lds * (cap + 2));
+	double *slQ = (double *)walloc(oh, 8ull * lds * (cap + 2));
+	double *slR = (double *)walloc(oh, 8ull * lds * (cap + 2));
+	const size_t small = 8ull * 3 * (cap + 2) + 8ull * (cap + 1) * (cap + 1) + 64 + 8ull * nhist;
+	char *sm = (char *)walloc(oh, small);
+	CgDev *S = (CgDev *)walloc(oh, sizeof(CgDev));
+	void *vscr = walloc(oh, vsplit_scratch_bytes());
+	VsPlan *plan = (VsPlan *)walloc(oh, sizeof(VsPlan));
+	if (!vscr || !plan) return OZBLAS_STATUS_WORKSPACE;
+	const int maxVec = (oh->cgDeviceSlices > 0 && oh->cgDeviceSlices < cap) ? oh->cgDeviceSlices : cap;
+	const int64_t ldc = rup(n, 32);
+	double *Cs = (double *)walloc(oh, 8ull * ldc * nA * maxVec);  // pair sums of the general product
+	if (!slP || !slQ || !slR || !sm || !S || !Cs) return OZBLAS_STATUS_WORKSPACE;
+	unsigned long long *mxP = (unsigned long long *)sm;
+	unsigned long long *mxQ = mxP + (cap + 2), *mxR = mxQ + (cap + 2);
+	double *vals = (double *)(mxR + (cap + 2));
+	int *info = (int *)(vals + (cap + 1) * (cap + 1));  // P, Q, R: two ints each
+	unsigned *ctr = (unsigned *)(info + 6);
+	double *dhist = (double *)(info + 16);
+	OZ_HIP(hipMemsetAsync(sm, 0, small, st));
+	OZ_HIP(hipMemsetAsync(vscr, 0, vsplit_scratch_bytes(), st));
+	CgDev h{};
+	h.nrmb = nrmb;
+	h.dold = dold;
+	h.tol = tol;
+	h.hist = (oh->verbose > 0) ? dhist : nullptr;
+	h.stopAt = INT32_MAX;
+	h.vcnt = 1;
+	h.verbose = oh->verbose;
+	h.maxVecSlices = maxVec;
+	OZ_HIP(hipMemcpyAsync(S, &h, sizeof(CgDev), hipMemcpyHostToDevice, st));
+	int *infoP = info, *infoQ = info + 2, *infoR = info + 4;
+	for (int it = 0; it < maxiter; it++) {
+		OZ_HIP(launch_vsplit(st, n, v.P, slP, lds, rho, 1, cap, oh->reproMode, b->dTau, mxP, infoP,
+		                     vscr, plan, &S->stopAt, it));
+		launch_csrmm_sum_dev(st, n, AS, rup(b->nnzA, 32), nA, colind, rowptr, slP, lds, infoP, fsum,
+		                     v.Q, Cs, ldc, b->dTau, S, it);
+		OZ_HIP(launch_vsplit(st, n, v.Q, slQ, lds, rho, 1, cap, oh->reproMode, b->dTau, mxQ, infoQ,
+		                     vscr, plan, &S->stopAt, it));
+		launch_dots_dev(st, n, slP, lds, infoP, slQ, lds, infoQ, vals, ctr, mxP, mxQ, cap + 2, b->dTau,
+		                fsum, S, it, 1, nullptr);
+		launch_axpy2_dev(st, n, S, it, v.P, x, v.Q, v.R);
+		OZ_HIP(launch_vsplit(st, n, v.R, slR, lds, rho, 1, cap, oh->reproMode, b->dTau, mxR, infoR,
+		                     vscr, plan, &S->stopAt, it));
+		launch_dots_dev(st, n, slR, lds, infoR, slR, lds, infoR, vals, ctr, mxR, mxR, cap + 2, b->dTau,
+		                fsum, S, it, 2, nullptr);
+		launch_scal_axpy_dev(st, n, S, it, v.R, v.P);
+	}
+	OZ_HIP(hipGetLastError());
+	OZ_HIP(hipMemcpyAsync(&h, S, sizeof(CgDev), hipMemcpyDeviceToHost, st));
+	OZ_HIP(hipStreamSynchronize(st));
+	*overflow = h.overflow;
+	if (h.overflow) return OZBLAS_STATUS_SUCCESS;
+	oh->cg_numiter = (h.stopAt < maxiter) ? h.stopAt : maxiter;
+	if (oh->verbose > 0 && hist && h.vcnt > 1)
+		OZ_HIP(hipMemcpy(hist + 1, dhist + 1, 8ull * (h.vcnt - 1), hipMemcpyDeviceToHost));
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+}  // namespace oz
+
 extern "C" int32_t ozblasDcg(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz,
                              const void *descrA, const double *A, const int32_t *colind,
                              const int32_t *rowptr, const double *b_, double *x, int32_t maxiter,
@@ -850,65 +991,85 @@ extern "C" int32_t ozblasDcg(ozblasHandle_t *oh, char transA, int32_t n, int32_t
 	hipStream_t st = b->stream;
 	double *hist = (double *)oh->cg_verbose1;
 	oh->cg_numiter = 0;
-	double *R, *P, *Q;
-	OZ_HIP(hipMalloc(&R, 8ull * n));
-	OZ_HIP(hipMalloc(&P, 8ull * n));
-	OZ_HIP(hipMalloc(&Q, 8ull * n));
+	oh->cgDevicePath = 0;
+	CgVecs v;
+	OZ_HIP(hipMalloc(&v.R, 8ull * n));
+	OZ_HIP(hipMalloc(&v.P, 8ull * n));
+	OZ_HIP(hipMalloc(&v.Q, 8ull * n));
+	OZ_HIP(hipMalloc(&v.X0, 8ull * n));
 	int32_t s = OZBLAS_STATUS_SUCCESS;
-	double nrmb = 0, dold = 0, dnew = 0, tmp = 0, alpha = 0, beta = 0, resi = 0;
-	double *AS = ozblasDcsrmvSplitA(oh, transA, n, n, nnz, descrA, A, rowptr);
+	double nrmb = 0, dold = 0, resi = 0;
+	int overflow = 1;
+	double *AS = nullptr;
+	// q starts finite: the first product reads it through beta * q with beta = 0
+	if (hipMemsetAsync(v.Q, 0, 8ull * n, st) != hipSuccess ||
+	    hipMemcpyAsync(v.X0, x, 8ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+		s = OZBLAS_STATUS_EXECUTION_FAILED;
+		goto done;
+	}
+	AS = ozblasDcsrmvSplitA(oh, transA, n, n, nnz, descrA, A, rowptr);
 	if (!AS) {
 		s = oh->lastStatus;
 		goto done;
 	}
+	b->nnzA = nnz;
 	s = ozblasDnrm2(oh, n, b_, 1, &nrmb);
 	if (s) goto done;
-	if (hipMemcpyAsync(R, b_, 8ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+	if (hipMemcpyAsync(v.R, b_, 8ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
 		s = OZBLAS_STATUS_EXECUTION_FAILED;
 		goto done;
 	}
-	s = csrmv_impl(oh, n, n, nnz, -1.0, AS, colind, rowptr, x, 1.0, R);
+	s = csrmv_impl(oh, n, n, nnz, -1.0, AS, colind, rowptr, x, 1.0, v.R);
 	if (s) goto done;
-	s = ozblasDdot(oh, n, R, 1, R, 1, &dold);
+	s = ozblasDdot(oh, n, v.R, 1, v.R, 1, &dold);
 	if (s) goto done;
 	resi = sqrt(dold);
 	if (oh->verbose > 0 && hist) hist[0] = resi / nrmb;
-	if (hipMemcpyAsync(P, R, 8ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+	if (hipMemcpyAsync(v.P, v.R, 8ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
 		s = OZBLAS_STATUS_EXECUTION_FAILED;
 		goto done;
 	}
 	{
-		int vcnt = 1;
-		while (oh->cg_numiter < maxiter) {
-			oh->cg_numiter++;
-			s = csrmv_impl(oh, n, n, nnz, 1.0, AS, colind, rowptr, P, 0.0, Q);
+		bool devicePath = oh->cgDeviceSlices >= 0 && oh->reproMode != 0 && oh->fastMode == 0 &&
+		                        oh->splitEpsMode != 2 &&
+		                        oh->nSplitA_ <= 4 && maxiter > 0;
+		if (devicePath) {
+			wreset(oh, b->keep);
+			s = cg_device_loop(oh, n, oh->nSplitA_, AS, colind, rowptr, x, maxiter, tol, nrmb, dold, v,
+			                   hist, &overflow);
+			wreset(oh, b->keep);
+			if (s == OZBLAS_STATUS_WORKSPACE) {  // nothing ran: iterate on the synchronous path
+				s = OZBLAS_STATUS_SUCCESS;
+				overflow = 1;
+				devicePath = false;
+			}
 			if (s) goto done;
-			s = ozblasDdot(oh, n, P, 1, Q, 1, &tmp);
-			if (s) goto done;
-			alpha = dold / tmp;
-			launch_axpy_host(st, n, alpha, P, x, oh->reproMode != 0);
-			launch_axpy_host(st, n, -alpha, Q, R, oh->reproMode != 0);
-			s = ozblasDdot(oh, n, R, 1, R, 1, &dnew);
-			if (s) goto done;
-			resi = sqrt(dnew);
-			beta = dnew / dold;
-			dold = dnew;
-			launch_scal_axpy(st, n, beta, R, P);
-			if (oh->verbose > 0) {
-				if (resi / nrmb < tol || oh->cg_numiter % oh->verbose == 0) {
-					if (hist) hist[vcnt] = resi / nrmb;
-					vcnt++;
+		}
+		oh->cgDevicePath = !devicePath ? 0 : (overflow ? 2 : 1);
+		if (overflow) {  // synchronous path (from the saved set-up state when the device path gave up)
+			if (devicePath) {
+				if (hipMemcpyAsync(x, v.X0, 8ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+				    hipMemcpyAsync(v.R, b_, 8ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+					s = OZBLAS_STATUS_EXECUTION_FAILED;
+					goto done;
+				}
+				s = csrmv_impl(oh, n, n, nnz, -1.0, AS, colind, rowptr, x, 1.0, v.R);
+				if (s) goto done;
+				if (hipMemcpyAsync(v.P, v.R, 8ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+					s = OZBLAS_STATUS_EXECUTION_FAILED;
+					goto done;
 				}
 			}
-			if (resi / nrmb < tol) break;
+			oh->cg_numiter = 0;
+			s = cg_host_loop(oh, n, nnz, AS, colind, rowptr, x, maxiter, tol, nrmb, dold, v, hist);
 		}
 	}
-	if (hipStreamSynchronize(st) != hipSuccess) s = OZBLAS_STATUS_EXECUTION_FAILED;
 done:
 	(void)hipStreamSynchronize(st);
-	(void)hipFree(R);
-	(void)hipFree(P);
-	(void)hipFree(Q);
+	(void)hipFree(v.R);
+	(void)hipFree(v.P);
+	(void)hipFree(v.Q);
+	(void)hipFree(v.X0);
 	oh->memMaskSplitA = 0;
 	b->keep = 0;
 	oh->splitShift = 1;

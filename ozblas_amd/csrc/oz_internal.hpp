@@ -49,6 +49,7 @@ struct Backend {
 	int32_t *dCounters = nullptr;  // device scratch counters (256 x 4 bytes)
 	hipEvent_t ev[8] = {};
 	void *batch = nullptr;  // cached plan of ozblasDgemmBatched (oz_batched.hip)
+	int nnzA = 0;           // nonzeros of the matrix pre-split for CG (slice pitch rup(nnz, 32))
 };
 
 void batch_plan_free(void *plan);

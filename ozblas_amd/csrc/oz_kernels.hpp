@@ -96,11 +96,47 @@ void launch_csrmm_sum(hipStream_t st, int m, int nnz, const double *As, int64_t 
 void launch_csrmm_store(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
                         const int *colind, const int *rowptr, const double *Xs, int64_t ldxs,
                         int nB, int maxlevel, double *Cs, int64_t ldcs);
-hipError_t launch_split_vec_coop(hipStream_t st, int n, const double *x, double *rem,
-                                 double *slices, int64_t lds, int rho, int shift, int cap,
-                                 int reproMode, const uint16_t *J, unsigned long long *maxv,
-                                 int *info);
+// barrier-free vector split (plan of levels, then slices); VsPlan lives in device memory
+constexpr int kVsMaxLevels = 64;
+struct VsPlan {
+	int count, last, zero, bad;
+	int k[kVsMaxLevels];  // log2 of the slice unit of level s
+};
+size_t vsplit_scratch_bytes();
+hipError_t launch_vsplit(hipStream_t st, int n, const double *x, double *slices, int64_t lds,
+                         int rho, int shift, int cap, int reproMode, const uint16_t *J,
+                         unsigned long long *maxv, int *info, void *scratch, VsPlan *plan,
+                         const int *stopAt = nullptr, int it = 0);
 void launch_axpy_host(hipStream_t st, int n, double alpha, const double *x, double *y, int fused);
 void launch_scal_axpy(hipStream_t st, int n, double beta, const double *r, double *p);
+
+// ---- device-resident CG (oz_level12.hip)
+// Scalars of ozblasRcg kept on the device between the kernels of one solve; a kernel of
+// iteration `it` does nothing once it >= stopAt.
+struct CgDev {
+	double nrmb, dold, dnew, tmp, alpha, beta, resi, tol;
+	double *hist;    // device history (resi / nrmb), index vcnt
+	int stopAt;      // first iteration not to run (INT32_MAX while iterating)
+	int overflow;    // a vector had more slices than the CSR product kernel holds
+	int numiter;
+	int vcnt;
+	int verbose;
+	int maxVecSlices;  // the CSR product gives up (overflow) when p has more slices than this
+	int pad[2];
+};
+// q = A p on the device path: a register kernel for p with <= 8 slices and a general one (pair
+// sums through Cs, m x nA x maxVecSlices doubles at pitch ldc) for more
+void launch_csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
+                          const int *colind, const int *rowptr, const double *Xs, int64_t ldxs,
+                          const int *infoX, int fsum, double *y, double *Cs, int64_t ldc,
+                          const uint16_t *J, CgDev *S, int it);
+void launch_dots_dev(hipStream_t st, int n, const double *xs, int64_t ldx, const int *infoX,
+                     const double *ys, int64_t ldy, const int *infoY, double *vals, unsigned *ctr,
+                     unsigned long long *maxvX, unsigned long long *maxvY, int nmax,
+                     const uint16_t *J, int fsum, CgDev *S, int it, int mode, double *C);
+void launch_axpy2_dev(hipStream_t st, int n, const CgDev *S, int it, const double *P, double *X,
+                      const double *Q, double *R);
+void launch_scal_axpy_dev(hipStream_t st, int n, const CgDev *S, int it, const double *R,
+                          double *P);
 
 }  // namespace oz

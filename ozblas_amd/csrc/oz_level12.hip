@@ -1,11 +1,9 @@
 // oz_level12.hip -- bandwidth-bound kernels of the plain form: vector split, slice DOT, CSR
 // slice products with fused NearSum/Fsum, AXPY.
-#include <hip/hip_cooperative_groups.h>
+#include <algorithm>
 
 #include "oz_kernels.hpp"
 #include "oz_nearsum.hpp"
-
-namespace cg = cooperative_groups;
 
 namespace oz {
 
@@ -34,57 +32,201 @@ __device__ inline double block_sum_d(double v, double *red) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Vector split in one cooperative launch (ozblasSplit with n == 1, reference
-// src/cpu/ozblas_split.cpp:75-104,254-289): the tau of slice s comes from the grid-wide max of the
-// previous remainder, so every slice ends with a grid barrier.  maxv[] (zeroed by the caller)
-// receives max|remainder| after each slice; info[0] the slice count of the reference loop.
+// Vector split without grid barriers.
+//
+// With sigma_s = 0.75 * 2^(rho+tau_s) (rho >= 3, sigma_s normal) the reference's slice
+// (r + sigma) - sigma is r rounded to the nearest multiple of u_s = ulp(sigma_s) = 2^(rho+tau_s-53),
+// ties to even (sigma_s / u_s = 3 * 2^51 is even and |r| <= 2^(rho+tau_s-2) keeps the sum in the
+// binade of sigma_s).  The units shrink (u_{s+1} <= u_s 2^(rho-54)) and ties-to-even commutes with
+// shifts by even multiples, so the remainder after slice s is the centred residue
+//   r_{s+1} = cmod(x, u_s) = x - u_s * rne(x / u_s)
+// of the INPUT, whatever the earlier units were.  Hence the remainder maxima the reference reduces
+// level by level are values of one function of the unit, F(k) = max_i |cmod(x_i, 2^k)|:
+// M_{s+1} = F(k_s), k_s = rho + tau(M_s) - 53, M_0 = max|x|.  k_vsplit_scan tabulates F over all k
+// in one pass (per element only k in (lsb, msb+1] are non-trivial: below, the residue is 0; above,
+// it is x itself), its last workgroup walks the chain of levels, and k_vsplit_write stores the
+// slices slice_s = cmod(x, u_{s-1}) - cmod(x, u_s) (exact).  Inputs outside the model (non-finite,
+// |x| >= 2^1022, rho < 3, a sigma outside the normal range, a non power-of-two shift, more levels
+// than VsPlan holds) set plan.bad and are split by k_vsplit_fallback instead, which runs the
+// reference loop in one workgroup.
 // ---------------------------------------------------------------------------------------------
+constexpr int kVsOff = 1100;          // F/G tables cover k in [-1100, 1099]
+constexpr int kVsK = 2200;
+constexpr int kVsBlocks = 256;
+constexpr int kVsBS = 512;
+
+__device__ inline double vs_cmod(double x, int k) {
+	if (x == 0.0) return 0.0;
+	const uint64_t b = d2u(x);
+	const int E = (int)((b >> 52) & 0x7ff);
+	const uint64_t f = b & ((uint64_t(1) << 52) - 1);
+	int e, q;
+	if (E == 0) {
+		e = -1074 + (63 - __builtin_clzll(f));
+		q = -1074 + __builtin_ctzll(f);
+	} else {
+		e = E - 1023;
+		q = e - 52 + __builtin_ctzll(f | (uint64_t(1) << 52));
+	}
+	if (k <= q) return 0.0;
+	if (k > e + 1) return x;
+	const double n = rint(ldexp(x, -k));
+	return x - ldexp(n, k);
+}
+
 template <int BS>
-__global__ __launch_bounds__(BS) void k_split_vec_coop(int n, const double *x, double *rem,
-                                                       double *slices, int64_t lds, int rho,
-                                                       int shift, int cap, int reproMode,
-                                                       const uint16_t *J,
-                                                       unsigned long long *maxv, int *info) {
-	cg::grid_group grid = cg::this_grid();
-	__shared__ double red[BS / 64];
-	const int64_t stride = (int64_t)gridDim.x * BS;
-	const int64_t i0 = (int64_t)blockIdx.x * BS + threadIdx.x;
-	double mx = 0.0;
-	for (int64_t i = i0; i < n; i += stride) mx = std_max(mx, fabs(x[i]));
-	mx = block_max_d<BS>(mx, red);
-	if (threadIdx.x == 0) atomicMax(&maxv[0], (unsigned long long)d2u(mx));
-	grid.sync();
-	int s;
-	bool zero = false;
-	for (s = 0; s < cap; s++) {
-		const double m = __builtin_bit_cast(double, maxv[s]);
-		const int tau = tau_of(m, J);
-		const double sigma = split_sigma(rho, tau, shift);
-		const double *src = (s == 0) ? x : rem;
-		double *sl = slices + (int64_t)s * lds;
-		double lm = 0.0;
-		for (int64_t i = i0; i < n; i += stride) {
-			const double v = src[i];
-			const double a = (v + sigma) - sigma;
-			const double rr = v - a;
-			sl[i] = a;
-			rem[i] = rr;
-			lm = std_max(lm, fabs(rr));
+__global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const double *x, unsigned long long *Fg,
+                                                    int2 *range,
+                                                    unsigned *ctr, int rho, int cap,
+                                                    int reproMode, const uint16_t *J, VsPlan *plan,
+                                                    int *info, unsigned long long *maxv,
+                                                    const int *stopAt, int it) {
+	if (stopAt && it >= *stopAt) return;
+	__shared__ unsigned long long Fb[kVsK], Gb[kVsK];  // Gb: scratch of the last workgroup
+	__shared__ int kminB, kmaxB, badB, isLast;
+	__shared__ int red[2][BS / 64];
+	for (int k = threadIdx.x; k < kVsK; k += BS) Fb[k] = 0ull;
+	if (threadIdx.x == 0) badB = 0;
+	__syncthreads();
+	const int lane = threadIdx.x % 64;
+	int tlo = INT32_MAX, thi = INT32_MIN;  // this thread's range of k
+	for (int64_t base = (int64_t)blockIdx.x * BS; base < n; base += (int64_t)gridDim.x * BS) {
+		const int64_t i = base + threadIdx.x;
+		const double v = (i < n) ? x[i] : 0.0;
+		// x = m 2^eb with m < 2^53 an integer (m = 0: no contribution)
+		const uint64_t b = d2u(v);
+		const int E = (int)((b >> 52) & 0x7ff);
+		uint64_t m = b & ((uint64_t(1) << 52) - 1);
+		int eb = -1074, lo = INT32_MAX, hi = INT32_MIN;
+		if (E == 0x7ff || E >= 1023 + 1022) {
+			badB = 1;
+			m = 0;
+		} else if (v != 0.0) {
+			if (E != 0) {
+				m |= uint64_t(1) << 52;
+				eb = E - 1075;
+			}
+			lo = eb + __builtin_ctzll(m) + 1;      // k below: residue 0
+			hi = eb + (64 - __builtin_clzll(m)) + 1;  // msb + 2: residue x itself
+		} else {
+			m = 0;
 		}
-		lm = block_max_d<BS>(lm, red);
-		if (threadIdx.x == 0) atomicMax(&maxv[s + 1], (unsigned long long)d2u(lm));
-		grid.sync();
-		if (__builtin_bit_cast(double, maxv[s + 1]) == 0.0) {
+		tlo = min(tlo, lo);
+		thi = max(thi, hi);
+		if (m == 0) continue;
+		// key(k) = |cmod(x, 2^k)| / 2^k * 2^64 for k in [lo, hi]: m modulo 2^s centred (ties to
+		// even), s = k - eb; exact (< 2^63, at most 53 significant bits).  Each lane starts at its
+		// own offset so one LDS atomic instruction hits different table entries.
+		const int K = hi - lo + 1;
+		for (int j = 0; j < K; j++) {
+			int t = j + lane;
+			if (t >= K) {
+				t -= K;
+				if (t >= K) t %= K;
+			}
+			const int k = lo + t;
+			const int sft = k - eb;  // 1 .. 54
+			uint64_t r = m & ((uint64_t(1) << sft) - 1);
+			const uint64_t half = uint64_t(1) << (sft - 1);
+			if (r > half || (r == half && ((m >> sft) & 1))) r = (uint64_t(1) << sft) - r;
+			if (r) atomicMax(&Fb[k + kVsOff], r << (64 - sft));
+		}
+	}
+	// workgroup range of k (a shuffle tree per wave, then the waves)
+	for (int o = 32; o > 0; o >>= 1) {
+		tlo = min(tlo, __shfl_xor(tlo, o, 64));
+		thi = max(thi, __shfl_xor(thi, o, 64));
+	}
+	if (lane == 0) {
+		red[0][threadIdx.x / 64] = tlo;
+		red[1][threadIdx.x / 64] = thi;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		int a = red[0][0], c = red[1][0];
+		for (int w = 1; w < BS / 64; w++) {
+			a = min(a, red[0][w]);
+			c = max(c, red[1][w]);
+		}
+		kminB = a;
+		kmaxB = c;
+	}
+	__syncthreads();
+	// fold this workgroup's table over its range into the global one (Fg: zero between uses)
+	const int r0 = kminB, r1 = kmaxB;
+	if (r0 <= r1) {  // an all-zero workgroup has the empty range [INT32_MAX, INT32_MIN]
+		for (int k = r0 + (int)threadIdx.x; k <= r1; k += BS) {
+			const unsigned long long f = Fb[k + kVsOff];
+			if (f) atomicMax(&Fg[k + kVsOff], f);
+		}
+	}
+	// range: [r0, r1] (empty when r0 > r1); x = INT32_MIN marks an input outside the model
+	if (threadIdx.x == 0) range[blockIdx.x] = badB ? make_int2(INT32_MIN, INT32_MIN) : make_int2(r0, r1);
+	__threadfence();
+	__syncthreads();
+	if (threadIdx.x == 0) isLast = (atomicAdd(ctr, 1u) == gridDim.x - 1) ? 1 : 0;
+	__syncthreads();
+	if (!isLast) return;
+	__threadfence();
+	// ---- last workgroup: the union of the ranges, the global tables, then the levels
+	if (threadIdx.x == 0) {
+		kminB = INT32_MAX;
+		kmaxB = INT32_MIN;
+		badB = 0;
+	}
+	__syncthreads();
+	for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) {
+		const int2 rg = range[b];
+		if (rg.x == INT32_MIN) {
+			badB = 1;
+		} else if (rg.x <= rg.y) {
+			atomicMin(&kminB, rg.x);
+			atomicMax(&kmaxB, rg.y);
+		}
+	}
+	__syncthreads();
+	const int K0 = kminB, K1 = kmaxB;  // empty ([INT32_MAX, INT32_MIN]) for an all-zero vector
+	for (int k = (K0 <= K1 ? K0 + (int)threadIdx.x : 1); k <= K1; k += BS) {
+		// F(k) = key 2^(k-64) (key < 2^63 has at most 53 significant bits: exact)
+		Gb[k + kVsOff] = d2u(ldexp((double)Fg[k + kVsOff], k - 64));
+		Fg[k + kVsOff] = 0ull;  // clear for the next split
+	}
+	__syncthreads();
+	if (threadIdx.x != 0) return;
+	int bad = badB;
+	// |cmod(x, 2^k)| does not decrease with k (a residue modulo 2^k' is one modulo 2^k, and the
+	// centred one is the smallest), so F is the prefix max of the entries; beyond K1 it is max |x|
+	unsigned long long run = 0ull;
+	for (int k = K0; k <= K1; k++) {
+		run = Gb[k + kVsOff] > run ? Gb[k + kVsOff] : run;
+		Gb[k + kVsOff] = run;
+	}
+	const unsigned long long gmax = run;  // max |x|
+	auto Fk = [&](int k) -> double {
+		if (K0 > K1 || k < K0) return 0.0;
+		if (k > K1) return __builtin_bit_cast(double, gmax);
+		return __builtin_bit_cast(double, Gb[k + kVsOff]);
+	};
+	if (rho < 3 || cap > kVsMaxLevels) bad = 1;
+	double M = __builtin_bit_cast(double, gmax);
+	int s = 0;
+	bool zero = false;
+	for (s = 0; s < cap && !bad; s++) {
+		const int tau = tau_of(M, J);
+		if (rho + tau > 1022 || rho + tau - 1 < -1022) {
+			bad = 1;
+			break;
+		}
+		const int k = rho + tau - 53;
+		plan->k[s] = k;
+		M = Fk(k);
+		if (M == 0.0) {
 			zero = true;
 			break;
 		}
 	}
-	const int last = zero ? s : cap - 1;
-	double *slot = slices + (int64_t)(last + 1) * lds;
-	for (int64_t i = i0; i < n; i += stride) slot[i] = zero ? 0.0 : rem[i];
-	if (blockIdx.x == 0 && threadIdx.x == 0) {
-		// reference count: first s >= 1 with all-zero remainder, +1 (slice 1 is computed even when
-		// slice 0 is already exact); cap slices when never exact; +1 remainder slot for reproMode 0
+	plan->bad = bad;
+	if (!bad) {
 		int count;
 		if (cap <= 1)
 			count = 1;
@@ -93,49 +235,117 @@ __global__ __launch_bounds__(BS) void k_split_vec_coop(int n, const double *x, d
 		else
 			count = cap;
 		if (reproMode == 0) count++;
+		plan->count = count;
+		plan->last = zero ? s : cap - 1;
+		plan->zero = zero ? 1 : 0;
+		info[0] = count;
+		info[1] = zero ? s : cap;
+	} else {
+		for (int q = 0; q < cap + 2; q++) maxv[q] = 0ull;  // the fallback reduces into maxv
+	}
+	*ctr = 0u;
+}
+
+// slices from the plan: slice_s = cmod(x, u_{s-1}) - cmod(x, u_s), remainder slot, zero slots
+__global__ __launch_bounds__(256) void k_vsplit_write(int n, const double *x, double *slices,
+                                                      int64_t lds, const VsPlan *plan,
+                                                      const int *stopAt, int it) {
+	if (stopAt && it >= *stopAt) return;
+	if (plan->bad) return;
+	const int last = plan->last, count = plan->count, zero = plan->zero;
+	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+	     i += (int64_t)gridDim.x * blockDim.x) {
+		const double v = x[i];
+		double r = v;
+		for (int s = 0; s <= last; s++) {
+			const double c = vs_cmod(v, plan->k[s]);
+			slices[(int64_t)s * lds + i] = r - c;
+			r = c;
+		}
+		slices[(int64_t)(last + 1) * lds + i] = zero ? 0.0 : r;
+		for (int q = last + 2; q < count; q++) slices[(int64_t)q * lds + i] = 0.0;
+	}
+}
+
+// the reference loop in one workgroup (inputs outside the model above): slice s at slot s, its
+// remainder at slot s + 1 (read back as the next level's input), as src/cpu/ozblas_split.cpp:96
+template <int BS>
+__global__ __launch_bounds__(BS) void k_vsplit_fallback(int n, const double *x, double *slices,
+                                                        int64_t lds, int rho, int shift, int cap,
+                                                        int reproMode, const uint16_t *J,
+                                                        const VsPlan *plan, int *info,
+                                                        const int *stopAt, int it) {
+	if (stopAt && it >= *stopAt) return;
+	if (!plan->bad) return;
+	__shared__ double red[BS / 64];
+	double mx = 0.0;
+	for (int64_t i = threadIdx.x; i < n; i += BS) mx = std_max(mx, fabs(x[i]));
+	mx = block_max_d<BS>(mx, red);
+	int s;
+	bool zero = false;
+	for (s = 0; s < cap; s++) {
+		const double sigma = split_sigma(rho, tau_of(mx, J), shift);
+		const double *src = (s == 0) ? x : slices + (int64_t)s * lds;
+		double lm = 0.0;
+		for (int64_t i = threadIdx.x; i < n; i += BS) {
+			const double v = src[i];
+			const double a = (v + sigma) - sigma;
+			const double rr = v - a;
+			slices[(int64_t)s * lds + i] = a;
+			slices[(int64_t)(s + 1) * lds + i] = rr;
+			lm = std_max(lm, fabs(rr));
+		}
+		mx = block_max_d<BS>(lm, red);
+		if (mx == 0.0) {
+			zero = true;
+			break;
+		}
+	}
+	const int last = zero ? s : cap - 1;
+	int count;
+	if (cap <= 1)
+		count = 1;
+	else if (zero)
+		count = (s < 1 ? 1 : s) + 1;
+	else
+		count = cap;
+	if (reproMode == 0) count++;
+	for (int64_t i = threadIdx.x; i < n; i += BS) {
+		if (zero) slices[(int64_t)(last + 1) * lds + i] = 0.0;
+		for (int q = last + 2; q < count; q++) slices[(int64_t)q * lds + i] = 0.0;
+	}
+	if (threadIdx.x == 0) {
 		info[0] = count;
 		info[1] = zero ? s : cap;
 	}
 }
 
-// zero slices [z+2, count) of a vector (the reference splits the zero remainder into them)
-__global__ void k_zero_vec_slots(int n, double *slices, int64_t lds, const int *info) {
-	const int count = info[0], z = info[1];
-	for (int s = z + 2 + blockIdx.y; s < count; s += gridDim.y)
-		for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-		     i += (int64_t)gridDim.x * blockDim.x)
-			slices[(int64_t)s * lds + i] = 0.0;
-}
+size_t vsplit_scratch_bytes() { return 8ull * kVsK + 8ull * kVsBlocks + 64; }
 
-int vec_split_grid() {
-	static int g = 0;
-	if (g == 0) {
-		int dev = 0, ncu = 0, per = 0;
-		(void)hipGetDevice(&dev);
-		(void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-		(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_split_vec_coop<256>, 256, 0);
-		if (per > 2) per = 2;
-		g = ncu * (per > 0 ? per : 1);
-		if (g < 1) g = 1;
+// Splits x (length n) into slices at stride lds (cap + 2 slots); the count goes to info[0] on the
+// device.  scratch: vsplit_scratch_bytes(), zeroed once (the kernels leave it zero again).
+hipError_t launch_vsplit(hipStream_t st, int n, const double *x, double *slices, int64_t lds,
+                         int rho, int shift, int cap, int reproMode, const uint16_t *J,
+                         unsigned long long *maxv, int *info, void *scratch, VsPlan *plan,
+                         const int *stopAt, int it) {
+	unsigned long long *Fg = (unsigned long long *)scratch;
+	int2 *range = (int2 *)(Fg + kVsK);
+	unsigned *ctr = (unsigned *)(range + kVsBlocks);
+	int rhoEff = rho;
+	if (shift > 1) {  // sigma * 2^j acts as rho + j
+		if (shift & (shift - 1)) rhoEff = -1000;  // not a power of two: the fallback splits
+		else rhoEff = rho + __builtin_ctz((unsigned)shift);
 	}
-	return g;
-}
-
-// returns hipSuccess or the launch error; grid is clamped to the vector size
-hipError_t launch_split_vec_coop(hipStream_t st, int n, const double *x, double *rem,
-                                 double *slices, int64_t lds, int rho, int shift, int cap,
-                                 int reproMode, const uint16_t *J, unsigned long long *maxv,
-                                 int *info) {
-	int grid = vec_split_grid();
-	const int need = (n + 255) / 256;
-	if (need < grid) grid = need < 1 ? 1 : need;
-	void *args[] = {&n, (void *)&x, &rem, &slices, &lds, &rho, &shift, &cap, &reproMode,
-	                (void *)&J, &maxv, &info};
-	hipError_t e = hipLaunchCooperativeKernel((const void *)k_split_vec_coop<256>, dim3(grid),
-	                                          dim3(256), args, 0, st);
-	if (e != hipSuccess) return e;
-	hipLaunchKernelGGL(k_zero_vec_slots, dim3((n + 255) / 256 < 256 ? (n + 255) / 256 : 256, 4),
-	                   dim3(256), 0, st, n, slices, lds, (const int *)info);
+	int grid = (int)std::min<int64_t>(kVsBlocks, ((int64_t)n + kVsBS - 1) / kVsBS);
+	if (grid < 1) grid = 1;
+	hipLaunchKernelGGL(k_vsplit_scan<kVsBS>, dim3(grid), dim3(kVsBS), 0, st, n, x, Fg, range, ctr,
+	                   rhoEff, cap, reproMode, J, plan, info, maxv, stopAt, it);
+	int gw = (int)std::min<int64_t>(2048, ((int64_t)n + 255) / 256);
+	if (gw < 1) gw = 1;
+	hipLaunchKernelGGL(k_vsplit_write, dim3(gw), dim3(256), 0, st, n, x, slices, lds,
+	                   (const VsPlan *)plan, stopAt, it);
+	hipLaunchKernelGGL(k_vsplit_fallback<1024>, dim3(1), dim3(1024), 0, st, n, x, slices, lds, rho,
+	                   shift, cap, reproMode, J, (const VsPlan *)plan, info, stopAt, it);
 	return hipGetLastError();
 }
 
@@ -283,24 +493,29 @@ void launch_dot2_slices(hipStream_t st, int n, const double *xs, int64_t ldx, in
 	                   (const dd *)part, vals);
 }
 
-// DOT / NRM2 summation of the nA x nB slice dots (sumOrder 2, src/cpu/ozblas_gemm.cpp:281-288),
-// one thread; C[0] = fma(alpha, t, beta * C[0])
-__global__ void k_dot_sum(int nvals, double *vals, double alpha, double beta, double *C, int nA,
-                          int nB, int maxlevel, int fsum, const uint16_t *J) {
-	if (threadIdx.x != 0 || blockIdx.x != 0) return;
-	double t;
+// DOT / NRM2 summation of the nA x nB slice dots (sumOrder 2, src/cpu/ozblas_gemm.cpp:281-288):
+// NearSum of all of them, or Fsum in (level, ia) order; vals is consumed (NearSum works in place)
+__device__ inline double dot_pairs_sum(int nvals, double *vals, int nA, int nB, int maxlevel,
+                                       int fsum, const uint16_t *J) {
 	if (fsum) {
-		t = 0.;
+		double t = 0.;
 		for (int ik = 0; ik <= maxlevel; ik++)
 			for (int ia = 0; ia < nA; ia++) {
 				const int ib = ik - ia;
 				if (ib < 0 || ib >= nB) continue;
 				t = t + vals[nA * ib + ia];
 			}
-	} else {
-		StridedVec v{vals, 1};
-		t = nearsum(nvals, v, J);
+		return t;
 	}
+	StridedVec v{vals, 1};
+	return nearsum(nvals, v, J);
+}
+
+// one thread; C[0] = fma(alpha, t, beta * C[0])
+__global__ void k_dot_sum(int nvals, double *vals, double alpha, double beta, double *C, int nA,
+                          int nB, int maxlevel, int fsum, const uint16_t *J) {
+	if (threadIdx.x != 0 || blockIdx.x != 0) return;
+	const double t = dot_pairs_sum(nvals, vals, nA, nB, maxlevel, fsum, J);
 	C[0] = fma(alpha, t, beta * C[0]);
 }
 
@@ -319,13 +534,10 @@ void launch_nearsum_small(hipStream_t st, int nvals, double *vals, double alpha,
 // y = fma(alpha, t, beta * y).
 // ---------------------------------------------------------------------------------------------
 template <int NAM, int NBM>
-__global__ __launch_bounds__(256) void k_csrmm_sum(int m, const double *As, int64_t ldas, int nA,
-                                                   const int *colind, const int *rowptr,
-                                                   const double *Xs, int64_t ldxs, int nB,
-                                                   int maxlevel, int fsum, double alpha,
-                                                   double beta, double *y, const uint16_t *J) {
-	const int r = blockIdx.x * blockDim.x + threadIdx.x;
-	if (r >= m) return;
+__device__ __forceinline__ double csrmm_row_sum(int r, const double *As, int64_t ldas, int nA,
+                                                const int *colind, const int *rowptr,
+                                                const double *Xs, int64_t ldxs, int nB,
+                                                int maxlevel, int fsum, const uint16_t *J) {
 	const int b = rowptr[r], e = rowptr[r + 1];
 	RegVec<NAM * NBM> v;
 #pragma unroll
@@ -373,6 +585,19 @@ __global__ __launch_bounds__(256) void k_csrmm_sum(int m, const double *As, int6
 			}
 		t = nearsum(cnt, u, J);
 	}
+	return t;
+}
+
+template <int NAM, int NBM>
+__global__ __launch_bounds__(256) void k_csrmm_sum(int m, const double *As, int64_t ldas, int nA,
+                                                   const int *colind, const int *rowptr,
+                                                   const double *Xs, int64_t ldxs, int nB,
+                                                   int maxlevel, int fsum, double alpha,
+                                                   double beta, double *y, const uint16_t *J) {
+	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= m) return;
+	const double t = csrmm_row_sum<NAM, NBM>(r, As, ldas, nA, colind, rowptr, Xs, ldxs, nB, maxlevel,
+	                                         fsum, J);
 	y[r] = fma(alpha, t, beta * y[r]);
 }
 
@@ -538,6 +763,274 @@ void launch_axpy_host(hipStream_t st, int n, double alpha, const double *x, doub
 
 void launch_scal_axpy(hipStream_t st, int n, double beta, const double *r, double *p) {
 	hipLaunchKernelGGL(k_scal_axpy, dim3(grid1(n)), dim3(256), 0, st, n, beta, r, p);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Device-resident CG (ozblasRcg, src/cpu/ozblas_cg.cpp:24-128).  Slice counts and the scalar
+// recurrences stay in device memory (CgDev), so the host enqueues every iteration without waiting.
+// A kernel of iteration `it` does nothing when it >= S->stopAt: the residual test of iteration it
+// sets stopAt = it + 1 (the p update of that iteration still runs: the reference breaks after it),
+// and a CSR product whose vector has more slices than its register bound sets stopAt = it and
+// overflow = 1, after which the host repeats the solve on the synchronous path.
+// ---------------------------------------------------------------------------------------------
+__device__ inline bool cg_skip(const CgDev *S, int it) { return S != nullptr && it >= S->stopAt; }
+
+// q = A p with the pre-split matrix (nA <= NAM) and the device slice count of p; alpha = 1, beta = 0
+// as ozblasRcsrmv(..., 1., ..., 0., q) at src/cpu/ozblas_cg.cpp:82
+template <int NAM, int NBM>
+__global__ __launch_bounds__(256) void k_csrmm_sum_dev(int m, const double *As, int64_t ldas, int nA,
+                                                       const int *colind, const int *rowptr,
+                                                       const double *Xs, int64_t ldxs,
+                                                       const int *infoX, int fsum, double *y,
+                                                       const uint16_t *J, CgDev *S, int it) {
+	if (cg_skip(S, it)) return;
+	const int nB = infoX[0];
+	if (nB > S->maxVecSlices) {
+		if (blockIdx.x == 0 && threadIdx.x == 0) {
+			S->overflow = 1;
+			S->stopAt = it;
+		}
+		return;
+	}
+	if (nB > NBM) return;  // k_csrmm_big_dev computes this product
+	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= m) return;
+	const int maxlevel = (nA - 1) + (nB - 1);
+	const double t = csrmm_row_sum<NAM, NBM>(r, As, ldas, nA, colind, rowptr, Xs, ldxs, nB, maxlevel,
+	                                         fsum, J);
+	y[r] = fma(1.0, t, 0.0 * y[r]);
+}
+
+// The same product when p has more slices than the register kernel holds: each pair sum runs over
+// the row's nonzeros in order (exact) into Cs[q * ldc + r], then NearSum over the row's pairs, or
+// Fsum in (level, ia) order computed on the fly.
+__global__ __launch_bounds__(256) void k_csrmm_big_dev(int m, const double *As, int64_t ldas, int nA,
+                                                       const int *colind, const int *rowptr,
+                                                       const double *Xs, int64_t ldxs,
+                                                       const int *infoX, int nbReg, int fsum,
+                                                       double *y, double *Cs, int64_t ldc,
+                                                       const uint16_t *J, const CgDev *S, int it) {
+	if (cg_skip(S, it)) return;
+	const int nB = infoX[0];
+	if (nB <= nbReg || nB > S->maxVecSlices) return;
+	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= m) return;
+	const int b = rowptr[r], e = rowptr[r + 1];
+	auto pair = [&](int ia, int ib) {
+		double acc = 0.0;
+		for (int j = b; j < e; j++)
+			acc = fma(As[(int64_t)ia * ldas + j], Xs[(int64_t)ib * ldxs + colind[j]], acc);
+		return acc;
+	};
+	const int maxlevel = (nA - 1) + (nB - 1);
+	double t;
+	if (fsum) {
+		t = 0.;
+		for (int ik = 0; ik <= maxlevel; ik++)
+			for (int ia = 0; ia < nA; ia++) {
+				const int ib = ik - ia;
+				if (ib >= 0 && ib < nB) t = t + pair(ia, ib);
+			}
+	} else {
+		int cnt = 0;
+		for (int ia = 0; ia < nA; ia++)
+			for (int ib = 0; ib < nB; ib++) Cs[(int64_t)(cnt++) * ldc + r] = pair(ia, ib);
+		StridedVec v{Cs + r, ldc};
+		t = nearsum(cnt, v, J);
+	}
+	y[r] = fma(1.0, t, 0.0 * y[r]);
+}
+
+// Completion of a device DOT by the last workgroup to finish (the caller has added its exact pair
+// partials to vals): sums the nA x nB pair dots (sumOrder 2) and applies `mode`
+//   0: C[0] = fma(1, t, 0 * C[0])                        (ozblasRdot's GEMM epilogue)
+//   1: tmp = t, alpha = dold / tmp                       (src/cpu/ozblas_cg.cpp:83-84)
+//   2: dnew = t, resi = sqrt(dnew), beta = dnew / dold,  (src/cpu/ozblas_cg.cpp:91-94,100-112)
+//      dold = dnew, history and the residual test
+// then clears vals, the counter and the split maxima it consumed for the next use.
+__device__ void dots_finish(int nA, int nB, double *vals, unsigned *ctr, unsigned long long *maxvX,
+                            unsigned long long *maxvY, int nmax, const uint16_t *J, int fsum,
+                            CgDev *S, int it, int mode, double *C) {
+	__shared__ int isLast;
+	__threadfence();
+	if (threadIdx.x == 0) isLast = (atomicAdd(ctr, 1u) == gridDim.x - 1) ? 1 : 0;
+	__syncthreads();
+	if (!isLast || threadIdx.x != 0) return;
+	__threadfence();
+	const int npairs = nA * nB;  // vals: plain loads, ordered after the other workgroups by the fence
+	const int maxlevel = (nA - 1) + (nB - 1);  // DOT forces fastMode = 0 (src/cpu/ozblas_gemm.cpp:25-28)
+	const double t = dot_pairs_sum(npairs, vals, nA, nB, maxlevel, fsum, J);
+	const double r = fma(1.0, t, 0.0 * 0.0);
+	if (mode == 0) {
+		C[0] = fma(1.0, t, 0.0 * C[0]);
+	} else if (mode == 1) {
+		S->tmp = r;
+		S->alpha = S->dold / r;
+	} else {
+		const double dnew = r;
+		const double resi = sqrt(dnew);
+		S->dnew = dnew;
+		S->resi = resi;
+		S->beta = dnew / S->dold;
+		S->dold = dnew;
+		S->numiter = it + 1;
+		if (S->verbose > 0) {
+			if (resi / S->nrmb < S->tol || S->numiter % S->verbose == 0) {
+				if (S->hist) S->hist[S->vcnt] = resi / S->nrmb;
+				S->vcnt = S->vcnt + 1;
+			}
+		}
+		if (resi / S->nrmb < S->tol) S->stopAt = it + 1;
+	}
+	for (int p = 0; p < npairs; p++) vals[p] = 0.0;
+	for (int q = 0; q < nmax; q++) {
+		maxvX[q] = 0ull;
+		maxvY[q] = 0ull;
+	}
+	*ctr = 0u;
+}
+
+// DOT of two split vectors with device slice counts, slice counts up to NAM x NBM: every element's
+// slices are read once and all pair products accumulate in registers; the exact pair partials of
+// a workgroup are added atomically (integer multiples of the product unit below 2^53: any order
+// is exact).  Larger counts are left to k_dots_dev.
+template <int BS, int NAM, int NBM>
+__global__ __launch_bounds__(BS) void k_dots_reg(int n, const double *xs, int64_t ldx,
+                                                 const int *infoX, const double *ys, int64_t ldy,
+                                                 const int *infoY, double *vals, unsigned *ctr,
+                                                 unsigned long long *maxvX,
+                                                 unsigned long long *maxvY, int nmax,
+                                                 const uint16_t *J, int fsum, CgDev *S, int it,
+                                                 int mode, double *C) {
+	if (cg_skip(S, it)) return;
+	const int nA = infoX[0], nB = infoY[0];
+	if (nA > NAM || nB > NBM) return;
+	__shared__ double red[BS / 64][NAM * NBM];
+	double acc[NAM * NBM];
+#pragma unroll
+	for (int q = 0; q < NAM * NBM; q++) acc[q] = 0.0;
+	for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+		double xa[NAM], yb[NBM];
+#pragma unroll
+		for (int a = 0; a < NAM; a++) xa[a] = (a < nA) ? xs[(int64_t)a * ldx + i] : 0.0;
+#pragma unroll
+		for (int b = 0; b < NBM; b++) yb[b] = (b < nB) ? ys[(int64_t)b * ldy + i] : 0.0;
+#pragma unroll
+		for (int a = 0; a < NAM; a++)
+#pragma unroll
+			for (int b = 0; b < NBM; b++) acc[a * NBM + b] = fma(xa[a], yb[b], acc[a * NBM + b]);
+	}
+	const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+#pragma unroll
+	for (int q = 0; q < NAM * NBM; q++) {
+		double v = acc[q];
+		for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+		if (l == 0) red[w][q] = v;
+	}
+	__syncthreads();
+	for (int q = threadIdx.x; q < NAM * NBM; q += BS) {
+		const int a = q / NBM, b = q % NBM;
+		if (a < nA && b < nB) {
+			double v = red[0][q];
+			for (int k = 1; k < BS / 64; k++) v += red[k][q];
+			if (v != 0.0) atomicAdd(&vals[a + nA * b], v);
+		}
+	}
+	dots_finish(nA, nB, vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C);
+}
+
+// The same for any slice counts (pairs in chunks of kDotChunk, slices re-read per chunk); does
+// nothing when k_dots_reg<.., NAM, NBM> takes the counts.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_dots_dev(int n, const double *xs, int64_t ldx,
+                                                 const int *infoX, const double *ys, int64_t ldy,
+                                                 const int *infoY, double *vals, unsigned *ctr,
+                                                 unsigned long long *maxvX,
+                                                 unsigned long long *maxvY, int nmax,
+                                                 const uint16_t *J, int fsum, CgDev *S, int it,
+                                                 int mode, double *C, int regA, int regB) {
+	if (cg_skip(S, it)) return;
+	__shared__ double red[BS / 64];
+	const int nA = infoX[0], nB = infoY[0];
+	if (nA <= regA && nB <= regB) return;
+	const int npairs = nA * nB;
+	for (int p0 = 0; p0 < npairs; p0 += kDotChunk) {
+		double acc[kDotChunk];
+#pragma unroll
+		for (int q = 0; q < kDotChunk; q++) acc[q] = 0.0;
+		for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+#pragma unroll
+			for (int q = 0; q < kDotChunk; q++) {
+				const int p = p0 + q;
+				if (p < npairs) {
+					const int ia = p % nA, ib = p / nA;
+					acc[q] = fma(xs[(int64_t)ia * ldx + i], ys[(int64_t)ib * ldy + i], acc[q]);
+				}
+			}
+		}
+#pragma unroll
+		for (int q = 0; q < kDotChunk; q++) {
+			const double v = block_sum_d<BS>(acc[q], red);
+			if (threadIdx.x == 0 && p0 + q < npairs && v != 0.0) atomicAdd(&vals[p0 + q], v);
+		}
+	}
+	dots_finish(nA, nB, vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C);
+}
+
+// x = fma(alpha, p, x); r = fma(-alpha, q, r)   (src/cpu/ozblas_cg.cpp:87-88, ozblas_axpy.cpp)
+__global__ void k_axpy2_dev(int n, const CgDev *S, int it, const double *P, double *X,
+                            const double *Q, double *R) {
+	if (cg_skip(S, it)) return;
+	const double a = S->alpha;
+	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+	     i += (int64_t)gridDim.x * blockDim.x) {
+		X[i] = fma(a, P[i], X[i]);
+		R[i] = fma(-a, Q[i], R[i]);
+	}
+}
+
+// p = beta * p, then p = fma(1, r, p)   (src/cpu/ozblas_cg.cpp:97-98)
+__global__ void k_scal_axpy_dev(int n, const CgDev *S, int it, const double *R, double *P) {
+	if (cg_skip(S, it)) return;
+	const double b = S->beta;
+	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+	     i += (int64_t)gridDim.x * blockDim.x)
+		P[i] = fma(1.0, R[i], b * P[i]);
+}
+
+void launch_csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
+                          const int *colind, const int *rowptr, const double *Xs, int64_t ldxs,
+                          const int *infoX, int fsum, double *y, double *Cs, int64_t ldc,
+                          const uint16_t *J, CgDev *S, int it) {
+	constexpr int kNbReg = 8;
+	hipLaunchKernelGGL((k_csrmm_sum_dev<4, kNbReg>), dim3((m + 255) / 256), dim3(256), 0, st, m, As,
+	                   ldas, nA, colind, rowptr, Xs, ldxs, infoX, fsum, y, J, S, it);
+	hipLaunchKernelGGL(k_csrmm_big_dev, dim3((m + 255) / 256), dim3(256), 0, st, m, As, ldas, nA,
+	                   colind, rowptr, Xs, ldxs, infoX, kNbReg, fsum, y, Cs, ldc, J, S, it);
+}
+
+void launch_dots_dev(hipStream_t st, int n, const double *xs, int64_t ldx, const int *infoX,
+                     const double *ys, int64_t ldy, const int *infoY, double *vals, unsigned *ctr,
+                     unsigned long long *maxvX, unsigned long long *maxvY, int nmax,
+                     const uint16_t *J, int fsum, CgDev *S, int it, int mode, double *C) {
+	constexpr int kRA = 8, kRB = 8;
+	int gx = (n + 255) / 256;
+	if (gx > 512) gx = 512;
+	if (gx < 1) gx = 1;
+	hipLaunchKernelGGL((k_dots_reg<256, kRA, kRB>), dim3(gx), dim3(256), 0, st, n, xs, ldx, infoX, ys,
+	                   ldy, infoY, vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C);
+	hipLaunchKernelGGL(k_dots_dev<256>, dim3(gx), dim3(256), 0, st, n, xs, ldx, infoX, ys, ldy, infoY,
+	                   vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, kRA, kRB);
+}
+
+void launch_axpy2_dev(hipStream_t st, int n, const CgDev *S, int it, const double *P, double *X,
+                      const double *Q, double *R) {
+	hipLaunchKernelGGL(k_axpy2_dev, dim3(grid1(n)), dim3(256), 0, st, n, S, it, P, X, Q, R);
+}
+
+void launch_scal_axpy_dev(hipStream_t st, int n, const CgDev *S, int it, const double *R,
+                          double *P) {
+	hipLaunchKernelGGL(k_scal_axpy_dev, dim3(grid1(n)), dim3(256), 0, st, n, S, it, R, P);
 }
 
 }  // namespace oz

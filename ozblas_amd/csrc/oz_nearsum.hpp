@@ -11,16 +11,30 @@
 namespace oz {
 
 struct StridedVec {
+	static constexpr int kMax = 0;  // length known only at run time
 	double *p;
 	int64_t ld;
 	__device__ double &operator[](int i) const { return p[(int64_t)i * ld]; }
 };
 
 template <int N>
-struct RegVec {  // fully unrolled register array (indices are compile-time after unrolling)
+struct RegVec {  // register array: loops over it run to N with a guard, so every index is static
+	static constexpr int kMax = N;
 	double v[N];
 	__device__ double &operator[](int i) { return v[i]; }
 };
+
+// for (i = i0; i < n; i++) f(i), unrolled to the static bound of V when it has one
+template <class V, class F>
+__device__ __forceinline__ void ns_for(int i0, int n, F &&f) {
+	if constexpr (V::kMax > 0) {
+#pragma unroll
+		for (int i = 0; i < V::kMax; i++)
+			if (i >= i0 && i < n) f(i);
+	} else {
+		for (int i = i0; i < n; i++) f(i);
+	}
+}
 
 __device__ inline double mac_max(double a, double b) { return (a > b) ? a : b; }  // MAX macro
 
@@ -38,7 +52,7 @@ __device__ inline void ns_transform(int n, V &vec, double rho, double &tau1, dou
 		return;
 	}
 	double mu = fabs(vec[0]);
-	for (int i = 1; i < n; i++) mu = mac_max(mu, fabs(vec[i]));
+	ns_for<V>(1, n, [&](int i) { mu = mac_max(mu, fabs(vec[i])); });
 	if (mu == 0.) {
 		tau1 = rho;
 		tau2 = 0.;
@@ -54,11 +68,11 @@ __device__ inline void ns_transform(int n, V &vec, double rho, double &tau1, dou
 	const double shrink = scalbn_clamped(1., m - 53);
 	while (true) {
 		double tau = 0.;
-		for (int i = 0; i < n; i++) {
+		ns_for<V>(0, n, [&](int i) {
 			const double tmp = (sigma + vec[i]) - sigma;
 			vec[i] = vec[i] - tmp;
 			tau = tau + tmp;
-		}
+		});
 		tau1 = t + tau;
 		if ((sigma <= tiny) || (fabs(tau1) >= fac * sigma)) {
 			tau2 = tau - (tau1 - t);
@@ -75,7 +89,7 @@ __device__ inline void ns_transform_k(int n, V &vec, double rho, double &res, do
 	// ozblasTransformK (src/cpu/ozblas_sum.cpp:58-73)
 	double tmp = 0., tau1, tau2;
 	ns_transform(n, vec, rho, tau1, tau2, J);
-	for (int i = 0; i < n; i++) tmp = tmp + vec[i];
+	ns_for<V>(0, n, [&](int i) { tmp = tmp + vec[i]; });
 	res = tau1 + (tau2 + tmp);
 	r = tau2 - (res - tau1);
 }

@@ -85,6 +85,9 @@ void launch_global_nearsum(hipStream_t st, int m, int n, int ic, double *Cs, int
 	if (ic <= 16)
 		hipLaunchKernelGGL(k_global_nearsum_reg<16>, dim3(g), dim3(256), 0, st, m, n, ic, Cs, llsc,
 		                   ldsc, C, ldc, alpha, beta, J);
+	else if (ic <= 32)
+		hipLaunchKernelGGL(k_global_nearsum_reg<32>, dim3(g), dim3(256), 0, st, m, n, ic, Cs, llsc,
+		                   ldsc, C, ldc, alpha, beta, J);
 	else if (ic <= 64)
 		hipLaunchKernelGGL(k_global_nearsum_reg<64>, dim3(g), dim3(256), 0, st, m, n, ic, Cs, llsc,
 		                   ldsc, C, ldc, alpha, beta, J);

@@ -33,6 +33,44 @@ def test_dot_nrm2_bitwise(oz, oracle, n, sumMode, reproMode):
     assert H.dnrm2(n, dx) == h.nrm2(n, x)
 
 
+def _split_edge_vectors():
+    rng = np.random.default_rng(31)
+    yield "uniform", rng.uniform(-1, 1, 5000)
+    yield "wide", rng.standard_normal(3000) * np.exp(8 * rng.standard_normal(3000))
+    yield "few-bits", np.ldexp(rng.integers(-2**20, 2**20, 2000).astype(float), -30)
+    v = np.ldexp(1.0, rng.integers(-40, 40, 1500)).astype(float)
+    v[::3] *= -1
+    yield "powers-of-two", v
+    w = rng.uniform(-1, 1, 1200)
+    w[::7] = np.nextafter(np.ldexp(1.0, -3), 1.0)
+    w[1::11] = 0.0
+    yield "just-above-2^e", w
+    yield "ties", np.ldexp(rng.integers(0, 2**52, 700).astype(float) + 0.5, -40)
+    yield "zeros", np.zeros(300)
+    yield "two-ranges", np.concatenate([rng.uniform(-1, 1, 500), rng.uniform(-1, 1, 500) * 1e-200])
+    # outside the residue model: the one-workgroup reference loop splits these
+    yield "huge", np.array([1.5 * 2.0 ** 1022, -3.0, 1.0, 2.0 ** 1000])
+    yield "subnormal", np.array([5e-324, -1e-310, 3e-320, 0.0])
+
+
+@pytest.mark.parametrize("sumMode", [0, 1])
+def test_dot_split_edge_cases_bitwise(oz, oracle, sumMode):
+    """DOT through the vector split on vectors with ties, powers of two, exact-early splits, zeros,
+    and inputs handled by the fallback loop: result and slice counts equal the oracle's."""
+    for name, x in _split_edge_vectors():
+        n = len(x)
+        y = np.random.default_rng(n).uniform(-1, 1, n)
+        for a, b in ((x, x), (x, y)):
+            h = oracle.Handle(1 << 30, sumMode=sumMode)
+            want = h.dot(n, a, b)
+            H = oz.Handle(1 << 28, sumMode=sumMode)
+            got = H.ddot(n, dev(a), dev(b))
+            say(f"ddot edge {name} n={n} sumMode={sumMode}")
+            same = np.float64(got).tobytes() == np.float64(want).tobytes()
+            assert same or (np.isnan(got) and np.isnan(want)), (name, got, want)
+            assert (H.nSplitA, H.nSplitB) == (h.nSplitA, h.nSplitB), name
+
+
 def test_dot_is_correctly_rounded_under_nearsum(oz, oracle):
     rng = np.random.default_rng(8)
     for n in (3, 999, 40000):
@@ -97,6 +135,52 @@ def test_cg_residual_history_bitwise(oz, oracle, sumMode):
     dx = dev(np.zeros(n))
     got = H.dcg(n, nnz, dev(val), dev(colind), dev(rowptr), dev(b), dx, iters, 0.0)
     assert len(got) == iters + 1
+    assert bits_equal(got, want), first_diff(got, want)
+    assert bits_equal(host(dx), xw), first_diff(host(dx), xw)
+
+
+@pytest.mark.parametrize("sumMode", [0, 1])
+@pytest.mark.parametrize("path", ["device", "host", "overflow"])
+def test_cg_paths_bitwise(oz, oracle, sumMode, path):
+    """The device-resident iterations, the synchronous ones, and the fallback from the device path
+    (forced by a slice bound of 1, so the first CSR product gives up) give the same bits."""
+    g, iters = 12, 60
+    rowptr, colind, val, n, nnz = _laplace(oracle, g)
+    oracle.srand48(5)
+    b = np.array([2.0 * oracle.drand48() - 1.0 for _ in range(n)])
+    say(f"dcg paths n={n} iters={iters} sumMode={sumMode} path={path}")
+    h = oracle.Handle(1 << 30, sumMode=sumMode)
+    xw = np.zeros(n)
+    want = h.cg(n, nnz, val, colind, rowptr, b, xw, iters, 0.0)
+    bound = {"device": 0, "host": -1, "overflow": 1}[path]
+    H = oz.Handle(1 << 28, sumMode=sumMode, cgDeviceSlices=bound)
+    dx = dev(np.zeros(n))
+    got = H.dcg(n, nnz, dev(val), dev(colind), dev(rowptr), dev(b), dx, iters, 0.0)
+    assert H.cgDevicePath == {"device": 1, "host": 0, "overflow": 2}[path]
+    assert len(got) == iters + 1
+    assert bits_equal(got, want), first_diff(got, want)
+    assert bits_equal(host(dx), xw), first_diff(host(dx), xw)
+
+
+@pytest.mark.parametrize("path", ["device", "host"])
+def test_cg_tolerance_stop_bitwise(oz, oracle, path):
+    """A tolerance reached mid-way stops both paths after the same iteration (the p update of that
+    iteration still runs, as in the reference), with the same history and x."""
+    g, iters = 10, 80
+    rowptr, colind, val, n, nnz = _laplace(oracle, g)
+    oracle.srand48(7)
+    b = np.array([2.0 * oracle.drand48() - 1.0 for _ in range(n)])
+    full = oracle.Handle(1 << 30, sumMode=1).cg(n, nnz, val, colind, rowptr, b, np.zeros(n), iters, 0.0)
+    tol = float(full[25]) * (1 + 2.0 ** -40)
+    h = oracle.Handle(1 << 30, sumMode=1)
+    xw = np.zeros(n)
+    want = h.cg(n, nnz, val, colind, rowptr, b, xw, iters, tol)
+    assert 1 < len(want) - 1 < iters
+    H = oz.Handle(1 << 28, sumMode=1, cgDeviceSlices=0 if path == "device" else -1)
+    dx = dev(np.zeros(n))
+    got = H.dcg(n, nnz, dev(val), dev(colind), dev(rowptr), dev(b), dx, iters, tol)
+    assert H.cgDevicePath == (1 if path == "device" else 0)
+    assert len(got) == len(want)
     assert bits_equal(got, want), first_diff(got, want)
     assert bits_equal(host(dx), xw), first_diff(host(dx), xw)
 

@@ -1,0 +1,142 @@
+"""The level model behind the barrier-free vector split kernels (k_vsplit_scan / k_vsplit_write).
+
+The model: with sigma_s = 0.75 * 2^(rho+tau_s) (rho >= 3), the remainder the reference split
+(src/cpu/ozblas_split.cpp:75-104,254-289) keeps after slice s is the centred residue
+cmod(x, u_s) = x - u_s * rne(x / u_s) of the input itself, u_s = 2^(rho+tau_s-53); so the level
+maxima are values of F(k) = max_i |cmod(x_i, 2^k)|, M_{s+1} = F(rho + tau(M_s) - 53), and
+slice s = cmod(x, u_{s-1}) - cmod(x, u_s).  This restates the model in numpy and checks every
+slice bit for bit, and the slice count, against the oracle's restatement of the reference loop.
+"""
+import numpy as np
+import pytest
+
+
+def cmod(x, k):
+    n = np.rint(np.ldexp(x, -k))  # x * 2^-k is exact for the magnitudes used here
+    return x - np.ldexp(n, k)
+
+
+def model_split(oracle, x, rho, cap, reproMode):
+    M = float(np.max(np.abs(x))) if len(x) else 0.0
+    ks = []
+    zero = False
+    s = 0
+    for s in range(cap):
+        tau = oracle.tau(M)
+        k = rho + tau - 53
+        ks.append(k)
+        M = float(np.max(np.abs(cmod(x, k)))) if len(x) else 0.0
+        if M == 0.0:
+            zero = True
+            break
+    else:
+        s = cap
+    if cap <= 1:
+        count = 1
+    elif zero:
+        count = max(s, 1) + 1
+    else:
+        count = cap
+    if reproMode == 0:
+        count += 1
+    last = s if zero else cap - 1
+    slots = []
+    r = x.copy()
+    for q in range(last + 1):
+        c = cmod(x, ks[q])
+        slots.append(r - c)
+        r = c
+    slots.append(np.zeros_like(x) if zero else r)
+    while len(slots) < count:
+        slots.append(np.zeros_like(x))
+    return count, slots
+
+
+def oracle_split(oracle, x, nSplitMax, reproMode):
+    L = oracle.lib()
+    n = len(x)
+    h = oracle.Handle(1 << 24, nSplitMax=nSplitMax, reproMode=reproMode)
+    cap = 31
+    split = np.zeros(n * (cap + 2))
+    out = np.zeros(n)
+    mx = np.zeros(max(n, 1))
+    xin = np.ascontiguousarray(x, dtype=np.float64)
+    cnt = L.ozo_split(h._h, n, 1, xin, n, out, n, split, n, mx)
+    return cnt, [split[s * n:(s + 1) * n] for s in range(cnt)]
+
+
+def vectors():
+    rng = np.random.default_rng(11)
+    yield rng.uniform(-1, 1, 1000)
+    yield rng.standard_normal(777) * np.exp(8 * rng.standard_normal(777))
+    yield np.ldexp(rng.integers(-2**20, 2**20, 500).astype(float), -30)  # few bits: exact early
+    v = np.ldexp(1.0, rng.integers(-40, 40, 300)).astype(float)             # powers of two
+    v[::3] *= -1
+    yield v
+    w = rng.uniform(-1, 1, 400)
+    w[::7] = np.nextafter(np.ldexp(1.0, -3), 1.0)                             # just above 2^e
+    w[1::11] = 0.0
+    yield w
+    t = np.ldexp(rng.integers(0, 2**52, 256).astype(float) + 0.5, -40)        # ties at a level
+    yield t
+    yield np.zeros(64)
+    yield rng.uniform(-1, 1, 1) * 1e-300
+    yield np.concatenate([rng.uniform(-1, 1, 100), rng.uniform(-1, 1, 100) * 1e-200])
+
+
+@pytest.mark.parametrize("reproMode,nSplitMax", [(1, 0), (0, 0), (1, 3), (1, 1), (0, 2)])
+def test_level_model_matches_reference_loop(oracle, reproMode, nSplitMax):
+    for x in vectors():
+        n = len(x)
+        rho = oracle.rho(n, 0)
+        cap = (nSplitMax if nSplitMax > 0 else 30) - (1 if reproMode == 0 else 0)
+        want_cnt, want = oracle_split(oracle, x, nSplitMax, reproMode)
+        got_cnt, got = model_split(oracle, x, rho, cap, reproMode)
+        assert got_cnt == want_cnt, (n, got_cnt, want_cnt)
+        for s in range(want_cnt):
+            assert np.array_equal(got[s].view(np.uint64), want[s].view(np.uint64)), (n, s)
+
+
+def test_model_needs_rho_at_least_3(oracle):
+    """With rho < 3 the sum x + sigma can leave the binade of sigma (the residue model fails), which
+    is why the kernels hand such splits (Dot2 on short vectors) to the fallback loop."""
+    x = np.array([0.99, -0.75, 0.5001])
+    rho = 1
+    M = float(np.max(np.abs(x)))
+    tau = oracle.tau(M)
+    sigma = 0.75 * 2.0 ** (rho + tau)
+    ref = (x + sigma) - sigma
+    model = x - cmod(x, rho + tau - 53)
+    assert not np.array_equal(ref, model)
+
+
+def residue_key(x, k):
+    """The scan kernel's integer form: x = m 2^eb, r = m mod 2^s centred (ties to even), s = k - eb,
+    key = |r| 2^(64-s) = |cmod(x, 2^k)| 2^(64-k)."""
+    m, e = np.frexp(abs(x))
+    mi = int(np.ldexp(m, 53))
+    eb = int(e) - 53
+    while mi and mi % 2 == 0:  # the kernel uses the stored mantissa; the key is the same
+        mi //= 2
+        eb += 1
+    s = k - eb
+    if s <= 0:
+        return 0
+    r = mi & ((1 << s) - 1)
+    half = 1 << (s - 1)
+    if r > half or (r == half and (mi >> s) & 1):
+        r = (1 << s) - r
+    return r << (64 - s) if s <= 64 else None
+
+
+def test_integer_residue_key_matches_cmod():
+    rng = np.random.default_rng(4)
+    xs = list(rng.standard_normal(200) * np.exp2(rng.integers(-60, 60, 200)))
+    xs += [np.ldexp(np.float64(2**52 + 2**51 + 1), -70), 0.75, 2.0 ** -30, 3 * 2.0 ** -40]
+    for x in xs:
+        m, e = np.frexp(abs(x))
+        for k in range(int(e) - 60, int(e) + 3):
+            want = abs(float(cmod(np.array([x]), k)[0]))
+            key = residue_key(x, k)
+            got = np.ldexp(float(key), k - 64)
+            assert got == want, (x, k, got, want)
