@@ -104,6 +104,38 @@ def lib():
         L.ozo_rand.restype = C.c_int32
         L.ozo_init_mat.argtypes = [C.c_int32, C.c_int32, C.c_int32, _dp, C.c_double, C.c_int32,
                                    C.c_int32]
+        _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+        L.ozo_s_rho.restype = C.c_int32
+        L.ozo_s_rho.argtypes = [C.c_int32, C.c_int32]
+        L.ozo_s_tau.restype = C.c_int32
+        L.ozo_s_tau.argtypes = [C.c_float]
+        L.ozo_s_nearsum.restype = C.c_float
+        L.ozo_s_nearsum.argtypes = [C.c_int32, _fp, C.c_int32]
+        L.ozo_s_split.restype = C.c_int32
+        L.ozo_s_split.argtypes = [H, C.c_int32, C.c_int32, _fp, C.c_int32, _fp, C.c_int32, _fp,
+                                  C.c_int32, _fp]
+        L.ozo_s_gemm.restype = C.c_int32
+        L.ozo_s_gemm.argtypes = [H, C.c_char, C.c_char, C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                                 _fp, C.c_int32, _fp, C.c_int32, C.c_float, _fp, C.c_int32]
+        L.ozo_s_dot.restype = C.c_float
+        L.ozo_s_dot.argtypes = [H, C.c_int32, _fp, _fp]
+        L.ozo_s_nrm2.restype = C.c_float
+        L.ozo_s_nrm2.argtypes = [H, C.c_int32, _fp]
+        L.ozo_s_gemv.restype = C.c_int32
+        L.ozo_s_gemv.argtypes = [H, C.c_char, C.c_int32, C.c_int32, C.c_float, _fp, C.c_int32, _fp,
+                                 C.c_float, _fp]
+        L.ozo_s_axpy.restype = C.c_int32
+        L.ozo_s_axpy.argtypes = [H, C.c_int32, C.c_float, _fp, _fp]
+        L.ozo_s_csrmv.restype = C.c_int32
+        L.ozo_s_csrmv.argtypes = [H, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_void_p, _ip,
+                                  _ip, _fp, C.c_float, _fp]
+        L.ozo_s_csrmv_splitA.restype = C.c_void_p
+        L.ozo_s_csrmv_splitA.argtypes = [H, C.c_int32, C.c_int32, C.c_int32, _fp, _ip]
+        L.ozo_s_cg.restype = C.c_int32
+        L.ozo_s_cg.argtypes = [H, C.c_int32, C.c_int32, _fp, _ip, _ip, _fp, _fp, C.c_int32,
+                               C.c_float]
+        L.ozo_s_init_mat.argtypes = [C.c_int32, C.c_int32, C.c_int32, _fp, C.c_double, C.c_int32,
+                                     C.c_int32]
         L.ozo_laplace3d_nnz.restype = C.c_int64
         L.ozo_laplace3d_nnz.argtypes = [C.c_int32]
         L.ozo_laplace3d.argtypes = [C.c_int32, C.c_double, _ip, _ip, _dp]
@@ -184,7 +216,74 @@ class Handle:
         return hist[: self._h.contents.cg_numiter + 1]
 
 
+    # binary32 routines (the <float,float,float> instantiations, ozoracle_s.c) ---------------
+    def sgemm(self, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+        rc = lib().ozo_s_gemm(self._h, transA.encode(), transB.encode(), m, n, k, alpha, A, lda, B,
+                              ldb, beta, C, ldc)
+        if rc != 0:
+            raise RuntimeError(f"ozo_s_gemm failed ({rc})")
+        return C
+
+    def sdot(self, n, x, y):
+        return np.float32(lib().ozo_s_dot(self._h, n, x, y))
+
+    def snrm2(self, n, x):
+        return np.float32(lib().ozo_s_nrm2(self._h, n, x))
+
+    def sgemv(self, trans, m, n, alpha, A, lda, x, beta, y):
+        rc = lib().ozo_s_gemv(self._h, trans.encode(), m, n, alpha, A, lda, x, beta, y)
+        if rc != 0:
+            raise RuntimeError(f"ozo_s_gemv failed ({rc})")
+        return y
+
+    def saxpy(self, n, alpha, x, y):
+        lib().ozo_s_axpy(self._h, n, alpha, x, y)
+        return y
+
+    def scsrmv(self, m, n, nnz, alpha, val, colind, rowptr, x, beta, y):
+        ptr = val if isinstance(val, int) else val.ctypes.data
+        rc = lib().ozo_s_csrmv(self._h, m, n, nnz, alpha, ptr, colind, rowptr, x, beta, y)
+        if rc != 0:
+            raise RuntimeError(f"ozo_s_csrmv failed ({rc})")
+        return y
+
+    def scsrmv_splitA(self, m, n, nnz, val, rowptr):
+        return lib().ozo_s_csrmv_splitA(self._h, m, n, nnz, val, rowptr)
+
+    def scg(self, n, nnz, val, colind, rowptr, b, x, maxiter, tol, history=True):
+        hist = np.zeros(maxiter + 2, dtype=np.float64)
+        if history:
+            self._h.contents.verbose = 1
+            self._h.contents.cg_verbose1 = hist.ctypes.data
+        rc = lib().ozo_s_cg(self._h, n, nnz, val, colind, rowptr, b, x, maxiter, tol)
+        if rc != 0:
+            raise RuntimeError(f"ozo_s_cg failed ({rc})")
+        self._h.contents.cg_verbose1 = None
+        return hist[: self._h.contents.cg_numiter + 1].astype(np.float32)
+
+
 # building blocks -------------------------------------------------------------------------------
+def s_rho(dim, mode=0):
+    return lib().ozo_s_rho(dim, mode)
+
+
+def s_tau(x):
+    return lib().ozo_s_tau(float(np.float32(x)))
+
+
+def s_nearsum(v):
+    v = np.ascontiguousarray(v, dtype=np.float32).copy()
+    return np.float32(lib().ozo_s_nearsum(len(v), v, 1))
+
+
+def s_init_mat(m, n, ld, phi, mode, seed):
+    """mublasInitMat of the binary32 harness: column-major buffer of ld*n floats."""
+    ld = max(ld, m)
+    buf = np.zeros(ld * n, dtype=np.float32)
+    lib().ozo_s_init_mat(m, n, ld, buf, float(phi), mode, seed)
+    return buf
+
+
 def rho(dim, mode=0):
     return lib().ozo_rho(dim, mode)
 

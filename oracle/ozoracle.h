@@ -130,6 +130,31 @@ void ozo_dd_dot(int64_t n, const double *x, int64_t incx, const double *y, int64
                 double *hi, double *lo);
 double ozo_exact_sum_rn(int64_t n, const double *v);
 
+/* binary32 restatement (ozoracle_s.c): the <float,float,float> instantiations */
+int32_t ozo_s_rho(int32_t dim, int32_t splitEpsMode);
+int32_t ozo_s_tau(float max);
+float ozo_s_nearsum(int32_t n, float *vec, int32_t ld);
+int32_t ozo_s_split(ozo_handle *h, int32_t m, int32_t n, const float *in, int32_t ldi, float *out,
+                    int32_t ldo, float *split, int32_t lds, float *max);
+int32_t ozo_s_gemm(ozo_handle *h, char transA, char transB, int32_t m, int32_t n, int32_t k,
+                   float alpha, const float *A, int32_t lda, const float *B, int32_t ldb,
+                   float beta, float *C, int32_t ldc);
+float ozo_s_dot(ozo_handle *h, int32_t n, const float *x, const float *y);
+float ozo_s_nrm2(ozo_handle *h, int32_t n, const float *x);
+int32_t ozo_s_gemv(ozo_handle *h, char trans, int32_t m, int32_t n, float alpha, const float *A,
+                   int32_t lda, const float *x, float beta, float *y);
+int32_t ozo_s_axpy(ozo_handle *h, int32_t n, float alpha, const float *x, float *y);
+int32_t ozo_s_csrmv(ozo_handle *h, int32_t m, int32_t n, int32_t nnz, float alpha, const float *val,
+                    const int32_t *colind, const int32_t *rowptr, const float *x, float beta,
+                    float *y);
+float *ozo_s_csrmv_splitA(ozo_handle *h, int32_t m, int32_t n, int32_t nnz, const float *val,
+                          const int32_t *rowptr);
+int32_t ozo_s_cg(ozo_handle *h, int32_t n, int32_t nnz, const float *val, const int32_t *colind,
+                 const int32_t *rowptr, const float *b, float *x, int32_t maxiter, float tol);
+/* erange-mode generator of the binary32 harness (FP_TYPE float, f_pi = (float)M_PI) */
+void ozo_s_init_mat(int32_t m, int32_t n, int32_t ld, float *mat, double phi, int32_t mode,
+                    int32_t seed);
+
 /* generators: testing/testing_common.cpp:664-806 */
 void ozo_srand48(int64_t seed);
 double ozo_drand48(void);

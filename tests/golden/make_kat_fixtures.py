@@ -4,8 +4,8 @@ The reference ships check-mode logs of its CPU OzBLAS build under script/a100/*.
 the sweep scripts that produced them (script/a100/_a100_{gemm,dot,gemv}.sh).  Each log row is
 ``m n k OK|NG max_rel_err (hex) n_wrong`` where n_wrong counts entries that differ from
 RN(exact) (testing/testing_common.cpp:839-930, MPFR 2048-bit reference).  This script pairs every
-``testing_ozblas_d{gemm,dot,gemv}`` command line with its log and stores the parsed rows -- data
-only, no reference source -- in tests/golden/a100_kat.json.
+``testing_ozblas_{d,s}{gemm,dot,gemv}`` command line (binary64 and binary32 builds) with its log and
+stores the parsed rows -- data only, no reference source -- in tests/golden/a100_kat.json.
 
 Usage: python tests/golden/make_kat_fixtures.py [/root/reference]
 """
@@ -26,15 +26,16 @@ def parse_cmd(line):
         return None
     cmd, out = line.split(">", 1)
     toks = shlex.split(cmd)
-    if not toks or not re.search(r"testing_ozblas_d(gemm|dot|gemv)$", toks[0]):
+    mt = re.search(r"testing_ozblas_([ds])(gemm|dot|gemv)$", toks[0]) if toks else None
+    if mt is None:
         return None
-    routine = re.search(r"testing_ozblas_d(\w+)$", toks[0]).group(1)
+    prec, routine = mt.group(1), mt.group(2)
     opts = {}
     for t in toks[1:]:
         if t.startswith("--") and "=" in t:
             k, v = t[2:].split("=", 1)
             opts[k] = v
-    return routine, opts, out.strip().split()[0]
+    return prec, routine, opts, out.strip().split()[0]
 
 
 def parse_dat(path):
@@ -59,14 +60,15 @@ def main():
             p = parse_cmd(line)
             if p is None:
                 continue
-            routine, opts, dat = p
+            prec, routine, opts, dat = p
             if dat in seen:
                 continue
             path = os.path.join(base, dat)
             if not os.path.exists(path):
                 continue
             seen.add(dat)
-            cases.append({"routine": routine, "opts": opts, "dat": dat, "rows": parse_dat(path)})
+            cases.append({"prec": prec, "routine": routine, "opts": opts, "dat": dat,
+                          "rows": parse_dat(path)})
     json.dump({"source": "script/a100 (OzBLAS 1.6a reference logs)", "cases": cases},
               open(OUT, "w"), indent=1)
     print(f"wrote {len(cases)} cases to {OUT}")

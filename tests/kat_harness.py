@@ -58,8 +58,24 @@ def handle_flags(th):
                 splitEpsMode=th["splitEpsMode"])
 
 
-def check(target, ref):
-    """mublasCheckMatrix comparison on binary64 (MPFR_CHECK undefined)."""
+def check(target, ref, prec="d"):
+    """mublasCheckMatrix comparison on FP_TYPE (MPFR_CHECK undefined): ref = ToFptype(mpref), which
+    for binary32 is the binary64 RN of the exact value rounded again to float
+    (testing/testing_common.cpp:621-640); differences and relative errors in FP_TYPE."""
+    if prec == "s":
+        target = np.asarray(target, dtype=np.float32).ravel()
+        ref = np.asarray(ref, dtype=np.float64).astype(np.float32).ravel()
+        with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+            dif = target - ref
+            rerr = np.where(ref == 0, np.float32(0), np.abs(dif / np.where(ref == 0, np.float32(1), ref)))
+        finite = np.isfinite(target)
+        nicnt = int((~finite).sum())
+        errcnt = int(((dif != 0) & finite).sum())
+        rmax = np.float32(0)
+        for r in rerr:
+            if rmax < r:
+                rmax = r
+        return float(rmax), errcnt, nicnt
     target = np.asarray(target, dtype=np.float64).ravel()
     ref = np.asarray(ref, dtype=np.float64).ravel()
     with np.errstate(divide="ignore", invalid="ignore"):
@@ -89,7 +105,11 @@ def _sizes(th, dims):
             return
 
 
-def run_gemm_sweep(opts, impl, max_size=None):
+def _gen(prec):
+    return O.s_init_mat if prec == "s" else O.init_mat
+
+
+def run_gemm_sweep(opts, impl, max_size=None, prec="d"):
     th = harness_opts(opts)
     stop = th["stop"]
     if th["tranA"] in "Nn":
@@ -103,9 +123,9 @@ def run_gemm_sweep(opts, impl, max_size=None):
     rdc, cdc = (th["m_const"] or stop), (th["n_const"] or stop)
     lda, ldb, ldc = rda, rdb, rdc
     O.srand(1)  # rand() is never seeded by the harness
-    A = O.init_mat(rda, cda, lda, th["phi"], 3, 1)
-    B = O.init_mat(rdb, cdb, ldb, th["phi"], 3, 2)
-    Cm = np.zeros(ldc * cdc, dtype=np.float64)
+    A = _gen(prec)(rda, cda, lda, th["phi"], 3, 1)
+    B = _gen(prec)(rdb, cdb, ldb, th["phi"], 3, 2)
+    Cm = np.zeros(ldc * cdc, dtype=A.dtype)
     out = []
     for d in _sizes(th, ("m", "n", "k")):
         m, n, k = d["m"], d["n"], d["k"]
@@ -115,30 +135,31 @@ def run_gemm_sweep(opts, impl, max_size=None):
         got = Cm.reshape(cdc, ldc).T[:m, :n]
         Am = A.reshape(cda, lda).T
         Bm = B.reshape(cdb, ldb).T
-        ref = O.exact_gemm_rn(th["tranA"], th["tranB"], m, n, k, A, lda, B, ldb)
-        out.append((m, n, k) + check(got, ref))
+        ref = O.exact_gemm_rn(th["tranA"], th["tranB"], m, n, k, A.astype(np.float64), lda,
+                              B.astype(np.float64), ldb)
+        out.append((m, n, k) + check(got, ref, prec))
         del Am, Bm
     return out
 
 
-def run_dot_sweep(opts, impl, max_size=None):
+def run_dot_sweep(opts, impl, max_size=None, prec="d"):
     th = harness_opts(opts)
     nh = th["n_const"] or th["stop"]
     O.srand(1)
-    X = O.init_mat(nh, 1, 0, th["phi"], 3, 1)
-    Y = O.init_mat(nh, 1, 0, th["phi"], 3, 2)
+    X = _gen(prec)(nh, 1, 0, th["phi"], 3, 1)
+    Y = _gen(prec)(nh, 1, 0, th["phi"], 3, 2)
     out = []
     for d in _sizes(th, ("n",)):
         n = d["n"]
         if max_size is not None and n > max_size:
             break
         r = impl.dot(n, X[:n].copy(), Y[:n].copy())
-        ref = O.exact_dot_rn(X[:n], Y[:n])
-        out.append((None, n, None) + check([r], [ref]))
+        ref = O.exact_dot_rn(X[:n].astype(np.float64), Y[:n].astype(np.float64))
+        out.append((None, n, None) + check([r], [ref], prec))
     return out
 
 
-def run_gemv_sweep(opts, impl, max_size=None):
+def run_gemv_sweep(opts, impl, max_size=None, prec="d"):
     th = harness_opts(opts)
     stop = th["stop"]
     rda, cda, vlx, vly = stop, stop, stop, stop
@@ -152,9 +173,9 @@ def run_gemv_sweep(opts, impl, max_size=None):
     cda = th["n_const"] or cda
     lda = rda
     O.srand(1)
-    A = O.init_mat(rda, cda, lda, th["phi"], 3, 1)
-    X = O.init_mat(vlx, 1, 0, th["phi"], 3, 2)
-    Y = np.zeros(vly, dtype=np.float64)
+    A = _gen(prec)(rda, cda, lda, th["phi"], 3, 1)
+    X = _gen(prec)(vlx, 1, 0, th["phi"], 3, 2)
+    Y = np.zeros(vly, dtype=A.dtype)
     out = []
     for d in _sizes(th, ("m", "n")):
         m, n = d["m"], d["n"]
@@ -162,11 +183,12 @@ def run_gemv_sweep(opts, impl, max_size=None):
             break
         lx, ly = (n, m) if th["tranA"] in "Nn" else (m, n)
         impl.gemv(th["tranA"], m, n, 1.0, A, lda, X, 0.0, Y)
+        A64, X64 = A.astype(np.float64), X.astype(np.float64)
         if th["tranA"] in "Nn":
-            ref = O.exact_gemm_rn("N", "N", m, 1, n, A, lda, X, max(n, 1))
+            ref = O.exact_gemm_rn("N", "N", m, 1, n, A64, lda, X64, max(n, 1))
         else:
-            ref = O.exact_gemm_rn("T", "N", n, 1, m, A, lda, X, max(m, 1))
-        out.append((m, n, None) + check(Y[:ly], ref))
+            ref = O.exact_gemm_rn("T", "N", n, 1, m, A64, lda, X64, max(m, 1))
+        out.append((m, n, None) + check(Y[:ly], ref, prec))
     return out
 
 
@@ -201,3 +223,16 @@ class OracleImpl:
 
     def gemv(self, *a):
         return self.h.gemv(*a)
+
+
+class OracleImplS(OracleImpl):
+    """The binary32 restatement with the same harness conventions."""
+
+    def gemm(self, *a):
+        return self.h.sgemm(*a)
+
+    def dot(self, n, x, y):
+        return self.h.sdot(n, x, y)
+
+    def gemv(self, *a):
+        return self.h.sgemv(*a)
