@@ -4,7 +4,7 @@
  * Drop-in for the OzBLAS GPU path: the handle keeps the fields of cuozblasHandle_t
  * (reference include/cuozblas.h:7-63) with the same names and meanings; the cuBLAS/cuSPARSE handles
  * are replaced by a HIP stream and a library-private backend pointer.  Entry points are the
- * binary64 (D) instantiations of the reference templates, given C names:
+ * binary64 (D) and binary32 (S) instantiations of the reference templates, given C names:
  *
  *   reference (include/cuozblas.h)                          here
  *   cuozblasCreate / cuozblasDestroy              :67-68    ozblasCreate / ozblasDestroy
@@ -16,6 +16,11 @@
  *   cuozblasRcsrmv<double,double>                 :78       ozblasDcsrmv
  *   cuozblasRcsrmvSplitA<double,double>           :79       ozblasDcsrmvSplitA
  *   cuozblasRcg<double,double>                    :80       ozblasDcg
+ *   the same templates at <float,float,float> / <float,float> / <float>
+ *   (instantiated for the CPU library in src/cpu/ozblas_dot.cpp:44, ozblas_nrm2.cpp:36,
+ *   ozblas_axpy.cpp:30, ozblas_gemv.cpp:34, ozblas_gemm.cpp:320, ozblas_csrmv.cpp:186,276,
+ *   ozblas_cg.cpp:133)                                      ozblasS{dot,nrm2,axpy,gemv,gemm,
+ *                                                                  csrmv,csrmvSplitA,cg}
  *
  * Semantics follow cuOzBLAS: matrices and vectors are device pointers (column-major, 0-based
  * CSR), alpha/beta are host scalars, DOT/NRM2 write their result to a host pointer, every call is
@@ -23,6 +28,13 @@
  * the epilogue fma(alpha, t, beta*C) that reads C even for beta == 0) follow the reference CPU
  * OzBLAS (src/cpu), so with sliceType = OZBLAS_SLICE_FP64 the output is bitwise identical to it.
  * Errors are returned as OZBLAS_STATUS_* codes instead of exit(1).
+ *
+ * Binary32: the split, NearSum/Fsum and epilogue follow the reference's float instantiations
+ * (float arithmetic, with its double-evaluated log2, fma and sqrt); slices are held as doubles and
+ * multiplied on the FP64 path, which is exact, then rounded once to float as the reference's float
+ * slice products.  sliceType is not consulted (the digit forms are binary64 only), and Dot2
+ * (splitEpsMode = 2) returns NOT_SUPPORTED, as the reference's float Dot2 GEMM is a stub
+ * (src/cpu/ozblas_blas_wrapper.cpp:179-181).  ozblasScg iterates on the synchronous path.
  */
 #ifndef OZBLAS_MI355X_H
 #define OZBLAS_MI355X_H
@@ -141,6 +153,11 @@ int32_t ozblasDdot(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx,
 int32_t ozblasDnrm2(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx, double *ret);
 int32_t ozblasDaxpy(ozblasHandle_t *oh, int32_t n, double alpha, const double *x, int32_t incx,
                     double *y, int32_t incy);
+int32_t ozblasSdot(ozblasHandle_t *oh, int32_t n, const float *x, int32_t incx, const float *y,
+                   int32_t incy, float *ret);
+int32_t ozblasSnrm2(ozblasHandle_t *oh, int32_t n, const float *x, int32_t incx, float *ret);
+int32_t ozblasSaxpy(ozblasHandle_t *oh, int32_t n, float alpha, const float *x, int32_t incx,
+                    float *y, int32_t incy);
 
 /* level 2/3: cuozblasRgemv / cuozblasRgemm (include/cuozblas.h:76-77) */
 int32_t ozblasDgemv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, double alpha,
@@ -149,6 +166,12 @@ int32_t ozblasDgemv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, doubl
 int32_t ozblasDgemm(ozblasHandle_t *oh, char transA, char transB, int32_t m, int32_t n, int32_t k,
                     double alpha, const double *A, int32_t lda, const double *B, int32_t ldb,
                     double beta, double *C, int32_t ldc);
+int32_t ozblasSgemv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, float alpha,
+                    const float *A, int32_t lda, const float *x, int32_t incx, float beta, float *y,
+                    int32_t incy);
+int32_t ozblasSgemm(ozblasHandle_t *oh, char transA, char transB, int32_t m, int32_t n, int32_t k,
+                    float alpha, const float *A, int32_t lda, const float *B, int32_t ldb, float beta,
+                    float *C, int32_t ldc);
 
 /* sparse: cuozblasRcsrmv / cuozblasRcsrmvSplitA / cuozblasRcg (include/cuozblas.h:78-80).
  * descrA is accepted and ignored (general matrix, 0-based indices). */
@@ -160,6 +183,16 @@ double *ozblasDcsrmvSplitA(ozblasHandle_t *oh, char transA, int32_t m, int32_t n
 int32_t ozblasDcg(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz, const void *descrA,
                   const double *A, const int32_t *colind, const int32_t *rowptr, const double *b,
                   double *x, int32_t maxiter, double tol);
+/* ozblasScsrmvSplitA returns the library's slice buffer (FP64 slices in the workspace), to be
+ * passed back as A of ozblasScsrmv; its layout is private, the pointer is typed as the reference's */
+int32_t ozblasScsrmv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, int32_t nnz,
+                     float alpha, const void *descrA, const float *A, const int32_t *colind,
+                     const int32_t *rowptr, const float *x, float beta, float *y);
+float *ozblasScsrmvSplitA(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, int32_t nnz,
+                          const void *descrA, const float *A, const int32_t *rowptr);
+int32_t ozblasScg(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz, const void *descrA,
+                  const float *A, const int32_t *colind, const int32_t *rowptr, const float *b,
+                  float *x, int32_t maxiter, float tol);
 
 /* batched uneven GEMM (no reference counterpart; the reference batches only the slice pairs of one
  * GEMM, src/cuda/cuozblas_gemm.cu:150-190).  Arrays of per-problem sizes and device pointers live

@@ -3,7 +3,7 @@
  *
  * Restatement of OzBLAS 1.6a src/cpu for the <float,float,float> instantiations
  * (src/cpu/ozblas_gemm.cpp:320, ozblas_dot.cpp:44, ozblas_nrm2.cpp:36, ozblas_gemv.cpp:34,
- * ozblas_csrmv.cpp:186, ozblas_cg.cpp:133, ozblas_axpy.cpp:27, ozblas_split.cpp:294,933).
+ * ozblas_csrmv.cpp:186, ozblas_cg.cpp:133, ozblas_axpy.cpp:30, ozblas_split.cpp:294,933).
  *
  * The reference compiles these templates with <cmath> and no using-directive, so inside them an
  * unqualified log2/fma/scalbn/fabs/sqrt of a float argument is the C library's double function:

@@ -80,12 +80,25 @@ _dgemm_batched = _sig("ozblasDgemmBatched", _i, _H, _i, C.c_char_p, C.c_char_p,
                       C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_d),
                       C.POINTER(_p), C.POINTER(_i), C.POINTER(_p), C.POINTER(_i),
                       C.POINTER(_d), C.POINTER(_p), C.POINTER(_i))
+_f = C.c_float
+_sgemm = _sig("ozblasSgemm", _i, _H, _c, _c, _i, _i, _i, _f, _p, _i, _p, _i, _f, _p, _i)
+_sgemv = _sig("ozblasSgemv", _i, _H, _c, _i, _i, _f, _p, _i, _p, _i, _f, _p, _i)
+_sdot = _sig("ozblasSdot", _i, _H, _i, _p, _i, _p, _i, C.POINTER(C.c_float))
+_snrm2 = _sig("ozblasSnrm2", _i, _H, _i, _p, _i, C.POINTER(C.c_float))
+_saxpy = _sig("ozblasSaxpy", _i, _H, _i, _f, _p, _i, _p, _i)
+_scsrmv = _sig("ozblasScsrmv", _i, _H, _c, _i, _i, _i, _f, _p, _p, _p, _p, _p, _f, _p)
+_scsrmv_split = _sig("ozblasScsrmvSplitA", _p, _H, _c, _i, _i, _i, _p, _p, _p)
+_scg = _sig("ozblasScg", _i, _H, _c, _i, _i, _p, _p, _p, _p, _p, _p, _i, _f)
 _version = _sig("ozblasGetVersionString", C.c_char_p)
 _gen_phi = _sig("ozblasGenPhi", None, _i, _i, _i, _p, _d, C.c_int64)
 internal_tau = _sig("ozblas_internal_tau", _i, _d)
 internal_rho = _sig("ozblas_internal_rho", _i, _i, _i)
 internal_rho_row = _sig("ozblas_internal_rho_row", _i, _i)
 internal_rho_row_mode = _sig("ozblas_internal_rho_row_mode", _i, _i, _i)
+internal_tau_f = _sig("ozblas_internal_tau_f", _i, C.c_float)
+internal_nearsum_exp_f = _sig("ozblas_internal_nearsum_exp_f", _i, C.c_float)
+internal_rho_f = _sig("ozblas_internal_rho_f", _i, _i, _i)
+internal_rho_row_f = _sig("ozblas_internal_rho_row_f", _i, _i, _i)
 
 
 class OzblasError(RuntimeError):
@@ -209,6 +222,50 @@ class Handle:
                   _ptr(x), maxiter, tol)
         self._h.cg_verbose1 = None
         self._check("ozblasDcg", st)
+        return hist[: self._h.cg_numiter + 1]
+
+    # ---- binary32 (the reference's <float,float,float> instantiations) -----------------------
+    def sgemm(self, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc):
+        self._check("ozblasSgemm", _sgemm(self.ref, transA.encode(), transB.encode(), m, n, k,
+                                          alpha, _ptr(A), lda, _ptr(B), ldb, beta, _ptr(Cm), ldc))
+
+    def sgemv(self, trans, m, n, alpha, A, lda, x, beta, y):
+        self._check("ozblasSgemv", _sgemv(self.ref, trans.encode(), m, n, alpha, _ptr(A), lda,
+                                          _ptr(x), 1, beta, _ptr(y), 1))
+
+    def sdot(self, n, x, y) -> float:
+        r = C.c_float(0.0)
+        self._check("ozblasSdot", _sdot(self.ref, n, _ptr(x), 1, _ptr(y), 1, C.byref(r)))
+        return r.value
+
+    def snrm2(self, n, x) -> float:
+        r = C.c_float(0.0)
+        self._check("ozblasSnrm2", _snrm2(self.ref, n, _ptr(x), 1, C.byref(r)))
+        return r.value
+
+    def saxpy(self, n, alpha, x, y):
+        self._check("ozblasSaxpy", _saxpy(self.ref, n, alpha, _ptr(x), 1, _ptr(y), 1))
+
+    def scsrmv(self, m, n, nnz, alpha, val, colind, rowptr, x, beta, y):
+        self._check("ozblasScsrmv", _scsrmv(self.ref, b"N", m, n, nnz, alpha, None, _ptr(val),
+                                            _ptr(colind), _ptr(rowptr), _ptr(x), beta, _ptr(y)))
+
+    def scsrmv_splitA(self, m, n, nnz, val, rowptr):
+        p = _scsrmv_split(self.ref, b"N", m, n, nnz, None, _ptr(val), _ptr(rowptr))
+        if not p:
+            raise OzblasError("ozblasScsrmvSplitA", self._h.lastStatus)
+        return p
+
+    def scg(self, n, nnz, val, colind, rowptr, b, x, maxiter, tol, history=True):
+        import numpy as np
+        hist = np.zeros(maxiter + 2, dtype=np.float64)
+        if history:
+            self._h.verbose = 1
+            self._h.cg_verbose1 = hist.ctypes.data
+        st = _scg(self.ref, b"N", n, nnz, None, _ptr(val), _ptr(colind), _ptr(rowptr), _ptr(b),
+                  _ptr(x), maxiter, tol)
+        self._h.cg_verbose1 = None
+        self._check("ozblasScg", st)
         return hist[: self._h.cg_numiter + 1]
 
     def dgemm_batched(self, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc):

@@ -2,7 +2,8 @@
 
 Every source under ``ozblas_amd/csrc`` is compiled for ``--offload-arch=gfx950`` with
 ``-ffp-contract=off`` (the plain form must round every + - * separately, like the reference CPU
-build) and with packed FP32 instructions disabled, then linked into one C-ABI library.
+build), with binary32 subnormals kept (the binary32 split and NearSum round as the host does), and
+with packed FP32 instructions disabled, then linked into one C-ABI library.
 """
 from __future__ import annotations
 
@@ -21,6 +22,7 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("OZBLAS_ARCH", "gfx950")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
+    "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-Wno-unused-command-line-argument",
 ]

@@ -43,6 +43,24 @@ static int32_t sync(ozblasHandle_t *oh) {
 	return OZBLAS_STATUS_SUCCESS;
 }
 
+// Per-precision rules of the drivers: rho of getRho and the device tau tables (the binary32 split
+// takes tau from (float)log2((double)max), its NearSum from log2f).  Both precisions store slices
+// as doubles and share the FP64 product kernels.
+template <class T>
+struct Prec;
+template <>
+struct Prec<double> {
+	static int rho(int dim, int mode) { return rho_host(dim, mode); }
+	static const uint16_t *Jsplit(Backend *b) { return b->dTau; }
+	static const uint16_t *Jsum(Backend *b) { return b->dTau; }
+};
+template <>
+struct Prec<float> {
+	static int rho(int dim, int mode) { return rho_host_f(dim, mode); }
+	static const uint16_t *Jsplit(Backend *b) { return b->dTauF; }
+	static const uint16_t *Jsum(Backend *b) { return b->dTauF2; }
+};
+
 // reference slice count from the per-column first-zero index (max over columns)
 static inline int split_count(const ozblasHandle_t *oh, int zmax) {
 	const int cap = split_cap(oh);
@@ -67,11 +85,13 @@ struct DenseSplit {
 	int count = 0;
 };
 
-static int32_t dense_split(ozblasHandle_t *oh, int k, int cols, const double *in, int64_t ldi,
-                           double *rem, int64_t ldr, DenseSplit &ds) {
+template <class T>
+static int32_t dense_split(ozblasHandle_t *oh, int k, int cols, const T *in, int64_t ldi, T *rem,
+                           int64_t ldr, DenseSplit &ds) {
 	Backend *b = be(oh);
 	const int cap = split_cap(oh);
-	const int rho = rho_host(k, oh->splitEpsMode);
+	const int rho = Prec<T>::rho(k, oh->splitEpsMode);
+	const uint16_t *J = Prec<T>::Jsplit(b);
 	ds.ldk = rup(k, 16);
 	ds.sstride = ds.ldk * cols;
 	int *zcol = (int *)walloc(oh, sizeof(int) * (size_t)cols);
@@ -83,7 +103,7 @@ static int32_t dense_split(ozblasHandle_t *oh, int k, int cols, const double *in
 	double *scratch = (double *)walloc(oh, sizeof(double) * (size_t)ds.ldk * cols * 2);
 	if (!scratch) return OZBLAS_STATUS_WORKSPACE;
 	launch_split_cols(b->stream, k, cols, in, ldi, rem, ldr, scratch, ds.ldk, 0, rho,
-	                  oh->splitShift, capMeasure, b->dTau, zcol, b->dCounters);
+	                  oh->splitShift, capMeasure, J, zcol, b->dCounters);
 	OZ_HIP(hipMemcpyAsync(b->hPinned, b->dCounters, sizeof(int), hipMemcpyDeviceToHost, b->stream));
 	OZ_TRY(sync(oh));
 	const int zmax = (int)((int32_t *)b->hPinned)[0];
@@ -93,7 +113,7 @@ static int32_t dense_split(ozblasHandle_t *oh, int k, int cols, const double *in
 	ds.slices = (double *)walloc(oh, sizeof(double) * (size_t)ds.sstride * slots);
 	if (!ds.slices) return OZBLAS_STATUS_WORKSPACE;
 	launch_split_cols(b->stream, k, cols, in, ldi, rem, ldr, ds.slices, ds.ldk, ds.sstride, rho,
-	                  oh->splitShift, cap, b->dTau, zcol, b->dCounters + 1);
+	                  oh->splitShift, cap, J, zcol, b->dCounters + 1);
 	launch_zero_slots(b->stream, k, cols, ds.slices, ds.ldk, ds.sstride, zcol, ds.count);
 	// zero the pad rows [k, ldk) of every slot the product kernels read
 	if (ds.ldk > k) {
@@ -107,19 +127,20 @@ static int32_t dense_split(ozblasHandle_t *oh, int k, int cols, const double *in
 // ---------------------------------------------------------------------------------------------
 // slice count of a k-contiguous operand without storing slices (for the blocking decision)
 // ---------------------------------------------------------------------------------------------
-static int32_t count_cols(ozblasHandle_t *oh, int k, int cols, const double *in, int64_t ldi,
+template <class T>
+static int32_t count_cols(ozblasHandle_t *oh, int k, int cols, const T *in, int64_t ldi,
                           int *count) {
 	Backend *b = be(oh);
 	const uint64_t mark = wmark(oh);
 	const int64_t ldk = rup(k, 16);
-	double *rem = (double *)walloc(oh, 8ull * ldk * cols);
+	T *rem = (T *)walloc(oh, sizeof(T) * ldk * cols);
 	double *scratch = (double *)walloc(oh, 8ull * ldk * cols);
 	int *zcol = (int *)walloc(oh, sizeof(int) * (size_t)cols);
 	if (!rem || !scratch || !zcol) return OZBLAS_STATUS_WORKSPACE;
 	OZ_HIP(hipMemsetAsync(b->dCounters, 0, sizeof(int), b->stream));
 	launch_split_cols(b->stream, k, cols, in, ldi, rem, ldk, scratch, ldk, 0,
-	                  rho_host(k, oh->splitEpsMode), oh->splitShift, split_cap(oh), b->dTau, zcol,
-	                  b->dCounters);
+	                  Prec<T>::rho(k, oh->splitEpsMode), oh->splitShift, split_cap(oh),
+	                  Prec<T>::Jsplit(b), zcol, b->dCounters);
 	OZ_HIP(hipMemcpyAsync(b->hPinned, b->dCounters, sizeof(int), hipMemcpyDeviceToHost, b->stream));
 	OZ_TRY(sync(oh));
 	*count = split_count(oh, ((int32_t *)b->hPinned)[0]);
@@ -130,9 +151,10 @@ static int32_t count_cols(ozblasHandle_t *oh, int k, int cols, const double *in,
 // ---------------------------------------------------------------------------------------------
 // GEMM, plain form (src/cpu/ozblas_gemm.cpp:3-315)
 // ---------------------------------------------------------------------------------------------
+template <class T>
 static int32_t gemm_plain(ozblasHandle_t *oh, char transA, char transB, int m, int n, int k,
-                          double alpha, const double *A, int lda, const double *B, int ldb,
-                          double beta, double *C, int ldc) {
+                          T alpha, const T *A, int lda, const T *B, int ldb, T beta, T *C,
+                          int ldc) {
 	Backend *b = be(oh);
 	hipStream_t st = b->stream;
 	const bool fsum = (oh->sumMode == 0);
@@ -144,17 +166,17 @@ static int32_t gemm_plain(ozblasHandle_t *oh, char transA, char transB, int m, i
 	const int64_t ldk = rup(k, 16);
 	WallTimer tt;
 	// whole-operand copies of op(A) / op(B) in k-contiguous form (transposed when needed)
-	const double *ain = A, *bin = B;
+	const T *ain = A, *bin = B;
 	int64_t ldai = lda, ldbi = ldb;
 	if (is_n(transA)) {
-		double *tA = (double *)walloc(oh, 8ull * ldk * m);
+		T *tA = (T *)walloc(oh, sizeof(T) * ldk * m);
 		if (!tA) return OZBLAS_STATUS_WORKSPACE;
 		launch_transpose(st, m, k, A, lda, tA, ldk);
 		ain = tA;
 		ldai = ldk;
 	}
 	if (!is_n(transB)) {
-		double *tB = (double *)walloc(oh, 8ull * ldk * n);
+		T *tB = (T *)walloc(oh, sizeof(T) * ldk * n);
 		if (!tB) return OZBLAS_STATUS_WORKSPACE;
 		launch_transpose(st, n, k, B, ldb, tB, ldk);
 		bin = tB;
@@ -191,7 +213,7 @@ static int32_t gemm_plain(ozblasHandle_t *oh, char transA, char transB, int m, i
 		for (int in = 0; in * nbk < n; in++) {
 			const int nbk_ = imin(nbk, n - in * nbk);
 			wreset(oh, base2);
-			double *rem = (double *)walloc(oh, 8ull * ldk * std::max(mbk_, nbk_));
+			T *rem = (T *)walloc(oh, sizeof(T) * ldk * std::max(mbk_, nbk_));
 			if (!rem) return OZBLAS_STATUS_WORKSPACE;
 			DenseSplit da, db;
 			OZ_TRY(dense_split(oh, k, mbk_, ain + (int64_t)im * mbk * ldai, ldai, rem, ldk, da));
@@ -225,7 +247,7 @@ static int32_t gemm_plain(ozblasHandle_t *oh, char transA, char transB, int m, i
 			int64_t *dOff = (int64_t *)walloc(oh, sizeof(int64_t) * 3 * (size_t)ic);
 			if (!dOff) return OZBLAS_STATUS_WORKSPACE;
 			OZ_HIP(hipMemcpyAsync(dOff, hoff.data(), sizeof(int64_t) * 3 * ic, hipMemcpyHostToDevice, st));
-			double *Cblk = C + (int64_t)ldc * ((int64_t)in * nbk) + (int64_t)im * mbk;
+			T *Cblk = C + (int64_t)ldc * ((int64_t)in * nbk) + (int64_t)im * mbk;
 			OZ_HIP(hipEventRecord(b->ev[3], st));
 			// fused Fsum is valid when the summation visits the products in pair order; the
 			// reference's GEMM with a width-1 column block reads them through the GEMV map instead
@@ -238,9 +260,9 @@ static int32_t gemm_plain(ozblasHandle_t *oh, char transA, char transB, int m, i
 				summed = true;
 			} else {
 				const int64_t nplanes = (gemvPath || dotPath || !identityMap) ? std::max<int64_t>((int64_t)nA * nB, ic) : ic;
-				double *Cs = (double *)walloc(oh, 8ull * llsc * nplanes);
+				T *Cs = (T *)walloc(oh, sizeof(T) * llsc * nplanes);
 				if (!Cs) return OZBLAS_STATUS_WORKSPACE;
-				if (nplanes > ic) launch_fill(st, Cs, llsc * nplanes, 0.0);
+				if (nplanes > ic) launch_fill(st, Cs, llsc * nplanes, (T)0);
 				launch_gemm_f64_batched(st, mbk_, nbk_, k, da.slices, da.ldk, db.slices, db.ldk, Cs, ldcs,
 				                        dOff, dOff + ic, dOff + 2 * ic, ic);
 				OZ_HIP(hipEventRecord(b->ev[4], st));
@@ -249,7 +271,7 @@ static int32_t gemm_plain(ozblasHandle_t *oh, char transA, char transB, int m, i
 					                   ldc, alpha, beta);
 				else
 					launch_global_nearsum(st, mbk_, nbk_, ic, Cs, llsc, ldcs, Cblk, ldc, alpha, beta,
-					                      b->dTau);
+					                      Prec<T>::Jsum(b));
 			}
 			OZ_HIP(hipEventRecord(b->ev[5], st));
 			OZ_HIP(hipGetLastError());
@@ -516,11 +538,11 @@ struct VecSplit {
 
 // Splits x; with readCount the slice count is read back to the host (one synchronisation),
 // otherwise it stays on the device in vs.info for kernels that read it there.
-static int32_t vec_split(ozblasHandle_t *oh, int n, const double *x, VecSplit &vs,
-                         bool readCount = true) {
+template <class T>
+static int32_t vec_split(ozblasHandle_t *oh, int n, const T *x, VecSplit &vs, bool readCount = true) {
 	Backend *b = be(oh);
 	const int cap = split_cap(oh);
-	const int rho = rho_host(n, oh->splitEpsMode);
+	const int rho = Prec<T>::rho(n, oh->splitEpsMode);
 	vs.lds = rup(n, 32);
 	vs.slices = (double *)walloc(oh, 8ull * vs.lds * (cap + 2));
 	vs.maxv = (unsigned long long *)walloc(oh, 8ull * (cap + 2));
@@ -531,7 +553,7 @@ static int32_t vec_split(ozblasHandle_t *oh, int n, const double *x, VecSplit &v
 	OZ_HIP(hipMemsetAsync(scratch, 0, vsplit_scratch_bytes(), b->stream));
 	OZ_HIP(hipMemsetAsync(vs.maxv, 0, 8ull * (cap + 2), b->stream));
 	OZ_HIP(launch_vsplit(b->stream, n, x, vs.slices, vs.lds, rho, oh->splitShift, cap, oh->reproMode,
-	                     b->dTau, vs.maxv, vs.info, scratch, plan));
+	                     Prec<T>::Jsplit(b), vs.maxv, vs.info, scratch, plan));
 	if (!readCount) return OZBLAS_STATUS_SUCCESS;
 	OZ_HIP(hipMemcpyAsync(b->hPinned, vs.info, sizeof(int), hipMemcpyDeviceToHost, b->stream));
 	OZ_TRY(sync(oh));
@@ -564,13 +586,17 @@ static int32_t dot_core(ozblasHandle_t *oh, int n, const VecSplit &xs, const Vec
 	return OZBLAS_STATUS_SUCCESS;
 }
 
-static int32_t dot_impl(ozblasHandle_t *oh, int n, const double *x, const double *y, double *ret) {
+template <class T>
+static int32_t dot_impl(ozblasHandle_t *oh, int n, const T *x, const T *y, T *ret) {
 	Backend *b = be(oh);
+	constexpr bool f32 = sizeof(T) == 4;
 	if (oh->sumMode != 0 && oh->sumMode != 1) return OZBLAS_STATUS_NOT_SUPPORTED;
 	if (n <= 0) {  // empty vectors: the sum is +0 (fma(1, +0, 0 * ret) with ret = +0)
-		*ret = 0.0;
+		*ret = 0;
 		return OZBLAS_STATUS_SUCCESS;
 	}
+	// the reference's binary32 Dot2 GEMM is a stub (src/cpu/ozblas_blas_wrapper.cpp:179-181)
+	if (f32 && oh->splitEpsMode == 2) return OZBLAS_STATUS_NOT_SUPPORTED;
 	const uint64_t base = wmark(oh);
 	oh->splitShift = 1;  // counterInit (src/cpu/ozblas.cpp:8)
 	const bool dot2 = (oh->splitEpsMode == 2);
@@ -581,12 +607,13 @@ static int32_t dot_impl(ozblasHandle_t *oh, int n, const double *x, const double
 	} else {
 		OZ_TRY(vec_split(oh, n, y, ys, dot2));
 	}
-	double *dres = (double *)walloc(oh, 8);
+	T *dres = (T *)walloc(oh, 8);
 	if (!dres) return OZBLAS_STATUS_WORKSPACE;
 	OZ_HIP(hipMemsetAsync(dres, 0, 8, b->stream));
-	if (dot2) {
-		OZ_TRY(dot_core(oh, n, xs, ys, dres));
-	} else {
+	if constexpr (!f32) {
+		if (dot2) OZ_TRY(dot_core(oh, n, xs, ys, dres));
+	}
+	if (!dot2) {
 		// slice counts stay on the device: one launch for the pair dots and their sum
 		const int cap = split_cap(oh);
 		double *vals = (double *)walloc(oh, 8ull * (cap + 1) * (cap + 1));
@@ -595,16 +622,16 @@ static int32_t dot_impl(ozblasHandle_t *oh, int n, const double *x, const double
 		OZ_HIP(hipMemsetAsync(vals, 0, 8ull * (cap + 1) * (cap + 1), b->stream));
 		OZ_HIP(hipMemsetAsync(ctr, 0, 8, b->stream));
 		launch_dots_dev(b->stream, n, xs.slices, xs.lds, xs.info, ys.slices, ys.lds, ys.info, vals, ctr,
-		                xs.maxv, ys.maxv, cap + 2, b->dTau, oh->sumMode == 0 ? 1 : 0, nullptr, 0, 0,
-		                dres);
+		                xs.maxv, ys.maxv, cap + 2, Prec<T>::Jsum(b), oh->sumMode == 0 ? 1 : 0, nullptr,
+		                0, 0, dres, f32 ? 1 : 0);
 	}
-	OZ_HIP(hipMemcpyAsync(b->hPinned, dres, 8, hipMemcpyDeviceToHost, b->stream));
+	OZ_HIP(hipMemcpyAsync(b->hPinned, dres, sizeof(T), hipMemcpyDeviceToHost, b->stream));
 	if (!dot2) {
 		OZ_HIP(hipMemcpyAsync((int32_t *)b->hPinned + 2, xs.info, 4, hipMemcpyDeviceToHost, b->stream));
 		OZ_HIP(hipMemcpyAsync((int32_t *)b->hPinned + 3, ys.info, 4, hipMemcpyDeviceToHost, b->stream));
 	}
 	OZ_TRY(sync(oh));
-	std::memcpy(ret, b->hPinned, 8);
+	std::memcpy(ret, b->hPinned, sizeof(T));
 	if (!dot2) {
 		xs.count = ((int32_t *)b->hPinned)[2];
 		ys.count = ((int32_t *)b->hPinned)[3];
@@ -625,12 +652,13 @@ struct CsrSplit {
 	int count = 0;
 };
 
-static int32_t csr_split(ozblasHandle_t *oh, int m, int nnz, const double *val, const int *rowptr,
+template <class T>
+static int32_t csr_split(ozblasHandle_t *oh, int m, int nnz, const T *val, const int *rowptr,
                          CsrSplit &cs) {
 	Backend *b = be(oh);
 	const int cap = split_cap(oh);
 	cs.lds = rup(nnz, 32);
-	double *work = (double *)walloc(oh, 8ull * cs.lds);
+	T *work = (T *)walloc(oh, sizeof(T) * cs.lds);
 	int *zrow = (int *)walloc(oh, sizeof(int) * (size_t)m);
 	if (!work || !zrow) return OZBLAS_STATUS_WORKSPACE;
 	if (!cs.slices) {
@@ -638,8 +666,8 @@ static int32_t csr_split(ozblasHandle_t *oh, int m, int nnz, const double *val, 
 		if (!cs.slices) return OZBLAS_STATUS_WORKSPACE;
 	}
 	OZ_HIP(hipMemsetAsync(b->dCounters, 0, sizeof(int), b->stream));
-	launch_split_csr(b->stream, m, val, rowptr, work, cs.slices, cs.lds, oh->splitShift, cap, b->dTau,
-	                 zrow, b->dCounters, oh->splitEpsMode);
+	launch_split_csr(b->stream, m, val, rowptr, work, cs.slices, cs.lds, oh->splitShift, cap,
+	                 Prec<T>::Jsplit(b), zrow, b->dCounters, oh->splitEpsMode);
 	OZ_HIP(hipMemcpyAsync(b->hPinned, b->dCounters, sizeof(int), hipMemcpyDeviceToHost, b->stream));
 	OZ_TRY(sync(oh));
 	const int zmax = ((int32_t *)b->hPinned)[0];
@@ -657,16 +685,18 @@ static int32_t csr_split(ozblasHandle_t *oh, int m, int nnz, const double *val, 
 	return OZBLAS_STATUS_SUCCESS;
 }
 
-static int32_t csrmv_core(ozblasHandle_t *oh, int m, int n, int nnz, double alpha,
-                          const CsrSplit &as, const int *colind, const int *rowptr,
-                          const VecSplit &xs, double beta, double *y) {
+template <class T>
+static int32_t csrmv_core(ozblasHandle_t *oh, int m, int n, int nnz, T alpha, const CsrSplit &as,
+                          const int *colind, const int *rowptr, const VecSplit &xs, T beta, T *y) {
 	Backend *b = be(oh);
 	const int nA = as.count, nB = xs.count;
 	int maxlevel = (nA - 1) + (nB - 1);
 	maxlevel = imax(maxlevel - oh->fastMode, imin(nA - 1, nB - 1));
 	const bool fsum = oh->sumMode == 0;
 	if (fsum && oh->fastMode) return OZBLAS_STATUS_NOT_SUPPORTED;  // reference reads unwritten slots
-	if (oh->splitEpsMode == 2) {  // Dot2 slice products (:139-142, 159-162)
+	if constexpr (sizeof(T) == 4) {  // no binary32 Dot2 (src/cpu/ozblas_blas_wrapper.cpp:179-181)
+		if (oh->splitEpsMode == 2) return OZBLAS_STATUS_NOT_SUPPORTED;
+	} else if (oh->splitEpsMode == 2) {  // Dot2 slice products (:139-142, 159-162)
 		// with fastMode the reference's 2 numB-column blocks no longer match its summation index
 		if (oh->fastMode || nA > 4 || nB > 4) return OZBLAS_STATUS_NOT_SUPPORTED;
 		launch_csrmm_sum_x2(b->stream, m, as.slices, as.lds, nA, colind, rowptr, xs.slices, xs.lds, nB,
@@ -677,19 +707,20 @@ static int32_t csrmv_core(ozblasHandle_t *oh, int m, int n, int nnz, double alph
 	}
 	if (nA <= 4 && nB <= 8) {
 		launch_csrmm_sum(b->stream, m, nnz, as.slices, as.lds, nA, colind, rowptr, xs.slices, xs.lds, nB,
-		                 maxlevel, fsum ? 1 : 0, alpha, beta, y, b->dTau);
+		                 maxlevel, fsum ? 1 : 0, alpha, beta, y, Prec<T>::Jsum(b));
 	} else {
 		int ic = 0;
 		for (int ia = 0; ia < imin(maxlevel + 1, nA); ia++) ic += imin(nB, maxlevel + 1 - ia);
 		const int64_t ldcs = rup(m, 8);
-		double *Cs = (double *)walloc(oh, 8ull * ldcs * ((int64_t)nA * nB > ic ? (int64_t)nA * nB : ic));
+		T *Cs = (T *)walloc(oh, sizeof(T) * ldcs * ((int64_t)nA * nB > ic ? (int64_t)nA * nB : ic));
 		if (!Cs) return OZBLAS_STATUS_WORKSPACE;
 		launch_csrmm_store(b->stream, m, as.slices, as.lds, nA, colind, rowptr, xs.slices, xs.lds, nB,
 		                   maxlevel, Cs, ldcs);
 		if (fsum)
 			launch_global_fsum(b->stream, m, 1, nA, nB, maxlevel, 3, Cs, ldcs, ldcs, y, 1, alpha, beta);
 		else
-			launch_global_nearsum(b->stream, m, 1, ic, Cs, ldcs, ldcs, y, 1, alpha, beta, b->dTau);
+			launch_global_nearsum(b->stream, m, 1, ic, Cs, ldcs, ldcs, y, 1, alpha, beta,
+			                      Prec<T>::Jsum(b));
 	}
 	(void)n;
 	return OZBLAS_STATUS_SUCCESS;
@@ -702,6 +733,42 @@ using namespace oz;
 // =============================================================================================
 // C ABI
 // =============================================================================================
+namespace oz {
+
+template <class T>
+static int32_t gemm_k0(ozblasHandle_t *oh, int m, int n, T alpha, T beta, T *C, int ldc) {
+	launch_gemm_k0(be(oh)->stream, m, n, alpha, beta, C, ldc);
+	OZ_HIP(hipGetLastError());
+	return sync(oh);
+}
+
+// ozblasRdot (src/cpu/ozblas_dot.cpp): the 1 x 1 x n GEMM of the split vectors
+template <class T>
+static int32_t dot_entry(ozblasHandle_t *oh, int32_t n, const T *x, int32_t incx, const T *y,
+                         int32_t incy, T *ret) {
+	OZ_TRY(ensure_init(oh));
+	if (incx != 1 || incy != 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	if (n <= 0) {
+		*ret = 0;
+		return oh->lastStatus = OZBLAS_STATUS_SUCCESS;
+	}
+	if (oh->reproMode == 0 && oh->nSplitMax == 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	return oh->lastStatus = dot_impl(oh, n, x, y, ret);
+}
+
+template <class T>
+static int32_t axpy_entry(ozblasHandle_t *oh, int32_t n, T alpha, const T *x, int32_t incx, T *y,
+                          int32_t incy) {
+	OZ_TRY(ensure_init(oh));
+	if (incx != 1 || incy != 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	if (n <= 0) return OZBLAS_STATUS_SUCCESS;
+	launch_axpy_host(be(oh)->stream, n, alpha, x, y, oh->reproMode != 0);
+	OZ_HIP(hipGetLastError());
+	return oh->lastStatus = sync(oh);
+}
+
+}  // namespace oz
+
 extern "C" int32_t ozblasDgemm(ozblasHandle_t *oh, char transA, char transB, int32_t m, int32_t n,
                                int32_t k, double alpha, const double *A, int32_t lda,
                                const double *B, int32_t ldb, double beta, double *C, int32_t ldc) {
@@ -710,12 +777,8 @@ extern "C" int32_t ozblasDgemm(ozblasHandle_t *oh, char transA, char transB, int
 	if (m == 0 || n == 0) return oh->lastStatus = OZBLAS_STATUS_SUCCESS;
 	oh->splitShift = 1;  // counterInit
 	if (oh->reproMode == 0 && oh->nSplitMax == 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
-	if (k == 0) {  // no slices to multiply: the reference's sum is +0
-		launch_gemm_k0(be(oh)->stream, m, n, alpha, beta, C, ldc);
-		OZ_HIP(hipGetLastError());
-		OZ_TRY(sync(oh));
-		return oh->lastStatus = OZBLAS_STATUS_SUCCESS;
-	}
+	if (k == 0)  // no slices to multiply: the reference's sum is +0
+		return oh->lastStatus = gemm_k0(oh, m, n, alpha, beta, C, ldc);
 	if (oh->sliceType != OZBLAS_SLICE_FP64) {
 		if (oh->sumMode == 1 && oh->nSplitMax == 0) {
 			bool handled = false;
@@ -725,6 +788,21 @@ extern "C" int32_t ozblasDgemm(ozblasHandle_t *oh, char transA, char transB, int
 			return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
 		}
 	}
+	return oh->lastStatus = gemm_plain(oh, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+// binary32: the plain form only (sliceType is not consulted: the digit forms are binary64)
+extern "C" int32_t ozblasSgemm(ozblasHandle_t *oh, char transA, char transB, int32_t m, int32_t n,
+                               int32_t k, float alpha, const float *A, int32_t lda, const float *B,
+                               int32_t ldb, float beta, float *C, int32_t ldc) {
+	OZ_TRY(ensure_init(oh));
+	if (m < 0 || n < 0 || k < 0) return oh->lastStatus = OZBLAS_STATUS_INVALID_VALUE;
+	if (m == 0 || n == 0) return oh->lastStatus = OZBLAS_STATUS_SUCCESS;
+	oh->splitShift = 1;  // counterInit
+	if (oh->reproMode == 0 && oh->nSplitMax == 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	// the reference's binary32 Dot2 GEMM is a stub (src/cpu/ozblas_blas_wrapper.cpp:179-181)
+	if (oh->splitEpsMode == 2) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	if (k == 0) return oh->lastStatus = gemm_k0(oh, m, n, alpha, beta, C, ldc);
 	return oh->lastStatus = gemm_plain(oh, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
@@ -744,16 +822,24 @@ extern "C" int32_t ozblasDgemv(ozblasHandle_t *oh, char transA, int32_t m, int32
 	return s;
 }
 
-extern "C" int32_t ozblasDdot(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx,
-                              const double *y, int32_t incy, double *ret) {
+extern "C" int32_t ozblasSgemv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, float alpha,
+                               const float *A, int32_t lda, const float *x, int32_t incx, float beta,
+                               float *y, int32_t incy) {
 	OZ_TRY(ensure_init(oh));
 	if (incx != 1 || incy != 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
-	if (n <= 0) {
-		*ret = 0.0;
-		return oh->lastStatus = OZBLAS_STATUS_SUCCESS;
-	}
-	if (oh->reproMode == 0 && oh->nSplitMax == 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
-	return oh->lastStatus = dot_impl(oh, n, x, y, ret);
+	if (is_n(transA))  // src/cpu/ozblas_gemv.cpp:23-26
+		return ozblasSgemm(oh, 'n', 'n', m, 1, n, alpha, A, lda, x, n, beta, y, m);
+	return ozblasSgemm(oh, 't', 'n', n, 1, m, alpha, A, lda, x, m, beta, y, n);
+}
+
+extern "C" int32_t ozblasDdot(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx,
+                              const double *y, int32_t incy, double *ret) {
+	return dot_entry(oh, n, x, incx, y, incy, ret);
+}
+
+extern "C" int32_t ozblasSdot(ozblasHandle_t *oh, int32_t n, const float *x, int32_t incx,
+                              const float *y, int32_t incy, float *ret) {
+	return dot_entry(oh, n, x, incx, y, incy, ret);
 }
 
 extern "C" int32_t ozblasDnrm2(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx,
@@ -766,21 +852,32 @@ extern "C" int32_t ozblasDnrm2(ozblasHandle_t *oh, int32_t n, const double *x, i
 	return OZBLAS_STATUS_SUCCESS;
 }
 
-extern "C" int32_t ozblasDaxpy(ozblasHandle_t *oh, int32_t n, double alpha, const double *x,
-                               int32_t incx, double *y, int32_t incy) {
+extern "C" int32_t ozblasSnrm2(ozblasHandle_t *oh, int32_t n, const float *x, int32_t incx,
+                               float *ret) {
 	OZ_TRY(ensure_init(oh));
-	if (incx != 1 || incy != 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
-	if (n <= 0) return OZBLAS_STATUS_SUCCESS;
-	launch_axpy_host(be(oh)->stream, n, alpha, x, y, oh->reproMode != 0);
-	OZ_HIP(hipGetLastError());
-	return oh->lastStatus = sync(oh);
+	float d = 0.f;
+	int32_t s = ozblasSdot(oh, n, x, incx, x, incx, &d);
+	if (s != OZBLAS_STATUS_SUCCESS) return s;
+	*ret = (float)sqrt((double)d);  // the double sqrt, rounded (src/cpu/ozblas_nrm2.cpp:27)
+	return OZBLAS_STATUS_SUCCESS;
 }
 
-extern "C" double *ozblasDcsrmvSplitA(ozblasHandle_t *oh, char transA, int32_t m, int32_t n,
-                                      int32_t nnz, const void *descrA, const double *A,
-                                      const int32_t *rowptr) {
-	(void)descrA;
-	(void)n;
+extern "C" int32_t ozblasDaxpy(ozblasHandle_t *oh, int32_t n, double alpha, const double *x,
+                               int32_t incx, double *y, int32_t incy) {
+	return axpy_entry(oh, n, alpha, x, incx, y, incy);
+}
+
+extern "C" int32_t ozblasSaxpy(ozblasHandle_t *oh, int32_t n, float alpha, const float *x,
+                               int32_t incx, float *y, int32_t incy) {
+	return axpy_entry(oh, n, alpha, x, incx, y, incy);
+}
+
+namespace oz {
+// ozblasRcsrmvSplitA (src/cpu/ozblas_csrmv.cpp:191-271): the slices stay in the workspace (kept
+// across calls until the next CG or SplitA); returns them, or nullptr with lastStatus set
+template <class T>
+static double *csrmv_split_a(ozblasHandle_t *oh, char transA, int32_t m, int32_t nnz, const T *A,
+                             const int32_t *rowptr) {
 	if (ensure_init(oh) != OZBLAS_STATUS_SUCCESS) return nullptr;
 	if (!is_n(transA)) {
 		oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
@@ -826,10 +923,10 @@ extern "C" double *ozblasDcsrmvSplitA(ozblasHandle_t *oh, char transA, int32_t m
 	return cs.slices;
 }
 
-namespace oz {
-static int32_t csrmv_impl(ozblasHandle_t *oh, int m, int n, int nnz, double alpha, const double *A,
-                          const int *colind, const int *rowptr, const double *x, double beta,
-                          double *y) {
+// A: the matrix values, or (memMaskSplitA != 0) the slices returned by csrmv_split_a
+template <class T>
+static int32_t csrmv_impl(ozblasHandle_t *oh, int m, int n, int nnz, T alpha, const void *A,
+                          const int *colind, const int *rowptr, const T *x, T beta, T *y) {
 	Backend *b = be(oh);
 	if (oh->sumMode != 0 && oh->sumMode != 1) return OZBLAS_STATUS_NOT_SUPPORTED;
 	const uint64_t base = (oh->memMaskSplitA != 0) ? b->keep : 0;
@@ -841,7 +938,7 @@ static int32_t csrmv_impl(ozblasHandle_t *oh, int m, int n, int nnz, double alph
 		as.count = oh->nSplitA_;
 	} else {
 		oh->splitShift = 1;
-		OZ_TRY(csr_split(oh, m, nnz, A, rowptr, as));
+		OZ_TRY(csr_split(oh, m, nnz, (const T *)A, rowptr, as));
 	}
 	oh->splitShift = 1;  // the vector split always uses splitShift = 1 (counterInit)
 	VecSplit xs;
@@ -854,17 +951,48 @@ static int32_t csrmv_impl(ozblasHandle_t *oh, int m, int n, int nnz, double alph
 	wreset(oh, base);
 	return OZBLAS_STATUS_SUCCESS;
 }
+
+template <class T>
+static int32_t csrmv_entry(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, int32_t nnz,
+                           T alpha, const T *A, const int32_t *colind, const int32_t *rowptr,
+                           const T *x, T beta, T *y) {
+	OZ_TRY(ensure_init(oh));
+	if (!is_n(transA)) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	if (oh->reproMode == 0 && oh->nSplitMax == 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	return oh->lastStatus = csrmv_impl(oh, m, n, nnz, alpha, (const void *)A, colind, rowptr, x, beta, y);
+}
 }  // namespace oz
+
+extern "C" double *ozblasDcsrmvSplitA(ozblasHandle_t *oh, char transA, int32_t m, int32_t n,
+                                      int32_t nnz, const void *descrA, const double *A,
+                                      const int32_t *rowptr) {
+	(void)descrA;
+	(void)n;
+	return csrmv_split_a(oh, transA, m, nnz, A, rowptr);
+}
+
+extern "C" float *ozblasScsrmvSplitA(ozblasHandle_t *oh, char transA, int32_t m, int32_t n,
+                                     int32_t nnz, const void *descrA, const float *A,
+                                     const int32_t *rowptr) {
+	(void)descrA;
+	(void)n;
+	return (float *)csrmv_split_a(oh, transA, m, nnz, A, rowptr);
+}
 
 extern "C" int32_t ozblasDcsrmv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, int32_t nnz,
                                 double alpha, const void *descrA, const double *A,
                                 const int32_t *colind, const int32_t *rowptr, const double *x,
                                 double beta, double *y) {
 	(void)descrA;
-	OZ_TRY(ensure_init(oh));
-	if (!is_n(transA)) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
-	if (oh->reproMode == 0 && oh->nSplitMax == 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
-	return oh->lastStatus = csrmv_impl(oh, m, n, nnz, alpha, A, colind, rowptr, x, beta, y);
+	return csrmv_entry(oh, transA, m, n, nnz, alpha, A, colind, rowptr, x, beta, y);
+}
+
+extern "C" int32_t ozblasScsrmv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, int32_t nnz,
+                                float alpha, const void *descrA, const float *A,
+                                const int32_t *colind, const int32_t *rowptr, const float *x,
+                                float beta, float *y) {
+	(void)descrA;
+	return csrmv_entry(oh, transA, m, n, nnz, alpha, A, colind, rowptr, x, beta, y);
 }
 
 // CG (src/cpu/ozblas_cg.cpp:24-128).  The set-up (pre-split of A, |b|, r0 = b - A x0, <r0,r0>)
@@ -874,28 +1002,31 @@ extern "C" int32_t ozblasDcsrmv(ozblasHandle_t *oh, char transA, int32_t m, int3
 // synchronous path with the scalar recurrences on the host (cg_host_loop); both give the same bits.
 namespace oz {
 
+template <class T>
 struct CgVecs {
-	double *R = nullptr, *P = nullptr, *Q = nullptr, *X0 = nullptr;
+	T *R = nullptr, *P = nullptr, *Q = nullptr, *X0 = nullptr;
 };
 
+// T = float: the <float,float> instantiation (float scalars, the double sqrt rounded to float)
+template <class T>
 static int32_t cg_host_loop(ozblasHandle_t *oh, int n, int nnz, const double *AS, const int *colind,
-                            const int *rowptr, double *x, int maxiter, double tol, double nrmb,
-                            double dold, const CgVecs &v, double *hist) {
+                            const int *rowptr, T *x, int maxiter, T tol, T nrmb, T dold,
+                            const CgVecs<T> &v, double *hist) {
 	hipStream_t st = be(oh)->stream;
-	double tmp, alpha, dnew, resi, beta;
+	T tmp, alpha, dnew, resi, beta;
 	int vcnt = 1;
 	while (oh->cg_numiter < maxiter) {
 		oh->cg_numiter++;
-		OZ_TRY(csrmv_impl(oh, n, n, nnz, 1.0, AS, colind, rowptr, v.P, 0.0, v.Q));
-		OZ_TRY(ozblasDdot(oh, n, v.P, 1, v.Q, 1, &tmp));
+		OZ_TRY(csrmv_impl(oh, n, n, nnz, (T)1, (const void *)AS, colind, rowptr, v.P, (T)0, v.Q));
+		OZ_TRY(dot_entry(oh, n, v.P, 1, v.Q, 1, &tmp));
 		alpha = dold / tmp;
 		launch_axpy_host(st, n, alpha, v.P, x, oh->reproMode != 0);
 		launch_axpy_host(st, n, -alpha, v.Q, v.R, oh->reproMode != 0);
-		OZ_TRY(ozblasDdot(oh, n, v.R, 1, v.R, 1, &dnew));
-		resi = sqrt(dnew);
+		OZ_TRY(dot_entry(oh, n, v.R, 1, v.R, 1, &dnew));
+		resi = (T)sqrt((double)dnew);
 		beta = dnew / dold;
 		dold = dnew;
-		launch_scal_axpy(st, n, beta, v.R, v.P);
+		launch_scal_axpy(st, n, beta, v.R, v.P, oh->reproMode != 0);
 		if (oh->verbose > 0) {
 			if (resi / nrmb < tol || oh->cg_numiter % oh->verbose == 0) {
 				if (hist) hist[vcnt] = resi / nrmb;
@@ -911,7 +1042,7 @@ static int32_t cg_host_loop(ozblasHandle_t *oh, int n, int nnz, const double *AS
 // Device-resident iterations; *overflow = 1 when the host path must redo them.
 static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *AS, const int *colind,
                               const int *rowptr, double *x, int maxiter, double tol, double nrmb,
-                              double dold, const CgVecs &v, double *hist, int *overflow) {
+                              double dold, const CgVecs<double> &v, double *hist, int *overflow) {
 	Backend *b = be(oh);
 	hipStream_t st = b->stream;
 	const int cap = split_cap(oh);
@@ -978,12 +1109,12 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 	return OZBLAS_STATUS_SUCCESS;
 }
 
-}  // namespace oz
-
-extern "C" int32_t ozblasDcg(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz,
-                             const void *descrA, const double *A, const int32_t *colind,
-                             const int32_t *rowptr, const double *b_, double *x, int32_t maxiter,
-                             double tol) {
+// binary64 iterates device-resident where it can; binary32 iterates on the synchronous path
+template <class T>
+static int32_t cg_entry(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz, const T *A,
+                        const int32_t *colind, const int32_t *rowptr, const T *b_, T *x,
+                        int32_t maxiter, T tol) {
+	constexpr size_t E = sizeof(T);
 	OZ_TRY(ensure_init(oh));
 	if (!is_n(transA)) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
 	if (oh->sumMode != 0 && oh->sumMode != 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
@@ -992,70 +1123,75 @@ extern "C" int32_t ozblasDcg(ozblasHandle_t *oh, char transA, int32_t n, int32_t
 	double *hist = (double *)oh->cg_verbose1;
 	oh->cg_numiter = 0;
 	oh->cgDevicePath = 0;
-	CgVecs v;
-	OZ_HIP(hipMalloc(&v.R, 8ull * n));
-	OZ_HIP(hipMalloc(&v.P, 8ull * n));
-	OZ_HIP(hipMalloc(&v.Q, 8ull * n));
-	OZ_HIP(hipMalloc(&v.X0, 8ull * n));
+	CgVecs<T> v;
+	OZ_HIP(hipMalloc(&v.R, E * n));
+	OZ_HIP(hipMalloc(&v.P, E * n));
+	OZ_HIP(hipMalloc(&v.Q, E * n));
+	OZ_HIP(hipMalloc(&v.X0, E * n));
 	int32_t s = OZBLAS_STATUS_SUCCESS;
-	double nrmb = 0, dold = 0, resi = 0;
+	T nrmb = 0, dold = 0, resi = 0;
 	int overflow = 1;
 	double *AS = nullptr;
 	// q starts finite: the first product reads it through beta * q with beta = 0
-	if (hipMemsetAsync(v.Q, 0, 8ull * n, st) != hipSuccess ||
-	    hipMemcpyAsync(v.X0, x, 8ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+	if (hipMemsetAsync(v.Q, 0, E * n, st) != hipSuccess ||
+	    hipMemcpyAsync(v.X0, x, E * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
 		s = OZBLAS_STATUS_EXECUTION_FAILED;
 		goto done;
 	}
-	AS = ozblasDcsrmvSplitA(oh, transA, n, n, nnz, descrA, A, rowptr);
+	AS = csrmv_split_a(oh, transA, n, nnz, A, rowptr);
 	if (!AS) {
 		s = oh->lastStatus;
 		goto done;
 	}
 	b->nnzA = nnz;
-	s = ozblasDnrm2(oh, n, b_, 1, &nrmb);
-	if (s) goto done;
-	if (hipMemcpyAsync(v.R, b_, 8ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+	{  // ozblasRnrm2 (src/cpu/ozblas_nrm2.cpp:27: the double sqrt, rounded to T)
+		T d = 0;
+		s = dot_entry(oh, n, b_, 1, b_, 1, &d);
+		if (s) goto done;
+		nrmb = (T)sqrt((double)d);
+	}
+	if (hipMemcpyAsync(v.R, b_, E * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
 		s = OZBLAS_STATUS_EXECUTION_FAILED;
 		goto done;
 	}
-	s = csrmv_impl(oh, n, n, nnz, -1.0, AS, colind, rowptr, x, 1.0, v.R);
+	s = csrmv_impl(oh, n, n, nnz, (T)-1, (const void *)AS, colind, rowptr, x, (T)1, v.R);
 	if (s) goto done;
-	s = ozblasDdot(oh, n, v.R, 1, v.R, 1, &dold);
+	s = dot_entry(oh, n, v.R, 1, v.R, 1, &dold);
 	if (s) goto done;
-	resi = sqrt(dold);
+	resi = (T)sqrt((double)dold);
 	if (oh->verbose > 0 && hist) hist[0] = resi / nrmb;
-	if (hipMemcpyAsync(v.P, v.R, 8ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+	if (hipMemcpyAsync(v.P, v.R, E * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
 		s = OZBLAS_STATUS_EXECUTION_FAILED;
 		goto done;
 	}
 	{
-		bool devicePath = oh->cgDeviceSlices >= 0 && oh->reproMode != 0 && oh->fastMode == 0 &&
-		                        oh->splitEpsMode != 2 &&
-		                        oh->nSplitA_ <= 4 && maxiter > 0;
-		if (devicePath) {
-			wreset(oh, b->keep);
-			s = cg_device_loop(oh, n, oh->nSplitA_, AS, colind, rowptr, x, maxiter, tol, nrmb, dold, v,
-			                   hist, &overflow);
-			wreset(oh, b->keep);
-			if (s == OZBLAS_STATUS_WORKSPACE) {  // nothing ran: iterate on the synchronous path
-				s = OZBLAS_STATUS_SUCCESS;
-				overflow = 1;
-				devicePath = false;
+		bool devicePath = E == 8 && oh->cgDeviceSlices >= 0 && oh->reproMode != 0 &&
+		                  oh->fastMode == 0 && oh->splitEpsMode != 2 && oh->nSplitA_ <= 4 && maxiter > 0;
+		if constexpr (E == 8) {
+			if (devicePath) {
+				wreset(oh, b->keep);
+				s = cg_device_loop(oh, n, oh->nSplitA_, AS, colind, rowptr, x, maxiter, tol, nrmb, dold, v,
+				                   hist, &overflow);
+				wreset(oh, b->keep);
+				if (s == OZBLAS_STATUS_WORKSPACE) {  // nothing ran: iterate on the synchronous path
+					s = OZBLAS_STATUS_SUCCESS;
+					overflow = 1;
+					devicePath = false;
+				}
+				if (s) goto done;
 			}
-			if (s) goto done;
 		}
 		oh->cgDevicePath = !devicePath ? 0 : (overflow ? 2 : 1);
 		if (overflow) {  // synchronous path (from the saved set-up state when the device path gave up)
 			if (devicePath) {
-				if (hipMemcpyAsync(x, v.X0, 8ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess ||
-				    hipMemcpyAsync(v.R, b_, 8ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+				if (hipMemcpyAsync(x, v.X0, E * n, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+				    hipMemcpyAsync(v.R, b_, E * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
 					s = OZBLAS_STATUS_EXECUTION_FAILED;
 					goto done;
 				}
-				s = csrmv_impl(oh, n, n, nnz, -1.0, AS, colind, rowptr, x, 1.0, v.R);
+				s = csrmv_impl(oh, n, n, nnz, (T)-1, (const void *)AS, colind, rowptr, x, (T)1, v.R);
 				if (s) goto done;
-				if (hipMemcpyAsync(v.P, v.R, 8ull * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+				if (hipMemcpyAsync(v.P, v.R, E * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
 					s = OZBLAS_STATUS_EXECUTION_FAILED;
 					goto done;
 				}
@@ -1075,4 +1211,22 @@ done:
 	oh->splitShift = 1;
 	wreset(oh, 0);
 	return oh->lastStatus = s;
+}
+
+}  // namespace oz
+
+extern "C" int32_t ozblasDcg(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz,
+                             const void *descrA, const double *A, const int32_t *colind,
+                             const int32_t *rowptr, const double *b, double *x, int32_t maxiter,
+                             double tol) {
+	(void)descrA;
+	return cg_entry(oh, transA, n, nnz, A, colind, rowptr, b, x, maxiter, tol);
+}
+
+extern "C" int32_t ozblasScg(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz,
+                             const void *descrA, const float *A, const int32_t *colind,
+                             const int32_t *rowptr, const float *b, float *x, int32_t maxiter,
+                             float tol) {
+	(void)descrA;
+	return cg_entry(oh, transA, n, nnz, A, colind, rowptr, b, x, maxiter, tol);
 }

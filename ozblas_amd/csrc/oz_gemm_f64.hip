@@ -13,6 +13,9 @@
 // k_gemm_f64<true> : fused Fsum -- the pairs are walked in the reference's (level, ia, ib) order
 //                    and t += product is accumulated in registers, then C = fma(alpha, t, beta*C)
 //                    (src/cpu/ozblas_sum.cpp:354-378), so nothing is materialised.
+// TC = float serves the binary32 instantiation: its slices (stored as doubles) give exact FP64
+// products that round once to float, as the reference's float slice GEMM; the Fsum then adds in
+// float and the epilogue is fma1<float> (the double fma, rounded to float).
 #include "oz_kernels.hpp"
 
 namespace oz {
@@ -54,22 +57,23 @@ __device__ inline void f64_store_lds(const double (&reg)[8], double *s) {
 	for (int q = 0; q < 8; q++) dst[q] = reg[q];
 }
 
-template <bool FSUM>
+template <bool FSUM, class TC>
 __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const double *A, int64_t lda,
-                                                  const double *B, int64_t ldb, double *C,
+                                                  const double *B, int64_t ldb, TC *C,
                                                   int64_t ldc, const int64_t *aOff,
                                                   const int64_t *bOff, const int64_t *cOff,
-                                                  int npairs, double alpha, double beta,
-                                                  int tilesM) {
+                                                  int npairs, TC alpha, TC beta, int tilesM) {
 	__shared__ double sA[FBM * FLD];
 	__shared__ double sB[FBN * FLD];
 	const int i0 = (blockIdx.x % tilesM) * FBM, j0 = (blockIdx.x / tilesM) * FBN;
 	const int w = threadIdx.x / 64, l = threadIdx.x % 64;
 	const int wm = (w % 2) * 64, wn = (w / 2) * 64;
-	d4v acc[4][4], tot[4][4];
+	d4v acc[4][4];
+	TC tot[4][4][4];
 	const int pbeg = FSUM ? 0 : blockIdx.y, pend = FSUM ? npairs : blockIdx.y + 1;
 	for (int a = 0; a < 4; a++)
-		for (int b = 0; b < 4; b++) tot[a][b] = d4v{0.0, 0.0, 0.0, 0.0};
+		for (int b = 0; b < 4; b++)
+			for (int r = 0; r < 4; r++) tot[a][b][r] = (TC)0;
 	for (int p = pbeg; p < pend; p++) {
 		const double *Ap = A + aOff[p];
 		const double *Bp = B + bOff[p];
@@ -105,41 +109,66 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const dou
 		if (FSUM) {
 			for (int a = 0; a < 4; a++)
 				for (int b = 0; b < 4; b++)
-					for (int r = 0; r < 4; r++) tot[a][b][r] = tot[a][b][r] + acc[a][b][r];
+					for (int r = 0; r < 4; r++) tot[a][b][r] = tot[a][b][r] + (TC)acc[a][b][r];
 		}
 	}
-	double *Cp = FSUM ? C : C + cOff[blockIdx.y];
+	TC *Cp = FSUM ? C : C + cOff[blockIdx.y];
 	for (int mi = 0; mi < 4; mi++)
 		for (int nj = 0; nj < 4; nj++)
 			for (int r = 0; r < 4; r++) {
 				const int i = i0 + wm + 16 * mi + F64_Q(l, r);
 				const int j = j0 + wn + 16 * nj + F64_P(l, r);
 				if (i < M && j < N) {
-					double *c = Cp + (int64_t)j * ldc + i;
+					TC *c = Cp + (int64_t)j * ldc + i;
 					if (FSUM)
-						*c = fma(alpha, tot[mi][nj][r], beta * *c);
+						*c = fma1_t(alpha, tot[mi][nj][r], beta * *c);
 					else
-						*c = acc[mi][nj][r];
+						*c = (TC)acc[mi][nj][r];
 				}
 			}
+}
+
+template <class TC>
+static void gemm_f64_batched(hipStream_t st, int M, int N, int K, const double *A, int64_t lda,
+                             const double *B, int64_t ldb, TC *C, int64_t ldc, const int64_t *aOff,
+                             const int64_t *bOff, const int64_t *cOff, int batch) {
+	const int tilesM = (M + FBM - 1) / FBM, tilesN = (N + FBN - 1) / FBN;
+	hipLaunchKernelGGL((k_gemm_f64<false, TC>), dim3(tilesM * tilesN, batch), dim3(256), 0, st, M, N,
+	                   K, A, lda, B, ldb, C, ldc, aOff, bOff, cOff, batch, (TC)1, (TC)0, tilesM);
+}
+
+template <class TC>
+static void gemm_f64_fsum(hipStream_t st, int M, int N, int K, const double *A, int64_t lda,
+                          const double *B, int64_t ldb, const int64_t *aOff, const int64_t *bOff,
+                          int npairs, TC alpha, TC beta, TC *C, int64_t ldc) {
+	const int tilesM = (M + FBM - 1) / FBM, tilesN = (N + FBN - 1) / FBN;
+	hipLaunchKernelGGL((k_gemm_f64<true, TC>), dim3(tilesM * tilesN, 1), dim3(256), 0, st, M, N, K, A,
+	                   lda, B, ldb, C, ldc, aOff, bOff, (const int64_t *)nullptr, npairs, alpha,
+	                   beta, tilesM);
 }
 
 void launch_gemm_f64_batched(hipStream_t st, int M, int N, int K, const double *A, int64_t lda,
                              const double *B, int64_t ldb, double *C, int64_t ldc,
                              const int64_t *aOff, const int64_t *bOff, const int64_t *cOff,
                              int batch) {
-	const int tilesM = (M + FBM - 1) / FBM, tilesN = (N + FBN - 1) / FBN;
-	hipLaunchKernelGGL(k_gemm_f64<false>, dim3(tilesM * tilesN, batch), dim3(256), 0, st, M, N, K,
-	                   A, lda, B, ldb, C, ldc, aOff, bOff, cOff, batch, 1.0, 0.0, tilesM);
+	gemm_f64_batched(st, M, N, K, A, lda, B, ldb, C, ldc, aOff, bOff, cOff, batch);
+}
+void launch_gemm_f64_batched(hipStream_t st, int M, int N, int K, const double *A, int64_t lda,
+                             const double *B, int64_t ldb, float *C, int64_t ldc,
+                             const int64_t *aOff, const int64_t *bOff, const int64_t *cOff,
+                             int batch) {
+	gemm_f64_batched(st, M, N, K, A, lda, B, ldb, C, ldc, aOff, bOff, cOff, batch);
 }
 
 void launch_gemm_f64_fsum(hipStream_t st, int M, int N, int K, const double *A, int64_t lda,
                           const double *B, int64_t ldb, const int64_t *aOff, const int64_t *bOff,
                           int npairs, double alpha, double beta, double *C, int64_t ldc) {
-	const int tilesM = (M + FBM - 1) / FBM, tilesN = (N + FBN - 1) / FBN;
-	hipLaunchKernelGGL(k_gemm_f64<true>, dim3(tilesM * tilesN, 1), dim3(256), 0, st, M, N, K, A,
-	                   lda, B, ldb, C, ldc, aOff, bOff, (const int64_t *)nullptr, npairs, alpha,
-	                   beta, tilesM);
+	gemm_f64_fsum(st, M, N, K, A, lda, B, ldb, aOff, bOff, npairs, alpha, beta, C, ldc);
+}
+void launch_gemm_f64_fsum(hipStream_t st, int M, int N, int K, const double *A, int64_t lda,
+                          const double *B, int64_t ldb, const int64_t *aOff, const int64_t *bOff,
+                          int npairs, float alpha, float beta, float *C, int64_t ldc) {
+	gemm_f64_fsum(st, M, N, K, A, lda, B, ldb, aOff, bOff, npairs, alpha, beta, C, ldc);
 }
 
 }  // namespace oz

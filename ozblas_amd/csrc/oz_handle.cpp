@@ -38,6 +38,50 @@ const uint16_t *tau_table_host() {
 	return g_tau;
 }
 
+// binary32 tables: for each binade 2^e the number of floats above 2^e for which the rule gives e
+static uint16_t g_tauf[kTauBinadesF], g_tauf2[kTauBinadesF];
+static std::once_flag g_tauf_once;
+
+static void build_tau_tables_f() {
+	for (int e = -149; e <= 127; e++) {
+		const float base = ldexpf(1.0f, e);
+		const float u = ldexpf(1.0f, e - 23 > -149 ? e - 23 : -149);
+		const uint32_t nvals = (e - 23 >= -149) ? ((1u << 23) - 1) : ((1u << (e + 149)) - 1);
+		uint32_t i = 1;
+		for (; i <= nvals && i < 65535; i++) {  // split rule: (float)log2((double)x)
+			const float x = base + (float)i * u;
+			if (ceil((double)(float)log2((double)x)) != (double)e) break;
+		}
+		g_tauf[e + kTauBiasF] = (uint16_t)(i - 1);
+		for (i = 1; i <= nvals && i < 65535; i++) {  // NearSum rule: log2f(x)
+			const float x = base + (float)i * u;
+			if (ceil((double)log2f(x)) != (double)e) break;
+		}
+		g_tauf2[e + kTauBiasF] = (uint16_t)(i - 1);
+	}
+}
+
+const uint16_t *tau_table_host_f() {
+	std::call_once(g_tauf_once, build_tau_tables_f);
+	return g_tauf;
+}
+const uint16_t *tau_table_host_f2() {
+	std::call_once(g_tauf_once, build_tau_tables_f);
+	return g_tauf2;
+}
+
+// getRho<float,float> (src/cpu/ozblas_split.cpp:20-34 with getEpse = 24, getEpse2 = 53)
+int rho_host_f(int dim, int splitEpsMode) {
+	double r;
+	switch (splitEpsMode) {
+	case 2: r = ceil((24 - (53 - log2(1. * dim)) / 2)); break;
+	case 9: r = ceil((24 - (24 - log2(2. * sqrt((double)dim))) / 2)); break;
+	default: r = ceil((24 - (24 - log2(1. * dim)) / 2)); break;
+	}
+	if (!(r >= -2147483648.0 && r < 2147483648.0)) return INT32_MIN;
+	return (int)r;
+}
+
 // getRho<double,double> (src/cpu/ozblas_split.cpp:20-34), evaluated with the host libm
 int rho_host(int dim, int splitEpsMode) {
 	double r;
@@ -76,6 +120,12 @@ extern "C" int32_t ozblasCreate(ozblasHandle_t *oh, uint64_t workSizeBytes) {
 	OZ_HIP(hipMalloc(&b->dTau, sizeof(uint16_t) * kTauBinades));
 	OZ_HIP(hipMemcpy(b->dTau, tau_table_host(), sizeof(uint16_t) * kTauBinades,
 	                 hipMemcpyHostToDevice));
+	OZ_HIP(hipMalloc(&b->dTauF, sizeof(uint16_t) * kTauBinadesF));
+	OZ_HIP(hipMemcpy(b->dTauF, tau_table_host_f(), sizeof(uint16_t) * kTauBinadesF,
+	                 hipMemcpyHostToDevice));
+	OZ_HIP(hipMalloc(&b->dTauF2, sizeof(uint16_t) * kTauBinadesF));
+	OZ_HIP(hipMemcpy(b->dTauF2, tau_table_host_f2(), sizeof(uint16_t) * kTauBinadesF,
+	                 hipMemcpyHostToDevice));
 	OZ_HIP(hipHostMalloc(&b->hPinned, 64 * sizeof(int64_t), hipHostMallocDefault));
 	OZ_HIP(hipMalloc(&b->dCounters, 256 * sizeof(int32_t)));
 	for (auto &e : b->ev) OZ_HIP(hipEventCreate(&e));
@@ -110,6 +160,8 @@ extern "C" int32_t ozblasDestroy(ozblasHandle_t *oh) {
 		if (e) (void)hipEventDestroy(e);
 	(void)hipFree(b->work);
 	(void)hipFree(b->dTau);
+	(void)hipFree(b->dTauF);
+	(void)hipFree(b->dTauF2);
 	(void)hipFree(b->dCounters);
 	(void)hipHostFree(b->hPinned);
 	delete b;
@@ -157,6 +209,13 @@ extern "C" int32_t ozblas_internal_rho(int32_t dim, int32_t splitEpsMode) {
 	return rho_host(dim, splitEpsMode);
 }
 extern "C" int32_t ozblas_internal_rho_row(int32_t dim) { return rho_of_dim(dim); }
+// binary32 rules: tau of the split, the NearSum exponent rule, rho (libm) and its integer form
+extern "C" int32_t ozblas_internal_tau_f(float max) { return tau_of_f(max, tau_table_host_f()); }
+extern "C" int32_t ozblas_internal_nearsum_exp_f(float mu) {
+	return ceil_log2_tab_f(mu, tau_table_host_f2());
+}
+extern "C" int32_t ozblas_internal_rho_f(int32_t dim, int32_t mode) { return rho_host_f(dim, mode); }
+extern "C" int32_t ozblas_internal_rho_row_f(int32_t dim, int32_t mode) { return rho_dev_f(dim, mode); }
 extern "C" int32_t ozblas_internal_rho_row_mode(int32_t dim, int32_t mode) { return rho_dev(dim, mode); }
 extern "C" void ozblas_internal_tau_batch(int64_t n, const double *x, int32_t *out) {
 	const uint16_t *J = tau_table_host();

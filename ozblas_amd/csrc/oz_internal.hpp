@@ -50,6 +50,8 @@ struct Backend {
 	hipEvent_t ev[8] = {};
 	void *batch = nullptr;  // cached plan of ozblasDgemmBatched (oz_batched.hip)
 	int nnzA = 0;           // nonzeros of the matrix pre-split for CG (slice pitch rup(nnz, 32))
+	uint16_t *dTauF = nullptr;   // device copies of the binary32 tables JF, JF2
+	uint16_t *dTauF2 = nullptr;
 };
 
 void batch_plan_free(void *plan);
@@ -109,6 +111,58 @@ __host__ __device__ inline int tau_of(double max, const uint16_t *J) {
 	return ceil_log2_tab(x, J);
 }
 
+// ---------------------------------------------------------------------------------------------
+// binary32: the reference's <float,float> split takes tau = (short)ceil(log21(max)) with
+// log21<float>(v) = (float)log2((double)v) (src/cpu/ozblas_aux.cpp:37-42), and its NearSum takes
+// ceil(std::log2(mu)), i.e. log2f, for a float mu (src/cpu/ozblas_sum.cpp:35).  Both round log2 to
+// float, so ceil gives e for the first few floats above 2^e; tables JF (split rule) and JF2
+// (NearSum rule), indexed by e + 149, hold those counts, measured on the host libm.
+// ---------------------------------------------------------------------------------------------
+constexpr int kTauBinadesF = 277;  // binary32 binades 2^-149 .. 2^127
+constexpr int kTauBiasF = 149;
+
+__host__ __device__ inline uint32_t f2u(float x) { return __builtin_bit_cast(uint32_t, x); }
+
+// x > 0 and finite
+__host__ __device__ inline int ceil_log2_tab_f(float x, const uint16_t *J) {
+	const uint32_t b = f2u(x);
+	const int E = (int)((b >> 23) & 0xff);
+	const uint32_t f = b & ((1u << 23) - 1);
+	int e;
+	uint32_t off;
+	if (E == 0) {  // subnormal: x = f * 2^-149
+		const int msb = 31 - __builtin_clz(f);
+		e = -149 + msb;
+		off = f - (1u << msb);
+	} else {
+		e = E - 127;
+		off = f;
+	}
+	if (off == 0) return e;
+	return (off <= (uint32_t)J[e + kTauBiasF]) ? e : e + 1;
+}
+
+// (short)ceil(log21(fabs1(max))) for a float max: 0 for zero, NaN and Inf
+__host__ __device__ inline int tau_of_f(float max, const uint16_t *JF) {
+	const float x = fabsf(max);
+	if (!(x > 0.0f) || !(x <= 3.40282347e38f)) return 0;
+	return ceil_log2_tab_f(x, JF);
+}
+
+// rho of getRho<float,float> (getEpse = 24, getEpse2 = 53) from integers only, as rho_dev:
+// mode 0: ceil(24 - (24 - log2 d) / 2) = ceil((24 + log2 d) / 2); mode 9: ceil(12.5 + log2(d) / 4)
+// = 12 + ceil((2 + q) / 4); mode 2: ceil((log2(d) - 5) / 2) = ceil((q - 5) / 2); q = ceil(log2 d)
+__host__ __device__ inline int rho_dev_f(int dim, int mode) {
+	if (dim <= 0) return INT32_MIN;
+	const int p = 31 - __builtin_clz((unsigned)dim);
+	const bool pow2 = (dim & (dim - 1)) == 0;
+	const int q = pow2 ? p : p + 1;
+	if (mode == 2) return (q - 5 >= 0) ? (q - 5 + 1) / 2 : -((5 - q) / 2);
+	if (mode == 9) return 12 + (q + 2 + 3) / 4;
+	if (pow2) return (24 + p + 1) / 2;
+	return (p & 1) ? (25 + p) / 2 : (26 + p) / 2;
+}
+
 // std::max semantics (NaN never replaces the running maximum)
 __host__ __device__ inline double std_max(double a, double b) { return (a < b) ? b : a; }
 
@@ -147,6 +201,52 @@ __host__ __device__ inline int rho_dev(int dim, int mode) {
 
 // host helpers (oz_handle.cpp)
 int rho_host(int dim, int splitEpsMode);
+int rho_host_f(int dim, int splitEpsMode);  // getRho<float,float> with the host libm
+
+// fma1<float> (src/cpu/ozblas_aux.cpp:4-9): the double fma of the promoted operands, rounded to float
+__host__ __device__ inline float fma1_f(float a, float b, float c) {
+	return (float)fma((double)a, (double)b, (double)c);
+}
+__host__ __device__ inline double fma1_t(double a, double b, double c) { return fma(a, b, c); }
+__host__ __device__ inline float fma1_t(float a, float b, float c) { return fma1_f(a, b, c); }
+
+// Rules of the split per element type: the reference's <double,double> and <float,float>
+// instantiations of src/cpu/ozblas_split.cpp.  Slices are always stored as doubles (a binary32
+// slice converts exactly), so one set of FP64 product kernels serves both.
+//   kPrec     significand bits (the unit of sigma is 2^(rho + tau - kPrec))
+//   kSigEmax  largest rho + tau for which 2^(rho+tau) and sigma stay finite (with margin)
+//   kEmin     exponent of the smallest normal number
+template <class T>
+struct SplitRule;
+template <>
+struct SplitRule<double> {
+	static constexpr int kPrec = 53, kSigEmax = 1022, kEmin = -1022;
+	__host__ __device__ static int tau(double mx, const uint16_t *J) { return tau_of(mx, J); }
+	// CONST * scalbn1(1, rho + tau) * splitShift (src/cpu/ozblas_split.cpp:88,159)
+	__host__ __device__ static double sigma(int rho, int tau, int shift) {
+		return split_sigma(rho, tau, shift);
+	}
+	// CONST * scalbn1(1, rho + tau) / splitShift (the sparse split, src/cpu/ozblas_split.cpp:809)
+	__host__ __device__ static double sigma_div(int rho, int tau, int shift) {
+		return 0.75 * scalbn_clamped(1.0, (int64_t)rho + tau) / (double)shift;
+	}
+	__host__ __device__ static int rho_row(int dim, int mode) { return rho_dev(dim, mode); }
+};
+template <>
+struct SplitRule<float> {
+	static constexpr int kPrec = 24, kSigEmax = 126, kEmin = -126;
+	__host__ __device__ static int tau(double mx, const uint16_t *JF) { return tau_of_f((float)mx, JF); }
+	// scalbn1<float> rounds 2^(rho+tau) to float; CONST * p * shift is a double, rounded to float
+	__host__ __device__ static float sigma(int rho, int tau, int shift) {
+		const float p = (float)scalbn_clamped(1.0, (int64_t)rho + tau);
+		return (float)(0.75 * (double)p * (double)shift);
+	}
+	__host__ __device__ static float sigma_div(int rho, int tau, int shift) {
+		const float p = (float)scalbn_clamped(1.0, (int64_t)rho + tau);
+		return (float)(0.75 * (double)p / (double)shift);
+	}
+	__host__ __device__ static int rho_row(int dim, int mode) { return rho_dev_f(dim, mode); }
+};
 
 // cap on slices of the reference split loop (src/cpu/ozblas_split.cpp:275-277)
 inline int split_cap(const ozblasHandle_t *oh) {
@@ -181,6 +281,8 @@ inline bool fast_valid(const ozblasHandle_t *oh, const ExpStatsH &a, const ExpSt
 }
 
 const uint16_t *tau_table_host();
+const uint16_t *tau_table_host_f();   // JF: the binary32 split rule
+const uint16_t *tau_table_host_f2();  // JF2: the binary32 NearSum rule (log2f)
 int32_t ensure_init(ozblasHandle_t *oh);
 
 }  // namespace oz

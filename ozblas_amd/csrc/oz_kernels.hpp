@@ -9,18 +9,27 @@ namespace oz {
 void launch_split_cols(hipStream_t st, int m, int n, const double *in, int64_t ldi, double *out,
                        int64_t ldo, double *split, int64_t lds, int64_t sstride, int rho,
                        int shift, int cap, const uint16_t *J, int *zcol, int *zmax);
+void launch_split_cols(hipStream_t st, int m, int n, const float *in, int64_t ldi, float *out,
+                       int64_t ldo, double *split, int64_t lds, int64_t sstride, int rho,
+                       int shift, int cap, const uint16_t *JF, int *zcol, int *zmax);
 void launch_zero_slots(hipStream_t st, int m, int n, double *split, int64_t lds, int64_t sstride,
                        const int *zcol, int count);
 void launch_transpose(hipStream_t st, int rows, int cols, const double *in, int64_t ldi,
                       double *out, int64_t ldo);
+void launch_transpose(hipStream_t st, int rows, int cols, const float *in, int64_t ldi, float *out,
+                      int64_t ldo);
 void launch_vec_max(hipStream_t st, int m, const double *x, unsigned long long *slot);
 void launch_split_vec_pass(hipStream_t st, int m, const double *src, double *rem, double *slice,
                            int rho, int shift, const uint16_t *J, const unsigned long long *maxIn,
                            unsigned long long *maxOut);
 void launch_fill(hipStream_t st, double *p, int64_t n, double v);
+void launch_fill(hipStream_t st, float *p, int64_t n, float v);
 void launch_copy(hipStream_t st, const double *src, double *dst, int64_t n);
 void launch_split_csr(hipStream_t st, int m, const double *val, const int *rowptr, double *work,
                       double *split, int64_t lds, int shift, int cap, const uint16_t *J, int *zrow,
+                      int *zmax, int epsMode);
+void launch_split_csr(hipStream_t st, int m, const float *val, const int *rowptr, float *work,
+                      double *split, int64_t lds, int shift, int cap, const uint16_t *JF, int *zrow,
                       int *zmax, int epsMode);
 void launch_zero_slots_csr(hipStream_t st, int m, const int *rowptr, double *split, int64_t lds,
                            const int *zrow, int count);
@@ -40,6 +49,14 @@ void launch_gemm_f64_batched(hipStream_t st, int M, int N, int K, const double *
 void launch_gemm_f64_fsum(hipStream_t st, int M, int N, int K, const double *A, int64_t lda,
                           const double *B, int64_t ldb, const int64_t *aOff, const int64_t *bOff,
                           int npairs, double alpha, double beta, double *C, int64_t ldc);
+// binary32 outputs: products rounded once to float; the Fsum adds in float, epilogue fma1<float>
+void launch_gemm_f64_batched(hipStream_t st, int M, int N, int K, const double *A, int64_t lda,
+                             const double *B, int64_t ldb, float *C, int64_t ldc,
+                             const int64_t *aOff, const int64_t *bOff, const int64_t *cOff,
+                             int batch);
+void launch_gemm_f64_fsum(hipStream_t st, int M, int N, int K, const double *A, int64_t lda,
+                          const double *B, int64_t ldb, const int64_t *aOff, const int64_t *bOff,
+                          int npairs, float alpha, float beta, float *C, int64_t ldc);
 
 // ---- sums (oz_sum.hip)
 void launch_global_nearsum(hipStream_t st, int m, int n, int ic, double *Cs, int64_t llsc,
@@ -48,6 +65,13 @@ void launch_global_nearsum(hipStream_t st, int m, int n, int ic, double *Cs, int
 void launch_global_fsum(hipStream_t st, int m, int n, int nA, int nB, int maxlevel, int sumOrder,
                         const double *Cs, int64_t llsc, int64_t ldsc, double *C, int64_t ldc,
                         double alpha, double beta);
+// binary32: NearSum<float> (JF2: the log2f table) / Fsum in float, epilogue fma1<float>
+void launch_global_nearsum(hipStream_t st, int m, int n, int ic, float *Cs, int64_t llsc,
+                           int64_t ldsc, float *C, int64_t ldc, float alpha, float beta,
+                           const uint16_t *JF2);
+void launch_global_fsum(hipStream_t st, int m, int n, int nA, int nB, int maxlevel, int sumOrder,
+                        const float *Cs, int64_t llsc, int64_t ldsc, float *C, int64_t ldc,
+                        float alpha, float beta);
 
 // ---- fast forms (oz_gemm_digits.hip)
 struct LevelBin {      // one exact int32 plane: sum over `npairs` digit pairs, k in [k0, k1)
@@ -80,6 +104,7 @@ void launch_reconstruct(hipStream_t st, int M, int N, const int32_t *T, int64_t 
 void launch_slice_dots(hipStream_t st, int n, const double *xs, int64_t ldx, int nA,
                        const double *ys, int64_t ldy, int nB, double *out);
 void launch_gemm_k0(hipStream_t st, int m, int n, double alpha, double beta, double *C, int64_t ldc);
+void launch_gemm_k0(hipStream_t st, int m, int n, float alpha, float beta, float *C, int64_t ldc);
 size_t dot2_partials_bytes(int nA, int nB);
 void launch_dot2_slices(hipStream_t st, int n, const double *xs, int64_t ldx, int nA,
                         const double *ys, int64_t ldy, int nB, void *part, double *vals);
@@ -96,6 +121,14 @@ void launch_csrmm_sum(hipStream_t st, int m, int nnz, const double *As, int64_t 
 void launch_csrmm_store(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
                         const int *colind, const int *rowptr, const double *Xs, int64_t ldxs,
                         int nB, int maxlevel, double *Cs, int64_t ldcs);
+// binary32 (slices stored as doubles; sums, y and Cs in float; JF2 for NearSum<float>)
+void launch_csrmm_sum(hipStream_t st, int m, int nnz, const double *As, int64_t ldas, int nA,
+                      const int *colind, const int *rowptr, const double *Xs, int64_t ldxs, int nB,
+                      int maxlevel, int fsum, float alpha, float beta, float *y,
+                      const uint16_t *JF2);
+void launch_csrmm_store(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
+                        const int *colind, const int *rowptr, const double *Xs, int64_t ldxs,
+                        int nB, int maxlevel, float *Cs, int64_t ldcs);
 // barrier-free vector split (plan of levels, then slices); VsPlan lives in device memory
 constexpr int kVsMaxLevels = 64;
 struct VsPlan {
@@ -107,8 +140,15 @@ hipError_t launch_vsplit(hipStream_t st, int n, const double *x, double *slices,
                          int rho, int shift, int cap, int reproMode, const uint16_t *J,
                          unsigned long long *maxv, int *info, void *scratch, VsPlan *plan,
                          const int *stopAt = nullptr, int it = 0);
+// binary32 input (JF: the binary32 tau table); slices are stored as doubles
+hipError_t launch_vsplit(hipStream_t st, int n, const float *x, double *slices, int64_t lds,
+                         int rho, int shift, int cap, int reproMode, const uint16_t *JF,
+                         unsigned long long *maxv, int *info, void *scratch, VsPlan *plan,
+                         const int *stopAt = nullptr, int it = 0);
 void launch_axpy_host(hipStream_t st, int n, double alpha, const double *x, double *y, int fused);
-void launch_scal_axpy(hipStream_t st, int n, double beta, const double *r, double *p);
+void launch_axpy_host(hipStream_t st, int n, float alpha, const float *x, float *y, int fused);
+void launch_scal_axpy(hipStream_t st, int n, double beta, const double *r, double *p, int fused);
+void launch_scal_axpy(hipStream_t st, int n, float beta, const float *r, float *p, int fused);
 
 // ---- device-resident CG (oz_level12.hip)
 // Scalars of ozblasRcg kept on the device between the kernels of one solve; a kernel of
@@ -133,7 +173,8 @@ void launch_csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas,
 void launch_dots_dev(hipStream_t st, int n, const double *xs, int64_t ldx, const int *infoX,
                      const double *ys, int64_t ldy, const int *infoY, double *vals, unsigned *ctr,
                      unsigned long long *maxvX, unsigned long long *maxvY, int nmax,
-                     const uint16_t *J, int fsum, CgDev *S, int it, int mode, double *C);
+                     const uint16_t *J, int fsum, CgDev *S, int it, int mode, void *C,
+                     int f32 = 0);
 void launch_axpy2_dev(hipStream_t st, int n, const CgDev *S, int it, const double *P, double *X,
                       const double *Q, double *R);
 void launch_scal_axpy_dev(hipStream_t st, int n, const CgDev *S, int it, const double *R,

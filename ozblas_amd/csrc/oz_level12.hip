@@ -46,9 +46,10 @@ __device__ inline double block_sum_d(double v, double *red) {
 // in one pass (per element only k in (lsb, msb+1] are non-trivial: below, the residue is 0; above,
 // it is x itself), its last workgroup walks the chain of levels, and k_vsplit_write stores the
 // slices slice_s = cmod(x, u_{s-1}) - cmod(x, u_s) (exact).  Inputs outside the model (non-finite,
-// |x| >= 2^1022, rho < 3, a sigma outside the normal range, a non power-of-two shift, more levels
-// than VsPlan holds) set plan.bad and are split by k_vsplit_fallback instead, which runs the
-// reference loop in one workgroup.
+// |x| >= 2^1022, rho < 3 or above the precision, a sigma outside the normal range, a non
+// power-of-two shift, more levels than VsPlan holds) set plan.bad and are split by
+// k_vsplit_fallback instead, which runs the reference loop in one workgroup.  The binary32 split
+// is the same with 24 for 53 (unit 2^(rho+tau-24), float sigma in its normal range).
 // ---------------------------------------------------------------------------------------------
 constexpr int kVsOff = 1100;          // F/G tables cover k in [-1100, 1099]
 constexpr int kVsK = 2200;
@@ -74,8 +75,10 @@ __device__ inline double vs_cmod(double x, int k) {
 	return x - ldexp(n, k);
 }
 
-template <int BS>
-__global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const double *x, unsigned long long *Fg,
+// TI = float: the binary32 split on the same tables (a float converts exactly; its residues are
+// floats), with the binary32 tau rule and unit 2^(rho + tau - 24)
+template <int BS, class TI>
+__global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned long long *Fg,
                                                     int2 *range,
                                                     unsigned *ctr, int rho, int cap,
                                                     int reproMode, const uint16_t *J, VsPlan *plan,
@@ -92,7 +95,7 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const double *x, unsi
 	int tlo = INT32_MAX, thi = INT32_MIN;  // this thread's range of k
 	for (int64_t base = (int64_t)blockIdx.x * BS; base < n; base += (int64_t)gridDim.x * BS) {
 		const int64_t i = base + threadIdx.x;
-		const double v = (i < n) ? x[i] : 0.0;
+		const double v = (i < n) ? (double)x[i] : 0.0;
 		// x = m 2^eb with m < 2^53 an integer (m = 0: no contribution)
 		const uint64_t b = d2u(v);
 		const int E = (int)((b >> 52) & 0x7ff);
@@ -207,17 +210,19 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const double *x, unsi
 		if (k > K1) return __builtin_bit_cast(double, gmax);
 		return __builtin_bit_cast(double, Gb[k + kVsOff]);
 	};
-	if (rho < 3 || cap > kVsMaxLevels) bad = 1;
+	// rho <= kPrec keeps every unit an even divisor of the one before (u_{s+1} <= u_s 2^(rho-1-kPrec)),
+	// which the residue chain needs; a binary32 vector longer than 2^24 exceeds it
+	if (rho < 3 || rho > SplitRule<TI>::kPrec || cap > kVsMaxLevels) bad = 1;
 	double M = __builtin_bit_cast(double, gmax);
 	int s = 0;
 	bool zero = false;
 	for (s = 0; s < cap && !bad; s++) {
-		const int tau = tau_of(M, J);
-		if (rho + tau > 1022 || rho + tau - 1 < -1022) {
+		const int tau = SplitRule<TI>::tau(M, J);
+		if (rho + tau > SplitRule<TI>::kSigEmax || rho + tau - 1 < SplitRule<TI>::kEmin) {
 			bad = 1;
 			break;
 		}
-		const int k = rho + tau - 53;
+		const int k = rho + tau - SplitRule<TI>::kPrec;
 		plan->k[s] = k;
 		M = Fk(k);
 		if (M == 0.0) {
@@ -247,7 +252,8 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const double *x, unsi
 }
 
 // slices from the plan: slice_s = cmod(x, u_{s-1}) - cmod(x, u_s), remainder slot, zero slots
-__global__ __launch_bounds__(256) void k_vsplit_write(int n, const double *x, double *slices,
+template <class TI>
+__global__ __launch_bounds__(256) void k_vsplit_write(int n, const TI *x, double *slices,
                                                       int64_t lds, const VsPlan *plan,
                                                       const int *stopAt, int it) {
 	if (stopAt && it >= *stopAt) return;
@@ -255,7 +261,7 @@ __global__ __launch_bounds__(256) void k_vsplit_write(int n, const double *x, do
 	const int last = plan->last, count = plan->count, zero = plan->zero;
 	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
 	     i += (int64_t)gridDim.x * blockDim.x) {
-		const double v = x[i];
+		const double v = (double)x[i];
 		double r = v;
 		for (int s = 0; s <= last; s++) {
 			const double c = vs_cmod(v, plan->k[s]);
@@ -269,8 +275,8 @@ __global__ __launch_bounds__(256) void k_vsplit_write(int n, const double *x, do
 
 // the reference loop in one workgroup (inputs outside the model above): slice s at slot s, its
 // remainder at slot s + 1 (read back as the next level's input), as src/cpu/ozblas_split.cpp:96
-template <int BS>
-__global__ __launch_bounds__(BS) void k_vsplit_fallback(int n, const double *x, double *slices,
+template <int BS, class TI>
+__global__ __launch_bounds__(BS) void k_vsplit_fallback(int n, const TI *x, double *slices,
                                                         int64_t lds, int rho, int shift, int cap,
                                                         int reproMode, const uint16_t *J,
                                                         const VsPlan *plan, int *info,
@@ -279,21 +285,21 @@ __global__ __launch_bounds__(BS) void k_vsplit_fallback(int n, const double *x, 
 	if (!plan->bad) return;
 	__shared__ double red[BS / 64];
 	double mx = 0.0;
-	for (int64_t i = threadIdx.x; i < n; i += BS) mx = std_max(mx, fabs(x[i]));
+	for (int64_t i = threadIdx.x; i < n; i += BS) mx = std_max(mx, fabs((double)x[i]));
 	mx = block_max_d<BS>(mx, red);
 	int s;
 	bool zero = false;
 	for (s = 0; s < cap; s++) {
-		const double sigma = split_sigma(rho, tau_of(mx, J), shift);
-		const double *src = (s == 0) ? x : slices + (int64_t)s * lds;
+		const TI sigma = SplitRule<TI>::sigma(rho, SplitRule<TI>::tau(mx, J), shift);
+		const double *src = slices + (int64_t)s * lds;
 		double lm = 0.0;
 		for (int64_t i = threadIdx.x; i < n; i += BS) {
-			const double v = src[i];
-			const double a = (v + sigma) - sigma;
-			const double rr = v - a;
-			slices[(int64_t)s * lds + i] = a;
-			slices[(int64_t)(s + 1) * lds + i] = rr;
-			lm = std_max(lm, fabs(rr));
+			const TI v = (s == 0) ? x[i] : (TI)src[i];  // a stored remainder converts back exactly
+			const TI a = (v + sigma) - sigma;
+			const TI rr = v - a;
+			slices[(int64_t)s * lds + i] = (double)a;
+			slices[(int64_t)(s + 1) * lds + i] = (double)rr;
+			lm = std_max(lm, fabs((double)rr));
 		}
 		mx = block_max_d<BS>(lm, red);
 		if (mx == 0.0) {
@@ -324,8 +330,9 @@ size_t vsplit_scratch_bytes() { return 8ull * kVsK + 8ull * kVsBlocks + 64; }
 
 // Splits x (length n) into slices at stride lds (cap + 2 slots); the count goes to info[0] on the
 // device.  scratch: vsplit_scratch_bytes(), zeroed once (the kernels leave it zero again).
-hipError_t launch_vsplit(hipStream_t st, int n, const double *x, double *slices, int64_t lds,
-                         int rho, int shift, int cap, int reproMode, const uint16_t *J,
+template <class TI>
+static hipError_t vsplit(hipStream_t st, int n, const TI *x, double *slices, int64_t lds, int rho,
+                         int shift, int cap, int reproMode, const uint16_t *J,
                          unsigned long long *maxv, int *info, void *scratch, VsPlan *plan,
                          const int *stopAt, int it) {
 	unsigned long long *Fg = (unsigned long long *)scratch;
@@ -338,15 +345,31 @@ hipError_t launch_vsplit(hipStream_t st, int n, const double *x, double *slices,
 	}
 	int grid = (int)std::min<int64_t>(kVsBlocks, ((int64_t)n + kVsBS - 1) / kVsBS);
 	if (grid < 1) grid = 1;
-	hipLaunchKernelGGL(k_vsplit_scan<kVsBS>, dim3(grid), dim3(kVsBS), 0, st, n, x, Fg, range, ctr,
-	                   rhoEff, cap, reproMode, J, plan, info, maxv, stopAt, it);
+	hipLaunchKernelGGL((k_vsplit_scan<kVsBS, TI>), dim3(grid), dim3(kVsBS), 0, st, n, x, Fg, range,
+	                   ctr, rhoEff, cap, reproMode, J, plan, info, maxv, stopAt, it);
 	int gw = (int)std::min<int64_t>(2048, ((int64_t)n + 255) / 256);
 	if (gw < 1) gw = 1;
-	hipLaunchKernelGGL(k_vsplit_write, dim3(gw), dim3(256), 0, st, n, x, slices, lds,
+	hipLaunchKernelGGL(k_vsplit_write<TI>, dim3(gw), dim3(256), 0, st, n, x, slices, lds,
 	                   (const VsPlan *)plan, stopAt, it);
-	hipLaunchKernelGGL(k_vsplit_fallback<1024>, dim3(1), dim3(1024), 0, st, n, x, slices, lds, rho,
-	                   shift, cap, reproMode, J, (const VsPlan *)plan, info, stopAt, it);
+	hipLaunchKernelGGL((k_vsplit_fallback<1024, TI>), dim3(1), dim3(1024), 0, st, n, x, slices, lds,
+	                   rho, shift, cap, reproMode, J, (const VsPlan *)plan, info, stopAt, it);
 	return hipGetLastError();
+}
+
+hipError_t launch_vsplit(hipStream_t st, int n, const double *x, double *slices, int64_t lds,
+                         int rho, int shift, int cap, int reproMode, const uint16_t *J,
+                         unsigned long long *maxv, int *info, void *scratch, VsPlan *plan,
+                         const int *stopAt, int it) {
+	return vsplit(st, n, x, slices, lds, rho, shift, cap, reproMode, J, maxv, info, scratch, plan,
+	              stopAt, it);
+}
+
+hipError_t launch_vsplit(hipStream_t st, int n, const float *x, double *slices, int64_t lds,
+                         int rho, int shift, int cap, int reproMode, const uint16_t *JF,
+                         unsigned long long *maxv, int *info, void *scratch, VsPlan *plan,
+                         const int *stopAt, int it) {
+	return vsplit(st, n, x, slices, lds, rho, shift, cap, reproMode, JF, maxv, info, scratch, plan,
+	              stopAt, it);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -395,20 +418,31 @@ void launch_slice_dots(hipStream_t st, int n, const double *xs, int64_t ldx, int
 
 // GEMM with k = 0: every slice product is zero, so the reference's sum is +0 and the epilogue
 // leaves C = fma(alpha, +0, beta * C)
-__global__ void k_gemm_k0(int m, int n, double alpha, double beta, double *C, int64_t ldc) {
+template <class T>
+__global__ void k_gemm_k0(int m, int n, T alpha, T beta, T *C, int64_t ldc) {
 	const int64_t total = (int64_t)m * n;
 	for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
 	     e += (int64_t)gridDim.x * blockDim.x) {
-		double *c = C + (e / m) * ldc + e % m;
-		*c = fma(alpha, 0.0, beta * *c);
+		T *c = C + (e / m) * ldc + e % m;
+		*c = fma1_t(alpha, (T)0, beta * *c);
 	}
 }
 
-void launch_gemm_k0(hipStream_t st, int m, int n, double alpha, double beta, double *C, int64_t ldc) {
+static unsigned grid_k0(int m, int n) {
 	int64_t g = ((int64_t)m * n + 255) / 256;
 	if (g > 4096) g = 4096;
 	if (g < 1) g = 1;
-	hipLaunchKernelGGL(k_gemm_k0, dim3((unsigned)g), dim3(256), 0, st, m, n, alpha, beta, C, ldc);
+	return (unsigned)g;
+}
+
+void launch_gemm_k0(hipStream_t st, int m, int n, double alpha, double beta, double *C, int64_t ldc) {
+	hipLaunchKernelGGL(k_gemm_k0<double>, dim3(grid_k0(m, n)), dim3(256), 0, st, m, n, alpha, beta, C,
+	                   ldc);
+}
+
+void launch_gemm_k0(hipStream_t st, int m, int n, float alpha, float beta, float *C, int64_t ldc) {
+	hipLaunchKernelGGL(k_gemm_k0<float>, dim3(grid_k0(m, n)), dim3(256), 0, st, m, n, alpha, beta, C,
+	                   ldc);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -511,6 +545,35 @@ __device__ inline double dot_pairs_sum(int nvals, double *vals, int nA, int nB, 
 	return nearsum(nvals, v, J);
 }
 
+// binary32: the same over the pair dots rounded to float (exact: 24 - rho bits per slice keep
+// every pair dot within 24 bits), NearSum<float> with the log2f table / Fsum in float
+__device__ inline float dot_pairs_sum_f(int nvals, float *vals, int nA, int nB, int maxlevel,
+                                        int fsum, const uint16_t *JF2) {
+	if (fsum) {
+		float t = 0.f;
+		for (int ik = 0; ik <= maxlevel; ik++)
+			for (int ia = 0; ia < nA; ia++) {
+				const int ib = ik - ia;
+				if (ib < 0 || ib >= nB) continue;
+				t = t + vals[nA * ib + ia];
+			}
+		return t;
+	}
+	StridedVecF v{vals, 1};
+	return nearsum_f(nvals, v, JF2);
+}
+
+// rounds vals[0, nvals) to float in place: float p occupies the bytes of double p / 2, which
+// ascending p has already read
+__device__ inline float *vals_to_float(int nvals, double *vals) {
+	float *f = (float *)vals;
+	for (int p = 0; p < nvals; p++) {
+		const double v = vals[p];
+		f[p] = (float)v;
+	}
+	return f;
+}
+
 // one thread; C[0] = fma(alpha, t, beta * C[0])
 __global__ void k_dot_sum(int nvals, double *vals, double alpha, double beta, double *C, int nA,
                           int nB, int maxlevel, int fsum, const uint16_t *J) {
@@ -533,15 +596,17 @@ void launch_nearsum_small(hipStream_t st, int nvals, double *vals, double alpha,
 // over the pairs with ia + ib <= maxlevel (or Fsum in (level, ia, ib) order), then
 // y = fma(alpha, t, beta * y).
 // ---------------------------------------------------------------------------------------------
-template <int NAM, int NBM>
-__device__ __forceinline__ double csrmm_row_sum(int r, const double *As, int64_t ldas, int nA,
-                                                const int *colind, const int *rowptr,
-                                                const double *Xs, int64_t ldxs, int nB,
-                                                int maxlevel, int fsum, const uint16_t *J) {
+// T = float: the binary32 instantiation -- each pair sum is the fma1<float> chain
+// (float)fma(a, x, (double)v) of csrmm_n<float>, then NearSum<float> (J = JF2) or Fsum in float
+template <int NAM, int NBM, class T = double>
+__device__ __forceinline__ T csrmm_row_sum(int r, const double *As, int64_t ldas, int nA,
+                                           const int *colind, const int *rowptr, const double *Xs,
+                                           int64_t ldxs, int nB, int maxlevel, int fsum,
+                                           const uint16_t *J) {
 	const int b = rowptr[r], e = rowptr[r + 1];
-	RegVec<NAM * NBM> v;
+	RegVec<NAM * NBM, T> v;
 #pragma unroll
-	for (int q = 0; q < NAM * NBM; q++) v[q] = 0.0;
+	for (int q = 0; q < NAM * NBM; q++) v[q] = (T)0;
 	for (int jj = b; jj < e; jj++) {
 		const int c = colind[jj];
 		double a[NAM], x[NBM];
@@ -552,11 +617,12 @@ __device__ __forceinline__ double csrmm_row_sum(int r, const double *As, int64_t
 #pragma unroll
 		for (int ia = 0; ia < NAM; ia++)
 #pragma unroll
-			for (int ib = 0; ib < NBM; ib++) v[ia * NBM + ib] = fma(a[ia], x[ib], v[ia * NBM + ib]);
+			for (int ib = 0; ib < NBM; ib++)
+				v[ia * NBM + ib] = (T)fma(a[ia], x[ib], (double)v[ia * NBM + ib]);
 	}
-	double t;
+	T t;
 	if (fsum) {
-		t = 0.;
+		t = (T)0;
 #pragma unroll
 		for (int ik = 0; ik < NAM + NBM - 1; ik++)
 #pragma unroll
@@ -567,12 +633,12 @@ __device__ __forceinline__ double csrmm_row_sum(int r, const double *As, int64_t
 	} else {
 		// NearSum over the products (its result does not depend on the order of the terms):
 		// compact the valid ones to the front of a register array
-		RegVec<NAM * NBM> u;
+		RegVec<NAM * NBM, T> u;
 		int cnt = 0;
 #pragma unroll
 		for (int ia = 0; ia < NAM; ia++)
 #pragma unroll
-			for (int ib = 0; ib < NBM; ib++) u[ia * NBM + ib] = 0.0;
+			for (int ib = 0; ib < NBM; ib++) u[ia * NBM + ib] = (T)0;
 #pragma unroll
 		for (int ia = 0; ia < NAM; ia++)
 #pragma unroll
@@ -583,22 +649,38 @@ __device__ __forceinline__ double csrmm_row_sum(int r, const double *As, int64_t
 					if (ok && q == cnt) u[q] = v[ia * NBM + ib];
 				cnt += ok ? 1 : 0;
 			}
-		t = nearsum(cnt, u, J);
+		if constexpr (sizeof(T) == 4)
+			t = nearsum_f(cnt, u, J);
+		else
+			t = nearsum(cnt, u, J);
 	}
 	return t;
 }
 
-template <int NAM, int NBM>
+template <int NAM, int NBM, class T>
 __global__ __launch_bounds__(256) void k_csrmm_sum(int m, const double *As, int64_t ldas, int nA,
                                                    const int *colind, const int *rowptr,
                                                    const double *Xs, int64_t ldxs, int nB,
-                                                   int maxlevel, int fsum, double alpha,
-                                                   double beta, double *y, const uint16_t *J) {
+                                                   int maxlevel, int fsum, T alpha, T beta, T *y,
+                                                   const uint16_t *J) {
 	const int r = blockIdx.x * blockDim.x + threadIdx.x;
 	if (r >= m) return;
-	const double t = csrmm_row_sum<NAM, NBM>(r, As, ldas, nA, colind, rowptr, Xs, ldxs, nB, maxlevel,
-	                                         fsum, J);
-	y[r] = fma(alpha, t, beta * y[r]);
+	const T t = csrmm_row_sum<NAM, NBM, T>(r, As, ldas, nA, colind, rowptr, Xs, ldxs, nB, maxlevel,
+	                                       fsum, J);
+	y[r] = fma1_t(alpha, t, beta * y[r]);
+}
+
+template <class T>
+static void csrmm_sum(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
+                      const int *colind, const int *rowptr, const double *Xs, int64_t ldxs, int nB,
+                      int maxlevel, int fsum, T alpha, T beta, T *y, const uint16_t *J) {
+	const dim3 g((m + 255) / 256), b(256);
+	if (nA <= 4 && nB <= 4)
+		hipLaunchKernelGGL((k_csrmm_sum<4, 4, T>), g, b, 0, st, m, As, ldas, nA, colind, rowptr, Xs,
+		                   ldxs, nB, maxlevel, fsum, alpha, beta, y, J);
+	else
+		hipLaunchKernelGGL((k_csrmm_sum<4, 8, T>), g, b, 0, st, m, As, ldas, nA, colind, rowptr, Xs,
+		                   ldxs, nB, maxlevel, fsum, alpha, beta, y, J);
 }
 
 void launch_csrmm_sum(hipStream_t st, int m, int nnz, const double *As, int64_t ldas, int nA,
@@ -606,13 +688,15 @@ void launch_csrmm_sum(hipStream_t st, int m, int nnz, const double *As, int64_t 
                       int maxlevel, int fsum, double alpha, double beta, double *y,
                       const uint16_t *J) {
 	(void)nnz;
-	const dim3 g((m + 255) / 256), b(256);
-	if (nA <= 4 && nB <= 4)
-		hipLaunchKernelGGL((k_csrmm_sum<4, 4>), g, b, 0, st, m, As, ldas, nA, colind, rowptr, Xs,
-		                   ldxs, nB, maxlevel, fsum, alpha, beta, y, J);
-	else
-		hipLaunchKernelGGL((k_csrmm_sum<4, 8>), g, b, 0, st, m, As, ldas, nA, colind, rowptr, Xs,
-		                   ldxs, nB, maxlevel, fsum, alpha, beta, y, J);
+	csrmm_sum(st, m, As, ldas, nA, colind, rowptr, Xs, ldxs, nB, maxlevel, fsum, alpha, beta, y, J);
+}
+
+void launch_csrmm_sum(hipStream_t st, int m, int nnz, const double *As, int64_t ldas, int nA,
+                      const int *colind, const int *rowptr, const double *Xs, int64_t ldxs, int nB,
+                      int maxlevel, int fsum, float alpha, float beta, float *y,
+                      const uint16_t *JF2) {
+	(void)nnz;
+	csrmm_sum(st, m, As, ldas, nA, colind, rowptr, Xs, ldxs, nB, maxlevel, fsum, alpha, beta, y, JF2);
 }
 
 // Dot2 CSRMV (csrmm_n_x2, src/cpu/ozblas_blas_wrapper.cpp:262-295, then the global sum over
@@ -700,10 +784,12 @@ void launch_csrmm_sum_x2(hipStream_t st, int m, const double *As, int64_t ldas, 
 	                   nA, colind, rowptr, Xs, ldxs, nB, fsum, alpha, beta, y, J);
 }
 
-// general case: store the products where blasRcsrmm puts them (column ic of CSplit, ld ldcs)
+// general case: store the products where blasRcsrmm puts them (column ic of CSplit, ld ldcs);
+// T = float: the fma1<float> chain of csrmm_n<float>
+template <class T>
 __global__ void k_csrmm_store(int m, const double *As, int64_t ldas, int nA, const int *colind,
                               const int *rowptr, const double *Xs, int64_t ldxs, int nB,
-                              int maxlevel, double *Cs, int64_t ldcs) {
+                              int maxlevel, T *Cs, int64_t ldcs) {
 	const int r = blockIdx.x * blockDim.x + threadIdx.x;
 	if (r >= m) return;
 	const int b = rowptr[r], e = rowptr[r + 1];
@@ -712,9 +798,9 @@ __global__ void k_csrmm_store(int m, const double *As, int64_t ldas, int nA, con
 	for (int ia = 0; ia < na; ia++) {
 		const int numB = (nB < maxlevel + 1 - ia) ? nB : maxlevel + 1 - ia;
 		for (int ib = 0; ib < numB; ib++) {
-			double tr = 0.;
+			T tr = (T)0;
 			for (int jj = b; jj < e; jj++)
-				tr = fma(As[(int64_t)ia * ldas + jj], Xs[(int64_t)ib * ldxs + colind[jj]], tr);
+				tr = (T)fma(As[(int64_t)ia * ldas + jj], Xs[(int64_t)ib * ldxs + colind[jj]], (double)tr);
 			Cs[(int64_t)(col + ib) * ldcs + r] = tr;
 		}
 		col += numB;
@@ -724,27 +810,41 @@ __global__ void k_csrmm_store(int m, const double *As, int64_t ldas, int nA, con
 void launch_csrmm_store(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
                         const int *colind, const int *rowptr, const double *Xs, int64_t ldxs,
                         int nB, int maxlevel, double *Cs, int64_t ldcs) {
-	hipLaunchKernelGGL(k_csrmm_store, dim3((m + 255) / 256), dim3(256), 0, st, m, As, ldas, nA,
+	hipLaunchKernelGGL(k_csrmm_store<double>, dim3((m + 255) / 256), dim3(256), 0, st, m, As, ldas, nA,
 	                   colind, rowptr, Xs, ldxs, nB, maxlevel, Cs, ldcs);
 }
 
-// ozblasRaxpy with reproMode != 0 (src/cpu/ozblas_axpy.cpp:142-144): y = fma(alpha, x, y)
-__global__ void k_axpy(int n, double alpha, const double *x, double *y) {
+void launch_csrmm_store(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
+                        const int *colind, const int *rowptr, const double *Xs, int64_t ldxs,
+                        int nB, int maxlevel, float *Cs, int64_t ldcs) {
+	hipLaunchKernelGGL(k_csrmm_store<float>, dim3((m + 255) / 256), dim3(256), 0, st, m, As, ldas, nA,
+	                   colind, rowptr, Xs, ldxs, nB, maxlevel, Cs, ldcs);
+}
+
+// ozblasRaxpy with reproMode != 0 (src/cpu/ozblas_axpy.cpp:142-144): y = fma1(alpha, x, y)
+template <class T>
+__global__ void k_axpy(int n, T alpha, const T *x, T *y) {
 	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
 	     i += (int64_t)gridDim.x * blockDim.x)
-		y[i] = fma(alpha, x[i], y[i]);
+		y[i] = fma1_t(alpha, x[i], y[i]);
 }
-// y = y + alpha * x without contraction (the cblas_daxpy path of reproMode == 0)
-__global__ void k_axpy_plain(int n, double alpha, const double *x, double *y) {
+// y = y + alpha * x without contraction (the cblas_?axpy path of reproMode == 0)
+template <class T>
+__global__ void k_axpy_plain(int n, T alpha, const T *x, T *y) {
 	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
 	     i += (int64_t)gridDim.x * blockDim.x)
 		y[i] = y[i] + alpha * x[i];
 }
-// CG update p = beta * p (cblas_dscal, src/cpu/ozblas_cg.cpp:97), then p = fma(1, r, p) (:98)
-__global__ void k_scal_axpy(int n, double beta, const double *r, double *p) {
+// CG update p = beta * p (cblas_?scal, src/cpu/ozblas_cg.cpp:97), then ozblasRaxpy(1, r, p) (:98):
+// fma1(1, r, p), or p + r on the reproMode == 0 path (the same for binary64, not for binary32,
+// where fma1 rounds twice)
+template <class T>
+__global__ void k_scal_axpy(int n, T beta, const T *r, T *p, int fused) {
 	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-	     i += (int64_t)gridDim.x * blockDim.x)
-		p[i] = fma(1.0, r[i], beta * p[i]);
+	     i += (int64_t)gridDim.x * blockDim.x) {
+		const T q = beta * p[i];
+		p[i] = fused ? fma1_t((T)1, r[i], q) : q + (T)1 * r[i];
+	}
 }
 
 static int grid1(int64_t n) {
@@ -756,13 +856,24 @@ static int grid1(int64_t n) {
 void launch_axpy_host(hipStream_t st, int n, double alpha, const double *x, double *y,
                       int fused) {
 	if (fused)
-		hipLaunchKernelGGL(k_axpy, dim3(grid1(n)), dim3(256), 0, st, n, alpha, x, y);
+		hipLaunchKernelGGL(k_axpy<double>, dim3(grid1(n)), dim3(256), 0, st, n, alpha, x, y);
 	else
-		hipLaunchKernelGGL(k_axpy_plain, dim3(grid1(n)), dim3(256), 0, st, n, alpha, x, y);
+		hipLaunchKernelGGL(k_axpy_plain<double>, dim3(grid1(n)), dim3(256), 0, st, n, alpha, x, y);
 }
 
-void launch_scal_axpy(hipStream_t st, int n, double beta, const double *r, double *p) {
-	hipLaunchKernelGGL(k_scal_axpy, dim3(grid1(n)), dim3(256), 0, st, n, beta, r, p);
+void launch_axpy_host(hipStream_t st, int n, float alpha, const float *x, float *y, int fused) {
+	if (fused)
+		hipLaunchKernelGGL(k_axpy<float>, dim3(grid1(n)), dim3(256), 0, st, n, alpha, x, y);
+	else
+		hipLaunchKernelGGL(k_axpy_plain<float>, dim3(grid1(n)), dim3(256), 0, st, n, alpha, x, y);
+}
+
+void launch_scal_axpy(hipStream_t st, int n, double beta, const double *r, double *p, int fused) {
+	hipLaunchKernelGGL(k_scal_axpy<double>, dim3(grid1(n)), dim3(256), 0, st, n, beta, r, p, fused);
+}
+
+void launch_scal_axpy(hipStream_t st, int n, float beta, const float *r, float *p, int fused) {
+	hipLaunchKernelGGL(k_scal_axpy<float>, dim3(grid1(n)), dim3(256), 0, st, n, beta, r, p, fused);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -848,9 +959,11 @@ __global__ __launch_bounds__(256) void k_csrmm_big_dev(int m, const double *As, 
 //   2: dnew = t, resi = sqrt(dnew), beta = dnew / dold,  (src/cpu/ozblas_cg.cpp:91-94,100-112)
 //      dold = dnew, history and the residual test
 // then clears vals, the counter and the split maxima it consumed for the next use.
+// With f32 (binary32 DOT; mode 0 only) the pair dots round to float first, the sum is the float
+// one (J is then JF2) and C is a float: C[0] = fma1<float>(1, t, 0 * C[0]).
 __device__ void dots_finish(int nA, int nB, double *vals, unsigned *ctr, unsigned long long *maxvX,
                             unsigned long long *maxvY, int nmax, const uint16_t *J, int fsum,
-                            CgDev *S, int it, int mode, double *C) {
+                            CgDev *S, int it, int mode, void *C, int f32) {
 	__shared__ int isLast;
 	__threadfence();
 	if (threadIdx.x == 0) isLast = (atomicAdd(ctr, 1u) == gridDim.x - 1) ? 1 : 0;
@@ -859,10 +972,24 @@ __device__ void dots_finish(int nA, int nB, double *vals, unsigned *ctr, unsigne
 	__threadfence();
 	const int npairs = nA * nB;  // vals: plain loads, ordered after the other workgroups by the fence
 	const int maxlevel = (nA - 1) + (nB - 1);  // DOT forces fastMode = 0 (src/cpu/ozblas_gemm.cpp:25-28)
+	if (f32) {
+		float *fv = vals_to_float(npairs, vals);
+		const float t = dot_pairs_sum_f(npairs, fv, nA, nB, maxlevel, fsum, J);
+		float *Cf = (float *)C;
+		Cf[0] = fma1_f(1.f, t, 0.f * Cf[0]);
+		for (int p = 0; p < npairs; p++) vals[p] = 0.0;
+		for (int q = 0; q < nmax; q++) {
+			maxvX[q] = 0ull;
+			maxvY[q] = 0ull;
+		}
+		*ctr = 0u;
+		return;
+	}
 	const double t = dot_pairs_sum(npairs, vals, nA, nB, maxlevel, fsum, J);
 	const double r = fma(1.0, t, 0.0 * 0.0);
 	if (mode == 0) {
-		C[0] = fma(1.0, t, 0.0 * C[0]);
+		double *Cd = (double *)C;
+		Cd[0] = fma(1.0, t, 0.0 * Cd[0]);
 	} else if (mode == 1) {
 		S->tmp = r;
 		S->alpha = S->dold / r;
@@ -901,7 +1028,7 @@ __global__ __launch_bounds__(BS) void k_dots_reg(int n, const double *xs, int64_
                                                  unsigned long long *maxvX,
                                                  unsigned long long *maxvY, int nmax,
                                                  const uint16_t *J, int fsum, CgDev *S, int it,
-                                                 int mode, double *C) {
+                                                 int mode, void *C, int f32) {
 	if (cg_skip(S, it)) return;
 	const int nA = infoX[0], nB = infoY[0];
 	if (nA > NAM || nB > NBM) return;
@@ -936,7 +1063,7 @@ __global__ __launch_bounds__(BS) void k_dots_reg(int n, const double *xs, int64_
 			if (v != 0.0) atomicAdd(&vals[a + nA * b], v);
 		}
 	}
-	dots_finish(nA, nB, vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C);
+	dots_finish(nA, nB, vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
 }
 
 // The same for any slice counts (pairs in chunks of kDotChunk, slices re-read per chunk); does
@@ -948,7 +1075,7 @@ __global__ __launch_bounds__(BS) void k_dots_dev(int n, const double *xs, int64_
                                                  unsigned long long *maxvX,
                                                  unsigned long long *maxvY, int nmax,
                                                  const uint16_t *J, int fsum, CgDev *S, int it,
-                                                 int mode, double *C, int regA, int regB) {
+                                                 int mode, void *C, int f32, int regA, int regB) {
 	if (cg_skip(S, it)) return;
 	__shared__ double red[BS / 64];
 	const int nA = infoX[0], nB = infoY[0];
@@ -974,7 +1101,7 @@ __global__ __launch_bounds__(BS) void k_dots_dev(int n, const double *xs, int64_
 			if (threadIdx.x == 0 && p0 + q < npairs && v != 0.0) atomicAdd(&vals[p0 + q], v);
 		}
 	}
-	dots_finish(nA, nB, vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C);
+	dots_finish(nA, nB, vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
 }
 
 // x = fma(alpha, p, x); r = fma(-alpha, q, r)   (src/cpu/ozblas_cg.cpp:87-88, ozblas_axpy.cpp)
@@ -1012,15 +1139,15 @@ void launch_csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas,
 void launch_dots_dev(hipStream_t st, int n, const double *xs, int64_t ldx, const int *infoX,
                      const double *ys, int64_t ldy, const int *infoY, double *vals, unsigned *ctr,
                      unsigned long long *maxvX, unsigned long long *maxvY, int nmax,
-                     const uint16_t *J, int fsum, CgDev *S, int it, int mode, double *C) {
+                     const uint16_t *J, int fsum, CgDev *S, int it, int mode, void *C, int f32) {
 	constexpr int kRA = 8, kRB = 8;
 	int gx = (n + 255) / 256;
 	if (gx > 512) gx = 512;
 	if (gx < 1) gx = 1;
 	hipLaunchKernelGGL((k_dots_reg<256, kRA, kRB>), dim3(gx), dim3(256), 0, st, n, xs, ldx, infoX, ys,
-	                   ldy, infoY, vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C);
+	                   ldy, infoY, vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
 	hipLaunchKernelGGL(k_dots_dev<256>, dim3(gx), dim3(256), 0, st, n, xs, ldx, infoX, ys, ldy, infoY,
-	                   vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, kRA, kRB);
+	                   vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32, kRA, kRB);
 }
 
 void launch_axpy2_dev(hipStream_t st, int n, const CgDev *S, int it, const double *P, double *X,

@@ -10,18 +10,21 @@
 
 namespace oz {
 
-struct StridedVec {
+template <class T>
+struct StridedVecT {
 	static constexpr int kMax = 0;  // length known only at run time
-	double *p;
+	T *p;
 	int64_t ld;
-	__device__ double &operator[](int i) const { return p[(int64_t)i * ld]; }
+	__device__ T &operator[](int i) const { return p[(int64_t)i * ld]; }
 };
+using StridedVec = StridedVecT<double>;
+using StridedVecF = StridedVecT<float>;
 
-template <int N>
+template <int N, class T = double>
 struct RegVec {  // register array: loops over it run to N with a guard, so every index is static
 	static constexpr int kMax = N;
-	double v[N];
-	__device__ double &operator[](int i) { return v[i]; }
+	T v[N];
+	__device__ T &operator[](int i) { return v[i]; }
 };
 
 // for (i = i0; i < n; i++) f(i), unrolled to the static bound of V when it has one
@@ -116,6 +119,92 @@ __device__ inline double nearsum(int n, V &vec, const uint16_t *J) {
 	ns_transform_k(n, vec, r2, delta2, tmp, J);
 	if (delta2 == 0) return res + mu;
 	if (ns_sign(delta2) == ns_sign(mu)) return res2;
+	return res;
+}
+
+// ---------------------------------------------------------------------------------------------
+// binary32 NearSum: the <float> instantiation of the same functions.  There the element type is
+// float, but unqualified fabs and scalbn and the double literals 2. and /2. promote part of each
+// expression to double before the result is rounded to float on assignment; ceil(std::log2(mu))
+// is log2f (table JF2).  Each mixed step below is written as that double expression, then cast.
+// ---------------------------------------------------------------------------------------------
+__device__ inline float mac_maxf(float a, float b) { return (a > b) ? a : b; }
+
+template <class V>
+__device__ inline void ns_transform_f(int n, V &vec, float rho, float &tau1, float &tau2,
+                                      const uint16_t *JF2) {
+	// ozblasTransform<float> (src/cpu/ozblas_sum.cpp:10-56): emin = -126, epse = 24
+	if (n == 0) {
+		tau1 = rho;
+		tau2 = 0.f;
+		return;
+	}
+	float mu = fabsf(vec[0]);
+	ns_for<V>(1, n, [&](int i) { mu = mac_maxf(mu, fabsf(vec[i])); });
+	if (mu == 0.f) {
+		tau1 = rho;
+		tau2 = 0.f;
+		return;
+	}
+	const int m = ceil_log2_int(n + 2);
+	// (float)scalbn(1., m + ceil(log2f(mu))): an infinite or NaN mu gives exponent INT_MIN -> 0
+	float sigma = (mu <= 3.40282347e38f)
+	                  ? (float)scalbn_clamped(1., (int64_t)m + ceil_log2_tab_f(mu, JF2))
+	                  : 0.f;
+	float t = rho;
+	const double fac = scalbn_clamped(1., 2 * m + 1 - 24);
+	const double shrink = scalbn_clamped(1., m - 24);
+	while (true) {
+		float tau = 0.f;
+		ns_for<V>(0, n, [&](int i) {
+			const float tmp = (sigma + vec[i]) - sigma;
+			vec[i] = vec[i] - tmp;
+			tau = tau + tmp;
+		});
+		tau1 = t + tau;
+		if (((double)sigma <= 0x1p-126) || (fabs((double)tau1) >= fac * (double)sigma)) {
+			tau2 = tau - (tau1 - t);
+			return;
+		}
+		sigma = (float)(shrink * (double)sigma);
+		t = tau1;
+	}
+}
+
+template <class V>
+__device__ inline void ns_transform_k_f(int n, V &vec, float rho, float &res, float &r,
+                                        const uint16_t *JF2) {
+	// ozblasTransformK<float> (src/cpu/ozblas_sum.cpp:58-73)
+	float tmp = 0.f, tau1, tau2;
+	ns_transform_f(n, vec, rho, tau1, tau2, JF2);
+	ns_for<V>(0, n, [&](int i) { tmp = tmp + vec[i]; });
+	res = tau1 + (tau2 + tmp);
+	r = tau2 - (res - tau1);
+}
+
+__device__ inline float ns_sign_f(float v) { return (v < 0) ? -1.f : 1.f; }
+
+template <class V>
+__device__ inline float nearsum_f(int n, V &vec, const uint16_t *JF2) {
+	// ozblasNearsum<float> (src/cpu/ozblas_sum.cpp:81-112)
+	float tmp, res, res2, r, r2, mu, delta, delta2;
+	const float eps = 0x1p-24f;
+	ns_transform_k_f(n, vec, 0.f, res, r, JF2);
+	ns_transform_k_f(n, vec, r, delta, r2, JF2);
+	if (delta == 0) return res;
+	const float se = ns_sign_f(delta) * eps;  // float product (exact)
+	res2 = (float)((double)res + (double)se * fabs((double)res));
+	if (res2 == res) {
+		mu = (float)((double)se * fabs((double)res));
+		res2 = (float)((double)res + 2. * (double)ns_sign_f(delta) * (double)eps * fabs((double)res));
+	} else {
+		mu = (float)((double)(res2 - res) / 2.);
+	}
+	if (fabsf(delta) < fabsf(mu)) return res;
+	if (fabsf(delta) > fabsf(mu)) return res2;
+	ns_transform_k_f(n, vec, r2, delta2, tmp, JF2);
+	if (delta2 == 0) return res + mu;
+	if (ns_sign_f(delta2) == ns_sign_f(mu)) return res2;
 	return res;
 }
 

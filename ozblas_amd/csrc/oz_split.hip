@@ -1,7 +1,8 @@
 // oz_split.hip -- error-free splitting kernels.
 //
 // Plain form (FP64 slices): restates ozblasSplit / ozblasSplitCKernel / ozblasSplitVecKernel /
-// ozblasSplitSparse (reference src/cpu/ozblas_split.cpp:75-289,756-928) bit for bit:
+// ozblasSplitSparse (reference src/cpu/ozblas_split.cpp:75-289,756-928) bit for bit, for the
+// binary64 and the binary32 instantiations (binary32 slices are stored as doubles):
 //   tau = (short)ceil(log2(max)), sigma = 0.75 * 2^(rho+tau) * splitShift,
 //   slice = (x + sigma) - sigma, remainder = x - slice, next max = max |remainder|.
 // Because tau is per column (dense) or per row (CSR), one workgroup (dense) or one thread (CSR)
@@ -17,33 +18,35 @@ namespace oz {
 // ---------------------------------------------------------------------------------------------
 // dense: one workgroup per column of a k-contiguous operand (column j = in + j*ldi, length m)
 // ---------------------------------------------------------------------------------------------
-template <int BS>
-__global__ __launch_bounds__(BS) void k_split_cols(int m, int n, const double *in, int64_t ldi,
-                                                   double *out, int64_t ldo, double *split,
+// TI = float: the binary32 split (float arithmetic, float remainders), slices stored as doubles;
+// maxima are reduced as doubles (exact for floats)
+template <int BS, class TI>
+__global__ __launch_bounds__(BS) void k_split_cols(int m, int n, const TI *in, int64_t ldi,
+                                                   TI *out, int64_t ldo, double *split,
                                                    int64_t lds, int64_t sstride, int rho, int shift,
                                                    int cap, const uint16_t *J, int *zcol, int *zmax) {
 	__shared__ double red[BS / 64];
 	for (int j = blockIdx.x; j < n; j += gridDim.x) {
-		const double *x = in + (int64_t)j * ldi;
-		double *r = out + (int64_t)j * ldo;
+		const TI *x = in + (int64_t)j * ldi;
+		TI *r = out + (int64_t)j * ldo;
 		double *sp = split + (int64_t)j * lds;
 		double mx = 0.0;  // ozblasFindMaxCKernel (src/cpu/ozblas_split.cpp:37-52)
-		for (int i = threadIdx.x; i < m; i += BS) mx = std_max(mx, fabs(x[i]));
+		for (int i = threadIdx.x; i < m; i += BS) mx = std_max(mx, fabs((double)x[i]));
 		mx = block_max<BS>(mx, red);
 		int s = 0;
 		bool zero = false;
 		for (s = 0; s < cap; s++) {
-			const int tau = tau_of(mx, J);
-			const double sigma = split_sigma(rho, tau, shift);
-			const double *src = (s == 0) ? x : r;
+			const int tau = SplitRule<TI>::tau(mx, J);
+			const TI sigma = SplitRule<TI>::sigma(rho, tau, shift);
+			const TI *src = (s == 0) ? x : r;
 			double lm = 0.0;
 			for (int i = threadIdx.x; i < m; i += BS) {
-				const double v = src[i];
-				const double a = (v + sigma) - sigma;
-				const double rr = v - a;
-				sp[(int64_t)s * sstride + i] = a;
+				const TI v = src[i];
+				const TI a = (v + sigma) - sigma;
+				const TI rr = v - a;
+				sp[(int64_t)s * sstride + i] = (double)a;
 				r[i] = rr;
-				lm = std_max(lm, fabs(rr));
+				lm = std_max(lm, fabs((double)rr));
 			}
 			mx = block_max<BS>(lm, red);
 			if (mx == 0.0) {
@@ -54,7 +57,7 @@ __global__ __launch_bounds__(BS) void k_split_cols(int m, int n, const double *i
 		const int last = zero ? s : cap - 1;
 		// the reference stores each remainder into the next slot (:166); the last one survives
 		for (int i = threadIdx.x; i < m; i += BS)
-			sp[(int64_t)(last + 1) * sstride + i] = zero ? 0.0 : r[i];
+			sp[(int64_t)(last + 1) * sstride + i] = zero ? 0.0 : (double)r[i];
 		if (threadIdx.x == 0) {
 			const int z = zero ? s : cap;  // cap == "never reached zero"
 			zcol[j] = z;
@@ -74,9 +77,10 @@ __global__ void k_zero_slots(int m, int n, double *split, int64_t lds, int64_t s
 }
 
 // out (cols x rows, ldo) = in^T (in: rows x cols, ldi), 32x32 tiles through LDS
-__global__ __launch_bounds__(256) void k_transpose(int rows, int cols, const double *in,
-                                                   int64_t ldi, double *out, int64_t ldo) {
-	__shared__ double t[32][33];
+template <class T>
+__global__ __launch_bounds__(256) void k_transpose(int rows, int cols, const T *in, int64_t ldi,
+                                                   T *out, int64_t ldo) {
+	__shared__ T t[32][33];
 	const int bx = blockIdx.x * 32, by = blockIdx.y * 32;  // bx over rows, by over cols
 	const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
 	for (int k = ty; k < 32; k += 8) {
@@ -127,7 +131,8 @@ __global__ __launch_bounds__(BS) void k_split_vec_pass(int m, const double *src,
 	if (threadIdx.x == 0) atomicMax(maxOut, (unsigned long long)d2u(lm));
 }
 
-__global__ void k_fill(double *p, int64_t n, double v) {
+template <class T>
+__global__ void k_fill(T *p, int64_t n, T v) {
 	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
 	     i += (int64_t)gridDim.x * blockDim.x)
 		p[i] = v;
@@ -141,30 +146,32 @@ __global__ void k_copy(const double *src, double *dst, int64_t n) {
 
 // ---------------------------------------------------------------------------------------------
 // CSR rows: one thread per row, every slice in one launch (ozblasSplitSparseNKernel :788-829,
-// ozblasSplitSparse :896-928); slices at split + s*lds, rho from the row length
+// ozblasSplitSparse :896-928); slices at split + s*lds, rho from the row length.  TI = float is
+// the binary32 split (slices stored as doubles).
 // ---------------------------------------------------------------------------------------------
-__global__ void k_split_csr(int m, const double *val, const int *rowptr, double *work,
-                            double *split, int64_t lds, int shift, int cap, const uint16_t *J,
-                            int *zrow, int *zmax, int epsMode) {
+template <class TI>
+__global__ void k_split_csr(int m, const TI *val, const int *rowptr, TI *work, double *split,
+                            int64_t lds, int shift, int cap, const uint16_t *J, int *zrow,
+                            int *zmax, int epsMode) {
 	const int r = blockIdx.x * blockDim.x + threadIdx.x;
 	if (r >= m) return;
 	const int b = rowptr[r], e = rowptr[r + 1];
-	const int rho = rho_dev(e - b, epsMode);
+	const int rho = SplitRule<TI>::rho_row(e - b, epsMode);
 	double mx = 0.0;
-	for (int i = b; i < e; i++) mx = std_max(mx, fabs(val[i]));
+	for (int i = b; i < e; i++) mx = std_max(mx, fabs((double)val[i]));
 	int s = 0;
 	bool zero = false;
 	for (s = 0; s < cap; s++) {
-		const int tau = tau_of(mx, J);
-		const double sigma = 0.75 * scalbn_clamped(1.0, (int64_t)rho + tau) / (double)shift;
+		const int tau = SplitRule<TI>::tau(mx, J);
+		const TI sigma = SplitRule<TI>::sigma_div(rho, tau, shift);
 		double lm = 0.0;
 		for (int i = b; i < e; i++) {
-			const double v = (s == 0) ? val[i] : work[i];
-			const double a = (v + sigma) - sigma;
-			const double rr = v - a;
-			split[(int64_t)s * lds + i] = a;
+			const TI v = (s == 0) ? val[i] : work[i];
+			const TI a = (v + sigma) - sigma;
+			const TI rr = v - a;
+			split[(int64_t)s * lds + i] = (double)a;
 			work[i] = rr;
-			lm = std_max(lm, fabs(rr));
+			lm = std_max(lm, fabs((double)rr));
 		}
 		mx = lm;
 		if (mx == 0.0) {
@@ -173,7 +180,7 @@ __global__ void k_split_csr(int m, const double *val, const int *rowptr, double 
 		}
 	}
 	const int last = zero ? s : cap - 1;
-	for (int i = b; i < e; i++) split[(int64_t)(last + 1) * lds + i] = zero ? 0.0 : work[i];
+	for (int i = b; i < e; i++) split[(int64_t)(last + 1) * lds + i] = zero ? 0.0 : (double)work[i];
 	const int z = zero ? s : cap;
 	zrow[r] = z;
 	atomicMax(zmax, z);
@@ -245,8 +252,16 @@ void launch_split_cols(hipStream_t st, int m, int n, const double *in, int64_t l
                        int64_t ldo, double *split, int64_t lds, int64_t sstride, int rho,
                        int shift, int cap, const uint16_t *J, int *zcol, int *zmax) {
 	const int grid = n < 65535 ? n : 65535;
-	hipLaunchKernelGGL(k_split_cols<256>, dim3(grid), dim3(256), 0, st, m, n, in, ldi, out, ldo,
-	                   split, lds, sstride, rho, shift, cap, J, zcol, zmax);
+	hipLaunchKernelGGL((k_split_cols<256, double>), dim3(grid), dim3(256), 0, st, m, n, in, ldi, out,
+	                   ldo, split, lds, sstride, rho, shift, cap, J, zcol, zmax);
+}
+
+void launch_split_cols(hipStream_t st, int m, int n, const float *in, int64_t ldi, float *out,
+                       int64_t ldo, double *split, int64_t lds, int64_t sstride, int rho,
+                       int shift, int cap, const uint16_t *JF, int *zcol, int *zmax) {
+	const int grid = n < 65535 ? n : 65535;
+	hipLaunchKernelGGL((k_split_cols<256, float>), dim3(grid), dim3(256), 0, st, m, n, in, ldi, out,
+	                   ldo, split, lds, sstride, rho, shift, cap, JF, zcol, zmax);
 }
 
 void launch_zero_slots(hipStream_t st, int m, int n, double *split, int64_t lds, int64_t sstride,
@@ -259,7 +274,13 @@ void launch_zero_slots(hipStream_t st, int m, int n, double *split, int64_t lds,
 
 void launch_transpose(hipStream_t st, int rows, int cols, const double *in, int64_t ldi,
                       double *out, int64_t ldo) {
-	hipLaunchKernelGGL(k_transpose, dim3((rows + 31) / 32, (cols + 31) / 32), dim3(256), 0, st,
+	hipLaunchKernelGGL(k_transpose<double>, dim3((rows + 31) / 32, (cols + 31) / 32), dim3(256), 0, st,
+	                   rows, cols, in, ldi, out, ldo);
+}
+
+void launch_transpose(hipStream_t st, int rows, int cols, const float *in, int64_t ldi, float *out,
+                      int64_t ldo) {
+	hipLaunchKernelGGL(k_transpose<float>, dim3((rows + 31) / 32, (cols + 31) / 32), dim3(256), 0, st,
 	                   rows, cols, in, ldi, out, ldo);
 }
 
@@ -284,7 +305,14 @@ void launch_fill(hipStream_t st, double *p, int64_t n, double v) {
 	if (n <= 0) return;
 	int64_t g = (n + 255) / 256;
 	if (g > 4096) g = 4096;
-	hipLaunchKernelGGL(k_fill, dim3((unsigned)g), dim3(256), 0, st, p, n, v);
+	hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)g), dim3(256), 0, st, p, n, v);
+}
+
+void launch_fill(hipStream_t st, float *p, int64_t n, float v) {
+	if (n <= 0) return;
+	int64_t g = (n + 255) / 256;
+	if (g > 4096) g = 4096;
+	hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)g), dim3(256), 0, st, p, n, v);
 }
 
 void launch_copy(hipStream_t st, const double *src, double *dst, int64_t n) {
@@ -297,8 +325,15 @@ void launch_copy(hipStream_t st, const double *src, double *dst, int64_t n) {
 void launch_split_csr(hipStream_t st, int m, const double *val, const int *rowptr, double *work,
                       double *split, int64_t lds, int shift, int cap, const uint16_t *J, int *zrow,
                       int *zmax, int epsMode) {
-	hipLaunchKernelGGL(k_split_csr, dim3((m + 255) / 256), dim3(256), 0, st, m, val, rowptr, work,
-	                   split, lds, shift, cap, J, zrow, zmax, epsMode);
+	hipLaunchKernelGGL(k_split_csr<double>, dim3((m + 255) / 256), dim3(256), 0, st, m, val, rowptr,
+	                   work, split, lds, shift, cap, J, zrow, zmax, epsMode);
+}
+
+void launch_split_csr(hipStream_t st, int m, const float *val, const int *rowptr, float *work,
+                      double *split, int64_t lds, int shift, int cap, const uint16_t *JF, int *zrow,
+                      int *zmax, int epsMode) {
+	hipLaunchKernelGGL(k_split_csr<float>, dim3((m + 255) / 256), dim3(256), 0, st, m, val, rowptr,
+	                   work, split, lds, shift, cap, JF, zrow, zmax, epsMode);
 }
 
 void launch_zero_slots_csr(hipStream_t st, int m, const int *rowptr, double *split, int64_t lds,

@@ -32,9 +32,10 @@ def declared_functions():
 
 def test_header_declares_reference_entry_points():
     names = declared_functions()
-    for n in ("ozblasCreate", "ozblasDestroy", "ozblasDdot", "ozblasDnrm2", "ozblasDaxpy",
-              "ozblasDgemv", "ozblasDgemm", "ozblasDcsrmv", "ozblasDcsrmvSplitA", "ozblasDcg"):
-        assert n in names
+    for p in "DS":
+        for r in ("dot", "nrm2", "axpy", "gemv", "gemm", "csrmv", "csrmvSplitA", "cg"):
+            assert f"ozblas{p}{r}" in names
+    assert "ozblasCreate" in names and "ozblasDestroy" in names
 
 
 def test_library_exports_every_declared_symbol(lib):
@@ -88,6 +89,51 @@ def test_rho_modes_match_libm(lib, oracle):
             want = oracle.rho(d, mode)
             assert lib.ozblas_internal_rho(d, mode) == want, (mode, d)
             assert lib.ozblas_internal_rho_row_mode(d, mode) == want, (mode, d)
+
+
+def _binary32_edge_floats():
+    """The first 300 floats above every binary32 power of two (where a log2 rounded to float may
+    equal the integer), the float just below, random floats, and the special values."""
+    xs = []
+    for e in range(-149, 128):
+        base = np.ldexp(np.float32(1), e)
+        u = np.ldexp(np.float32(1), max(e - 23, -149))
+        nv = (1 << 23) - 1 if e - 23 >= -149 else (1 << (e + 149)) - 1
+        xs += [base + np.float32(i) * u for i in range(0, min(300, nv + 1))]
+        if e > -149:
+            xs.append(np.nextafter(base, np.float32(0)))
+    rng = np.random.default_rng(2)
+    xs += list(np.ldexp(rng.uniform(1, 2, 20000), rng.integers(-149, 127, 20000)).astype(np.float32))
+    xs += [0.0, -0.0, np.inf, np.nan, -3.0, 1e-45, 3.4028235e38]
+    return np.array(xs, dtype=np.float32)
+
+
+def test_binary32_tau_rules_match_libm(lib, oracle):
+    """The binary32 split's tau = (short)ceil((float)log2((double)max)) and NearSum's
+    ceil(log2f(mu)) through their tables equal glibc on every binade's critical values."""
+    import ozblas_amd
+    libm = ctypes.CDLL("libm.so.6")
+    libm.log2f.restype = ctypes.c_float
+    libm.log2f.argtypes = [ctypes.c_float]
+    for x in _binary32_edge_floats():
+        xf = float(x)
+        assert ozblas_amd.internal_tau_f(xf) == oracle.s_tau(xf), xf
+        if np.isfinite(x) and x > 0:
+            want = math.ceil(libm.log2f(xf))
+            assert ozblas_amd.internal_nearsum_exp_f(xf) == want, xf
+
+
+def test_binary32_rho_rules_match_libm(lib, oracle):
+    """getRho<float,float> for splitEpsMode 0, 2 and 9: the host form and the integer form the
+    device uses for CSR rows agree with the oracle's libm evaluation."""
+    import ozblas_amd
+    dims = list(range(0, 5000)) + [2 ** p + o for p in range(12, 31) for o in (-1, 0, 1)]
+    dims += [10 ** 6, 123456789, 2 ** 31 - 1]
+    for mode in (0, 2, 9):
+        for d in dims:
+            want = oracle.s_rho(d, mode)
+            assert ozblas_amd.internal_rho_f(d, mode) == want, (mode, d)
+            assert ozblas_amd.internal_rho_row_f(d, mode) == want, (mode, d)
 
 
 def test_gen_phi_matches_oracle_generator(oracle):

@@ -140,3 +140,95 @@ def test_integer_residue_key_matches_cmod():
             key = residue_key(x, k)
             got = np.ldexp(float(key), k - 64)
             assert got == want, (x, k, got, want)
+
+
+# ---- binary32: the same model with 24 for 53 (k_vsplit_scan<.., float>) ------------------------
+def model_split_f(oracle, x, rho, cap, reproMode):
+    """Returns (count, slots) of the model, or None where the kernels hand the split to the
+    fallback loop (rho outside [3, 24], sigma outside the binary32 normal range)."""
+    x = x.astype(np.float64)
+    if rho < 3 or rho > 24:
+        return None
+    M = float(np.max(np.abs(x))) if len(x) else 0.0
+    ks = []
+    zero = False
+    s = 0
+    for s in range(cap):
+        tau = oracle.s_tau(np.float32(M))
+        if rho + tau > 126 or rho + tau - 1 < -126:
+            return None
+        k = rho + tau - 24
+        ks.append(k)
+        M = float(np.max(np.abs(cmod(x, k)))) if len(x) else 0.0
+        if M == 0.0:
+            zero = True
+            break
+    else:
+        s = cap
+    count = 1 if cap <= 1 else (max(s, 1) + 1 if zero else cap)
+    if reproMode == 0:
+        count += 1
+    last = s if zero else cap - 1
+    slots, r = [], x.copy()
+    for q in range(last + 1):
+        c = cmod(x, ks[q])
+        slots.append(r - c)
+        r = c
+    slots.append(np.zeros_like(x) if zero else r)
+    while len(slots) < count:
+        slots.append(np.zeros_like(x))
+    return count, slots
+
+
+def oracle_split_f(oracle, x, nSplitMax, reproMode):
+    import ctypes as C
+    L = oracle.lib()
+    n = len(x)
+    h = oracle.Handle(1 << 24, nSplitMax=nSplitMax, reproMode=reproMode)
+    cap = 31
+    split = np.zeros(n * (cap + 2), dtype=np.float32)
+    out = np.zeros(n, dtype=np.float32)
+    mx = np.zeros(max(n, 1), dtype=np.float32)
+    xin = np.ascontiguousarray(x, dtype=np.float32)
+    cnt = L.ozo_s_split(h._h, n, 1, xin, n, out, n, split, n, mx)
+    return cnt, [split[s * n:(s + 1) * n] for s in range(cnt)]
+
+
+def vectors_f():
+    rng = np.random.default_rng(12)
+    yield rng.uniform(-1, 1, 1000).astype(np.float32)
+    yield (rng.standard_normal(777) * np.exp(6 * rng.standard_normal(777))).astype(np.float32)
+    yield np.ldexp(rng.integers(-2**12, 2**12, 500).astype(np.float32), -20)  # few bits
+    v = np.ldexp(np.float32(1), rng.integers(-30, 30, 300)).astype(np.float32)
+    v[::3] *= -1
+    yield v
+    w = rng.uniform(-1, 1, 400).astype(np.float32)
+    w[::7] = np.nextafter(np.float32(0.125), np.float32(1))
+    w[1::11] = 0
+    yield w
+    yield np.ldexp(rng.integers(0, 2**23, 256).astype(np.float32) + np.float32(0.5), -20)  # ties
+    yield np.zeros(64, dtype=np.float32)
+    yield np.concatenate([rng.uniform(-1, 1, 100), rng.uniform(-1, 1, 100) * 1e-30]).astype(np.float32)
+    yield np.array([1e-40, -3e-42, 1e-45, 0.0], dtype=np.float32)       # subnormal: fallback
+    yield np.array([3e38, -1.0, 2.0 ** 100], dtype=np.float32)         # huge: fallback
+
+
+@pytest.mark.parametrize("reproMode,nSplitMax", [(1, 0), (0, 0), (1, 3), (1, 1), (0, 2)])
+def test_binary32_level_model_matches_reference_loop(oracle, reproMode, nSplitMax):
+    used = 0
+    for x in vectors_f():
+        n = len(x)
+        rho = oracle.s_rho(n, 0)
+        cap = (nSplitMax if nSplitMax > 0 else 30) - (1 if reproMode == 0 else 0)
+        want_cnt, want = oracle_split_f(oracle, x, nSplitMax, reproMode)
+        m = model_split_f(oracle, x, rho, cap, reproMode)
+        if m is None:
+            continue
+        used += 1
+        got_cnt, got = m
+        assert got_cnt == want_cnt, (n, got_cnt, want_cnt)
+        for s in range(want_cnt):
+            g = got[s].astype(np.float32)
+            assert np.array_equal(g.astype(np.float64), got[s]), (n, s)  # slices are floats
+            assert np.array_equal(g.view(np.uint32), want[s].view(np.uint32)), (n, s)
+    assert used >= 7
