@@ -1,0 +1,151 @@
+/*
+ * cuozblas_mi355x.hpp -- the cuOzBLAS C++ templates over the C ABI of ozblas_mi355x.h.
+ *
+ * Callers written against the reference's GPU interface (include/cuozblas.h:67-80, used by the test
+ * drivers through the macros of testing/testing_common.h:223-233,355-365) compile against this
+ * header with no edits to their calls.  Each specialization forwards to the C entry point of the
+ * same instantiation:
+ *
+ *   cuozblasCreate / cuozblasDestroy                     (include/cuozblas.h:67-68)
+ *   cuozblasRnrm2 / Raxpy / Rdot / Rgemv / Rgemm         (include/cuozblas.h:73-77)  <double,...>, <float,...>
+ *   cuozblasRcsrmv / RcsrmvSplitA / Rcg                  (include/cuozblas.h:78-80)  <double,double>, <float,float>
+ *
+ * The reference returns 0 and calls exit(1) on an error; these return the OZBLAS_STATUS_* code
+ * (0 on success), or exit(1) as the reference does when OZBLAS_SHIM_EXIT_ON_ERROR is defined.
+ * Instantiations without a C entry point (the mixed <double,float,float> and binary128 ones) are
+ * declared by the primary templates and left undefined, so using one fails at link time.
+ */
+#ifndef CUOZBLAS_MI355X_HPP
+#define CUOZBLAS_MI355X_HPP
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+
+#include "ozblas_mi355x.h"
+
+typedef ozblasHandle_t cuozblasHandle_t;
+#ifndef OZBLAS_SHIM_NO_CUSPARSE_DESCR
+typedef const void *cusparseMatDescr_t;  // accepted and ignored: general matrix, 0-based CSR
+#endif
+
+inline int32_t cuozblas_status(int32_t s, const char *where) {
+#ifdef OZBLAS_SHIM_EXIT_ON_ERROR
+	if (s != OZBLAS_STATUS_SUCCESS) {
+		fprintf(stderr, "OzBLAS error: %s (status %d)\n", where, (int)s);
+		exit(1);
+	}
+#else
+	(void)where;
+#endif
+	return s;
+}
+
+inline void cuozblasCreate(cuozblasHandle_t *oh, uint64_t workSizeBytes) {
+	cuozblas_status(ozblasCreate(oh, workSizeBytes), "cuozblasCreate");
+}
+inline void cuozblasDestroy(cuozblasHandle_t *oh) { ozblasDestroy(oh); }
+
+// ---- primary templates (the reference's signatures) ----------------------------------------------
+template <typename T1, typename T2, typename T3>
+int32_t cuozblasRnrm2(cuozblasHandle_t *oh, const int32_t n, const T1 *devX, const int32_t incx, T1 *ret);
+template <typename T>
+int32_t cuozblasRaxpy(cuozblasHandle_t *oh, const int32_t n, const T alpha, const T *devX,
+                      const int32_t incx, T *devY, const int32_t incy);
+template <typename T1, typename T2, typename T3>
+int32_t cuozblasRdot(cuozblasHandle_t *oh, const int32_t n, const T1 *devA, const int32_t incx,
+                     const T1 *devB, const int32_t incy, T1 *ret);
+template <typename T1, typename T2, typename T3>
+int32_t cuozblasRgemv(cuozblasHandle_t *oh, const char tranA, const int32_t m, const int32_t n,
+                      const T1 alpha, const T1 *devA, const int32_t lda, const T1 *devB,
+                      const int32_t incx, const T1 beta, T1 *devC, const int32_t incy);
+template <typename T1, typename T2, typename T3>
+int32_t cuozblasRgemm(cuozblasHandle_t *oh, const char tranA, const char tranB, const int32_t m,
+                      const int32_t n, const int32_t k, const T1 alpha, const T1 *devA,
+                      const int32_t lda, const T1 *devB, const int32_t ldb, const T1 beta, T1 *devC,
+                      const int32_t ldc);
+template <typename T1, typename T2>
+int32_t cuozblasRcsrmv(cuozblasHandle_t *oh, const char tranA, const int32_t m, const int32_t n,
+                       const int32_t nnz, const T1 alpha, const cusparseMatDescr_t descrA,
+                       const T1 *devA, const int32_t *devAcolind, const int32_t *devArowptr,
+                       const T1 *devB, const T1 beta, T1 *devC);
+template <typename T1, typename T2>
+T2 *cuozblasRcsrmvSplitA(cuozblasHandle_t *oh, const char tranA, const int32_t m, const int32_t n,
+                         const int32_t nnz, const cusparseMatDescr_t descrA, const T1 *devA,
+                         const int32_t *devArowptr);
+template <typename T1, typename T2>
+int32_t cuozblasRcg(cuozblasHandle_t *oh, const char tranA, const int32_t dimN, const int32_t dimNNZ,
+                    const cusparseMatDescr_t descrA, const T1 *matA, const int32_t *matAcolind,
+                    const int32_t *matArowptr, const T1 *vecB, T1 *vecX, int32_t maxiter, T1 tol);
+
+// ---- specializations: one per C entry point -------------------------------------------------------
+#define CUOZBLAS_MI355X_LEVEL123(T, P)                                                                   \
+	template <>                                                                                         \
+	inline int32_t cuozblasRnrm2<T, T, T>(cuozblasHandle_t * oh, const int32_t n, const T *devX,          \
+	                                      const int32_t incx, T *ret) {                                 \
+		return cuozblas_status(ozblas##P##nrm2(oh, n, devX, incx, ret), "cuozblasRnrm2");               \
+	}                                                                                                   \
+	template <>                                                                                         \
+	inline int32_t cuozblasRaxpy<T>(cuozblasHandle_t * oh, const int32_t n, const T alpha, const T *devX, \
+	                                const int32_t incx, T *devY, const int32_t incy) {                   \
+		return cuozblas_status(ozblas##P##axpy(oh, n, alpha, devX, incx, devY, incy), "cuozblasRaxpy"); \
+	}                                                                                                   \
+	template <>                                                                                         \
+	inline int32_t cuozblasRdot<T, T, T>(cuozblasHandle_t * oh, const int32_t n, const T *devA,           \
+	                                     const int32_t incx, const T *devB, const int32_t incy, T *ret) { \
+		return cuozblas_status(ozblas##P##dot(oh, n, devA, incx, devB, incy, ret), "cuozblasRdot");     \
+	}                                                                                                   \
+	template <>                                                                                         \
+	inline int32_t cuozblasRgemv<T, T, T>(cuozblasHandle_t * oh, const char tranA, const int32_t m,       \
+	                                      const int32_t n, const T alpha, const T *devA,                \
+	                                      const int32_t lda, const T *devB, const int32_t incx,          \
+	                                      const T beta, T *devC, const int32_t incy) {                  \
+		return cuozblas_status(                                                                         \
+		    ozblas##P##gemv(oh, tranA, m, n, alpha, devA, lda, devB, incx, beta, devC, incy),           \
+		    "cuozblasRgemv");                                                                           \
+	}                                                                                                   \
+	template <>                                                                                         \
+	inline int32_t cuozblasRgemm<T, T, T>(cuozblasHandle_t * oh, const char tranA, const char tranB,      \
+	                                      const int32_t m, const int32_t n, const int32_t k,            \
+	                                      const T alpha, const T *devA, const int32_t lda,              \
+	                                      const T *devB, const int32_t ldb, const T beta, T *devC,      \
+	                                      const int32_t ldc) {                                          \
+		return cuozblas_status(                                                                         \
+		    ozblas##P##gemm(oh, tranA, tranB, m, n, k, alpha, devA, lda, devB, ldb, beta, devC, ldc),   \
+		    "cuozblasRgemm");                                                                           \
+	}                                                                                                   \
+	template <>                                                                                         \
+	inline int32_t cuozblasRcsrmv<T, T>(cuozblasHandle_t * oh, const char tranA, const int32_t m,         \
+	                                    const int32_t n, const int32_t nnz, const T alpha,              \
+	                                    const cusparseMatDescr_t descrA, const T *devA,                 \
+	                                    const int32_t *devAcolind, const int32_t *devArowptr,           \
+	                                    const T *devB, const T beta, T *devC) {                         \
+		return cuozblas_status(ozblas##P##csrmv(oh, tranA, m, n, nnz, alpha, descrA, devA, devAcolind,   \
+		                                        devArowptr, devB, beta, devC),                          \
+		                       "cuozblasRcsrmv");                                                       \
+	}                                                                                                   \
+	template <>                                                                                         \
+	inline T *cuozblasRcsrmvSplitA<T, T>(cuozblasHandle_t * oh, const char tranA, const int32_t m,       \
+	                                     const int32_t n, const int32_t nnz,                            \
+	                                     const cusparseMatDescr_t descrA, const T *devA,                \
+	                                     const int32_t *devArowptr) {                                   \
+		T *p = ozblas##P##csrmvSplitA(oh, tranA, m, n, nnz, descrA, devA, devArowptr);                  \
+		if (!p) cuozblas_status(oh->lastStatus, "cuozblasRcsrmvSplitA");                                \
+		return p;                                                                                       \
+	}                                                                                                   \
+	template <>                                                                                         \
+	inline int32_t cuozblasRcg<T, T>(cuozblasHandle_t * oh, const char tranA, const int32_t dimN,        \
+	                                 const int32_t dimNNZ, const cusparseMatDescr_t descrA,             \
+	                                 const T *matA, const int32_t *matAcolind,                          \
+	                                 const int32_t *matArowptr, const T *vecB, T *vecX,                 \
+	                                 int32_t maxiter, T tol) {                                          \
+		return cuozblas_status(ozblas##P##cg(oh, tranA, dimN, dimNNZ, descrA, matA, matAcolind,          \
+		                                     matArowptr, vecB, vecX, maxiter, tol),                      \
+		                       "cuozblasRcg");                                                          \
+	}
+
+CUOZBLAS_MI355X_LEVEL123(double, D)
+CUOZBLAS_MI355X_LEVEL123(float, S)
+#undef CUOZBLAS_MI355X_LEVEL123
+
+#endif

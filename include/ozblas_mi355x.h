@@ -109,7 +109,15 @@ typedef struct ozblasHandle {
 	int32_t splitShift;
 	uint64_t memMaskSplitA;
 
-	/* CG (include/cuozblas.h:50-61): cg_verbose1 is a host array of >= maxiter/verbose + 2 */
+	/* CG (include/cuozblas.h:50-61, src/cpu/ozblas_cg.cpp:40-112).  With verbose > 0 the solver
+	 * writes resi/nrmb of iteration 0 and of every verbose-th iteration (and of a tolerance stop)
+	 * into cg_verbose1, a host array of >= maxiter/verbose + 2 elements of the routine's type
+	 * (double for Dcg, float for Scg).  When cg_verbose2 is set, the same entries of it get the true
+	 * relative residual |b - A x| / |b| (trueresFlag = 1; routine's type; iterates synchronously)
+	 * or the seconds since the start of the solve (trueresFlag = 0; double).  The t_*_total fields
+	 * get the solve's time split as the reference's categories (the synchronous iterations only;
+	 * t_SplitVec_total and t_Sum_total are counted in the SpMV and DOT totals, where they run
+	 * fused). */
 	int32_t trueresFlag;
 	int32_t verbose;
 	int32_t cg_numiter;

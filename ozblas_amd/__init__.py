@@ -212,17 +212,32 @@ class Handle:
             raise OzblasError("ozblasDcsrmvSplitA", self._h.lastStatus)
         return p
 
-    def dcg(self, n, nnz, val, colind, rowptr, b, x, maxiter, tol, history=True):
+    def _cg(self, fn, name, dtype, n, nnz, val, colind, rowptr, b, x, maxiter, tol, history,
+            second):
         import numpy as np
-        hist = np.zeros(maxiter + 2, dtype=np.float64)
+        hist = np.zeros(maxiter + 2, dtype=dtype)
+        v2 = None
         if history:
             self._h.verbose = 1
             self._h.cg_verbose1 = hist.ctypes.data
-        st = _dcg(self.ref, b"N", n, nnz, None, _ptr(val), _ptr(colind), _ptr(rowptr), _ptr(b),
-                  _ptr(x), maxiter, tol)
+            if second is not None:  # "trueres": |b - A x| / |b| (dtype); "times": seconds (float64)
+                v2 = np.zeros(maxiter + 2, dtype=dtype if second == "trueres" else np.float64)
+                self._h.trueresFlag = 1 if second == "trueres" else 0
+                self._h.cg_verbose2 = v2.ctypes.data
+        st = fn(self.ref, b"N", n, nnz, None, _ptr(val), _ptr(colind), _ptr(rowptr), _ptr(b),
+                _ptr(x), maxiter, tol)
         self._h.cg_verbose1 = None
-        self._check("ozblasDcg", st)
-        return hist[: self._h.cg_numiter + 1]
+        self._h.cg_verbose2 = None
+        self._h.trueresFlag = 0
+        self._check(name, st)
+        k = self._h.cg_numiter + 1
+        return hist[:k] if v2 is None else (hist[:k], v2[:k])
+
+    def dcg(self, n, nnz, val, colind, rowptr, b, x, maxiter, tol, history=True, second=None):
+        """Residual history resi/nrmb (cg_verbose1); with second="trueres" or "times" also the
+        second history (cg_verbose2) as a tuple."""
+        return self._cg(_dcg, "ozblasDcg", "float64", n, nnz, val, colind, rowptr, b, x, maxiter,
+                        tol, history, second)
 
     # ---- binary32 (the reference's <float,float,float> instantiations) -----------------------
     def sgemm(self, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc):
@@ -256,17 +271,10 @@ class Handle:
             raise OzblasError("ozblasScsrmvSplitA", self._h.lastStatus)
         return p
 
-    def scg(self, n, nnz, val, colind, rowptr, b, x, maxiter, tol, history=True):
-        import numpy as np
-        hist = np.zeros(maxiter + 2, dtype=np.float64)
-        if history:
-            self._h.verbose = 1
-            self._h.cg_verbose1 = hist.ctypes.data
-        st = _scg(self.ref, b"N", n, nnz, None, _ptr(val), _ptr(colind), _ptr(rowptr), _ptr(b),
-                  _ptr(x), maxiter, tol)
-        self._h.cg_verbose1 = None
-        self._check("ozblasScg", st)
-        return hist[: self._h.cg_numiter + 1]
+    def scg(self, n, nnz, val, colind, rowptr, b, x, maxiter, tol, history=True, second=None):
+        """As dcg; the histories of the binary32 solver are float arrays, as the reference's."""
+        return self._cg(_scg, "ozblasScg", "float32", n, nnz, val, colind, rowptr, b, x, maxiter,
+                        tol, history, second)
 
     def dgemm_batched(self, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc):
         bsz = len(m)

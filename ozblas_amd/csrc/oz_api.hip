@@ -1004,32 +1004,101 @@ namespace oz {
 
 template <class T>
 struct CgVecs {
-	T *R = nullptr, *P = nullptr, *Q = nullptr, *X0 = nullptr;
+	T *R = nullptr, *P = nullptr, *Q = nullptr, *X0 = nullptr, *T_ = nullptr;
 };
 
-// T = float: the <float,float> instantiation (float scalars, the double sqrt rounded to float)
+// The second history of ozblasRcg (src/cpu/ozblas_cg.cpp:47-49,64-69,101-109), written next to
+// each entry of cg_verbose1: with trueresFlag the true relative residual |b - A x| / |b| (a T array),
+// otherwise the seconds since the start of the solve (doubles).  The true residual takes t = b - A x
+// from the reference's plain CSRMV and |t| as sqrt(RN(sum t_i^2)) (an Ozaki DOT under NearSum); the
+// reference's cblas_?nrm2 rounds in an order of its BLAS's own choosing.
+template <class T>
+struct CgDiag {
+	void *v2 = nullptr;
+	bool trueres = false;
+	WallTimer t1;
+	const T *A = nullptr, *b = nullptr;
+	const int *colind = nullptr, *rowptr = nullptr;
+	T *t = nullptr;
+	int n = 0;
+	T nrmb = 0;
+};
+
+template <class T>
+static int32_t cg_record2(ozblasHandle_t *oh, const CgDiag<T> &d, int idx, const T *x) {
+	if (!d.v2) return OZBLAS_STATUS_SUCCESS;
+	if (!d.trueres) {
+		((double *)d.v2)[idx] = d.t1.sec();
+		return OZBLAS_STATUS_SUCCESS;
+	}
+	launch_true_residual(be(oh)->stream, d.n, d.A, d.colind, d.rowptr, x, d.b, d.t);
+	// a correctly rounded sum of squares: NearSum with the exact split, whatever the handle holds
+	const int32_t sv[5] = {oh->sumMode, oh->reproMode, oh->nSplitMax, oh->splitEpsMode, oh->splitShift};
+	const float st[3] = {oh->nSplitA, oh->nSplitB, oh->nSplitC};
+	oh->sumMode = 1;
+	oh->reproMode = 1;
+	oh->nSplitMax = 0;
+	oh->splitEpsMode = 0;
+	T dd = 0;
+	const int32_t s = dot_impl(oh, d.n, (const T *)d.t, (const T *)d.t, &dd);
+	oh->sumMode = sv[0];
+	oh->reproMode = sv[1];
+	oh->nSplitMax = sv[2];
+	oh->splitEpsMode = sv[3];
+	oh->splitShift = sv[4];
+	oh->nSplitA = st[0];
+	oh->nSplitB = st[1];
+	oh->nSplitC = st[2];
+	OZ_TRY(s);
+	((T *)d.v2)[idx] = (T)sqrt((double)dd) / d.nrmb;
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+// T = float: the <float,float> instantiation (float scalars, the double sqrt rounded to float).
+// The time totals of the handle follow the reference's categories (src/cpu/ozblas_cg.cpp:89,99,
+// ozblas_csrmv.cpp:172-177, ozblas_dot.cpp:30-35) with the vector split and the sum of a CSRMV or DOT
+// counted in its SpMV or DOT total, where they run fused.
 template <class T>
 static int32_t cg_host_loop(ozblasHandle_t *oh, int n, int nnz, const double *AS, const int *colind,
                             const int *rowptr, T *x, int maxiter, T tol, T nrmb, T dold,
-                            const CgVecs<T> &v, double *hist) {
-	hipStream_t st = be(oh)->stream;
+                            const CgVecs<T> &v, T *hist, const CgDiag<T> &diag) {
+	Backend *b = be(oh);
+	hipStream_t st = b->stream;
 	T tmp, alpha, dnew, resi, beta;
 	int vcnt = 1;
 	while (oh->cg_numiter < maxiter) {
 		oh->cg_numiter++;
+		WallTimer tc;
 		OZ_TRY(csrmv_impl(oh, n, n, nnz, (T)1, (const void *)AS, colind, rowptr, v.P, (T)0, v.Q));
+		oh->t_SpMV_SpMM_total += tc.sec();
+		WallTimer td;
 		OZ_TRY(dot_entry(oh, n, v.P, 1, v.Q, 1, &tmp));
+		double tdot = td.sec();
 		alpha = dold / tmp;
+		OZ_HIP(hipEventRecord(b->ev[6], st));
 		launch_axpy_host(st, n, alpha, v.P, x, oh->reproMode != 0);
 		launch_axpy_host(st, n, -alpha, v.Q, v.R, oh->reproMode != 0);
+		OZ_HIP(hipEventRecord(b->ev[7], st));
+		WallTimer td2;
 		OZ_TRY(dot_entry(oh, n, v.R, 1, v.R, 1, &dnew));
+		tdot += td2.sec();
+		float axpyMs = 0.f;
+		(void)hipEventElapsedTime(&axpyMs, b->ev[6], b->ev[7]);
 		resi = (T)sqrt((double)dnew);
 		beta = dnew / dold;
 		dold = dnew;
+		OZ_HIP(hipEventRecord(b->ev[6], st));
 		launch_scal_axpy(st, n, beta, v.R, v.P, oh->reproMode != 0);
+		OZ_HIP(hipEventRecord(b->ev[7], st));
+		OZ_HIP(hipEventSynchronize(b->ev[7]));
+		float scalMs = 0.f;
+		(void)hipEventElapsedTime(&scalMs, b->ev[6], b->ev[7]);
+		oh->t_AXPY_SCAL_total += 1e-3 * ((double)axpyMs + scalMs);
+		oh->t_DOT_NRM2_total += tdot;
 		if (oh->verbose > 0) {
 			if (resi / nrmb < tol || oh->cg_numiter % oh->verbose == 0) {
 				if (hist) hist[vcnt] = resi / nrmb;
+				OZ_TRY(cg_record2(oh, diag, vcnt, (const T *)x));
 				vcnt++;
 			}
 		}
@@ -1042,9 +1111,21 @@ static int32_t cg_host_loop(ozblasHandle_t *oh, int n, int nnz, const double *AS
 // Device-resident iterations; *overflow = 1 when the host path must redo them.
 static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *AS, const int *colind,
                               const int *rowptr, double *x, int maxiter, double tol, double nrmb,
-                              double dold, const CgVecs<double> &v, double *hist, int *overflow) {
+                              double dold, const CgVecs<double> &v, double *hist, int *overflow,
+                              const CgDiag<double> &diag) {
 	Backend *b = be(oh);
 	hipStream_t st = b->stream;
+	// times of the second history: an event after every iteration whose residual may be recorded
+	// (every verbose-th one, and the last enqueued one for a tolerance stop)
+	const bool stamps = oh->verbose > 0 && diag.v2 != nullptr && !diag.trueres;
+	std::vector<hipEvent_t> evs;
+	std::vector<int> evIt;
+	hipEvent_t ev0 = nullptr;
+	double off = 0.0;
+	auto free_events = [&]() {
+		for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+		if (ev0) (void)hipEventDestroy(ev0);
+	};
 	const int cap = split_cap(oh);
 	const int rho = rho_host(n, oh->splitEpsMode);
 	const int64_t lds = rup(n, 32);
@@ -1082,6 +1163,11 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 	h.maxVecSlices = maxVec;
 	OZ_HIP(hipMemcpyAsync(S, &h, sizeof(CgDev), hipMemcpyHostToDevice, st));
 	int *infoP = info, *infoQ = info + 2, *infoR = info + 4;
+	if (stamps) {
+		OZ_HIP(hipEventCreate(&ev0));
+		OZ_HIP(hipEventRecord(ev0, st));
+		off = diag.t1.sec();
+	}
 	for (int it = 0; it < maxiter; it++) {
 		OZ_HIP(launch_vsplit(st, n, v.P, slP, lds, rho, 1, cap, oh->reproMode, b->dTau, mxP, infoP,
 		                     vscr, plan, &S->stopAt, it));
@@ -1097,15 +1183,42 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 		launch_dots_dev(st, n, slR, lds, infoR, slR, lds, infoR, vals, ctr, mxR, mxR, cap + 2, b->dTau,
 		                fsum, S, it, 2, nullptr);
 		launch_scal_axpy_dev(st, n, S, it, v.R, v.P);
+		if (stamps && ((it + 1) % oh->verbose == 0 || it + 1 == maxiter || it % 16 == 15)) {
+			hipEvent_t e;
+			OZ_HIP(hipEventCreate(&e));
+			OZ_HIP(hipEventRecord(e, st));
+			evs.push_back(e);
+			evIt.push_back(it);
+		}
 	}
 	OZ_HIP(hipGetLastError());
 	OZ_HIP(hipMemcpyAsync(&h, S, sizeof(CgDev), hipMemcpyDeviceToHost, st));
 	OZ_HIP(hipStreamSynchronize(st));
 	*overflow = h.overflow;
-	if (h.overflow) return OZBLAS_STATUS_SUCCESS;
+	if (h.overflow) {
+		free_events();
+		return OZBLAS_STATUS_SUCCESS;
+	}
 	oh->cg_numiter = (h.stopAt < maxiter) ? h.stopAt : maxiter;
 	if (oh->verbose > 0 && hist && h.vcnt > 1)
 		OZ_HIP(hipMemcpy(hist + 1, dhist + 1, 8ull * (h.vcnt - 1), hipMemcpyDeviceToHost));
+	if (stamps) {
+		// entry j of the history is the j-th iteration (1-based count) with count % verbose == 0,
+		// or the last one when the tolerance stopped the solve; an iteration past the stop did no
+		// work, so the first event at or after it carries its time
+		double *t2 = (double *)diag.v2;
+		const int last = oh->cg_numiter - 1;
+		size_t q = 0;
+		int j = 1;
+		for (int it = 0; it <= last && j < h.vcnt; it++) {
+			if ((it + 1) % oh->verbose != 0 && it != last) continue;
+			while (q + 1 < evIt.size() && evIt[q] < it) q++;
+			float ms = 0.f;
+			(void)hipEventElapsedTime(&ms, ev0, evs[q]);
+			t2[j++] = off + 1e-3 * ms;
+		}
+	}
+	free_events();
 	return OZBLAS_STATUS_SUCCESS;
 }
 
@@ -1120,14 +1233,28 @@ static int32_t cg_entry(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz,
 	if (oh->sumMode != 0 && oh->sumMode != 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
 	Backend *b = be(oh);
 	hipStream_t st = b->stream;
-	double *hist = (double *)oh->cg_verbose1;
+	CgDiag<T> diag;  // starts the clock of the second history (t1 of src/cpu/ozblas_cg.cpp:58)
+	T *hist = (T *)oh->cg_verbose1;  // TYPE1 *ptr_cg_verbose1 (src/cpu/ozblas_cg.cpp:47)
 	oh->cg_numiter = 0;
 	oh->cgDevicePath = 0;
+	oh->t_SplitMat_total = oh->t_SplitVec_total = oh->t_Sum_total = 0.;
+	oh->t_AXPY_SCAL_total = oh->t_DOT_NRM2_total = oh->t_SpMV_SpMM_total = 0.;
 	CgVecs<T> v;
 	OZ_HIP(hipMalloc(&v.R, E * n));
 	OZ_HIP(hipMalloc(&v.P, E * n));
 	OZ_HIP(hipMalloc(&v.Q, E * n));
 	OZ_HIP(hipMalloc(&v.X0, E * n));
+	if (oh->verbose > 0 && oh->cg_verbose2) {
+		diag.v2 = oh->cg_verbose2;
+		diag.trueres = oh->trueresFlag != 0;
+		diag.A = A;
+		diag.b = b_;
+		diag.colind = colind;
+		diag.rowptr = rowptr;
+		diag.n = n;
+		if (diag.trueres) OZ_HIP(hipMalloc(&v.T_, E * n));
+		diag.t = v.T_;
+	}
 	int32_t s = OZBLAS_STATUS_SUCCESS;
 	T nrmb = 0, dold = 0, resi = 0;
 	int overflow = 1;
@@ -1138,7 +1265,11 @@ static int32_t cg_entry(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz,
 		s = OZBLAS_STATUS_EXECUTION_FAILED;
 		goto done;
 	}
-	AS = csrmv_split_a(oh, transA, n, nnz, A, rowptr);
+	{
+		WallTimer ts;
+		AS = csrmv_split_a(oh, transA, n, nnz, A, rowptr);
+		oh->t_SplitMat_total = ts.sec();
+	}
 	if (!AS) {
 		s = oh->lastStatus;
 		goto done;
@@ -1159,19 +1290,26 @@ static int32_t cg_entry(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz,
 	s = dot_entry(oh, n, v.R, 1, v.R, 1, &dold);
 	if (s) goto done;
 	resi = (T)sqrt((double)dold);
-	if (oh->verbose > 0 && hist) hist[0] = resi / nrmb;
+	diag.nrmb = nrmb;
+	if (oh->verbose > 0) {
+		if (hist) hist[0] = resi / nrmb;
+		s = cg_record2(oh, diag, 0, (const T *)x);
+		if (s) goto done;
+	}
 	if (hipMemcpyAsync(v.P, v.R, E * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
 		s = OZBLAS_STATUS_EXECUTION_FAILED;
 		goto done;
 	}
 	{
+		// the true residual of a recorded iteration needs its x: that history runs synchronously
 		bool devicePath = E == 8 && oh->cgDeviceSlices >= 0 && oh->reproMode != 0 &&
-		                  oh->fastMode == 0 && oh->splitEpsMode != 2 && oh->nSplitA_ <= 4 && maxiter > 0;
+		                  oh->fastMode == 0 && oh->splitEpsMode != 2 && oh->nSplitA_ <= 4 && maxiter > 0 &&
+		                  !diag.trueres;
 		if constexpr (E == 8) {
 			if (devicePath) {
 				wreset(oh, b->keep);
 				s = cg_device_loop(oh, n, oh->nSplitA_, AS, colind, rowptr, x, maxiter, tol, nrmb, dold, v,
-				                   hist, &overflow);
+				                   hist, &overflow, diag);
 				wreset(oh, b->keep);
 				if (s == OZBLAS_STATUS_WORKSPACE) {  // nothing ran: iterate on the synchronous path
 					s = OZBLAS_STATUS_SUCCESS;
@@ -1197,7 +1335,7 @@ static int32_t cg_entry(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz,
 				}
 			}
 			oh->cg_numiter = 0;
-			s = cg_host_loop(oh, n, nnz, AS, colind, rowptr, x, maxiter, tol, nrmb, dold, v, hist);
+			s = cg_host_loop(oh, n, nnz, AS, colind, rowptr, x, maxiter, tol, nrmb, dold, v, hist, diag);
 		}
 	}
 done:
@@ -1206,6 +1344,7 @@ done:
 	(void)hipFree(v.P);
 	(void)hipFree(v.Q);
 	(void)hipFree(v.X0);
+	(void)hipFree(v.T_);
 	oh->memMaskSplitA = 0;
 	b->keep = 0;
 	oh->splitShift = 1;

@@ -147,6 +147,11 @@ hipError_t launch_vsplit(hipStream_t st, int n, const float *x, double *slices, 
                          const int *stopAt = nullptr, int it = 0);
 void launch_axpy_host(hipStream_t st, int n, double alpha, const double *x, double *y, int fused);
 void launch_axpy_host(hipStream_t st, int n, float alpha, const float *x, float *y, int fused);
+// t = b - A x by the reference's plain CSRMV (the CG's true residual)
+void launch_true_residual(hipStream_t st, int n, const double *val, const int *colind,
+                          const int *rowptr, const double *x, const double *b, double *t);
+void launch_true_residual(hipStream_t st, int n, const float *val, const int *colind,
+                          const int *rowptr, const float *x, const float *b, float *t);
 void launch_scal_axpy(hipStream_t st, int n, double beta, const double *r, double *p, int fused);
 void launch_scal_axpy(hipStream_t st, int n, float beta, const float *r, float *p, int fused);
 

@@ -847,6 +847,20 @@ __global__ void k_scal_axpy(int n, T beta, const T *r, T *p, int fused) {
 	}
 }
 
+// t = b - A x with the reference's plain CSRMV (csrmv, src/cpu/ozblas_blas_wrapper.cpp:755-778:
+// an fma1 chain per row, then fma1(-1, Tr, 1 * t) as beta = 1), the first step of the CG's true
+// residual (getTrueResidual, src/cpu/ozblas_cg.cpp:4-21)
+template <class T>
+__global__ void k_true_residual(int n, const T *val, const int *colind, const int *rowptr, const T *x,
+                                const T *b, T *t) {
+	for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
+	     r += (int64_t)gridDim.x * blockDim.x) {
+		T tr = (T)0;
+		for (int i = rowptr[r]; i < rowptr[r + 1]; i++) tr = fma1_t(val[i], x[colind[i]], tr);
+		t[r] = fma1_t((T)-1, tr, (T)1 * b[r]);
+	}
+}
+
 static int grid1(int64_t n) {
 	int64_t g = (n + 255) / 256;
 	if (g > 2048) g = 2048;
@@ -866,6 +880,18 @@ void launch_axpy_host(hipStream_t st, int n, float alpha, const float *x, float 
 		hipLaunchKernelGGL(k_axpy<float>, dim3(grid1(n)), dim3(256), 0, st, n, alpha, x, y);
 	else
 		hipLaunchKernelGGL(k_axpy_plain<float>, dim3(grid1(n)), dim3(256), 0, st, n, alpha, x, y);
+}
+
+void launch_true_residual(hipStream_t st, int n, const double *val, const int *colind,
+                          const int *rowptr, const double *x, const double *b, double *t) {
+	hipLaunchKernelGGL(k_true_residual<double>, dim3(grid1(n)), dim3(256), 0, st, n, val, colind, rowptr,
+	                   x, b, t);
+}
+
+void launch_true_residual(hipStream_t st, int n, const float *val, const int *colind,
+                          const int *rowptr, const float *x, const float *b, float *t) {
+	hipLaunchKernelGGL(k_true_residual<float>, dim3(grid1(n)), dim3(256), 0, st, n, val, colind, rowptr,
+	                   x, b, t);
 }
 
 void launch_scal_axpy(hipStream_t st, int n, double beta, const double *r, double *p, int fused) {

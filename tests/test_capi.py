@@ -219,3 +219,18 @@ def test_certified_rounding_from_leading_levels(lib):
             refused += 1
             assert lo != hi or (lo == hi == 0.0 and np.signbit(lo) != np.signbit(hi))
     assert accepted > 100 and refused > 100
+
+
+def test_cpp_shim_binds_every_entry_point():
+    """The cuOzBLAS C++ templates of include/cuozblas_mi355x.hpp compile (the check program built
+    by build()) and resolve to the library's D and S entry points."""
+    import subprocess
+    exe = os.path.join(ROOT, "ozblas_amd", "_build", "shim_check")
+    if not os.path.exists(exe):
+        pytest.fail("ozblas_amd/_build/shim_check is not built (run __graft_entry__.build())")
+    out = subprocess.run(["nm", "-D", "--undefined-only", exe], capture_output=True, text=True,
+                         check=True).stdout
+    used = set(re.findall(r"\b(ozblas\w+)\b", out))
+    for p in "DS":
+        for r in ("dot", "nrm2", "axpy", "gemv", "gemm", "csrmv", "csrmvSplitA", "cg"):
+            assert f"ozblas{p}{r}" in used
