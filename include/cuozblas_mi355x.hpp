@@ -7,12 +7,13 @@
  * same instantiation:
  *
  *   cuozblasCreate / cuozblasDestroy                     (include/cuozblas.h:67-68)
- *   cuozblasRnrm2 / Raxpy / Rdot / Rgemv / Rgemm         (include/cuozblas.h:73-77)  <double,...>, <float,...>
- *   cuozblasRcsrmv / RcsrmvSplitA / Rcg                  (include/cuozblas.h:78-80)  <double,double>, <float,float>
+ *   cuozblasRnrm2 / Raxpy / Rdot / Rgemv / Rgemm         (include/cuozblas.h:73-77)
+ *   cuozblasRcsrmv / RcsrmvSplitA / Rcg                  (include/cuozblas.h:78-80)
+ * at <double,...> (ozblasD*), <float,...> (ozblasS*) and the mixed <double,float,...> (ozblasDS*).
  *
  * The reference returns 0 and calls exit(1) on an error; these return the OZBLAS_STATUS_* code
  * (0 on success), or exit(1) as the reference does when OZBLAS_SHIM_EXIT_ON_ERROR is defined.
- * Instantiations without a C entry point (the mixed <double,float,float> and binary128 ones) are
+ * Instantiations without a C entry point (<float,double,double> and the binary128 ones) are
  * declared by the primary templates and left undefined, so using one fails at link time.
  */
 #ifndef CUOZBLAS_MI355X_HPP
@@ -147,5 +148,65 @@ int32_t cuozblasRcg(cuozblasHandle_t *oh, const char tranA, const int32_t dimN, 
 CUOZBLAS_MI355X_LEVEL123(double, D)
 CUOZBLAS_MI355X_LEVEL123(float, S)
 #undef CUOZBLAS_MI355X_LEVEL123
+
+// ---- the mixed instantiations (binary64 data, binary32 slices) -----------------------------------
+template <>
+inline int32_t cuozblasRnrm2<double, float, float>(cuozblasHandle_t *oh, const int32_t n, const double *devX,
+                                                   const int32_t incx, double *ret) {
+	return cuozblas_status(ozblasDSnrm2(oh, n, devX, incx, ret), "cuozblasRnrm2");
+}
+template <>
+inline int32_t cuozblasRdot<double, float, float>(cuozblasHandle_t *oh, const int32_t n, const double *devA,
+                                                  const int32_t incx, const double *devB,
+                                                  const int32_t incy, double *ret) {
+	return cuozblas_status(ozblasDSdot(oh, n, devA, incx, devB, incy, ret), "cuozblasRdot");
+}
+template <>
+inline int32_t cuozblasRgemv<double, float, float>(cuozblasHandle_t *oh, const char tranA, const int32_t m,
+                                                   const int32_t n, const double alpha, const double *devA,
+                                                   const int32_t lda, const double *devB, const int32_t incx,
+                                                   const double beta, double *devC, const int32_t incy) {
+	return cuozblas_status(ozblasDSgemv(oh, tranA, m, n, alpha, devA, lda, devB, incx, beta, devC, incy),
+	                       "cuozblasRgemv");
+}
+template <>
+inline int32_t cuozblasRgemm<double, float, float>(cuozblasHandle_t *oh, const char tranA, const char tranB,
+                                                   const int32_t m, const int32_t n, const int32_t k,
+                                                   const double alpha, const double *devA, const int32_t lda,
+                                                   const double *devB, const int32_t ldb, const double beta,
+                                                   double *devC, const int32_t ldc) {
+	return cuozblas_status(
+	    ozblasDSgemm(oh, tranA, tranB, m, n, k, alpha, devA, lda, devB, ldb, beta, devC, ldc),
+	    "cuozblasRgemm");
+}
+template <>
+inline int32_t cuozblasRcsrmv<double, float>(cuozblasHandle_t *oh, const char tranA, const int32_t m,
+                                             const int32_t n, const int32_t nnz, const double alpha,
+                                             const cusparseMatDescr_t descrA, const double *devA,
+                                             const int32_t *devAcolind, const int32_t *devArowptr,
+                                             const double *devB, const double beta, double *devC) {
+	return cuozblas_status(ozblasDScsrmv(oh, tranA, m, n, nnz, alpha, descrA, devA, devAcolind, devArowptr,
+	                                     devB, beta, devC),
+	                       "cuozblasRcsrmv");
+}
+template <>
+inline float *cuozblasRcsrmvSplitA<double, float>(cuozblasHandle_t *oh, const char tranA, const int32_t m,
+                                                  const int32_t n, const int32_t nnz,
+                                                  const cusparseMatDescr_t descrA, const double *devA,
+                                                  const int32_t *devArowptr) {
+	float *p = ozblasDScsrmvSplitA(oh, tranA, m, n, nnz, descrA, devA, devArowptr);
+	if (!p) cuozblas_status(oh->lastStatus, "cuozblasRcsrmvSplitA");
+	return p;
+}
+template <>
+inline int32_t cuozblasRcg<double, float>(cuozblasHandle_t *oh, const char tranA, const int32_t dimN,
+                                          const int32_t dimNNZ, const cusparseMatDescr_t descrA,
+                                          const double *matA, const int32_t *matAcolind,
+                                          const int32_t *matArowptr, const double *vecB, double *vecX,
+                                          int32_t maxiter, double tol) {
+	return cuozblas_status(ozblasDScg(oh, tranA, dimN, dimNNZ, descrA, matA, matAcolind, matArowptr, vecB,
+	                                  vecX, maxiter, tol),
+	                       "cuozblasRcg");
+}
 
 #endif

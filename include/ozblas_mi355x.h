@@ -202,6 +202,31 @@ int32_t ozblasScg(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz, const
                   const float *A, const int32_t *colind, const int32_t *rowptr, const float *b,
                   float *x, int32_t maxiter, float tol);
 
+/* mixed: binary64 data split into binary32 slices -- the reference's <double,float,float> /
+ * <double,float> instantiations (src/cpu/ozblas_gemm.cpp:319, ozblas_gemv.cpp:33, ozblas_dot.cpp:43,
+ * ozblas_nrm2.cpp:35, ozblas_csrmv.cpp:185,275, ozblas_cg.cpp:132), reached through the same
+ * templates of include/cuozblas.h:73-80.  Bitwise equal to the reference CPU build; sumMode = 0
+ * (Fsum) only, as the reference's mixed NearSum is not available (src/cpu/ozblas_sum.cpp:275-330):
+ * other sumModes and Dot2 return NOT_SUPPORTED.  ozblasDScsrmvSplitA returns the library's slice
+ * buffer typed as the reference's float *, to be passed back as A of ozblasDScsrmv. */
+int32_t ozblasDSgemm(ozblasHandle_t *oh, char transA, char transB, int32_t m, int32_t n, int32_t k,
+                     double alpha, const double *A, int32_t lda, const double *B, int32_t ldb,
+                     double beta, double *C, int32_t ldc);
+int32_t ozblasDSgemv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, double alpha,
+                     const double *A, int32_t lda, const double *x, int32_t incx, double beta,
+                     double *y, int32_t incy);
+int32_t ozblasDSdot(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx, const double *y,
+                    int32_t incy, double *ret);
+int32_t ozblasDSnrm2(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx, double *ret);
+int32_t ozblasDScsrmv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, int32_t nnz,
+                      double alpha, const void *descrA, const double *A, const int32_t *colind,
+                      const int32_t *rowptr, const double *x, double beta, double *y);
+float *ozblasDScsrmvSplitA(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, int32_t nnz,
+                           const void *descrA, const double *A, const int32_t *rowptr);
+int32_t ozblasDScg(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz, const void *descrA,
+                   const double *A, const int32_t *colind, const int32_t *rowptr, const double *b,
+                   double *x, int32_t maxiter, double tol);
+
 /* batched uneven GEMM (no reference counterpart; the reference batches only the slice pairs of one
  * GEMM, src/cuda/cuozblas_gemm.cu:150-190).  Arrays of per-problem sizes and device pointers live
  * in host memory.  Every C[p] is bitwise equal to ozblasDgemm on problem p alone. */

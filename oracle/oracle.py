@@ -136,6 +136,30 @@ def lib():
                                C.c_float]
         L.ozo_s_init_mat.argtypes = [C.c_int32, C.c_int32, C.c_int32, _fp, C.c_double, C.c_int32,
                                      C.c_int32]
+        _sp = np.ctypeslib.ndpointer(dtype=np.int16, flags="C_CONTIGUOUS")
+        L.ozo_ds_rho.restype = C.c_int32
+        L.ozo_ds_rho.argtypes = [C.c_int32, C.c_int32]
+        L.ozo_ds_split.restype = C.c_int32
+        L.ozo_ds_split.argtypes = [H, C.c_int32, C.c_int32, _dp, C.c_int32, _dp, C.c_int32, _fp,
+                                   C.c_int32, _sp, C.c_int32, _dp]
+        L.ozo_ds_gemm.restype = C.c_int32
+        L.ozo_ds_gemm.argtypes = [H, C.c_char, C.c_char, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                  _dp, C.c_int32, _dp, C.c_int32, C.c_double, _dp, C.c_int32]
+        L.ozo_ds_dot.restype = C.c_double
+        L.ozo_ds_dot.argtypes = [H, C.c_int32, _dp, _dp]
+        L.ozo_ds_nrm2.restype = C.c_double
+        L.ozo_ds_nrm2.argtypes = [H, C.c_int32, _dp]
+        L.ozo_ds_gemv.restype = C.c_int32
+        L.ozo_ds_gemv.argtypes = [H, C.c_char, C.c_int32, C.c_int32, C.c_double, _dp, C.c_int32, _dp,
+                                  C.c_double, _dp]
+        L.ozo_ds_csrmv.restype = C.c_int32
+        L.ozo_ds_csrmv.argtypes = [H, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_void_p, _ip,
+                                   _ip, _dp, C.c_double, _dp]
+        L.ozo_ds_csrmv_splitA.restype = C.c_void_p
+        L.ozo_ds_csrmv_splitA.argtypes = [H, C.c_int32, C.c_int32, C.c_int32, _dp, _ip]
+        L.ozo_ds_cg.restype = C.c_int32
+        L.ozo_ds_cg.argtypes = [H, C.c_int32, C.c_int32, _dp, _ip, _ip, _dp, _dp, C.c_int32,
+                                C.c_double]
         L.ozo_laplace3d_nnz.restype = C.c_int64
         L.ozo_laplace3d_nnz.argtypes = [C.c_int32]
         L.ozo_laplace3d.argtypes = [C.c_int32, C.c_double, _ip, _ip, _dp]
@@ -262,7 +286,53 @@ class Handle:
         return hist[: self._h.contents.cg_numiter + 1].astype(np.float32)
 
 
+    # mixed routines (the <double,float,float> instantiations, ozoracle_ds.c) -----------------
+    def dsgemm(self, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+        rc = lib().ozo_ds_gemm(self._h, transA.encode(), transB.encode(), m, n, k, alpha, A, lda, B,
+                               ldb, beta, C, ldc)
+        if rc != 0:
+            raise RuntimeError(f"ozo_ds_gemm failed ({rc})")
+        return C
+
+    def dsdot(self, n, x, y):
+        return lib().ozo_ds_dot(self._h, n, x, y)
+
+    def dsnrm2(self, n, x):
+        return lib().ozo_ds_nrm2(self._h, n, x)
+
+    def dsgemv(self, trans, m, n, alpha, A, lda, x, beta, y):
+        rc = lib().ozo_ds_gemv(self._h, trans.encode(), m, n, alpha, A, lda, x, beta, y)
+        if rc != 0:
+            raise RuntimeError(f"ozo_ds_gemv failed ({rc})")
+        return y
+
+    def dscsrmv(self, m, n, nnz, alpha, val, colind, rowptr, x, beta, y):
+        ptr = val if isinstance(val, int) else val.ctypes.data
+        rc = lib().ozo_ds_csrmv(self._h, m, n, nnz, alpha, ptr, colind, rowptr, x, beta, y)
+        if rc != 0:
+            raise RuntimeError(f"ozo_ds_csrmv failed ({rc})")
+        return y
+
+    def dscsrmv_splitA(self, m, n, nnz, val, rowptr):
+        return lib().ozo_ds_csrmv_splitA(self._h, m, n, nnz, val, rowptr)
+
+    def dscg(self, n, nnz, val, colind, rowptr, b, x, maxiter, tol, history=True):
+        hist = np.zeros(maxiter + 2, dtype=np.float64)
+        if history:
+            self._h.contents.verbose = 1
+            self._h.contents.cg_verbose1 = hist.ctypes.data
+        rc = lib().ozo_ds_cg(self._h, n, nnz, val, colind, rowptr, b, x, maxiter, tol)
+        if rc != 0:
+            raise RuntimeError(f"ozo_ds_cg failed ({rc})")
+        self._h.contents.cg_verbose1 = None
+        return hist[: self._h.contents.cg_numiter + 1]
+
+
 # building blocks -------------------------------------------------------------------------------
+def ds_rho(dim, mode=0):
+    return lib().ozo_ds_rho(dim, mode)
+
+
 def s_rho(dim, mode=0):
     return lib().ozo_s_rho(dim, mode)
 

@@ -155,6 +155,27 @@ int32_t ozo_s_cg(ozo_handle *h, int32_t n, int32_t nnz, const float *val, const 
 void ozo_s_init_mat(int32_t m, int32_t n, int32_t ld, float *mat, double phi, int32_t mode,
                     int32_t seed);
 
+/* mixed restatement (ozoracle_ds.c): the <double,float,float> / <double,float> instantiations;
+   -3 (or NaN / NULL) where the reference has no such path (NearSum, Dot2, the plain bypass) */
+int32_t ozo_ds_rho(int32_t dim, int32_t splitEpsMode);
+int32_t ozo_ds_split(ozo_handle *h, int32_t m, int32_t n, const double *in, int32_t ldi, double *out,
+                     int32_t ldo, float *split, int32_t lds, int16_t *exps, int32_t ldse,
+                     double *max);
+int32_t ozo_ds_gemm(ozo_handle *h, char transA, char transB, int32_t m, int32_t n, int32_t k,
+                    double alpha, const double *A, int32_t lda, const double *B, int32_t ldb,
+                    double beta, double *C, int32_t ldc);
+double ozo_ds_dot(ozo_handle *h, int32_t n, const double *x, const double *y);
+double ozo_ds_nrm2(ozo_handle *h, int32_t n, const double *x);
+int32_t ozo_ds_gemv(ozo_handle *h, char trans, int32_t m, int32_t n, double alpha, const double *A,
+                    int32_t lda, const double *x, double beta, double *y);
+int32_t ozo_ds_csrmv(ozo_handle *h, int32_t m, int32_t n, int32_t nnz, double alpha, const double *val,
+                     const int32_t *colind, const int32_t *rowptr, const double *x, double beta,
+                     double *y);
+float *ozo_ds_csrmv_splitA(ozo_handle *h, int32_t m, int32_t n, int32_t nnz, const double *val,
+                           const int32_t *rowptr);
+int32_t ozo_ds_cg(ozo_handle *h, int32_t n, int32_t nnz, const double *val, const int32_t *colind,
+                  const int32_t *rowptr, const double *b, double *x, int32_t maxiter, double tol);
+
 /* generators: testing/testing_common.cpp:664-806 */
 void ozo_srand48(int64_t seed);
 double ozo_drand48(void);

@@ -89,12 +89,21 @@ _saxpy = _sig("ozblasSaxpy", _i, _H, _i, _f, _p, _i, _p, _i)
 _scsrmv = _sig("ozblasScsrmv", _i, _H, _c, _i, _i, _i, _f, _p, _p, _p, _p, _p, _f, _p)
 _scsrmv_split = _sig("ozblasScsrmvSplitA", _p, _H, _c, _i, _i, _i, _p, _p, _p)
 _scg = _sig("ozblasScg", _i, _H, _c, _i, _i, _p, _p, _p, _p, _p, _p, _i, _f)
+_dsgemm = _sig("ozblasDSgemm", _i, _H, _c, _c, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i)
+_dsgemv = _sig("ozblasDSgemv", _i, _H, _c, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i)
+_dsdot = _sig("ozblasDSdot", _i, _H, _i, _p, _i, _p, _i, C.POINTER(C.c_double))
+_dsnrm2 = _sig("ozblasDSnrm2", _i, _H, _i, _p, _i, C.POINTER(C.c_double))
+_dscsrmv = _sig("ozblasDScsrmv", _i, _H, _c, _i, _i, _i, _d, _p, _p, _p, _p, _p, _d, _p)
+_dscsrmv_split = _sig("ozblasDScsrmvSplitA", _p, _H, _c, _i, _i, _i, _p, _p, _p)
+_dscg = _sig("ozblasDScg", _i, _H, _c, _i, _i, _p, _p, _p, _p, _p, _p, _i, _d)
 _version = _sig("ozblasGetVersionString", C.c_char_p)
 _gen_phi = _sig("ozblasGenPhi", None, _i, _i, _i, _p, _d, C.c_int64)
 internal_tau = _sig("ozblas_internal_tau", _i, _d)
 internal_rho = _sig("ozblas_internal_rho", _i, _i, _i)
 internal_rho_row = _sig("ozblas_internal_rho_row", _i, _i)
 internal_rho_row_mode = _sig("ozblas_internal_rho_row_mode", _i, _i, _i)
+internal_rho_ds = _sig("ozblas_internal_rho_ds", _i, _i, _i)
+internal_rho_row_ds = _sig("ozblas_internal_rho_row_ds", _i, _i, _i)
 internal_tau_f = _sig("ozblas_internal_tau_f", _i, C.c_float)
 internal_nearsum_exp_f = _sig("ozblas_internal_nearsum_exp_f", _i, C.c_float)
 internal_rho_f = _sig("ozblas_internal_rho_f", _i, _i, _i)
@@ -274,6 +283,39 @@ class Handle:
     def scg(self, n, nnz, val, colind, rowptr, b, x, maxiter, tol, history=True, second=None):
         """As dcg; the histories of the binary32 solver are float arrays, as the reference's."""
         return self._cg(_scg, "ozblasScg", "float32", n, nnz, val, colind, rowptr, b, x, maxiter,
+                        tol, history, second)
+
+    # ---- mixed: binary64 data, binary32 slices (<double,float,float>) -----------------------
+    def dsgemm(self, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc):
+        self._check("ozblasDSgemm", _dsgemm(self.ref, transA.encode(), transB.encode(), m, n, k,
+                                            alpha, _ptr(A), lda, _ptr(B), ldb, beta, _ptr(Cm), ldc))
+
+    def dsgemv(self, trans, m, n, alpha, A, lda, x, beta, y):
+        self._check("ozblasDSgemv", _dsgemv(self.ref, trans.encode(), m, n, alpha, _ptr(A), lda,
+                                            _ptr(x), 1, beta, _ptr(y), 1))
+
+    def dsdot(self, n, x, y) -> float:
+        r = C.c_double(0.0)
+        self._check("ozblasDSdot", _dsdot(self.ref, n, _ptr(x), 1, _ptr(y), 1, C.byref(r)))
+        return r.value
+
+    def dsnrm2(self, n, x) -> float:
+        r = C.c_double(0.0)
+        self._check("ozblasDSnrm2", _dsnrm2(self.ref, n, _ptr(x), 1, C.byref(r)))
+        return r.value
+
+    def dscsrmv(self, m, n, nnz, alpha, val, colind, rowptr, x, beta, y):
+        self._check("ozblasDScsrmv", _dscsrmv(self.ref, b"N", m, n, nnz, alpha, None, _ptr(val),
+                                              _ptr(colind), _ptr(rowptr), _ptr(x), beta, _ptr(y)))
+
+    def dscsrmv_splitA(self, m, n, nnz, val, rowptr):
+        p = _dscsrmv_split(self.ref, b"N", m, n, nnz, None, _ptr(val), _ptr(rowptr))
+        if not p:
+            raise OzblasError("ozblasDScsrmvSplitA", self._h.lastStatus)
+        return p
+
+    def dscg(self, n, nnz, val, colind, rowptr, b, x, maxiter, tol, history=True, second=None):
+        return self._cg(_dscg, "ozblasDScg", "float64", n, nnz, val, colind, rowptr, b, x, maxiter,
                         tol, history, second)
 
     def dgemm_batched(self, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc):

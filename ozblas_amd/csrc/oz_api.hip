@@ -45,18 +45,23 @@ static int32_t sync(ozblasHandle_t *oh) {
 
 // Per-precision rules of the drivers: rho of getRho and the device tau tables (the binary32 split
 // takes tau from (float)log2((double)max), its NearSum from log2f).  Both precisions store slices
-// as doubles and share the FP64 product kernels.
+// as doubles and share the FP64 product kernels.  The mixed routines (binary64 data split into
+// binary32 slices, Backend::mixed) differ from binary64 only in rho and in the split loop's slot
+// rule: their slices, multiplied unscaled in FP64, are exactly the reference's
+// scalbn(float product, seA + seB).
 template <class T>
 struct Prec;
 template <>
 struct Prec<double> {
-	static int rho(int dim, int mode) { return rho_host(dim, mode); }
+	static int rho(const ozblasHandle_t *oh, int dim) {
+		return be(oh)->mixed ? rho_host_ds(dim, oh->splitEpsMode) : rho_host(dim, oh->splitEpsMode);
+	}
 	static const uint16_t *Jsplit(Backend *b) { return b->dTau; }
 	static const uint16_t *Jsum(Backend *b) { return b->dTau; }
 };
 template <>
 struct Prec<float> {
-	static int rho(int dim, int mode) { return rho_host_f(dim, mode); }
+	static int rho(const ozblasHandle_t *oh, int dim) { return rho_host_f(dim, oh->splitEpsMode); }
 	static const uint16_t *Jsplit(Backend *b) { return b->dTauF; }
 	static const uint16_t *Jsum(Backend *b) { return b->dTauF2; }
 };
@@ -71,7 +76,7 @@ static inline int split_count(const ozblasHandle_t *oh, int zmax) {
 		count = imax(zmax, 1) + 1;
 	else
 		count = cap;
-	if (oh->reproMode == 0) count++;
+	if (repro_slot(oh)) count++;
 	return count;
 }
 
@@ -90,7 +95,7 @@ static int32_t dense_split(ozblasHandle_t *oh, int k, int cols, const T *in, int
                            int64_t ldr, DenseSplit &ds) {
 	Backend *b = be(oh);
 	const int cap = split_cap(oh);
-	const int rho = Prec<T>::rho(k, oh->splitEpsMode);
+	const int rho = Prec<T>::rho(oh, k);
 	const uint16_t *J = Prec<T>::Jsplit(b);
 	ds.ldk = rup(k, 16);
 	ds.sstride = ds.ldk * cols;
@@ -139,7 +144,7 @@ static int32_t count_cols(ozblasHandle_t *oh, int k, int cols, const T *in, int6
 	if (!rem || !scratch || !zcol) return OZBLAS_STATUS_WORKSPACE;
 	OZ_HIP(hipMemsetAsync(b->dCounters, 0, sizeof(int), b->stream));
 	launch_split_cols(b->stream, k, cols, in, ldi, rem, ldk, scratch, ldk, 0,
-	                  Prec<T>::rho(k, oh->splitEpsMode), oh->splitShift, split_cap(oh),
+	                  Prec<T>::rho(oh, k), oh->splitShift, split_cap(oh),
 	                  Prec<T>::Jsplit(b), zcol, b->dCounters);
 	OZ_HIP(hipMemcpyAsync(b->hPinned, b->dCounters, sizeof(int), hipMemcpyDeviceToHost, b->stream));
 	OZ_TRY(sync(oh));
@@ -160,6 +165,8 @@ static int32_t gemm_plain(ozblasHandle_t *oh, char transA, char transB, int m, i
 	const bool fsum = (oh->sumMode == 0);
 	if (oh->sumMode != 0 && oh->sumMode != 1) return OZBLAS_STATUS_NOT_SUPPORTED;
 	if (oh->splitEpsMode != 0 && oh->splitEpsMode != 9) return OZBLAS_STATUS_NOT_SUPPORTED;
+	// the reference's mixed NearSum prints an error and exits (src/cpu/ozblas_sum.cpp:275-330)
+	if (b->mixed && !fsum) return OZBLAS_STATUS_NOT_SUPPORTED;
 	const bool dotPath = (m == 1 && n == 1);  // :25-28: DOT ignores fastMode
 	const int fastMode = dotPath ? 0 : oh->fastMode;
 	const uint64_t base = wmark(oh);
@@ -542,7 +549,7 @@ template <class T>
 static int32_t vec_split(ozblasHandle_t *oh, int n, const T *x, VecSplit &vs, bool readCount = true) {
 	Backend *b = be(oh);
 	const int cap = split_cap(oh);
-	const int rho = Prec<T>::rho(n, oh->splitEpsMode);
+	const int rho = Prec<T>::rho(oh, n);
 	vs.lds = rup(n, 32);
 	vs.slices = (double *)walloc(oh, 8ull * vs.lds * (cap + 2));
 	vs.maxv = (unsigned long long *)walloc(oh, 8ull * (cap + 2));
@@ -552,8 +559,8 @@ static int32_t vec_split(ozblasHandle_t *oh, int n, const T *x, VecSplit &vs, bo
 	if (!vs.slices || !vs.maxv || !vs.info || !scratch || !plan) return OZBLAS_STATUS_WORKSPACE;
 	OZ_HIP(hipMemsetAsync(scratch, 0, vsplit_scratch_bytes(), b->stream));
 	OZ_HIP(hipMemsetAsync(vs.maxv, 0, 8ull * (cap + 2), b->stream));
-	OZ_HIP(launch_vsplit(b->stream, n, x, vs.slices, vs.lds, rho, oh->splitShift, cap, oh->reproMode,
-	                     Prec<T>::Jsplit(b), vs.maxv, vs.info, scratch, plan));
+	OZ_HIP(launch_vsplit(b->stream, n, x, vs.slices, vs.lds, rho, oh->splitShift, cap,
+	                     repro_slot(oh) ? 0 : 1, Prec<T>::Jsplit(b), vs.maxv, vs.info, scratch, plan));
 	if (!readCount) return OZBLAS_STATUS_SUCCESS;
 	OZ_HIP(hipMemcpyAsync(b->hPinned, vs.info, sizeof(int), hipMemcpyDeviceToHost, b->stream));
 	OZ_TRY(sync(oh));
@@ -595,8 +602,10 @@ static int32_t dot_impl(ozblasHandle_t *oh, int n, const T *x, const T *y, T *re
 		*ret = 0;
 		return OZBLAS_STATUS_SUCCESS;
 	}
-	// the reference's binary32 Dot2 GEMM is a stub (src/cpu/ozblas_blas_wrapper.cpp:179-181)
-	if (f32 && oh->splitEpsMode == 2) return OZBLAS_STATUS_NOT_SUPPORTED;
+	// the reference's binary32 Dot2 GEMM is a stub (src/cpu/ozblas_blas_wrapper.cpp:179-181); its
+	// mixed NearSum exits (src/cpu/ozblas_sum.cpp:275-330)
+	if ((f32 || b->mixed) && oh->splitEpsMode == 2) return OZBLAS_STATUS_NOT_SUPPORTED;
+	if (b->mixed && oh->sumMode != 0) return OZBLAS_STATUS_NOT_SUPPORTED;
 	const uint64_t base = wmark(oh);
 	oh->splitShift = 1;  // counterInit (src/cpu/ozblas.cpp:8)
 	const bool dot2 = (oh->splitEpsMode == 2);
@@ -666,8 +675,12 @@ static int32_t csr_split(ozblasHandle_t *oh, int m, int nnz, const T *val, const
 		if (!cs.slices) return OZBLAS_STATUS_WORKSPACE;
 	}
 	OZ_HIP(hipMemsetAsync(b->dCounters, 0, sizeof(int), b->stream));
-	launch_split_csr(b->stream, m, val, rowptr, work, cs.slices, cs.lds, oh->splitShift, cap,
-	                 Prec<T>::Jsplit(b), zrow, b->dCounters, oh->splitEpsMode);
+	if constexpr (sizeof(T) == 8)
+		launch_split_csr(b->stream, m, val, rowptr, work, cs.slices, cs.lds, oh->splitShift, cap,
+		                 Prec<T>::Jsplit(b), zrow, b->dCounters, oh->splitEpsMode, b->mixed);
+	else
+		launch_split_csr(b->stream, m, val, rowptr, work, cs.slices, cs.lds, oh->splitShift, cap,
+		                 Prec<T>::Jsplit(b), zrow, b->dCounters, oh->splitEpsMode);
 	OZ_HIP(hipMemcpyAsync(b->hPinned, b->dCounters, sizeof(int), hipMemcpyDeviceToHost, b->stream));
 	OZ_TRY(sync(oh));
 	const int zmax = ((int32_t *)b->hPinned)[0];
@@ -675,11 +688,11 @@ static int32_t csr_split(ozblasHandle_t *oh, int m, int nnz, const T *val, const
 	// (src/cpu/ozblas_split.cpp:919-927)
 	int count;
 	if (cap <= 1)
-		count = 1 + (oh->reproMode == 0 ? 1 : 0);
+		count = 1 + (repro_slot(oh) ? 1 : 0);
 	else if (zmax < cap)
 		count = imax(zmax, 1) + 1;
 	else
-		count = cap + (oh->reproMode == 0 ? 1 : 0);
+		count = cap + (repro_slot(oh) ? 1 : 0);
 	cs.count = count;
 	launch_zero_slots_csr(b->stream, m, rowptr, cs.slices, cs.lds, zrow, count);
 	return OZBLAS_STATUS_SUCCESS;
@@ -694,6 +707,7 @@ static int32_t csrmv_core(ozblasHandle_t *oh, int m, int n, int nnz, T alpha, co
 	maxlevel = imax(maxlevel - oh->fastMode, imin(nA - 1, nB - 1));
 	const bool fsum = oh->sumMode == 0;
 	if (fsum && oh->fastMode) return OZBLAS_STATUS_NOT_SUPPORTED;  // reference reads unwritten slots
+	if (b->mixed && (!fsum || oh->splitEpsMode == 2)) return OZBLAS_STATUS_NOT_SUPPORTED;
 	if constexpr (sizeof(T) == 4) {  // no binary32 Dot2 (src/cpu/ozblas_blas_wrapper.cpp:179-181)
 		if (oh->splitEpsMode == 2) return OZBLAS_STATUS_NOT_SUPPORTED;
 	} else if (oh->splitEpsMode == 2) {  // Dot2 slice products (:139-142, 159-162)
@@ -1032,9 +1046,12 @@ static int32_t cg_record2(ozblasHandle_t *oh, const CgDiag<T> &d, int idx, const
 		return OZBLAS_STATUS_SUCCESS;
 	}
 	launch_true_residual(be(oh)->stream, d.n, d.A, d.colind, d.rowptr, x, d.b, d.t);
-	// a correctly rounded sum of squares: NearSum with the exact split, whatever the handle holds
+	// a correctly rounded sum of squares: NearSum with the exact binary64 (or binary32) split,
+	// whatever the handle holds
 	const int32_t sv[5] = {oh->sumMode, oh->reproMode, oh->nSplitMax, oh->splitEpsMode, oh->splitShift};
 	const float st[3] = {oh->nSplitA, oh->nSplitB, oh->nSplitC};
+	const int mixed = be(oh)->mixed;
+	be(oh)->mixed = 0;
 	oh->sumMode = 1;
 	oh->reproMode = 1;
 	oh->nSplitMax = 0;
@@ -1049,6 +1066,7 @@ static int32_t cg_record2(ozblasHandle_t *oh, const CgDiag<T> &d, int idx, const
 	oh->nSplitA = st[0];
 	oh->nSplitB = st[1];
 	oh->nSplitC = st[2];
+	be(oh)->mixed = mixed;
 	OZ_TRY(s);
 	((T *)d.v2)[idx] = (T)sqrt((double)dd) / d.nrmb;
 	return OZBLAS_STATUS_SUCCESS;
@@ -1127,7 +1145,8 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 		if (ev0) (void)hipEventDestroy(ev0);
 	};
 	const int cap = split_cap(oh);
-	const int rho = rho_host(n, oh->splitEpsMode);
+	const int rho = Prec<double>::rho(oh, n);
+	const int reproCount = repro_slot(oh) ? 0 : 1;
 	const int64_t lds = rup(n, 32);
 	const int fsum = oh->sumMode == 0 ? 1 : 0;
 	const int nhist = (oh->verbose > 0) ? maxiter / oh->verbose + 2 : 1;
@@ -1169,16 +1188,16 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 		off = diag.t1.sec();
 	}
 	for (int it = 0; it < maxiter; it++) {
-		OZ_HIP(launch_vsplit(st, n, v.P, slP, lds, rho, 1, cap, oh->reproMode, b->dTau, mxP, infoP,
+		OZ_HIP(launch_vsplit(st, n, v.P, slP, lds, rho, 1, cap, reproCount, b->dTau, mxP, infoP,
 		                     vscr, plan, &S->stopAt, it));
 		launch_csrmm_sum_dev(st, n, AS, rup(b->nnzA, 32), nA, colind, rowptr, slP, lds, infoP, fsum,
 		                     v.Q, Cs, ldc, b->dTau, S, it);
-		OZ_HIP(launch_vsplit(st, n, v.Q, slQ, lds, rho, 1, cap, oh->reproMode, b->dTau, mxQ, infoQ,
+		OZ_HIP(launch_vsplit(st, n, v.Q, slQ, lds, rho, 1, cap, reproCount, b->dTau, mxQ, infoQ,
 		                     vscr, plan, &S->stopAt, it));
 		launch_dots_dev(st, n, slP, lds, infoP, slQ, lds, infoQ, vals, ctr, mxP, mxQ, cap + 2, b->dTau,
 		                fsum, S, it, 1, nullptr);
 		launch_axpy2_dev(st, n, S, it, v.P, x, v.Q, v.R);
-		OZ_HIP(launch_vsplit(st, n, v.R, slR, lds, rho, 1, cap, oh->reproMode, b->dTau, mxR, infoR,
+		OZ_HIP(launch_vsplit(st, n, v.R, slR, lds, rho, 1, cap, reproCount, b->dTau, mxR, infoR,
 		                     vscr, plan, &S->stopAt, it));
 		launch_dots_dev(st, n, slR, lds, infoR, slR, lds, infoR, vals, ctr, mxR, mxR, cap + 2, b->dTau,
 		                fsum, S, it, 2, nullptr);
@@ -1302,7 +1321,7 @@ static int32_t cg_entry(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz,
 	}
 	{
 		// the true residual of a recorded iteration needs its x: that history runs synchronously
-		bool devicePath = E == 8 && oh->cgDeviceSlices >= 0 && oh->reproMode != 0 &&
+		bool devicePath = E == 8 && oh->cgDeviceSlices >= 0 && !repro_slot(oh) &&
 		                  oh->fastMode == 0 && oh->splitEpsMode != 2 && oh->nSplitA_ <= 4 && maxiter > 0 &&
 		                  !diag.trueres;
 		if constexpr (E == 8) {
@@ -1367,5 +1386,90 @@ extern "C" int32_t ozblasScg(ozblasHandle_t *oh, char transA, int32_t n, int32_t
                              const int32_t *rowptr, const float *b, float *x, int32_t maxiter,
                              float tol) {
 	(void)descrA;
+	return cg_entry(oh, transA, n, nnz, A, colind, rowptr, b, x, maxiter, tol);
+}
+
+// =============================================================================================
+// mixed routines: binary64 data, binary32 slices (the reference's <double,float,float> and
+// <double,float> instantiations).  Fsum only: the reference's mixed NearSum and its float Dot2 GEMM
+// are not available.
+// =============================================================================================
+namespace oz {
+struct MixedScope {  // marks the handle's backend as running a mixed routine for one call
+	Backend *b;
+	int save;
+	explicit MixedScope(ozblasHandle_t *oh) : b(be(oh)), save(b->mixed) { b->mixed = 1; }
+	~MixedScope() { b->mixed = save; }
+};
+}  // namespace oz
+
+extern "C" int32_t ozblasDSgemm(ozblasHandle_t *oh, char transA, char transB, int32_t m, int32_t n,
+                                int32_t k, double alpha, const double *A, int32_t lda,
+                                const double *B, int32_t ldb, double beta, double *C, int32_t ldc) {
+	OZ_TRY(ensure_init(oh));
+	if (m < 0 || n < 0 || k < 0) return oh->lastStatus = OZBLAS_STATUS_INVALID_VALUE;
+	if (m == 0 || n == 0) return oh->lastStatus = OZBLAS_STATUS_SUCCESS;
+	oh->splitShift = 1;  // counterInit
+	if (oh->reproMode == 0 && oh->nSplitMax == 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	if (oh->splitEpsMode == 2 || oh->sumMode != 0) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	if (k == 0) return oh->lastStatus = gemm_k0(oh, m, n, alpha, beta, C, ldc);
+	MixedScope mx(oh);
+	return oh->lastStatus = gemm_plain(oh, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+extern "C" int32_t ozblasDSgemv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, double alpha,
+                                const double *A, int32_t lda, const double *x, int32_t incx,
+                                double beta, double *y, int32_t incy) {
+	OZ_TRY(ensure_init(oh));
+	if (incx != 1 || incy != 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	if (is_n(transA))  // src/cpu/ozblas_gemv.cpp:23-26
+		return ozblasDSgemm(oh, 'n', 'n', m, 1, n, alpha, A, lda, x, n, beta, y, m);
+	return ozblasDSgemm(oh, 't', 'n', n, 1, m, alpha, A, lda, x, m, beta, y, n);
+}
+
+extern "C" int32_t ozblasDSdot(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx,
+                               const double *y, int32_t incy, double *ret) {
+	OZ_TRY(ensure_init(oh));
+	MixedScope mx(oh);
+	return dot_entry(oh, n, x, incx, y, incy, ret);
+}
+
+extern "C" int32_t ozblasDSnrm2(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx,
+                                double *ret) {
+	OZ_TRY(ensure_init(oh));
+	double d = 0.0;
+	const int32_t s = ozblasDSdot(oh, n, x, incx, x, incx, &d);
+	if (s != OZBLAS_STATUS_SUCCESS) return s;
+	*ret = sqrt(d);
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+extern "C" float *ozblasDScsrmvSplitA(ozblasHandle_t *oh, char transA, int32_t m, int32_t n,
+                                      int32_t nnz, const void *descrA, const double *A,
+                                      const int32_t *rowptr) {
+	(void)descrA;
+	(void)n;
+	if (ensure_init(oh) != OZBLAS_STATUS_SUCCESS) return nullptr;
+	MixedScope mx(oh);
+	return (float *)csrmv_split_a(oh, transA, m, nnz, A, rowptr);
+}
+
+extern "C" int32_t ozblasDScsrmv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, int32_t nnz,
+                                 double alpha, const void *descrA, const double *A,
+                                 const int32_t *colind, const int32_t *rowptr, const double *x,
+                                 double beta, double *y) {
+	(void)descrA;
+	OZ_TRY(ensure_init(oh));
+	MixedScope mx(oh);
+	return csrmv_entry(oh, transA, m, n, nnz, alpha, A, colind, rowptr, x, beta, y);
+}
+
+extern "C" int32_t ozblasDScg(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz,
+                              const void *descrA, const double *A, const int32_t *colind,
+                              const int32_t *rowptr, const double *b, double *x, int32_t maxiter,
+                              double tol) {
+	(void)descrA;
+	OZ_TRY(ensure_init(oh));
+	MixedScope mx(oh);
 	return cg_entry(oh, transA, n, nnz, A, colind, rowptr, b, x, maxiter, tol);
 }

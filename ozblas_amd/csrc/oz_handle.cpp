@@ -70,6 +70,19 @@ const uint16_t *tau_table_host_f2() {
 	return g_tauf2;
 }
 
+// getRho<double,float> (src/cpu/ozblas_split.cpp:20-34 with getEpse<double> = 53,
+// getEpse<float> = 24, getEpse2<float> = 53)
+int rho_host_ds(int dim, int splitEpsMode) {
+	double r;
+	switch (splitEpsMode) {
+	case 2: r = ceil((53 - (53 - log2(1. * dim)) / 2)); break;
+	case 9: r = ceil((53 - (24 - log2(2. * sqrt((double)dim))) / 2)); break;
+	default: r = ceil((53 - (24 - log2(1. * dim)) / 2)); break;
+	}
+	if (!(r >= -2147483648.0 && r < 2147483648.0)) return INT32_MIN;
+	return (int)r;
+}
+
 // getRho<float,float> (src/cpu/ozblas_split.cpp:20-34 with getEpse = 24, getEpse2 = 53)
 int rho_host_f(int dim, int splitEpsMode) {
 	double r;
@@ -217,6 +230,8 @@ extern "C" int32_t ozblas_internal_nearsum_exp_f(float mu) {
 extern "C" int32_t ozblas_internal_rho_f(int32_t dim, int32_t mode) { return rho_host_f(dim, mode); }
 extern "C" int32_t ozblas_internal_rho_row_f(int32_t dim, int32_t mode) { return rho_dev_f(dim, mode); }
 extern "C" int32_t ozblas_internal_rho_row_mode(int32_t dim, int32_t mode) { return rho_dev(dim, mode); }
+extern "C" int32_t ozblas_internal_rho_ds(int32_t dim, int32_t mode) { return rho_host_ds(dim, mode); }
+extern "C" int32_t ozblas_internal_rho_row_ds(int32_t dim, int32_t mode) { return rho_dev_ds(dim, mode); }
 extern "C" void ozblas_internal_tau_batch(int64_t n, const double *x, int32_t *out) {
 	const uint16_t *J = tau_table_host();
 	for (int64_t i = 0; i < n; i++) out[i] = tau_of(x[i], J);

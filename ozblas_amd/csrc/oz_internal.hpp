@@ -52,11 +52,15 @@ struct Backend {
 	int nnzA = 0;           // nonzeros of the matrix pre-split for CG (slice pitch rup(nnz, 32))
 	uint16_t *dTauF = nullptr;   // device copies of the binary32 tables JF, JF2
 	uint16_t *dTauF2 = nullptr;
+	int mixed = 0;  // 1 while a mixed (DS: binary64 data, binary32 slices) routine runs
 };
 
 void batch_plan_free(void *plan);
 
 inline Backend *be(ozblasHandle_t *oh) { return reinterpret_cast<Backend *>(oh->backend); }
+inline const Backend *be(const ozblasHandle_t *oh) {
+	return reinterpret_cast<const Backend *>(oh->backend);
+}
 
 // bump allocation inside the workspace, 256-byte aligned; returns nullptr when it does not fit
 inline void *walloc(ozblasHandle_t *oh, uint64_t bytes) {
@@ -248,12 +252,28 @@ struct SplitRule<float> {
 	__host__ __device__ static int rho_row(int dim, int mode) { return rho_dev_f(dim, mode); }
 };
 
+// the reproMode = 0 remainder slot of the split loop: same-type instantiations only
+// (`if constexpr (std::is_same<TYPE1, TYPE2>::value)`, src/cpu/ozblas_split.cpp:275-287,916-925)
+inline bool repro_slot(const ozblasHandle_t *oh) { return oh->reproMode == 0 && !be(oh)->mixed; }
+
 // cap on slices of the reference split loop (src/cpu/ozblas_split.cpp:275-277)
 inline int split_cap(const ozblasHandle_t *oh) {
 	int maxS = (oh->nSplitMax > 0) ? oh->nSplitMax : kNumSplitDefaultMax;
-	if (oh->reproMode == 0) maxS--;
+	if (repro_slot(oh)) maxS--;
 	return maxS;
 }
+
+// getRho<double,float> (the mixed split: epse of TYPE1 53, of TYPE2 24, epse2 of TYPE2 53) from
+// integers: mode 0 ceil(41 + log2(d) / 2) = 41 + ceil(q / 2); mode 9 ceil(41.5 + log2(d) / 4)
+// = 41 + ceil((2 + q) / 4); mode 2 is the binary64 mode-0 value; q = ceil(log2 d)
+__host__ __device__ inline int rho_dev_ds(int dim, int mode) {
+	if (mode == 2) return rho_of_dim(dim);
+	if (dim <= 0) return INT32_MIN;
+	const int p = 31 - __builtin_clz((unsigned)dim);
+	const int q = ((dim & (dim - 1)) == 0) ? p : p + 1;
+	return (mode == 9) ? 41 + (q + 5) / 4 : 41 + (q + 1) / 2;
+}
+int rho_host_ds(int dim, int splitEpsMode);  // getRho<double,float> with the host libm
 
 // host image of the digit statistics of one operand (ExpStats of oz_digits.hpp)
 struct ExpStatsH {

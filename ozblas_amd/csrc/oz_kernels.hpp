@@ -25,9 +25,10 @@ void launch_split_vec_pass(hipStream_t st, int m, const double *src, double *rem
 void launch_fill(hipStream_t st, double *p, int64_t n, double v);
 void launch_fill(hipStream_t st, float *p, int64_t n, float v);
 void launch_copy(hipStream_t st, const double *src, double *dst, int64_t n);
+// mixed = 1: the rho of the mixed split (binary64 data, binary32 slices)
 void launch_split_csr(hipStream_t st, int m, const double *val, const int *rowptr, double *work,
                       double *split, int64_t lds, int shift, int cap, const uint16_t *J, int *zrow,
-                      int *zmax, int epsMode);
+                      int *zmax, int epsMode, int mixed = 0);
 void launch_split_csr(hipStream_t st, int m, const float *val, const int *rowptr, float *work,
                       double *split, int64_t lds, int shift, int cap, const uint16_t *JF, int *zrow,
                       int *zmax, int epsMode);

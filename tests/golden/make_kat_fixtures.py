@@ -4,7 +4,7 @@ The reference ships check-mode logs of its CPU OzBLAS build under script/a100/*.
 the sweep scripts that produced them (script/a100/_a100_{gemm,dot,gemv}.sh).  Each log row is
 ``m n k OK|NG max_rel_err (hex) n_wrong`` where n_wrong counts entries that differ from
 RN(exact) (testing/testing_common.cpp:839-930, MPFR 2048-bit reference).  This script pairs every
-``testing_ozblas_{d,s}{gemm,dot,gemv}`` command line (binary64 and binary32 builds) with its log and
+``testing_ozblas_{d,s,ds}{gemm,dot,gemv}`` command line (binary64, binary32 and mixed builds) with its log and
 stores the parsed rows -- data only, no reference source -- in tests/golden/a100_kat.json.
 
 Usage: python tests/golden/make_kat_fixtures.py [/root/reference]
@@ -26,7 +26,7 @@ def parse_cmd(line):
         return None
     cmd, out = line.split(">", 1)
     toks = shlex.split(cmd)
-    mt = re.search(r"testing_ozblas_([ds])(gemm|dot|gemv)$", toks[0]) if toks else None
+    mt = re.search(r"testing_ozblas_(ds|d|s)(gemm|dot|gemv)$", toks[0]) if toks else None
     if mt is None:
         return None
     prec, routine = mt.group(1), mt.group(2)

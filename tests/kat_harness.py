@@ -236,3 +236,16 @@ class OracleImplS(OracleImpl):
 
     def gemv(self, *a):
         return self.h.sgemv(*a)
+
+
+class OracleImplDS(OracleImpl):
+    """The mixed <double,float,float> restatement with the same harness conventions."""
+
+    def gemm(self, *a):
+        return self.h.dsgemm(*a)
+
+    def dot(self, n, x, y):
+        return self.h.dsdot(n, x, y)
+
+    def gemv(self, *a):
+        return self.h.dsgemv(*a)

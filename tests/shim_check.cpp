@@ -1,5 +1,5 @@
-// shim_check.cpp -- calls every cuOzBLAS template of include/cuozblas_mi355x.hpp (binary64 and
-// binary32) and the C entry point it forwards to on the same inputs, and compares the outputs byte
+// shim_check.cpp -- calls every cuOzBLAS template of include/cuozblas_mi355x.hpp (binary64,
+// binary32 and mixed) and the C entry point it forwards to on the same inputs, and compares the outputs byte
 // for byte.  Built by ozblas_amd/build.py into ozblas_amd/_build/shim_check; run by
 // tests/test_gpu_shim.py.  Prints one line per routine and "shim check: all ok" on success.
 #include <hip/hip_runtime.h>
@@ -125,6 +125,93 @@ struct CS {
 	}
 };
 
+// the mixed routines: binary64 data, binary32 slices (templates at <double,float,...>)
+struct CDS {
+	static int32_t gemm(ozblasHandle_t *h, char a, char b, int m, int n, int k, double al, const double *A,
+	                    int la, const double *B, int lb, double be, double *C, int lc) {
+		return ozblasDSgemm(h, a, b, m, n, k, al, A, la, B, lb, be, C, lc);
+	}
+};
+
+static int run_mixed() {
+	cuozblasHandle_t h1, h2;
+	cuozblasCreate(&h1, 256ull << 20);
+	if (ozblasCreate(&h2, 256ull << 20) != 0) return 2;
+	h1.sumMode = h2.sumMode = 0;  // Fsum: the only sum of the mixed routines
+	const int m = 37, n = 29, k = 53;
+	{
+		const int lda = m + 3, ldb = n + 1, ldc = m + 2;
+		Dev<double> A(rand_vec<double>((size_t)lda * k, 4.0)), B(rand_vec<double>((size_t)ldb * k, 0.5));
+		const std::vector<double> C0 = rand_vec<double>((size_t)ldc * n, 1.0);
+		Dev<double> C1(C0), C2(C0);
+		const int32_t s1 = cuozblasRgemm<double, float, float>(&h1, 'n', 't', m, n, k, 1.5, A.p, lda, B.p,
+		                                                       ldb, -0.25, C1.p, ldc);
+		const int32_t s2 = CDS::gemm(&h2, 'n', 't', m, n, k, 1.5, A.p, lda, B.p, ldb, -0.25, C2.p, ldc);
+		report("DS", "gemm", same(C1.get(), C2.get()) && !same(C1.get(), C0), s1, s2);
+	}
+	{
+		Dev<double> A(rand_vec<double>((size_t)m * n, 2.0)), x(rand_vec<double>(m, 1.0));
+		const std::vector<double> y0 = rand_vec<double>(n, 1.0);
+		Dev<double> y1(y0), y2(y0);
+		const int32_t s1 = cuozblasRgemv<double, float, float>(&h1, 't', m, n, 0.75, A.p, m, x.p, 1, 2.0, y1.p, 1);
+		const int32_t s2 = ozblasDSgemv(&h2, 't', m, n, 0.75, A.p, m, x.p, 1, 2.0, y2.p, 1);
+		report("DS", "gemv", same(y1.get(), y2.get()), s1, s2);
+	}
+	{
+		const int len = 1001;
+		Dev<double> x(rand_vec<double>(len, 3.0)), y(rand_vec<double>(len, 1.0));
+		double r1 = 0, r2 = 0;
+		int32_t s1 = cuozblasRdot<double, float, float>(&h1, len, x.p, 1, y.p, 1, &r1);
+		int32_t s2 = ozblasDSdot(&h2, len, x.p, 1, y.p, 1, &r2);
+		report("DS", "dot", std::memcmp(&r1, &r2, 8) == 0, s1, s2);
+		s1 = cuozblasRnrm2<double, float, float>(&h1, len, x.p, 1, &r1);
+		s2 = ozblasDSnrm2(&h2, len, x.p, 1, &r2);
+		report("DS", "nrm2", std::memcmp(&r1, &r2, 8) == 0 && r1 > 0, s1, s2);
+	}
+	{
+		const int nn = 300;
+		std::vector<int> rp(nn + 1), ci;
+		std::vector<double> val;
+		for (int r = 0; r < nn; r++) {
+			rp[r] = (int)ci.size();
+			for (int c = r - 1; c <= r + 1; c++) {
+				if (c < 0 || c >= nn) continue;
+				ci.push_back(c);
+				val.push_back(c == r ? 2.001 : -1.0);
+			}
+		}
+		rp[nn] = (int)ci.size();
+		const int nnz = (int)val.size();
+		Dev<double> A(val), x(rand_vec<double>(nn, 1.0)), b(rand_vec<double>(nn, 1.0));
+		Dev<int> CI(ci), RP(rp);
+		const std::vector<double> y0 = rand_vec<double>(nn, 1.0);
+		Dev<double> y1(y0), y2(y0);
+		int32_t s1 = cuozblasRcsrmv<double, float>(&h1, 'n', nn, nn, nnz, 1.5, nullptr, A.p, CI.p, RP.p, x.p,
+		                                           -0.5, y1.p);
+		int32_t s2 = ozblasDScsrmv(&h2, 'n', nn, nn, nnz, 1.5, nullptr, A.p, CI.p, RP.p, x.p, -0.5, y2.p);
+		report("DS", "csrmv", same(y1.get(), y2.get()), s1, s2);
+		float *sp1 = cuozblasRcsrmvSplitA<double, float>(&h1, 'n', nn, nn, nnz, nullptr, A.p, RP.p);
+		float *sp2 = ozblasDScsrmvSplitA(&h2, 'n', nn, nn, nnz, nullptr, A.p, RP.p);
+		Dev<double> z1(y0), z2(y0);
+		s1 = sp1 ? cuozblasRcsrmv<double, float>(&h1, 'n', nn, nn, nnz, 1.0, nullptr, (const double *)sp1,
+		                                         CI.p, RP.p, x.p, 0.0, z1.p)
+		         : h1.lastStatus;
+		s2 = sp2 ? ozblasDScsrmv(&h2, 'n', nn, nn, nnz, 1.0, nullptr, (const double *)sp2, CI.p, RP.p, x.p,
+		                         0.0, z2.p)
+		         : h2.lastStatus;
+		h1.memMaskSplitA = h2.memMaskSplitA = 0;
+		report("DS", "csrmvSplitA", sp1 && sp2 && same(z1.get(), z2.get()), s1, s2);
+		const std::vector<double> x0(nn, 0.0);
+		Dev<double> xc1(x0), xc2(x0);
+		s1 = cuozblasRcg<double, float>(&h1, 'n', nn, nnz, nullptr, A.p, CI.p, RP.p, b.p, xc1.p, 25, 0.0);
+		s2 = ozblasDScg(&h2, 'n', nn, nnz, nullptr, A.p, CI.p, RP.p, b.p, xc2.p, 25, 0.0);
+		report("DS", "cg", same(xc1.get(), xc2.get()) && h1.cg_numiter == 25, s1, s2);
+	}
+	cuozblasDestroy(&h1);
+	ozblasDestroy(&h2);
+	return 0;
+}
+
 template <class P>
 static int run() {
 	using T = typename P::T;
@@ -210,7 +297,7 @@ static int run() {
 }
 
 int main() {
-	if (run<CD>() != 0 || run<CS>() != 0) return 2;
+	if (run<CD>() != 0 || run<CS>() != 0 || run_mixed() != 0) return 2;
 	if (g_fail) {
 		printf("shim check: FAILED\n");
 		return 1;

@@ -35,6 +35,8 @@ def test_header_declares_reference_entry_points():
     for p in "DS":
         for r in ("dot", "nrm2", "axpy", "gemv", "gemm", "csrmv", "csrmvSplitA", "cg"):
             assert f"ozblas{p}{r}" in names
+    for r in ("dot", "nrm2", "gemv", "gemm", "csrmv", "csrmvSplitA", "cg"):
+        assert f"ozblasDS{r}" in names
     assert "ozblasCreate" in names and "ozblasDestroy" in names
 
 
@@ -136,6 +138,19 @@ def test_binary32_rho_rules_match_libm(lib, oracle):
             assert ozblas_amd.internal_rho_row_f(d, mode) == want, (mode, d)
 
 
+def test_mixed_rho_rules_match_libm(lib, oracle):
+    """getRho<double,float> (binary64 data, binary32 slices) for splitEpsMode 0, 2 and 9: the host
+    form and the integer form of the CSR rows agree with the oracle's libm evaluation."""
+    import ozblas_amd
+    dims = list(range(0, 5000)) + [2 ** p + o for p in range(12, 31) for o in (-1, 0, 1)]
+    dims += [10 ** 6, 123456789, 2 ** 31 - 1]
+    for mode in (0, 2, 9):
+        for d in dims:
+            want = oracle.ds_rho(d, mode)
+            assert ozblas_amd.internal_rho_ds(d, mode) == want, (mode, d)
+            assert ozblas_amd.internal_rho_row_ds(d, mode) == want, (mode, d)
+
+
 def test_gen_phi_matches_oracle_generator(oracle):
     import ozblas_amd
     for phi, seed in ((1.0, 1), (0.5, 2), (4.0, 7)):
@@ -231,6 +246,6 @@ def test_cpp_shim_binds_every_entry_point():
     out = subprocess.run(["nm", "-D", "--undefined-only", exe], capture_output=True, text=True,
                          check=True).stdout
     used = set(re.findall(r"\b(ozblas\w+)\b", out))
-    for p in "DS":
-        for r in ("dot", "nrm2", "axpy", "gemv", "gemm", "csrmv", "csrmvSplitA", "cg"):
+    for p in ("D", "S", "DS"):
+        for r in ("dot", "nrm2", "gemv", "gemm", "csrmv", "csrmvSplitA", "cg"):
             assert f"ozblas{p}{r}" in used

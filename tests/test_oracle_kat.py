@@ -1,7 +1,8 @@
 """The oracle against the reference's own accuracy logs (script/a100/*.dat, tests/golden/a100_kat.json).
 
 Each log row holds the max relative error (hex) and the number of entries that differ from
-RN(exact) for one size of a sweep of testing_ozblas_{d,s}{gemm,dot,gemv} (binary64 and binary32).  Regenerating the harness
+RN(exact) for one size of a sweep of testing_ozblas_{d,s,ds}{gemm,dot,gemv} (binary64, binary32
+and the mixed binary64 data / binary32 slices).  Regenerating the harness
 inputs and replaying the sweep through the oracle must give identical rows: this pins the split
 (slice boundaries and counts), the Fsum order, NearSum and the GEMM/GEMV/DOT drivers bit for bit.
 """
@@ -29,7 +30,7 @@ STALE_SLOT_LOGS = {"_sdot_sum1_s.dat", "_sdot_sum0_s.dat"}
 def test_reference_log(case, oracle):
     th = K.harness_opts(case["opts"])
     prec = case.get("prec", "d")
-    impl = (K.OracleImplS if prec == "s" else K.OracleImpl)(K.handle_flags(th))
+    impl = {"s": K.OracleImplS, "ds": K.OracleImplDS}.get(prec, K.OracleImpl)(K.handle_flags(th))
     if th["fastMode"]:
         # the fastmode=1 logs were produced with the legacy level rule (see ozoracle.h)
         impl.h.set(fastModeRule=1)
