@@ -90,9 +90,11 @@ struct DenseSplit {
 	int count = 0;
 };
 
+// knownZ >= 0: the first-zero level of these very columns, measured by count_cols (skips the
+// counting pass)
 template <class T>
 static int32_t dense_split(ozblasHandle_t *oh, int k, int cols, const T *in, int64_t ldi, T *rem,
-                           int64_t ldr, DenseSplit &ds) {
+                           int64_t ldr, DenseSplit &ds, int knownZ = -1) {
 	Backend *b = be(oh);
 	const int cap = split_cap(oh);
 	const int rho = Prec<T>::rho(oh, k);
@@ -101,25 +103,26 @@ static int32_t dense_split(ozblasHandle_t *oh, int k, int cols, const T *in, int
 	ds.sstride = ds.ldk * cols;
 	int *zcol = (int *)walloc(oh, sizeof(int) * (size_t)cols);
 	if (!zcol) return OZBLAS_STATUS_WORKSPACE;
-	// the store pass needs the count; measure it first without storing (slices -> rem scratch)
-	OZ_HIP(hipMemsetAsync(b->dCounters, 0, sizeof(int), b->stream));
-	const int capMeasure = cap;
-	// counting pass: slices land in a scratch of one slot that is overwritten (sstride = 0)
-	double *scratch = (double *)walloc(oh, sizeof(double) * (size_t)ds.ldk * cols * 2);
-	if (!scratch) return OZBLAS_STATUS_WORKSPACE;
-	launch_split_cols(b->stream, k, cols, in, ldi, rem, ldr, scratch, ds.ldk, 0, rho,
-	                  oh->splitShift, capMeasure, J, zcol, b->dCounters);
-	OZ_HIP(hipMemcpyAsync(b->hPinned, b->dCounters, sizeof(int), hipMemcpyDeviceToHost, b->stream));
-	OZ_TRY(sync(oh));
-	const int zmax = (int)((int32_t *)b->hPinned)[0];
+	int zmax = knownZ;
+	if (zmax < 0) {
+		// the store pass needs the count; measure it first without storing (slices -> rem scratch)
+		OZ_HIP(hipMemsetAsync(b->dCounters, 0, sizeof(int), b->stream));
+		// counting pass: slices land in a scratch of one slot that is overwritten (sstride = 0)
+		double *scratch = (double *)walloc(oh, sizeof(double) * (size_t)ds.ldk * cols * 2);
+		if (!scratch) return OZBLAS_STATUS_WORKSPACE;
+		launch_split_cols(b->stream, k, cols, in, ldi, rem, ldr, scratch, ds.ldk, 0, rho,
+		                  oh->splitShift, cap, J, zcol, b->dCounters);
+		OZ_HIP(hipMemcpyAsync(b->hPinned, b->dCounters, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+		OZ_TRY(sync(oh));
+		zmax = (int)((int32_t *)b->hPinned)[0];
+	}
 	ds.count = split_count(oh, zmax);
 	// store pass: count slots (+1 for the remainder slot the kernel writes after the last slice)
 	const int slots = imax(ds.count, imin(zmax, cap) + 2);
 	ds.slices = (double *)walloc(oh, sizeof(double) * (size_t)ds.sstride * slots);
 	if (!ds.slices) return OZBLAS_STATUS_WORKSPACE;
 	launch_split_cols(b->stream, k, cols, in, ldi, rem, ldr, ds.slices, ds.ldk, ds.sstride, rho,
-	                  oh->splitShift, cap, J, zcol, b->dCounters + 1);
-	launch_zero_slots(b->stream, k, cols, ds.slices, ds.ldk, ds.sstride, zcol, ds.count);
+	                  oh->splitShift, cap, J, zcol, b->dCounters + 1, ds.count);
 	// zero the pad rows [k, ldk) of every slot the product kernels read
 	if (ds.ldk > k) {
 		for (int s = 0; s < ds.count; s++)
@@ -134,7 +137,7 @@ static int32_t dense_split(ozblasHandle_t *oh, int k, int cols, const T *in, int
 // ---------------------------------------------------------------------------------------------
 template <class T>
 static int32_t count_cols(ozblasHandle_t *oh, int k, int cols, const T *in, int64_t ldi,
-                          int *count) {
+                          int *count, int *zmaxOut = nullptr) {
 	Backend *b = be(oh);
 	const uint64_t mark = wmark(oh);
 	const int64_t ldk = rup(k, 16);
@@ -149,6 +152,7 @@ static int32_t count_cols(ozblasHandle_t *oh, int k, int cols, const T *in, int6
 	OZ_HIP(hipMemcpyAsync(b->hPinned, b->dCounters, sizeof(int), hipMemcpyDeviceToHost, b->stream));
 	OZ_TRY(sync(oh));
 	*count = split_count(oh, ((int32_t *)b->hPinned)[0]);
+	if (zmaxOut) *zmaxOut = ((int32_t *)b->hPinned)[0];
 	wreset(oh, mark);
 	return OZBLAS_STATUS_SUCCESS;
 }
@@ -192,9 +196,9 @@ static int32_t gemm_plain(ozblasHandle_t *oh, char transA, char transB, int m, i
 	const uint64_t base2 = wmark(oh);
 	// blocking: slice counts of the whole operands bound those of any block, so halving the
 	// larger side (as src/cpu/ozblas_gemm.cpp:91-96) until the exact need fits is safe
-	int nAf = 0, nBf = 0;
-	OZ_TRY(count_cols(oh, k, m, ain, ldai, &nAf));
-	OZ_TRY(count_cols(oh, k, n, bin, ldbi, &nBf));
+	int nAf = 0, nBf = 0, zAf = -1, zBf = -1;
+	OZ_TRY(count_cols(oh, k, m, ain, ldai, &nAf, &zAf));
+	OZ_TRY(count_cols(oh, k, n, bin, ldbi, &nBf, &zBf));
 	auto need = [&](int64_t mb, int64_t nb) -> uint64_t {
 		uint64_t bytes = 8ull * ldk * std::max(mb, nb) * 2;            // remainder + count scratch
 		bytes += 8ull * ldk * (mb * (nAf + 1) + nb * (nBf + 1));        // slices (+ remainder slot)
@@ -223,8 +227,11 @@ static int32_t gemm_plain(ozblasHandle_t *oh, char transA, char transB, int m, i
 			T *rem = (T *)walloc(oh, sizeof(T) * ldk * std::max(mbk_, nbk_));
 			if (!rem) return OZBLAS_STATUS_WORKSPACE;
 			DenseSplit da, db;
-			OZ_TRY(dense_split(oh, k, mbk_, ain + (int64_t)im * mbk * ldai, ldai, rem, ldk, da));
-			OZ_TRY(dense_split(oh, k, nbk_, bin + (int64_t)in * nbk * ldbi, ldbi, rem, ldk, db));
+			// a block that is the whole operand reuses the first-zero level of count_cols
+			OZ_TRY(dense_split(oh, k, mbk_, ain + (int64_t)im * mbk * ldai, ldai, rem, ldk, da,
+			                   mbk == m ? zAf : -1));
+			OZ_TRY(dense_split(oh, k, nbk_, bin + (int64_t)in * nbk * ldbi, ldbi, rem, ldk, db,
+			                   nbk == n ? zBf : -1));
 			const int nA = da.count, nB = db.count;
 			// ---- pairs (src/cpu/ozblas_gemm.cpp:163-164,237-264)
 			int maxlevel = (nA - 1) + (nB - 1);
@@ -260,7 +267,13 @@ static int32_t gemm_plain(ozblasHandle_t *oh, char transA, char transB, int m, i
 			// reference's GEMM with a width-1 column block reads them through the GEMV map instead
 			const bool identityMap = gemvPath || dotPath || sumorder == 1;
 			bool summed = false;
-			if (fsum && identityMap) {
+			// GEMV: the bandwidth-bound slice kernel with the sum fused (all nA x nB pairs: fastMode 0)
+			if (gemvPath && nbk_ == 1 &&
+			    launch_gemv_slices(st, mbk_, k, da.slices, da.ldk, da.sstride, nA, db.slices, db.sstride, nB,
+			                       fsum ? 1 : 0, alpha, beta, Cblk, Prec<T>::Jsum(b))) {
+				OZ_HIP(hipEventRecord(b->ev[4], st));
+				summed = true;
+			} else if (fsum && identityMap) {
 				launch_gemm_f64_fsum(st, mbk_, nbk_, k, da.slices, da.ldk, db.slices, db.ldk, dOff,
 				                     dOff + ic, ic, alpha, beta, Cblk, ldc);
 				OZ_HIP(hipEventRecord(b->ev[4], st));

@@ -6,12 +6,14 @@
 namespace oz {
 
 // ---- split (oz_split.hip)
+// sstride = 0: the counting pass (zcol / zmax only); otherwise slots [0, fill) of every column are
+// valid afterwards (slices, the remainder slot, zeros), fill being the operand's slice count
 void launch_split_cols(hipStream_t st, int m, int n, const double *in, int64_t ldi, double *out,
                        int64_t ldo, double *split, int64_t lds, int64_t sstride, int rho,
-                       int shift, int cap, const uint16_t *J, int *zcol, int *zmax);
+                       int shift, int cap, const uint16_t *J, int *zcol, int *zmax, int fill = 0);
 void launch_split_cols(hipStream_t st, int m, int n, const float *in, int64_t ldi, float *out,
                        int64_t ldo, double *split, int64_t lds, int64_t sstride, int rho,
-                       int shift, int cap, const uint16_t *JF, int *zcol, int *zmax);
+                       int shift, int cap, const uint16_t *JF, int *zcol, int *zmax, int fill = 0);
 void launch_zero_slots(hipStream_t st, int m, int n, double *split, int64_t lds, int64_t sstride,
                        const int *zcol, int count);
 void launch_transpose(hipStream_t st, int rows, int cols, const double *in, int64_t ldi,
@@ -130,6 +132,15 @@ void launch_csrmm_sum(hipStream_t st, int m, int nnz, const double *As, int64_t 
 void launch_csrmm_store(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
                         const int *colind, const int *rowptr, const double *Xs, int64_t ldxs,
                         int nB, int maxlevel, float *Cs, int64_t ldcs);
+// GEMV of the plain form on k-contiguous slices (slice s of A column i at As + s*sA + i*ldk, of x
+// at Xs + s*sX), fused with the sum (Fsum in level order or NearSum) and the epilogue into y;
+// false when the slice counts exceed its register bound (8 x 8)
+bool launch_gemv_slices(hipStream_t st, int M, int K, const double *As, int64_t ldk, int64_t sA, int nA,
+                        const double *Xs, int64_t sX, int nB, int fsum, double alpha, double beta,
+                        double *y, const uint16_t *J);
+bool launch_gemv_slices(hipStream_t st, int M, int K, const double *As, int64_t ldk, int64_t sA, int nA,
+                        const double *Xs, int64_t sX, int nB, int fsum, float alpha, float beta,
+                        float *y, const uint16_t *JF2);
 // barrier-free vector split (plan of levels, then slices); VsPlan lives in device memory
 constexpr int kVsMaxLevels = 64;
 struct VsPlan {

@@ -784,6 +784,136 @@ void launch_csrmm_sum_x2(hipStream_t st, int m, const double *As, int64_t ldas, 
 	                   nA, colind, rowptr, Xs, ldxs, nB, fsum, alpha, beta, y, J);
 }
 
+// ---------------------------------------------------------------------------------------------
+// GEMV of the plain form (the reference's GEMV branch, src/cpu/ozblas_gemm.cpp:216-225, summed
+// with sumOrder 3, src/cpu/ozblas_sum.cpp:331-380, or NearSum :231-271), bandwidth-bound: one wave
+// per output element i reads column i of every A slice once (k-contiguous, 16-byte loads) and the
+// x slices through the cache, v[ia][ib] = sum_k As_ia[k, i] Xs_ib[k] (exact: any order), then
+// lane 0 sums the nA x nB values -- Fsum in (level, ia) order, the order of the reference's index
+// map nB*ia + ib, or NearSum -- and y_i = fma1(alpha, t, beta * y_i).  Slices are padded with zeros
+// to a multiple of 16 in k, so the paired loads never leave a slice.  T = float: the binary32 sum.
+// ---------------------------------------------------------------------------------------------
+template <class T, int NAM, int NBM, int R>
+__global__ __launch_bounds__(256) void k_gemv_slices(int M, int K, const double *As, int64_t ldk,
+                                                     int64_t sA, int nA, const double *Xs,
+                                                     int64_t sX, int nB, int fsum, T alpha, T beta,
+                                                     T *y, const uint16_t *J) {
+	// R rows per wave share each load of the x slices
+	const int lane = threadIdx.x % 64;
+	const int row0 = (blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) * R;
+	if (row0 >= M) return;  // the whole wave
+	double acc[R][NAM][NBM];
+#pragma unroll
+	for (int r = 0; r < R; r++)
+#pragma unroll
+		for (int a = 0; a < NAM; a++)
+#pragma unroll
+			for (int b = 0; b < NBM; b++) acc[r][a][b] = 0.0;
+	const int rows = (M - row0 < R) ? M - row0 : R;
+	for (int k = 2 * lane; k < K; k += 128) {
+		// every load of the step first, so that they are all in flight together
+		double2 xv[NBM], av[R][NAM];
+#pragma unroll
+		for (int b = 0; b < NBM; b++)
+			xv[b] = (b < nB) ? *(const double2 *)(Xs + (int64_t)b * sX + k) : make_double2(0.0, 0.0);
+#pragma unroll
+		for (int r = 0; r < R; r++) {
+			const double *arow = As + (int64_t)(row0 + r) * ldk + k;
+#pragma unroll
+			for (int a = 0; a < NAM; a++)
+				av[r][a] = (r < rows && a < nA) ? *(const double2 *)(arow + (int64_t)a * sA)
+				                                : make_double2(0.0, 0.0);
+		}
+#pragma unroll
+		for (int r = 0; r < R; r++)
+#pragma unroll
+			for (int a = 0; a < NAM; a++)
+#pragma unroll
+				for (int b = 0; b < NBM; b++) {
+					acc[r][a][b] = fma(av[r][a].x, xv[b].x, acc[r][a][b]);
+					acc[r][a][b] = fma(av[r][a].y, xv[b].y, acc[r][a][b]);
+				}
+	}
+#pragma unroll
+	for (int r = 0; r < R; r++)
+#pragma unroll
+		for (int a = 0; a < NAM; a++)
+#pragma unroll
+			for (int b = 0; b < NBM; b++)
+				for (int o = 32; o > 0; o >>= 1) acc[r][a][b] += __shfl_xor(acc[r][a][b], o, 64);
+	// lane r sums row0 + r
+#pragma unroll
+	for (int r = 0; r < R; r++) {
+		if (lane != r || r >= rows) continue;
+		T t;
+		if (fsum) {
+			t = (T)0;
+#pragma unroll
+			for (int ik = 0; ik < NAM + NBM - 1; ik++)
+#pragma unroll
+				for (int a = 0; a < NAM; a++) {
+					const int b = ik - a;
+					if (b >= 0 && b < NBM && a < nA && b < nB) t = t + (T)acc[r][a][b];
+				}
+		} else {
+			RegVec<NAM * NBM, T> u;
+			int cnt = 0;
+#pragma unroll
+			for (int q = 0; q < NAM * NBM; q++) u[q] = (T)0;
+#pragma unroll
+			for (int a = 0; a < NAM; a++)
+#pragma unroll
+				for (int b = 0; b < NBM; b++) {
+					const bool ok = a < nA && b < nB;
+#pragma unroll
+					for (int q = 0; q < NAM * NBM; q++)
+						if (ok && q == cnt) u[q] = (T)acc[r][a][b];
+					cnt += ok ? 1 : 0;
+				}
+			if constexpr (sizeof(T) == 4)
+				t = nearsum_f(cnt, u, J);
+			else
+				t = nearsum(cnt, u, J);
+		}
+		y[row0 + r] = fma1_t(alpha, t, beta * y[row0 + r]);
+	}
+}
+
+template <class T>
+static bool gemv_slices(hipStream_t st, int M, int K, const double *As, int64_t ldk, int64_t sA,
+                        int nA, const double *Xs, int64_t sX, int nB, int fsum, T alpha, T beta, T *y,
+                        const uint16_t *J) {
+	const dim3 b(256);
+	auto grid = [&](int R) { return dim3((unsigned)((M + 4 * R - 1) / (4 * R))); };
+	if (nA <= 4 && nB <= 4)
+		hipLaunchKernelGGL((k_gemv_slices<T, 4, 4, 1>), grid(1), b, 0, st, M, K, As, ldk, sA, nA, Xs, sX, nB,
+		                   fsum, alpha, beta, y, J);
+	else if (nA <= 8 && nB <= 4)
+		hipLaunchKernelGGL((k_gemv_slices<T, 8, 4, 1>), grid(1), b, 0, st, M, K, As, ldk, sA, nA, Xs, sX, nB,
+		                   fsum, alpha, beta, y, J);
+	else if (nA <= 4 && nB <= 8)
+		hipLaunchKernelGGL((k_gemv_slices<T, 4, 8, 1>), grid(1), b, 0, st, M, K, As, ldk, sA, nA, Xs, sX, nB,
+		                   fsum, alpha, beta, y, J);
+	else if (nA <= 8 && nB <= 8)
+		hipLaunchKernelGGL((k_gemv_slices<T, 8, 8, 1>), grid(1), b, 0, st, M, K, As, ldk, sA, nA, Xs, sX, nB,
+		                   fsum, alpha, beta, y, J);
+	else
+		return false;
+	return true;
+}
+
+bool launch_gemv_slices(hipStream_t st, int M, int K, const double *As, int64_t ldk, int64_t sA, int nA,
+                        const double *Xs, int64_t sX, int nB, int fsum, double alpha, double beta,
+                        double *y, const uint16_t *J) {
+	return gemv_slices(st, M, K, As, ldk, sA, nA, Xs, sX, nB, fsum, alpha, beta, y, J);
+}
+
+bool launch_gemv_slices(hipStream_t st, int M, int K, const double *As, int64_t ldk, int64_t sA, int nA,
+                        const double *Xs, int64_t sX, int nB, int fsum, float alpha, float beta,
+                        float *y, const uint16_t *JF2) {
+	return gemv_slices(st, M, K, As, ldk, sA, nA, Xs, sX, nB, fsum, alpha, beta, y, JF2);
+}
+
 // general case: store the products where blasRcsrmm puts them (column ic of CSplit, ld ldcs);
 // T = float: the fma1<float> chain of csrmm_n<float>
 template <class T>

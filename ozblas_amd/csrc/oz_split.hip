@@ -66,6 +66,71 @@ __global__ __launch_bounds__(BS) void k_split_cols(int m, int n, const TI *in, i
 	}
 }
 
+// The same split with the column held in registers (R elements per thread, m <= BS * R): the
+// remainder never goes back to memory, so a column costs one read and the slice writes (none in
+// the counting pass, store = 0).  Same operations, same order per element, same bits.  Slots
+// [last + 1, fill) -- the reproMode = 0 remainder slot and the all-zero slices up to the operand's
+// slice count -- are written here too; slots the products never read are not.
+template <int BS, int R, class TI>
+__global__ __launch_bounds__(BS) void k_split_cols_reg(int m, int n, const TI *in, int64_t ldi,
+                                                       double *split, int64_t lds, int64_t sstride,
+                                                       int rho, int shift, int cap,
+                                                       const uint16_t *J, int *zcol, int *zmax,
+                                                       int store, int fill) {
+	__shared__ double red[BS / 64];
+	for (int j = blockIdx.x; j < n; j += gridDim.x) {
+		const TI *x = in + (int64_t)j * ldi;
+		double *sp = split + (int64_t)j * lds;
+		TI v[R];
+		double mx = 0.0;
+#pragma unroll
+		for (int q = 0; q < R; q++) {
+			const int i = threadIdx.x + BS * q;
+			v[q] = (i < m) ? x[i] : (TI)0;
+			mx = std_max(mx, fabs((double)v[q]));
+		}
+		mx = block_max<BS>(mx, red);
+		int s = 0;
+		bool zero = false;
+		for (s = 0; s < cap; s++) {
+			const int tau = SplitRule<TI>::tau(mx, J);
+			const TI sigma = SplitRule<TI>::sigma(rho, tau, shift);
+			double lm = 0.0;
+#pragma unroll
+			for (int q = 0; q < R; q++) {
+				const int i = threadIdx.x + BS * q;
+				if (i < m) {
+					const TI a = (v[q] + sigma) - sigma;
+					const TI rr = v[q] - a;
+					if (store) sp[(int64_t)s * sstride + i] = (double)a;
+					v[q] = rr;
+					lm = std_max(lm, fabs((double)rr));
+				}
+			}
+			mx = block_max<BS>(lm, red);
+			if (mx == 0.0) {
+				zero = true;
+				break;
+			}
+		}
+		const int last = zero ? s : cap - 1;
+		if (store) {
+			for (int t = last + 1; t < fill; t++) {
+#pragma unroll
+				for (int q = 0; q < R; q++) {
+					const int i = threadIdx.x + BS * q;
+					if (i < m) sp[(int64_t)t * sstride + i] = (zero || t > last + 1) ? 0.0 : (double)v[q];
+				}
+			}
+		}
+		if (threadIdx.x == 0) {
+			const int z = zero ? s : cap;
+			zcol[j] = z;
+			atomicMax(zmax, z);
+		}
+	}
+}
+
 // zero slots [zcol[j]+2, count) of every column (what the reference computes by splitting zeros)
 __global__ void k_zero_slots(int m, int n, double *split, int64_t lds, int64_t sstride,
                              const int *zcol, int count) {
@@ -249,20 +314,43 @@ __global__ void k_digit_pad(int m, int n, int S, T *planes, int64_t ldp, int64_t
 // ---------------------------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------------------------
+// columns of up to 256 * 32 elements split in registers; longer ones through the remainder buffer.
+// sstride = 0 marks the counting pass (nothing stored); the store pass leaves `fill` valid slots.
+template <class TI>
+static void split_cols(hipStream_t st, int m, int n, const TI *in, int64_t ldi, TI *out, int64_t ldo,
+                       double *split, int64_t lds, int64_t sstride, int rho, int shift, int cap,
+                       const uint16_t *J, int *zcol, int *zmax, int fill) {
+	const int grid = n < 65535 ? n : 65535;
+	const int store = sstride != 0 ? 1 : 0;
+	if (m <= 256 * 4)
+		hipLaunchKernelGGL((k_split_cols_reg<256, 4, TI>), dim3(grid), dim3(256), 0, st, m, n, in, ldi,
+		                   split, lds, sstride, rho, shift, cap, J, zcol, zmax, store, fill);
+	else if (m <= 256 * 8)
+		hipLaunchKernelGGL((k_split_cols_reg<256, 8, TI>), dim3(grid), dim3(256), 0, st, m, n, in, ldi,
+		                   split, lds, sstride, rho, shift, cap, J, zcol, zmax, store, fill);
+	else if (m <= 256 * 16)
+		hipLaunchKernelGGL((k_split_cols_reg<256, 16, TI>), dim3(grid), dim3(256), 0, st, m, n, in, ldi,
+		                   split, lds, sstride, rho, shift, cap, J, zcol, zmax, store, fill);
+	else if (m <= 256 * 32)
+		hipLaunchKernelGGL((k_split_cols_reg<256, 32, TI>), dim3(grid), dim3(256), 0, st, m, n, in, ldi,
+		                   split, lds, sstride, rho, shift, cap, J, zcol, zmax, store, fill);
+	else {
+		hipLaunchKernelGGL((k_split_cols<256, TI>), dim3(grid), dim3(256), 0, st, m, n, in, ldi, out, ldo,
+		                   split, lds, sstride, rho, shift, cap, J, zcol, zmax);
+		if (store) launch_zero_slots(st, m, n, split, lds, sstride, zcol, fill);
+	}
+}
+
 void launch_split_cols(hipStream_t st, int m, int n, const double *in, int64_t ldi, double *out,
                        int64_t ldo, double *split, int64_t lds, int64_t sstride, int rho,
-                       int shift, int cap, const uint16_t *J, int *zcol, int *zmax) {
-	const int grid = n < 65535 ? n : 65535;
-	hipLaunchKernelGGL((k_split_cols<256, double>), dim3(grid), dim3(256), 0, st, m, n, in, ldi, out,
-	                   ldo, split, lds, sstride, rho, shift, cap, J, zcol, zmax);
+                       int shift, int cap, const uint16_t *J, int *zcol, int *zmax, int fill) {
+	split_cols(st, m, n, in, ldi, out, ldo, split, lds, sstride, rho, shift, cap, J, zcol, zmax, fill);
 }
 
 void launch_split_cols(hipStream_t st, int m, int n, const float *in, int64_t ldi, float *out,
                        int64_t ldo, double *split, int64_t lds, int64_t sstride, int rho,
-                       int shift, int cap, const uint16_t *JF, int *zcol, int *zmax) {
-	const int grid = n < 65535 ? n : 65535;
-	hipLaunchKernelGGL((k_split_cols<256, float>), dim3(grid), dim3(256), 0, st, m, n, in, ldi, out,
-	                   ldo, split, lds, sstride, rho, shift, cap, JF, zcol, zmax);
+                       int shift, int cap, const uint16_t *JF, int *zcol, int *zmax, int fill) {
+	split_cols(st, m, n, in, ldi, out, ldo, split, lds, sstride, rho, shift, cap, JF, zcol, zmax, fill);
 }
 
 void launch_zero_slots(hipStream_t st, int m, int n, double *split, int64_t lds, int64_t sstride,
