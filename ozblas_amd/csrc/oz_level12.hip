@@ -56,6 +56,57 @@ constexpr int kVsK = 2200;
 constexpr int kVsBlocks = 256;
 constexpr int kVsBS = 512;
 
+constexpr int kVsW = 64;   // k values a workgroup keeps in registers when its range is that narrow
+constexpr int kVsEC = 8;   // elements per thread kept in registers between the scan's two passes
+
+// x = m 2^eb with m < 2^53 an integer; lo/hi: the range of k with a non-trivial residue (below lo
+// it is 0, above hi it is x itself); m = 0 for zero and for inputs outside the model (bad = 1)
+struct VsElem {
+	uint64_t m;
+	int eb, lo, hi;
+};
+__device__ inline VsElem vs_decompose(double v, int &bad) {
+	VsElem r{0ull, -1074, INT32_MAX, INT32_MIN};
+	const uint64_t b = d2u(v);
+	const int E = (int)((b >> 52) & 0x7ff);
+	uint64_t m = b & ((uint64_t(1) << 52) - 1);
+	if (E == 0x7ff || E >= 1023 + 1022) {
+		bad = 1;
+		return r;
+	}
+	if (v == 0.0) return r;
+	int eb = -1074;
+	if (E != 0) {
+		m |= uint64_t(1) << 52;
+		eb = E - 1075;
+	}
+	r.m = m;
+	r.eb = eb;
+	r.lo = eb + __builtin_ctzll(m) + 1;
+	r.hi = eb + (64 - __builtin_clzll(m)) + 1;
+	return r;
+}
+
+// max-accumulates the keys of x for k = w0 .. w0 + kVsW - 1 into Fr (w0 at or below the element's
+// lowest set bit + 1)
+__device__ __forceinline__ void vs_window_keys(double v, int w0, uint64_t (&Fr)[kVsW]) {
+	int dummy = 0;
+	const VsElem q = vs_decompose(v, dummy);
+	if (q.m == 0) return;
+	const int s0 = w0 - q.eb;  // <= 53
+	uint64_t t = (s0 >= 1) ? (q.m << (64 - s0)) : 0ull;
+#pragma unroll
+	for (int kk = 0; kk < kVsW; kk++) {
+		if (kk > 0) {
+			const int bi = s0 + kk - 1;
+			const uint64_t bit = (bi >= 0 && bi < 64) ? ((q.m >> bi) & 1ull) : 0ull;
+			t = (t >> 1) | (bit << 63);
+		}
+		const uint64_t key = (t > (uint64_t(1) << 63)) ? (0ull - t) : t;
+		Fr[kk] = key > Fr[kk] ? key : Fr[kk];
+	}
+}
+
 __device__ inline double vs_cmod(double x, int k) {
 	if (x == 0.0) return 0.0;
 	const uint64_t b = d2u(x);
@@ -85,56 +136,33 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned
                                                     int *info, unsigned long long *maxv,
                                                     const int *stopAt, int it) {
 	if (stopAt && it >= *stopAt) return;
-	__shared__ unsigned long long Fb[kVsK], Gb[kVsK];  // Gb: scratch of the last workgroup
+	__shared__ unsigned long long Fb[kVsK];
+	unsigned long long *Gb = Fb;  // the last workgroup reuses its table once folded into Fg
 	__shared__ int kminB, kmaxB, badB, isLast;
 	__shared__ int red[2][BS / 64];
 	for (int k = threadIdx.x; k < kVsK; k += BS) Fb[k] = 0ull;
 	if (threadIdx.x == 0) badB = 0;
 	__syncthreads();
 	const int lane = threadIdx.x % 64;
-	int tlo = INT32_MAX, thi = INT32_MIN;  // this thread's range of k
-	for (int64_t base = (int64_t)blockIdx.x * BS; base < n; base += (int64_t)gridDim.x * BS) {
-		const int64_t i = base + threadIdx.x;
-		const double v = (i < n) ? (double)x[i] : 0.0;
-		// x = m 2^eb with m < 2^53 an integer (m = 0: no contribution)
-		const uint64_t b = d2u(v);
-		const int E = (int)((b >> 52) & 0x7ff);
-		uint64_t m = b & ((uint64_t(1) << 52) - 1);
-		int eb = -1074, lo = INT32_MAX, hi = INT32_MIN;
-		if (E == 0x7ff || E >= 1023 + 1022) {
-			badB = 1;
-			m = 0;
-		} else if (v != 0.0) {
-			if (E != 0) {
-				m |= uint64_t(1) << 52;
-				eb = E - 1075;
-			}
-			lo = eb + __builtin_ctzll(m) + 1;      // k below: residue 0
-			hi = eb + (64 - __builtin_clzll(m)) + 1;  // msb + 2: residue x itself
-		} else {
-			m = 0;
-		}
-		tlo = min(tlo, lo);
-		thi = max(thi, hi);
-		if (m == 0) continue;
-		// key(k) = |cmod(x, 2^k)| / 2^k * 2^64 for k in [lo, hi]: m modulo 2^s centred (ties to
-		// even), s = k - eb; exact (< 2^63, at most 53 significant bits).  Each lane starts at its
-		// own offset so one LDS atomic instruction hits different table entries.
-		const int K = hi - lo + 1;
-		for (int j = 0; j < K; j++) {
-			int t = j + lane;
-			if (t >= K) {
-				t -= K;
-				if (t >= K) t %= K;
-			}
-			const int k = lo + t;
-			const int sft = k - eb;  // 1 .. 54
-			uint64_t r = m & ((uint64_t(1) << sft) - 1);
-			const uint64_t half = uint64_t(1) << (sft - 1);
-			if (r > half || (r == half && ((m >> sft) & 1))) r = (uint64_t(1) << sft) - r;
-			if (r) atomicMax(&Fb[k + kVsOff], r << (64 - sft));
-		}
+	// pass 1: the range of k of this workgroup (elements kept in registers for pass 2)
+	const int64_t stride = (int64_t)gridDim.x * BS;
+	const int64_t i0 = (int64_t)blockIdx.x * BS + threadIdx.x;
+	double xc[kVsEC];
+	int tlo = INT32_MAX, thi = INT32_MIN, tbad = 0;
+#pragma unroll
+	for (int e = 0; e < kVsEC; e++) {
+		const int64_t i = i0 + e * stride;
+		xc[e] = (i < n) ? (double)x[i] : 0.0;
+		const VsElem q = vs_decompose(xc[e], tbad);
+		tlo = min(tlo, q.lo);
+		thi = max(thi, q.hi);
 	}
+	for (int64_t i = i0 + kVsEC * stride; i < n; i += stride) {
+		const VsElem q = vs_decompose((double)x[i], tbad);
+		tlo = min(tlo, q.lo);
+		thi = max(thi, q.hi);
+	}
+	if (tbad) badB = 1;
 	// workgroup range of k (a shuffle tree per wave, then the waves)
 	for (int o = 32; o > 0; o >>= 1) {
 		tlo = min(tlo, __shfl_xor(tlo, o, 64));
@@ -155,8 +183,56 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned
 		kmaxB = c;
 	}
 	__syncthreads();
-	// fold this workgroup's table over its range into the global one (Fg: zero between uses)
 	const int r0 = kminB, r1 = kmaxB;
+	// pass 2: key(k) = |cmod(x, 2^k)| / 2^k * 2^64 -- m modulo 2^s centred, s = k - eb, exact
+	// (< 2^63, at most 53 significant bits) -- maximised over the workgroup for every k.
+	if (r0 <= r1 && r1 - r0 < kVsW && n <= kVsEC * stride) {
+		// narrow range, every element in registers: the keys of the window k = r0 .. r0 + kVsW - 1
+		// in registers too, t_s = (m mod 2^s) << (64 - s) built up one bit per k and
+		// key = min(t, 2^64 - t); below an element's lowest bit t is 0, above its msb + 2 the key is
+		// at most |x| (the table's prefix max is unchanged by it)
+		uint64_t Fr[kVsW];
+#pragma unroll
+		for (int kk = 0; kk < kVsW; kk++) Fr[kk] = 0ull;
+#pragma unroll
+		for (int e = 0; e < kVsEC; e++) vs_window_keys(xc[e], r0, Fr);
+#pragma unroll
+		for (int kk = 0; kk < kVsW; kk++) {
+			uint64_t v = Fr[kk];
+			for (int o = 32; o > 0; o >>= 1) {
+				const uint64_t w = __shfl_xor(v, o, 64);
+				v = w > v ? w : v;
+			}
+			if (lane == 0 && v) atomicMax(&Fb[r0 + kk + kVsOff], (unsigned long long)v);
+		}
+	} else if (r0 <= r1) {
+		// wide range: the keys of k in [lo, hi] of each element into the table directly; each lane
+		// starts at its own offset so one LDS atomic instruction hits different table entries
+		auto add = [&](double v) {
+			int dummy = 0;
+			const VsElem q = vs_decompose(v, dummy);
+			if (q.m == 0) return;
+			const int K = q.hi - q.lo + 1;
+			for (int j = 0; j < K; j++) {
+				int t = j + lane;
+				if (t >= K) {
+					t -= K;
+					if (t >= K) t %= K;
+				}
+				const int k = q.lo + t;
+				const int sft = k - q.eb;  // 1 .. 54
+				uint64_t r = q.m & ((uint64_t(1) << sft) - 1);
+				const uint64_t half = uint64_t(1) << (sft - 1);
+				if (r > half || (r == half && ((q.m >> sft) & 1))) r = (uint64_t(1) << sft) - r;
+				if (r) atomicMax(&Fb[k + kVsOff], r << (64 - sft));
+			}
+		};
+#pragma unroll
+		for (int e = 0; e < kVsEC; e++) add(xc[e]);
+		for (int64_t i = i0 + kVsEC * stride; i < n; i += stride) add((double)x[i]);
+	}
+	__syncthreads();
+	// fold this workgroup's table over its range into the global one (Fg: zero between uses)
 	if (r0 <= r1) {  // an all-zero workgroup has the empty range [INT32_MAX, INT32_MIN]
 		for (int k = r0 + (int)threadIdx.x; k <= r1; k += BS) {
 			const unsigned long long f = Fb[k + kVsOff];

@@ -232,3 +232,54 @@ def test_binary32_level_model_matches_reference_loop(oracle, reproMode, nSplitMa
             assert np.array_equal(g.astype(np.float64), got[s]), (n, s)  # slices are floats
             assert np.array_equal(g.view(np.uint32), want[s].view(np.uint32)), (n, s)
     assert used >= 7
+
+
+def window_keys(x, w0, width):
+    """The scan kernel's register form: keys of one element for k = w0 .. w0 + width - 1 from the
+    recurrence t_s = (t_{s-1} >> 1) | bit_{s-1} << 63 (t_s = (m mod 2^s) << (64 - s), s = k - eb),
+    key = t > 2^63 ? 2^64 - t : t; valid while s <= 64 (keys past the element's msb + 2 may lose
+    bits below 2^-64 and then stay at or below |x|)."""
+    m, e = np.frexp(abs(x))
+    mi = int(np.ldexp(m, 53))
+    eb = int(e) - 53
+    M64 = (1 << 64) - 1
+    s0 = w0 - eb
+    t = (mi << (64 - s0)) & M64 if s0 >= 1 else 0
+    out = []
+    for kk in range(width):
+        s = s0 + kk
+        if kk > 0:
+            bi = s - 1
+            bit = (mi >> bi) & 1 if 0 <= bi < 64 else 0
+            t = (t >> 1) | (bit << 63)
+        out.append(((1 << 64) - t) & M64 if t > (1 << 63) else t)
+    return out
+
+
+def test_window_key_recurrence_matches_cmod():
+    """Inside [lowest set bit + 1, msb + 2] the recurrence gives the exact scaled residue; below it
+    gives 0 and above it at most |x| 2^(64-k): what the table's prefix max tolerates."""
+    rng = np.random.default_rng(7)
+    xs = list(rng.standard_normal(300) * np.exp2(rng.integers(-80, 80, 300)))
+    xs += [0.75, 2.0 ** -30, 3 * 2.0 ** -40, np.ldexp(np.float64(2**52 + 2**51 + 1), -70),
+           np.ldexp(1.0, -1074) * 12345, 2.0 ** 1000 * 1.5]
+    for x in xs:
+        m, e = np.frexp(abs(x))
+        mi = int(np.ldexp(m, 53))
+        eb = int(e) - 53
+        tz = (mi & -mi).bit_length() - 1
+        lo, hi = eb + tz + 1, eb + mi.bit_length() + 1
+        for w0 in (lo - 5, lo, lo + 3):
+            if w0 > lo:
+                continue  # the kernel's window always starts at or below every element's lo
+            keys = window_keys(x, w0, 64)
+            for kk, key in enumerate(keys):
+                k = w0 + kk
+                exact = abs(float(cmod(np.array([x]), k)[0]))
+                got = np.ldexp(float(key), k - 64) if key else 0.0
+                if k < lo:
+                    assert key == 0, (x, k)
+                elif k <= hi:
+                    assert got == exact, (x, k, got, exact)
+                else:
+                    assert got <= abs(x), (x, k)
