@@ -683,18 +683,33 @@ __device__ __forceinline__ T csrmm_row_sum(int r, const double *As, int64_t ldas
 	RegVec<NAM * NBM, T> v;
 #pragma unroll
 	for (int q = 0; q < NAM * NBM; q++) v[q] = (T)0;
-	for (int jj = b; jj < e; jj++) {
-		const int c = colind[jj];
-		double a[NAM], x[NBM];
+	// nonzeros in chunks of CH: the chunk's column indices, then its A and x loads, all in flight
+	// together, then the fma chains in row order (the products of one pair stay in sequence)
+	constexpr int CH = (NAM * NBM <= 16) ? 4 : 2;
+	for (int j0 = b; j0 < e; j0 += CH) {
+		int c[CH];
 #pragma unroll
-		for (int ia = 0; ia < NAM; ia++) a[ia] = (ia < nA) ? As[(int64_t)ia * ldas + jj] : 0.0;
+		for (int q = 0; q < CH; q++) c[q] = (j0 + q < e) ? colind[j0 + q] : 0;
+		double a[CH][NAM], x[CH][NBM];
 #pragma unroll
-		for (int ib = 0; ib < NBM; ib++) x[ib] = (ib < nB) ? Xs[(int64_t)ib * ldxs + c] : 0.0;
+		for (int q = 0; q < CH; q++) {
+			const bool in = j0 + q < e;
 #pragma unroll
-		for (int ia = 0; ia < NAM; ia++)
+			for (int ia = 0; ia < NAM; ia++)
+				a[q][ia] = (in && ia < nA) ? As[(int64_t)ia * ldas + j0 + q] : 0.0;
 #pragma unroll
 			for (int ib = 0; ib < NBM; ib++)
-				v[ia * NBM + ib] = (T)fma(a[ia], x[ib], (double)v[ia * NBM + ib]);
+				x[q][ib] = (in && ib < nB) ? Xs[(int64_t)ib * ldxs + c[q]] : 0.0;
+		}
+#pragma unroll
+		for (int q = 0; q < CH; q++) {
+			if (j0 + q >= e) break;
+#pragma unroll
+			for (int ia = 0; ia < NAM; ia++)
+#pragma unroll
+				for (int ib = 0; ib < NBM; ib++)
+					v[ia * NBM + ib] = (T)fma(a[q][ia], x[q][ib], (double)v[ia * NBM + ib]);
+		}
 	}
 	T t;
 	if (fsum) {
@@ -750,8 +765,15 @@ template <class T>
 static void csrmm_sum(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
                       const int *colind, const int *rowptr, const double *Xs, int64_t ldxs, int nB,
                       int maxlevel, int fsum, T alpha, T beta, T *y, const uint16_t *J) {
+	// register bounds as tight as the counts allow (the per-row sum works on NAM x NBM values)
 	const dim3 g((m + 255) / 256), b(256);
-	if (nA <= 4 && nB <= 4)
+	if (nA <= 2 && nB <= 4)
+		hipLaunchKernelGGL((k_csrmm_sum<2, 4, T>), g, b, 0, st, m, As, ldas, nA, colind, rowptr, Xs,
+		                   ldxs, nB, maxlevel, fsum, alpha, beta, y, J);
+	else if (nA <= 2)
+		hipLaunchKernelGGL((k_csrmm_sum<2, 8, T>), g, b, 0, st, m, As, ldas, nA, colind, rowptr, Xs,
+		                   ldxs, nB, maxlevel, fsum, alpha, beta, y, J);
+	else if (nB <= 4)
 		hipLaunchKernelGGL((k_csrmm_sum<4, 4, T>), g, b, 0, st, m, As, ldas, nA, colind, rowptr, Xs,
 		                   ldxs, nB, maxlevel, fsum, alpha, beta, y, J);
 	else
@@ -1362,8 +1384,12 @@ void launch_csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas,
                           const int *infoX, int fsum, double *y, double *Cs, int64_t ldc,
                           const uint16_t *J, CgDev *S, int it) {
 	constexpr int kNbReg = 8;
-	hipLaunchKernelGGL((k_csrmm_sum_dev<4, kNbReg>), dim3((m + 255) / 256), dim3(256), 0, st, m, As,
-	                   ldas, nA, colind, rowptr, Xs, ldxs, infoX, fsum, y, J, S, it);
+	if (nA <= 2)  // the matrix's count is known here; p's count only on the device
+		hipLaunchKernelGGL((k_csrmm_sum_dev<2, kNbReg>), dim3((m + 255) / 256), dim3(256), 0, st, m, As,
+		                   ldas, nA, colind, rowptr, Xs, ldxs, infoX, fsum, y, J, S, it);
+	else
+		hipLaunchKernelGGL((k_csrmm_sum_dev<4, kNbReg>), dim3((m + 255) / 256), dim3(256), 0, st, m, As,
+		                   ldas, nA, colind, rowptr, Xs, ldxs, infoX, fsum, y, J, S, it);
 	hipLaunchKernelGGL(k_csrmm_big_dev, dim3((m + 255) / 256), dim3(256), 0, st, m, As, ldas, nA,
 	                   colind, rowptr, Xs, ldxs, infoX, kNbReg, fsum, y, Cs, ldc, J, S, it);
 }
