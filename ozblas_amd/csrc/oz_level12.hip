@@ -184,13 +184,14 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned
 	}
 	__syncthreads();
 	const int r0 = kminB, r1 = kmaxB;
-	// pass 2: key(k) = |cmod(x, 2^k)| / 2^k * 2^64 -- m modulo 2^s centred, s = k - eb, exact
-	// (< 2^63, at most 53 significant bits) -- maximised over the workgroup for every k.
+	// pass 2: key(k) = |cmod(x, 2^k)| / 2^k, a binary64 in (0, 1/2] kept as its bits (positive
+	// binary64 values order like their bit patterns), maximised over the workgroup for every k.
 	if (r0 <= r1 && r1 - r0 < kVsW && n <= kVsEC * stride) {
 		// narrow range, every element in registers: the keys of the window k = r0 .. r0 + kVsW - 1
-		// in registers too, t_s = (m mod 2^s) << (64 - s) built up one bit per k and
-		// key = min(t, 2^64 - t); below an element's lowest bit t is 0, above its msb + 2 the key is
-		// at most |x| (the table's prefix max is unchanged by it)
+		// in registers too, as the integer key * 2^64: t_s = (m mod 2^s) << (64 - s) built up one
+		// bit per k and min(t, 2^64 - t) (at most 53 significant bits); below an element's lowest
+		// bit t is 0, above its msb + 2 the key is at most |x| (the table's prefix max is unchanged
+		// by it)
 		uint64_t Fr[kVsW];
 #pragma unroll
 		for (int kk = 0; kk < kVsW; kk++) Fr[kk] = 0ull;
@@ -203,11 +204,15 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned
 				const uint64_t w = __shfl_xor(v, o, 64);
 				v = w > v ? w : v;
 			}
-			if (lane == 0 && v) atomicMax(&Fb[r0 + kk + kVsOff], (unsigned long long)v);
+			// table key: the bits of the binary64 key / 2^64 (exact: at most 53 significant bits)
+			if (lane == 0 && v)
+				atomicMax(&Fb[r0 + kk + kVsOff], (unsigned long long)d2u(ldexp((double)v, -64)));
 		}
 	} else if (r0 <= r1) {
 		// wide range: the keys of k in [lo, hi] of each element into the table directly; each lane
-		// starts at its own offset so one LDS atomic instruction hits different table entries
+		// starts at its own offset so one LDS atomic instruction hits different table entries.
+		// y = x / 2^k has |y| in [1/4, 2^53) for those k (exact), and |y - rint(y)| is exactly
+		// |cmod(x, 2^k)| / 2^k in (0, 1/2]: its bits (a positive binary64) order like the value.
 		auto add = [&](double v) {
 			int dummy = 0;
 			const VsElem q = vs_decompose(v, dummy);
@@ -220,11 +225,9 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned
 					if (t >= K) t %= K;
 				}
 				const int k = q.lo + t;
-				const int sft = k - q.eb;  // 1 .. 54
-				uint64_t r = q.m & ((uint64_t(1) << sft) - 1);
-				const uint64_t half = uint64_t(1) << (sft - 1);
-				if (r > half || (r == half && ((q.m >> sft) & 1))) r = (uint64_t(1) << sft) - r;
-				if (r) atomicMax(&Fb[k + kVsOff], r << (64 - sft));
+				const double y = ldexp(v, -k);
+				const uint64_t key = d2u(y - rint(y)) & ~(uint64_t(1) << 63);
+				if (key) atomicMax(&Fb[k + kVsOff], (unsigned long long)key);
 			}
 		};
 #pragma unroll
@@ -266,8 +269,8 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned
 	__syncthreads();
 	const int K0 = kminB, K1 = kmaxB;  // empty ([INT32_MAX, INT32_MIN]) for an all-zero vector
 	for (int k = (K0 <= K1 ? K0 + (int)threadIdx.x : 1); k <= K1; k += BS) {
-		// F(k) = key 2^(k-64) (key < 2^63 has at most 53 significant bits: exact)
-		Gb[k + kVsOff] = d2u(ldexp((double)Fg[k + kVsOff], k - 64));
+		// F(k) = key 2^k (a residue of a binary64: exact)
+		Gb[k + kVsOff] = d2u(ldexp(__builtin_bit_cast(double, (uint64_t)Fg[k + kVsOff]), k));
 		Fg[k + kVsOff] = 0ull;  // clear for the next split
 	}
 	__syncthreads();
@@ -454,6 +457,8 @@ hipError_t launch_vsplit(hipStream_t st, int n, const float *x, double *slices, 
 // product unit below 2^53, so block partials added with FP64 atomics are exact in any order.
 // ---------------------------------------------------------------------------------------------
 constexpr int kDotChunk = 16;
+constexpr int kDotPairs = 64;    // pair slots per workgroup partial of k_dots_reg
+constexpr int kDotMaxWG = 256;  // workgroups of k_dots_reg at most
 
 template <int BS>
 __global__ __launch_bounds__(BS) void k_slice_dots(int n, const double *xs, int64_t ldx, int nA,
@@ -1215,19 +1220,48 @@ __global__ __launch_bounds__(256) void k_csrmm_big_dev(int m, const double *As, 
 // then clears vals, the counter and the split maxima it consumed for the next use.
 // With f32 (binary32 DOT; mode 0 only) the pair dots round to float first, the sum is the float
 // one (J is then JF2) and C is a float: C[0] = fma1<float>(1, t, 0 * C[0]).
-__device__ void dots_finish(int nA, int nB, double *vals, unsigned *ctr, unsigned long long *maxvX,
-                            unsigned long long *maxvY, int nmax, const uint16_t *J, int fsum,
-                            CgDev *S, int it, int mode, void *C, int f32) {
+// With `part` (k_dots_reg) the pair dots are first summed from the per-workgroup partials
+// part[w * kDotPairs + p] by the whole last workgroup (exact in any order, like the atomics of
+// k_dots_dev into vals).
+__device__ void dots_finish(int nA, int nB, double *vals, const double *part, unsigned *ctr,
+                            unsigned long long *maxvX, unsigned long long *maxvY, int nmax,
+                            const uint16_t *J, int fsum, CgDev *S, int it, int mode, void *C,
+                            int f32) {
 	__shared__ int isLast;
+	__shared__ double psum[1][kDotPairs];
 	__threadfence();
 	if (threadIdx.x == 0) isLast = (atomicAdd(ctr, 1u) == gridDim.x - 1) ? 1 : 0;
 	__syncthreads();
-	if (!isLast || threadIdx.x != 0) return;
+	if (!isLast) return;
 	__threadfence();
 	const int npairs = nA * nB;  // vals: plain loads, ordered after the other workgroups by the fence
+	double *src = vals;
+	if (part) {
+		// entry k = w * npairs + p of the partials, spread over the threads, loads issued in groups
+		// of 8 before their LDS additions
+		if (threadIdx.x < kDotPairs) psum[0][threadIdx.x] = 0.0;
+		__syncthreads();
+		const int total = (int)gridDim.x * npairs;
+		for (int k0 = threadIdx.x; k0 < total; k0 += 8 * (int)blockDim.x) {
+			double v[8];
+#pragma unroll
+			for (int u = 0; u < 8; u++) {
+				const int k = k0 + u * (int)blockDim.x;
+				v[u] = (k < total) ? part[(int64_t)(k / npairs) * kDotPairs + k % npairs] : 0.0;
+			}
+#pragma unroll
+			for (int u = 0; u < 8; u++) {
+				const int k = k0 + u * (int)blockDim.x;
+				if (k < total && v[u] != 0.0) atomicAdd(&psum[0][k % npairs], v[u]);
+			}
+		}
+		__syncthreads();
+		src = psum[0];
+	}
+	if (threadIdx.x != 0) return;
 	const int maxlevel = (nA - 1) + (nB - 1);  // DOT forces fastMode = 0 (src/cpu/ozblas_gemm.cpp:25-28)
 	if (f32) {
-		float *fv = vals_to_float(npairs, vals);
+		float *fv = vals_to_float(npairs, src);
 		const float t = dot_pairs_sum_f(npairs, fv, nA, nB, maxlevel, fsum, J);
 		float *Cf = (float *)C;
 		Cf[0] = fma1_f(1.f, t, 0.f * Cf[0]);
@@ -1239,7 +1273,7 @@ __device__ void dots_finish(int nA, int nB, double *vals, unsigned *ctr, unsigne
 		*ctr = 0u;
 		return;
 	}
-	const double t = dot_pairs_sum(npairs, vals, nA, nB, maxlevel, fsum, J);
+	const double t = dot_pairs_sum(npairs, src, nA, nB, maxlevel, fsum, J);
 	const double r = fma(1.0, t, 0.0 * 0.0);
 	if (mode == 0) {
 		double *Cd = (double *)C;
@@ -1272,17 +1306,19 @@ __device__ void dots_finish(int nA, int nB, double *vals, unsigned *ctr, unsigne
 }
 
 // DOT of two split vectors with device slice counts, slice counts up to NAM x NBM: every element's
-// slices are read once and all pair products accumulate in registers; the exact pair partials of
-// a workgroup are added atomically (integer multiples of the product unit below 2^53: any order
-// is exact).  Larger counts are left to k_dots_dev.
+// slices are read once and all pair products accumulate in registers.  Each workgroup stores its pair partials to
+// part[blockIdx.x * kDotPairs + p] -- no atomics on a few shared addresses -- and the last one
+// sums them (integer multiples of the product unit below 2^53: any order is exact).  Larger
+// counts are left to k_dots_dev.
 template <int BS, int NAM, int NBM>
 __global__ __launch_bounds__(BS) void k_dots_reg(int n, const double *xs, int64_t ldx,
                                                  const int *infoX, const double *ys, int64_t ldy,
-                                                 const int *infoY, double *vals, unsigned *ctr,
-                                                 unsigned long long *maxvX,
+                                                 const int *infoY, double *vals, double *part,
+                                                 unsigned *ctr, unsigned long long *maxvX,
                                                  unsigned long long *maxvY, int nmax,
                                                  const uint16_t *J, int fsum, CgDev *S, int it,
                                                  int mode, void *C, int f32) {
+	static_assert(NAM * NBM <= kDotPairs && BS % 64 == 0 && BS <= 256, "pair partial layout");
 	if (cg_skip(S, it)) return;
 	const int nA = infoX[0], nB = infoY[0];
 	if (nA > NAM || nB > NBM) return;
@@ -1302,11 +1338,25 @@ __global__ __launch_bounds__(BS) void k_dots_reg(int n, const double *xs, int64_
 			for (int b = 0; b < NBM; b++) acc[a * NBM + b] = fma(xa[a], yb[b], acc[a * NBM + b]);
 	}
 	const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+	// wave sums: one butterfly step over 16 pairs at a time (independent lane exchanges in flight
+	// together), skipping the rows a >= nA; absent pairs hold zeros and are not read
+	constexpr int QC = 16;
+	static_assert((NAM * NBM) % QC == 0, "pair chunks");
 #pragma unroll
-	for (int q = 0; q < NAM * NBM; q++) {
-		double v = acc[q];
-		for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-		if (l == 0) red[w][q] = v;
+	for (int q0 = 0; q0 < NAM * NBM; q0 += QC) {
+		if (q0 / NBM >= nA) continue;
+#pragma unroll
+		for (int o = 32; o > 0; o >>= 1) {
+			double t[QC];
+#pragma unroll
+			for (int q = 0; q < QC; q++) t[q] = __shfl_xor(acc[q0 + q], o, 64);
+#pragma unroll
+			for (int q = 0; q < QC; q++) acc[q0 + q] += t[q];
+		}
+		if (l == 0) {
+#pragma unroll
+			for (int q = 0; q < QC; q++) red[w][q0 + q] = acc[q0 + q];
+		}
 	}
 	__syncthreads();
 	for (int q = threadIdx.x; q < NAM * NBM; q += BS) {
@@ -1314,10 +1364,10 @@ __global__ __launch_bounds__(BS) void k_dots_reg(int n, const double *xs, int64_
 		if (a < nA && b < nB) {
 			double v = red[0][q];
 			for (int k = 1; k < BS / 64; k++) v += red[k][q];
-			if (v != 0.0) atomicAdd(&vals[a + nA * b], v);
+			part[(int64_t)blockIdx.x * kDotPairs + a + nA * b] = v;
 		}
 	}
-	dots_finish(nA, nB, vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
+	dots_finish(nA, nB, vals, part, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
 }
 
 // The same for any slice counts (pairs in chunks of kDotChunk, slices re-read per chunk); does
@@ -1355,7 +1405,7 @@ __global__ __launch_bounds__(BS) void k_dots_dev(int n, const double *xs, int64_
 			if (threadIdx.x == 0 && p0 + q < npairs && v != 0.0) atomicAdd(&vals[p0 + q], v);
 		}
 	}
-	dots_finish(nA, nB, vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
+	dots_finish(nA, nB, vals, nullptr, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
 }
 
 // x = fma(alpha, p, x); r = fma(-alpha, q, r)   (src/cpu/ozblas_cg.cpp:87-88, ozblas_axpy.cpp)
@@ -1394,16 +1444,18 @@ void launch_csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas,
 	                   colind, rowptr, Xs, ldxs, infoX, kNbReg, fsum, y, Cs, ldc, J, S, it);
 }
 
+size_t dots_part_bytes() { return sizeof(double) * kDotPairs * kDotMaxWG; }
+
 void launch_dots_dev(hipStream_t st, int n, const double *xs, int64_t ldx, const int *infoX,
-                     const double *ys, int64_t ldy, const int *infoY, double *vals, unsigned *ctr,
-                     unsigned long long *maxvX, unsigned long long *maxvY, int nmax,
+                     const double *ys, int64_t ldy, const int *infoY, double *vals, double *part,
+                     unsigned *ctr, unsigned long long *maxvX, unsigned long long *maxvY, int nmax,
                      const uint16_t *J, int fsum, CgDev *S, int it, int mode, void *C, int f32) {
 	constexpr int kRA = 8, kRB = 8;
 	int gx = (n + 255) / 256;
-	if (gx > 512) gx = 512;
+	if (gx > kDotMaxWG) gx = kDotMaxWG;
 	if (gx < 1) gx = 1;
 	hipLaunchKernelGGL((k_dots_reg<256, kRA, kRB>), dim3(gx), dim3(256), 0, st, n, xs, ldx, infoX, ys,
-	                   ldy, infoY, vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
+	                   ldy, infoY, vals, part, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
 	hipLaunchKernelGGL(k_dots_dev<256>, dim3(gx), dim3(256), 0, st, n, xs, ldx, infoX, ys, ldy, infoY,
 	                   vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32, kRA, kRB);
 }
