@@ -58,24 +58,44 @@ __device__ inline int block_max_i(int v, int *red) {
 	return r;
 }
 
+// Global reductions of the column statistics, accumulated by thread 0 of a workgroup over its
+// columns and folded into ExpStats once (few atomics on the shared counters).
+struct ColAcc {
+	int minLsb = INT32_MAX, maxSpan = INT32_MIN, maxS = 0, maxE = INT32_MIN, bad = 0;
+};
+
+__device__ inline void col_acc_flush(const ColAcc &a, ExpStats *st) {
+	if (a.minLsb != INT32_MAX) atomicMin(&st->minLsb, a.minLsb);
+	if (a.maxSpan != INT32_MIN) atomicMax(&st->maxSpan, a.maxSpan);
+	atomicMax(&st->maxS, a.maxS);
+	if (a.maxE != INT32_MIN) atomicMax(&st->maxE, a.maxE);
+	if (a.bad) atomicOr(&st->nonfinite, 1);
+}
+
 // Digit statistics of one k-contiguous column x[0..m) (whole workgroup of BS threads):
 // the column scale E (|x| <= 127 * 2^(E-8) for every element) and digit count S (every element is
-// a multiple of 2^(E - 8S)); global reductions go to *st.  Column j of the operand is
-// x = sum_{s<S} d_s 2^(E - 8(s+1)) with d_s in [-128, 127].
+// a multiple of 2^(E - 8S)); thread 0 adds the column to `acc`.  Column j of the operand is
+// x = sum_{s<S} d_s 2^(E - 8(s+1)) with d_s in [-128, 127].  Eight loads per thread are issued
+// before they are used.
 template <int BS>
-__device__ inline void digit_col_stats(const double *x, int m, int *colE, int *colS, ExpStats *st,
-                                       int *red, double *redd) {
+__device__ inline void digit_col_stats_acc(const double *x, int m, int *colE, int *colS, ColAcc &acc,
+                                           int *red, double *redd) {
 	int nlo = INT32_MIN, bad = 0;  // nlo = -(min lsb)
 	double mx = 0.0;
-	for (int i = threadIdx.x; i < m; i += BS) {
-		const double v = x[i];
-		if (!isfinite(v)) {
-			bad = 1;
-			continue;
+	for (int i0 = threadIdx.x; i0 < m; i0 += 8 * BS) {
+		double v[8];
+#pragma unroll
+		for (int u = 0; u < 8; u++) v[u] = (i0 + u * BS < m) ? x[i0 + u * BS] : 0.0;
+#pragma unroll
+		for (int u = 0; u < 8; u++) {
+			if (!isfinite(v[u])) {
+				bad = 1;
+				continue;
+			}
+			if (v[u] == 0.0) continue;
+			mx = std_max(mx, fabs(v[u]));
+			nlo = max(nlo, -lsb_exp(v[u]));
 		}
-		if (v == 0.0) continue;
-		mx = std_max(mx, fabs(v));
-		nlo = max(nlo, -lsb_exp(v));
 	}
 	mx = block_max<BS>(mx, redd);
 	nlo = block_max_i<BS>(nlo, red);
@@ -89,15 +109,24 @@ __device__ inline void digit_col_stats(const double *x, int m, int *colE, int *c
 			E = (mx <= ldexp(127.0, hi - 7)) ? hi + 1 : hi + 2;
 			const int lo = -nlo;
 			S = (E - lo + 7) / 8;  // 8S >= E - lo
-			atomicMin(&st->minLsb, lo);
-			atomicMax(&st->maxSpan, hi - lo);
+			acc.minLsb = min(acc.minLsb, lo);
+			acc.maxSpan = max(acc.maxSpan, hi - lo);
 		}
 		*colE = E;
 		*colS = S;
-		atomicMax(&st->maxS, S);
-		atomicMax(&st->maxE, E);
-		if (bad) atomicOr(&st->nonfinite, 1);
+		acc.maxS = max(acc.maxS, S);
+		acc.maxE = max(acc.maxE, E);
+		if (bad) acc.bad = 1;
 	}
+}
+
+// the same for one column with its reductions folded into *st at once
+template <int BS>
+__device__ inline void digit_col_stats(const double *x, int m, int *colE, int *colS, ExpStats *st,
+                                       int *red, double *redd) {
+	ColAcc acc;
+	digit_col_stats_acc<BS>(x, m, colE, colS, acc, red, redd);
+	if (threadIdx.x == 0) col_acc_flush(acc, st);
 }
 
 template <typename T>

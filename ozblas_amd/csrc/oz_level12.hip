@@ -87,23 +87,19 @@ __device__ inline VsElem vs_decompose(double v, int &bad) {
 	return r;
 }
 
-// max-accumulates the keys of x for k = w0 .. w0 + kVsW - 1 into Fr (w0 at or below the element's
-// lowest set bit + 1)
-__device__ __forceinline__ void vs_window_keys(double v, int w0, uint64_t (&Fr)[kVsW]) {
+// max-accumulates the keys of x for k = w0 .. w0 + kVsW - 1 into Fr: with y = x / 2^k,
+// |y - rint(y)| = |cmod(x, 2^k)| / 2^k (zero below the element's lowest bit, |x| / 2^k above its
+// msb + 1).  Every element of a window is within kVsW binades of w0, so y stays a normal binary64
+// below 2^(kVsW + 2) and each halving is exact.
+__device__ __forceinline__ void vs_window_keys(double v, int w0, double (&Fr)[kVsW]) {
 	int dummy = 0;
 	const VsElem q = vs_decompose(v, dummy);
 	if (q.m == 0) return;
-	const int s0 = w0 - q.eb;  // <= 53
-	uint64_t t = (s0 >= 1) ? (q.m << (64 - s0)) : 0ull;
+	double y = ldexp(v, -w0);
 #pragma unroll
 	for (int kk = 0; kk < kVsW; kk++) {
-		if (kk > 0) {
-			const int bi = s0 + kk - 1;
-			const uint64_t bit = (bi >= 0 && bi < 64) ? ((q.m >> bi) & 1ull) : 0ull;
-			t = (t >> 1) | (bit << 63);
-		}
-		const uint64_t key = (t > (uint64_t(1) << 63)) ? (0ull - t) : t;
-		Fr[kk] = key > Fr[kk] ? key : Fr[kk];
+		Fr[kk] = fmax(Fr[kk], fabs(y - rint(y)));
+		y *= 0.5;
 	}
 }
 
@@ -188,25 +184,29 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned
 	// binary64 values order like their bit patterns), maximised over the workgroup for every k.
 	if (r0 <= r1 && r1 - r0 < kVsW && n <= kVsEC * stride) {
 		// narrow range, every element in registers: the keys of the window k = r0 .. r0 + kVsW - 1
-		// in registers too, as the integer key * 2^64: t_s = (m mod 2^s) << (64 - s) built up one
-		// bit per k and min(t, 2^64 - t) (at most 53 significant bits); below an element's lowest
-		// bit t is 0, above its msb + 2 the key is at most |x| (the table's prefix max is unchanged
-		// by it)
-		uint64_t Fr[kVsW];
+		// in registers too, then a butterfly max over the wave, 16 window entries per step
+		double Fr[kVsW];
 #pragma unroll
-		for (int kk = 0; kk < kVsW; kk++) Fr[kk] = 0ull;
+		for (int kk = 0; kk < kVsW; kk++) Fr[kk] = 0.0;
 #pragma unroll
 		for (int e = 0; e < kVsEC; e++) vs_window_keys(xc[e], r0, Fr);
+		constexpr int KC = 16;
 #pragma unroll
-		for (int kk = 0; kk < kVsW; kk++) {
-			uint64_t v = Fr[kk];
+		for (int k0 = 0; k0 < kVsW; k0 += KC) {
+#pragma unroll
 			for (int o = 32; o > 0; o >>= 1) {
-				const uint64_t w = __shfl_xor(v, o, 64);
-				v = w > v ? w : v;
+				double t[KC];
+#pragma unroll
+				for (int kk = 0; kk < KC; kk++) t[kk] = __shfl_xor(Fr[k0 + kk], o, 64);
+#pragma unroll
+				for (int kk = 0; kk < KC; kk++) Fr[k0 + kk] = fmax(Fr[k0 + kk], t[kk]);
 			}
-			// table key: the bits of the binary64 key / 2^64 (exact: at most 53 significant bits)
-			if (lane == 0 && v)
-				atomicMax(&Fb[r0 + kk + kVsOff], (unsigned long long)d2u(ldexp((double)v, -64)));
+			if (lane == 0) {
+#pragma unroll
+				for (int kk = 0; kk < KC; kk++)
+					if (Fr[k0 + kk] > 0.0)
+						atomicMax(&Fb[r0 + k0 + kk + kVsOff], (unsigned long long)d2u(Fr[k0 + kk]));
+			}
 		}
 	} else if (r0 <= r1) {
 		// wide range: the keys of k in [lo, hi] of each element into the table directly; each lane
@@ -458,7 +458,7 @@ hipError_t launch_vsplit(hipStream_t st, int n, const float *x, double *slices, 
 // ---------------------------------------------------------------------------------------------
 constexpr int kDotChunk = 16;
 constexpr int kDotPairs = 64;    // pair slots per workgroup partial of k_dots_reg
-constexpr int kDotMaxWG = 256;  // workgroups of k_dots_reg at most
+constexpr int kDotMaxWG = 256;  // workgroups of k_dots_reg at most (one wave per SIMD)
 
 template <int BS>
 __global__ __launch_bounds__(BS) void k_slice_dots(int n, const double *xs, int64_t ldx, int nA,
@@ -1306,7 +1306,8 @@ __device__ void dots_finish(int nA, int nB, double *vals, const double *part, un
 }
 
 // DOT of two split vectors with device slice counts, slice counts up to NAM x NBM: every element's
-// slices are read once and all pair products accumulate in registers.  Each workgroup stores its pair partials to
+// slices are read once (two elements per thread and step, their loads issued together) and all
+// pair products accumulate in registers.  Each workgroup stores its pair partials to
 // part[blockIdx.x * kDotPairs + p] -- no atomics on a few shared addresses -- and the last one
 // sums them (integer multiples of the product unit below 2^53: any order is exact).  Larger
 // counts are left to k_dots_dev.
@@ -1326,16 +1327,28 @@ __global__ __launch_bounds__(BS) void k_dots_reg(int n, const double *xs, int64_
 	double acc[NAM * NBM];
 #pragma unroll
 	for (int q = 0; q < NAM * NBM; q++) acc[q] = 0.0;
-	for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
-		double xa[NAM], yb[NBM];
+	const int64_t stride = (int64_t)gridDim.x * BS;
+	for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += 2 * stride) {
+		const int64_t i1 = i + stride;
+		const bool two = i1 < n;
+		double xa[2][NAM], yb[2][NBM];
 #pragma unroll
-		for (int a = 0; a < NAM; a++) xa[a] = (a < nA) ? xs[(int64_t)a * ldx + i] : 0.0;
+		for (int a = 0; a < NAM; a++) {
+			xa[0][a] = (a < nA) ? xs[(int64_t)a * ldx + i] : 0.0;
+			xa[1][a] = (a < nA && two) ? xs[(int64_t)a * ldx + i1] : 0.0;
+		}
 #pragma unroll
-		for (int b = 0; b < NBM; b++) yb[b] = (b < nB) ? ys[(int64_t)b * ldy + i] : 0.0;
+		for (int b = 0; b < NBM; b++) {
+			yb[0][b] = (b < nB) ? ys[(int64_t)b * ldy + i] : 0.0;
+			yb[1][b] = (b < nB && two) ? ys[(int64_t)b * ldy + i1] : 0.0;
+		}
 #pragma unroll
-		for (int a = 0; a < NAM; a++)
+		for (int e = 0; e < 2; e++)
 #pragma unroll
-			for (int b = 0; b < NBM; b++) acc[a * NBM + b] = fma(xa[a], yb[b], acc[a * NBM + b]);
+			for (int a = 0; a < NAM; a++)
+#pragma unroll
+				for (int b = 0; b < NBM; b++)
+					acc[a * NBM + b] = fma(xa[e][a], yb[e][b], acc[a * NBM + b]);
 	}
 	const int w = threadIdx.x / 64, l = threadIdx.x % 64;
 	// wave sums: one butterfly step over 16 pairs at a time (independent lane exchanges in flight
@@ -1451,7 +1464,7 @@ void launch_dots_dev(hipStream_t st, int n, const double *xs, int64_t ldx, const
                      unsigned *ctr, unsigned long long *maxvX, unsigned long long *maxvY, int nmax,
                      const uint16_t *J, int fsum, CgDev *S, int it, int mode, void *C, int f32) {
 	constexpr int kRA = 8, kRB = 8;
-	int gx = (n + 255) / 256;
+	int gx = (n + 511) / 512;  // two elements per thread and step
 	if (gx > kDotMaxWG) gx = kDotMaxWG;
 	if (gx < 1) gx = 1;
 	hipLaunchKernelGGL((k_dots_reg<256, kRA, kRB>), dim3(gx), dim3(256), 0, st, n, xs, ldx, infoX, ys,

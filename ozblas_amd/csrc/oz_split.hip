@@ -273,8 +273,10 @@ __global__ __launch_bounds__(BS) void k_digit_stats(int m, int n, const double *
                                                     int *colE, int *colS, ExpStats *st) {
 	__shared__ int red[BS / 64];
 	__shared__ double redd[BS / 64];
+	ColAcc acc;
 	for (int j = blockIdx.x; j < n; j += gridDim.x)
-		digit_col_stats<BS>(in + (int64_t)j * ldi, m, colE + j, colS + j, st, red, redd);
+		digit_col_stats_acc<BS>(in + (int64_t)j * ldi, m, colE + j, colS + j, acc, red, redd);
+	if (threadIdx.x == 0) col_acc_flush(acc, st);
 }
 
 // digit plane writer for 2-byte digits: plane s of column j at planes + s*pstride + j*ldp
@@ -434,7 +436,7 @@ void launch_zero_slots_csr(hipStream_t st, int m, const int *rowptr, double *spl
 
 void launch_digit_stats(hipStream_t st, int m, int n, const double *in, int64_t ldi, int *colE,
                         int *colS, void *stats) {
-	const int grid = n < 65535 ? n : 65535;
+	const int grid = n < 1024 ? n : 1024;  // columns per workgroup share its few counter atomics
 	hipLaunchKernelGGL(k_digit_stats<256>, dim3(grid), dim3(256), 0, st, m, n, in, ldi, colE, colS,
 	                   (ExpStats *)stats);
 }
