@@ -48,7 +48,7 @@ __device__ inline double block_sum_d(double v, double *red) {
 // slices slice_s = cmod(x, u_{s-1}) - cmod(x, u_s) (exact).  Inputs outside the model (non-finite,
 // |x| >= 2^1022, rho < 3 or above the precision, a sigma outside the normal range, a non
 // power-of-two shift, more levels than VsPlan holds) set plan.bad and are split by
-// k_vsplit_fallback instead, which runs the reference loop in one workgroup.  The binary32 split
+// workgroup 0 of k_vsplit_write instead, which runs the reference loop.  The binary32 split
 // is the same with 24 for 53 (unit 2^(rho+tau-24), float sigma in its normal range).
 // ---------------------------------------------------------------------------------------------
 constexpr int kVsOff = 1100;          // F/G tables cover k in [-1100, 1099]
@@ -330,38 +330,11 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned
 	*ctr = 0u;
 }
 
-// slices from the plan: slice_s = cmod(x, u_{s-1}) - cmod(x, u_s), remainder slot, zero slots
-template <class TI>
-__global__ __launch_bounds__(256) void k_vsplit_write(int n, const TI *x, double *slices,
-                                                      int64_t lds, const VsPlan *plan,
-                                                      const int *stopAt, int it) {
-	if (stopAt && it >= *stopAt) return;
-	if (plan->bad) return;
-	const int last = plan->last, count = plan->count, zero = plan->zero;
-	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-	     i += (int64_t)gridDim.x * blockDim.x) {
-		const double v = (double)x[i];
-		double r = v;
-		for (int s = 0; s <= last; s++) {
-			const double c = vs_cmod(v, plan->k[s]);
-			slices[(int64_t)s * lds + i] = r - c;
-			r = c;
-		}
-		slices[(int64_t)(last + 1) * lds + i] = zero ? 0.0 : r;
-		for (int q = last + 2; q < count; q++) slices[(int64_t)q * lds + i] = 0.0;
-	}
-}
-
-// the reference loop in one workgroup (inputs outside the model above): slice s at slot s, its
+// the reference loop in one workgroup of BS threads (inputs outside the model above): slice s at slot s, its
 // remainder at slot s + 1 (read back as the next level's input), as src/cpu/ozblas_split.cpp:96
 template <int BS, class TI>
-__global__ __launch_bounds__(BS) void k_vsplit_fallback(int n, const TI *x, double *slices,
-                                                        int64_t lds, int rho, int shift, int cap,
-                                                        int reproMode, const uint16_t *J,
-                                                        const VsPlan *plan, int *info,
-                                                        const int *stopAt, int it) {
-	if (stopAt && it >= *stopAt) return;
-	if (!plan->bad) return;
+__device__ void vsplit_fallback(int n, const TI *x, double *slices, int64_t lds, int rho, int shift,
+                                int cap, int reproMode, const uint16_t *J, int *info) {
 	__shared__ double red[BS / 64];
 	double mx = 0.0;
 	for (int64_t i = threadIdx.x; i < n; i += BS) mx = std_max(mx, fabs((double)x[i]));
@@ -405,6 +378,35 @@ __global__ __launch_bounds__(BS) void k_vsplit_fallback(int n, const TI *x, doub
 	}
 }
 
+
+// slices from the plan: slice_s = cmod(x, u_{s-1}) - cmod(x, u_s), remainder slot, zero slots;
+// a plan marked bad is split by workgroup 0 with the reference loop instead
+template <class TI>
+__global__ __launch_bounds__(256) void k_vsplit_write(int n, const TI *x, double *slices,
+                                                      int64_t lds, const VsPlan *plan, int rho,
+                                                      int shift, int cap, int reproMode,
+                                                      const uint16_t *J, int *info,
+                                                      const int *stopAt, int it) {
+	if (stopAt && it >= *stopAt) return;
+	if (plan->bad) {
+		if (blockIdx.x == 0) vsplit_fallback<256, TI>(n, x, slices, lds, rho, shift, cap, reproMode, J, info);
+		return;
+	}
+	const int last = plan->last, count = plan->count, zero = plan->zero;
+	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+	     i += (int64_t)gridDim.x * blockDim.x) {
+		const double v = (double)x[i];
+		double r = v;
+		for (int s = 0; s <= last; s++) {
+			const double c = vs_cmod(v, plan->k[s]);
+			slices[(int64_t)s * lds + i] = r - c;
+			r = c;
+		}
+		slices[(int64_t)(last + 1) * lds + i] = zero ? 0.0 : r;
+		for (int q = last + 2; q < count; q++) slices[(int64_t)q * lds + i] = 0.0;
+	}
+}
+
 size_t vsplit_scratch_bytes() { return 8ull * kVsK + 8ull * kVsBlocks + 64; }
 
 // Splits x (length n) into slices at stride lds (cap + 2 slots); the count goes to info[0] on the
@@ -429,9 +431,7 @@ static hipError_t vsplit(hipStream_t st, int n, const TI *x, double *slices, int
 	int gw = (int)std::min<int64_t>(2048, ((int64_t)n + 255) / 256);
 	if (gw < 1) gw = 1;
 	hipLaunchKernelGGL(k_vsplit_write<TI>, dim3(gw), dim3(256), 0, st, n, x, slices, lds,
-	                   (const VsPlan *)plan, stopAt, it);
-	hipLaunchKernelGGL((k_vsplit_fallback<1024, TI>), dim3(1), dim3(1024), 0, st, n, x, slices, lds,
-	                   rho, shift, cap, reproMode, J, (const VsPlan *)plan, info, stopAt, it);
+	                   (const VsPlan *)plan, rho, shift, cap, reproMode, J, info, stopAt, it);
 	return hipGetLastError();
 }
 
@@ -1222,7 +1222,7 @@ __global__ __launch_bounds__(256) void k_csrmm_big_dev(int m, const double *As, 
 // one (J is then JF2) and C is a float: C[0] = fma1<float>(1, t, 0 * C[0]).
 // With `part` (k_dots_reg) the pair dots are first summed from the per-workgroup partials
 // part[w * kDotPairs + p] by the whole last workgroup (exact in any order, like the atomics of
-// k_dots_dev into vals).
+// dots_general into vals).
 __device__ void dots_finish(int nA, int nB, double *vals, const double *part, unsigned *ctr,
                             unsigned long long *maxvX, unsigned long long *maxvY, int nmax,
                             const uint16_t *J, int fsum, CgDev *S, int it, int mode, void *C,
@@ -1305,12 +1305,44 @@ __device__ void dots_finish(int nA, int nB, double *vals, const double *part, un
 	*ctr = 0u;
 }
 
+// The same for any slice counts (pairs in chunks of kDotChunk, slices re-read per chunk, exact
+// partials added atomically to vals); k_dots_reg runs it when the counts exceed its registers.
+template <int BS>
+__device__ void dots_general(int n, int nA, int nB, const double *xs, int64_t ldx, const double *ys,
+                             int64_t ldy, double *vals, unsigned *ctr, unsigned long long *maxvX,
+                             unsigned long long *maxvY, int nmax, const uint16_t *J, int fsum,
+                             CgDev *S, int it, int mode, void *C, int f32) {
+	__shared__ double red[BS / 64];
+	const int npairs = nA * nB;
+	for (int p0 = 0; p0 < npairs; p0 += kDotChunk) {
+		double acc[kDotChunk];
+#pragma unroll
+		for (int q = 0; q < kDotChunk; q++) acc[q] = 0.0;
+		for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+#pragma unroll
+			for (int q = 0; q < kDotChunk; q++) {
+				const int p = p0 + q;
+				if (p < npairs) {
+					const int ia = p % nA, ib = p / nA;
+					acc[q] = fma(xs[(int64_t)ia * ldx + i], ys[(int64_t)ib * ldy + i], acc[q]);
+				}
+			}
+		}
+#pragma unroll
+		for (int q = 0; q < kDotChunk; q++) {
+			const double v = block_sum_d<BS>(acc[q], red);
+			if (threadIdx.x == 0 && p0 + q < npairs && v != 0.0) atomicAdd(&vals[p0 + q], v);
+		}
+	}
+	dots_finish(nA, nB, vals, nullptr, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
+}
+
 // DOT of two split vectors with device slice counts, slice counts up to NAM x NBM: every element's
 // slices are read once (two elements per thread and step, their loads issued together) and all
 // pair products accumulate in registers.  Each workgroup stores its pair partials to
 // part[blockIdx.x * kDotPairs + p] -- no atomics on a few shared addresses -- and the last one
 // sums them (integer multiples of the product unit below 2^53: any order is exact).  Larger
-// counts are left to k_dots_dev.
+// counts take dots_general.
 template <int BS, int NAM, int NBM>
 __global__ __launch_bounds__(BS) void k_dots_reg(int n, const double *xs, int64_t ldx,
                                                  const int *infoX, const double *ys, int64_t ldy,
@@ -1322,7 +1354,11 @@ __global__ __launch_bounds__(BS) void k_dots_reg(int n, const double *xs, int64_
 	static_assert(NAM * NBM <= kDotPairs && BS % 64 == 0 && BS <= 256, "pair partial layout");
 	if (cg_skip(S, it)) return;
 	const int nA = infoX[0], nB = infoY[0];
-	if (nA > NAM || nB > NBM) return;
+	if (nA > NAM || nB > NBM) {
+		dots_general<BS>(n, nA, nB, xs, ldx, ys, ldy, vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it,
+		                 mode, C, f32);
+		return;
+	}
 	__shared__ double red[BS / 64][NAM * NBM];
 	double acc[NAM * NBM];
 #pragma unroll
@@ -1383,44 +1419,6 @@ __global__ __launch_bounds__(BS) void k_dots_reg(int n, const double *xs, int64_
 	dots_finish(nA, nB, vals, part, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
 }
 
-// The same for any slice counts (pairs in chunks of kDotChunk, slices re-read per chunk); does
-// nothing when k_dots_reg<.., NAM, NBM> takes the counts.
-template <int BS>
-__global__ __launch_bounds__(BS) void k_dots_dev(int n, const double *xs, int64_t ldx,
-                                                 const int *infoX, const double *ys, int64_t ldy,
-                                                 const int *infoY, double *vals, unsigned *ctr,
-                                                 unsigned long long *maxvX,
-                                                 unsigned long long *maxvY, int nmax,
-                                                 const uint16_t *J, int fsum, CgDev *S, int it,
-                                                 int mode, void *C, int f32, int regA, int regB) {
-	if (cg_skip(S, it)) return;
-	__shared__ double red[BS / 64];
-	const int nA = infoX[0], nB = infoY[0];
-	if (nA <= regA && nB <= regB) return;
-	const int npairs = nA * nB;
-	for (int p0 = 0; p0 < npairs; p0 += kDotChunk) {
-		double acc[kDotChunk];
-#pragma unroll
-		for (int q = 0; q < kDotChunk; q++) acc[q] = 0.0;
-		for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
-#pragma unroll
-			for (int q = 0; q < kDotChunk; q++) {
-				const int p = p0 + q;
-				if (p < npairs) {
-					const int ia = p % nA, ib = p / nA;
-					acc[q] = fma(xs[(int64_t)ia * ldx + i], ys[(int64_t)ib * ldy + i], acc[q]);
-				}
-			}
-		}
-#pragma unroll
-		for (int q = 0; q < kDotChunk; q++) {
-			const double v = block_sum_d<BS>(acc[q], red);
-			if (threadIdx.x == 0 && p0 + q < npairs && v != 0.0) atomicAdd(&vals[p0 + q], v);
-		}
-	}
-	dots_finish(nA, nB, vals, nullptr, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
-}
-
 // x = fma(alpha, p, x); r = fma(-alpha, q, r)   (src/cpu/ozblas_cg.cpp:87-88, ozblas_axpy.cpp)
 __global__ void k_axpy2_dev(int n, const CgDev *S, int it, const double *P, double *X,
                             const double *Q, double *R) {
@@ -1469,8 +1467,6 @@ void launch_dots_dev(hipStream_t st, int n, const double *xs, int64_t ldx, const
 	if (gx < 1) gx = 1;
 	hipLaunchKernelGGL((k_dots_reg<256, kRA, kRB>), dim3(gx), dim3(256), 0, st, n, xs, ldx, infoX, ys,
 	                   ldy, infoY, vals, part, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
-	hipLaunchKernelGGL(k_dots_dev<256>, dim3(gx), dim3(256), 0, st, n, xs, ldx, infoX, ys, ldy, infoY,
-	                   vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32, kRA, kRB);
 }
 
 void launch_axpy2_dev(hipStream_t st, int n, const CgDev *S, int it, const double *P, double *X,
