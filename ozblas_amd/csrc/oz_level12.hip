@@ -581,18 +581,36 @@ __global__ __launch_bounds__(256) void k_dot2_partials(int n, const double *xs, 
 	const int64_t lo = t * q + (t < r ? t : r), hi = lo + q + (t < r ? 1 : 0);
 	const double *x = xs + (int64_t)ia * ldx, *y = ys + (int64_t)ib * ldy;
 	dd L{0.0, 0.0};
-	for (int64_t k = lo; k < hi; k++) dot2i(x[k], y[k], L);
+	// the chunk in order, 16 elements of each operand loaded together ahead of their steps
+	constexpr int B = 16;
+	int64_t k = lo;
+	for (; k + B <= hi; k += B) {
+		double xv[B], yv[B];
+#pragma unroll
+		for (int u = 0; u < B; u++) {
+			xv[u] = x[k + u];
+			yv[u] = y[k + u];
+		}
+#pragma unroll
+		for (int u = 0; u < B; u++) dot2i(xv[u], yv[u], L);
+	}
+	for (; k < hi; k++) dot2i(x[k], y[k], L);
 	part[(int64_t)pr * kDot2Chunks + t] = L;
 }
 
-// one thread per pair: QuadAdd of the partials in chunk order, QuadMul(1, .), then hi to
-// vals[nA * ib + ia] and lo to vals[nA * (nB + ib) + ia] (C and C + n*ldc of the reference)
-__global__ void k_dot2_combine(int nA, int nB, const dd *part, double *vals) {
-	const int pr = blockIdx.x * blockDim.x + threadIdx.x;
+// one workgroup per pair: the partials staged in LDS, then thread 0 QuadAdds them in chunk order,
+// QuadMul(1, .), hi to vals[nA * ib + ia] and lo to vals[nA * (nB + ib) + ia] (C and C + n*ldc of
+// the reference)
+__global__ __launch_bounds__(256) void k_dot2_combine(int nA, int nB, const dd *part, double *vals) {
+	__shared__ dd sp[kDot2Chunks];
+	const int pr = blockIdx.x;
 	if (pr >= nA * nB) return;
+	for (int t = threadIdx.x; t < kDot2Chunks; t += blockDim.x) sp[t] = part[(int64_t)pr * kDot2Chunks + t];
+	__syncthreads();
+	if (threadIdx.x != 0) return;
 	const int ia = pr % nA, ib = pr / nA;
 	dd G{0.0, 0.0};
-	for (int t = 0; t < kDot2Chunks; t++) G = quad_add(G, part[(int64_t)pr * kDot2Chunks + t]);
+	for (int t = 0; t < kDot2Chunks; t++) G = quad_add(G, sp[t]);
 	G = quad_mul(1.0, G);
 	vals[nA * ib + ia] = G.x;
 	vals[nA * (nB + ib) + ia] = G.y;
@@ -604,8 +622,7 @@ void launch_dot2_slices(hipStream_t st, int n, const double *xs, int64_t ldx, in
                         const double *ys, int64_t ldy, int nB, void *part, double *vals) {
 	hipLaunchKernelGGL(k_dot2_partials, dim3((kDot2Chunks + 255) / 256, nA * nB), dim3(256), 0, st, n,
 	                   xs, ldx, nA, ys, ldy, (dd *)part);
-	hipLaunchKernelGGL(k_dot2_combine, dim3((nA * nB + 63) / 64), dim3(64), 0, st, nA, nB,
-	                   (const dd *)part, vals);
+	hipLaunchKernelGGL(k_dot2_combine, dim3(nA * nB), dim3(256), 0, st, nA, nB, (const dd *)part, vals);
 }
 
 // DOT / NRM2 summation of the nA x nB slice dots (sumOrder 2, src/cpu/ozblas_gemm.cpp:281-288):
