@@ -218,12 +218,10 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned
 			const VsElem q = vs_decompose(v, dummy);
 			if (q.m == 0) return;
 			const int K = q.hi - q.lo + 1;
+			const int off = lane % K;  // this lane's starting entry
 			for (int j = 0; j < K; j++) {
-				int t = j + lane;
-				if (t >= K) {
-					t -= K;
-					if (t >= K) t %= K;
-				}
+				int t = j + off;
+				if (t >= K) t -= K;
 				const int k = q.lo + t;
 				const double y = ldexp(v, -k);
 				const uint64_t key = d2u(y - rint(y)) & ~(uint64_t(1) << 63);
@@ -273,17 +271,39 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned
 		Gb[k + kVsOff] = d2u(ldexp(__builtin_bit_cast(double, (uint64_t)Fg[k + kVsOff]), k));
 		Fg[k + kVsOff] = 0ull;  // clear for the next split
 	}
-	__syncthreads();
-	if (threadIdx.x != 0) return;
-	int bad = badB;
+	// the tau table of the level walk into LDS
+	constexpr int nJ = std::is_same<TI, float>::value ? kTauBinadesF : kTauBinades;
+	__shared__ uint16_t Js[kTauBinades];
+	for (int q = threadIdx.x; q < nJ; q += BS) Js[q] = J[q];
 	// |cmod(x, 2^k)| does not decrease with k (a residue modulo 2^k' is one modulo 2^k, and the
-	// centred one is the smallest), so F is the prefix max of the entries; beyond K1 it is max |x|
+	// centred one is the smallest), so F is the prefix max of the entries; beyond K1 it is max |x|.
+	// Prefix max in parallel: each thread's segment, then a scan of the segment maxima.
+	__shared__ unsigned long long segMax[BS];
+	const int Lk = (K0 <= K1) ? K1 - K0 + 1 : 0;
+	const int per = (Lk + BS - 1) / BS;
+	const int s0 = (Lk > 0) ? K0 + (int)threadIdx.x * per : 0;
+	const int s1 = (Lk > 0) ? min(s0 + per, K1 + 1) : 0;
 	unsigned long long run = 0ull;
-	for (int k = K0; k <= K1; k++) {
+	for (int k = s0; k < s1; k++) {
 		run = Gb[k + kVsOff] > run ? Gb[k + kVsOff] : run;
 		Gb[k + kVsOff] = run;
 	}
-	const unsigned long long gmax = run;  // max |x|
+	segMax[threadIdx.x] = run;
+	__syncthreads();
+	for (int o = 1; o < BS; o <<= 1) {
+		const unsigned long long v = (threadIdx.x >= (unsigned)o) ? segMax[threadIdx.x - o] : 0ull;
+		__syncthreads();
+		if (v > segMax[threadIdx.x]) segMax[threadIdx.x] = v;
+		__syncthreads();
+	}
+	const unsigned long long pre = (threadIdx.x > 0) ? segMax[threadIdx.x - 1] : 0ull;
+	for (int k = s0; k < s1; k++)
+		if (pre > Gb[k + kVsOff]) Gb[k + kVsOff] = pre;
+	__syncthreads();
+	if (threadIdx.x != 0) return;
+	J = Js;
+	int bad = badB;
+	const unsigned long long gmax = segMax[BS - 1];  // max |x|
 	auto Fk = [&](int k) -> double {
 		if (K0 > K1 || k < K0) return 0.0;
 		if (k > K1) return __builtin_bit_cast(double, gmax);
