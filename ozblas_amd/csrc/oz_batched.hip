@@ -132,13 +132,29 @@ __global__ __launch_bounds__(256) void k_transpose_grouped(const GTrans *g, cons
 	}
 }
 
-__global__ __launch_bounds__(256) void k_stats_grouped(const GStat *g, const int *start, int n) {
+// kStatCols consecutive columns (of all operands, in order) per workgroup; the statistics of each
+// operand's columns are folded into its counters once per workgroup
+constexpr int kStatCols = 8;
+__global__ __launch_bounds__(256) void k_stats_grouped(const GStat *g, const int *start, int n,
+                                                       int total) {
 	__shared__ int red[4];
 	__shared__ double redd[4];
-	const int o = owner(start, n, blockIdx.x);
-	const GStat s = g[o];
-	const int j = blockIdx.x - start[o];
-	digit_col_stats<256>(s.in + (int64_t)j * s.ldi, s.m, s.colE + j, s.colS + j, s.st, red, redd);
+	const int c0 = blockIdx.x * kStatCols, c1 = min(c0 + kStatCols, total);
+	int cur = -1;
+	ColAcc acc;
+	GStat s{};
+	for (int c = c0; c < c1; c++) {
+		const int o = owner(start, n, c);
+		if (o != cur) {
+			if (cur >= 0 && threadIdx.x == 0) col_acc_flush(acc, s.st);
+			acc = ColAcc();
+			cur = o;
+			s = g[o];
+		}
+		const int j = c - start[o];
+		digit_col_stats_acc<256>(s.in + (int64_t)j * s.ldi, s.m, s.colE + j, s.colS + j, acc, red, redd);
+	}
+	if (cur >= 0 && threadIdx.x == 0) col_acc_flush(acc, s.st);
 }
 
 // flags a problem whose operands no longer satisfy what its plan was built for
@@ -306,8 +322,8 @@ static int32_t launch_execute(BatchPlan *p, hipStream_t st) {
 		hipLaunchKernelGGL(k_transpose_grouped, dim3(p->nTransBlocks), dim3(256), 0, st, p->dTrans,
 		                   p->dTransStart, p->nTrans);
 	if (p->nStatBlocks > 0)
-		hipLaunchKernelGGL(k_stats_grouped, dim3(p->nStatBlocks), dim3(256), 0, st, p->dStat,
-		                   p->dStatStart, p->nops);
+		hipLaunchKernelGGL(k_stats_grouped, dim3((p->nStatBlocks + kStatCols - 1) / kStatCols), dim3(256),
+		                   0, st, p->dStat, p->dStatStart, p->nops, p->nStatBlocks);
 	if (p->nCheck > 0)
 		hipLaunchKernelGGL(k_check_grouped, dim3((p->nCheck + 255) / 256), dim3(256), 0, st, p->dCheck,
 		                   p->nCheck, p->dSt, p->dFlag);
@@ -442,7 +458,7 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 			g.colS = p->opColE[o] ? (int *)(M1 + colSOff[o]) : nullptr;
 			g.st = p->dSt + o;
 			statStart[o] = sb;
-			sb += p->opCols[o];  // one workgroup per column (0 for absent problems)
+			sb += p->opCols[o];  // columns, in operand order (0 for absent problems)
 		}
 		statStart[p->nops] = sb;
 		p->nStatBlocks = sb;
@@ -466,8 +482,8 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 		hipLaunchKernelGGL(k_transpose_grouped, dim3(p->nTransBlocks), dim3(256), 0, st, p->dTrans,
 		                   p->dTransStart, p->nTrans);
 	if (p->nStatBlocks > 0)
-		hipLaunchKernelGGL(k_stats_grouped, dim3(p->nStatBlocks), dim3(256), 0, st, p->dStat,
-		                   p->dStatStart, p->nops);
+		hipLaunchKernelGGL(k_stats_grouped, dim3((p->nStatBlocks + kStatCols - 1) / kStatCols), dim3(256),
+		                   0, st, p->dStat, p->dStatStart, p->nops, p->nStatBlocks);
 	OZ_HIP(hipGetLastError());
 	OZ_HIP(hipMemcpyAsync(p->hSt, p->dSt, sizeof(ExpStatsH) * p->nops, hipMemcpyDeviceToHost, st));
 	OZ_HIP(hipStreamSynchronize(st));
