@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "oz_digits.hpp"
@@ -226,6 +227,34 @@ static int32_t gemm_plain(ozblasHandle_t *oh, char transA, char transB, int m, i
 			wreset(oh, base2);
 			T *rem = (T *)walloc(oh, sizeof(T) * ldk * std::max(mbk_, nbk_));
 			if (!rem) return OZBLAS_STATUS_WORKSPACE;
+			// binary64 GEMV over the whole matrix: the split of op(A) is fused into the slice
+			// products (the matrix is read once, its slices never stored); x is split as usual
+			if constexpr (std::is_same<T, double>::value) {
+				if (!dotPath && n == 1 && fastMode == 0 && nbk_ == 1 && mbk == m && !b->mixed &&
+				    split_gemv_fits(k, nAf, nBf)) {
+					DenseSplit dx;
+					OZ_TRY(dense_split(oh, k, nbk_, bin + (int64_t)in * nbk * ldbi, ldbi, rem, ldk, dx,
+					                   nbk == n ? zBf : -1));
+					T *Cblk = C + (int64_t)ldc * ((int64_t)in * nbk) + (int64_t)im * mbk;
+					double *P = (double *)walloc(oh, split_gemv_pairs_bytes(mbk_));
+					if (!P) return OZBLAS_STATUS_WORKSPACE;
+					OZ_HIP(hipEventRecord(b->ev[3], st));
+					if (launch_split_gemv(st, k, mbk_, ain + (int64_t)im * mbk * ldai, ldai, Prec<T>::rho(oh, k),
+					                      oh->splitShift, split_cap(oh), Prec<T>::Jsplit(b), nAf, dx.slices,
+					                      dx.sstride, dx.count, fsum ? 1 : 0, alpha, beta, Cblk,
+					                      Prec<T>::Jsum(b), P)) {
+						OZ_HIP(hipEventRecord(b->ev[4], st));
+						OZ_HIP(hipGetLastError());
+						OZ_TRY(sync(oh));
+						tcomp += ev_sec(b->ev[3], b->ev[4]);
+						sA += nAf;
+						sB += dx.count;
+						sC += (double)nAf * dx.count;
+						blocks++;
+						continue;
+					}
+				}
+			}
 			DenseSplit da, db;
 			// a block that is the whole operand reuses the first-zero level of count_cols
 			OZ_TRY(dense_split(oh, k, mbk_, ain + (int64_t)im * mbk * ldai, ldai, rem, ldk, da,

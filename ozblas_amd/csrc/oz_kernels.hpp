@@ -135,6 +135,14 @@ void launch_csrmm_store(hipStream_t st, int m, const double *As, int64_t ldas, i
 // GEMV of the plain form on k-contiguous slices (slice s of A column i at As + s*sA + i*ldk, of x
 // at Xs + s*sX), fused with the sum (Fsum in level order or NearSum) and the epilogue into y;
 // false when the slice counts exceed its register bound (8 x 8)
+// GEMV with the split of op(A) fused (binary64; rows of op(A) k-contiguous in `in`): false when the
+// sizes or slice counts are outside the kernel's (then nothing was launched; split_gemv_fits tells)
+// P: split_gemv_pairs_bytes(M) of scratch for the pair values of every row
+bool split_gemv_fits(int K, int nA, int nB);
+size_t split_gemv_pairs_bytes(int M);
+bool launch_split_gemv(hipStream_t st, int K, int M, const double *in, int64_t ldi, int rho, int shift,
+                       int cap, const uint16_t *Js, int nA, const double *Xs, int64_t sX, int nB,
+                       int fsum, double alpha, double beta, double *y, const uint16_t *J, double *P);
 bool launch_gemv_slices(hipStream_t st, int M, int K, const double *As, int64_t ldk, int64_t sA, int nA,
                         const double *Xs, int64_t sX, int nB, int fsum, double alpha, double beta,
                         double *y, const uint16_t *J);

@@ -1054,6 +1054,162 @@ bool launch_gemv_slices(hipStream_t st, int M, int K, const double *As, int64_t 
 	return gemv_slices(st, M, K, As, ldk, sA, nA, Xs, sX, nB, fsum, alpha, beta, y, JF2);
 }
 
+// ---------------------------------------------------------------------------------------------
+// GEMV with the split of op(A) fused (binary64): one 512-thread workgroup per row of op(A), which
+// is a k-contiguous column of `in` (K <= 512 R; up to 8 slices of A and 4 of x).  The row is split in registers level by level
+// with the rule of k_split_cols_reg (slot s: the part of level s; then the remainder slot and zero
+// slots up to nA), and each slot is multiplied at once by the x slices, which every thread keeps
+// in registers for its R indices: the pair dots sum_k As_s[k] Xs_b[k] are exact, so the workgroup
+// sums them in any order (one LDS round per level carries the nB pair sums and the next level's
+// remainder maximum).  The nA x nB pair values of row i go to P[i * kSgPairs + a * NBM + b]; the
+// slices of A are never stored, the matrix is read once.  k_pairs_sum then sums each row's values
+// as k_gemv_slices does (Fsum in (level, ia) order, or NearSum) and writes y_i.
+// ---------------------------------------------------------------------------------------------
+constexpr int kSgBS = 512, kSgNAM = 8, kSgPairs = 64;
+template <int R, int NBM>
+__global__ __launch_bounds__(kSgBS) void k_split_gemv(int K, int M, const double *in, int64_t ldi,
+                                                      int rho, int shift, int cap, const uint16_t *Js,
+                                                      int nA, const double *Xs, int64_t sX, int nB,
+                                                      double *P) {
+	constexpr int BS = kSgBS, NW = BS / 64, NV = NBM + 1;  // per round: NBM sums and a maximum
+	__shared__ double red[2][NW][NV];
+	const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+	double xr[NBM][R];
+#pragma unroll
+	for (int b = 0; b < NBM; b++)
+#pragma unroll
+		for (int q = 0; q < R; q++) {
+			const int i = threadIdx.x + BS * q;
+			xr[b][q] = (b < nB && i < K) ? Xs[(int64_t)b * sX + i] : 0.0;
+		}
+	int round = 0;
+	// one reduction round over the workgroup: val[0..NBM) summed, val[NBM] maximised; every
+	// thread gets the totals (double-buffered by round parity: one barrier per round)
+	auto reduce = [&](double (&val)[NV]) {
+#pragma unroll
+		for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+			for (int b = 0; b < NBM; b++) val[b] += __shfl_xor(val[b], o, 64);
+			val[NBM] = std_max(val[NBM], __shfl_xor(val[NBM], o, 64));
+		}
+		const int buf = round & 1;
+		if (lane == 0) {
+#pragma unroll
+			for (int c = 0; c < NV; c++) red[buf][w][c] = val[c];
+		}
+		__syncthreads();
+#pragma unroll
+		for (int c = 0; c < NV; c++) val[c] = red[buf][0][c];
+		for (int u = 1; u < NW; u++) {
+#pragma unroll
+			for (int b = 0; b < NBM; b++) val[b] += red[buf][u][b];
+			val[NBM] = std_max(val[NBM], red[buf][u][NBM]);
+		}
+		round++;
+	};
+	for (int j = blockIdx.x; j < M; j += gridDim.x) {
+		const double *x = in + (int64_t)j * ldi;
+		double *Pj = P + (int64_t)j * kSgPairs;
+		double v[R];
+		double val[NV];
+#pragma unroll
+		for (int c = 0; c < NV; c++) val[c] = 0.0;
+#pragma unroll
+		for (int q = 0; q < R; q++) {
+			const int i = threadIdx.x + BS * q;
+			v[q] = (i < K) ? x[i] : 0.0;
+			val[NBM] = std_max(val[NBM], fabs(v[q]));
+		}
+		reduce(val);
+		double mx = val[NBM];
+		int s = 0;
+		bool zero = false;
+		for (s = 0; s < cap; s++) {
+			const int tau = SplitRule<double>::tau(mx, Js);
+			const double sigma = SplitRule<double>::sigma(rho, tau, shift);
+#pragma unroll
+			for (int c = 0; c < NV; c++) val[c] = 0.0;
+#pragma unroll
+			for (int q = 0; q < R; q++) {
+				const double part = (v[q] + sigma) - sigma;
+				const double rr = v[q] - part;
+				v[q] = rr;
+#pragma unroll
+				for (int b = 0; b < NBM; b++) val[b] = fma(part, xr[b][q], val[b]);
+				val[NBM] = std_max(val[NBM], fabs(rr));
+			}
+			reduce(val);
+			if (threadIdx.x < NBM && s < kSgNAM) Pj[s * NBM + threadIdx.x] = val[threadIdx.x];
+			mx = val[NBM];
+			if (mx == 0.0) {
+				zero = true;
+				break;
+			}
+		}
+		const int last = zero ? s : cap - 1;
+		if (!zero && last + 1 < nA) {  // the remainder slot
+#pragma unroll
+			for (int c = 0; c < NV; c++) val[c] = 0.0;
+#pragma unroll
+			for (int q = 0; q < R; q++)
+#pragma unroll
+				for (int b = 0; b < NBM; b++) val[b] = fma(v[q], xr[b][q], val[b]);
+			reduce(val);
+			if (threadIdx.x < NBM && last + 1 < kSgNAM) Pj[(last + 1) * NBM + threadIdx.x] = val[threadIdx.x];
+		}
+		// zero slots
+		const int z0 = zero ? last + 1 : last + 2;
+		for (int t = z0 + (int)threadIdx.x / NBM; t < nA && t < kSgNAM; t += BS / NBM)
+			Pj[t * NBM + threadIdx.x % NBM] = 0.0;
+	}
+}
+
+// y_i = fma(alpha, sum of row i's pair values, beta * y_i); one thread per row
+template <int NBM>
+__global__ __launch_bounds__(256) void k_pairs_sum(int M, double *P, int nA, int nB, int fsum,
+                                                   double alpha, double beta, double *y,
+                                                   const uint16_t *J) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= M) return;
+	const double *Pi = P + (int64_t)i * kSgPairs;
+	double t;
+	if (fsum) {
+		t = 0.0;
+		for (int ik = 0; ik < kSgNAM + NBM - 1; ik++)
+			for (int a = 0; a < kSgNAM; a++) {
+				const int b = ik - a;
+				if (b >= 0 && b < NBM && a < nA && b < nB) t = t + Pi[a * NBM + b];
+			}
+	} else {
+		// the values in (a, b) order, compacted in place (scratch), then NearSum over them
+		double *Pw = P + (int64_t)i * kSgPairs;
+		int cnt = 0;
+		for (int a = 0; a < nA; a++)
+			for (int b = 0; b < nB; b++) Pw[cnt++] = Pw[a * NBM + b];
+		StridedVec u{Pw, 1};
+		t = nearsum(cnt, u, J);
+	}
+	y[i] = fma(alpha, t, beta * y[i]);
+}
+
+size_t split_gemv_pairs_bytes(int M) { return sizeof(double) * kSgPairs * (size_t)(M > 0 ? M : 1); }
+
+bool split_gemv_fits(int K, int nA, int nB) {
+	return nA >= 1 && nA <= kSgNAM && nB >= 1 && nB <= 4 && K >= 1 && K <= 16 * kSgBS;
+}
+
+bool launch_split_gemv(hipStream_t st, int K, int M, const double *in, int64_t ldi, int rho, int shift,
+                       int cap, const uint16_t *Js, int nA, const double *Xs, int64_t sX, int nB,
+                       int fsum, double alpha, double beta, double *y, const uint16_t *J, double *P) {
+	if (!split_gemv_fits(K, nA, nB) || M < 1) return false;
+	const int grid = M < 1024 ? M : 1024;
+	const int gs = (M + 255) / 256;
+	hipLaunchKernelGGL((k_split_gemv<16, 4>), dim3(grid), dim3(kSgBS), 0, st, K, M, in, ldi, rho, shift, cap,
+	                   Js, nA, Xs, sX, nB, P);
+	hipLaunchKernelGGL((k_pairs_sum<4>), dim3(gs), dim3(256), 0, st, M, P, nA, nB, fsum, alpha, beta, y, J);
+	return true;
+}
+
 // general case: store the products where blasRcsrmm puts them (column ic of CSplit, ld ldcs);
 // T = float: the fma1<float> chain of csrmm_n<float>
 template <class T>
