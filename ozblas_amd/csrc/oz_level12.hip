@@ -126,15 +126,10 @@ __device__ inline double vs_cmod(double x, int k) {
 // floats), with the binary32 tau rule and unit 2^(rho + tau - 24)
 template <int BS, class TI>
 __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned long long *Fg,
-                                                    int2 *range,
-                                                    unsigned *ctr, int rho, int cap,
-                                                    int reproMode, const uint16_t *J, VsPlan *plan,
-                                                    int *info, unsigned long long *maxv,
-                                                    const int *stopAt, int it) {
+                                                    int2 *range, const int *stopAt, int it) {
 	if (stopAt && it >= *stopAt) return;
 	__shared__ unsigned long long Fb[kVsK];
-	unsigned long long *Gb = Fb;  // the last workgroup reuses its table once folded into Fg
-	__shared__ int kminB, kmaxB, badB, isLast;
+	__shared__ int kminB, kmaxB, badB;
 	__shared__ int red[2][BS / 64];
 	for (int k = threadIdx.x; k < kVsK; k += BS) Fb[k] = 0ull;
 	if (threadIdx.x == 0) badB = 0;
@@ -242,20 +237,27 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned
 	}
 	// range: [r0, r1] (empty when r0 > r1); x = INT32_MIN marks an input outside the model
 	if (threadIdx.x == 0) range[blockIdx.x] = badB ? make_int2(INT32_MIN, INT32_MIN) : make_int2(r0, r1);
-	__threadfence();
-	__syncthreads();
-	if (threadIdx.x == 0) isLast = (atomicAdd(ctr, 1u) == gridDim.x - 1) ? 1 : 0;
-	__syncthreads();
-	if (!isLast) return;
-	__threadfence();
-	// ---- last workgroup: the union of the ranges, the global tables, then the levels
+}
+
+// The level walk of the split, one workgroup after k_vsplit_scan (its nwg workgroups' ranges and
+// the global table): the union of the ranges, F as the prefix max of the table, then the levels
+// M_{s+1} = F(rho + tau(M_s) - kPrec) into the plan.
+template <int BS, class TI>
+__global__ __launch_bounds__(BS) void k_vsplit_walk(int nwg, unsigned long long *Fg, const int2 *range,
+                                                    int rho, int cap, int reproMode,
+                                                    const uint16_t *J, VsPlan *plan, int *info,
+                                                    unsigned long long *maxv, const int *stopAt,
+                                                    int it) {
+	if (stopAt && it >= *stopAt) return;
+	__shared__ unsigned long long Gb[kVsK];
+	__shared__ int kminB, kmaxB, badB;
 	if (threadIdx.x == 0) {
 		kminB = INT32_MAX;
 		kmaxB = INT32_MIN;
 		badB = 0;
 	}
 	__syncthreads();
-	for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) {
+	for (int b = threadIdx.x; b < nwg; b += BS) {
 		const int2 rg = range[b];
 		if (rg.x == INT32_MIN) {
 			badB = 1;
@@ -271,6 +273,7 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned
 		Gb[k + kVsOff] = d2u(ldexp(__builtin_bit_cast(double, (uint64_t)Fg[k + kVsOff]), k));
 		Fg[k + kVsOff] = 0ull;  // clear for the next split
 	}
+	__syncthreads();  // the prefix segments below span entries other threads converted
 	// the tau table of the level walk into LDS
 	constexpr int nJ = std::is_same<TI, float>::value ? kTauBinadesF : kTauBinades;
 	__shared__ uint16_t Js[kTauBinades];
@@ -347,7 +350,6 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned
 	} else {
 		for (int q = 0; q < cap + 2; q++) maxv[q] = 0ull;  // the fallback reduces into maxv
 	}
-	*ctr = 0u;
 }
 
 // the reference loop in one workgroup of BS threads (inputs outside the model above): slice s at slot s, its
@@ -438,7 +440,6 @@ static hipError_t vsplit(hipStream_t st, int n, const TI *x, double *slices, int
                          const int *stopAt, int it) {
 	unsigned long long *Fg = (unsigned long long *)scratch;
 	int2 *range = (int2 *)(Fg + kVsK);
-	unsigned *ctr = (unsigned *)(range + kVsBlocks);
 	int rhoEff = rho;
 	if (shift > 1) {  // sigma * 2^j acts as rho + j
 		if (shift & (shift - 1)) rhoEff = -1000;  // not a power of two: the fallback splits
@@ -446,8 +447,10 @@ static hipError_t vsplit(hipStream_t st, int n, const TI *x, double *slices, int
 	}
 	int grid = (int)std::min<int64_t>(kVsBlocks, ((int64_t)n + kVsBS - 1) / kVsBS);
 	if (grid < 1) grid = 1;
-	hipLaunchKernelGGL((k_vsplit_scan<kVsBS, TI>), dim3(grid), dim3(kVsBS), 0, st, n, x, Fg, range,
-	                   ctr, rhoEff, cap, reproMode, J, plan, info, maxv, stopAt, it);
+	hipLaunchKernelGGL((k_vsplit_scan<kVsBS, TI>), dim3(grid), dim3(kVsBS), 0, st, n, x, Fg, range, stopAt,
+	                   it);
+	hipLaunchKernelGGL((k_vsplit_walk<kVsBS, TI>), dim3(1), dim3(kVsBS), 0, st, grid, Fg,
+	                   (const int2 *)range, rhoEff, cap, reproMode, J, plan, info, maxv, stopAt, it);
 	int gw = (int)std::min<int64_t>(2048, ((int64_t)n + 255) / 256);
 	if (gw < 1) gw = 1;
 	hipLaunchKernelGGL(k_vsplit_write<TI>, dim3(gw), dim3(256), 0, st, n, x, slices, lds,

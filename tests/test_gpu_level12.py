@@ -71,6 +71,25 @@ def test_dot_split_edge_cases_bitwise(oz, oracle, sumMode):
             assert (H.nSplitA, H.nSplitB) == (h.nSplitA, h.nSplitB), name
 
 
+@pytest.mark.parametrize("nSplitMax", [0, 3])
+def test_dot_split_range_over_512_binades_bitwise(oz, oracle, nSplitMax):
+    """A vector whose exponents span about 600 binades: the split's residue table then holds more
+    entries than the level walk has threads, so its prefix maximum spans several entries per
+    thread (written by other threads)."""
+    rng = np.random.default_rng(77)
+    n = 600
+    x = rng.standard_normal(n) * np.exp2(rng.integers(-300, 300, n).astype(float))
+    y = rng.uniform(-1, 1, n)
+    for a, b in ((x, x), (x, y)):
+        h = oracle.Handle(1 << 30, sumMode=1, nSplitMax=nSplitMax)
+        want = h.dot(n, a, b)
+        H = oz.Handle(1 << 28, sumMode=1, nSplitMax=nSplitMax)
+        say(f"ddot range 600 binades nSplitMax={nSplitMax}")
+        got = H.ddot(n, dev(a), dev(b))
+        assert np.float64(got).tobytes() == np.float64(want).tobytes(), (got, want)
+        assert (H.nSplitA, H.nSplitB) == (h.nSplitA, h.nSplitB)
+
+
 def test_dot_is_correctly_rounded_under_nearsum(oz, oracle):
     rng = np.random.default_rng(8)
     for n in (3, 999, 40000):
