@@ -669,11 +669,9 @@ static int32_t dot_impl(ozblasHandle_t *oh, int n, const T *x, const T *y, T *re
 		const int cap = split_cap(oh);
 		double *vals = (double *)walloc(oh, 8ull * (cap + 1) * (cap + 1));
 		double *part = (double *)walloc(oh, dots_part_bytes());
-		unsigned *ctr = (unsigned *)walloc(oh, 8);
-		if (!vals || !part || !ctr) return OZBLAS_STATUS_WORKSPACE;
+		if (!vals || !part) return OZBLAS_STATUS_WORKSPACE;
 		OZ_HIP(hipMemsetAsync(vals, 0, 8ull * (cap + 1) * (cap + 1), b->stream));
-		OZ_HIP(hipMemsetAsync(ctr, 0, 8, b->stream));
-		launch_dots_dev(b->stream, n, xs.slices, xs.lds, xs.info, ys.slices, ys.lds, ys.info, vals, part, ctr,
+		launch_dots_dev(b->stream, n, xs.slices, xs.lds, xs.info, ys.slices, ys.lds, ys.info, vals, part,
 		                xs.maxv, ys.maxv, cap + 2, Prec<T>::Jsum(b), oh->sumMode == 0 ? 1 : 0, nullptr,
 		                0, 0, dres, f32 ? 1 : 0);
 	}
@@ -1211,7 +1209,6 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 	unsigned long long *mxQ = mxP + (cap + 2), *mxR = mxQ + (cap + 2);
 	double *vals = (double *)(mxR + (cap + 2));
 	int *info = (int *)(vals + (cap + 1) * (cap + 1));  // P, Q, R: two ints each
-	unsigned *ctr = (unsigned *)(info + 6);
 	double *dhist = (double *)(info + 16);
 	OZ_HIP(hipMemsetAsync(sm, 0, small, st));
 	OZ_HIP(hipMemsetAsync(vscr, 0, vsplit_scratch_bytes(), st));
@@ -1238,12 +1235,12 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 		                     v.Q, Cs, ldc, b->dTau, S, it);
 		OZ_HIP(launch_vsplit(st, n, v.Q, slQ, lds, rho, 1, cap, reproCount, b->dTau, mxQ, infoQ,
 		                     vscr, plan, &S->stopAt, it));
-		launch_dots_dev(st, n, slP, lds, infoP, slQ, lds, infoQ, vals, part, ctr, mxP, mxQ, cap + 2, b->dTau,
+		launch_dots_dev(st, n, slP, lds, infoP, slQ, lds, infoQ, vals, part, mxP, mxQ, cap + 2, b->dTau,
 		                fsum, S, it, 1, nullptr);
 		launch_axpy2_dev(st, n, S, it, v.P, x, v.Q, v.R);
 		OZ_HIP(launch_vsplit(st, n, v.R, slR, lds, rho, 1, cap, reproCount, b->dTau, mxR, infoR,
 		                     vscr, plan, &S->stopAt, it));
-		launch_dots_dev(st, n, slR, lds, infoR, slR, lds, infoR, vals, part, ctr, mxR, mxR, cap + 2, b->dTau,
+		launch_dots_dev(st, n, slR, lds, infoR, slR, lds, infoR, vals, part, mxR, mxR, cap + 2, b->dTau,
 		                fsum, S, it, 2, nullptr);
 		launch_scal_axpy_dev(st, n, S, it, v.R, v.P);
 		if (stamps && ((it + 1) % oh->verbose == 0 || it + 1 == maxiter || it % 16 == 15)) {

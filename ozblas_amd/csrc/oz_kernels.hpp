@@ -199,7 +199,7 @@ void launch_csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas,
 size_t dots_part_bytes();
 void launch_dots_dev(hipStream_t st, int n, const double *xs, int64_t ldx, const int *infoX,
                      const double *ys, int64_t ldy, const int *infoY, double *vals, double *part,
-                     unsigned *ctr, unsigned long long *maxvX, unsigned long long *maxvY, int nmax,
+                     unsigned long long *maxvX, unsigned long long *maxvY, int nmax,
                      const uint16_t *J, int fsum, CgDev *S, int it, int mode, void *C,
                      int f32 = 0);
 void launch_axpy2_dev(hipStream_t st, int n, const CgDev *S, int it, const double *P, double *X,

@@ -1407,37 +1407,31 @@ __global__ __launch_bounds__(256) void k_csrmm_big_dev(int m, const double *As, 
 	y[r] = fma(1.0, t, 0.0 * y[r]);
 }
 
-// Completion of a device DOT by the last workgroup to finish (the caller has added its exact pair
-// partials to vals): sums the nA x nB pair dots (sumOrder 2) and applies `mode`
+// Completion of a device DOT in one workgroup, after the product kernel: sums the nA x nB pair
+// dots (sumOrder 2) and applies `mode`
 //   0: C[0] = fma(1, t, 0 * C[0])                        (ozblasRdot's GEMM epilogue)
 //   1: tmp = t, alpha = dold / tmp                       (src/cpu/ozblas_cg.cpp:83-84)
 //   2: dnew = t, resi = sqrt(dnew), beta = dnew / dold,  (src/cpu/ozblas_cg.cpp:91-94,100-112)
 //      dold = dnew, history and the residual test
-// then clears vals, the counter and the split maxima it consumed for the next use.
+// then clears vals and the split maxima it consumed for the next use.
 // With f32 (binary32 DOT; mode 0 only) the pair dots round to float first, the sum is the float
 // one (J is then JF2) and C is a float: C[0] = fma1<float>(1, t, 0 * C[0]).
-// With `part` (k_dots_reg) the pair dots are first summed from the per-workgroup partials
-// part[w * kDotPairs + p] by the whole last workgroup (exact in any order, like the atomics of
-// dots_general into vals).
-__device__ void dots_finish(int nA, int nB, double *vals, const double *part, unsigned *ctr,
+// The pair dots are the per-workgroup partials part[w * kDotPairs + p] of k_dots_reg's nwg
+// workgroups, summed here by the whole workgroup (exact in any order), or with part == nullptr
+// the sums dots_general accumulated in vals.
+__device__ void dots_finish(int nA, int nB, double *vals, const double *part, int nwg,
                             unsigned long long *maxvX, unsigned long long *maxvY, int nmax,
                             const uint16_t *J, int fsum, CgDev *S, int it, int mode, void *C,
                             int f32) {
-	__shared__ int isLast;
 	__shared__ double psum[1][kDotPairs];
-	__threadfence();
-	if (threadIdx.x == 0) isLast = (atomicAdd(ctr, 1u) == gridDim.x - 1) ? 1 : 0;
-	__syncthreads();
-	if (!isLast) return;
-	__threadfence();
-	const int npairs = nA * nB;  // vals: plain loads, ordered after the other workgroups by the fence
+	const int npairs = nA * nB;
 	double *src = vals;
 	if (part) {
 		// entry k = w * npairs + p of the partials, spread over the threads, loads issued in groups
 		// of 8 before their LDS additions
 		if (threadIdx.x < kDotPairs) psum[0][threadIdx.x] = 0.0;
 		__syncthreads();
-		const int total = (int)gridDim.x * npairs;
+		const int total = nwg * npairs;
 		for (int k0 = threadIdx.x; k0 < total; k0 += 8 * (int)blockDim.x) {
 			double v[8];
 #pragma unroll
@@ -1466,7 +1460,6 @@ __device__ void dots_finish(int nA, int nB, double *vals, const double *part, un
 			maxvX[q] = 0ull;
 			maxvY[q] = 0ull;
 		}
-		*ctr = 0u;
 		return;
 	}
 	const double t = dot_pairs_sum(npairs, src, nA, nB, maxlevel, fsum, J);
@@ -1498,16 +1491,14 @@ __device__ void dots_finish(int nA, int nB, double *vals, const double *part, un
 		maxvX[q] = 0ull;
 		maxvY[q] = 0ull;
 	}
-	*ctr = 0u;
+
 }
 
 // The same for any slice counts (pairs in chunks of kDotChunk, slices re-read per chunk, exact
 // partials added atomically to vals); k_dots_reg runs it when the counts exceed its registers.
 template <int BS>
 __device__ void dots_general(int n, int nA, int nB, const double *xs, int64_t ldx, const double *ys,
-                             int64_t ldy, double *vals, unsigned *ctr, unsigned long long *maxvX,
-                             unsigned long long *maxvY, int nmax, const uint16_t *J, int fsum,
-                             CgDev *S, int it, int mode, void *C, int f32) {
+                             int64_t ldy, double *vals) {
 	__shared__ double red[BS / 64];
 	const int npairs = nA * nB;
 	for (int p0 = 0; p0 < npairs; p0 += kDotChunk) {
@@ -1530,7 +1521,6 @@ __device__ void dots_general(int n, int nA, int nB, const double *xs, int64_t ld
 			if (threadIdx.x == 0 && p0 + q < npairs && v != 0.0) atomicAdd(&vals[p0 + q], v);
 		}
 	}
-	dots_finish(nA, nB, vals, nullptr, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
 }
 
 // DOT of two split vectors with device slice counts, slice counts up to NAM x NBM: every element's
@@ -1543,16 +1533,12 @@ template <int BS, int NAM, int NBM>
 __global__ __launch_bounds__(BS) void k_dots_reg(int n, const double *xs, int64_t ldx,
                                                  const int *infoX, const double *ys, int64_t ldy,
                                                  const int *infoY, double *vals, double *part,
-                                                 unsigned *ctr, unsigned long long *maxvX,
-                                                 unsigned long long *maxvY, int nmax,
-                                                 const uint16_t *J, int fsum, CgDev *S, int it,
-                                                 int mode, void *C, int f32) {
+                                                 const CgDev *S, int it) {
 	static_assert(NAM * NBM <= kDotPairs && BS % 64 == 0 && BS <= 256, "pair partial layout");
 	if (cg_skip(S, it)) return;
 	const int nA = infoX[0], nB = infoY[0];
 	if (nA > NAM || nB > NBM) {
-		dots_general<BS>(n, nA, nB, xs, ldx, ys, ldy, vals, ctr, maxvX, maxvY, nmax, J, fsum, S, it,
-		                 mode, C, f32);
+		dots_general<BS>(n, nA, nB, xs, ldx, ys, ldy, vals);
 		return;
 	}
 	__shared__ double red[BS / 64][NAM * NBM];
@@ -1612,7 +1598,21 @@ __global__ __launch_bounds__(BS) void k_dots_reg(int n, const double *xs, int64_
 			part[(int64_t)blockIdx.x * kDotPairs + a + nA * b] = v;
 		}
 	}
-	dots_finish(nA, nB, vals, part, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
+}
+
+// The completion of k_dots_reg in one workgroup (its nwg workgroups' partials, or the pair sums
+// the general path accumulated in vals)
+template <int BS, int NAM, int NBM>
+__global__ __launch_bounds__(BS) void k_dots_fin(int nwg, const int *infoX, const int *infoY,
+                                                 double *vals, const double *part,
+                                                 unsigned long long *maxvX,
+                                                 unsigned long long *maxvY, int nmax,
+                                                 const uint16_t *J, int fsum, CgDev *S, int it,
+                                                 int mode, void *C, int f32) {
+	if (cg_skip(S, it)) return;
+	const int nA = infoX[0], nB = infoY[0];
+	const bool reg = nA <= NAM && nB <= NBM;
+	dots_finish(nA, nB, vals, reg ? part : nullptr, nwg, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
 }
 
 // x = fma(alpha, p, x); r = fma(-alpha, q, r)   (src/cpu/ozblas_cg.cpp:87-88, ozblas_axpy.cpp)
@@ -1655,14 +1655,16 @@ size_t dots_part_bytes() { return sizeof(double) * kDotPairs * kDotMaxWG; }
 
 void launch_dots_dev(hipStream_t st, int n, const double *xs, int64_t ldx, const int *infoX,
                      const double *ys, int64_t ldy, const int *infoY, double *vals, double *part,
-                     unsigned *ctr, unsigned long long *maxvX, unsigned long long *maxvY, int nmax,
+                     unsigned long long *maxvX, unsigned long long *maxvY, int nmax,
                      const uint16_t *J, int fsum, CgDev *S, int it, int mode, void *C, int f32) {
 	constexpr int kRA = 8, kRB = 8;
 	int gx = (n + 511) / 512;  // two elements per thread and step
 	if (gx > kDotMaxWG) gx = kDotMaxWG;
 	if (gx < 1) gx = 1;
 	hipLaunchKernelGGL((k_dots_reg<256, kRA, kRB>), dim3(gx), dim3(256), 0, st, n, xs, ldx, infoX, ys,
-	                   ldy, infoY, vals, part, ctr, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
+	                   ldy, infoY, vals, part, (const CgDev *)S, it);
+	hipLaunchKernelGGL((k_dots_fin<256, kRA, kRB>), dim3(1), dim3(256), 0, st, gx, infoX, infoY, vals,
+	                   (const double *)part, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
 }
 
 void launch_axpy2_dev(hipStream_t st, int n, const CgDev *S, int it, const double *P, double *X,
