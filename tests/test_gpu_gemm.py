@@ -170,6 +170,62 @@ def test_gemv_bitwise(oz, oracle, trans, sumMode):
         assert bits_equal(host(dy), yw), first_diff(host(dy), yw)
 
 
+def _gemv_edge_cases():
+    """(name, m, n, A, x, flags): inner lengths off the 512-lane row, rows of zeros and of one value,
+    rows and x spanning many binades (up to 8 slices), the split cap reached (remainder slot)."""
+    rng = np.random.default_rng(90)
+    out = []
+    for (m, n) in ((5, 1), (33, 7), (64, 513), (40, 8192)):
+        A = rng.standard_normal(m * n)
+        A[: n * 2 : 2] = 0.0                               # zeros spread over rows
+        A.reshape(n, m)[:, 1] = 0.0                        # row 1 of A: a zero row of op(A) for 'N'
+        x = rng.standard_normal(n)
+        out.append((f"plain {m}x{n}", m, n, A, x, {}))
+    m, n = 48, 1500
+    A = rng.standard_normal(m * n) * np.exp2(rng.integers(-40, 40, m * n).astype(float))
+    x = rng.standard_normal(n) * np.exp2(rng.integers(-10, 10, n).astype(float))
+    out.append(("wide", m, n, A, x, {}))
+    out.append(("wide cap 2", m, n, A, x, {"nSplitMax": 2}))
+    out.append(("wide cap 3 repro 0", m, n, A, x, {"nSplitMax": 3, "reproMode": 0}))
+    out.append(("wide repro 0", m, n, A, x, {"reproMode": 0}))
+    A1 = np.tile(np.array([2.0 ** -3, 3.0, -1.0]), m * n // 3 + 1)[: m * n]
+    out.append(("exact early", m, n, A1, np.ones(n), {}))
+    return out
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("sumMode", [0, 1])
+def test_gemv_split_edge_cases_bitwise(oz, oracle, trans, sumMode):
+    """The binary64 GEMV whose split of op(A) is fused into the slice products, on the edge cases
+    of the split: y and the slice counts equal the oracle's (y to rounding-error bounds where the
+    reproMode 0 remainder slice makes the products inexact)."""
+    for name, m, n, A, x0, flags in _gemv_edge_cases():
+        lx, ly = (n, m) if trans == "N" else (m, n)
+        x = x0[:lx] if len(x0) >= lx else np.resize(x0, lx)
+        y0 = np.linspace(-1.0, 1.0, ly)
+        h = oracle.Handle(1 << 30, sumMode=sumMode, **flags)
+        yw = y0.copy()
+        h.gemv(trans, m, n, 0.75, A, m, x, -0.5, yw)
+        H = oz.Handle(1 << 28, sumMode=sumMode, **flags)
+        dy = dev(y0)
+        say(f"dgemv edge {name} {trans} sumMode={sumMode}")
+        H.dgemv(trans, m, n, 0.75, dev(A), m, dev(x), -0.5, dy)
+        assert (H.nSplitA, H.nSplitB) == (h.nSplitA, h.nSplitB), name
+        if flags.get("reproMode") == 0 and flags.get("nSplitMax", 0) > 0:
+            # the unsplit remainder is the last slice: its products round, in an order of each
+            # implementation's own (as test_truncated_degree_with_remainder_slice_is_close)
+            Am = A.reshape(n, m).T
+            Aop = Am if trans == "N" else Am.T
+            # each of the nA x nB pair dots is within K u sum|A||x| of its exact value in either
+            # implementation (u = 2^-53), hence the bound on the difference
+            scale = 0.75 * (np.abs(Aop) @ np.abs(x)) + 0.5 * np.abs(y0)
+            npairs = h.nSplitA * h.nSplitB
+            bound = 2 * npairs * (Aop.shape[1] + 2) * 2.0 ** -53 * scale
+            assert np.all(np.abs(host(dy) - yw) <= bound), name
+        else:
+            assert bits_equal(host(dy), yw), (name, first_diff(host(dy), yw))
+
+
 def test_plain_gemm_k0_and_empty(oz, oracle):
     """k = 0: no slices, C = fma(alpha, +0, beta * C) as the reference computes it; m = 0 / n = 0:
     C untouched."""
