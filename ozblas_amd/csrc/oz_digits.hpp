@@ -551,10 +551,48 @@ __host__ __device__ inline double big_finish(BigInt<NW> a, int e0) {
 }
 
 // Exact value of one output element: fma(alpha, RN(N 2^e0), beta * c) over all the bins.
+// The certified rounding in binary64 when it is clear-cut (k_reconstruct_check with the bound R of
+// the levels not computed, the full reconstruction with R = 0): the bin terms
+// T_b 2^(8 (Lmax - L_b)) (exact binary64 values) are summed in double-double, hi + lo with an error
+// below 4 nb^2 u^2 sum|terms| (u = 2^-53: TwoSum steps, lo accumulated with one rounding per step),
+// and renormalised to hi2 + lo2.  When |lo2| plus that error plus R stays below half the smaller
+// spacing of binary64 next to hi2 (with a 2^-20 margin for the rounding of this test itself), every
+// N in [N1 - R, N1 + R] rounds to hi2; hi2 2^e0 is then exact when its exponent is well inside the
+// normal range, so *t = hi2 2^e0 is the value the exact certification returns.  Otherwise false,
+// and the exact multi-word path decides.
+__host__ __device__ inline bool recon_certify_fast(const int32_t *Te, int64_t tstride, const int *binLevel,
+                                          int nb1, int Lmax, double rUp, int e0, double *t) {
+	double hi = 0.0, lo = 0.0, asum = 0.0;
+	for (int b = 0; b < nb1; b++) {
+		const double tb = ldexp((double)Te[(int64_t)b * tstride], 8 * (Lmax - binLevel[b]));
+		const double s = hi + tb;
+		const double bb = s - hi;
+		const double err = (hi - (s - bb)) + (tb - bb);
+		hi = s;
+		lo = lo + err;
+		asum = asum + fabs(tb);
+	}
+	const double h2 = hi + lo;
+	const double bb = h2 - hi;
+	const double l2 = (hi - (h2 - bb)) + (lo - bb);
+	if (h2 == 0.0) return false;
+	int ex;
+	const double f = frexp(h2, &ex);  // |f| in [1/2, 1)
+	if (ex + e0 < -1020 || ex + e0 > 1022) return false;
+	const double nb = (double)nb1;
+	const double delta = 4.0 * nb * nb * 0x1p-106 * asum * (1.0 + 0x1p-40);
+	const double half = ldexp(1.0, ex - ((fabs(f) == 0.5) ? 55 : 54));  // half the smaller spacing
+	if (fabs(l2) + delta + rUp >= half * (1.0 - 0x1p-20)) return false;
+	*t = ldexp(h2, e0);
+	return true;
+}
+
 template <int NW>
 __host__ __device__ inline double recon_elem(const int32_t *Te, int64_t tstride, const int *binLevel,
                                             int nbins, int Lmax, int e0, double alpha, double beta,
                                             double c) {
+	double t;
+	if (recon_certify_fast(Te, tstride, binLevel, nbins, Lmax, 0.0, e0, &t)) return fma(alpha, t, beta * c);
 	BigInt<NW> a;
 	big_zero<NW>(a);
 	recon_accumulate<NW>(a, Te, tstride, binLevel, 0, nbins, Lmax);
