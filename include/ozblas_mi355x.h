@@ -137,9 +137,9 @@ typedef struct ozblasHandle {
 	int32_t batchTile;    /* ozblasDgemmBatched: output tile edge picked by the autotuner (128, 256) */
 	int32_t batchReplay;  /* ozblasDgemmBatched: 1 when the call replayed the cached hipGraph */
 	int32_t batchGrouped; /* ozblasDgemmBatched: problems run in the grouped launch */
-	int32_t digitLevels;  /* INT8 form: digit levels computed before the certified rounding check;
+	int32_t digitLevels;  /* digit forms: digit levels computed before the certified rounding check;
 	                         0 = automatic (default), -1 = all levels, n > 0 = the first n levels */
-	int32_t fallbackTiles; /* INT8 form: output tiles of the last GEMM that needed the levels after
+	int32_t fallbackTiles; /* digit forms: output tiles of the last GEMM that needed the levels after
 	                          the check */
 	int32_t cgDeviceSlices; /* CG: 0 = iterate device-resident (default); -1 = iterate on the
 	                           synchronous path; n > 0 = give up the device path (and redo the

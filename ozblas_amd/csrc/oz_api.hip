@@ -472,8 +472,8 @@ static int32_t gemm_fast(ozblasHandle_t *oh, char transA, char transB, int m, in
 	// product is at most 2^14 in magnitude) and keeps the elements whose rounding of N1 +- R agrees;
 	// pass 2 computes the remaining levels only for the tiles holding the other elements.
 	int L1 = Lmax;
-	if (elem == 1 && oh->digitLevels > 0) L1 = imin(Lmax, oh->digitLevels - 1);
-	if (elem == 1 && oh->digitLevels == 0) {
+	if (oh->digitLevels > 0) L1 = imin(Lmax, oh->digitLevels - 1);
+	if (oh->digitLevels == 0) {
 		// automatic: the first level with R / 2^(8 Lmax) <= 2^-62, i.e. 10 bits below the binary64
 		// precision of an element of size 2^(8 Lmax) units (the product of the two scales)
 		std::vector<double> np(Lmax + 1, 0.0);

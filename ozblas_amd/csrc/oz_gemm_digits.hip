@@ -115,12 +115,13 @@ __global__ __launch_bounds__(256) void k_digit_gemm_h(int M, int N, const char *
                                                       int64_t ldbb, int64_t bpstrideb,
                                                       const int2 *pairs, const LevelBin *bins,
                                                       const int *binOrder, int32_t *T, int64_t ldt,
-                                                      int64_t tstride, int tilesM) {
+                                                      int64_t tstride, int tilesM, const int *tileList) {
 	__shared__ __attribute__((aligned(16))) char sA[BM * LDSROW];
 	__shared__ __attribute__((aligned(16))) char sB[BN * LDSROW];
 	const int binIdx = binOrder[blockIdx.y];
 	const LevelBin bin = bins[binIdx];
-	const int i0 = (blockIdx.x % tilesM) * BM, j0 = (blockIdx.x / tilesM) * BN;
+	const int tile = tileList ? tileList[blockIdx.x] : (int)blockIdx.x;  // a tile subset: fallback pass
+	const int i0 = (tile % tilesM) * BM, j0 = (tile / tilesM) * BN;
 	const int w = threadIdx.x / 64, l = threadIdx.x % 64;
 	const int wm = (w % 2) * 64, wn = (w / 2) * 64;
 	int32_t iacc[4][4][4];
@@ -187,7 +188,6 @@ void launch_digit_gemm(hipStream_t st, int elem, int M, int N, int Kpad, const v
                        const int *tileList, int ntiles) {
 	(void)Kpad;
 	const int tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
-	dim3 grid(tilesM * tilesN, nbins);
 	if (elem == 1) {
 		(void)ldbp;  // both operands share the padded k extent ldap
 		if (M >= 512 && N >= 512) {
@@ -203,14 +203,16 @@ void launch_digit_gemm(hipStream_t st, int elem, int M, int N, int Kpad, const v
 			                   0, st, (const int8_t *)Ap, apstride, (const int8_t *)Bp, bpstride, ldap,
 			                   pairs, bins, binOrder, T, ldt, tstride, tm, tileList);
 		}
-	} else if (elem == 2) {
-		hipLaunchKernelGGL(k_digit_gemm_h<h8v>, grid, dim3(256), 0, st, M, N, (const char *)Ap,
-		                   2 * ldap, 2 * apstride, (const char *)Bp, 2 * ldbp, 2 * bpstride, pairs,
-		                   bins, binOrder, T, ldt, tstride, tilesM);
 	} else {
-		hipLaunchKernelGGL(k_digit_gemm_h<b8v>, grid, dim3(256), 0, st, M, N, (const char *)Ap,
-		                   2 * ldap, 2 * apstride, (const char *)Bp, 2 * ldbp, 2 * bpstride, pairs,
-		                   bins, binOrder, T, ldt, tstride, tilesM);
+		const dim3 hgrid(tileList ? ntiles : tilesM * tilesN, nbins);
+		if (elem == 2)
+			hipLaunchKernelGGL(k_digit_gemm_h<h8v>, hgrid, dim3(256), 0, st, M, N, (const char *)Ap,
+			                   2 * ldap, 2 * apstride, (const char *)Bp, 2 * ldbp, 2 * bpstride, pairs,
+			                   bins, binOrder, T, ldt, tstride, tilesM, tileList);
+		else
+			hipLaunchKernelGGL(k_digit_gemm_h<b8v>, hgrid, dim3(256), 0, st, M, N, (const char *)Ap,
+			                   2 * ldap, 2 * apstride, (const char *)Bp, 2 * ldbp, 2 * bpstride, pairs,
+			                   bins, binOrder, T, ldt, tstride, tilesM, tileList);
 	}
 }
 
