@@ -116,8 +116,8 @@ __global__ __launch_bounds__(256) void k_digit_gemm_h(int M, int N, const char *
                                                       const int2 *pairs, const LevelBin *bins,
                                                       const int *binOrder, int32_t *T, int64_t ldt,
                                                       int64_t tstride, int tilesM, const int *tileList) {
-	__shared__ __attribute__((aligned(16))) char sA[BM * LDSROW];
-	__shared__ __attribute__((aligned(16))) char sB[BN * LDSROW];
+	__shared__ __attribute__((aligned(16))) char sA[2][BM * LDSROW];  // two stage buffers
+	__shared__ __attribute__((aligned(16))) char sB[2][BN * LDSROW];
 	const int binIdx = binOrder[blockIdx.y];
 	const LevelBin bin = bins[binIdx];
 	const int tile = tileList ? tileList[blockIdx.x] : (int)blockIdx.x;  // a tile subset: fallback pass
@@ -138,18 +138,23 @@ __global__ __launch_bounds__(256) void k_digit_gemm_h(int M, int N, const char *
 		const int2 st = pairs[bin.pair0 + p];
 		const char *Ap = A + (int64_t)st.x * apstrideb;
 		const char *Bp = B + (int64_t)st.y * bpstrideb;
+		// stage kb in buffer cur while the next one is stored into the other: one barrier a stage
 		Stage stg;
 		load_stage(stg, Ap, ldab, Bp, ldbb, i0, j0, M, N, kb0);
+		store_stage(stg, sA[0], sB[0]);
+		if (kb0 + BKB < kb1) load_stage(stg, Ap, ldab, Bp, ldbb, i0, j0, M, N, kb0 + BKB);
+		__syncthreads();
+		int cur = 0;
 		for (int kb = kb0; kb < kb1; kb += BKB) {
-			__syncthreads();
-			store_stage(stg, sA, sB);
-			__syncthreads();
-			if (kb + BKB < kb1) load_stage(stg, Ap, ldab, Bp, ldbb, i0, j0, M, N, kb + BKB);
 			V af[4], bf[4];
 			for (int mi = 0; mi < 4; mi++)
-				af[mi] = *(const V *)(sA + (wm + 16 * mi + l % 16) * LDSROW + 16 * (l / 16));
+				af[mi] = *(const V *)(sA[cur] + (wm + 16 * mi + l % 16) * LDSROW + 16 * (l / 16));
 			for (int nj = 0; nj < 4; nj++)
-				bf[nj] = *(const V *)(sB + (wn + 16 * nj + l % 16) * LDSROW + 16 * (l / 16));
+				bf[nj] = *(const V *)(sB[cur] + (wn + 16 * nj + l % 16) * LDSROW + 16 * (l / 16));
+			if (kb + BKB < kb1) {
+				store_stage(stg, sA[cur ^ 1], sB[cur ^ 1]);
+				if (kb + 2 * BKB < kb1) load_stage(stg, Ap, ldab, Bp, ldbb, i0, j0, M, N, kb + 2 * BKB);
+			}
 			for (int mi = 0; mi < 4; mi++)
 				for (int nj = 0; nj < 4; nj++) facc[mi][nj] = mfma16<V>(bf[nj], af[mi], facc[mi][nj]);
 			since += 32;
@@ -162,6 +167,8 @@ __global__ __launch_bounds__(256) void k_digit_gemm_h(int M, int N, const char *
 						}
 				since = 0;
 			}
+			__syncthreads();
+			cur ^= 1;
 		}
 	}
 	// D[p][q]: p = B side (column j) = 4*(l/16) + r, q = A side (row i) = l%16
