@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 INT8_PEAK_TOPS = 5000.0   # MI355X dense INT8 MFMA, TOPS
+HALF_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 / BF16 MFMA, TFLOPS
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 MFMA
 
 
@@ -339,11 +340,13 @@ def main():
     # 2 m n k per digit pair (nSplitC = number of pairs)
     ops = 2.0 * n * n * n * r["info_fast"]["nSplitC"]
     achieved = ops / r["tk_fast"] / 1e12
-    roof = {"bound": "mfma", "achieved": round(achieved, 1), "peak": INT8_PEAK_TOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / INT8_PEAK_TOPS, 4),
+    peak = INT8_PEAK_TOPS if args.form == "int8" else HALF_PEAK_TFLOPS
+    roof = {"bound": "mfma", "achieved": round(achieved, 1), "peak": peak,
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
             "traffic": measured_traffic(n, args.phi, args.form),
             "kernel": "k_digit_gemm_i8" if args.form == "int8" else "k_digit_gemm_h",
-            "note": "int8 ops (TOPS); ops per launch = 2*n^3*pairs"}
+            "note": (f"{args.form} MFMA ops ({'TOPS' if args.form == 'int8' else 'TFLOPS'}); "
+                     "ops per launch = 2*n^3*pairs")}
     cpu = cpu_baseline(args.cpu_n, args.phi) if args.cpu_n > 0 else None
     extra = {}
     for phi, rr in res.items():
