@@ -43,8 +43,14 @@ def parse():
     ap.add_argument("--phi", type=float, default=0.5)
     ap.add_argument("--phi2", type=float, default=4.0, help="second phi value (0 = skip)")
     ap.add_argument("--form", default="int8", choices=["int8", "fp16", "bf16"])
+    ap.add_argument("--scheme", default="auto", choices=["auto", "residues", "digits"],
+                    help="slices of the fast form: residues modulo small moduli or radix-256 digits")
+    ap.add_argument("--all-forms", type=int, default=1,
+                    help="also time INT8/FP16/BF16 x residues/digits at the first phi (0 = skip)")
+    ap.add_argument("--form-steps", type=int, default=2)
     ap.add_argument("--fp64-steps", type=int, default=2)
-    ap.add_argument("--cpu-n", type=int, default=2560, help="CPU baseline sample size (0 = skip)")
+    ap.add_argument("--cpu-n", type=int, default=4096, help="CPU baseline sample size (0 = skip)")
+    ap.add_argument("--cpu-reps", type=int, default=2, help="CPU baseline runs (the minimum is reported)")
     ap.add_argument("--cg-iters", type=int, default=500, help="CG iterations on 100^3 (0 = skip)")
     ap.add_argument("--batch", type=int, default=256, help="batched problems (0 = skip)")
     ap.add_argument("--level12", type=int, default=1, help="DGEMV / DDOT / DCSRMV rates (0 = skip)")
@@ -57,7 +63,7 @@ def parse():
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_digit_gemm_i8_traffic.md")
 
 
-def measured_traffic(n, phi, form):
+def measured_traffic(n, phi, form, scheme):
     """HBM bytes per launch of the dominant digit-kernel launch, as the committed PMC summary states
     them (FETCH_SIZE + WRITE_SIZE of that launch, each scaled by the same counter's reading for a
     plain copy of known size).  Counters cannot be read inside this timed run, so the figure comes
@@ -66,10 +72,10 @@ def measured_traffic(n, phi, form):
         text = open(TRAFFIC_PROFILE).read()
     except OSError:
         return None
-    m = re.search(r"^traffic: n=(\d+) phi=([0-9.]+) form=(\w+) bytes=(\d+)$", text, re.M)
-    if not m or int(m.group(1)) != n or float(m.group(2)) != phi or m.group(3) != form:
-        return None
-    return int(m.group(4))
+    for m in re.finditer(r"^traffic: n=(\d+) phi=([0-9.]+) form=(\w+) scheme=(\w+) bytes=(\d+)$", text, re.M):
+        if int(m.group(1)) == n and float(m.group(2)) == phi and m.group(3) == form and m.group(4) == scheme:
+            return int(m.group(5))
+    return None
 
 
 def copy_calibration(gib, torch):
@@ -98,23 +104,43 @@ def timed_gemm(H, n, dA, dB, dC, steps, warmup, torch):
                                                       t_sum=H.t_sum)
 
 
-def cpu_baseline(n_cpu, phi):
-    """Oracle restatement of the reference CPU OzBLAS (plain FP64 slices, NearSum) on this host."""
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n_cpu, phi, reps=2):
+    """Oracle restatement of the reference CPU OzBLAS (plain FP64 slices, NearSum) on this host's
+    cores, its slice products through OpenBLAS's cblas_dgemm as in the reference
+    (src/cpu/ozblas_blas_wrapper.cpp:158-168); the split and the sums use OpenMP over columns /
+    elements as the reference does.  The minimum over `reps` runs."""
     import numpy as np
     from oracle import oracle as O
+    blas = O.use_openblas(True)
     A = O.init_mat(n_cpu, n_cpu, n_cpu, phi, 2, 1)
     B = O.init_mat(n_cpu, n_cpu, n_cpu, phi, 2, 2)
     C = np.zeros(n_cpu * n_cpu)
-    h = O.Handle(1 << 34, sumMode=1, reproMode=1)
-    t0 = time.perf_counter()
-    h.gemm("N", "N", n_cpu, n_cpu, n_cpu, 1.0, A, n_cpu, B, n_cpu, 0.0, C, n_cpu)
-    t = time.perf_counter() - t0
+    h = O.Handle(1 << 35, sumMode=1, reproMode=1)
+    ts = []
+    for _ in range(max(1, reps)):
+        t0 = time.perf_counter()
+        h.gemm("N", "N", n_cpu, n_cpu, n_cpu, 1.0, A, n_cpu, B, n_cpu, 0.0, C, n_cpu)
+        ts.append(time.perf_counter() - t0)
+    O.use_openblas(False)
+    t = min(ts)
     cores = len(os.sched_getaffinity(0))
-    return {"value": 2.0 * n_cpu ** 3 / t / 1e12, "unit": "TFLOPS (2mnk/t)", "cores": cores,
+    return {"value": round(2.0 * n_cpu ** 3 / t / 1e12, 5), "unit": "TFLOPS (2mnk/t)", "cores": cores,
             "kind": "port",
-            "sample": f"one plain-form Ozaki DGEMM m=n=k={n_cpu}, phi={phi}, NearSum, degree auto "
-                      f"({h.nSplitA:.0f}x{h.nSplitB:.0f} FP64 slices), oracle C restatement with "
-                      f"OpenMP on {cores} threads, {t:.2f} s"}
+            "sample": f"plain-form Ozaki DGEMM m=n=k={n_cpu}, phi={phi}, NearSum, degree auto "
+                      f"({h.nSplitA:.0f}x{h.nSplitB:.0f} FP64 slices, {h.nSplitC:.0f} products): the oracle "
+                      f"C restatement, slice products through "
+                      f"{'OpenBLAS cblas_dgemm (' + os.path.basename(blas) + ')' if blas else 'its own loop'}, "
+                      f"OpenMP on {cores} threads of {cpu_model()}; min of {len(ts)} runs: {t:.2f} s"}
 
 
 def drand48_stream(seed):
@@ -266,7 +292,9 @@ def bench_batched(count, torch, oz):
            "eff_tflops": round(flops / tb / 1e12, 3),
            "grouped_mfma_kernel_ms": round(H.t_comp * 1e3, 3),
            "grouped_mfma_tops": round(H.n_comp / H.t_comp / 1e12, 1) if H.t_comp > 0 else None}
-    Hs = oz.Handle(8 << 30, sumMode=1, sliceType="int8")
+    # the same problems one plain-form (FP64 slices, NearSum) ozblasDgemm at a time: the bits the
+    # batch must reproduce
+    Hs = oz.Handle(8 << 30, sumMode=1, sliceType="fp64")
     Cl = [torch.zeros_like(c) for c in Cs]
     for rep in range(2):
         torch.cuda.synchronize()
@@ -275,13 +303,33 @@ def bench_batched(count, torch, oz):
             Hs.dgemm("N", "N", m, n, k, 1.0, As[p], m, Bs[p], k, 0.0, Cl[p], m)
         torch.cuda.synchronize()
         tl = time.perf_counter() - t0
-    out["loop_ms"] = round(tl * 1e3, 3)
-    out["speedup_vs_loop"] = round(tl / tb, 2)
-    out["bitwise_equal_to_individual_calls"] = all(
+    out["plain_form_loop_ms"] = round(tl * 1e3, 3)
+    out["speedup_vs_plain_form_loop"] = round(tl / tb, 2)
+    out["bitwise_equal_to_plain_form_solves"] = all(
         torch.equal(a.view(torch.int64), c.view(torch.int64)) for a, c in zip(Cs, Cl))
     H.close()
     Hs.close()
     return out
+
+
+def run_form(oz, work, form, scheme, n, dA, dB, steps, warmup, torch):
+    """Times one fast form; returns (seconds/step, MFMA-kernel seconds, info, output tensor)."""
+    dC = torch.zeros(n * n, dtype=torch.float64, device="cuda:0")
+    H = oz.Handle(work, sumMode=1, reproMode=1, sliceType=form, digitScheme=scheme)
+    t, tk, info = timed_gemm(H, n, dA, dB, dC, steps, warmup, torch)
+    info["scheme"] = {0: "fp64", 1: "digits", 2: "residues"}[int(H.schemeUsed)]
+    info["second_pass_tiles"] = int(H.fallbackTiles)
+    H.close()
+    return t, tk, info, dC
+
+
+def form_entry(n, t, tk, info, same):
+    flops = 2.0 * n ** 3
+    ops = flops * info["nSplitC"]
+    return {"scheme": info["scheme"], "eff_tflops": round(flops / t / 1e12, 3), "ms_per_step": round(t * 1e3, 2),
+            "products": info["nSplitC"], "mfma_kernel_ms": round(tk * 1e3, 2),
+            "mfma_kernel_rate": round(ops / tk / 1e12, 1), "split_ms": round(info["t_split"] * 1e3, 2),
+            "reconstruct_ms": round(info["t_sum"] * 1e3, 2), "bitwise_equal_to_fp64_slice_form": same}
 
 
 def main():
@@ -301,80 +349,91 @@ def main():
     n = args.n
     work = int(args.work_gb * (1 << 30))
     res = {}
-    outs = {}
+    forms = {}
+    eq = lambda a, b: bool(torch.equal(a.view(torch.int64), b.view(torch.int64)))
     for phi in [args.phi] + ([args.phi2] if args.phi2 else []):
         A = ozblas_amd.gen_phi(n, n, n, phi, 1)
         B = ozblas_amd.gen_phi(n, n, n, phi, 2)
         dA = torch.from_numpy(A).to("cuda:0")
         dB = torch.from_numpy(B).to("cuda:0")
         del A, B
-        dC = torch.zeros(n * n, dtype=torch.float64, device="cuda:0")
-        Hf = ozblas_amd.Handle(work, sumMode=1, reproMode=1, sliceType=args.form)
         steps = args.steps if phi == args.phi else max(1, args.steps // 2)
-        t_fast, tk_fast, info_fast = timed_gemm(Hf, n, dA, dB, dC, steps, args.warmup, torch)
-        fast_bits = dC.clone()
-        Hf.close()
-        dC.zero_()
+        t_fast, tk_fast, info_fast, C_fast = run_form(ozblas_amd, work, args.form, args.scheme, n, dA, dB,
+                                                      steps, args.warmup, torch)
         if args.fp64_steps > 0:
+            dC = torch.zeros(n * n, dtype=torch.float64, device="cuda:0")
             H64 = ozblas_amd.Handle(work, sumMode=1, reproMode=1, sliceType="fp64")
             t_64, tk_64, info_64 = timed_gemm(H64, n, dA, dB, dC, args.fp64_steps, 1, torch)
             H64.close()
-            same = bool(torch.equal(fast_bits.view(torch.int64), dC.view(torch.int64)))
+            same = eq(C_fast, dC)
         else:
+            dC = None
             t_64 = tk_64 = float("nan")
             info_64 = dict(nSplitA=0, nSplitB=0, nSplitC=0)
             same = None
         res[phi] = dict(t_fast=t_fast, tk_fast=tk_fast, info_fast=info_fast, t_64=t_64,
                         tk_64=tk_64, info_64=info_64, same=same, steps=steps)
-        print(f"phi={phi}: {args.form} {t_fast*1e3:.1f} ms/step (MFMA kernel {tk_fast*1e3:.1f} ms, "
-              f"{info_fast['nSplitA']:.0f}x{info_fast['nSplitB']:.0f} digit planes), fp64-slice "
+        print(f"phi={phi}: {args.form} ({info_fast['scheme']}) {t_fast*1e3:.1f} ms/step (MFMA kernel "
+              f"{tk_fast*1e3:.1f} ms, {info_fast['nSplitC']:.0f} products), fp64-slice "
               f"{t_64*1e3:.1f} ms ({info_64['nSplitA']:.0f}x{info_64['nSplitB']:.0f} slices), "
               f"bitwise equal: {same}", file=sys.stderr, flush=True)
-        del dA, dB, dC, fast_bits
+        del C_fast
+        if phi == args.phi and args.all_forms:
+            for form in ("int8", "fp16", "bf16"):
+                for scheme in ("residues", "digits"):
+                    if form == args.form and scheme == info_fast["scheme"]:
+                        continue
+                    t, tk, info, Cf = run_form(ozblas_amd, work, form, scheme, n, dA, dB, args.form_steps,
+                                               1, torch)
+                    forms[f"{form}_{scheme}"] = form_entry(n, t, tk, info, eq(Cf, dC) if dC is not None else None)
+                    print(f"  {form} {scheme}: {t*1e3:.1f} ms/step, {info['nSplitC']:.0f} products, "
+                          f"bitwise equal: {forms[f'{form}_{scheme}']['bitwise_equal_to_fp64_slice_form']}",
+                          file=sys.stderr, flush=True)
+                    del Cf
+        del dA, dB, dC
         torch.cuda.empty_cache()
 
     r = res[args.phi]
     flops = 2.0 * n ** 3
     value = flops / r["t_fast"] / 1e12
-    # dominant kernel: the digit-plane MFMA kernel; algorithmic int8 ops per launch =
-    # 2 m n k per digit pair (nSplitC = number of pairs)
+    # dominant kernel: the slice-product MFMA kernel (k_digit_gemm_i8 / k_digit_gemm_h, one launch);
+    # algorithmic ops per launch = 2 m n k per product (nSplitC = moduli, or digit pairs of pass 1)
     ops = 2.0 * n * n * n * r["info_fast"]["nSplitC"]
     achieved = ops / r["tk_fast"] / 1e12
     peak = INT8_PEAK_TOPS if args.form == "int8" else HALF_PEAK_TFLOPS
+    unit = "TOPS" if args.form == "int8" else "TFLOP/s"
     roof = {"bound": "mfma", "achieved": round(achieved, 1), "peak": peak,
-            "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
-            "traffic": measured_traffic(n, args.phi, args.form),
+            "unit": unit, "frac": round(achieved / peak, 4),
+            "traffic": measured_traffic(n, args.phi, args.form, r["info_fast"]["scheme"]),
+            "traffic_source": "profiles/ PMC summary of this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated); "
+                              "not measured in this run",
             "kernel": "k_digit_gemm_i8" if args.form == "int8" else "k_digit_gemm_h",
-            "note": (f"{args.form} MFMA ops ({'TOPS' if args.form == 'int8' else 'TFLOPS'}); "
-                     "ops per launch = 2*n^3*pairs")}
-    cpu = cpu_baseline(args.cpu_n, args.phi) if args.cpu_n > 0 else None
+            "note": (f"{args.form} MFMA ops; ops per launch = 2*n^3*products "
+                     f"({r['info_fast']['scheme']}: {r['info_fast']['nSplitC']:.0f} products)")}
+    cpu = cpu_baseline(args.cpu_n, args.phi, args.cpu_reps) if args.cpu_n > 0 else None
     extra = {}
     for phi, rr in res.items():
-        if rr["same"] is None:  # FP64-slice form not run (--fp64-steps 0)
-            extra[f"phi{phi:g}"] = {
-                f"{args.form}_tflops": round(flops / rr["t_fast"] / 1e12, 3),
-                "digit_planes": [rr["info_fast"]["nSplitA"], rr["info_fast"]["nSplitB"]],
-                "digit_pairs": rr["info_fast"]["nSplitC"],
-                f"{args.form}_mfma_kernel_ms": round(rr["tk_fast"] * 1e3, 2),
-                "split_ms": round(rr["info_fast"]["t_split"] * 1e3, 2),
-                "reconstruct_ms": round(rr["info_fast"]["t_sum"] * 1e3, 2),
-            }
-            continue
-        extra[f"phi{phi:g}"] = {
-            f"{args.form}_tflops": round(flops / rr["t_fast"] / 1e12, 3),
-            "fp64_slice_tflops": round(flops / rr["t_64"] / 1e12, 3),
-            "speedup_vs_fp64_slice": round(rr["t_64"] / rr["t_fast"], 2),
-            "bitwise_equal_to_fp64_slice_form": rr["same"],
-            "digit_planes": [rr["info_fast"]["nSplitA"], rr["info_fast"]["nSplitB"]],
-            "digit_pairs": rr["info_fast"]["nSplitC"],
-            "fp64_slices": [rr["info_64"]["nSplitA"], rr["info_64"]["nSplitB"]],
-            "fp64_mfma_kernel_ms": round(rr["tk_64"] * 1e3, 2),
-            "fp64_mfma_frac": round(2.0 * n ** 3 * rr["info_64"]["nSplitC"] / rr["tk_64"] / 1e12
-                                    / FP64_PEAK_TFLOPS, 4),
-            f"{args.form}_mfma_kernel_ms": round(rr["tk_fast"] * 1e3, 2),
-            "split_ms": round(rr["info_fast"]["t_split"] * 1e3, 2),
-            "reconstruct_ms": round(rr["info_fast"]["t_sum"] * 1e3, 2),
-        }
+        e = {f"{args.form}_tflops": round(flops / rr["t_fast"] / 1e12, 3),
+             "scheme": rr["info_fast"]["scheme"],
+             "products": rr["info_fast"]["nSplitC"],
+             f"{args.form}_mfma_kernel_ms": round(rr["tk_fast"] * 1e3, 2),
+             "split_ms": round(rr["info_fast"]["t_split"] * 1e3, 2),
+             "reconstruct_ms": round(rr["info_fast"]["t_sum"] * 1e3, 2)}
+        if rr["same"] is not None:
+            e.update({
+                "fp64_slice_tflops": round(flops / rr["t_64"] / 1e12, 3),
+                "speedup_vs_fp64_slice": round(rr["t_64"] / rr["t_fast"], 2),
+                "bitwise_equal_to_fp64_slice_form": rr["same"],
+                "fp64_slices": [rr["info_64"]["nSplitA"], rr["info_64"]["nSplitB"]],
+                "fp64_mfma_kernel_ms": round(rr["tk_64"] * 1e3, 2),
+                "fp64_mfma_frac": round(2.0 * n ** 3 * rr["info_64"]["nSplitC"] / rr["tk_64"] / 1e12
+                                        / FP64_PEAK_TFLOPS, 4)})
+        extra[f"phi{phi:g}"] = e
+    if forms:
+        t64 = res[args.phi]["t_64"]
+        for k2, v in forms.items():
+            v["speedup_vs_fp64_slice"] = round(t64 / (v["ms_per_step"] / 1e3), 2)
+        extra[f"forms_phi{args.phi:g}"] = forms
     if args.cg_iters > 0:
         extra["cg_laplace100"] = bench_cg(args.cg_iters, torch, ozblas_amd)
     if args.batch > 0:
@@ -388,7 +447,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": args.form, "data": "synthetic (reference phi-mode generator, seeds 1/2)",
         "config": {"workload": f"Ozaki DGEMM m=n=k={n}, phi={args.phi:g}, degree auto, NearSum, "
-                               f"{args.form} digit slices", "m": n, "n": n, "k": n,
+                               f"{args.form} slices ({r['info_fast']['scheme']})", "m": n, "n": n, "k": n,
                    "phi": args.phi, "sumMode": 1, "parallelism": "single GPU"},
         "roofline": roof, "cpu_baseline": cpu, "results": extra,
     }

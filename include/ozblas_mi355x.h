@@ -56,9 +56,12 @@ enum {
 };
 
 /* slice formats of the products (no reference counterpart).  FP64 is the reference's plain form.
- * INT8/FP16/BF16 cut every operand into exact 8-bit digit planes whose products are exact on
- * INT8 (INT32 accumulate) or FP16/BF16 (FP32 accumulate) MFMA, and round the exact sum once: they
- * give the bits of sumMode = 1 (NearSum) whenever the FP64 split is exact (nSplitMax = 0). */
+ * INT8/FP16/BF16 cut every operand into exact 8-bit slices whose products are exact on INT8
+ * (INT32 accumulate) or FP16/BF16 (FP32 accumulate) MFMA, and round the exact sum once: they give
+ * the bits of sumMode = 1 (NearSum) whenever the FP64 split is exact (nSplitMax = 0).  The slices
+ * are either residues of the row/column-scaled integers modulo small coprime moduli (one product
+ * per modulus, exact Chinese-remainder reconstruction) or radix-256 digit planes (one product per
+ * digit pair, certified early termination); digitScheme selects. */
 enum {
 	OZBLAS_SLICE_FP64 = 0,
 	OZBLAS_SLICE_INT8 = 1,
@@ -147,6 +150,11 @@ typedef struct ozblasHandle {
 	                           slices */
 	int32_t cgDevicePath;   /* last CG: 1 iterated device-resident, 2 gave up and redid the iterations
 	                           synchronously, 0 iterated synchronously */
+	int32_t digitScheme;    /* INT8/FP16/BF16 forms: 0 = automatic (default: residues, unless
+	                           digitLevels != 0 or the moduli cannot cover the product, then digits),
+	                           1 = radix-256 digit planes, 2 = residues modulo small coprime moduli
+	                           (digits when they cannot cover the product) */
+	int32_t schemeUsed;     /* last DGEMM: 0 = plain FP64 slices, 1 = digit planes, 2 = residues */
 } ozblasHandle_t;
 
 /* lifecycle: cuozblasCreate/cuozblasDestroy (include/cuozblas.h:67-68, src/cuda/cuozblas.cpp:23-75).
@@ -158,6 +166,18 @@ int32_t ozblasSetStream(ozblasHandle_t *oh, void *hipStream);
 /* level 1: cuozblasRdot / cuozblasRnrm2 / cuozblasRaxpy (include/cuozblas.h:73-75) */
 int32_t ozblasDdot(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx, const double *y,
                    int32_t incy, double *ret);
+/* ozblasDdot without allocation -- no counterpart in the reference.  For callers that cannot
+ * allocate device memory at call time (tight serving loops, harnesses that forbid hipMalloc): the
+ * same result bits as ozblasDdot, computed in a static 64 MiB scratch array that the library keeps
+ * on each GPU until the process ends (looked up per device at ozblasCreate), instead of the handle's
+ * workspace.  It calls no hipMalloc / hipFree.  Limit: both vectors' slot buffers plus the sums must
+ * fit the 64 MiB, about 16 * (cap + 2) * n bytes with cap the split cap (30 for nSplitMax = 0 and
+ * reproMode = 1), so n <= 130,688 for two distinct vectors at the defaults; beyond it the call
+ * returns OZBLAS_STATUS_WORKSPACE before any work.  Every call on one GPU shares
+ * that GPU's array: calls that overlap in time on one GPU (other host threads, other streams) race
+ * and must be serialised by the caller. */
+int32_t ozblasDdotNoAlloc(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx,
+                          const double *y, int32_t incy, double *ret);
 int32_t ozblasDnrm2(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx, double *ret);
 int32_t ozblasDaxpy(ozblasHandle_t *oh, int32_t n, double alpha, const double *x, int32_t incx,
                     double *y, int32_t incy);

@@ -49,6 +49,8 @@ def lib():
         H = C.POINTER(OzoHandle)
         L.ozo_handle_new.restype = H
         L.ozo_handle_new.argtypes = [C.c_uint64]
+        L.ozo_set_cblas_dgemm.restype = None
+        L.ozo_set_cblas_dgemm.argtypes = [C.c_void_p]
         L.ozo_handle_free.argtypes = [H]
         dp = C.POINTER(C.c_double)
         L.ozo_dd_dot2i.argtypes = [C.c_double, C.c_double, dp, dp]
@@ -165,6 +167,32 @@ def lib():
         L.ozo_laplace3d.argtypes = [C.c_int32, C.c_double, _ip, _ip, _dp]
         _lib = L
     return _lib
+
+
+def use_openblas(enable=True):
+    """Slice products of the binary64 oracle through OpenBLAS's cblas_dgemm (the LP64 build that
+    scipy bundles, entry scipy_cblas_dgemm), as the reference CPU OzBLAS does them; False restores
+    the built-in loop.  Returns the library path, or None when it is not found (the loop stays)."""
+    import glob
+    if not enable:
+        lib().ozo_set_cblas_dgemm(None)
+        return None
+    try:
+        import scipy
+    except ImportError:
+        return None
+    libs = glob.glob(os.path.join(os.path.dirname(os.path.dirname(scipy.__file__)), "scipy.libs",
+                                  "libscipy_openblas-*.so"))
+    if not libs:
+        return None
+    global _openblas
+    _openblas = C.CDLL(libs[0])
+    fn = _openblas.scipy_cblas_dgemm
+    lib().ozo_set_cblas_dgemm(C.cast(fn, C.c_void_p))
+    return libs[0]
+
+
+_openblas = None
 
 
 class Handle:

@@ -131,6 +131,8 @@ static double split_c_kernel(int32_t m, int32_t n, int32_t rho, const double *in
                              double *out, int32_t ldo, double *split, int32_t lds, double *max,
                              int32_t shift) {
 	double check = 0.;
+	/* OpenMP over the columns as the reference (:156); check is only compared with 0 */
+#pragma omp parallel for schedule(static) reduction(+ : check)
 	for (int32_t j = 0; j < n; j++) {
 		const int32_t tau = ozo_tau(max[j]);
 		const double sigma = sigma_of(rho, tau, shift);
@@ -302,6 +304,7 @@ static void global_nearsum(int32_t m, int32_t n, int32_t nA, int32_t nB, double 
 		for (int32_t ia = 0; ia < nA; ia++)
 			for (int32_t ib = 0; ib < nB; ib++)
 				if (ik == ia + ib) ic++;
+#pragma omp parallel for schedule(static)
 	for (int32_t y = 0; y < n; y++)
 		for (int32_t x = 0; x < m; x++) {
 			double t = ozo_nearsum(ic, &Cs[(int64_t)y * ldsc + x], (int32_t)llsc);
@@ -314,6 +317,7 @@ static void global_nearsum(int32_t m, int32_t n, int32_t nA, int32_t nB, double 
 static void global_fsum(int32_t m, int32_t n, int32_t nA, int32_t nB, const double *Cs, int64_t llsc,
                         int32_t ldsc, double *C, int32_t ldc, double alpha, double beta,
                         int32_t maxlevel, int32_t sumOrder) {
+#pragma omp parallel for schedule(static)
 	for (int32_t y = 0; y < n; y++)
 		for (int32_t x = 0; x < m; x++) {
 			double t = 0.;
@@ -352,9 +356,23 @@ static int32_t global_sum(ozo_handle *h, int32_t m, int32_t n, int32_t nA, int32
 /* ---------------------------------------------------------------------------------------------
  * slice-product backends (exact by construction, so any summation order gives the same bits)
  * ------------------------------------------------------------------------------------------- */
+/* An optional CBLAS dgemm (cblas_dgemm's signature) for the slice products, as the reference
+ * calls one (src/cpu/ozblas_blas_wrapper.cpp:158-168); NULL selects the loop below.  The products
+ * are exact, so the bits are the same either way: only the time differs. */
+typedef void (*ozo_cblas_dgemm_fn)(int order, int transA, int transB, int M, int N, int K,
+                                   double alpha, const double *A, int lda, const double *B, int ldb,
+                                   double beta, double *C, int ldc);
+static ozo_cblas_dgemm_fn g_cblas_dgemm = NULL;
+void ozo_set_cblas_dgemm(void *fn) { g_cblas_dgemm = (ozo_cblas_dgemm_fn)fn; }
+
 /* C(M x N, ldc) = A^T B with A (K x M, lda), B (K x N, ldb); beta = 0 (blasRgemm 't','n') */
 static void gemm_tn(int32_t M, int32_t N, int32_t K, const double *A, int32_t lda, const double *B,
                     int32_t ldb, double *C, int32_t ldc) {
+	if (g_cblas_dgemm && M > 0 && N > 0 && K > 0) {
+		g_cblas_dgemm(102 /* CblasColMajor */, 112 /* CblasTrans */, 111 /* CblasNoTrans */, M, N, K,
+		              1.0, A, lda, B, ldb, 0.0, C, ldc);
+		return;
+	}
 #pragma omp parallel for schedule(static)
 	for (int32_t j = 0; j < N; j++)
 		for (int32_t i = 0; i < M; i++) {

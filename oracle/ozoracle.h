@@ -80,6 +80,8 @@ typedef struct {
 void ozo_create(ozo_handle *h, uint64_t workSizeBytes);
 void ozo_destroy(ozo_handle *h);
 ozo_handle *ozo_handle_new(uint64_t workSizeBytes);
+/* slice products through a CBLAS dgemm (cblas_dgemm signature), or the built-in loop for NULL */
+void ozo_set_cblas_dgemm(void *fn);
 void ozo_handle_free(ozo_handle *h);
 
 /* building blocks */

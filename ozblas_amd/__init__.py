@@ -9,8 +9,11 @@ import fails.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libozblas_mi355x.so")
@@ -25,6 +28,8 @@ STATUS_NAMES = {0: "SUCCESS", 1: "NOT_INITIALIZED", 2: "ALLOC_FAILED", 3: "INVAL
                 4: "NOT_SUPPORTED", 5: "WORKSPACE", 6: "EXECUTION_FAILED"}
 SLICE_FP64, SLICE_INT8, SLICE_FP16, SLICE_BF16 = 0, 1, 2, 3
 SLICE_NAMES = {"fp64": SLICE_FP64, "int8": SLICE_INT8, "fp16": SLICE_FP16, "bf16": SLICE_BF16}
+SCHEME_AUTO, SCHEME_DIGITS, SCHEME_RESIDUES = 0, 1, 2
+SCHEME_NAMES = {"auto": SCHEME_AUTO, "digits": SCHEME_DIGITS, "residues": SCHEME_RESIDUES}
 
 
 class OzblasHandleT(C.Structure):
@@ -51,6 +56,7 @@ class OzblasHandleT(C.Structure):
         ("batchTile", C.c_int32), ("batchReplay", C.c_int32), ("batchGrouped", C.c_int32),
         ("digitLevels", C.c_int32), ("fallbackTiles", C.c_int32),
         ("cgDeviceSlices", C.c_int32), ("cgDevicePath", C.c_int32),
+        ("digitScheme", C.c_int32), ("schemeUsed", C.c_int32),
     ]
 
 
@@ -141,6 +147,27 @@ def gen_phi(m, n, ld, phi, seed):
     return buf
 
 
+# Library teardown is explicit: at interpreter exit (before the modules are cleared, and before
+# torch's own exit handlers when this package is imported after torch) the registered callbacks
+# run, then every live handle is destroyed.  A handle's destructor makes no HIP call once the
+# interpreter is finalising: by then the streams a handle was given (torch's) may be gone.
+_LIVE = weakref.WeakSet()
+_TEARDOWN = []
+
+
+def register_teardown(fn):
+    """Runs fn() at interpreter exit, before the live handles are destroyed."""
+    _TEARDOWN.append(fn)
+
+
+@atexit.register
+def _teardown():
+    while _TEARDOWN:
+        _TEARDOWN.pop()()
+    for h in list(_LIVE):
+        h.close()
+
+
 class Handle:
     """ozblasHandle_t wrapper: ``Handle(work_bytes, sumMode=1, sliceType='int8', ...)``."""
 
@@ -149,12 +176,15 @@ class Handle:
         st = _create(C.byref(self._h), int(work_bytes))
         if st != SUCCESS:
             raise OzblasError("ozblasCreate", st)
+        _LIVE.add(self)
         self.set(**flags)
 
     def set(self, **flags):
         for k, v in flags.items():
             if k == "sliceType" and isinstance(v, str):
                 v = SLICE_NAMES[v]
+            if k == "digitScheme" and isinstance(v, str):
+                v = SCHEME_NAMES[v]
             if not hasattr(self._h, k):
                 raise AttributeError(k)
             setattr(self._h, k, v)
@@ -179,6 +209,8 @@ class Handle:
             _destroy(self.ref)
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -12,18 +12,32 @@ from __future__ import annotations
 
 import torch
 
-from . import Handle
+from . import Handle, register_teardown
 
 _HANDLES = {}
 
 
+def _close_handles():
+    """Destroys the cached handles while HIP and torch are still up (run at interpreter exit)."""
+    while _HANDLES:
+        _, h = _HANDLES.popitem()
+        h.close()
+
+
+register_teardown(_close_handles)
+
+
 def _handle(form: str, work_bytes: int) -> Handle:
+    """The cached handle of (form, device), its work issued on torch's current stream: the stream
+    is set on every call, so the work is ordered after whatever produced the inputs there."""
     key = (form, torch.cuda.current_device())
     h = _HANDLES.get(key)
     if h is None or h.workSizeBytes < work_bytes:
+        if h is not None:
+            h.close()
         h = Handle(work_bytes, sumMode=1, reproMode=1, nSplitMax=0, sliceType=form)
-        h.set_stream(torch.cuda.current_stream().cuda_stream)
         _HANDLES[key] = h
+    h.set_stream(torch.cuda.current_stream().cuda_stream)
     return h
 
 

@@ -575,6 +575,173 @@ static int32_t gemm_fast(ozblasHandle_t *oh, char transA, char transB, int m, in
 }
 
 // ---------------------------------------------------------------------------------------------
+// GEMM, residue form (no reference counterpart; same bits as NearSum): INT8 (or FP16/BF16)
+// products of residues modulo small coprime moduli, exact Chinese-remainder reconstruction
+// (oz_residue.hip).  Returns with *handled = false when the moduli cannot cover the product (the
+// caller then takes the digit form or the plain form).
+// ---------------------------------------------------------------------------------------------
+struct ResOperand {
+	int *colE = nullptr, *colLo = nullptr;
+	ExpStatsH *dst = nullptr;
+	ExpStatsH st{};
+	void *planes = nullptr;
+	int64_t ldp = 0, pstride = 0;
+};
+
+static int32_t res_stats(ozblasHandle_t *oh, int k, int cols, const double *in, int64_t ldi,
+                         ResOperand &r) {
+	Backend *b = be(oh);
+	r.colE = (int *)walloc(oh, sizeof(int) * (size_t)cols * 3);
+	r.dst = (ExpStatsH *)walloc(oh, sizeof(ExpStatsH));
+	if (!r.colE || !r.dst) return OZBLAS_STATUS_WORKSPACE;
+	r.colLo = r.colE + 2 * (size_t)cols;
+	static const ExpStatsH init{0, INT32_MAX, INT32_MIN, 0, INT32_MIN, {0, 0, 0}};
+	OZ_HIP(hipMemcpyAsync(r.dst, &init, sizeof(init), hipMemcpyHostToDevice, b->stream));
+	launch_digit_stats(b->stream, k, cols, in, ldi, r.colE, r.colE + cols, r.dst, r.colLo);
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+static int32_t gemm_residue(ozblasHandle_t *oh, char transA, char transB, int m, int n, int k,
+                            double alpha, const double *A, int lda, const double *B, int ldb,
+                            double beta, double *C, int ldc, bool *handled) {
+	Backend *b = be(oh);
+	hipStream_t st = b->stream;
+	*handled = false;
+	const int elem = (oh->sliceType == OZBLAS_SLICE_INT8) ? 1 : (oh->sliceType == OZBLAS_SLICE_FP16 ? 2 : 3);
+	const uint64_t base = wmark(oh);
+	WallTimer tt;
+	const int64_t ldk = rup(k, 16);
+	const double *ain = A, *bin = B;
+	int64_t ldai = lda, ldbi = ldb;
+	OZ_HIP(hipEventRecord(b->ev[0], st));
+	if (is_n(transA)) {
+		double *tA = (double *)walloc(oh, 8ull * ldk * m);
+		if (!tA) return OZBLAS_STATUS_WORKSPACE;
+		launch_transpose(st, m, k, A, lda, tA, ldk);
+		ain = tA;
+		ldai = ldk;
+	}
+	if (!is_n(transB)) {
+		double *tB = (double *)walloc(oh, 8ull * ldk * n);
+		if (!tB) return OZBLAS_STATUS_WORKSPACE;
+		launch_transpose(st, n, k, B, ldb, tB, ldk);
+		bin = tB;
+		ldbi = ldk;
+	}
+	ResOperand ra, rb;
+	OZ_TRY(res_stats(oh, k, m, ain, ldai, ra));
+	OZ_TRY(res_stats(oh, k, n, bin, ldbi, rb));
+	OZ_HIP(hipMemcpyAsync(b->hPinned, ra.dst, sizeof(ExpStatsH), hipMemcpyDeviceToHost, st));
+	OZ_HIP(hipMemcpyAsync((char *)b->hPinned + sizeof(ExpStatsH), rb.dst, sizeof(ExpStatsH),
+	                      hipMemcpyDeviceToHost, st));
+	OZ_TRY(sync(oh));  // the one host round trip: the statistics decide the number of moduli
+	std::memcpy(&ra.st, b->hPinned, sizeof(ExpStatsH));
+	std::memcpy(&rb.st, (char *)b->hPinned + sizeof(ExpStatsH), sizeof(ExpStatsH));
+	auto give_up = [&]() {
+		wreset(oh, base);
+		return OZBLAS_STATUS_SUCCESS;
+	};
+	if (!fast_valid(oh, ra.st, rb.st, k)) return give_up();
+	if (ra.st.maxS == 0 || rb.st.maxS == 0) {  // an all-zero operand: C = fma(alpha, +0, beta C)
+		*handled = true;
+		int *lv = (int *)walloc(oh, sizeof(int));
+		if (!lv) return OZBLAS_STATUS_WORKSPACE;
+		launch_reconstruct(st, m, n, nullptr, 0, 0, lv, 0, 0, ra.colE, rb.colE, alpha, beta, C, ldc);
+		OZ_HIP(hipGetLastError());
+		OZ_TRY(sync(oh));
+		oh->nSplitA = oh->nSplitB = oh->nSplitC = 0;
+		oh->schemeUsed = 2;
+		wreset(oh, base);
+		return OZBLAS_STATUS_SUCCESS;
+	}
+	// |X| < k 2^(spanA + spanB) must stay within M / 4
+	int lk = 0;
+	while ((int64_t(1) << lk) < (int64_t)k) lk++;
+	const double need = (double)ra.st.maxSpan + (double)rb.st.maxSpan + lk + 2;
+	if (ra.st.maxSpan >= kResMaxShift || rb.st.maxSpan >= kResMaxShift) return give_up();
+	int nmod = 1;
+	while (nmod <= kResMaxMod && res_log2_product(nmod) < need + 1e-6) nmod++;
+	if (nmod > kResMaxMod) return give_up();
+	const ResConst &rc = res_const_host(nmod);
+	ResConst *dRc = (ResConst *)walloc(oh, sizeof(ResConst));
+	if (elem == 1) {
+		ra.ldp = rb.ldp = rup(k, 128);
+		ra.pstride = ra.ldp * rup(m, 256);
+		rb.pstride = rb.ldp * rup(n, 256);
+	} else {
+		ra.ldp = rb.ldp = rup(k, 64);
+		ra.pstride = ra.ldp * m;
+		rb.pstride = rb.ldp * n;
+	}
+	const size_t esz = (elem == 1) ? 1 : 2;
+	ra.planes = walloc(oh, esz * (size_t)ra.pstride * nmod);
+	rb.planes = walloc(oh, esz * (size_t)rb.pstride * nmod);
+	const int Kpad = (int)ra.ldp;
+	const int kc = 65536;  // |acc| <= kc 2^14 < 2^31 per chunk
+	const int nchunk = (Kpad + kc - 1) / kc;
+	const int nbins = nmod * nchunk;
+	std::vector<int2> pairs(nmod);
+	std::vector<LevelBin> bins(nbins);
+	std::vector<int> order(nbins);
+	for (int t = 0; t < nmod; t++) {
+		pairs[t] = make_int2(t, t);
+		for (int c = 0; c < nchunk; c++) {
+			LevelBin &bn = bins[t * nchunk + c];
+			bn.level = t;
+			bn.npairs = 1;
+			bn.pair0 = t;
+			bn.k0 = c * kc;
+			bn.k1 = std::min(Kpad, (c + 1) * kc);
+		}
+	}
+	for (int q = 0; q < nbins; q++) order[q] = q;
+	const int64_t ldt = (elem == 1) ? rup(m, 256) : rup(m, 64);
+	const int64_t tstride = ldt * ((elem == 1) ? rup(n, 256) : n);
+	int2 *dPairs = (int2 *)walloc(oh, sizeof(int2) * pairs.size());
+	LevelBin *dBins = (LevelBin *)walloc(oh, sizeof(LevelBin) * bins.size());
+	int *dOrder = (int *)walloc(oh, sizeof(int) * order.size());
+	uint8_t *Rp = (uint8_t *)walloc(oh, (size_t)tstride * nbins);
+	if (!dRc || !ra.planes || !rb.planes || !dPairs || !dBins || !dOrder || !Rp) {
+		wreset(oh, base);
+		return OZBLAS_STATUS_WORKSPACE;
+	}
+	*handled = true;
+	OZ_HIP(hipMemcpyAsync(dRc, &rc, sizeof(ResConst), hipMemcpyHostToDevice, st));
+	OZ_HIP(hipMemcpyAsync(dPairs, pairs.data(), sizeof(int2) * pairs.size(), hipMemcpyHostToDevice, st));
+	OZ_HIP(hipMemcpyAsync(dBins, bins.data(), sizeof(LevelBin) * bins.size(), hipMemcpyHostToDevice, st));
+	OZ_HIP(hipMemcpyAsync(dOrder, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, st));
+	OZ_HIP(hipEventRecord(b->ev[1], st));
+	launch_res_write(st, elem, k, m, ain, ldai, ra.colLo, dRc, nmod, ra.planes, ra.ldp, ra.pstride);
+	OZ_HIP(hipEventRecord(b->ev[2], st));
+	launch_res_write(st, elem, k, n, bin, ldbi, rb.colLo, dRc, nmod, rb.planes, rb.ldp, rb.pstride);
+	OZ_HIP(hipEventRecord(b->ev[3], st));
+	launch_digit_gemm(st, elem, m, n, Kpad, ra.planes, ra.ldp, ra.pstride, rb.planes, rb.ldp,
+	                  rb.pstride, dPairs, dBins, nbins, dOrder, nullptr, ldt, tstride, nullptr, 0, dRc, Rp);
+	OZ_HIP(hipEventRecord(b->ev[4], st));
+	launch_res_recon(st, m, n, Rp, ldt, tstride, nchunk, dRc, nmod, rc.nl, ra.colLo, rb.colLo, alpha,
+	                 beta, C, ldc);
+	OZ_HIP(hipEventRecord(b->ev[5], st));
+	OZ_HIP(hipGetLastError());
+	OZ_TRY(sync(oh));
+	oh->fallbackTiles = 0;
+	oh->nLevelBins = nbins;
+	oh->schemeUsed = 2;
+	oh->t_SplitA = ev_sec(b->ev[0], b->ev[1]) + ev_sec(b->ev[1], b->ev[2]);  // transposes + stats + A
+	oh->t_SplitB = ev_sec(b->ev[2], b->ev[3]);
+	oh->t_comp = ev_sec(b->ev[3], b->ev[4]);  // the residue MFMA kernel alone
+	oh->t_sum = ev_sec(b->ev[4], b->ev[5]);
+	oh->nSplitA = (float)nmod;
+	oh->nSplitB = (float)nmod;
+	oh->nSplitC = (float)nmod;  // products: one per modulus
+	oh->mbk = m;
+	oh->nbk = n;
+	oh->t_total = tt.sec();
+	oh->n_comp = (float)(2.0 * m * n * (double)k * nmod);
+	wreset(oh, base);
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------------------------
 // vector split (ozblasSplit with one column) -> slots at stride lds
 // ---------------------------------------------------------------------------------------------
 struct VecSplit {
@@ -834,10 +1001,19 @@ extern "C" int32_t ozblasDgemm(ozblasHandle_t *oh, char transA, char transB, int
 	if (oh->reproMode == 0 && oh->nSplitMax == 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
 	if (k == 0)  // no slices to multiply: the reference's sum is +0
 		return oh->lastStatus = gemm_k0(oh, m, n, alpha, beta, C, ldc);
+	oh->schemeUsed = 0;
 	if (oh->sliceType != OZBLAS_SLICE_FP64) {
 		if (oh->sumMode == 1 && oh->nSplitMax == 0) {
 			bool handled = false;
+			// residues first (automatic unless digit levels are asked for, or forced), then digits
+			const bool residues = oh->digitScheme == 2 || (oh->digitScheme == 0 && oh->digitLevels == 0);
+			if (residues) {
+				int32_t s = gemm_residue(oh, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+				                         &handled);
+				if (s != OZBLAS_STATUS_SUCCESS || handled) return oh->lastStatus = s;
+			}
 			int32_t s = gemm_fast(oh, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, &handled);
+			if (handled && s == OZBLAS_STATUS_SUCCESS) oh->schemeUsed = 1;
 			if (s != OZBLAS_STATUS_SUCCESS || handled) return oh->lastStatus = s;
 		} else if (oh->sumMode != 1) {
 			return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
@@ -890,6 +1066,53 @@ extern "C" int32_t ozblasSgemv(ozblasHandle_t *oh, char transA, int32_t m, int32
 extern "C" int32_t ozblasDdot(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx,
                               const double *y, int32_t incy, double *ret) {
 	return dot_entry(oh, n, x, incx, y, incy, ret);
+}
+
+// Bytes the binary64 DOT takes from the workspace: the bump allocations of dot_impl / vec_split in
+// their order (256-byte aligned), with the Dot2 buffers at the largest slice counts.
+static uint64_t dot_workspace_bytes(const ozblasHandle_t *oh, int n, bool same) {
+	auto bump = [](uint64_t top, uint64_t bytes) { return ((top + 255) & ~uint64_t(255)) + bytes; };
+	const int cap = split_cap(oh);
+	const uint64_t lds = (uint64_t)rup(n, 32);
+	uint64_t t = 0;
+	for (int v = 0; v < (same ? 1 : 2); v++) {
+		t = bump(t, 8ull * lds * (cap + 2));  // slices
+		t = bump(t, 8ull * (cap + 2));        // remainder maxima
+		t = bump(t, 16);                      // count
+		t = bump(t, vsplit_scratch_bytes());
+		t = bump(t, sizeof(VsPlan));
+	}
+	t = bump(t, 8);  // result
+	if (oh->splitEpsMode == 2) {
+		t = bump(t, 16ull * (cap + 2) * (cap + 2));
+		t = bump(t, dot2_partials_bytes(cap + 2, cap + 2));
+	} else {
+		t = bump(t, 8ull * (cap + 1) * (cap + 1));
+		t = bump(t, dots_part_bytes());
+	}
+	return t;
+}
+
+// ozblasDdot with the library's static scratch array as its workspace: no allocation on any path.
+// The handle's own workspace is swapped out for the call and restored after it.
+extern "C" int32_t ozblasDdotNoAlloc(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx,
+                                     const double *y, int32_t incy, double *ret) {
+	OZ_TRY(ensure_init(oh));
+	Backend *b = be(oh);
+	if (b->scratch == nullptr) return oh->lastStatus = OZBLAS_STATUS_NOT_INITIALIZED;
+	if (n > 0 && dot_workspace_bytes(oh, n, x == y) > kStaticScratchBytes)
+		return oh->lastStatus = OZBLAS_STATUS_WORKSPACE;  // beyond the static scratch array
+	char *work = b->work;
+	const uint64_t cap = b->cap, top = b->top, memAddr = oh->memAddr;
+	b->work = b->scratch;
+	b->cap = kStaticScratchBytes;
+	b->top = 0;
+	const int32_t st = dot_entry(oh, n, x, incx, y, incy, ret);
+	b->work = work;
+	b->cap = cap;
+	b->top = top;
+	oh->memAddr = memAddr;
+	return st;
 }
 
 extern "C" int32_t ozblasSdot(ozblasHandle_t *oh, int32_t n, const float *x, int32_t incx,

@@ -202,6 +202,7 @@ __global__ __launch_bounds__(256, 1) void k_gemm_grouped(const GProb *P, const G
 		const LevelBin bin = p.bins[w.bin];
 		const int2 st0 = p.pairs[bin.pair0];
 		I8Tile g;
+		g.R = nullptr;
 		g.A = p.A;
 		g.B = p.B;
 		g.apstride = p.apstride;

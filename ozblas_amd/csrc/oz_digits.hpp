@@ -79,7 +79,7 @@ __device__ inline void col_acc_flush(const ColAcc &a, ExpStats *st) {
 // before they are used.
 template <int BS>
 __device__ inline void digit_col_stats_acc(const double *x, int m, int *colE, int *colS, ColAcc &acc,
-                                           int *red, double *redd) {
+                                           int *red, double *redd, int *colLo = nullptr) {
 	int nlo = INT32_MIN, bad = 0;  // nlo = -(min lsb)
 	double mx = 0.0;
 	for (int i0 = threadIdx.x; i0 < m; i0 += 8 * BS) {
@@ -114,6 +114,7 @@ __device__ inline void digit_col_stats_acc(const double *x, int m, int *colE, in
 		}
 		*colE = E;
 		*colS = S;
+		if (colLo) *colLo = (mx > 0.0) ? -nlo : 0;  // exponent of the column's lowest set bit
 		acc.maxS = max(acc.maxS, S);
 		acc.maxE = max(acc.maxE, E);
 		if (bad) acc.bad = 1;
@@ -247,7 +248,19 @@ struct I8Tile {
 	int i0, j0;
 	int32_t *T;  // result plane of the bin
 	int64_t ldt;
+	uint8_t *R;  // residue form: result plane of acc mod p in [0, p) (bytes, leading dimension ldt)
+	int p;       //   instead of T when R is set
+	double invp;  //   1 / p rounded
 };
+
+// v mod p in [0, p) for |v| < 2^31 and 2 <= p <= 256 (binary64 quotient estimate off by at most one)
+__device__ __forceinline__ int mod_small(int v, int p, double invp) {
+	const double q = floor((double)v * invp);
+	int r = v - (int)q * p;
+	r += (r < 0) ? p : 0;
+	r -= (r >= p) ? p : 0;
+	return r;
+}
 
 template <int TM, int TN>
 __device__ __forceinline__ void i8_tile(const I8Tile &g, char *stage0, char *stage1) {
@@ -398,6 +411,22 @@ __device__ __forceinline__ void i8_tile(const I8Tile &g, char *stage0, char *sta
 	// D[p][q]: p = B-side (column j) = 8*(r/4) + 4*(l/32) + r%4, q = A-side (row i) = l%32
 	int32_t *Tb = g.T;
 	const int64_t ldt = g.ldt;
+	if (g.R) {  // residue form: the plane of acc mod p
+		uint8_t *Rb = g.R;
+		const int p = g.p;
+		const double invp = g.invp;
+#pragma unroll
+		for (int mi = 0; mi < TM; mi++)
+#pragma unroll
+			for (int nj = 0; nj < TN; nj++)
+#pragma unroll
+				for (int r = 0; r < 16; r++) {
+					const int i = i0 + wm + 32 * mi + l % 32;
+					const int j = j0 + wn + 32 * nj + 8 * (r / 4) + 4 * (l / 32) + r % 4;
+					Rb[(int64_t)j * ldt + i] = (uint8_t)mod_small(acc[mi][nj][r], p, invp);
+				}
+		return;
+	}
 #pragma unroll
 	for (int mi = 0; mi < TM; mi++)
 #pragma unroll

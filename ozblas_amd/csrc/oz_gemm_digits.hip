@@ -70,7 +70,8 @@ __global__ __launch_bounds__(256, 1) void k_digit_gemm_i8(const int8_t *A, int64
                                                           int64_t ldp, const int2 *pairs,
                                                           const LevelBin *bins, const int *binOrder,
                                                           int32_t *T, int64_t ldt, int64_t tstride,
-                                                          int tilesM, const int *tileList) {
+                                                          int tilesM, const int *tileList,
+                                                          const ResConst *rc, uint8_t *R) {
 	constexpr int TBM = 64 * TM, TBN = 64 * TN;
 	__shared__ __attribute__((aligned(16))) char stage0[(TBM + TBN) * I8_BK];
 	__shared__ __attribute__((aligned(16))) char stage1[(TBM + TBN) * I8_BK];
@@ -93,6 +94,12 @@ __global__ __launch_bounds__(256, 1) void k_digit_gemm_i8(const int8_t *A, int64
 	g.j0 = (tile / tilesM) * TBN;
 	g.T = T + (int64_t)binIdx * tstride;
 	g.ldt = ldt;
+	g.R = nullptr;
+	if (R) {  // residue form: bin = (modulus bin.level, k chunk), plane of acc mod p
+		g.R = R + (int64_t)binIdx * tstride;
+		g.p = rc->p[bin.level];
+		g.invp = rc->invpd[bin.level];
+	}
 	i8_tile<TM, TN>(g, stage0, stage1);
 }
 
@@ -115,7 +122,8 @@ __global__ __launch_bounds__(256) void k_digit_gemm_h(int M, int N, const char *
                                                       int64_t ldbb, int64_t bpstrideb,
                                                       const int2 *pairs, const LevelBin *bins,
                                                       const int *binOrder, int32_t *T, int64_t ldt,
-                                                      int64_t tstride, int tilesM, const int *tileList) {
+                                                      int64_t tstride, int tilesM, const int *tileList,
+                                                      const ResConst *rc, uint8_t *R) {
 	__shared__ __attribute__((aligned(16))) char sA[2][BM * LDSROW];  // two stage buffers
 	__shared__ __attribute__((aligned(16))) char sB[2][BN * LDSROW];
 	const int binIdx = binOrder[blockIdx.y];
@@ -172,6 +180,20 @@ __global__ __launch_bounds__(256) void k_digit_gemm_h(int M, int N, const char *
 		}
 	}
 	// D[p][q]: p = B side (column j) = 4*(l/16) + r, q = A side (row i) = l%16
+	if (R) {  // residue form: bin = (modulus bin.level, k chunk), plane of acc mod p
+		uint8_t *Rb = R + (int64_t)binIdx * tstride;
+		const int p = rc->p[bin.level];
+		const double invp = rc->invpd[bin.level];
+		for (int mi = 0; mi < 4; mi++)
+			for (int nj = 0; nj < 4; nj++)
+				for (int r = 0; r < 4; r++) {
+					const int i = i0 + wm + 16 * mi + l % 16;
+					const int j = j0 + wn + 16 * nj + 4 * (l / 16) + r;
+					const int32_t v = iacc[mi][nj][r] + (int32_t)facc[mi][nj][r];
+					if (i < M && j < N) Rb[(int64_t)j * ldt + i] = (uint8_t)mod_small(v, p, invp);
+				}
+		return;
+	}
 	int32_t *Tb = T + (int64_t)binIdx * tstride;
 	for (int mi = 0; mi < 4; mi++)
 		for (int nj = 0; nj < 4; nj++)
@@ -192,7 +214,7 @@ void launch_digit_gemm(hipStream_t st, int elem, int M, int N, int Kpad, const v
                        int64_t ldap, int64_t apstride, const void *Bp, int64_t ldbp,
                        int64_t bpstride, const int2 *pairs, const LevelBin *bins, int nbins,
                        const int *binOrder, int32_t *T, int64_t ldt, int64_t tstride,
-                       const int *tileList, int ntiles) {
+                       const int *tileList, int ntiles, const ResConst *rc, uint8_t *R) {
 	(void)Kpad;
 	const int tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
 	if (elem == 1) {
@@ -202,24 +224,24 @@ void launch_digit_gemm(hipStream_t st, int elem, int M, int N, int Kpad, const v
 			const int tm = (M + 64 * TM - 1) / (64 * TM), tn = (N + 64 * TN - 1) / (64 * TN);
 			hipLaunchKernelGGL((k_digit_gemm_i8<TM, TN>), dim3(tileList ? ntiles : tm * tn, nbins), dim3(256),
 			                   0, st, (const int8_t *)Ap, apstride, (const int8_t *)Bp, bpstride, ldap,
-			                   pairs, bins, binOrder, T, ldt, tstride, tm, tileList);
+			                   pairs, bins, binOrder, T, ldt, tstride, tm, tileList, rc, R);
 		} else {
 			constexpr int TM = 2, TN = 2;
 			const int tm = (M + 64 * TM - 1) / (64 * TM), tn = (N + 64 * TN - 1) / (64 * TN);
 			hipLaunchKernelGGL((k_digit_gemm_i8<TM, TN>), dim3(tileList ? ntiles : tm * tn, nbins), dim3(256),
 			                   0, st, (const int8_t *)Ap, apstride, (const int8_t *)Bp, bpstride, ldap,
-			                   pairs, bins, binOrder, T, ldt, tstride, tm, tileList);
+			                   pairs, bins, binOrder, T, ldt, tstride, tm, tileList, rc, R);
 		}
 	} else {
 		const dim3 hgrid(tileList ? ntiles : tilesM * tilesN, nbins);
 		if (elem == 2)
 			hipLaunchKernelGGL(k_digit_gemm_h<h8v>, hgrid, dim3(256), 0, st, M, N, (const char *)Ap,
 			                   2 * ldap, 2 * apstride, (const char *)Bp, 2 * ldbp, 2 * bpstride, pairs,
-			                   bins, binOrder, T, ldt, tstride, tilesM, tileList);
+			                   bins, binOrder, T, ldt, tstride, tilesM, tileList, rc, R);
 		else
 			hipLaunchKernelGGL(k_digit_gemm_h<b8v>, hgrid, dim3(256), 0, st, M, N, (const char *)Ap,
 			                   2 * ldap, 2 * apstride, (const char *)Bp, 2 * ldbp, 2 * bpstride, pairs,
-			                   bins, binOrder, T, ldt, tstride, tilesM, tileList);
+			                   bins, binOrder, T, ldt, tstride, tilesM, tileList, rc, R);
 	}
 }
 

@@ -107,6 +107,29 @@ int rho_host(int dim, int splitEpsMode) {
 	return (int)r;
 }
 
+// Device memory of ozblasDdotNoAlloc: one copy per GPU, at its own address on each.  Host code
+// never passes `scratch` itself to a kernel (in host code the name is a host-side stand-in, not a
+// device address); it uses the address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) char scratch[kStaticScratchBytes];
+static std::mutex g_scratch_mu;
+static char *g_scratch_dev[64];
+
+char *static_scratch(int device) {
+	if (device < 0 || device >= 64) return nullptr;
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	if (g_scratch_dev[device] == nullptr) {
+		void *ptr = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || ptr == nullptr) {
+			fprintf(stderr, "ozblas_mi355x: hipGetSymbolAddress(scratch) failed on device %d: %s\n",
+			        device, hipGetErrorString(e));
+			return nullptr;
+		}
+		g_scratch_dev[device] = (char *)ptr;
+	}
+	return g_scratch_dev[device];
+}
+
 int32_t ensure_init(ozblasHandle_t *oh) {
 	if (oh == nullptr || oh->initialized != 1 || oh->backend == nullptr)
 		return OZBLAS_STATUS_NOT_INITIALIZED;
@@ -142,6 +165,7 @@ extern "C" int32_t ozblasCreate(ozblasHandle_t *oh, uint64_t workSizeBytes) {
 	OZ_HIP(hipHostMalloc(&b->hPinned, 64 * sizeof(int64_t), hipHostMallocDefault));
 	OZ_HIP(hipMalloc(&b->dCounters, 256 * sizeof(int32_t)));
 	for (auto &e : b->ev) OZ_HIP(hipEventCreate(&e));
+	b->scratch = static_scratch(b->device);  // with the handle's device current
 	oh->backend = b;
 	oh->stream = nullptr;
 	oh->devWork = b->work;

@@ -53,7 +53,14 @@ struct Backend {
 	uint16_t *dTauF = nullptr;   // device copies of the binary32 tables JF, JF2
 	uint16_t *dTauF2 = nullptr;
 	int mixed = 0;  // 1 while a mixed (DS: binary64 data, binary32 slices) routine runs
+	char *scratch = nullptr;  // this device's copy of the static scratch array (ozblasDdotNoAlloc)
 };
+
+// The static scratch array of the allocation-free entry point: kStaticScratchBytes of device
+// memory per GPU, held until the process ends.  static_scratch() returns the current device's
+// address of it (looked up once per device), or nullptr on failure.
+constexpr uint64_t kStaticScratchBytes = 64ull << 20;
+char *static_scratch(int device);
 
 void batch_plan_free(void *plan);
 

@@ -37,7 +37,7 @@ void launch_split_csr(hipStream_t st, int m, const float *val, const int *rowptr
 void launch_zero_slots_csr(hipStream_t st, int m, const int *rowptr, double *split, int64_t lds,
                            const int *zrow, int count);
 void launch_digit_stats(hipStream_t st, int m, int n, const double *in, int64_t ldi, int *colE,
-                        int *colS, void *stats);
+                        int *colS, void *stats, int *colLo = nullptr);
 void launch_digit_write(hipStream_t st, int elem, int m, int n, const double *in, int64_t ldi,
                         const int *colE, int S, void *planes, int64_t ldp, int64_t pstride);
 
@@ -83,11 +83,42 @@ struct LevelBin {      // one exact int32 plane: sum over `npairs` digit pairs, 
 	int pair0;         // first entry in the pair list
 	int k0, k1;
 };
+// Residue form (oz_residue.hip): the moduli and the constants of the Chinese-remainder
+// reconstruction for a product of nmod moduli M = p_0 ... p_{nmod-1}.
+constexpr int kResMaxMod = 49;   // 256 and the 48 odd pairwise coprime moduli up to 255
+constexpr int kResMaxLimb = 18;  // 32-bit limbs of M (<= 392 bits) plus headroom
+constexpr int kResMaxShift = 512;  // element shifts below 2^512 (256^a mod p for a < 64)
+struct ResConst {
+	int nmod;          // moduli in use
+	int nl;            // 32-bit limbs carried by the reconstruction
+	int p[kResMaxMod];
+	float invpf[kResMaxMod];   // 1 / p rounded to binary32
+	double invpd[kResMaxMod];  // 1 / p rounded to binary64
+	int u[kResMaxMod];         // (M / p)^-1 mod p
+	uint32_t Mp[kResMaxMod][kResMaxLimb];  // M / p, little-endian limbs
+	uint32_t M[kResMaxLimb];
+	uint8_t pow256[kResMaxMod][kResMaxShift / 8];  // 256^a mod p
+};
+// the moduli list and per-count constants (host, built once); log2 of the product of the first n
+const ResConst &res_const_host(int nmod);
+double res_log2_product(int nmod);
+
 void launch_digit_gemm(hipStream_t st, int elem, int M, int N, int Kpad, const void *Ap,
                        int64_t ldap, int64_t apstride, const void *Bp, int64_t ldbp,
                        int64_t bpstride, const int2 *pairs, const LevelBin *bins, int nbins,
                        const int *binOrder, int32_t *T, int64_t ldt, int64_t tstride,
-                       const int *tileList = nullptr, int ntiles = 0);
+                       const int *tileList = nullptr, int ntiles = 0, const ResConst *rc = nullptr,
+                       uint8_t *R = nullptr);
+// residue planes of one operand (k-contiguous columns; INT8 tiled layout for elem 1, else
+// 2-byte k-contiguous planes), element x of column j as the integer x 2^-colLo[j] modulo each p
+void launch_res_write(hipStream_t st, int elem, int m, int n, const double *in, int64_t ldi,
+                      const int *colLo, const ResConst *rc, int nmod, void *planes, int64_t ldp,
+                      int64_t pstride);
+// C = fma(alpha, RN(X 2^(rowLo[i] + colLo[j])), beta C), X from the residue planes (nchunk k-chunk
+// planes per modulus, summed modulo p)
+void launch_res_recon(hipStream_t st, int M, int N, const uint8_t *R, int64_t ldr, int64_t rstride,
+                      int nchunk, const ResConst *rc, int nmod, int nl, const int *rowLo,
+                      const int *colLo, double alpha, double beta, double *C, int64_t ldc);
 int digit_gemm_tile(int elem, int M, int N);
 int recon_words(int Lmax);
 void launch_reconstruct_check(hipStream_t st, int M, int N, const int32_t *T, int64_t ldt,

@@ -228,8 +228,10 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned
 		for (int64_t i = i0 + kVsEC * stride; i < n; i += stride) add((double)x[i]);
 	}
 	__syncthreads();
-	// fold this workgroup's table over its range into the global one (Fg: zero between uses)
-	if (r0 <= r1) {  // an all-zero workgroup has the empty range [INT32_MAX, INT32_MIN]
+	// fold this workgroup's table over its range into the global one (Fg: zero between uses).  A
+	// workgroup holding an input outside the model folds nothing: it reports no range, so the walk
+	// would not clear what it folded, and the split takes the reference loop anyway.
+	if (r0 <= r1 && !badB) {  // an all-zero workgroup has the empty range [INT32_MAX, INT32_MIN]
 		for (int k = r0 + (int)threadIdx.x; k <= r1; k += BS) {
 			const unsigned long long f = Fb[k + kVsOff];
 			if (f) atomicMax(&Fg[k + kVsOff], f);

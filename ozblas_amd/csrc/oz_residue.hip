@@ -1,0 +1,347 @@
+// oz_residue.hip -- the residue form of the fast GEMM (no reference counterpart; same bits as
+// NearSum): the exact product from INT8 (or FP16/BF16) products of residues modulo small moduli.
+//
+// Scheme (Chinese remainder theorem, as in Ozaki, Uchino and Imamura's "Ozaki scheme II", used
+// here for the EXACT product).  Row i of op(A) is the integer vector a'_i = a_i 2^-lo_i, lo_i the
+// exponent of the lowest set bit in the row (every element is a multiple of 2^lo_i); column j of
+// op(B) likewise with lo_j.  X_ij = sum_k a'_ik b'_kj is then an integer with
+// |X| < k 2^(spanA + spanB) (span = top exponent - lo), and C_ij = X_ij 2^(lo_i + lo_j) exactly.
+// With pairwise coprime moduli p_t <= 256 whose product M >= 4 k 2^(spanA + spanB):
+//   - each operand element is reduced modulo every p_t to a balanced residue in [-128, 127]
+//     (exact in int8, fp16 and bf16);
+//   - one INT8 GEMM per modulus (int32 accumulation is exact below 2^31: k <= 2^17 per chunk)
+//     gives X mod p_t;
+//   - the reconstruction: c_t = (X mod p_t) (M/p_t)^-1 mod p_t, X = sum_t c_t M/p_t - K M with
+//     K = round(sum_t c_t / p_t) (exact, since |X| <= M/4 keeps the sum a quarter away from the
+//     rounding boundary), in a multi-word integer, then one RN-even of X 2^(lo_i + lo_j).
+// The number of GEMMs is the number of moduli, about (spanA + spanB + log2 k + 2) / 7.9, against
+// about (spanA / 8) (spanB / 8) / 2 digit-pair GEMMs of the radix-256 digit form.
+#include "oz_digits.hpp"
+
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+namespace oz {
+
+// ---- host: moduli and reconstruction constants --------------------------------------------
+static std::vector<int> res_moduli() {
+	// 256, then the odd numbers below it that are coprime with every one taken so far
+	std::vector<int> ms{256};
+	auto gcd = [](int a, int b) {
+		while (b) {
+			const int t = a % b;
+			a = b;
+			b = t;
+		}
+		return a;
+	};
+	for (int p = 255; p >= 3 && (int)ms.size() < kResMaxMod; p -= 2) {
+		bool ok = true;
+		for (int q : ms) ok = ok && gcd(p, q) == 1;
+		if (ok) ms.push_back(p);
+	}
+	return ms;
+}
+
+static ResConst g_rc[kResMaxMod + 1];
+static double g_log2M[kResMaxMod + 1];
+static std::once_flag g_rc_once;
+
+// little-endian 32-bit limb integers
+static void limb_mul_small(std::vector<uint32_t> &a, uint32_t s) {
+	uint64_t carry = 0;
+	for (uint32_t &w : a) {
+		const uint64_t v = (uint64_t)w * s + carry;
+		w = (uint32_t)v;
+		carry = v >> 32;
+	}
+	if (carry) a.push_back((uint32_t)carry);
+}
+static uint32_t limb_mod_small(const std::vector<uint32_t> &a, uint32_t p) {
+	uint64_t r = 0;
+	for (size_t q = a.size(); q-- > 0;) r = ((r << 32) | a[q]) % p;
+	return (uint32_t)r;
+}
+
+static void build_res_consts() {
+	const std::vector<int> ms = res_moduli();
+	const int avail = (int)ms.size();
+	for (int nmod = 1; nmod <= kResMaxMod; nmod++) {
+		ResConst &c = g_rc[nmod];
+		std::memset(&c, 0, sizeof(c));
+		if (nmod > avail) continue;
+		c.nmod = nmod;
+		double lg = 0.0;
+		std::vector<uint32_t> M{1};
+		for (int t = 0; t < nmod; t++) {
+			limb_mul_small(M, (uint32_t)ms[t]);
+			lg += std::log2((double)ms[t]);
+		}
+		g_log2M[nmod] = lg;
+		// limbs carried: X and sum_t c_t M/p_t < nmod M < 2^(bits(M) + 6), plus a sign bit
+		const int bitsM = 32 * (int)M.size();
+		c.nl = (bitsM + 8 + 31) / 32;
+		if (c.nl > kResMaxLimb) c.nl = kResMaxLimb;
+		for (size_t q = 0; q < M.size() && q < (size_t)kResMaxLimb; q++) c.M[q] = M[q];
+		for (int t = 0; t < nmod; t++) {
+			const int p = ms[t];
+			c.p[t] = p;
+			c.invpf[t] = 1.0f / (float)p;
+			c.invpd[t] = 1.0 / (double)p;
+			std::vector<uint32_t> Mp{1};
+			for (int s = 0; s < nmod; s++)
+				if (s != t) limb_mul_small(Mp, (uint32_t)ms[s]);
+			for (size_t q = 0; q < Mp.size() && q < (size_t)kResMaxLimb; q++) c.Mp[t][q] = Mp[q];
+			const uint32_t w = limb_mod_small(Mp, (uint32_t)p);
+			int u = 0;
+			for (int v = 1; v < p; v++)
+				if ((uint32_t)((w * (uint32_t)v) % (uint32_t)p) == 1u) {
+					u = v;
+					break;
+				}
+			c.u[t] = u;
+			uint32_t pw = 1 % (uint32_t)p;
+			for (int a = 0; a < kResMaxShift / 8; a++) {
+				c.pow256[t][a] = (uint8_t)pw;
+				pw = (pw * 256u) % (uint32_t)p;
+			}
+		}
+	}
+}
+
+const ResConst &res_const_host(int nmod) {
+	std::call_once(g_rc_once, build_res_consts);
+	return g_rc[nmod < 1 ? 1 : (nmod > kResMaxMod ? kResMaxMod : nmod)];
+}
+
+double res_log2_product(int nmod) {
+	std::call_once(g_rc_once, build_res_consts);
+	if (nmod < 1) return 0.0;
+	if (nmod > kResMaxMod) return 1e300;
+	return g_rc[nmod].nmod == nmod ? g_log2M[nmod] : 0.0;
+}
+
+// ---- device: residues of one element ------------------------------------------------------
+// x = +-f 2^e (f odd, or 0) relative to the column unit 2^lo: x 2^-lo = +-f 2^sh with sh >= 0
+struct ResElem {
+	double fd;  // f as a binary64 (exact: f < 2^53)
+	int sh;
+	int neg;
+};
+
+__device__ inline ResElem res_elem(double x, int lo) {
+	ResElem r{0.0, 0, 0};
+	if (x == 0.0) return r;
+	const uint64_t b = d2u(x);
+	const int E = (int)((b >> 52) & 0x7ff);
+	uint64_t f = b & ((uint64_t(1) << 52) - 1);
+	int e0;
+	if (E == 0) {
+		e0 = -1074;
+	} else {
+		f |= (uint64_t(1) << 52);
+		e0 = E - 1075;
+	}
+	const int tz = __builtin_ctzll(f);
+	f >>= tz;
+	r.fd = (double)f;
+	r.sh = e0 + tz - lo;  // in [0, span) by the choice of lo
+	r.neg = (int)(b >> 63);
+	return r;
+}
+
+// balanced residue of +-f 2^sh modulo p: in [-(p-1)/2, (p-1)/2] for odd p, [-128, 127] for 256
+__device__ __forceinline__ int res_of(const ResElem &e, int p, double invpd, float invpf,
+                                      const uint8_t *pw256) {
+	if (e.fd == 0.0) return 0;
+	double q = floor(e.fd * invpd);  // f mod p: quotient off by at most one
+	double r = fma(-q, (double)p, e.fd);
+	r += (r < 0.0) ? (double)p : 0.0;
+	r -= (r >= (double)p) ? (double)p : 0.0;
+	// (f mod p) 2^(sh mod 8) (256^(sh / 8) mod p) < 2^23: exact in binary32
+	const float v = (float)r * (float)(1 << (e.sh & 7)) * (float)pw256[e.sh >> 3];
+	const float q2 = floorf(v * invpf);
+	float r2 = fmaf(-q2, (float)p, v);
+	r2 += (r2 < 0.0f) ? (float)p : 0.0f;
+	r2 -= (r2 >= (float)p) ? (float)p : 0.0f;
+	int res = (int)r2;
+	if (e.neg && res) res = p - res;
+	const int thr = (p & 1) ? (p >> 1) : (p >> 1) - 1;
+	return res > thr ? res - p : res;
+}
+
+// INT8 tiled planes (the layout of the digit planes): one 256-thread workgroup writes 16 columns x
+// 64 k-bytes of every modulus plane per step (4 bytes per thread), looping over k tiles.
+__global__ __launch_bounds__(256) void k_res_write_i8t(int m, int n, const double *in, int64_t ldi,
+                                                       const int *colLo, const ResConst *rc, int nmod,
+                                                       int8_t *planes, int64_t ldp, int64_t pstride) {
+	__shared__ uint8_t pw[kResMaxMod][kResMaxShift / 8];
+	__shared__ int ps[kResMaxMod];
+	__shared__ float ipf[kResMaxMod];
+	__shared__ double ipd[kResMaxMod];
+	for (int q = threadIdx.x; q < nmod * (kResMaxShift / 8); q += 256)
+		pw[q / (kResMaxShift / 8)][q % (kResMaxShift / 8)] = rc->pow256[q / (kResMaxShift / 8)][q % (kResMaxShift / 8)];
+	for (int q = threadIdx.x; q < nmod; q += 256) {
+		ps[q] = rc->p[q];
+		ipf[q] = rc->invpf[q];
+		ipd[q] = rc->invpd[q];
+	}
+	__syncthreads();
+	const int t = threadIdx.x;
+	const int jb = blockIdx.y;
+	const int j = jb * 16 + t / 16;
+	const int lo = (j < n) ? colLo[j] : 0;
+	const double *x = in + (int64_t)(j < n ? j : 0) * ldi;
+	const int ktiles = (int)(ldp / 64);
+	for (int kt = blockIdx.x; kt < ktiles; kt += gridDim.x) {
+		const int q0 = kt * 64 + 4 * (t % 16);
+		ResElem e[4];
+#pragma unroll
+		for (int u = 0; u < 4; u++) e[u] = res_elem((j < n && q0 + u < m) ? x[q0 + u] : 0.0, lo);
+		int8_t *dst = planes + (int64_t)jb * 16 * ldp + (int64_t)kt * 1024 + 4 * t;
+		for (int s = 0; s < nmod; s++) {
+			uint32_t word = 0;
+#pragma unroll
+			for (int u = 0; u < 4; u++)
+				word |= (uint32_t)(uint8_t)(int8_t)res_of(e[u], ps[s], ipd[s], ipf[s], pw[s]) << (8 * u);
+			*(uint32_t *)(dst + (int64_t)s * pstride) = word;
+		}
+	}
+}
+
+// 2-byte planes (FP16 / BF16): plane s of column j at planes + s*pstride + j*ldp, rows [m, ldp) zero
+template <typename T>
+__global__ __launch_bounds__(256) void k_res_write_h(int m, int n, const double *in, int64_t ldi,
+                                                     const int *colLo, const ResConst *rc, int nmod,
+                                                     T *planes, int64_t ldp, int64_t pstride) {
+	__shared__ uint8_t pw[kResMaxMod][kResMaxShift / 8];
+	for (int q = threadIdx.x; q < nmod * (kResMaxShift / 8); q += 256)
+		pw[q / (kResMaxShift / 8)][q % (kResMaxShift / 8)] = rc->pow256[q / (kResMaxShift / 8)][q % (kResMaxShift / 8)];
+	__syncthreads();
+	const int j = blockIdx.y;
+	const double *x = in + (int64_t)j * ldi;
+	const int lo = colLo[j];
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ldp; i += gridDim.x * blockDim.x) {
+		const ResElem e = res_elem(i < m ? x[i] : 0.0, lo);
+		for (int s = 0; s < nmod; s++)
+			planes[(int64_t)s * pstride + (int64_t)j * ldp + i] =
+			    digit_cast<T>(res_of(e, rc->p[s], rc->invpd[s], rc->invpf[s], pw[s]));
+	}
+}
+
+void launch_res_write(hipStream_t st, int elem, int m, int n, const double *in, int64_t ldi,
+                      const int *colLo, const ResConst *rc, int nmod, void *planes, int64_t ldp,
+                      int64_t pstride) {
+	if (elem == 1) {  // tiled layout; ldp % 128 == 0 and pstride = ldp * (n rounded up to 256)
+		const int ktiles = (int)(ldp / 64);
+		const int gx = ktiles < 16 ? ktiles : 16;
+		hipLaunchKernelGGL(k_res_write_i8t, dim3((unsigned)gx, (unsigned)(pstride / ldp / 16)), dim3(256), 0,
+		                   st, m, n, in, ldi, colLo, rc, nmod, (int8_t *)planes, ldp, pstride);
+		return;
+	}
+	int gx = (int)((ldp + 255) / 256);
+	if (gx > 16) gx = 16;
+	if (elem == 2)
+		hipLaunchKernelGGL(k_res_write_h<_Float16>, dim3(gx, n), dim3(256), 0, st, m, n, in, ldi, colLo,
+		                   rc, nmod, (_Float16 *)planes, ldp, pstride);
+	else
+		hipLaunchKernelGGL(k_res_write_h<__bf16>, dim3(gx, n), dim3(256), 0, st, m, n, in, ldi, colLo, rc,
+		                   nmod, (__bf16 *)planes, ldp, pstride);
+}
+
+// ---- device: Chinese-remainder reconstruction ----------------------------------------------
+// v mod p for 0 <= v < 2^24 (binary32 quotient estimate off by at most one)
+__device__ __forceinline__ int mod_f(int v, int p, float invpf) {
+	const float q = floorf((float)v * invpf);
+	int r = v - (int)q * p;
+	r += (r < 0) ? p : 0;
+	r -= (r >= p) ? p : 0;
+	return r;
+}
+
+// X (NL 32-bit limbs, two's complement) of one element from its residues, then RN(X 2^e0)
+template <int NW>
+__device__ inline double res_elem_value(const uint8_t *Re, int64_t rstride, int nchunk,
+                                        const ResConst *rc, int nmod, int e0) {
+	constexpr int NL = 2 * NW;
+	int64_t acc[NL];
+#pragma unroll
+	for (int l = 0; l < NL; l++) acc[l] = 0;
+	double S = 0.0;
+	for (int t = 0; t < nmod; t++) {
+		const int p = rc->p[t];
+		const float ipf = rc->invpf[t];
+		int r = 0;
+		for (int c = 0; c < nchunk; c++) r += Re[(int64_t)(t * nchunk + c) * rstride];
+		if (nchunk > 1) r = mod_f(r, p, ipf);
+		const int ct = mod_f(r * rc->u[t], p, ipf);  // < 2^16 before the reduction
+		S += (double)ct * rc->invpd[t];
+#pragma unroll
+		for (int l = 0; l < NL; l++) acc[l] += (int64_t)((uint64_t)(uint32_t)ct * rc->Mp[t][l]);
+	}
+	const int64_t K = (int64_t)rint(S);
+#pragma unroll
+	for (int l = 0; l < NL; l++) acc[l] -= K * (int64_t)rc->M[l];
+	BigInt<NW> X;
+	int64_t carry = 0;
+#pragma unroll
+	for (int q = 0; q < NW; q++) {
+		const int64_t v0 = acc[2 * q] + carry;
+		const uint64_t w0 = (uint32_t)v0;
+		carry = v0 >> 32;
+		const int64_t v1 = acc[2 * q + 1] + carry;
+		const uint64_t w1 = (uint32_t)v1;
+		carry = v1 >> 32;
+		X.w[q] = w0 | (w1 << 32);
+	}
+	return big_finish<NW>(X, e0);
+}
+
+template <int NW>
+__global__ __launch_bounds__(256) void k_res_recon(int M, int N, const uint8_t *R, int64_t ldr,
+                                                   int64_t rstride, int nchunk, const ResConst *rc,
+                                                   int nmod, const int *rowLo, const int *colLo,
+                                                   double alpha, double beta, double *C, int64_t ldc) {
+	const int64_t total = (int64_t)M * N;
+	for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+	     e += (int64_t)gridDim.x * blockDim.x) {
+		const int i = (int)(e % M), j = (int)(e / M);
+		const double t = res_elem_value<NW>(R + (int64_t)j * ldr + i, rstride, nchunk, rc, nmod,
+		                                    rowLo[i] + colLo[j]);
+		double *c = C + (int64_t)j * ldc + i;
+		*c = fma(alpha, t, beta * *c);
+	}
+}
+
+void launch_res_recon(hipStream_t st, int M, int N, const uint8_t *R, int64_t ldr, int64_t rstride,
+                      int nchunk, const ResConst *rc, int nmod, int nl, const int *rowLo,
+                      const int *colLo, double alpha, double beta, double *C, int64_t ldc) {
+	int64_t g = ((int64_t)M * N + 255) / 256;
+	if (g > 16384) g = 16384;
+	const int nw = (nl + 1) / 2;
+	if (nw <= 3)
+		hipLaunchKernelGGL(k_res_recon<3>, dim3((unsigned)g), dim3(256), 0, st, M, N, R, ldr, rstride, nchunk,
+		                   rc, nmod, rowLo, colLo, alpha, beta, C, ldc);
+	else if (nw <= 4)
+		hipLaunchKernelGGL(k_res_recon<4>, dim3((unsigned)g), dim3(256), 0, st, M, N, R, ldr, rstride, nchunk,
+		                   rc, nmod, rowLo, colLo, alpha, beta, C, ldc);
+	else if (nw <= 6)
+		hipLaunchKernelGGL(k_res_recon<6>, dim3((unsigned)g), dim3(256), 0, st, M, N, R, ldr, rstride, nchunk,
+		                   rc, nmod, rowLo, colLo, alpha, beta, C, ldc);
+	else
+		hipLaunchKernelGGL(k_res_recon<9>, dim3((unsigned)g), dim3(256), 0, st, M, N, R, ldr, rstride, nchunk,
+		                   rc, nmod, rowLo, colLo, alpha, beta, C, ldc);
+}
+
+}  // namespace oz
+
+// test hooks: the host constants for nmod moduli (copied into out when it holds sizeof(ResConst)
+// bytes; returns sizeof(ResConst)) and log2 of their product
+extern "C" int32_t ozblas_internal_res_const(int32_t nmod, void *out, uint64_t bytes) {
+	const oz::ResConst &c = oz::res_const_host(nmod);
+	if (out && bytes >= sizeof(oz::ResConst)) std::memcpy(out, &c, sizeof(oz::ResConst));
+	return (int32_t)sizeof(oz::ResConst);
+}
+
+extern "C" double ozblas_internal_res_log2(int32_t nmod) { return oz::res_log2_product(nmod); }

@@ -147,3 +147,34 @@ def test_batched_other_forms_loop(oz, oracle):
             H2.dgemm(q["ta"], q["tb"], q["m"], q["n"], q["k"], q["alpha"], dev(q["A"]), q["lda"],
                      dev(q["B"]), q["ldb"], q["beta"], dc, q["ldc"])
             assert bits_equal(host(d), host(dc))
+
+
+def test_batched_config4_full_size(oz, oracle):
+    """The full batched workload: 256 problems with m, n, k = 64 + floor(961 drand48()) after
+    srand48(4), phi = 2 inputs (seeds 1000 + 2p, 1001 + 2p), alpha = 1, beta = 0, in one grouped
+    launch; every C[p] bitwise equal to its solo plain-form (FP64 slices, NearSum) ozblasDgemm, and
+    the three smallest problems equal to RN of the exact product."""
+    oracle.srand48(4)
+    sizes = [tuple(64 + int(961 * oracle.drand48()) for _ in range(3)) for _ in range(256)]
+    probs = []
+    for p, (m, n, k) in enumerate(sizes):
+        probs.append(dict(m=m, n=n, k=k, ta="N", tb="N", A=oz.gen_phi(m, k, m, 2.0, 1000 + 2 * p),
+                          B=oz.gen_phi(k, n, k, 2.0, 1001 + 2 * p), lda=m, ldb=k, ldc=m, alpha=1.0,
+                          beta=0.0, C0=np.zeros(m * n)))
+    say(f"batched config 4: {len(probs)} problems, sum 2mnk = {sum(2.0 * m * n * k for m, n, k in sizes):.3e}")
+    want = solo_plain(oz, probs)
+    H = oz.Handle(1 << 30, sumMode=1, sliceType="int8")
+    dA, dB, dC = run_batched(H, probs)
+    assert H.batchGrouped == len(probs)
+    check_all(probs, dC, want)
+    say("batched config 4: replay")
+    for q, d in zip(probs, dC):
+        d.zero_()
+    run_batched(H, probs, dA, dB, dC)
+    assert H.batchReplay == 1
+    check_all(probs, dC, want)
+    for p in sorted(range(len(sizes)), key=lambda q: sizes[q][0] * sizes[q][1] * sizes[q][2])[:3]:
+        q = probs[p]
+        ex = oracle.exact_gemm_rn("N", "N", q["m"], q["n"], q["k"], q["A"], q["m"], q["B"], q["k"])
+        got = host(dC[p]).reshape(q["n"], q["m"]).T
+        assert bits_equal(got, ex), (p, first_diff(got, ex))

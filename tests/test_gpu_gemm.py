@@ -5,6 +5,8 @@ for every transposition, summation mode (Fsum, NearSum), reproMode, fastMode, de
 below.  Fast forms (INT8/FP16/BF16 digit slices): bitwise identical to the plain form under
 NearSum, i.e. RN of the exact product, checked against the oracle's exact accumulator too.
 """
+import types
+
 import numpy as np
 import pytest
 
@@ -24,6 +26,10 @@ def _shape(trans, rows, cols):
     return (rows, cols) if trans in "Nn" else (cols, rows)
 
 
+_ORACLE_CACHE = {}
+_DEVICE_ONLY = ("digitLevels", "digitScheme")  # handle fields the oracle has no counterpart for
+
+
 def run_pair(oz, oracle, ta, tb, m, n, k, phi=1.0, seeds=(1, 2), alpha=1.0, beta=0.0, cinit=None,
              slice_type="fp64", ws=1 << 30, **flags):
     ar, ac = _shape(ta, m, k)
@@ -32,9 +38,18 @@ def run_pair(oz, oracle, ta, tb, m, n, k, phi=1.0, seeds=(1, 2), alpha=1.0, beta
     A = oracle.init_mat(ar, ac, lda, phi, 2, seeds[0])
     B = oracle.init_mat(br, bc, ldb, phi, 2, seeds[1])
     C0 = np.zeros(ldc * n) if cinit is None else cinit(ldc * n)
-    h = oracle.Handle(1 << 31, **flags)
-    Cw = C0.copy()
-    h.gemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cw, ldc)
+    oflags = {f: v for f, v in flags.items() if f not in _DEVICE_ONLY}
+    # the oracle's result of a case is computed once (the fast forms compare against the same one)
+    key = (ta, tb, m, n, k, phi, seeds, alpha, beta, tuple(sorted(oflags.items())),
+           None if cinit is None else C0.tobytes())
+    if key not in _ORACLE_CACHE:
+        ho = oracle.Handle(1 << 31, **oflags)
+        Cw = C0.copy()
+        ho.gemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, Cw, ldc)
+        _ORACLE_CACHE[key] = (Cw, types.SimpleNamespace(nSplitA=ho.nSplitA, nSplitB=ho.nSplitB))
+        ho.close()
+    Cw, h = _ORACLE_CACHE[key]
+    Cw = Cw.copy()
     H = oz.Handle(ws, sliceType=slice_type, **flags)
     dA, dB, dC = dev(A), dev(B), dev(C0)
     H.dgemm(ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc)
@@ -105,15 +120,44 @@ def test_plain_gemm_blocked_workspace(oz, oracle):
     assert bits_equal(got, want), first_diff(got, want)
 
 
+@pytest.mark.parametrize("scheme", ["digits", "residues"])
 @pytest.mark.parametrize("slice_type", ["int8", "fp16", "bf16"])
 @pytest.mark.parametrize("m,n,k,phi,trans", [(256, 256, 256, 1.0, "NN"), (77, 133, 199, 4.0, "TN"),
                                              (130, 129, 300, 0.5, "NT"), (200, 64, 1000, 2.0, "TT"),
-                                             (1, 1, 37, 1.0, "NN"), (64, 1, 300, 1.0, "NN")])
-def test_fast_forms_match_plain_nearsum(oz, oracle, slice_type, m, n, k, phi, trans):
-    say(f"{slice_type} dgemm {trans} m={m} n={n} k={k} phi={phi}")
+                                             (1, 1, 37, 1.0, "NN"), (64, 1, 300, 1.0, "NN"),
+                                             (600, 520, 700, 8.0, "NT")])
+def test_fast_forms_match_plain_nearsum(oz, oracle, slice_type, scheme, m, n, k, phi, trans):
+    say(f"{slice_type} {scheme} dgemm {trans} m={m} n={n} k={k} phi={phi}")
     got, want, H, _ = run_pair(oz, oracle, trans[0], trans[1], m, n, k, phi=phi, sumMode=1,
-                               slice_type=slice_type)
+                               slice_type=slice_type, digitScheme=scheme)
+    assert H.schemeUsed == (1 if scheme == "digits" else 2)
     assert bits_equal(got, want), first_diff(got, want)
+
+
+@pytest.mark.parametrize("slice_type", ["int8", "fp16", "bf16"])
+def test_residue_form_edges(oz, oracle, slice_type):
+    """Residue form: k over one int32 chunk (two k chunks summed modulo p), the most moduli, and
+    operands whose span the moduli cannot cover (the digit form takes over) -- each RN(exact)."""
+    rng = np.random.default_rng(31)
+    cases = []
+    m, n, k = 24, 40, 70000  # two k chunks of 65536
+    cases.append(("two chunks", m, n, k, rng.standard_normal(m * k), rng.standard_normal(k * n), 2))
+    m, n, k = 50, 30, 300  # spans near 130 bits per operand: about 36 moduli
+    A = rng.standard_normal(m * k) * np.exp2(rng.integers(-35, 35, m * k).astype(float))
+    B = rng.standard_normal(k * n) * np.exp2(rng.integers(-35, 35, k * n).astype(float))
+    cases.append(("wide", m, n, k, A, B, 2))
+    A2 = A.copy()
+    A2[::7] *= 2.0 ** 100  # spans beyond what the 49 moduli cover: digit planes
+    cases.append(("beyond the moduli", m, n, k, A2, B, 1))
+    for name, m, n, k, A, B, used in cases:
+        want = oracle.exact_gemm_rn("N", "N", m, n, k, A, m, B, k)
+        H = oz.Handle(1 << 30, sumMode=1, sliceType=slice_type, digitScheme="residues")
+        dC = dev(np.zeros(m * n))
+        say(f"residue edge {name} {slice_type} m={m} n={n} k={k}")
+        H.dgemm("N", "N", m, n, k, 1.0, dev(A), m, dev(B), k, 0.0, dC, m)
+        got = host(dC).reshape(n, m).T
+        assert H.schemeUsed == used, (name, H.schemeUsed)
+        assert bits_equal(got, want), (name, first_diff(got, want))
 
 
 def test_fast_form_alpha_beta_and_exact(oz, oracle):
@@ -144,12 +188,13 @@ def test_fast_form_edge_values(oz, oracle):
     B[10, :] = 2.0 ** -60
     Af, Bf = np.asfortranarray(A).T.ravel().copy(), np.asfortranarray(B).T.ravel().copy()
     want = oracle.exact_gemm_rn("N", "N", m, n, k, Af, m, Bf, k)
-    for st in ("int8", "fp16", "bf16", "fp64"):
-        H = oz.Handle(1 << 28, sumMode=1, sliceType=st)
+    for st, scheme in (("int8", "digits"), ("fp16", "digits"), ("bf16", "digits"), ("int8", "residues"),
+                       ("fp16", "residues"), ("bf16", "residues"), ("fp64", "auto")):
+        H = oz.Handle(1 << 28, sumMode=1, sliceType=st, digitScheme=scheme)
         dC = dev(np.zeros(m * n))
         H.dgemm("N", "N", m, n, k, 1.0, dev(Af), m, dev(Bf), k, 0.0, dC, m)
         got = host(dC).reshape(n, m).T
-        assert bits_equal(got, want), (st, first_diff(got, want))
+        assert bits_equal(got, want), (st, scheme, first_diff(got, want))
 
 
 @pytest.mark.parametrize("trans", ["N", "T"])
@@ -240,24 +285,28 @@ def test_plain_gemm_k0_and_empty(oz, oracle):
     assert bits_equal(host(C), np.full(12, 7.0))
 
 
+@pytest.mark.parametrize("slice_type", ["int8", "fp16", "bf16"])
 @pytest.mark.parametrize("levels", [1, 2, 4, 0, -1])
 @pytest.mark.parametrize("m,n,k,ta,tb", [(600, 520, 700, "N", "N"), (200, 130, 3000, "T", "N")])
-def test_int8_certified_levels_bitwise(oz, oracle, levels, m, n, k, ta, tb):
-    """INT8 form with certified early termination: the first `levels` digit levels, then the
+def test_digit_certified_levels_bitwise(oz, oracle, slice_type, levels, m, n, k, ta, tb):
+    """Digit forms with certified early termination: the first `levels` digit levels, then the
     rounding check and the remaining levels for the tiles that need them (levels 0: automatic,
     -1: every level in one pass) -- bitwise equal to the plain form in every case."""
-    say(f"int8 certified levels={levels} {ta}{tb} m={m} n={n} k={k}")
+    say(f"{slice_type} certified levels={levels} {ta}{tb} m={m} n={n} k={k}")
     want, _, _, _ = run_pair(oz, oracle, ta, tb, m, n, k, phi=1.0, sumMode=1, slice_type="fp64")
-    got, _, H, _ = run_pair(oz, oracle, ta, tb, m, n, k, phi=1.0, sumMode=1, slice_type="int8",
-                            digitLevels=levels)
+    got, _, H, _ = run_pair(oz, oracle, ta, tb, m, n, k, phi=1.0, sumMode=1, slice_type=slice_type,
+                            digitLevels=levels, digitScheme="digits")
+    assert H.schemeUsed == 1
     assert bits_equal(got, want), first_diff(got, want)
     if levels in (1, 2):
         assert H.fallbackTiles > 0  # the check cannot settle most elements from so few levels
 
 
-def test_int8_certified_cancellation(oz, oracle):
+@pytest.mark.parametrize("slice_type", ["int8", "fp16", "bf16"])
+def test_digit_certified_cancellation(oz, oracle, slice_type):
     """Exact cancellation (every C entry is 0 or tiny): the check cannot settle the sign from the
-    leading levels, so every tile takes the second pass; the result is still RN(exact)."""
+    leading levels, so every tile takes the second pass; the result is still RN(exact).  The
+    residue form is checked on the same product."""
     m, n, k = 300, 260, 512
     A = oracle.init_mat(m, k, m, 1.0, 2, 41).reshape(k, m).T.copy()
     B = oracle.init_mat(k, n, k, 1.0, 2, 42).reshape(n, k).T.copy()
@@ -267,10 +316,10 @@ def test_int8_certified_cancellation(oz, oracle):
     Af = np.asfortranarray(A).T.ravel().copy()
     Bf = np.asfortranarray(B).T.ravel().copy()
     ex = np.ascontiguousarray(np.asarray(oracle.exact_gemm_rn("N", "N", m, n, k, Af, m, Bf, k)).T).ravel()
-    for levels in (0, 6):
-        H = oz.Handle(1 << 30, sumMode=1, sliceType="int8", digitLevels=levels)
+    for levels, scheme in ((0, "digits"), (6, "digits"), (0, "residues")):
+        H = oz.Handle(1 << 30, sumMode=1, sliceType=slice_type, digitLevels=levels, digitScheme=scheme)
         dC = dev(np.zeros(m * n))
         H.dgemm("N", "N", m, n, k, 1.0, dev(Af), m, dev(Bf), k, 0.0, dC, m)
-        assert bits_equal(host(dC), ex), first_diff(host(dC), ex)
+        assert bits_equal(host(dC), ex), (levels, scheme, first_diff(host(dC), ex))
         if levels == 6:
             assert H.fallbackTiles == ((m + 127) // 128) * ((n + 127) // 128)  # every tile

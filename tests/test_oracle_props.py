@@ -244,3 +244,31 @@ def test_laplacian_structure(oracle):
         cols = colind[rowptr[r]:rowptr[r + 1]]
         assert list(cols) == sorted(cols)
         assert val[rowptr[r]:rowptr[r + 1]].sum() == pytest.approx(1e-3 + 6.0 - (len(cols) - 1))
+
+
+def test_openblas_slice_products_same_bits(oracle):
+    """The CPU baseline runs the oracle's slice products through OpenBLAS's cblas_dgemm, as the
+    reference does: the products are exact, so every sum mode and transposition keeps its bits."""
+    m, n, k = 150, 90, 300
+    A = oracle.init_mat(k, m, k, 2.0, 2, 1)
+    B = oracle.init_mat(k, n, k, 2.0, 2, 2)
+    outs = []
+    for use in (False, True):
+        if use and oracle.use_openblas(True) is None:
+            pytest.skip("scipy's OpenBLAS is not present")
+        try:
+            for sumMode in (0, 1):
+                for ta in "NT":
+                    h = oracle.Handle(1 << 30, sumMode=sumMode)
+                    C = np.zeros(m * n)
+                    if ta == "T":
+                        h.gemm("T", "N", m, n, k, 1.0, A, k, B, k, 0.0, C, m)
+                    else:
+                        At = A.reshape(m, k).T.copy().ravel()
+                        h.gemm("N", "N", m, n, k, 1.0, At, m, B, k, 0.0, C, m)
+                    outs.append((use, sumMode, ta, C))
+        finally:
+            oracle.use_openblas(False)
+    half = len(outs) // 2
+    for a, b in zip(outs[:half], outs[half:]):
+        assert a[3].view(np.uint64).tolist() == b[3].view(np.uint64).tolist(), a[1:3]
