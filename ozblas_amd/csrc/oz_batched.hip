@@ -24,7 +24,7 @@
 #include <cstring>
 #include <vector>
 
-#include "oz_digits.hpp"
+#include "oz_residue.hpp"
 #include "oz_internal.hpp"
 
 extern "C" int32_t ozblasDgemm(ozblasHandle_t *oh, char transA, char transB, int32_t m, int32_t n,
@@ -40,26 +40,26 @@ struct GTrans {  // out[r * ldo + c] = in[c * ldi + r], r < rows, c < cols
 	int64_t ldi, ldo;
 	int rows, cols, tilesR, pad;
 };
-struct GStat {  // digit statistics of `cols` k-contiguous columns of length m
+struct GStat {  // statistics of `cols` k-contiguous columns of length m
 	const double *in;
 	int64_t ldi;
 	int m, pad;
-	int *colE, *colS;
+	int *colE, *colS, *colLo;
 	ExpStats *st;
 };
-struct GWrite {  // tiled INT8 digit planes of one operand
+struct GWrite {  // tiled INT8 residue planes of one operand
 	const double *in;
 	int64_t ldi;
 	int m, n;
-	const int *colE;
+	const int *colLo;
 	int8_t *planes;
 	int64_t ldp, pstride;
-	int S, kts;
+	int nmod, kts;
 };
 struct GProb {  // one problem of the grouped MFMA launch
 	const int8_t *A, *B;
 	int64_t apstride, bpstride, ldp;
-	int32_t *T;
+	uint8_t *R;  // residue planes of the products, one per (modulus, k chunk)
 	int64_t ldt, tstride;
 	const LevelBin *bins;
 	const int2 *pairs;
@@ -68,19 +68,19 @@ struct GProb {  // one problem of the grouped MFMA launch
 struct GItem {
 	int prob, bin, tile, pad;
 };
-struct GRec {  // exact reconstruction + epilogue of one problem
-	const int32_t *T;
-	const int *binLevel;
-	const int *rowE, *colE;
+struct GRec {  // Chinese-remainder reconstruction + epilogue of one problem
+	const uint8_t *R;
+	const ResConst *rc;  // constants of the problem's moduli count
+	const int *rowLo, *colLo;
 	double *C;
 	int64_t ldt, tstride, ldc;
 	double alpha, beta;
-	int M, N, nbins, Lmax;
+	int M, N, nmod, nchunk;
 };
 struct GCheck {  // plan assumptions of one problem, re-checked on every execution
-	int opA, opB;  // operand indices
-	int SA, SB;    // digit counts the planes and bins were built for
-	int spanLimit, pad;
+	int opA, opB;    // operand indices
+	int spanBudget;  // largest spanA + spanB the problem's moduli cover (-1: planned as a zero product)
+	int spanLimit;   // largest span for which the plain split is exact (the fast-form precondition)
 };
 
 // largest o < n with start[o] <= b (start[] ascending, start[0] = 0)
@@ -152,7 +152,8 @@ __global__ __launch_bounds__(256) void k_stats_grouped(const GStat *g, const int
 			s = g[o];
 		}
 		const int j = c - start[o];
-		digit_col_stats_acc<256>(s.in + (int64_t)j * s.ldi, s.m, s.colE + j, s.colS + j, acc, red, redd);
+		digit_col_stats_acc<256>(s.in + (int64_t)j * s.ldi, s.m, s.colE + j, s.colS + j, acc, red, redd,
+		                         s.colLo + j);
 	}
 	if (cur >= 0 && threadIdx.x == 0) col_acc_flush(acc, s.st);
 }
@@ -162,8 +163,10 @@ __global__ void k_check_grouped(const GCheck *c, int n, const ExpStats *st, int 
 	for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
 		const GCheck q = c[p];
 		const ExpStats a = st[q.opA], b = st[q.opB];
-		bool bad = a.nonfinite || b.nonfinite || a.maxS > q.SA || b.maxS > q.SB;
+		bool bad = a.nonfinite || b.nonfinite;
 		if (a.maxS > 0 && b.maxS > 0) {
+			bad = bad || q.spanBudget < 0 || a.maxSpan + b.maxSpan > q.spanBudget;
+			bad = bad || a.maxSpan >= kResMaxShift || b.maxSpan >= kResMaxShift;
 			bad = bad || a.maxSpan > q.spanLimit || b.maxSpan > q.spanLimit;
 			bad = bad || a.minLsb + b.minLsb < -1000 || a.maxE + b.maxE > 1000;
 			bad = bad || a.minLsb < -1000 || b.minLsb < -1000;
@@ -172,19 +175,24 @@ __global__ void k_check_grouped(const GCheck *c, int n, const ExpStats *st, int 
 	}
 }
 
-__global__ __launch_bounds__(256) void k_write_grouped(const GWrite *g, const int *start, int n) {
+__global__ __launch_bounds__(256) void k_write_grouped(const GWrite *g, const int *start, int n,
+                                                       const ResConst *rc) {
+	__shared__ ResLds T;
+	res_lds_load(T, rc, kResMaxMod);
+	__syncthreads();
 	const int o = owner(start, n, blockIdx.x);
 	const GWrite w = g[o];
 	const int lb = blockIdx.x - start[o];
-	digit_write_tile_i8(w.m, w.n, w.in, w.ldi, w.colE, w.S, w.planes, w.ldp, w.pstride, lb % w.kts,
-	                    lb / w.kts);
+	res_write_tile_i8(w.m, w.n, w.in, w.ldi, w.colLo, w.nmod, T, w.planes, w.ldp, w.pstride, lb % w.kts,
+	                  lb / w.kts);
 }
 
 // Persistent grouped MFMA launch: every workgroup takes the next (problem, bin, tile) item from
 // the counter until the list is exhausted.  Items are sorted by decreasing cost.
 template <int TM, int TN>
 __global__ __launch_bounds__(256, 1) void k_gemm_grouped(const GProb *P, const GItem *items,
-                                                         int nitems, int *counter, const int *flag) {
+                                                         int nitems, int *counter, const int *flag,
+                                                         const ResConst *rc) {
 	constexpr int TBM = 64 * TM, TBN = 64 * TN;
 	__shared__ __attribute__((aligned(16))) char stage0[(TBM + TBN) * I8_BK];
 	__shared__ __attribute__((aligned(16))) char stage1[(TBM + TBN) * I8_BK];
@@ -202,7 +210,6 @@ __global__ __launch_bounds__(256, 1) void k_gemm_grouped(const GProb *P, const G
 		const LevelBin bin = p.bins[w.bin];
 		const int2 st0 = p.pairs[bin.pair0];
 		I8Tile g;
-		g.R = nullptr;
 		g.A = p.A;
 		g.B = p.B;
 		g.apstride = p.apstride;
@@ -215,8 +222,11 @@ __global__ __launch_bounds__(256, 1) void k_gemm_grouped(const GProb *P, const G
 		g.k1 = bin.k1;
 		g.i0 = (w.tile % p.tilesM) * TBM;
 		g.j0 = (w.tile / p.tilesM) * TBN;
-		g.T = p.T + (int64_t)w.bin * p.tstride;
+		g.T = nullptr;
 		g.ldt = p.ldt;
+		g.R = p.R + (int64_t)w.bin * p.tstride;  // bin = (modulus bin.level, k chunk)
+		g.p = rc->p[bin.level];
+		g.invp = rc->invpd[bin.level];
 		i8_tile<TM, TN>(g, stage0, stage1);
 	}
 }
@@ -231,8 +241,9 @@ __global__ __launch_bounds__(256) void k_recon_grouped(const GRec *g, const int 
 	if (e >= (int64_t)r.M * r.N) return;
 	const int i = (int)(e % r.M), j = (int)(e / r.M);
 	double *c = r.C + (int64_t)j * r.ldc + i;
-	*c = recon_elem<NW>(r.T + (int64_t)j * r.ldt + i, r.tstride, r.binLevel, r.nbins, r.Lmax,
-	                    r.rowE[i] + r.colE[j] - 16 - 8 * r.Lmax, r.alpha, r.beta, *c);
+	const double t = res_elem_value<NW>(r.R + (int64_t)j * r.ldt + i, r.tstride, r.nchunk, r.rc, r.nmod,
+	                                    r.rowLo[i] + r.colLo[j]);
+	*c = fma(r.alpha, t, r.beta * *c);
 }
 
 // ---- host plan -------------------------------------------------------------------------------
@@ -269,7 +280,7 @@ struct BatchPlan {
 	std::vector<const double *> opIn;  // k-contiguous source of every operand
 	std::vector<int64_t> opLd;
 	std::vector<int> opCols, opK;
-	std::vector<int *> opColE;
+	std::vector<int *> opColE, opColLo;
 	// phase 2
 	std::vector<int> fast;      // problems in the grouped launch
 	std::vector<int> fallback;  // problems computed by ozblasDgemm one by one
@@ -287,7 +298,8 @@ struct BatchPlan {
 	int grid[2] = {0, 0};
 	float tuneMs[2] = {0.f, 0.f};
 	double ops = 0.0;
-	int64_t totalPairs = 0;
+	int64_t totalPairs = 0;  // slice products (moduli) over the grouped problems
+	const ResConst *dRcAll = nullptr;  // constants of all the moduli (residue writer, GEMM epilogue)
 };
 
 static void plan_free(BatchPlan *p) {
@@ -330,15 +342,15 @@ static int32_t launch_execute(BatchPlan *p, hipStream_t st) {
 		                   p->nCheck, p->dSt, p->dFlag);
 	if (p->nWriteBlocks > 0)
 		hipLaunchKernelGGL(k_write_grouped, dim3(p->nWriteBlocks), dim3(256), 0, st, p->dWrite,
-		                   p->dWriteStart, p->nWrite);
+		                   p->dWriteStart, p->nWrite, p->dRcAll);
 	const int c = p->tcfg;
 	if (p->nItems[c] > 0) {
 		if (c == 0)
 			hipLaunchKernelGGL((k_gemm_grouped<2, 2>), dim3(p->grid[0]), dim3(256), 0, st, p->dProb[0],
-			                   p->dItems[0], p->nItems[0], p->dCounter, p->dFlag);
+			                   p->dItems[0], p->nItems[0], p->dCounter, p->dFlag, p->dRcAll);
 		else
 			hipLaunchKernelGGL((k_gemm_grouped<4, 4>), dim3(p->grid[1]), dim3(256), 0, st, p->dProb[1],
-			                   p->dItems[1], p->nItems[1], p->dCounter, p->dFlag);
+			                   p->dItems[1], p->nItems[1], p->dCounter, p->dFlag, p->dRcAll);
 	}
 	if (p->nRecBlocks > 0) {
 		if (p->NW <= 3)
@@ -377,8 +389,9 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 	p->opCols.assign(p->nops, 0);
 	p->opK.assign(p->nops, 0);
 	p->opColE.assign(p->nops, nullptr);
+	p->opColLo.assign(p->nops, nullptr);
 	Layout L1;
-	std::vector<uint64_t> transOff(p->nops, UINT64_MAX), colEOff(p->nops), colSOff(p->nops);
+	std::vector<uint64_t> transOff(p->nops, UINT64_MAX), colEOff(p->nops), colSOff(p->nops), colLoOff(p->nops);
 	std::vector<GTrans> hTrans;
 	std::vector<int> transStart;
 	for (int q : live) {
@@ -396,6 +409,8 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 		colSOff[2 * q] = L1.take(4ull * m[q]);
 		colEOff[2 * q + 1] = L1.take(4ull * n[q]);
 		colSOff[2 * q + 1] = L1.take(4ull * n[q]);
+		colLoOff[2 * q] = L1.take(4ull * m[q]);
+		colLoOff[2 * q + 1] = L1.take(4ull * n[q]);
 	}
 	const uint64_t stOff = L1.take(sizeof(ExpStats) * (uint64_t)p->nops);
 	const uint64_t flagOff = L1.take(256);
@@ -444,6 +459,7 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 					p->opLd[o] = ld;
 				}
 				p->opColE[o] = (int *)(M1 + colEOff[o]);
+				p->opColLo[o] = (int *)(M1 + colLoOff[o]);
 			}
 		}
 		p->nTrans = (int)hTrans.size();
@@ -457,6 +473,7 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 			g.pad = 0;
 			g.colE = p->opColE[o];
 			g.colS = p->opColE[o] ? (int *)(M1 + colSOff[o]) : nullptr;
+			g.colLo = p->opColLo[o];
 			g.st = p->dSt + o;
 			statStart[o] = sb;
 			sb += p->opCols[o];  // columns, in operand order (0 for absent problems)
@@ -488,74 +505,95 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 	OZ_HIP(hipGetLastError());
 	OZ_HIP(hipMemcpyAsync(p->hSt, p->dSt, sizeof(ExpStatsH) * p->nops, hipMemcpyDeviceToHost, st));
 	OZ_HIP(hipStreamSynchronize(st));
-	// ---- phase 2: planes, bins, items ----------------------------------------------------------
+	// ---- phase 2: residue planes, bins, items ---------------------------------------------------
+	// moduli per problem: |X| < k 2^(spanA + spanB) must stay within M / 4 (oz_residue.hip)
+	auto ceil_log2 = [](int v) {
+		int l = 0;
+		while ((int64_t(1) << l) < (int64_t)v) l++;
+		return l;
+	};
+	struct ProbH {
+		int q, nmod, nchunk, budget;
+		int64_t ldp, apstride, bpstride, ldt, tstride;
+		uint64_t planesA, planesB, R;
+		int bin0, nbins;
+	};
+	std::vector<ProbH> probs;
 	for (int q : live) {
-		if (k[q] > 0 && fast_valid(oh, p->hSt[2 * q], p->hSt[2 * q + 1], k[q]))
-			p->fast.push_back(q);
-		else
+		const ExpStatsH &sa = p->hSt[2 * q], &sb = p->hSt[2 * q + 1];
+		if (!(k[q] > 0 && fast_valid(oh, sa, sb, k[q]))) {
 			p->fallback.push_back(q);
+			continue;
+		}
+		ProbH h{};
+		h.q = q;
+		if (sa.maxS > 0 && sb.maxS > 0) {
+			if (sa.maxSpan >= kResMaxShift || sb.maxSpan >= kResMaxShift) {
+				p->fallback.push_back(q);
+				continue;
+			}
+			const double need = (double)sa.maxSpan + sb.maxSpan + ceil_log2(k[q]) + 2;
+			int nm = 1;
+			while (nm <= kResMaxMod && res_log2_product(nm) < need + 1e-6) nm++;
+			if (nm > kResMaxMod) {  // beyond the moduli: ozblasDgemm takes the problem alone
+				p->fallback.push_back(q);
+				continue;
+			}
+			h.nmod = nm;
+			h.budget = (int)std::floor(res_log2_product(nm) - 1e-6) - ceil_log2(k[q]) - 2;
+		} else {
+			h.nmod = 0;  // an all-zero operand: C = fma(alpha, +0, beta C)
+			h.budget = -1;
+		}
+		p->fast.push_back(q);
+		probs.push_back(h);
 	}
 	const int cap = split_cap(oh);
 	Layout L2;
-	struct ProbH {
-		int q, SA, SB, Lmax, nbins, bin0;
-		int64_t ldp, apstride, bpstride, ldt, tstride;
-		uint64_t planesA, planesB, T;
-	};
-	std::vector<ProbH> probs;
 	std::vector<LevelBin> hBins;
 	std::vector<int2> hPairs;
-	std::vector<int> hLevel;
-	int maxLmax = 0;
-	for (int q : p->fast) {
-		ProbH h{};
-		h.q = q;
-		h.SA = p->hSt[2 * q].maxS;
-		h.SB = p->hSt[2 * q + 1].maxS;
+	for (int t = 0; t < kResMaxMod; t++) hPairs.push_back(make_int2(t, t));  // product t: plane t x plane t
+	int maxNl = 2;
+	std::vector<int> usedNmod;
+	for (ProbH &h : probs) {
+		const int q = h.q;
 		h.ldp = rup64(k[q], 128);
 		h.apstride = h.ldp * rup64(m[q], 256);
 		h.bpstride = h.ldp * rup64(n[q], 256);
 		h.ldt = rup64(m[q], 256);
 		h.tstride = h.ldt * rup64(n[q], 256);
 		h.bin0 = (int)hBins.size();
-		h.Lmax = 0;
 		h.nbins = 0;
-		if (h.SA > 0 && h.SB > 0) {
-			h.planesA = L2.take((uint64_t)h.apstride * h.SA);
-			h.planesB = L2.take((uint64_t)h.bpstride * h.SB);
-			h.Lmax = h.SA + h.SB - 2;
+		h.nchunk = 1;
+		if (h.nmod > 0) {
+			h.planesA = L2.take((uint64_t)h.apstride * h.nmod);
+			h.planesB = L2.take((uint64_t)h.bpstride * h.nmod);
 			const int Kpad = (int)h.ldp;
-			const int kc = Kpad <= 65536 ? Kpad : 65536;
-			const int G = std::max(1, 131071 / kc);
-			for (int Lv = 0; Lv <= h.Lmax; Lv++) {
-				std::vector<int2> lp;
-				for (int s = 0; s < h.SA; s++) {
-					const int t = Lv - s;
-					if (t >= 0 && t < h.SB) lp.push_back(make_int2(s, t));
+			const int kc = 65536;  // |acc| <= kc 2^14 < 2^31 per chunk
+			h.nchunk = (Kpad + kc - 1) / kc;
+			for (int t = 0; t < h.nmod; t++)
+				for (int c = 0; c < h.nchunk; c++) {
+					LevelBin bn;
+					bn.level = t;
+					bn.pair0 = t;
+					bn.npairs = 1;
+					bn.k0 = c * kc;
+					bn.k1 = std::min(Kpad, (c + 1) * kc);
+					hBins.push_back(bn);
 				}
-				for (size_t g0 = 0; g0 < lp.size(); g0 += G)
-					for (int k0 = 0; k0 < Kpad; k0 += kc) {
-						LevelBin bn;
-						bn.level = Lv;
-						bn.pair0 = (int)hPairs.size();
-						bn.npairs = (int)std::min<size_t>(G, lp.size() - g0);
-						bn.k0 = k0;
-						bn.k1 = std::min(Kpad, k0 + kc);
-						for (int e = 0; e < bn.npairs; e++) hPairs.push_back(lp[g0 + e]);
-						hBins.push_back(bn);
-						hLevel.push_back(Lv);
-					}
-			}
 			h.nbins = (int)hBins.size() - h.bin0;
-			h.T = L2.take(4ull * (uint64_t)h.tstride * h.nbins);
-			p->totalPairs += (int64_t)h.SA * h.SB;
-			p->ops += 2.0 * m[q] * (double)n[q] * k[q] * ((double)h.SA * h.SB);
+			h.R = L2.take((uint64_t)h.tstride * h.nbins);
+			p->totalPairs += h.nmod;
+			p->ops += 2.0 * m[q] * (double)n[q] * k[q] * h.nmod;
+			maxNl = std::max(maxNl, res_const_host(h.nmod).nl);
+			if (std::find(usedNmod.begin(), usedNmod.end(), h.nmod) == usedNmod.end()) usedNmod.push_back(h.nmod);
 		}
-		maxLmax = std::max(maxLmax, h.Lmax);
-		probs.push_back(h);
 	}
-	const int bits = 8 * maxLmax + 32 + 8 + 1;
-	p->NW = (bits + 63) / 64;
+	p->NW = (maxNl + 1) / 2;
+	// constants: the full moduli table, then one per moduli count in use
+	const uint64_t rcAllOff = L2.take(sizeof(ResConst));
+	std::vector<uint64_t> rcOff(kResMaxMod + 1, 0);
+	for (int nm : usedNmod) rcOff[nm] = L2.take(sizeof(ResConst));
 	// tables
 	std::vector<GWrite> hWrite;
 	std::vector<int> writeStart;
@@ -567,7 +605,6 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 	std::vector<double> itemCost[2];
 	const uint64_t binsOff = L2.take(sizeof(LevelBin) * (hBins.size() + 1));
 	const uint64_t pairsOff = L2.take(sizeof(int2) * (hPairs.size() + 1));
-	const uint64_t levelOff = L2.take(4 * (hLevel.size() + 1));
 	// sizes of the remaining tables are known from probs
 	size_t nItemsMax[2] = {0, 0};
 	for (const ProbH &h : probs)
@@ -589,11 +626,14 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 	char *M2 = p->mem2;
 	const LevelBin *dBins = (const LevelBin *)(M2 + binsOff);
 	const int2 *dPairs = (const int2 *)(M2 + pairsOff);
-	const int *dLevel = (const int *)(M2 + levelOff);
+	p->dRcAll = (const ResConst *)(M2 + rcAllOff);
+	OZ_HIP(hipMemcpyAsync(M2 + rcAllOff, &res_const_host(kResMaxMod), sizeof(ResConst), hipMemcpyHostToDevice, st));
+	for (int nm : usedNmod)
+		OZ_HIP(hipMemcpyAsync(M2 + rcOff[nm], &res_const_host(nm), sizeof(ResConst), hipMemcpyHostToDevice, st));
 	int wb = 0, rb = 0;
 	for (const ProbH &h : probs) {
 		const int q = h.q;
-		if (h.SA > 0 && h.SB > 0) {
+		if (h.nmod > 0) {
 			for (int side = 0; side < 2; side++) {
 				const int o = 2 * q + side;
 				GWrite w;
@@ -601,11 +641,11 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 				w.ldi = p->opLd[o];
 				w.m = k[q];
 				w.n = p->opCols[o];
-				w.colE = p->opColE[o];
+				w.colLo = p->opColLo[o];
 				w.planes = (int8_t *)(M2 + (side == 0 ? h.planesA : h.planesB));
 				w.ldp = h.ldp;
 				w.pstride = side == 0 ? h.apstride : h.bpstride;
-				w.S = side == 0 ? h.SA : h.SB;
+				w.nmod = h.nmod;
 				w.kts = (int)(h.ldp / 64);
 				writeStart.push_back(wb);
 				wb += w.kts * (int)(rup64(w.n, 256) / 16);
@@ -619,7 +659,7 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 				g.apstride = h.apstride;
 				g.bpstride = h.bpstride;
 				g.ldp = h.ldp;
-				g.T = (int32_t *)(M2 + h.T);
+				g.R = (uint8_t *)(M2 + h.R);
 				g.ldt = h.ldt;
 				g.tstride = h.tstride;
 				g.bins = dBins + h.bin0;
@@ -640,10 +680,10 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 			}
 		}
 		GRec r;
-		r.T = (h.nbins > 0) ? (const int32_t *)(M2 + h.T) : nullptr;
-		r.binLevel = dLevel + h.bin0;
-		r.rowE = p->opColE[2 * q];
-		r.colE = p->opColE[2 * q + 1];
+		r.R = (h.nmod > 0) ? (const uint8_t *)(M2 + h.R) : nullptr;
+		r.rc = (h.nmod > 0) ? (const ResConst *)(M2 + rcOff[h.nmod]) : p->dRcAll;
+		r.rowLo = p->opColLo[2 * q];
+		r.colLo = p->opColLo[2 * q + 1];
 		r.C = C[q];
 		r.ldt = h.ldt;
 		r.tstride = h.tstride;
@@ -652,19 +692,17 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 		r.beta = beta[q];
 		r.M = m[q];
 		r.N = n[q];
-		r.nbins = h.nbins;
-		r.Lmax = h.Lmax;
+		r.nmod = h.nmod;
+		r.nchunk = h.nchunk;
 		recStart.push_back(rb);
 		rb += (int)(((int64_t)m[q] * n[q] + 255) / 256);
 		hRec.push_back(r);
 		GCheck ck;
 		ck.opA = 2 * q;
 		ck.opB = 2 * q + 1;
-		ck.SA = h.SA;
-		ck.SB = h.SB;
+		ck.spanBudget = h.budget;
 		const int rho = rho_host(k[q], oh->splitEpsMode);
 		ck.spanLimit = (53 - rho - 1) * (cap - 3);
-		ck.pad = 0;
 		hCheck.push_back(ck);
 	}
 	// longest items first (the work counter then balances the tail)
@@ -693,11 +731,9 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 	p->dRecStart = (int *)(M2 + recStartOff);
 	p->nCheck = (int)hCheck.size();
 	p->dCheck = (GCheck *)(M2 + checkOff);
-	if (!hBins.empty()) {
+	if (!hBins.empty())
 		OZ_HIP(hipMemcpyAsync(M2 + binsOff, hBins.data(), sizeof(LevelBin) * hBins.size(), hipMemcpyHostToDevice, st));
-		OZ_HIP(hipMemcpyAsync(M2 + pairsOff, hPairs.data(), sizeof(int2) * hPairs.size(), hipMemcpyHostToDevice, st));
-		OZ_HIP(hipMemcpyAsync(M2 + levelOff, hLevel.data(), 4 * hLevel.size(), hipMemcpyHostToDevice, st));
-	}
+	OZ_HIP(hipMemcpyAsync(M2 + pairsOff, hPairs.data(), sizeof(int2) * hPairs.size(), hipMemcpyHostToDevice, st));
 	if (p->nWrite > 0) {
 		OZ_HIP(hipMemcpyAsync(p->dWrite, hWrite.data(), sizeof(GWrite) * hWrite.size(), hipMemcpyHostToDevice, st));
 		OZ_HIP(hipMemcpyAsync(p->dWriteStart, writeStart.data(), 4 * writeStart.size(), hipMemcpyHostToDevice, st));
@@ -721,7 +757,7 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 	// ---- autotune: digit planes once, then one grouped launch per tile shape -------------------
 	if (p->nWriteBlocks > 0)
 		hipLaunchKernelGGL(k_write_grouped, dim3(p->nWriteBlocks), dim3(256), 0, st, p->dWrite,
-		                   p->dWriteStart, p->nWrite);
+		                   p->dWriteStart, p->nWrite, p->dRcAll);
 	p->tcfg = 0;
 	if (p->nItems[0] > 0) {
 		for (int c = 0; c < 2; c++) {
@@ -729,10 +765,10 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 			OZ_HIP(hipEventRecord(b->ev[6], st));
 			if (c == 0)
 				hipLaunchKernelGGL((k_gemm_grouped<2, 2>), dim3(p->grid[0]), dim3(256), 0, st, p->dProb[0],
-				                   p->dItems[0], p->nItems[0], p->dCounter, p->dFlag);
+				                   p->dItems[0], p->nItems[0], p->dCounter, p->dFlag, p->dRcAll);
 			else
 				hipLaunchKernelGGL((k_gemm_grouped<4, 4>), dim3(p->grid[1]), dim3(256), 0, st, p->dProb[1],
-				                   p->dItems[1], p->nItems[1], p->dCounter, p->dFlag);
+				                   p->dItems[1], p->nItems[1], p->dCounter, p->dFlag, p->dRcAll);
 			OZ_HIP(hipEventRecord(b->ev[7], st));
 			OZ_HIP(hipEventSynchronize(b->ev[7]));
 			OZ_HIP(hipEventElapsedTime(&p->tuneMs[c], b->ev[6], b->ev[7]));

@@ -16,7 +16,7 @@
 //     rounding boundary), in a multi-word integer, then one RN-even of X 2^(lo_i + lo_j).
 // The number of GEMMs is the number of moduli, about (spanA + spanB + log2 k + 2) / 7.9, against
 // about (spanA / 8) (spanB / 8) / 2 digit-pair GEMMs of the radix-256 digit form.
-#include "oz_digits.hpp"
+#include "oz_residue.hpp"
 
 #include <cstring>
 #include <mutex>
@@ -122,92 +122,17 @@ double res_log2_product(int nmod) {
 	return g_rc[nmod].nmod == nmod ? g_log2M[nmod] : 0.0;
 }
 
-// ---- device: residues of one element ------------------------------------------------------
-// x = +-f 2^e (f odd, or 0) relative to the column unit 2^lo: x 2^-lo = +-f 2^sh with sh >= 0
-struct ResElem {
-	double fd;  // f as a binary64 (exact: f < 2^53)
-	int sh;
-	int neg;
-};
-
-__device__ inline ResElem res_elem(double x, int lo) {
-	ResElem r{0.0, 0, 0};
-	if (x == 0.0) return r;
-	const uint64_t b = d2u(x);
-	const int E = (int)((b >> 52) & 0x7ff);
-	uint64_t f = b & ((uint64_t(1) << 52) - 1);
-	int e0;
-	if (E == 0) {
-		e0 = -1074;
-	} else {
-		f |= (uint64_t(1) << 52);
-		e0 = E - 1075;
-	}
-	const int tz = __builtin_ctzll(f);
-	f >>= tz;
-	r.fd = (double)f;
-	r.sh = e0 + tz - lo;  // in [0, span) by the choice of lo
-	r.neg = (int)(b >> 63);
-	return r;
-}
-
-// balanced residue of +-f 2^sh modulo p: in [-(p-1)/2, (p-1)/2] for odd p, [-128, 127] for 256
-__device__ __forceinline__ int res_of(const ResElem &e, int p, double invpd, float invpf,
-                                      const uint8_t *pw256) {
-	if (e.fd == 0.0) return 0;
-	double q = floor(e.fd * invpd);  // f mod p: quotient off by at most one
-	double r = fma(-q, (double)p, e.fd);
-	r += (r < 0.0) ? (double)p : 0.0;
-	r -= (r >= (double)p) ? (double)p : 0.0;
-	// (f mod p) 2^(sh mod 8) (256^(sh / 8) mod p) < 2^23: exact in binary32
-	const float v = (float)r * (float)(1 << (e.sh & 7)) * (float)pw256[e.sh >> 3];
-	const float q2 = floorf(v * invpf);
-	float r2 = fmaf(-q2, (float)p, v);
-	r2 += (r2 < 0.0f) ? (float)p : 0.0f;
-	r2 -= (r2 >= (float)p) ? (float)p : 0.0f;
-	int res = (int)r2;
-	if (e.neg && res) res = p - res;
-	const int thr = (p & 1) ? (p >> 1) : (p >> 1) - 1;
-	return res > thr ? res - p : res;
-}
-
 // INT8 tiled planes (the layout of the digit planes): one 256-thread workgroup writes 16 columns x
 // 64 k-bytes of every modulus plane per step (4 bytes per thread), looping over k tiles.
 __global__ __launch_bounds__(256) void k_res_write_i8t(int m, int n, const double *in, int64_t ldi,
                                                        const int *colLo, const ResConst *rc, int nmod,
                                                        int8_t *planes, int64_t ldp, int64_t pstride) {
-	__shared__ uint8_t pw[kResMaxMod][kResMaxShift / 8];
-	__shared__ int ps[kResMaxMod];
-	__shared__ float ipf[kResMaxMod];
-	__shared__ double ipd[kResMaxMod];
-	for (int q = threadIdx.x; q < nmod * (kResMaxShift / 8); q += 256)
-		pw[q / (kResMaxShift / 8)][q % (kResMaxShift / 8)] = rc->pow256[q / (kResMaxShift / 8)][q % (kResMaxShift / 8)];
-	for (int q = threadIdx.x; q < nmod; q += 256) {
-		ps[q] = rc->p[q];
-		ipf[q] = rc->invpf[q];
-		ipd[q] = rc->invpd[q];
-	}
+	__shared__ ResLds T;
+	res_lds_load(T, rc, nmod);
 	__syncthreads();
-	const int t = threadIdx.x;
-	const int jb = blockIdx.y;
-	const int j = jb * 16 + t / 16;
-	const int lo = (j < n) ? colLo[j] : 0;
-	const double *x = in + (int64_t)(j < n ? j : 0) * ldi;
 	const int ktiles = (int)(ldp / 64);
-	for (int kt = blockIdx.x; kt < ktiles; kt += gridDim.x) {
-		const int q0 = kt * 64 + 4 * (t % 16);
-		ResElem e[4];
-#pragma unroll
-		for (int u = 0; u < 4; u++) e[u] = res_elem((j < n && q0 + u < m) ? x[q0 + u] : 0.0, lo);
-		int8_t *dst = planes + (int64_t)jb * 16 * ldp + (int64_t)kt * 1024 + 4 * t;
-		for (int s = 0; s < nmod; s++) {
-			uint32_t word = 0;
-#pragma unroll
-			for (int u = 0; u < 4; u++)
-				word |= (uint32_t)(uint8_t)(int8_t)res_of(e[u], ps[s], ipd[s], ipf[s], pw[s]) << (8 * u);
-			*(uint32_t *)(dst + (int64_t)s * pstride) = word;
-		}
-	}
+	for (int kt = blockIdx.x; kt < ktiles; kt += gridDim.x)
+		res_write_tile_i8(m, n, in, ldi, colLo, nmod, T, planes, ldp, pstride, kt, blockIdx.y);
 }
 
 // 2-byte planes (FP16 / BF16): plane s of column j at planes + s*pstride + j*ldp, rows [m, ldp) zero
@@ -248,54 +173,6 @@ void launch_res_write(hipStream_t st, int elem, int m, int n, const double *in, 
 	else
 		hipLaunchKernelGGL(k_res_write_h<__bf16>, dim3(gx, n), dim3(256), 0, st, m, n, in, ldi, colLo, rc,
 		                   nmod, (__bf16 *)planes, ldp, pstride);
-}
-
-// ---- device: Chinese-remainder reconstruction ----------------------------------------------
-// v mod p for 0 <= v < 2^24 (binary32 quotient estimate off by at most one)
-__device__ __forceinline__ int mod_f(int v, int p, float invpf) {
-	const float q = floorf((float)v * invpf);
-	int r = v - (int)q * p;
-	r += (r < 0) ? p : 0;
-	r -= (r >= p) ? p : 0;
-	return r;
-}
-
-// X (NL 32-bit limbs, two's complement) of one element from its residues, then RN(X 2^e0)
-template <int NW>
-__device__ inline double res_elem_value(const uint8_t *Re, int64_t rstride, int nchunk,
-                                        const ResConst *rc, int nmod, int e0) {
-	constexpr int NL = 2 * NW;
-	int64_t acc[NL];
-#pragma unroll
-	for (int l = 0; l < NL; l++) acc[l] = 0;
-	double S = 0.0;
-	for (int t = 0; t < nmod; t++) {
-		const int p = rc->p[t];
-		const float ipf = rc->invpf[t];
-		int r = 0;
-		for (int c = 0; c < nchunk; c++) r += Re[(int64_t)(t * nchunk + c) * rstride];
-		if (nchunk > 1) r = mod_f(r, p, ipf);
-		const int ct = mod_f(r * rc->u[t], p, ipf);  // < 2^16 before the reduction
-		S += (double)ct * rc->invpd[t];
-#pragma unroll
-		for (int l = 0; l < NL; l++) acc[l] += (int64_t)((uint64_t)(uint32_t)ct * rc->Mp[t][l]);
-	}
-	const int64_t K = (int64_t)rint(S);
-#pragma unroll
-	for (int l = 0; l < NL; l++) acc[l] -= K * (int64_t)rc->M[l];
-	BigInt<NW> X;
-	int64_t carry = 0;
-#pragma unroll
-	for (int q = 0; q < NW; q++) {
-		const int64_t v0 = acc[2 * q] + carry;
-		const uint64_t w0 = (uint32_t)v0;
-		carry = v0 >> 32;
-		const int64_t v1 = acc[2 * q + 1] + carry;
-		const uint64_t w1 = (uint32_t)v1;
-		carry = v1 >> 32;
-		X.w[q] = w0 | (w1 << 32);
-	}
-	return big_finish<NW>(X, e0);
 }
 
 template <int NW>

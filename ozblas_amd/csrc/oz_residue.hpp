@@ -1,0 +1,148 @@
+// oz_residue.hpp -- device routines of the residue form shared by the single-problem kernels
+// (oz_residue.hip) and the grouped batched kernels (oz_batched.hip): the balanced residues of one
+// element modulo each modulus, the tiled INT8 residue-plane writer, and the Chinese-remainder
+// reconstruction of one output element.  The scheme itself is described in oz_residue.hip.
+#pragma once
+#include "oz_digits.hpp"
+
+namespace oz {
+
+// ---- device: residues of one element ------------------------------------------------------
+// x = +-f 2^e (f odd, or 0) relative to the column unit 2^lo: x 2^-lo = +-f 2^sh with sh >= 0
+struct ResElem {
+	double fd;  // f as a binary64 (exact: f < 2^53)
+	int sh;
+	int neg;
+};
+
+__device__ inline ResElem res_elem(double x, int lo) {
+	ResElem r{0.0, 0, 0};
+	if (x == 0.0) return r;
+	const uint64_t b = d2u(x);
+	const int E = (int)((b >> 52) & 0x7ff);
+	uint64_t f = b & ((uint64_t(1) << 52) - 1);
+	int e0;
+	if (E == 0) {
+		e0 = -1074;
+	} else {
+		f |= (uint64_t(1) << 52);
+		e0 = E - 1075;
+	}
+	const int tz = __builtin_ctzll(f);
+	f >>= tz;
+	r.fd = (double)f;
+	r.sh = e0 + tz - lo;  // in [0, span) by the choice of lo
+	r.neg = (int)(b >> 63);
+	return r;
+}
+
+// balanced residue of +-f 2^sh modulo p: in [-(p-1)/2, (p-1)/2] for odd p, [-128, 127] for 256
+__device__ __forceinline__ int res_of(const ResElem &e, int p, double invpd, float invpf,
+                                      const uint8_t *pw256) {
+	if (e.fd == 0.0) return 0;
+	double q = floor(e.fd * invpd);  // f mod p: quotient off by at most one
+	double r = fma(-q, (double)p, e.fd);
+	r += (r < 0.0) ? (double)p : 0.0;
+	r -= (r >= (double)p) ? (double)p : 0.0;
+	// (f mod p) 2^(sh mod 8) (256^(sh / 8) mod p) < 2^23: exact in binary32
+	const float v = (float)r * (float)(1 << (e.sh & 7)) * (float)pw256[e.sh >> 3];
+	const float q2 = floorf(v * invpf);
+	float r2 = fmaf(-q2, (float)p, v);
+	r2 += (r2 < 0.0f) ? (float)p : 0.0f;
+	r2 -= (r2 >= (float)p) ? (float)p : 0.0f;
+	int res = (int)r2;
+	if (e.neg && res) res = p - res;
+	const int thr = (p & 1) ? (p >> 1) : (p >> 1) - 1;
+	return res > thr ? res - p : res;
+}
+
+// the per-modulus tables of the residue writer, staged in LDS by the whole workgroup
+struct ResLds {
+	uint8_t pw[kResMaxMod][kResMaxShift / 8];
+	int p[kResMaxMod];
+	float ipf[kResMaxMod];
+	double ipd[kResMaxMod];
+};
+
+__device__ inline void res_lds_load(ResLds &T, const ResConst *rc, int nmod) {
+	constexpr int A = kResMaxShift / 8;
+	for (int q = threadIdx.x; q < nmod * A; q += blockDim.x) T.pw[q / A][q % A] = rc->pow256[q / A][q % A];
+	for (int q = threadIdx.x; q < nmod; q += blockDim.x) {
+		T.p[q] = rc->p[q];
+		T.ipf[q] = rc->invpf[q];
+		T.ipd[q] = rc->invpd[q];
+	}
+}
+
+// One 256-thread workgroup: residues of the 16 columns x 64 k-bytes tile (kt, jb) of every modulus
+// plane in the INT8 tiled layout (4 bytes per thread), with the 256^a table and moduli in LDS.
+__device__ inline void res_write_tile_i8(int m, int n, const double *in, int64_t ldi, const int *colLo,
+                                         int nmod, const ResLds &T, int8_t *planes, int64_t ldp,
+                                         int64_t pstride, int kt, int jb) {
+	const int t = threadIdx.x;
+	const int j = jb * 16 + t / 16;
+	const int lo = (j < n) ? colLo[j] : 0;
+	const double *x = in + (int64_t)(j < n ? j : 0) * ldi;
+	const int q0 = kt * 64 + 4 * (t % 16);
+	ResElem e[4];
+#pragma unroll
+	for (int u = 0; u < 4; u++) e[u] = res_elem((j < n && q0 + u < m) ? x[q0 + u] : 0.0, lo);
+	int8_t *dst = planes + (int64_t)jb * 16 * ldp + (int64_t)kt * 1024 + 4 * t;
+	for (int s = 0; s < nmod; s++) {
+		uint32_t word = 0;
+#pragma unroll
+		for (int u = 0; u < 4; u++)
+			word |= (uint32_t)(uint8_t)(int8_t)res_of(e[u], T.p[s], T.ipd[s], T.ipf[s], T.pw[s]) << (8 * u);
+		*(uint32_t *)(dst + (int64_t)s * pstride) = word;
+	}
+}
+
+// ---- device: Chinese-remainder reconstruction ----------------------------------------------
+// v mod p for 0 <= v < 2^24 (binary32 quotient estimate off by at most one)
+__device__ __forceinline__ int mod_f(int v, int p, float invpf) {
+	const float q = floorf((float)v * invpf);
+	int r = v - (int)q * p;
+	r += (r < 0) ? p : 0;
+	r -= (r >= p) ? p : 0;
+	return r;
+}
+
+// X (NL 32-bit limbs, two's complement) of one element from its residues, then RN(X 2^e0)
+template <int NW>
+__device__ inline double res_elem_value(const uint8_t *Re, int64_t rstride, int nchunk,
+                                        const ResConst *rc, int nmod, int e0) {
+	constexpr int NL = 2 * NW;
+	int64_t acc[NL];
+#pragma unroll
+	for (int l = 0; l < NL; l++) acc[l] = 0;
+	double S = 0.0;
+	for (int t = 0; t < nmod; t++) {
+		const int p = rc->p[t];
+		const float ipf = rc->invpf[t];
+		int r = 0;
+		for (int c = 0; c < nchunk; c++) r += Re[(int64_t)(t * nchunk + c) * rstride];
+		if (nchunk > 1) r = mod_f(r, p, ipf);
+		const int ct = mod_f(r * rc->u[t], p, ipf);  // < 2^16 before the reduction
+		S += (double)ct * rc->invpd[t];
+#pragma unroll
+		for (int l = 0; l < NL; l++) acc[l] += (int64_t)((uint64_t)(uint32_t)ct * rc->Mp[t][l]);
+	}
+	const int64_t K = (int64_t)rint(S);
+#pragma unroll
+	for (int l = 0; l < NL; l++) acc[l] -= K * (int64_t)rc->M[l];
+	BigInt<NW> X;
+	int64_t carry = 0;
+#pragma unroll
+	for (int q = 0; q < NW; q++) {
+		const int64_t v0 = acc[2 * q] + carry;
+		const uint64_t w0 = (uint32_t)v0;
+		carry = v0 >> 32;
+		const int64_t v1 = acc[2 * q + 1] + carry;
+		const uint64_t w1 = (uint32_t)v1;
+		carry = v1 >> 32;
+		X.w[q] = w0 | (w1 << 32);
+	}
+	return big_finish<NW>(X, e0);
+}
+
+}  // namespace oz
