@@ -695,8 +695,9 @@ static int32_t gemm_residue(ozblasHandle_t *oh, char transA, char transB, int m,
 		}
 	}
 	for (int q = 0; q < nbins; q++) order[q] = q;
+	// product residues: whole INT8 tiles, or (FP16/BF16) every 4-column word of the m x n product
 	const int64_t ldt = (elem == 1) ? rup(m, 256) : rup(m, 64);
-	const int64_t tstride = ldt * ((elem == 1) ? rup(n, 256) : n);
+	const int64_t tstride = ldt * ((elem == 1) ? rup(n, 256) : rup(n, 4));
 	int2 *dPairs = (int2 *)walloc(oh, sizeof(int2) * pairs.size());
 	LevelBin *dBins = (LevelBin *)walloc(oh, sizeof(LevelBin) * bins.size());
 	int *dOrder = (int *)walloc(oh, sizeof(int) * order.size());

@@ -226,8 +226,8 @@ __global__ __launch_bounds__(256, 1) void k_gemm_grouped(const GProb *P, const G
 		g.ldt = p.ldt;
 		g.R = p.R + (int64_t)w.bin * p.tstride;  // bin = (modulus bin.level, k chunk)
 		g.p = rc->p[bin.level];
-		g.invp = rc->invpd[bin.level];
-		i8_tile<TM, TN>(g, stage0, stage1);
+		g.invpf = rc->invpf[bin.level];
+		i8_tile<TM, TN, true>(g, stage0, stage1);
 	}
 }
 
@@ -238,12 +238,9 @@ __global__ __launch_bounds__(256) void k_recon_grouped(const GRec *g, const int 
 	const int o = owner(start, n, blockIdx.x);
 	const GRec r = g[o];
 	const int64_t e = (int64_t)(blockIdx.x - start[o]) * 256 + threadIdx.x;
-	if (e >= (int64_t)r.M * r.N) return;
-	const int i = (int)(e % r.M), j = (int)(e / r.M);
-	double *c = r.C + (int64_t)j * r.ldc + i;
-	const double t = res_elem_value<NW>(r.R + (int64_t)j * r.ldt + i, r.tstride, r.nchunk, r.rc, r.nmod,
-	                                    r.rowLo[i] + r.colLo[j]);
-	*c = fma(r.alpha, t, r.beta * *c);
+	if (e >= (int64_t)r.M * ((r.N + 3) / 4)) return;
+	res_recon_group<NW>(r.M, r.N, r.R, r.ldt, r.tstride, r.nchunk, r.rc, r.nmod, r.rowLo, r.colLo, r.alpha,
+	                    r.beta, r.C, r.ldc, (int)(e % r.M), (int)(e / r.M));
 }
 
 // ---- host plan -------------------------------------------------------------------------------
@@ -695,7 +692,7 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 		r.nmod = h.nmod;
 		r.nchunk = h.nchunk;
 		recStart.push_back(rb);
-		rb += (int)(((int64_t)m[q] * n[q] + 255) / 256);
+		rb += (int)(((int64_t)m[q] * ((n[q] + 3) / 4) + 255) / 256);
 		hRec.push_back(r);
 		GCheck ck;
 		ck.opA = 2 * q;

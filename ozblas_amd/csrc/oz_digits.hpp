@@ -248,21 +248,32 @@ struct I8Tile {
 	int i0, j0;
 	int32_t *T;  // result plane of the bin
 	int64_t ldt;
-	uint8_t *R;  // residue form: result plane of acc mod p in [0, p) (bytes, leading dimension ldt)
-	int p;       //   instead of T when R is set
-	double invp;  //   1 / p rounded
+	uint8_t *R;   // residue form: result plane of acc mod p in [0, p), instead of T when R is set
+	int p;
+	float invpf;  // 1 / p rounded to binary32
 };
 
-// v mod p in [0, p) for |v| < 2^31 and 2 <= p <= 256 (binary64 quotient estimate off by at most one)
-__device__ __forceinline__ int mod_small(int v, int p, double invp) {
-	const double q = floor((double)v * invp);
-	int r = v - (int)q * p;
+// v mod p in [0, p) for |v| <= 2^30 and 3 <= p <= 256 (256 included: 1/256 is exact), in binary32: a first quotient estimate
+// leaves |r| <= 129 p (the estimate's error is below |v| 3 2^-24 / p + 1/2), which the second
+// step (exact below 2^24) reduces to [0, p) up to one correction.
+__device__ __forceinline__ int mod_i32(int v, int p, float invpf) {
+	const int q1 = (int)rintf((float)v * invpf);
+	int r = v - q1 * p;
+	const int q2 = (int)floorf((float)r * invpf);
+	r -= q2 * p;
 	r += (r < 0) ? p : 0;
 	r -= (r >= p) ? p : 0;
 	return r;
 }
 
-template <int TM, int TN>
+// Product residues of the residue form: one byte per element, four consecutive columns of a row
+// in one 32-bit word -- element (i, j) of a plane at (j / 4) * 4 * ldr + 4 * i + j % 4, so a lane's
+// four MFMA result columns are one word and a wave's words are contiguous.
+__device__ __forceinline__ int64_t res_word_offset(int i, int j, int64_t ldr) {
+	return (int64_t)(j >> 2) * 4 * ldr + 4 * (int64_t)i;
+}
+
+template <int TM, int TN, bool RES>
 __device__ __forceinline__ void i8_tile(const I8Tile &g, char *stage0, char *stage1) {
 	constexpr int TBM = 64 * TM, TBN = 64 * TN;
 	constexpr int NA = 2 * TM, NB = 2 * TN;  // 16-byte chunks per thread per stage
@@ -411,19 +422,25 @@ __device__ __forceinline__ void i8_tile(const I8Tile &g, char *stage0, char *sta
 	// D[p][q]: p = B-side (column j) = 8*(r/4) + 4*(l/32) + r%4, q = A-side (row i) = l%32
 	int32_t *Tb = g.T;
 	const int64_t ldt = g.ldt;
-	if (g.R) {  // residue form: the plane of acc mod p
+	if constexpr (RES) {  // residue form: the plane of acc mod p, four columns per word
 		uint8_t *Rb = g.R;
 		const int p = g.p;
-		const double invp = g.invp;
+		const float invpf = g.invpf;
 #pragma unroll
 		for (int mi = 0; mi < TM; mi++)
 #pragma unroll
 			for (int nj = 0; nj < TN; nj++)
 #pragma unroll
-				for (int r = 0; r < 16; r++) {
+				for (int r4 = 0; r4 < 4; r4++) {
 					const int i = i0 + wm + 32 * mi + l % 32;
-					const int j = j0 + wn + 32 * nj + 8 * (r / 4) + 4 * (l / 32) + r % 4;
-					Rb[(int64_t)j * ldt + i] = (uint8_t)mod_small(acc[mi][nj][r], p, invp);
+					const int j = j0 + wn + 32 * nj + 8 * r4 + 4 * (l / 32);
+					uint32_t w = 0;
+#pragma unroll
+					for (int c = 0; c < 4; c++)
+						w |= (uint32_t)mod_i32(acc[mi][nj][4 * r4 + c], p, invpf) << (8 * c);
+					*(uint32_t *)(Rb + res_word_offset(i, j, ldt)) = w;
+					// one word at a time: interleaving all 64 would hold their accumulators in VGPRs
+					__builtin_amdgcn_sched_barrier(0);
 				}
 		return;
 	}

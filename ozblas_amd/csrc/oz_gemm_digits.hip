@@ -64,7 +64,7 @@ __device__ inline void store_stage(const Stage &s, char *sA, char *sB) {
 // pair p uses plane s0 + p of A and plane t0 - p of B.
 // The MFMA is issued as B-fragment x A-fragment so that lane l%32 indexes the A row: stores of the
 // column-major result planes are contiguous across lanes.
-template <int TM, int TN>
+template <int TM, int TN, bool RES>
 __global__ __launch_bounds__(256, 1) void k_digit_gemm_i8(const int8_t *A, int64_t apstride,
                                                           const int8_t *B, int64_t bpstride,
                                                           int64_t ldp, const int2 *pairs,
@@ -95,12 +95,12 @@ __global__ __launch_bounds__(256, 1) void k_digit_gemm_i8(const int8_t *A, int64
 	g.T = T + (int64_t)binIdx * tstride;
 	g.ldt = ldt;
 	g.R = nullptr;
-	if (R) {  // residue form: bin = (modulus bin.level, k chunk), plane of acc mod p
+	if constexpr (RES) {  // residue form: bin = (modulus bin.level, k chunk), plane of acc mod p
 		g.R = R + (int64_t)binIdx * tstride;
 		g.p = rc->p[bin.level];
-		g.invp = rc->invpd[bin.level];
+		g.invpf = rc->invpf[bin.level];
 	}
-	i8_tile<TM, TN>(g, stage0, stage1);
+	i8_tile<TM, TN, RES>(g, stage0, stage1);
 }
 
 // ---- FP16 / BF16 ---------------------------------------------------------------------------
@@ -116,7 +116,7 @@ __device__ inline f4v mfma16<b8v>(b8v a, b8v b, f4v c) {
 }
 
 // 2-byte digits: a stage is 32 k (64 bytes); a wave's 64x64 region is 4x4 MFMA 16x16 tiles.
-template <typename V>
+template <typename V, bool RES>
 __global__ __launch_bounds__(256) void k_digit_gemm_h(int M, int N, const char *A, int64_t ldab,
                                                       int64_t apstrideb, const char *B,
                                                       int64_t ldbb, int64_t bpstrideb,
@@ -180,18 +180,21 @@ __global__ __launch_bounds__(256) void k_digit_gemm_h(int M, int N, const char *
 		}
 	}
 	// D[p][q]: p = B side (column j) = 4*(l/16) + r, q = A side (row i) = l%16
-	if (R) {  // residue form: bin = (modulus bin.level, k chunk), plane of acc mod p
+	if constexpr (RES) {  // residue form: bin = (modulus bin.level, k chunk), acc mod p, 4 columns a word
 		uint8_t *Rb = R + (int64_t)binIdx * tstride;
 		const int p = rc->p[bin.level];
-		const double invp = rc->invpd[bin.level];
+		const float invpf = rc->invpf[bin.level];
 		for (int mi = 0; mi < 4; mi++)
-			for (int nj = 0; nj < 4; nj++)
+			for (int nj = 0; nj < 4; nj++) {
+				const int i = i0 + wm + 16 * mi + l % 16;
+				const int j = j0 + wn + 16 * nj + 4 * (l / 16);
+				uint32_t w = 0;
 				for (int r = 0; r < 4; r++) {
-					const int i = i0 + wm + 16 * mi + l % 16;
-					const int j = j0 + wn + 16 * nj + 4 * (l / 16) + r;
 					const int32_t v = iacc[mi][nj][r] + (int32_t)facc[mi][nj][r];
-					if (i < M && j < N) Rb[(int64_t)j * ldt + i] = (uint8_t)mod_small(v, p, invp);
+					w |= (uint32_t)mod_i32(v, p, invpf) << (8 * r);
 				}
+				if (i < M && j < N) *(uint32_t *)(Rb + res_word_offset(i, j, ldt)) = w;
+			}
 		return;
 	}
 	int32_t *Tb = T + (int64_t)binIdx * tstride;
@@ -219,29 +222,35 @@ void launch_digit_gemm(hipStream_t st, int elem, int M, int N, int Kpad, const v
 	const int tilesM = (M + BM - 1) / BM, tilesN = (N + BN - 1) / BN;
 	if (elem == 1) {
 		(void)ldbp;  // both operands share the padded k extent ldap
+#define OZ_LAUNCH_I8(TMV, RESV)                                                                           \
+	do {                                                                                              \
+		const int tm = (M + 64 * TMV - 1) / (64 * TMV), tn = (N + 64 * TMV - 1) / (64 * TMV);          \
+		hipLaunchKernelGGL((k_digit_gemm_i8<TMV, TMV, RESV>), dim3(tileList ? ntiles : tm * tn, nbins), \
+		                   dim3(256), 0, st, (const int8_t *)Ap, apstride, (const int8_t *)Bp, bpstride, \
+		                   ldap, pairs, bins, binOrder, T, ldt, tstride, tm, tileList, rc, R);         \
+	} while (0)
 		if (M >= 512 && N >= 512) {
-			constexpr int TM = 4, TN = 4;
-			const int tm = (M + 64 * TM - 1) / (64 * TM), tn = (N + 64 * TN - 1) / (64 * TN);
-			hipLaunchKernelGGL((k_digit_gemm_i8<TM, TN>), dim3(tileList ? ntiles : tm * tn, nbins), dim3(256),
-			                   0, st, (const int8_t *)Ap, apstride, (const int8_t *)Bp, bpstride, ldap,
-			                   pairs, bins, binOrder, T, ldt, tstride, tm, tileList, rc, R);
+			if (R) OZ_LAUNCH_I8(4, true);
+			else OZ_LAUNCH_I8(4, false);
 		} else {
-			constexpr int TM = 2, TN = 2;
-			const int tm = (M + 64 * TM - 1) / (64 * TM), tn = (N + 64 * TN - 1) / (64 * TN);
-			hipLaunchKernelGGL((k_digit_gemm_i8<TM, TN>), dim3(tileList ? ntiles : tm * tn, nbins), dim3(256),
-			                   0, st, (const int8_t *)Ap, apstride, (const int8_t *)Bp, bpstride, ldap,
-			                   pairs, bins, binOrder, T, ldt, tstride, tm, tileList, rc, R);
+			if (R) OZ_LAUNCH_I8(2, true);
+			else OZ_LAUNCH_I8(2, false);
 		}
+#undef OZ_LAUNCH_I8
 	} else {
 		const dim3 hgrid(tileList ? ntiles : tilesM * tilesN, nbins);
-		if (elem == 2)
-			hipLaunchKernelGGL(k_digit_gemm_h<h8v>, hgrid, dim3(256), 0, st, M, N, (const char *)Ap,
-			                   2 * ldap, 2 * apstride, (const char *)Bp, 2 * ldbp, 2 * bpstride, pairs,
-			                   bins, binOrder, T, ldt, tstride, tilesM, tileList, rc, R);
-		else
-			hipLaunchKernelGGL(k_digit_gemm_h<b8v>, hgrid, dim3(256), 0, st, M, N, (const char *)Ap,
-			                   2 * ldap, 2 * apstride, (const char *)Bp, 2 * ldbp, 2 * bpstride, pairs,
-			                   bins, binOrder, T, ldt, tstride, tilesM, tileList, rc, R);
+#define OZ_LAUNCH_H(V, RESV)                                                                       \
+	hipLaunchKernelGGL((k_digit_gemm_h<V, RESV>), hgrid, dim3(256), 0, st, M, N, (const char *)Ap, \
+	                   2 * ldap, 2 * apstride, (const char *)Bp, 2 * ldbp, 2 * bpstride, pairs, bins,  \
+	                   binOrder, T, ldt, tstride, tilesM, tileList, rc, R)
+		if (elem == 2) {
+			if (R) OZ_LAUNCH_H(h8v, true);
+			else OZ_LAUNCH_H(h8v, false);
+		} else {
+			if (R) OZ_LAUNCH_H(b8v, true);
+			else OZ_LAUNCH_H(b8v, false);
+		}
+#undef OZ_LAUNCH_H
 	}
 }
 

@@ -79,9 +79,12 @@ static void build_res_consts() {
 			lg += std::log2((double)ms[t]);
 		}
 		g_log2M[nmod] = lg;
-		// limbs carried: X and sum_t c_t M/p_t < nmod M < 2^(bits(M) + 6), plus a sign bit
-		const int bitsM = 32 * (int)M.size();
-		c.nl = (bitsM + 8 + 31) / 32;
+		// limbs carried: sum_t c_t M/p_t < nmod M and |X| <= M/4, so bits(nmod M) plus a sign bit
+		std::vector<uint32_t> nM = M;
+		limb_mul_small(nM, (uint32_t)nmod);
+		while (nM.size() > 1 && nM.back() == 0) nM.pop_back();
+		const int bitsNM = 32 * ((int)nM.size() - 1) + (32 - __builtin_clz(nM.back()));
+		c.nl = (bitsNM + 1 + 31) / 32;
 		if (c.nl > kResMaxLimb) c.nl = kResMaxLimb;
 		for (size_t q = 0; q < M.size() && q < (size_t)kResMaxLimb; q++) c.M[q] = M[q];
 		for (int t = 0; t < nmod; t++) {
@@ -180,35 +183,32 @@ __global__ __launch_bounds__(256) void k_res_recon(int M, int N, const uint8_t *
                                                    int64_t rstride, int nchunk, const ResConst *rc,
                                                    int nmod, const int *rowLo, const int *colLo,
                                                    double alpha, double beta, double *C, int64_t ldc) {
-	const int64_t total = (int64_t)M * N;
+	const int G = (N + 3) / 4;
+	const int64_t total = (int64_t)M * G;
 	for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-	     e += (int64_t)gridDim.x * blockDim.x) {
-		const int i = (int)(e % M), j = (int)(e / M);
-		const double t = res_elem_value<NW>(R + (int64_t)j * ldr + i, rstride, nchunk, rc, nmod,
-		                                    rowLo[i] + colLo[j]);
-		double *c = C + (int64_t)j * ldc + i;
-		*c = fma(alpha, t, beta * *c);
-	}
+	     e += (int64_t)gridDim.x * blockDim.x)
+		res_recon_group<NW>(M, N, R, ldr, rstride, nchunk, rc, nmod, rowLo, colLo, alpha, beta, C, ldc,
+		                    (int)(e % M), (int)(e / M));
 }
 
 void launch_res_recon(hipStream_t st, int M, int N, const uint8_t *R, int64_t ldr, int64_t rstride,
                       int nchunk, const ResConst *rc, int nmod, int nl, const int *rowLo,
                       const int *colLo, double alpha, double beta, double *C, int64_t ldc) {
-	int64_t g = ((int64_t)M * N + 255) / 256;
+	int64_t g = ((int64_t)M * ((N + 3) / 4) + 255) / 256;
 	if (g > 16384) g = 16384;
 	const int nw = (nl + 1) / 2;
+#define OZ_RES_RECON(NWV)                                                                              \
+	hipLaunchKernelGGL(k_res_recon<NWV>, dim3((unsigned)g), dim3(256), 0, st, M, N, R, ldr, rstride, nchunk, \
+	                   rc, nmod, rowLo, colLo, alpha, beta, C, ldc)
 	if (nw <= 3)
-		hipLaunchKernelGGL(k_res_recon<3>, dim3((unsigned)g), dim3(256), 0, st, M, N, R, ldr, rstride, nchunk,
-		                   rc, nmod, rowLo, colLo, alpha, beta, C, ldc);
+		OZ_RES_RECON(3);
 	else if (nw <= 4)
-		hipLaunchKernelGGL(k_res_recon<4>, dim3((unsigned)g), dim3(256), 0, st, M, N, R, ldr, rstride, nchunk,
-		                   rc, nmod, rowLo, colLo, alpha, beta, C, ldc);
+		OZ_RES_RECON(4);
 	else if (nw <= 6)
-		hipLaunchKernelGGL(k_res_recon<6>, dim3((unsigned)g), dim3(256), 0, st, M, N, R, ldr, rstride, nchunk,
-		                   rc, nmod, rowLo, colLo, alpha, beta, C, ldc);
+		OZ_RES_RECON(6);
 	else
-		hipLaunchKernelGGL(k_res_recon<9>, dim3((unsigned)g), dim3(256), 0, st, M, N, R, ldr, rstride, nchunk,
-		                   rc, nmod, rowLo, colLo, alpha, beta, C, ldc);
+		OZ_RES_RECON(9);
+#undef OZ_RES_RECON
 }
 
 }  // namespace oz

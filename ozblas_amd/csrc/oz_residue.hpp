@@ -107,42 +107,84 @@ __device__ __forceinline__ int mod_f(int v, int p, float invpf) {
 	return r;
 }
 
-// X (NL 32-bit limbs, two's complement) of one element from its residues, then RN(X 2^e0)
+// The four elements (i, 4g .. 4g + 3) of one product-residue word: X of each from its residues
+// (nmod moduli, nchunk k-chunk planes per modulus summed modulo p), then t[e] = RN(X 2^e0[e]).
+// Rw addresses the word in plane 0; plane b is rstride bytes further.
 template <int NW>
-__device__ inline double res_elem_value(const uint8_t *Re, int64_t rstride, int nchunk,
-                                        const ResConst *rc, int nmod, int e0) {
+__device__ inline void res_group_values(const uint8_t *Rw, int64_t rstride, int nchunk,
+                                        const ResConst *rc, int nmod, const int *e0, double *t) {
 	constexpr int NL = 2 * NW;
-	int64_t acc[NL];
+	int64_t acc[4][NL];
+	double S[4];
 #pragma unroll
-	for (int l = 0; l < NL; l++) acc[l] = 0;
-	double S = 0.0;
-	for (int t = 0; t < nmod; t++) {
-		const int p = rc->p[t];
-		const float ipf = rc->invpf[t];
-		int r = 0;
-		for (int c = 0; c < nchunk; c++) r += Re[(int64_t)(t * nchunk + c) * rstride];
-		if (nchunk > 1) r = mod_f(r, p, ipf);
-		const int ct = mod_f(r * rc->u[t], p, ipf);  // < 2^16 before the reduction
-		S += (double)ct * rc->invpd[t];
+	for (int e = 0; e < 4; e++) {
+		S[e] = 0.0;
 #pragma unroll
-		for (int l = 0; l < NL; l++) acc[l] += (int64_t)((uint64_t)(uint32_t)ct * rc->Mp[t][l]);
+		for (int l = 0; l < NL; l++) acc[e][l] = 0;
 	}
-	const int64_t K = (int64_t)rint(S);
+	for (int q = 0; q < nmod; q++) {
+		const int p = rc->p[q];
+		const float ipf = rc->invpf[q];
+		const int u = rc->u[q];
+		const double ipd = rc->invpd[q];
+		int rs[4] = {0, 0, 0, 0};
+		for (int c = 0; c < nchunk; c++) {
+			const uint32_t w = *(const uint32_t *)(Rw + (int64_t)(q * nchunk + c) * rstride);
 #pragma unroll
-	for (int l = 0; l < NL; l++) acc[l] -= K * (int64_t)rc->M[l];
-	BigInt<NW> X;
-	int64_t carry = 0;
+			for (int e = 0; e < 4; e++) rs[e] += (int)((w >> (8 * e)) & 255);
+		}
+		uint32_t mp[NL];
 #pragma unroll
-	for (int q = 0; q < NW; q++) {
-		const int64_t v0 = acc[2 * q] + carry;
-		const uint64_t w0 = (uint32_t)v0;
-		carry = v0 >> 32;
-		const int64_t v1 = acc[2 * q + 1] + carry;
-		const uint64_t w1 = (uint32_t)v1;
-		carry = v1 >> 32;
-		X.w[q] = w0 | (w1 << 32);
+		for (int l = 0; l < NL; l++) mp[l] = rc->Mp[q][l];
+#pragma unroll
+		for (int e = 0; e < 4; e++) {
+			const int r = (nchunk > 1) ? mod_f(rs[e], p, ipf) : rs[e];
+			const int ct = mod_f(r * u, p, ipf);  // < 2^16 before the reduction
+			S[e] += (double)ct * ipd;
+#pragma unroll
+			for (int l = 0; l < NL; l++) acc[e][l] += (int64_t)((uint64_t)(uint32_t)ct * mp[l]);
+		}
 	}
-	return big_finish<NW>(X, e0);
+#pragma unroll
+	for (int e = 0; e < 4; e++) {
+		const int64_t K = (int64_t)rint(S[e]);
+		BigInt<NW> X;
+		int64_t carry = 0;
+#pragma unroll
+		for (int q = 0; q < NW; q++) {
+			const int64_t v0 = acc[e][2 * q] - K * (int64_t)rc->M[2 * q] + carry;
+			const uint64_t w0 = (uint32_t)v0;
+			carry = v0 >> 32;
+			const int64_t v1 = acc[e][2 * q + 1] - K * (int64_t)rc->M[2 * q + 1] + carry;
+			const uint64_t w1 = (uint32_t)v1;
+			carry = v1 >> 32;
+			X.w[q] = w0 | (w1 << 32);
+		}
+		t[e] = big_finish<NW>(X, e0[e]);
+	}
+}
+
+// one thread per (row i, column group g) of an M x N product: C(i, 4g + e) = fma(alpha, RN(X 2^(lo_i +
+// lo_j)), beta C) for the columns of the group inside N
+template <int NW>
+__device__ inline void res_recon_group(int M, int N, const uint8_t *R, int64_t ldr, int64_t rstride,
+                                       int nchunk, const ResConst *rc, int nmod, const int *rowLo,
+                                       const int *colLo, double alpha, double beta, double *C,
+                                       int64_t ldc, int i, int g) {
+	int e0[4];
+	const int li = rowLo[i];
+#pragma unroll
+	for (int e = 0; e < 4; e++) e0[e] = li + ((4 * g + e < N) ? colLo[4 * g + e] : 0);
+	double t[4];
+	res_group_values<NW>(R + res_word_offset(i, 4 * g, ldr), rstride, nchunk, rc, nmod, e0, t);
+#pragma unroll
+	for (int e = 0; e < 4; e++) {
+		const int j = 4 * g + e;
+		if (j < N) {
+			double *c = C + (int64_t)j * ldc + i;
+			*c = fma(alpha, t[e], beta * *c);
+		}
+	}
 }
 
 }  // namespace oz

@@ -66,7 +66,7 @@ def test_moduli_and_constants(rc):
         M = math.prod(ps[:nmod])
         assert abs(lg - math.log2(M)) < 1e-9
         # X and sum_t c_t M/p_t < nmod M fit the carried limbs with a sign bit
-        assert 32 * int(c["nl"]) >= M.bit_length() + math.ceil(math.log2(nmod + 1)) + 1
+        assert 32 * int(c["nl"]) >= (nmod * M).bit_length() + 1
         assert [int(w) for w in c["M"]] == limbs(M, MAXLIMB)
         for t in range(nmod):
             p = ps[t]
@@ -156,3 +156,29 @@ def test_reconstruction_is_exact_within_quarter(rc, nmod):
             n += 1
         if n <= MAXMOD:
             assert rc[n][1] >= need and (n == 1 or rc[n - 1][1] < need + 1e-6)
+
+
+def mod_i32(v, p, invpf):
+    """mod_i32 of the MFMA epilogues restated in binary32 (numpy) and int32 arithmetic."""
+    q1 = int(np.rint(np.float32(np.float32(v) * invpf)))
+    r = v - q1 * p
+    assert -(2 ** 31) <= q1 * p < 2 ** 31
+    q2 = int(np.floor(np.float32(np.float32(r) * invpf)))
+    r -= q2 * p
+    r = r + p if r < 0 else r
+    return r - p if r >= p else r
+
+
+def test_epilogue_mod_against_integers(rc):
+    """acc mod p in the MFMA epilogues for |acc| <= 2^30 (65,536 k x 2^14 per chunk), every modulus
+    (256 included), at the ends of the range and near multiples of p."""
+    c, _ = rc[MAXMOD]
+    rng = random.Random(11)
+    vs = [0, 1, -1, 2 ** 30, -(2 ** 30), 2 ** 30 - 1, -(2 ** 30) + 1, 2 ** 24, 2 ** 24 + 1, -(2 ** 24) - 3]
+    vs += [rng.randrange(-(2 ** 30), 2 ** 30 + 1) for _ in range(300)]
+    for t in range(MAXMOD):
+        p = int(c["p"][t])
+        near = [q * p + d for q in (rng.randrange(-(2 ** 30) // p, 2 ** 30 // p) for _ in range(20))
+                for d in (-1, 0, 1)]
+        for v in vs + [x for x in near if abs(x) <= 2 ** 30]:
+            assert mod_i32(v, p, c["invpf"][t]) == v % p, (p, v)
