@@ -361,7 +361,7 @@ static int32_t digit_split(ozblasHandle_t *oh, int elem, int k, int cols, const 
 	d.colE = (int *)walloc(oh, sizeof(int) * (size_t)cols * 2);
 	ExpStatsH *dst = (ExpStatsH *)walloc(oh, sizeof(ExpStatsH));
 	if (!d.colE || !dst) return OZBLAS_STATUS_WORKSPACE;
-	ExpStatsH init{0, INT32_MAX, INT32_MIN, 0, INT32_MIN, {0, 0, 0}};
+	ExpStatsH init{0, INT32_MAX, INT32_MIN, 0, INT32_MIN, INT32_MIN, {0, 0}};
 	OZ_HIP(hipMemcpyAsync(dst, &init, sizeof(init), hipMemcpyHostToDevice, b->stream));
 	launch_digit_stats(b->stream, k, cols, in, ldi, d.colE, d.colE + cols, dst);
 	OZ_HIP(hipMemcpyAsync(b->hPinned, dst, sizeof(ExpStatsH), hipMemcpyDeviceToHost, b->stream));
@@ -595,7 +595,7 @@ static int32_t res_stats(ozblasHandle_t *oh, int k, int cols, const double *in, 
 	r.dst = (ExpStatsH *)walloc(oh, sizeof(ExpStatsH));
 	if (!r.colE || !r.dst) return OZBLAS_STATUS_WORKSPACE;
 	r.colLo = r.colE + 2 * (size_t)cols;
-	static const ExpStatsH init{0, INT32_MAX, INT32_MIN, 0, INT32_MIN, {0, 0, 0}};
+	static const ExpStatsH init{0, INT32_MAX, INT32_MIN, 0, INT32_MIN, INT32_MIN, {0, 0}};
 	OZ_HIP(hipMemcpyAsync(r.dst, &init, sizeof(init), hipMemcpyHostToDevice, b->stream));
 	launch_digit_stats(b->stream, k, cols, in, ldi, r.colE, r.colE + cols, r.dst, r.colLo);
 	return OZBLAS_STATUS_SUCCESS;
@@ -654,10 +654,11 @@ static int32_t gemm_residue(ozblasHandle_t *oh, char transA, char transB, int m,
 		wreset(oh, base);
 		return OZBLAS_STATUS_SUCCESS;
 	}
-	// |X| < k 2^(spanA + spanB) must stay within M / 4
+	// |X| must stay within M / 4; two bounds hold: |X| < k 2^(spanA + spanB) and, by Cauchy-Schwarz,
+	// |X| <= ||a'_i|| ||b'_j|| < 2^(normA + normB); the smaller one decides
 	int lk = 0;
 	while ((int64_t(1) << lk) < (int64_t)k) lk++;
-	const double need = (double)ra.st.maxSpan + (double)rb.st.maxSpan + lk + 2;
+	const double need = (double)std::min(ra.st.maxSpan + rb.st.maxSpan + lk, ra.st.maxNorm + rb.st.maxNorm) + 2;
 	if (ra.st.maxSpan >= kResMaxShift || rb.st.maxSpan >= kResMaxShift) return give_up();
 	int nmod = 1;
 	while (nmod <= kResMaxMod && res_log2_product(nmod) < need + 1e-6) nmod++;
@@ -677,23 +678,32 @@ static int32_t gemm_residue(ozblasHandle_t *oh, char transA, char transB, int m,
 	ra.planes = walloc(oh, esz * (size_t)ra.pstride * nmod);
 	rb.planes = walloc(oh, esz * (size_t)rb.pstride * nmod);
 	const int Kpad = (int)ra.ldp;
-	const int kc = 65536;  // |acc| <= kc 2^14 < 2^31 per chunk
+	const int kc = kResChunk;
 	const int nchunk = (Kpad + kc - 1) / kc;
-	const int nbins = nmod * nchunk;
+	// bins: groups of consecutive moduli over one k chunk, one workgroup running its group's products
+	// back to back; groups small enough to give the grid about 1024 (tile, bin) items
+	const int tile = digit_gemm_tile(elem, m, n);
+	const int64_t tiles = (int64_t)((m + tile - 1) / tile) * ((n + tile - 1) / tile);
+	int ngroups = (int)std::min<int64_t>(nmod, std::max<int64_t>(1, (1024 + tiles * nchunk - 1) / (tiles * nchunk)));
+	// one modulus per bin for the half-precision kernel and the 256 x 256 INT8 tile, whose
+	// registers leave no room to flush inside the loop
+	if (elem != 1 || tile == 256) ngroups = nmod;
+	const int G = (nmod + ngroups - 1) / ngroups;
 	std::vector<int2> pairs(nmod);
-	std::vector<LevelBin> bins(nbins);
-	std::vector<int> order(nbins);
-	for (int t = 0; t < nmod; t++) {
-		pairs[t] = make_int2(t, t);
-		for (int c = 0; c < nchunk; c++) {
-			LevelBin &bn = bins[t * nchunk + c];
-			bn.level = t;
-			bn.npairs = 1;
-			bn.pair0 = t;
+	std::vector<LevelBin> bins;
+	for (int t = 0; t < nmod; t++) pairs[t] = make_int2(t, t);
+	for (int c = 0; c < nchunk; c++)
+		for (int t0 = 0; t0 < nmod; t0 += G) {
+			LevelBin bn;
+			bn.level = t0;
+			bn.pair0 = t0;
+			bn.npairs = std::min(G, nmod - t0);
 			bn.k0 = c * kc;
 			bn.k1 = std::min(Kpad, (c + 1) * kc);
+			bins.push_back(bn);
 		}
-	}
+	const int nbins = (int)bins.size();
+	std::vector<int> order(nbins);
 	for (int q = 0; q < nbins; q++) order[q] = q;
 	// product residues: whole INT8 tiles, or (FP16/BF16) every 4-column word of the m x n product
 	const int64_t ldt = (elem == 1) ? rup(m, 256) : rup(m, 64);
@@ -701,7 +711,7 @@ static int32_t gemm_residue(ozblasHandle_t *oh, char transA, char transB, int m,
 	int2 *dPairs = (int2 *)walloc(oh, sizeof(int2) * pairs.size());
 	LevelBin *dBins = (LevelBin *)walloc(oh, sizeof(LevelBin) * bins.size());
 	int *dOrder = (int *)walloc(oh, sizeof(int) * order.size());
-	uint8_t *Rp = (uint8_t *)walloc(oh, (size_t)tstride * nbins);
+	uint8_t *Rp = (uint8_t *)walloc(oh, (size_t)tstride * nmod * nchunk);
 	if (!dRc || !ra.planes || !rb.planes || !dPairs || !dBins || !dOrder || !Rp) {
 		wreset(oh, base);
 		return OZBLAS_STATUS_WORKSPACE;
@@ -712,9 +722,9 @@ static int32_t gemm_residue(ozblasHandle_t *oh, char transA, char transB, int m,
 	OZ_HIP(hipMemcpyAsync(dBins, bins.data(), sizeof(LevelBin) * bins.size(), hipMemcpyHostToDevice, st));
 	OZ_HIP(hipMemcpyAsync(dOrder, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, st));
 	OZ_HIP(hipEventRecord(b->ev[1], st));
-	launch_res_write(st, elem, k, m, ain, ldai, ra.colLo, dRc, nmod, ra.planes, ra.ldp, ra.pstride);
+	launch_res_write(st, elem, k, m, ain, ldai, ra.colLo, dRc, nmod, ra.st.maxSpan, ra.planes, ra.ldp, ra.pstride);
 	OZ_HIP(hipEventRecord(b->ev[2], st));
-	launch_res_write(st, elem, k, n, bin, ldbi, rb.colLo, dRc, nmod, rb.planes, rb.ldp, rb.pstride);
+	launch_res_write(st, elem, k, n, bin, ldbi, rb.colLo, dRc, nmod, rb.st.maxSpan, rb.planes, rb.ldp, rb.pstride);
 	OZ_HIP(hipEventRecord(b->ev[3], st));
 	launch_digit_gemm(st, elem, m, n, Kpad, ra.planes, ra.ldp, ra.pstride, rb.planes, rb.ldp,
 	                  rb.pstride, dPairs, dBins, nbins, dOrder, nullptr, ldt, tstride, nullptr, 0, dRc, Rp);

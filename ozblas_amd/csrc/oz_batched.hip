@@ -55,6 +55,7 @@ struct GWrite {  // tiled INT8 residue planes of one operand
 	int8_t *planes;
 	int64_t ldp, pstride;
 	int nmod, kts;
+	int shLimit, pad;  // 2^sh mod P tables staged for sh below the planned span
 };
 struct GProb {  // one problem of the grouped MFMA launch
 	const int8_t *A, *B;
@@ -63,7 +64,7 @@ struct GProb {  // one problem of the grouped MFMA launch
 	int64_t ldt, tstride;
 	const LevelBin *bins;
 	const int2 *pairs;
-	int tilesM, pad;
+	int tilesM, nmod;
 };
 struct GItem {
 	int prob, bin, tile, pad;
@@ -79,8 +80,11 @@ struct GRec {  // Chinese-remainder reconstruction + epilogue of one problem
 };
 struct GCheck {  // plan assumptions of one problem, re-checked on every execution
 	int opA, opB;    // operand indices
-	int spanBudget;  // largest spanA + spanB the problem's moduli cover (-1: planned as a zero product)
+	int spanBudget;  // largest bound bits (min of spanA + spanB + log2 k and normA + normB) the moduli cover
+	                 // (-1: planned as a zero product)
+	int lk;          // ceil(log2 k)
 	int spanLimit;   // largest span for which the plain split is exact (the fast-form precondition)
+	int shA, shB;    // spans the residue writer's tables were staged for
 };
 
 // largest o < n with start[o] <= b (start[] ascending, start[0] = 0)
@@ -103,6 +107,7 @@ __global__ void k_batch_reset(ExpStats *st, int nops, int *flag, int *counter) {
 		st[o].maxE = INT32_MIN;
 		st[o].nonfinite = 0;
 		st[o].maxSpan = INT32_MIN;
+		st[o].maxNorm = INT32_MIN;
 	}
 	if (blockIdx.x == 0 && threadIdx.x == 0) {
 		*flag = 0;
@@ -165,8 +170,10 @@ __global__ void k_check_grouped(const GCheck *c, int n, const ExpStats *st, int 
 		const ExpStats a = st[q.opA], b = st[q.opB];
 		bool bad = a.nonfinite || b.nonfinite;
 		if (a.maxS > 0 && b.maxS > 0) {
-			bad = bad || q.spanBudget < 0 || a.maxSpan + b.maxSpan > q.spanBudget;
+			bad = bad || q.spanBudget < 0 ||
+			      min(a.maxSpan + b.maxSpan + q.lk, a.maxNorm + b.maxNorm) > q.spanBudget;
 			bad = bad || a.maxSpan >= kResMaxShift || b.maxSpan >= kResMaxShift;
+			bad = bad || a.maxSpan > q.shA || b.maxSpan > q.shB;
 			bad = bad || a.maxSpan > q.spanLimit || b.maxSpan > q.spanLimit;
 			bad = bad || a.minLsb + b.minLsb < -1000 || a.maxE + b.maxE > 1000;
 			bad = bad || a.minLsb < -1000 || b.minLsb < -1000;
@@ -175,13 +182,14 @@ __global__ void k_check_grouped(const GCheck *c, int n, const ExpStats *st, int 
 	}
 }
 
+// one workgroup per (operand, tile): the tables of the operand's moduli, staged up to its span
 __global__ __launch_bounds__(256) void k_write_grouped(const GWrite *g, const int *start, int n,
                                                        const ResConst *rc) {
 	__shared__ ResLds T;
-	res_lds_load(T, rc, kResMaxMod);
-	__syncthreads();
 	const int o = owner(start, n, blockIdx.x);
 	const GWrite w = g[o];
+	res_lds_load(T, rc, w.nmod, w.shLimit);
+	__syncthreads();
 	const int lb = blockIdx.x - start[o];
 	res_write_tile_i8(w.m, w.n, w.in, w.ldi, w.colLo, w.nmod, T, w.planes, w.ldp, w.pstride, lb % w.kts,
 	                  lb / w.kts);
@@ -224,9 +232,9 @@ __global__ __launch_bounds__(256, 1) void k_gemm_grouped(const GProb *P, const G
 		g.j0 = (w.tile / p.tilesM) * TBN;
 		g.T = nullptr;
 		g.ldt = p.ldt;
-		g.R = p.R + (int64_t)w.bin * p.tstride;  // bin = (modulus bin.level, k chunk)
-		g.p = rc->p[bin.level];
-		g.invpf = rc->invpf[bin.level];
+		g.R = p.R + (int64_t)(bin.k0 / kResChunk) * p.nmod * p.tstride;  // bin: moduli over one k chunk
+		g.rstride = p.tstride;
+		g.rc = rc;
 		i8_tile<TM, TN, true>(g, stage0, stage1);
 	}
 }
@@ -513,7 +521,7 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 		int q, nmod, nchunk, budget;
 		int64_t ldp, apstride, bpstride, ldt, tstride;
 		uint64_t planesA, planesB, R;
-		int bin0, nbins;
+		int bin0[2], nbins[2];  // per tile shape: all moduli of a chunk per bin (128), one modulus (256)
 	};
 	std::vector<ProbH> probs;
 	for (int q : live) {
@@ -529,7 +537,7 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 				p->fallback.push_back(q);
 				continue;
 			}
-			const double need = (double)sa.maxSpan + sb.maxSpan + ceil_log2(k[q]) + 2;
+			const double need = (double)std::min(sa.maxSpan + sb.maxSpan + ceil_log2(k[q]), sa.maxNorm + sb.maxNorm) + 2;
 			int nm = 1;
 			while (nm <= kResMaxMod && res_log2_product(nm) < need + 1e-6) nm++;
 			if (nm > kResMaxMod) {  // beyond the moduli: ozblasDgemm takes the problem alone
@@ -537,7 +545,7 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 				continue;
 			}
 			h.nmod = nm;
-			h.budget = (int)std::floor(res_log2_product(nm) - 1e-6) - ceil_log2(k[q]) - 2;
+			h.budget = (int)std::floor(res_log2_product(nm) - 1e-6) - 2;
 		} else {
 			h.nmod = 0;  // an all-zero operand: C = fma(alpha, +0, beta C)
 			h.budget = -1;
@@ -547,7 +555,7 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 	}
 	const int cap = split_cap(oh);
 	Layout L2;
-	std::vector<LevelBin> hBins;
+	std::vector<LevelBin> hBins;  // the bins of both tile shapes
 	std::vector<int2> hPairs;
 	for (int t = 0; t < kResMaxMod; t++) hPairs.push_back(make_int2(t, t));  // product t: plane t x plane t
 	int maxNl = 2;
@@ -559,17 +567,30 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 		h.bpstride = h.ldp * rup64(n[q], 256);
 		h.ldt = rup64(m[q], 256);
 		h.tstride = h.ldt * rup64(n[q], 256);
-		h.bin0 = (int)hBins.size();
-		h.nbins = 0;
+		h.nbins[0] = h.nbins[1] = 0;
+		h.bin0[0] = h.bin0[1] = (int)hBins.size();
 		h.nchunk = 1;
 		if (h.nmod > 0) {
 			h.planesA = L2.take((uint64_t)h.apstride * h.nmod);
 			h.planesB = L2.take((uint64_t)h.bpstride * h.nmod);
 			const int Kpad = (int)h.ldp;
-			const int kc = 65536;  // |acc| <= kc 2^14 < 2^31 per chunk
+			const int kc = kResChunk;
 			h.nchunk = (Kpad + kc - 1) / kc;
-			for (int t = 0; t < h.nmod; t++)
-				for (int c = 0; c < h.nchunk; c++) {
+			// 128 x 128 tiles: one item streams every modulus of a tile and chunk through one
+			// pipeline; 256 x 256 tiles: one modulus per item
+			for (int c = 0; c < h.nchunk; c++) {
+				LevelBin bn;
+				bn.level = 0;
+				bn.pair0 = 0;
+				bn.npairs = h.nmod;
+				bn.k0 = c * kc;
+				bn.k1 = std::min(Kpad, (c + 1) * kc);
+				hBins.push_back(bn);
+			}
+			h.nbins[0] = (int)hBins.size() - h.bin0[0];
+			h.bin0[1] = (int)hBins.size();
+			for (int c = 0; c < h.nchunk; c++)
+				for (int t = 0; t < h.nmod; t++) {
 					LevelBin bn;
 					bn.level = t;
 					bn.pair0 = t;
@@ -578,8 +599,8 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 					bn.k1 = std::min(Kpad, (c + 1) * kc);
 					hBins.push_back(bn);
 				}
-			h.nbins = (int)hBins.size() - h.bin0;
-			h.R = L2.take((uint64_t)h.tstride * h.nbins);
+			h.nbins[1] = (int)hBins.size() - h.bin0[1];
+			h.R = L2.take((uint64_t)h.tstride * h.nmod * h.nchunk);
 			p->totalPairs += h.nmod;
 			p->ops += 2.0 * m[q] * (double)n[q] * k[q] * h.nmod;
 			maxNl = std::max(maxNl, res_const_host(h.nmod).nl);
@@ -607,7 +628,7 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 	for (const ProbH &h : probs)
 		for (int c = 0; c < 2; c++) {
 			const int TB = c == 0 ? 128 : 256;
-			nItemsMax[c] += (size_t)h.nbins * ((m[h.q] + TB - 1) / TB) * ((n[h.q] + TB - 1) / TB);
+			nItemsMax[c] += (size_t)h.nbins[c] * ((m[h.q] + TB - 1) / TB) * ((n[h.q] + TB - 1) / TB);
 		}
 	const uint64_t writeOff = L2.take(sizeof(GWrite) * (2 * probs.size() + 1));
 	const uint64_t writeStartOff = L2.take(4 * (2 * probs.size() + 1));
@@ -644,6 +665,8 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 				w.pstride = side == 0 ? h.apstride : h.bpstride;
 				w.nmod = h.nmod;
 				w.kts = (int)(h.ldp / 64);
+				w.shLimit = std::max(1, p->hSt[o].maxSpan);
+				w.pad = 0;
 				writeStart.push_back(wb);
 				wb += w.kts * (int)(rup64(w.n, 256) / 16);
 				hWrite.push_back(w);
@@ -659,15 +682,15 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 				g.R = (uint8_t *)(M2 + h.R);
 				g.ldt = h.ldt;
 				g.tstride = h.tstride;
-				g.bins = dBins + h.bin0;
+				g.bins = dBins + h.bin0[c];
 				g.pairs = dPairs;
 				g.tilesM = (m[q] + TB - 1) / TB;
-				g.pad = 0;
+				g.nmod = h.nmod;
 				const int pi = (int)hProb[c].size();
 				hProb[c].push_back(g);
 				const int tiles = g.tilesM * ((n[q] + TB - 1) / TB);
-				for (int bi = 0; bi < h.nbins; bi++) {
-					const LevelBin &bn = hBins[h.bin0 + bi];
+				for (int bi = 0; bi < h.nbins[c]; bi++) {
+					const LevelBin &bn = hBins[h.bin0[c] + bi];
 					const double cost = (double)bn.npairs * (bn.k1 - bn.k0);
 					for (int t = 0; t < tiles; t++) {
 						hItems[c].push_back(GItem{pi, bi, t, 0});
@@ -698,8 +721,11 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 		ck.opA = 2 * q;
 		ck.opB = 2 * q + 1;
 		ck.spanBudget = h.budget;
+		ck.lk = ceil_log2(std::max(1, k[q]));
 		const int rho = rho_host(k[q], oh->splitEpsMode);
 		ck.spanLimit = (53 - rho - 1) * (cap - 3);
+		ck.shA = std::max(1, p->hSt[2 * q].maxSpan);
+		ck.shB = std::max(1, p->hSt[2 * q + 1].maxSpan);
 		hCheck.push_back(ck);
 	}
 	// longest items first (the work counter then balances the tail)

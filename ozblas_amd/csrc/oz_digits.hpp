@@ -32,7 +32,8 @@ struct ExpStats {  // global reductions written by k_digit_stats
 	int maxE;        // max column scale exponent
 	int nonfinite;   // any Inf/NaN
 	int maxSpan;     // max (top exponent - lowest bit) over columns
-	int pad[3];
+	int maxNorm;     // max over columns of an integer upper bound on log2 ||x 2^-lowest bit||_2
+	int pad[2];
 };
 
 // exponent of the lowest set bit of a finite nonzero double
@@ -61,12 +62,13 @@ __device__ inline int block_max_i(int v, int *red) {
 // Global reductions of the column statistics, accumulated by thread 0 of a workgroup over its
 // columns and folded into ExpStats once (few atomics on the shared counters).
 struct ColAcc {
-	int minLsb = INT32_MAX, maxSpan = INT32_MIN, maxS = 0, maxE = INT32_MIN, bad = 0;
+	int minLsb = INT32_MAX, maxSpan = INT32_MIN, maxS = 0, maxE = INT32_MIN, bad = 0, maxNorm = INT32_MIN;
 };
 
 __device__ inline void col_acc_flush(const ColAcc &a, ExpStats *st) {
 	if (a.minLsb != INT32_MAX) atomicMin(&st->minLsb, a.minLsb);
 	if (a.maxSpan != INT32_MIN) atomicMax(&st->maxSpan, a.maxSpan);
+	if (a.maxNorm != INT32_MIN) atomicMax(&st->maxNorm, a.maxNorm);
 	atomicMax(&st->maxS, a.maxS);
 	if (a.maxE != INT32_MIN) atomicMax(&st->maxE, a.maxE);
 	if (a.bad) atomicOr(&st->nonfinite, 1);
@@ -77,6 +79,18 @@ __device__ inline void col_acc_flush(const ColAcc &a, ExpStats *st) {
 // a multiple of 2^(E - 8S)); thread 0 adds the column to `acc`.  Column j of the operand is
 // x = sum_{s<S} d_s 2^(E - 8(s+1)) with d_s in [-128, 127].  Eight loads per thread are issued
 // before they are used.
+template <int BS>
+__device__ inline double block_sum(double v, double *red) {
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+	const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+	__syncthreads();
+	if (l == 0) red[w] = v;
+	__syncthreads();
+	double r = red[0];
+	for (int i = 1; i < BS / 64; i++) r += red[i];
+	return r;
+}
+
 template <int BS>
 __device__ inline void digit_col_stats_acc(const double *x, int m, int *colE, int *colS, ColAcc &acc,
                                            int *red, double *redd, int *colLo = nullptr) {
@@ -100,7 +114,27 @@ __device__ inline void digit_col_stats_acc(const double *x, int m, int *colE, in
 	mx = block_max<BS>(mx, redd);
 	nlo = block_max_i<BS>(nlo, red);
 	bad = block_max_i<BS>(bad, red);
+	// residue form: an upper bound on log2 of the Euclidean norm of the column's integers x 2^-lo
+	// (Cauchy-Schwarz bounds |X| by the product of two such norms).  The squares are summed as
+	// (x 2^-hi)^2 <= 1, in a second pass over the column (in L2); terms that underflow are covered by
+	// the m 2^-1000 added, the rounding of the m-term sum by the factor 1 + m 2^-52.
+	double ss = 0.0;
+	int hiN = 0;
+	if (colLo && mx > 0.0 && !bad) {
+		frexp(mx, &hiN);
+		for (int i = threadIdx.x; i < m; i += BS) {
+			const double v = ldexp(x[i], -hiN);
+			ss += v * v;
+		}
+		ss = block_sum<BS>(ss, redd);
+	}
 	if (threadIdx.x == 0) {
+		if (colLo && mx > 0.0 && !bad) {
+			const double up = sqrt((ss * (1.0 + m * 0x1p-52) + m * 0x1p-1000)) * (1.0 + 0x1p-50);
+			int eN;
+			frexp(up, &eN);  // up < 2^eN
+			acc.maxNorm = max(acc.maxNorm, eN + hiN + nlo);  // log2 ||x 2^-lo|| < eN + hi - lo
+		}
 		int E = 0, S = 0;
 		if (mx > 0.0) {
 			int hi;
@@ -248,9 +282,11 @@ struct I8Tile {
 	int i0, j0;
 	int32_t *T;  // result plane of the bin
 	int64_t ldt;
-	uint8_t *R;   // residue form: result plane of acc mod p in [0, p), instead of T when R is set
-	int p;
-	float invpf;  // 1 / p rounded to binary32
+	// residue form: the pairs are moduli s0 .. s0 + npairs - 1 (plane t of A with plane t of B), and
+	// after each modulus's k range acc mod p goes to plane t of R (t * rstride) instead of T
+	uint8_t *R;
+	int64_t rstride;
+	const ResConst *rc;
 };
 
 // v mod p in [0, p) for |v| <= 2^30 and 3 <= p <= 256 (256 included: 1/256 is exact), in binary32: a first quotient estimate
@@ -319,7 +355,7 @@ __device__ __forceinline__ void i8_tile(const I8Tile &g, char *stage0, char *sta
 	i4v ra0[NA], rb0[NB], ra1[NA], rb1[NB];
 	auto gload = [&](i4v(&ra)[NA], i4v(&rb)[NB]) {
 		const char *Ap = Ab + (int64_t)lpair * apstride + lkt * 2048;
-		const char *Bp = Bb - (int64_t)lpair * bpstride + lkt * 2048;
+		const char *Bp = Bb + (int64_t)(RES ? lpair : -lpair) * bpstride + lkt * 2048;
 #pragma unroll
 		for (int h = 0; h < NA; h++) ra[h] = *(const i4v *)(Ap + 2 * h * rbs);
 #pragma unroll
@@ -361,7 +397,7 @@ __device__ __forceinline__ void i8_tile(const I8Tile &g, char *stage0, char *sta
 	// the slots in this order.
 	auto step = [&](i4v(&raF)[NA], i4v(&rbF)[NB], const i4v(&raN)[NA], const i4v(&rbN)[NB], int cur) {
 		const char *Ap = Ab + (int64_t)lpair * apstride + lkt * 2048;
-		const char *Bp = Bb - (int64_t)lpair * bpstride + lkt * 2048;
+		const char *Bp = Bb + (int64_t)(RES ? lpair : -lpair) * bpstride + lkt * 2048;
 		advance();
 		char *wa = stage(cur ^ 1) + lofs, *wb = stage(cur ^ 1) + TBM * I8_BK + lofs;
 #pragma unroll
@@ -407,6 +443,39 @@ __device__ __forceinline__ void i8_tile(const I8Tile &g, char *stage0, char *sta
 			}
 		}
 	};
+	// residue form: the product of modulus s0 + q is complete after its kst stages; its residues go
+	// to plane s0 + q (four columns per 32-bit word) and the accumulators restart at zero.  With
+	// 2 x 2 wave tiles the flush runs inside the flattened (modulus, k) loop, so a workgroup streams
+	// several moduli through one pipeline; the 4 x 4 tile has no registers left for that inside the
+	// loop (npairs must be 1) and flushes its one modulus after it.
+	constexpr bool FLAT = RES && TM * TN <= 4;
+	int left = kst, mq = 0;
+	auto flush = [&]() {
+		if constexpr (RES) {
+			if (--left > 0) return;
+			left = kst;
+			const int t = g.s0 + mq++;
+			const int p = g.rc->p[t];
+			const float invpf = g.rc->invpf[t];
+			uint8_t *Rb = g.R + (int64_t)t * g.rstride;
+#pragma unroll
+			for (int mi = 0; mi < TM; mi++)
+#pragma unroll
+				for (int nj = 0; nj < TN; nj++)
+#pragma unroll
+					for (int r4 = 0; r4 < 4; r4++) {
+						const int i = i0 + wm + 32 * mi + l % 32;
+						const int j = j0 + wn + 32 * nj + 8 * r4 + 4 * (l / 32);
+						uint32_t w = 0;
+#pragma unroll
+						for (int c = 0; c < 4; c++) {
+							w |= (uint32_t)mod_i32(acc[mi][nj][4 * r4 + c], p, invpf) << (8 * c);
+							acc[mi][nj][4 * r4 + c] = 0;
+						}
+						*(uint32_t *)(Rb + res_word_offset(i, j, g.ldt)) = w;
+					}
+		}
+	};
 	// prologue: stage 0 -> buffer 0, stage 1 -> registers set 1
 	gload(ra0, rb0);
 	gload(ra1, rb1);
@@ -416,34 +485,24 @@ __device__ __forceinline__ void i8_tile(const I8Tile &g, char *stage0, char *sta
 	int s = 0;
 	for (; s + 1 < total; s += 2) {
 		step(ra0, rb0, ra1, rb1, 0);
+		if constexpr (FLAT) flush();
 		step(ra1, rb1, ra0, rb0, 1);
+		if constexpr (FLAT) flush();
 	}
-	if (s < total) step(ra0, rb0, ra1, rb1, 0);
+	if (s < total) {
+		step(ra0, rb0, ra1, rb1, 0);
+		if constexpr (FLAT) flush();
+	}
+	if constexpr (RES) {
+		if constexpr (!FLAT) {
+			left = 1;
+			flush();
+		}
+		return;  // every modulus flushed
+	}
 	// D[p][q]: p = B-side (column j) = 8*(r/4) + 4*(l/32) + r%4, q = A-side (row i) = l%32
 	int32_t *Tb = g.T;
 	const int64_t ldt = g.ldt;
-	if constexpr (RES) {  // residue form: the plane of acc mod p, four columns per word
-		uint8_t *Rb = g.R;
-		const int p = g.p;
-		const float invpf = g.invpf;
-#pragma unroll
-		for (int mi = 0; mi < TM; mi++)
-#pragma unroll
-			for (int nj = 0; nj < TN; nj++)
-#pragma unroll
-				for (int r4 = 0; r4 < 4; r4++) {
-					const int i = i0 + wm + 32 * mi + l % 32;
-					const int j = j0 + wn + 32 * nj + 8 * r4 + 4 * (l / 32);
-					uint32_t w = 0;
-#pragma unroll
-					for (int c = 0; c < 4; c++)
-						w |= (uint32_t)mod_i32(acc[mi][nj][4 * r4 + c], p, invpf) << (8 * c);
-					*(uint32_t *)(Rb + res_word_offset(i, j, ldt)) = w;
-					// one word at a time: interleaving all 64 would hold their accumulators in VGPRs
-					__builtin_amdgcn_sched_barrier(0);
-				}
-		return;
-	}
 #pragma unroll
 	for (int mi = 0; mi < TM; mi++)
 #pragma unroll

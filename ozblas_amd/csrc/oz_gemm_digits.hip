@@ -95,10 +95,10 @@ __global__ __launch_bounds__(256, 1) void k_digit_gemm_i8(const int8_t *A, int64
 	g.T = T + (int64_t)binIdx * tstride;
 	g.ldt = ldt;
 	g.R = nullptr;
-	if constexpr (RES) {  // residue form: bin = (modulus bin.level, k chunk), plane of acc mod p
-		g.R = R + (int64_t)binIdx * tstride;
-		g.p = rc->p[bin.level];
-		g.invpf = rc->invpf[bin.level];
+	if constexpr (RES) {  // residue form: bin = moduli level .. level + npairs - 1 over one k chunk
+		g.R = R + (int64_t)(bin.k0 / kResChunk) * rc->nmod * tstride;
+		g.rstride = tstride;
+		g.rc = rc;
 	}
 	i8_tile<TM, TN, RES>(g, stage0, stage1);
 }

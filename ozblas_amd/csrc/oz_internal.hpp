@@ -284,7 +284,7 @@ int rho_host_ds(int dim, int splitEpsMode);  // getRho<double,float> with the ho
 
 // host image of the digit statistics of one operand (ExpStats of oz_digits.hpp)
 struct ExpStatsH {
-	int maxS, minLsb, maxE, nonfinite, maxSpan, pad[3];
+	int maxS, minLsb, maxE, nonfinite, maxSpan, maxNorm, pad[2];
 };
 
 // True when the plain FP64 split of these operands (digit statistics a of A, b of B, inner

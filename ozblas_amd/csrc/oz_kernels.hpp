@@ -88,6 +88,9 @@ struct LevelBin {      // one exact int32 plane: sum over `npairs` digit pairs, 
 constexpr int kResMaxMod = 49;   // 256 and the 48 odd pairwise coprime moduli up to 255
 constexpr int kResMaxLimb = 18;  // 32-bit limbs of M (<= 392 bits) plus headroom
 constexpr int kResMaxShift = 512;  // element shifts below 2^512 (256^a mod p for a < 64)
+constexpr int kResMaxGroups = (kResMaxMod + 2) / 3;
+constexpr int kResChunk = 65536;   // k per exact int32 product chunk (|acc| <= 2^16 2^14 = 2^30)
+// product residues: plane c * nmod + t holds modulus t over k chunk c
 struct ResConst {
 	int nmod;          // moduli in use
 	int nl;            // 32-bit limbs carried by the reconstruction
@@ -98,6 +101,10 @@ struct ResConst {
 	uint32_t Mp[kResMaxMod][kResMaxLimb];  // M / p, little-endian limbs
 	uint32_t M[kResMaxLimb];
 	uint8_t pow256[kResMaxMod][kResMaxShift / 8];  // 256^a mod p
+	// moduli in groups of three (3g, 3g + 1, 3g + 2): P_g = their product < 2^24, and 2^sh mod P_g
+	double Pg[kResMaxGroups];
+	double invPg[kResMaxGroups];  // 1 / P_g rounded to binary64
+	uint32_t pow2g[kResMaxGroups][kResMaxShift];
 };
 // the moduli list and per-count constants (host, built once); log2 of the product of the first n
 const ResConst &res_const_host(int nmod);
@@ -112,8 +119,8 @@ void launch_digit_gemm(hipStream_t st, int elem, int M, int N, int Kpad, const v
 // residue planes of one operand (k-contiguous columns; INT8 tiled layout for elem 1, else
 // 2-byte k-contiguous planes), element x of column j as the integer x 2^-colLo[j] modulo each p
 void launch_res_write(hipStream_t st, int elem, int m, int n, const double *in, int64_t ldi,
-                      const int *colLo, const ResConst *rc, int nmod, void *planes, int64_t ldp,
-                      int64_t pstride);
+                      const int *colLo, const ResConst *rc, int nmod, int shLimit, void *planes,
+                      int64_t ldp, int64_t pstride);
 // C = fma(alpha, RN(X 2^(rowLo[i] + colLo[j])), beta C), X from the residue planes (nchunk k-chunk
 // planes per modulus, summed modulo p)
 void launch_res_recon(hipStream_t st, int M, int N, const uint8_t *R, int64_t ldr, int64_t rstride,

@@ -110,6 +110,19 @@ static void build_res_consts() {
 				pw = (pw * 256u) % (uint32_t)p;
 			}
 		}
+		// groups of three consecutive moduli of the full list (the last one shorter), independent of
+		// nmod: a group's members beyond nmod are simply not used
+		for (int gi = 0; gi < kResMaxGroups; gi++) {
+			uint64_t P = 1;
+			for (int t = 3 * gi; t < 3 * gi + 3 && t < avail; t++) P *= (uint64_t)ms[t];
+			c.Pg[gi] = (double)P;
+			c.invPg[gi] = 1.0 / (double)P;
+			uint64_t pw2 = 1 % P;
+			for (int sh = 0; sh < kResMaxShift; sh++) {
+				c.pow2g[gi][sh] = (uint32_t)pw2;
+				pw2 = (pw2 * 2) % P;
+			}
+		}
 	}
 }
 
@@ -129,53 +142,59 @@ double res_log2_product(int nmod) {
 // 64 k-bytes of every modulus plane per step (4 bytes per thread), looping over k tiles.
 __global__ __launch_bounds__(256) void k_res_write_i8t(int m, int n, const double *in, int64_t ldi,
                                                        const int *colLo, const ResConst *rc, int nmod,
-                                                       int8_t *planes, int64_t ldp, int64_t pstride) {
+                                                       int shLimit, int8_t *planes, int64_t ldp,
+                                                       int64_t pstride) {
 	__shared__ ResLds T;
-	res_lds_load(T, rc, nmod);
+	res_lds_load(T, rc, nmod, shLimit);
 	__syncthreads();
 	const int ktiles = (int)(ldp / 64);
 	for (int kt = blockIdx.x; kt < ktiles; kt += gridDim.x)
 		res_write_tile_i8(m, n, in, ldi, colLo, nmod, T, planes, ldp, pstride, kt, blockIdx.y);
 }
 
-// 2-byte planes (FP16 / BF16): plane s of column j at planes + s*pstride + j*ldp, rows [m, ldp) zero
+// 2-byte planes (FP16 / BF16): plane s of column j at planes + s*pstride + j*ldp, rows [m, ldp) zero;
+// a workgroup takes columns blockIdx.x, blockIdx.x + gridDim.x, ... over their whole length
 template <typename T>
 __global__ __launch_bounds__(256) void k_res_write_h(int m, int n, const double *in, int64_t ldi,
                                                      const int *colLo, const ResConst *rc, int nmod,
-                                                     T *planes, int64_t ldp, int64_t pstride) {
-	__shared__ uint8_t pw[kResMaxMod][kResMaxShift / 8];
-	for (int q = threadIdx.x; q < nmod * (kResMaxShift / 8); q += 256)
-		pw[q / (kResMaxShift / 8)][q % (kResMaxShift / 8)] = rc->pow256[q / (kResMaxShift / 8)][q % (kResMaxShift / 8)];
+                                                     int shLimit, T *planes, int64_t ldp, int64_t pstride) {
+	__shared__ ResLds L;
+	res_lds_load(L, rc, nmod, shLimit);
 	__syncthreads();
-	const int j = blockIdx.y;
-	const double *x = in + (int64_t)j * ldi;
-	const int lo = colLo[j];
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ldp; i += gridDim.x * blockDim.x) {
-		const ResElem e = res_elem(i < m ? x[i] : 0.0, lo);
-		for (int s = 0; s < nmod; s++)
-			planes[(int64_t)s * pstride + (int64_t)j * ldp + i] =
-			    digit_cast<T>(res_of(e, rc->p[s], rc->invpd[s], rc->invpf[s], pw[s]));
+	const int ng = (nmod + 2) / 3;
+	for (int j = blockIdx.x; j < n; j += gridDim.x) {
+		const double *x = in + (int64_t)j * ldi;
+		const int lo = colLo[j];
+		for (int i = threadIdx.x; i < ldp; i += blockDim.x) {
+			const ResElem e = res_elem(i < m ? x[i] : 0.0, lo);
+			for (int gi = 0; gi < ng; gi++) {
+				const int rx = res_group(e, L.Pg[gi], L.invPg[gi], L.pw2[gi]);
+				const int s1 = min(3 * gi + 3, nmod);
+				for (int s = 3 * gi; s < s1; s++)
+					planes[(int64_t)s * pstride + (int64_t)j * ldp + i] = digit_cast<T>(res_member(rx, L.p[s], L.ipf[s]));
+			}
+		}
 	}
 }
 
 void launch_res_write(hipStream_t st, int elem, int m, int n, const double *in, int64_t ldi,
-                      const int *colLo, const ResConst *rc, int nmod, void *planes, int64_t ldp,
-                      int64_t pstride) {
+                      const int *colLo, const ResConst *rc, int nmod, int shLimit, void *planes,
+                      int64_t ldp, int64_t pstride) {
+	if (shLimit < 1) shLimit = 1;
 	if (elem == 1) {  // tiled layout; ldp % 128 == 0 and pstride = ldp * (n rounded up to 256)
 		const int ktiles = (int)(ldp / 64);
 		const int gx = ktiles < 16 ? ktiles : 16;
 		hipLaunchKernelGGL(k_res_write_i8t, dim3((unsigned)gx, (unsigned)(pstride / ldp / 16)), dim3(256), 0,
-		                   st, m, n, in, ldi, colLo, rc, nmod, (int8_t *)planes, ldp, pstride);
+		                   st, m, n, in, ldi, colLo, rc, nmod, shLimit, (int8_t *)planes, ldp, pstride);
 		return;
 	}
-	int gx = (int)((ldp + 255) / 256);
-	if (gx > 16) gx = 16;
+	const int g = n < 2048 ? n : 2048;
 	if (elem == 2)
-		hipLaunchKernelGGL(k_res_write_h<_Float16>, dim3(gx, n), dim3(256), 0, st, m, n, in, ldi, colLo,
-		                   rc, nmod, (_Float16 *)planes, ldp, pstride);
+		hipLaunchKernelGGL(k_res_write_h<_Float16>, dim3(g), dim3(256), 0, st, m, n, in, ldi, colLo, rc, nmod,
+		                   shLimit, (_Float16 *)planes, ldp, pstride);
 	else
-		hipLaunchKernelGGL(k_res_write_h<__bf16>, dim3(gx, n), dim3(256), 0, st, m, n, in, ldi, colLo, rc,
-		                   nmod, (__bf16 *)planes, ldp, pstride);
+		hipLaunchKernelGGL(k_res_write_h<__bf16>, dim3(g), dim3(256), 0, st, m, n, in, ldi, colLo, rc, nmod,
+		                   shLimit, (__bf16 *)planes, ldp, pstride);
 }
 
 template <int NW>

@@ -36,46 +36,62 @@ __device__ inline ResElem res_elem(double x, int lo) {
 	return r;
 }
 
-// balanced residue of +-f 2^sh modulo p: in [-(p-1)/2, (p-1)/2] for odd p, [-128, 127] for 256
-__device__ __forceinline__ int res_of(const ResElem &e, int p, double invpd, float invpf,
-                                      const uint8_t *pw256) {
+// Residues by groups of three moduli.  First r = (+-f mod P) (2^sh mod P) mod P in binary64, P the
+// group's product (< 2^24): f < 2^53 and the product of two values below 2^24 stay exact, the
+// quotient estimates are off by at most one (|error| <= 2 f 2^-53 / P < 1).  Then each member's
+// balanced residue from r < 2^24 in binary32: in [-(p-1)/2, (p-1)/2] for odd p, [-128, 127] for 256.
+__device__ __forceinline__ int res_group(const ResElem &e, double P, double invP, const uint32_t *pw2) {
 	if (e.fd == 0.0) return 0;
-	double q = floor(e.fd * invpd);  // f mod p: quotient off by at most one
-	double r = fma(-q, (double)p, e.fd);
-	r += (r < 0.0) ? (double)p : 0.0;
-	r -= (r >= (double)p) ? (double)p : 0.0;
-	// (f mod p) 2^(sh mod 8) (256^(sh / 8) mod p) < 2^23: exact in binary32
-	const float v = (float)r * (float)(1 << (e.sh & 7)) * (float)pw256[e.sh >> 3];
-	const float q2 = floorf(v * invpf);
-	float r2 = fmaf(-q2, (float)p, v);
-	r2 += (r2 < 0.0f) ? (float)p : 0.0f;
-	r2 -= (r2 >= (float)p) ? (float)p : 0.0f;
-	int res = (int)r2;
-	if (e.neg && res) res = p - res;
-	const int thr = (p & 1) ? (p >> 1) : (p >> 1) - 1;
-	return res > thr ? res - p : res;
+	double q = floor(e.fd * invP);
+	double r = fma(-q, P, e.fd);
+	r += (r < 0.0) ? P : 0.0;
+	r -= (r >= P) ? P : 0.0;
+	const double x = r * (double)pw2[e.sh];  // < 2^48: exact
+	q = floor(x * invP);
+	r = fma(-q, P, x);
+	r += (r < 0.0) ? P : 0.0;
+	r -= (r >= P) ? P : 0.0;
+	// the sign: (-v) mod P, so that every member's residue comes out of res_member already signed
+	if (e.neg && r != 0.0) r = P - r;
+	return (int)r;
 }
 
-// the per-modulus tables of the residue writer, staged in LDS by the whole workgroup
+// rx - p rint(rx / p) is the balanced residue up to one correction either way (the binary32 quotient
+// estimate is off by at most one near a half-integer); q p with a 24-bit multiply (q < 2^20, p < 2^9)
+__device__ __forceinline__ int res_member(int rx, int p, float invpf) {
+	const int q = (int)rintf((float)rx * invpf);  // rx < 2^24 is exact
+	int r = rx - (int)__umul24((uint32_t)q, (uint32_t)p);
+	const int hi = (p & 1) ? (p >> 1) : (p >> 1) - 1;  // [-(p-1)/2, (p-1)/2] odd p, [-128, 127] for 256
+	const int lo = -(p >> 1);
+	r -= (r > hi) ? p : 0;
+	r += (r < lo) ? p : 0;
+	return r;
+}
+
+// the per-group and per-modulus tables of the residue writers, staged in LDS by the whole workgroup
+// (2^sh mod P_g only for sh below the operand's span)
 struct ResLds {
-	uint8_t pw[kResMaxMod][kResMaxShift / 8];
+	uint32_t pw2[kResMaxGroups][kResMaxShift];
+	double Pg[kResMaxGroups], invPg[kResMaxGroups];
 	int p[kResMaxMod];
 	float ipf[kResMaxMod];
-	double ipd[kResMaxMod];
 };
 
-__device__ inline void res_lds_load(ResLds &T, const ResConst *rc, int nmod) {
-	constexpr int A = kResMaxShift / 8;
-	for (int q = threadIdx.x; q < nmod * A; q += blockDim.x) T.pw[q / A][q % A] = rc->pow256[q / A][q % A];
+__device__ inline void res_lds_load(ResLds &T, const ResConst *rc, int nmod, int shLimit) {
+	const int ng = (nmod + 2) / 3;
+	for (int q = threadIdx.x; q < ng * shLimit; q += blockDim.x) T.pw2[q / shLimit][q % shLimit] = rc->pow2g[q / shLimit][q % shLimit];
+	for (int q = threadIdx.x; q < ng; q += blockDim.x) {
+		T.Pg[q] = rc->Pg[q];
+		T.invPg[q] = rc->invPg[q];
+	}
 	for (int q = threadIdx.x; q < nmod; q += blockDim.x) {
 		T.p[q] = rc->p[q];
 		T.ipf[q] = rc->invpf[q];
-		T.ipd[q] = rc->invpd[q];
 	}
 }
 
 // One 256-thread workgroup: residues of the 16 columns x 64 k-bytes tile (kt, jb) of every modulus
-// plane in the INT8 tiled layout (4 bytes per thread), with the 256^a table and moduli in LDS.
+// plane in the INT8 tiled layout (4 bytes per thread).
 __device__ inline void res_write_tile_i8(int m, int n, const double *in, int64_t ldi, const int *colLo,
                                          int nmod, const ResLds &T, int8_t *planes, int64_t ldp,
                                          int64_t pstride, int kt, int jb) {
@@ -88,16 +104,22 @@ __device__ inline void res_write_tile_i8(int m, int n, const double *in, int64_t
 #pragma unroll
 	for (int u = 0; u < 4; u++) e[u] = res_elem((j < n && q0 + u < m) ? x[q0 + u] : 0.0, lo);
 	int8_t *dst = planes + (int64_t)jb * 16 * ldp + (int64_t)kt * 1024 + 4 * t;
-	for (int s = 0; s < nmod; s++) {
-		uint32_t word = 0;
+	const int ng = (nmod + 2) / 3;
+	for (int gi = 0; gi < ng; gi++) {
+		int rx[4];
 #pragma unroll
-		for (int u = 0; u < 4; u++)
-			word |= (uint32_t)(uint8_t)(int8_t)res_of(e[u], T.p[s], T.ipd[s], T.ipf[s], T.pw[s]) << (8 * u);
-		*(uint32_t *)(dst + (int64_t)s * pstride) = word;
+		for (int u = 0; u < 4; u++) rx[u] = res_group(e[u], T.Pg[gi], T.invPg[gi], T.pw2[gi]);
+		const int s1 = min(3 * gi + 3, nmod);
+		for (int s = 3 * gi; s < s1; s++) {
+			uint32_t word = 0;
+#pragma unroll
+			for (int u = 0; u < 4; u++)
+				word |= (uint32_t)(uint8_t)(int8_t)res_member(rx[u], T.p[s], T.ipf[s]) << (8 * u);
+			*(uint32_t *)(dst + (int64_t)s * pstride) = word;
+		}
 	}
 }
 
-// ---- device: Chinese-remainder reconstruction ----------------------------------------------
 // v mod p for 0 <= v < 2^24 (binary32 quotient estimate off by at most one)
 __device__ __forceinline__ int mod_f(int v, int p, float invpf) {
 	const float q = floorf((float)v * invpf);
@@ -129,7 +151,7 @@ __device__ inline void res_group_values(const uint8_t *Rw, int64_t rstride, int 
 		const double ipd = rc->invpd[q];
 		int rs[4] = {0, 0, 0, 0};
 		for (int c = 0; c < nchunk; c++) {
-			const uint32_t w = *(const uint32_t *)(Rw + (int64_t)(q * nchunk + c) * rstride);
+			const uint32_t w = *(const uint32_t *)(Rw + (int64_t)(c * nmod + q) * rstride);
 #pragma unroll
 			for (int e = 0; e < 4; e++) rs[e] += (int)((w >> (8 * e)) & 255);
 		}

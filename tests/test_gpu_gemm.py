@@ -149,6 +149,13 @@ def test_residue_form_edges(oz, oracle, slice_type):
     A2 = A.copy()
     A2[::7] *= 2.0 ** 100  # spans beyond what the 49 moduli cover: digit planes
     cases.append(("beyond the moduli", m, n, k, A2, B, 1))
+    # |X| attains the Cauchy-Schwarz bound ||a'|| ||b'||: constant 53-bit entries of one sign
+    m, n, k = 16, 12, 4096
+    v = 1.0 + 2.0 ** -52
+    A3 = np.full(m * k, v)
+    B3 = np.full(k * n, -v)
+    B3[k * 5:k * 6] = v  # one column of the other sign
+    cases.append(("bound attained", m, n, k, A3, B3, 2))
     for name, m, n, k, A, B, used in cases:
         want = oracle.exact_gemm_rn("N", "N", m, n, k, A, m, B, k)
         H = oz.Handle(1 << 30, sumMode=1, sliceType=slice_type, digitScheme="residues")

@@ -2,8 +2,9 @@
 integers:
   - the host constants the kernels read (moduli, (M/p)^-1 mod p, the limbs of M and M/p, 256^a
     mod p), taken from the library itself, against Python integers;
-  - the element residue (res_of: f mod p in binary64, then (f mod p) 2^(sh mod 8) 256^(sh/8) mod p
-    in binary32), restated in IEEE arithmetic (numpy), against (+-f 2^sh) mod p, balanced;
+  - the element residues (res_group: (+-f mod P) (2^sh mod P) mod P in binary64 for the product P of
+    three moduli; res_member: the balanced residue of that value modulo each member in binary32),
+    restated in IEEE arithmetic (numpy), against (+-f 2^sh) mod p;
   - the reconstruction (res_elem_value: c_t = r_t u_t mod p_t, K = rint(sum c_t / p_t) in
     binary64, X = sum c_t M/p_t - K M), restated the same way, against X itself for X at 0, +-1,
     +-M/4 and random values in between: the claim that |X| <= M/4 makes K exact.
@@ -20,12 +21,15 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "ozblas_amd", "libozblas_mi355x.so")
 MAXMOD, MAXLIMB, MAXSH = 49, 18, 512
+NGROUP = (MAXMOD + 2) // 3
 
 
 def _dtype():
     return np.dtype([("nmod", "<i4"), ("nl", "<i4"), ("p", "<i4", MAXMOD), ("invpf", "<f4", MAXMOD),
                      ("invpd", "<f8", MAXMOD), ("u", "<i4", MAXMOD), ("Mp", "<u4", (MAXMOD, MAXLIMB)),
-                     ("M", "<u4", MAXLIMB), ("pow256", "u1", (MAXMOD, MAXSH // 8))], align=True)
+                     ("M", "<u4", MAXLIMB), ("pow256", "u1", (MAXMOD, MAXSH // 8)),
+                     ("Pg", "<f8", NGROUP), ("invPg", "<f8", NGROUP), ("pow2g", "<u4", (NGROUP, MAXSH))],
+                    align=True)
 
 
 @pytest.fixture(scope="module")
@@ -78,25 +82,35 @@ def test_moduli_and_constants(rc):
             assert [int(v) for v in c["pow256"][t]] == [pow(256, a, p) for a in range(MAXSH // 8)]
 
 
-def res_of(fd, sh, neg, p, invpd, invpf, pw):
-    """res_of restated in binary64 / binary32 (numpy rounds each operation like the device)."""
+def res_group(fd, sh, neg, P, invP, pw2):
+    """res_group restated in binary64 (numpy rounds each operation like the device; the fma is exact)."""
     if fd == 0:
         return 0
     fd = np.float64(fd)
-    q = np.floor(fd * invpd)
-    r = np.float64(fd - q * np.float64(p))  # fma(-q, p, f): exact, q p < 2^53
-    r = r + p if r < 0 else r
-    r = r - p if r >= p else r
-    v = np.float32(np.float32(np.float32(r) * np.float32(1 << (sh & 7))) * np.float32(pw[sh >> 3]))
-    q2 = np.floor(np.float32(v * invpf))
-    r2 = np.float32(np.float64(v) - np.float64(q2) * p)  # fmaf(-q2, p, v): exact below 2^24
-    r2 = r2 + p if r2 < 0 else r2
-    r2 = r2 - p if r2 >= p else r2
-    res = int(r2)
-    if neg and res:
-        res = p - res
-    thr = p >> 1 if p & 1 else (p >> 1) - 1
-    return res - p if res > thr else res
+    q = np.floor(fd * invP)
+    r = np.float64(fd - q * P)  # fma(-q, P, f): exact, q P < 2^53
+    r = r + P if r < 0 else r
+    r = r - P if r >= P else r
+    x = np.float64(r * np.float64(pw2[sh]))
+    assert x < 2.0 ** 53
+    q = np.floor(x * invP)
+    r = np.float64(float(x) - float(q) * float(P))  # exact: |x - q P| < 2 P
+    r = r + P if r < 0 else r
+    r = r - P if r >= P else r
+    if neg and r != 0:
+        r = P - r
+    return int(r)
+
+
+def res_member(rx, p, invpf):
+    """res_member restated in binary32 / int32 (rint: nearest, ties to even, as the device)."""
+    q = int(np.rint(np.float32(np.float32(rx) * invpf)))
+    r = rx - q * p
+    hi = p >> 1 if p & 1 else (p >> 1) - 1
+    lo = -(p >> 1)
+    r = r - p if r > hi else r
+    r = r + p if r < lo else r
+    return r
 
 
 def balanced(v, p):
@@ -105,18 +119,32 @@ def balanced(v, p):
     return r - p if r > thr else r
 
 
+def test_group_tables(rc):
+    c, _ = rc[MAXMOD]
+    ps = [int(p) for p in c["p"][:MAXMOD]]
+    for g in range(NGROUP):
+        P = math.prod(ps[3 * g: 3 * g + 3])
+        assert P < 2 ** 24 and c["Pg"][g] == P and c["invPg"][g] == np.float64(1.0) / np.float64(P)
+        assert [int(v) for v in c["pow2g"][g]] == [pow(2, sh, P) for sh in range(MAXSH)]
+
+
 def test_element_residues_against_integers(rc):
     c, _ = rc[MAXMOD]
     rng = random.Random(7)
     fs = [1, 3, (1 << 53) - 1, (1 << 52) + 1, 255, 257, 65535, 1234567890123457]
-    fs += [rng.randrange(1, 1 << 53) | 1 for _ in range(300)]
-    shs = [0, 1, 7, 8, 9, 63, 64, 255, 256, 511] + [rng.randrange(0, MAXSH) for _ in range(40)]
-    for t in range(MAXMOD):
-        p = int(c["p"][t])
-        for f in fs[:40] if t > 3 else fs:
-            for sh in shs[:12] if t > 3 else shs:
-                for neg in (0, 1):
-                    got = res_of(f, sh, neg, p, c["invpd"][t], c["invpf"][t], c["pow256"][t])
+    fs += [rng.randrange(1, 1 << 53) | 1 for _ in range(200)]
+    shs = [0, 1, 7, 8, 9, 63, 64, 255, 256, 511] + [rng.randrange(0, MAXSH) for _ in range(30)]
+    for g in range(NGROUP):
+        P = float(c["Pg"][g])
+        cases = [(f, sh) for f in (fs if g < 2 else fs[:30]) for sh in (shs if g < 2 else shs[:12])]
+        for f, sh in cases:
+            for neg in (0, 1):
+                v = (-1 if neg else 1) * f * pow(2, sh, int(P))
+                rx = res_group(f, sh, neg, P, c["invPg"][g], c["pow2g"][g])
+                assert rx == v % int(P), (P, f, sh, neg)
+                for t in range(3 * g, min(3 * g + 3, MAXMOD)):
+                    p = int(c["p"][t])
+                    got = res_member(rx, p, c["invpf"][t])
                     want = balanced((-1 if neg else 1) * f * 2 ** sh, p)
                     assert got == want, (p, f, sh, neg)
                     assert -128 <= got <= 127
