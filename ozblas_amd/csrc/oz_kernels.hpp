@@ -191,6 +191,8 @@ bool launch_gemv_slices(hipStream_t st, int M, int K, const double *As, int64_t 
 constexpr int kVsMaxLevels = 64;
 struct VsPlan {
 	int count, last, zero, bad;
+	int needScan;  // the predicted chain did not hold: the table scan and its walk make the plan
+	int pad[3];
 	int k[kVsMaxLevels];  // log2 of the slice unit of level s
 };
 size_t vsplit_scratch_bytes();

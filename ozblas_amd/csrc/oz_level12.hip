@@ -126,8 +126,10 @@ __device__ inline double vs_cmod(double x, int k) {
 // floats), with the binary32 tau rule and unit 2^(rho + tau - 24)
 template <int BS, class TI>
 __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned long long *Fg,
-                                                    int2 *range, const int *stopAt, int it) {
+                                                    int2 *range, const int *stopAt, int it,
+                                                    const int *gate) {
 	if (stopAt && it >= *stopAt) return;
+	if (gate && *gate == 0) return;  // the predicted chain held: no table needed
 	__shared__ unsigned long long Fb[kVsK];
 	__shared__ int kminB, kmaxB, badB;
 	__shared__ int red[2][BS / 64];
@@ -249,8 +251,9 @@ __global__ __launch_bounds__(BS) void k_vsplit_walk(int nwg, unsigned long long 
                                                     int rho, int cap, int reproMode,
                                                     const uint16_t *J, VsPlan *plan, int *info,
                                                     unsigned long long *maxv, const int *stopAt,
-                                                    int it) {
+                                                    int it, const int *gate) {
 	if (stopAt && it >= *stopAt) return;
+	if (gate && *gate == 0) return;
 	__shared__ unsigned long long Gb[kVsK];
 	__shared__ int kminB, kmaxB, badB;
 	if (threadIdx.x == 0) {
@@ -354,6 +357,162 @@ __global__ __launch_bounds__(BS) void k_vsplit_walk(int nwg, unsigned long long 
 	}
 }
 
+// ---- predicted chain: F at the few units the level walk will visit -------------------------------
+// Level s+1's unit exponent is k_{s+1} = rho + tau(F(k_s)) - prec with F(k_s) <= 2^(k_s - 1), so it is
+// at most k_s - d, d = prec - rho + 1, and exactly that whenever F(k_s) > 2^(k_s - 2) (tau rounding
+// aside) -- the case for all but the last level of vectors of varied mantissas.  k_vsplit_pre reduces
+// max |x| and the lowest non-trivial unit exponent min lo; k_vsplit_levels evaluates F at
+// k_s = k_0 - d s for the levels that can be non-zero (plus the zero one); k_vsplit_verify walks the
+// chain with the exact tau rule and keeps the plan when every k_s it meets is the predicted one.
+// Otherwise it sets plan->needScan and the table scan and walk (gated on it) make the plan.  The
+// F values are exact maxima either way, so the plan -- and every slice -- is the same.
+constexpr int kVsPred = 12;      // predicted levels evaluated at most
+constexpr int kVsLoOff = 4096;   // min lo is kept as the maximum of kVsLoOff - lo
+struct VsPre {
+	unsigned long long maxBits;  // max |x| (bits of a non-negative binary64)
+	int loOff;                   // max over non-zero x of kVsLoOff - lo (lo: first unit exponent with a
+	int bad;                     //   non-zero residue); bad: an input outside the model
+	unsigned long long F[kVsPred];
+};
+
+template <class TI>
+__device__ inline void vs_predict(const VsPre *pre, int rho, int cap, const uint16_t *J, double &M0,
+                                  int &k0, int &d, int &L) {
+	M0 = __builtin_bit_cast(double, (uint64_t)pre->maxBits);
+	k0 = rho + SplitRule<TI>::tau(M0, J) - SplitRule<TI>::kPrec;
+	d = SplitRule<TI>::kPrec - rho + 1;
+	const int minLo = kVsLoOff - pre->loOff;
+	L = (d > 0 && k0 >= minLo) ? (k0 - minLo) / d + 2 : 1;
+	if (L > cap) L = cap;
+	if (L > kVsPred) L = kVsPred;
+}
+
+template <int BS, class TI>
+__global__ __launch_bounds__(BS) void k_vsplit_pre(int n, const TI *x, VsPre *pre, const int *stopAt, int it) {
+	if (stopAt && it >= *stopAt) return;
+	__shared__ double redd[BS / 64];
+	__shared__ int redi[BS / 64];
+	double mx = 0.0;
+	int lo = 0, bad = 0;
+	for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+		const double v = (double)x[i];
+		const VsElem q = vs_decompose(v, bad);
+		if (q.m == 0) continue;
+		mx = std_max(mx, fabs(v));
+		lo = max(lo, kVsLoOff - q.lo);
+	}
+	mx = block_max_d<BS>(mx, redd);
+	for (int o = 32; o > 0; o >>= 1) {
+		lo = max(lo, __shfl_xor(lo, o, 64));
+		bad = max(bad, __shfl_xor(bad, o, 64));
+	}
+	__syncthreads();
+	if (threadIdx.x % 64 == 0) redi[threadIdx.x / 64] = lo | (bad << 30);
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		int l = 0, bd = 0;
+		for (int w = 0; w < BS / 64; w++) {
+			l = max(l, redi[w] & ((1 << 30) - 1));
+			bd |= redi[w] >> 30;
+		}
+		if (mx > 0.0) atomicMax(&pre->maxBits, (unsigned long long)d2u(mx));
+		if (l > 0) atomicMax(&pre->loOff, l);
+		if (bd) atomicOr(&pre->bad, 1);
+	}
+}
+
+template <int BS, class TI>
+__global__ __launch_bounds__(BS) void k_vsplit_levels(int n, const TI *x, VsPre *pre, int rho, int cap,
+                                                      const uint16_t *J, const int *stopAt, int it) {
+	if (stopAt && it >= *stopAt) return;
+	if (pre->bad || pre->maxBits == 0ull) return;  // the verify step sends these to the scan
+	double M0;
+	int k0, d, L;
+	vs_predict<TI>(pre, rho, cap, J, M0, k0, d, L);
+	__shared__ double red[BS / 64];
+	double Fm[kVsPred];
+#pragma unroll
+	for (int s = 0; s < kVsPred; s++) Fm[s] = 0.0;
+	for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+		const double v = (double)x[i];
+#pragma unroll
+		for (int s = 0; s < kVsPred; s++)
+			if (s < L) Fm[s] = std_max(Fm[s], fabs(vs_cmod(v, k0 - d * s)));
+	}
+#pragma unroll
+	for (int s = 0; s < kVsPred; s++) {
+		if (s < L) {  // L is uniform: every thread takes the reductions it needs
+			const double f = block_max_d<BS>(Fm[s], red);
+			if (threadIdx.x == 0 && f > 0.0) atomicMax(&pre->F[s], (unsigned long long)d2u(f));
+		}
+	}
+}
+
+// one thread: the walk over the predicted F values (the same steps as k_vsplit_walk), then the
+// prediction state is cleared for the next split
+template <class TI>
+__global__ void k_vsplit_verify(VsPre *pre, int rho, int cap, int reproMode, const uint16_t *J,
+                                VsPlan *plan, int *info, unsigned long long *maxv, const int *stopAt,
+                                int it) {
+	if (stopAt && it >= *stopAt) return;
+	if (threadIdx.x != 0) return;
+	double M0;
+	int k0, d, L;
+	vs_predict<TI>(pre, rho, cap, J, M0, k0, d, L);
+	double F[kVsPred];
+	for (int s = 0; s < kVsPred; s++) F[s] = __builtin_bit_cast(double, (uint64_t)pre->F[s]);
+	int bad = pre->bad;
+	pre->maxBits = 0ull;
+	pre->loOff = 0;
+	pre->bad = 0;
+	for (int s = 0; s < kVsPred; s++) pre->F[s] = 0ull;
+	if (bad || M0 == 0.0) {  // outside the model, or all zero: the table path decides as before
+		plan->needScan = 1;
+		return;
+	}
+	if (rho < 3 || rho > SplitRule<TI>::kPrec || cap > kVsMaxLevels) bad = 1;
+	double M = M0;
+	int s = 0;
+	bool zero = false;
+	for (s = 0; s < cap && !bad; s++) {
+		const int tau = SplitRule<TI>::tau(M, J);
+		if (rho + tau > SplitRule<TI>::kSigEmax || rho + tau - 1 < SplitRule<TI>::kEmin) {
+			bad = 1;
+			break;
+		}
+		const int k = rho + tau - SplitRule<TI>::kPrec;
+		if (s >= L || k != k0 - d * s) {  // off the predicted chain
+			plan->needScan = 1;
+			return;
+		}
+		plan->k[s] = k;
+		M = F[s];
+		if (M == 0.0) {
+			zero = true;
+			break;
+		}
+	}
+	plan->needScan = 0;
+	plan->bad = bad;
+	if (!bad) {
+		int count;
+		if (cap <= 1)
+			count = 1;
+		else if (zero)
+			count = (s < 1 ? 1 : s) + 1;
+		else
+			count = cap;
+		if (reproMode == 0) count++;
+		plan->count = count;
+		plan->last = zero ? s : cap - 1;
+		plan->zero = zero ? 1 : 0;
+		info[0] = count;
+		info[1] = zero ? s : cap;
+	} else {
+		for (int q = 0; q < cap + 2; q++) maxv[q] = 0ull;  // the fallback reduces into maxv
+	}
+}
+
 // the reference loop in one workgroup of BS threads (inputs outside the model above): slice s at slot s, its
 // remainder at slot s + 1 (read back as the next level's input), as src/cpu/ozblas_split.cpp:96
 template <int BS, class TI>
@@ -431,7 +590,7 @@ __global__ __launch_bounds__(256) void k_vsplit_write(int n, const TI *x, double
 	}
 }
 
-size_t vsplit_scratch_bytes() { return 8ull * kVsK + 8ull * kVsBlocks + 64; }
+size_t vsplit_scratch_bytes() { return 8ull * kVsK + 8ull * kVsBlocks + 64 + sizeof(VsPre) + 64; }
 
 // Splits x (length n) into slices at stride lds (cap + 2 slots); the count goes to info[0] on the
 // device.  scratch: vsplit_scratch_bytes(), zeroed once (the kernels leave it zero again).
@@ -442,6 +601,7 @@ static hipError_t vsplit(hipStream_t st, int n, const TI *x, double *slices, int
                          const int *stopAt, int it) {
 	unsigned long long *Fg = (unsigned long long *)scratch;
 	int2 *range = (int2 *)(Fg + kVsK);
+	VsPre *pre = (VsPre *)((char *)scratch + 8ull * kVsK + 8ull * kVsBlocks + 64);
 	int rhoEff = rho;
 	if (shift > 1) {  // sigma * 2^j acts as rho + j
 		if (shift & (shift - 1)) rhoEff = -1000;  // not a power of two: the fallback splits
@@ -449,10 +609,17 @@ static hipError_t vsplit(hipStream_t st, int n, const TI *x, double *slices, int
 	}
 	int grid = (int)std::min<int64_t>(kVsBlocks, ((int64_t)n + kVsBS - 1) / kVsBS);
 	if (grid < 1) grid = 1;
+	// the predicted chain first; the table scan and walk only when it does not hold
+	hipLaunchKernelGGL((k_vsplit_pre<kVsBS, TI>), dim3(grid), dim3(kVsBS), 0, st, n, x, pre, stopAt, it);
+	hipLaunchKernelGGL((k_vsplit_levels<kVsBS, TI>), dim3(grid), dim3(kVsBS), 0, st, n, x, pre, rhoEff, cap,
+	                   J, stopAt, it);
+	hipLaunchKernelGGL(k_vsplit_verify<TI>, dim3(1), dim3(64), 0, st, pre, rhoEff, cap, reproMode, J, plan,
+	                   info, maxv, stopAt, it);
+	const int *gate = &plan->needScan;
 	hipLaunchKernelGGL((k_vsplit_scan<kVsBS, TI>), dim3(grid), dim3(kVsBS), 0, st, n, x, Fg, range, stopAt,
-	                   it);
+	                   it, gate);
 	hipLaunchKernelGGL((k_vsplit_walk<kVsBS, TI>), dim3(1), dim3(kVsBS), 0, st, grid, Fg,
-	                   (const int2 *)range, rhoEff, cap, reproMode, J, plan, info, maxv, stopAt, it);
+	                   (const int2 *)range, rhoEff, cap, reproMode, J, plan, info, maxv, stopAt, it, gate);
 	int gw = (int)std::min<int64_t>(2048, ((int64_t)n + 255) / 256);
 	if (gw < 1) gw = 1;
 	hipLaunchKernelGGL(k_vsplit_write<TI>, dim3(gw), dim3(256), 0, st, n, x, slices, lds,

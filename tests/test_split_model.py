@@ -283,3 +283,95 @@ def test_window_key_recurrence_matches_cmod():
                     assert got == exact, (x, k, got, exact)
                 else:
                     assert got <= abs(x), (x, k)
+
+
+# ---- predicted chain (k_vsplit_pre / k_vsplit_levels / k_vsplit_verify) ---------------------------
+PRED = 12  # kVsPred: levels the prediction evaluates at most
+
+
+def lowest_unit(x):
+    """min over non-zero x of lo = the lowest k with cmod(x, 2^k) != 0 (one above the lowest set bit)."""
+    los = []
+    for v in x:
+        if v == 0:
+            continue
+        m, e = np.frexp(abs(float(v)))
+        mi, eb = int(np.ldexp(m, 53)), int(e) - 53
+        while mi % 2 == 0:
+            mi //= 2
+            eb += 1
+        los.append(eb + 1)
+    return min(los) if los else None
+
+
+def predicted_plan(oracle, x, rho, cap, prec=53, tau=None):
+    """The unit exponents k_s of the split if the predicted chain k_s = k_0 - d s (d = prec - rho + 1)
+    holds for every level the walk visits, else None (the kernels then run the table scan)."""
+    tau = tau or oracle.tau
+    x = x.astype(np.float64)
+    M0 = float(np.max(np.abs(x))) if len(x) else 0.0
+    lo = lowest_unit(x)
+    if M0 == 0.0 or lo is None:
+        return None
+    k0 = rho + tau(M0) - prec
+    d = prec - rho + 1
+    L = (k0 - lo) // d + 2 if (d > 0 and k0 >= lo) else 1
+    L = min(L, cap, PRED)
+    F = [float(np.max(np.abs(cmod(x, k0 - d * s)))) for s in range(L)]
+    M, ks = M0, []
+    for s in range(cap):
+        k = rho + tau(M) - prec
+        if s >= L or k != k0 - d * s:
+            return None
+        ks.append(k)
+        M = F[s]
+        if M == 0.0:
+            break
+    return ks
+
+
+def model_ks(oracle, x, rho, cap, prec=53, tau=None):
+    tau = tau or oracle.tau
+    x = x.astype(np.float64)
+    M = float(np.max(np.abs(x)))
+    ks = []
+    for _ in range(cap):
+        k = rho + tau(M) - prec
+        ks.append(k)
+        M = float(np.max(np.abs(cmod(x, k))))
+        if M == 0.0:
+            break
+    return ks
+
+
+def test_predicted_chain_agrees_with_the_level_walk(oracle):
+    """Where the prediction is accepted its units are the walk's; the vectors of varied mantissas (the
+    CG and DOT inputs) are accepted, and vectors with gaps in their bits go to the table scan."""
+    rng = np.random.default_rng(5)
+    cases = {
+        "uniform": rng.uniform(-1, 1, 5000),
+        "normal 2^20": rng.standard_normal(1 << 20),
+        "wide": rng.standard_normal(3000) * np.exp(8 * rng.standard_normal(3000)),
+        "one": np.array([1.0 / 3.0]),
+        "powers-of-two": np.ldexp(1.0, rng.integers(-40, 40, 1500)).astype(float),
+        "two-ranges": np.concatenate([rng.uniform(-1, 1, 500), rng.uniform(-1, 1, 500) * 1e-200]),
+        "sparse bits": np.array([1.0 + 2.0 ** -30, -3.0]),
+    }
+    accepted = set()
+    for name, x in cases.items():
+        n = len(x)
+        rho = oracle.rho(n, 0)
+        for cap in (30, 3, 1):
+            ks = predicted_plan(oracle, x, rho, cap)
+            if ks is not None:
+                assert ks == model_ks(oracle, x, rho, cap), (name, cap)
+                if cap == 30:
+                    accepted.add(name)
+    assert {"uniform", "normal 2^20", "one"} <= accepted
+    assert "sparse bits" not in accepted and "two-ranges" not in accepted
+    # binary32: prec 24 and the binary32 tau rule
+    x = rng.uniform(-1, 1, 4000).astype(np.float32)
+    rho = oracle.s_rho(len(x), 0)
+    tf = lambda M: oracle.s_tau(np.float32(M))
+    ks = predicted_plan(oracle, x, rho, 30, prec=24, tau=tf)
+    assert ks is not None and ks == model_ks(oracle, x, rho, 30, prec=24, tau=tf)
