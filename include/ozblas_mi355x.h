@@ -172,7 +172,7 @@ int32_t ozblasDdot(ozblasHandle_t *oh, int32_t n, const double *x, int32_t incx,
  * on each GPU until the process ends (looked up per device at ozblasCreate), instead of the handle's
  * workspace.  It calls no hipMalloc / hipFree.  Limit: both vectors' slot buffers plus the sums must
  * fit the 64 MiB, about 16 * (cap + 2) * n bytes with cap the split cap (30 for nSplitMax = 0 and
- * reproMode = 1), so n <= 130,688 for two distinct vectors at the defaults; beyond it the call
+ * reproMode = 1), so n <= 130,432 for two distinct vectors at the defaults; beyond it the call
  * returns OZBLAS_STATUS_WORKSPACE before any work.  Every call on one GPU shares
  * that GPU's array: calls that overlap in time on one GPU (other host threads, other streams) race
  * and must be serialised by the caller. */

@@ -756,31 +756,40 @@ static int32_t gemm_residue(ozblasHandle_t *oh, char transA, char transB, int m,
 // vector split (ozblasSplit with one column) -> slots at stride lds
 // ---------------------------------------------------------------------------------------------
 struct VecSplit {
+	const void *x = nullptr;  // the vector itself
 	double *slices = nullptr;
 	int64_t lds = 0;
 	int count = 0;     // host copy of the slice count (valid when read back)
 	int *info = nullptr;  // device: info[0] slice count, info[1] first all-zero remainder
-	unsigned long long *maxv = nullptr;  // device: remainder maxima per slice (cap + 2)
+	VsPlan *plan = nullptr;
+	VsRef ref() const { return VsRef{x, plan, slices, lds, info}; }
 };
 
 // Splits x; with readCount the slice count is read back to the host (one synchronisation),
-// otherwise it stays on the device in vs.info for kernels that read it there.
+// otherwise it stays on the device in vs.info for kernels that read it there.  With planOnly the
+// slices are stored only for a plan outside the residue model: the consumer generates them from
+// (x, plan) (VsRef).
 template <class T>
-static int32_t vec_split(ozblasHandle_t *oh, int n, const T *x, VecSplit &vs, bool readCount = true) {
+static int32_t vec_split(ozblasHandle_t *oh, int n, const T *x, VecSplit &vs, bool readCount = true,
+                         bool planOnly = false) {
 	Backend *b = be(oh);
 	const int cap = split_cap(oh);
 	const int rho = Prec<T>::rho(oh, n);
+	vs.x = x;
 	vs.lds = rup(n, 32);
 	vs.slices = (double *)walloc(oh, 8ull * vs.lds * (cap + 2));
-	vs.maxv = (unsigned long long *)walloc(oh, 8ull * (cap + 2));
 	vs.info = (int *)walloc(oh, 16);
 	void *scratch = walloc(oh, vsplit_scratch_bytes());
-	VsPlan *plan = (VsPlan *)walloc(oh, sizeof(VsPlan));
-	if (!vs.slices || !vs.maxv || !vs.info || !scratch || !plan) return OZBLAS_STATUS_WORKSPACE;
+	vs.plan = (VsPlan *)walloc(oh, sizeof(VsPlan));
+	if (!vs.slices || !vs.info || !scratch || !vs.plan) return OZBLAS_STATUS_WORKSPACE;
 	OZ_HIP(hipMemsetAsync(scratch, 0, vsplit_scratch_bytes(), b->stream));
-	OZ_HIP(hipMemsetAsync(vs.maxv, 0, 8ull * (cap + 2), b->stream));
-	OZ_HIP(launch_vsplit(b->stream, n, x, vs.slices, vs.lds, rho, oh->splitShift, cap,
-	                     repro_slot(oh) ? 0 : 1, Prec<T>::Jsplit(b), vs.maxv, vs.info, scratch, plan));
+	if (planOnly)
+		OZ_HIP(launch_vsplit_plan(b->stream, n, x, vs.slices, vs.lds, rho, oh->splitShift, cap,
+		                          repro_slot(oh) ? 0 : 1, Prec<T>::Jsplit(b), vs.info, scratch, nullptr,
+		                          vs.plan, false));
+	else
+		OZ_HIP(launch_vsplit(b->stream, n, x, vs.slices, vs.lds, rho, oh->splitShift, cap,
+		                     repro_slot(oh) ? 0 : 1, Prec<T>::Jsplit(b), vs.info, scratch, vs.plan));
 	if (!readCount) return OZBLAS_STATUS_SUCCESS;
 	OZ_HIP(hipMemcpyAsync(b->hPinned, vs.info, sizeof(int), hipMemcpyDeviceToHost, b->stream));
 	OZ_TRY(sync(oh));
@@ -830,11 +839,11 @@ static int32_t dot_impl(ozblasHandle_t *oh, int n, const T *x, const T *y, T *re
 	oh->splitShift = 1;  // counterInit (src/cpu/ozblas.cpp:8)
 	const bool dot2 = (oh->splitEpsMode == 2);
 	VecSplit xs, ys;
-	OZ_TRY(vec_split(oh, n, x, xs, dot2));
+	OZ_TRY(vec_split(oh, n, x, xs, dot2, !dot2));
 	if (y == x) {
 		ys = xs;
 	} else {
-		OZ_TRY(vec_split(oh, n, y, ys, dot2));
+		OZ_TRY(vec_split(oh, n, y, ys, dot2, !dot2));
 	}
 	T *dres = (T *)walloc(oh, 8);
 	if (!dres) return OZBLAS_STATUS_WORKSPACE;
@@ -849,9 +858,8 @@ static int32_t dot_impl(ozblasHandle_t *oh, int n, const T *x, const T *y, T *re
 		double *part = (double *)walloc(oh, dots_part_bytes());
 		if (!vals || !part) return OZBLAS_STATUS_WORKSPACE;
 		OZ_HIP(hipMemsetAsync(vals, 0, 8ull * (cap + 1) * (cap + 1), b->stream));
-		launch_dots_dev(b->stream, n, xs.slices, xs.lds, xs.info, ys.slices, ys.lds, ys.info, vals, part,
-		                xs.maxv, ys.maxv, cap + 2, Prec<T>::Jsum(b), oh->sumMode == 0 ? 1 : 0, nullptr,
-		                0, 0, dres, f32 ? 1 : 0);
+		launch_dots_dev(b->stream, n, xs.ref(), ys.ref(), vals, part, Prec<T>::Jsum(b),
+		                oh->sumMode == 0 ? 1 : 0, nullptr, 0, 0, dres, f32 ? 1 : 0);
 	}
 	OZ_HIP(hipMemcpyAsync(b->hPinned, dres, sizeof(T), hipMemcpyDeviceToHost, b->stream));
 	if (!dot2) {
@@ -1087,8 +1095,7 @@ static uint64_t dot_workspace_bytes(const ozblasHandle_t *oh, int n, bool same) 
 	const uint64_t lds = (uint64_t)rup(n, 32);
 	uint64_t t = 0;
 	for (int v = 0; v < (same ? 1 : 2); v++) {
-		t = bump(t, 8ull * lds * (cap + 2));  // slices
-		t = bump(t, 8ull * (cap + 2));        // remainder maxima
+		t = bump(t, 8ull * lds * (cap + 2));  // slices (stored for a plan outside the model)
 		t = bump(t, 16);                      // count
 		t = bump(t, vsplit_scratch_bytes());
 		t = bump(t, sizeof(VsPlan));
@@ -1425,27 +1432,28 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 	const int64_t lds = rup(n, 32);
 	const int fsum = oh->sumMode == 0 ? 1 : 0;
 	const int nhist = (oh->verbose > 0) ? maxiter / oh->verbose + 2 : 1;
+	// slices are stored only for a vector whose plan falls outside the residue model
 	double *slP = (double *)walloc(oh, 8ull * lds * (cap + 2));
 	double *slQ = (double *)walloc(oh, 8ull * lds * (cap + 2));
 	double *slR = (double *)walloc(oh, 8ull * lds * (cap + 2));
-	const size_t small = 8ull * 3 * (cap + 2) + 8ull * (cap + 1) * (cap + 1) + 64 + 8ull * nhist;
+	const size_t small = 8ull * (cap + 1) * (cap + 1) + 64 + 8ull * nhist;
 	char *sm = (char *)walloc(oh, small);
 	CgDev *S = (CgDev *)walloc(oh, sizeof(CgDev));
 	void *vscr = walloc(oh, vsplit_scratch_bytes());
-	VsPlan *plan = (VsPlan *)walloc(oh, sizeof(VsPlan));
+	VsPlan *plans = (VsPlan *)walloc(oh, 3 * sizeof(VsPlan));  // P, Q, R
+	VsPre *pres = (VsPre *)walloc(oh, 3 * sizeof(VsPre));
 	double *part = (double *)walloc(oh, dots_part_bytes());
-	if (!vscr || !plan || !part) return OZBLAS_STATUS_WORKSPACE;
+	if (!vscr || !plans || !pres || !part) return OZBLAS_STATUS_WORKSPACE;
 	const int maxVec = (oh->cgDeviceSlices > 0 && oh->cgDeviceSlices < cap) ? oh->cgDeviceSlices : cap;
 	const int64_t ldc = rup(n, 32);
 	double *Cs = (double *)walloc(oh, 8ull * ldc * nA * maxVec);  // pair sums of the general product
 	if (!slP || !slQ || !slR || !sm || !S || !Cs) return OZBLAS_STATUS_WORKSPACE;
-	unsigned long long *mxP = (unsigned long long *)sm;
-	unsigned long long *mxQ = mxP + (cap + 2), *mxR = mxQ + (cap + 2);
-	double *vals = (double *)(mxR + (cap + 2));
+	double *vals = (double *)sm;
 	int *info = (int *)(vals + (cap + 1) * (cap + 1));  // P, Q, R: two ints each
 	double *dhist = (double *)(info + 16);
 	OZ_HIP(hipMemsetAsync(sm, 0, small, st));
 	OZ_HIP(hipMemsetAsync(vscr, 0, vsplit_scratch_bytes(), st));
+	OZ_HIP(hipMemsetAsync(pres, 0, 3 * sizeof(VsPre), st));
 	CgDev h{};
 	h.nrmb = nrmb;
 	h.dold = dold;
@@ -1455,28 +1463,35 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 	h.vcnt = 1;
 	h.verbose = oh->verbose;
 	h.maxVecSlices = maxVec;
+	h.clkCat = kClkNone;
 	OZ_HIP(hipMemcpyAsync(S, &h, sizeof(CgDev), hipMemcpyHostToDevice, st));
 	int *infoP = info, *infoQ = info + 2, *infoR = info + 4;
+	VsPlan *planP = plans, *planQ = plans + 1, *planR = plans + 2;
+	VsPre *preP = pres, *preQ = pres + 1, *preR = pres + 2;
+	const VsRef refP{v.P, planP, slP, lds, infoP}, refQ{v.Q, planQ, slQ, lds, infoQ},
+	    refR{v.R, planR, slR, lds, infoR};
 	if (stamps) {
 		OZ_HIP(hipEventCreate(&ev0));
 		OZ_HIP(hipEventRecord(ev0, st));
 		off = diag.t1.sec();
 	}
+	// Each vector's split reductions come from the kernel that wrote it (the CSR product for q, the
+	// AXPYs for r and p); p of the first iteration is the copy of r0, reduced here.  The first
+	// kernel of each category opens its clock: SpMV (split of p, q = A p), DOT (split of q or r and
+	// the dot), AXPY/SCAL.
 	for (int it = 0; it < maxiter; it++) {
-		OZ_HIP(launch_vsplit(st, n, v.P, slP, lds, rho, 1, cap, reproCount, b->dTau, mxP, infoP,
-		                     vscr, plan, &S->stopAt, it));
-		launch_csrmm_sum_dev(st, n, AS, rup(b->nnzA, 32), nA, colind, rowptr, slP, lds, infoP, fsum,
-		                     v.Q, Cs, ldc, b->dTau, S, it);
-		OZ_HIP(launch_vsplit(st, n, v.Q, slQ, lds, rho, 1, cap, reproCount, b->dTau, mxQ, infoQ,
-		                     vscr, plan, &S->stopAt, it));
-		launch_dots_dev(st, n, slP, lds, infoP, slQ, lds, infoQ, vals, part, mxP, mxQ, cap + 2, b->dTau,
-		                fsum, S, it, 1, nullptr);
-		launch_axpy2_dev(st, n, S, it, v.P, x, v.Q, v.R);
-		OZ_HIP(launch_vsplit(st, n, v.R, slR, lds, rho, 1, cap, reproCount, b->dTau, mxR, infoR,
-		                     vscr, plan, &S->stopAt, it));
-		launch_dots_dev(st, n, slR, lds, infoR, slR, lds, infoR, vals, part, mxR, mxR, cap + 2, b->dTau,
-		                fsum, S, it, 2, nullptr);
-		launch_scal_axpy_dev(st, n, S, it, v.R, v.P);
+		OZ_HIP(launch_vsplit_plan(st, n, v.P, slP, lds, rho, 1, cap, reproCount, b->dTau, infoP, vscr, preP,
+		                          planP, it > 0, S, it, kClkSpmv));
+		launch_csrmm_sum_dev(st, n, AS, rup(b->nnzA, 32), nA, colind, rowptr, refP, fsum, v.Q, Cs, ldc,
+		                     b->dTau, S, it, preQ);
+		OZ_HIP(launch_vsplit_plan(st, n, v.Q, slQ, lds, rho, 1, cap, reproCount, b->dTau, infoQ, vscr, preQ,
+		                          planQ, true, S, it, kClkDot));
+		launch_dots_dev(st, n, refP, refQ, vals, part, b->dTau, fsum, S, it, 1, nullptr);
+		launch_axpy2_dev(st, n, S, it, v.P, x, v.Q, v.R, preR);
+		OZ_HIP(launch_vsplit_plan(st, n, v.R, slR, lds, rho, 1, cap, reproCount, b->dTau, infoR, vscr, preR,
+		                          planR, true, S, it, kClkDot));
+		launch_dots_dev(st, n, refR, refR, vals, part, b->dTau, fsum, S, it, 2, nullptr);
+		launch_scal_axpy_dev(st, n, S, it, v.R, v.P, preP);
 		if (stamps && ((it + 1) % oh->verbose == 0 || it + 1 == maxiter || it % 16 == 15)) {
 			hipEvent_t e;
 			OZ_HIP(hipEventCreate(&e));
@@ -1485,6 +1500,7 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 			evIt.push_back(it);
 		}
 	}
+	launch_cg_clock_close(st, S);
 	OZ_HIP(hipGetLastError());
 	OZ_HIP(hipMemcpyAsync(&h, S, sizeof(CgDev), hipMemcpyDeviceToHost, st));
 	OZ_HIP(hipStreamSynchronize(st));
@@ -1492,6 +1508,14 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 	if (h.overflow) {
 		free_events();
 		return OZBLAS_STATUS_SUCCESS;
+	}
+	{  // category totals (src/cpu/ozblas_cg.cpp:89,99 and the per-call totals of DOT and CSRMV)
+		int khz = 0;
+		OZ_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device));
+		const double tick = khz > 0 ? 1e-3 / khz : 0.0;
+		oh->t_SpMV_SpMM_total += tick * (double)h.clkTot[kClkSpmv];
+		oh->t_DOT_NRM2_total += tick * (double)h.clkTot[kClkDot];
+		oh->t_AXPY_SCAL_total += tick * (double)h.clkTot[kClkAxpy];
 	}
 	oh->cg_numiter = (h.stopAt < maxiter) ? h.stopAt : maxiter;
 	if (oh->verbose > 0 && hist && h.vcnt > 1)

@@ -112,6 +112,7 @@ __host__ __device__ inline int ceil_log2_tab(double x, const uint16_t *J) {
 		off = f;
 	}
 	if (off == 0) return e;
+	if (off > 0xFFFFu) return e + 1;  // beyond any table entry (16 bits): no lookup
 	return (off <= (uint64_t)J[e + kTauBias]) ? e : e + 1;
 }
 
@@ -150,6 +151,7 @@ __host__ __device__ inline int ceil_log2_tab_f(float x, const uint16_t *J) {
 		off = f;
 	}
 	if (off == 0) return e;
+	if (off > 0xFFFFu) return e + 1;  // beyond any table entry (16 bits): no lookup
 	return (off <= (uint32_t)J[e + kTauBiasF]) ? e : e + 1;
 }
 

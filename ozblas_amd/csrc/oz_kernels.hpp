@@ -195,16 +195,44 @@ struct VsPlan {
 	int pad[3];
 	int k[kVsMaxLevels];  // log2 of the slice unit of level s
 };
+// Per-vector reductions of the predicted level chain (zero between uses: the kernel that consumes
+// them clears them).  A kernel that produces the vector may accumulate maxBits / loOff / bad itself
+// (vs_stats_* in oz_level12.hip), so that the split does not read the vector for them again.
+constexpr int kVsPred = 12;     // predicted levels evaluated at most
+constexpr int kVsLoOff = 4096;  // min lo is kept as the maximum of kVsLoOff - lo
+constexpr int kVsSpread = 16;   // workgroup b reduces into slot b % kVsSpread (one cache line each)
+struct alignas(128) VsSlot {
+	unsigned long long maxBits;  // max |x| (bits of a non-negative binary64)
+	int loOff;                   // max over non-zero x of kVsLoOff - lo (lo: the lowest unit exponent
+	int bad;                     //   with a non-zero residue); bad: an input outside the model
+	unsigned long long F[kVsPred];
+};
+struct VsPre {
+	VsSlot slot[kVsSpread];
+};
 size_t vsplit_scratch_bytes();
+// Splits x into slices at stride lds (cap + 2 slots) and its plan; the count goes to info[0].
 hipError_t launch_vsplit(hipStream_t st, int n, const double *x, double *slices, int64_t lds,
-                         int rho, int shift, int cap, int reproMode, const uint16_t *J,
-                         unsigned long long *maxv, int *info, void *scratch, VsPlan *plan,
-                         const int *stopAt = nullptr, int it = 0);
+                         int rho, int shift, int cap, int reproMode, const uint16_t *J, int *info,
+                         void *scratch, VsPlan *plan);
 // binary32 input (JF: the binary32 tau table); slices are stored as doubles
 hipError_t launch_vsplit(hipStream_t st, int n, const float *x, double *slices, int64_t lds,
-                         int rho, int shift, int cap, int reproMode, const uint16_t *JF,
-                         unsigned long long *maxv, int *info, void *scratch, VsPlan *plan,
-                         const int *stopAt = nullptr, int it = 0);
+                         int rho, int shift, int cap, int reproMode, const uint16_t *JF, int *info,
+                         void *scratch, VsPlan *plan);
+// The plan only: consumers that take (x, plan) generate the slices themselves; the slices are
+// stored only for a plan marked bad (the reference loop in one workgroup).  pre: the vector's
+// VsPre (zero, or holding the producer's reductions when statsDone).  In a device CG, S gives the
+// iteration bound (its kernels of iteration it >= S->stopAt do nothing) and the category clock
+// that the split's first kernel opens (clkCat).
+struct CgDev;
+hipError_t launch_vsplit_plan(hipStream_t st, int n, const double *x, double *slices, int64_t lds,
+                              int rho, int shift, int cap, int reproMode, const uint16_t *J, int *info,
+                              void *scratch, VsPre *pre, VsPlan *plan, bool statsDone,
+                              CgDev *S = nullptr, int it = 0, int clkCat = -1);
+hipError_t launch_vsplit_plan(hipStream_t st, int n, const float *x, double *slices, int64_t lds,
+                              int rho, int shift, int cap, int reproMode, const uint16_t *JF, int *info,
+                              void *scratch, VsPre *pre, VsPlan *plan, bool statsDone,
+                              CgDev *S = nullptr, int it = 0, int clkCat = -1);
 void launch_axpy_host(hipStream_t st, int n, double alpha, const double *x, double *y, int fused);
 void launch_axpy_host(hipStream_t st, int n, float alpha, const float *x, float *y, int fused);
 // t = b - A x by the reference's plain CSRMV (the CG's true residual)
@@ -227,24 +255,39 @@ struct CgDev {
 	int vcnt;
 	int verbose;
 	int maxVecSlices;  // the CSR product gives up (overflow) when p has more slices than this
-	int pad[2];
+	int clkCat;        // category clock: the category open since clkLast (-1: none)
+	int pad;
+	unsigned long long clkLast;
+	unsigned long long clkTot[3];  // wall-clock ticks of SpMV, DOT and AXPY/SCAL
+};
+enum CgClock { kClkSpmv = 0, kClkDot = 1, kClkAxpy = 2, kClkNone = -1 };
+// closes the category clock after the last iteration (one thread)
+void launch_cg_clock_close(hipStream_t st, CgDev *S);
+
+// A split vector as its consumers read it: the vector and its plan, from which they generate the
+// slices, or the stored slices when the plan is marked bad.
+struct VsRef {
+	const void *x;  // binary64 (or binary32 where the consumer says so)
+	const VsPlan *plan;
+	const double *slices;
+	int64_t lds;
+	const int *info;  // info[0]: the slice count
 };
 // q = A p on the device path: a register kernel for p with <= 8 slices and a general one (pair
-// sums through Cs, m x nA x maxVecSlices doubles at pitch ldc) for more
+// sums through Cs, m x nA x maxVecSlices doubles at pitch ldc) for more; the reductions of the
+// predicted split chain of q accumulate into preY
 void launch_csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
-                          const int *colind, const int *rowptr, const double *Xs, int64_t ldxs,
-                          const int *infoX, int fsum, double *y, double *Cs, int64_t ldc,
-                          const uint16_t *J, CgDev *S, int it);
+                          const int *colind, const int *rowptr, VsRef X, int fsum, double *y,
+                          double *Cs, int64_t ldc, const uint16_t *J, CgDev *S, int it, VsPre *preY);
 // part: dots_part_bytes() of scratch for the per-workgroup pair partials (no initial value)
 size_t dots_part_bytes();
-void launch_dots_dev(hipStream_t st, int n, const double *xs, int64_t ldx, const int *infoX,
-                     const double *ys, int64_t ldy, const int *infoY, double *vals, double *part,
-                     unsigned long long *maxvX, unsigned long long *maxvY, int nmax,
-                     const uint16_t *J, int fsum, CgDev *S, int it, int mode, void *C,
-                     int f32 = 0);
-void launch_axpy2_dev(hipStream_t st, int n, const CgDev *S, int it, const double *P, double *X,
-                      const double *Q, double *R);
-void launch_scal_axpy_dev(hipStream_t st, int n, const CgDev *S, int it, const double *R,
-                          double *P);
+// DOT of two split vectors (binary32 inputs with f32), completed by `mode` (0: C[0] = the DOT)
+void launch_dots_dev(hipStream_t st, int n, VsRef X, VsRef Y, double *vals, double *part,
+                     const uint16_t *J, int fsum, CgDev *S, int it, int mode, void *C, int f32 = 0);
+// x += alpha p, r -= alpha q (reductions of r's split into preR); p = r + beta p (into preP)
+void launch_axpy2_dev(hipStream_t st, int n, CgDev *S, int it, const double *P, double *X,
+                      const double *Q, double *R, VsPre *preR);
+void launch_scal_axpy_dev(hipStream_t st, int n, CgDev *S, int it, const double *R, double *P,
+                          VsPre *preP);
 
 }  // namespace oz

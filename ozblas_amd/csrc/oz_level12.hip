@@ -51,6 +51,11 @@ __device__ inline double block_sum_d(double v, double *red) {
 // workgroup 0 of k_vsplit_write instead, which runs the reference loop.  The binary32 split
 // is the same with 24 for 53 (unit 2^(rho+tau-24), float sigma in its normal range).
 // ---------------------------------------------------------------------------------------------
+// a load at the device scope (values other workgroups wrote with atomics or before their release)
+__device__ inline unsigned long long vs_load(const unsigned long long *p) {
+	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 constexpr int kVsOff = 1100;          // F/G tables cover k in [-1100, 1099]
 constexpr int kVsK = 2200;
 constexpr int kVsBlocks = 256;
@@ -125,9 +130,18 @@ __device__ inline double vs_cmod(double x, int k) {
 // TI = float: the binary32 split on the same tables (a float converts exactly; its residues are
 // floats), with the binary32 tau rule and unit 2^(rho + tau - 24)
 template <int BS, class TI>
+__device__ void vs_walk(int nwg, unsigned long long *Fg, const int2 *range, int rho, int cap, int reproMode,
+                        const uint16_t *J, VsPlan *plan, int *info, int n, const TI *x, double *slices,
+                        int64_t lds, int rhoIn, int shift, int fallback);
+
+// the table scan; its last workgroup then walks the levels (vs_walk)
+template <int BS, class TI>
 __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned long long *Fg,
-                                                    int2 *range, const int *stopAt, int it,
-                                                    const int *gate) {
+                                                    int2 *range, unsigned int *done, const int *stopAt,
+                                                    int it, const int *gate, int rho, int cap,
+                                                    int reproMode, const uint16_t *J, VsPlan *plan,
+                                                    int *info, double *slices, int64_t lds, int rhoIn,
+                                                    int shift, int fallback) {
 	if (stopAt && it >= *stopAt) return;
 	if (gate && *gate == 0) return;  // the predicted chain held: no table needed
 	__shared__ unsigned long long Fb[kVsK];
@@ -241,19 +255,32 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned
 	}
 	// range: [r0, r1] (empty when r0 > r1); x = INT32_MIN marks an input outside the model
 	if (threadIdx.x == 0) range[blockIdx.x] = badB ? make_int2(INT32_MIN, INT32_MIN) : make_int2(r0, r1);
+	// the last workgroup to finish walks the levels
+	__shared__ int lastWG;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		__threadfence();
+		lastWG = atomicAdd(done, 1u) == gridDim.x - 1;
+	}
+	__syncthreads();
+	if (!lastWG) return;
+	__threadfence();
+	if (threadIdx.x == 0) *done = 0u;
+	vs_walk<BS, TI>(gridDim.x, Fg, range, rho, cap, reproMode, J, plan, info, n, x, slices, lds, rhoIn, shift,
+	                fallback);
 }
 
-// The level walk of the split, one workgroup after k_vsplit_scan (its nwg workgroups' ranges and
-// the global table): the union of the ranges, F as the prefix max of the table, then the levels
+template <int BS, class TI>
+__device__ void vsplit_fallback(int n, const TI *x, double *slices, int64_t lds, int rho, int shift,
+                                int cap, int reproMode, const uint16_t *J, int *info);
+
+// The level walk of the split, by the last workgroup of k_vsplit_scan (the nwg workgroups' ranges
+// and the global table): the union of the ranges, F as the prefix max of the table, then the levels
 // M_{s+1} = F(rho + tau(M_s) - kPrec) into the plan.
 template <int BS, class TI>
-__global__ __launch_bounds__(BS) void k_vsplit_walk(int nwg, unsigned long long *Fg, const int2 *range,
-                                                    int rho, int cap, int reproMode,
-                                                    const uint16_t *J, VsPlan *plan, int *info,
-                                                    unsigned long long *maxv, const int *stopAt,
-                                                    int it, const int *gate) {
-	if (stopAt && it >= *stopAt) return;
-	if (gate && *gate == 0) return;
+__device__ void vs_walk(int nwg, unsigned long long *Fg, const int2 *range, int rho, int cap, int reproMode,
+                        const uint16_t *J, VsPlan *plan, int *info, int n, const TI *x, double *slices,
+                        int64_t lds, int rhoIn, int shift, int fallback) {
 	__shared__ unsigned long long Gb[kVsK];
 	__shared__ int kminB, kmaxB, badB;
 	if (threadIdx.x == 0) {
@@ -263,7 +290,9 @@ __global__ __launch_bounds__(BS) void k_vsplit_walk(int nwg, unsigned long long 
 	}
 	__syncthreads();
 	for (int b = threadIdx.x; b < nwg; b += BS) {
-		const int2 rg = range[b];
+		// the other workgroups' ranges and table entries: read at the device scope
+		const int2 rg = make_int2(__hip_atomic_load(&range[b].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+		                          __hip_atomic_load(&range[b].y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 		if (rg.x == INT32_MIN) {
 			badB = 1;
 		} else if (rg.x <= rg.y) {
@@ -275,7 +304,7 @@ __global__ __launch_bounds__(BS) void k_vsplit_walk(int nwg, unsigned long long 
 	const int K0 = kminB, K1 = kmaxB;  // empty ([INT32_MAX, INT32_MIN]) for an all-zero vector
 	for (int k = (K0 <= K1 ? K0 + (int)threadIdx.x : 1); k <= K1; k += BS) {
 		// F(k) = key 2^k (a residue of a binary64: exact)
-		Gb[k + kVsOff] = d2u(ldexp(__builtin_bit_cast(double, (uint64_t)Fg[k + kVsOff]), k));
+		Gb[k + kVsOff] = d2u(ldexp(__builtin_bit_cast(double, (uint64_t)vs_load(&Fg[k + kVsOff])), k));
 		Fg[k + kVsOff] = 0ull;  // clear for the next split
 	}
 	__syncthreads();  // the prefix segments below span entries other threads converted
@@ -308,105 +337,139 @@ __global__ __launch_bounds__(BS) void k_vsplit_walk(int nwg, unsigned long long 
 	for (int k = s0; k < s1; k++)
 		if (pre > Gb[k + kVsOff]) Gb[k + kVsOff] = pre;
 	__syncthreads();
-	if (threadIdx.x != 0) return;
-	J = Js;
-	int bad = badB;
-	const unsigned long long gmax = segMax[BS - 1];  // max |x|
-	auto Fk = [&](int k) -> double {
-		if (K0 > K1 || k < K0) return 0.0;
-		if (k > K1) return __builtin_bit_cast(double, gmax);
-		return __builtin_bit_cast(double, Gb[k + kVsOff]);
-	};
-	// rho <= kPrec keeps every unit an even divisor of the one before (u_{s+1} <= u_s 2^(rho-1-kPrec)),
-	// which the residue chain needs; a binary32 vector longer than 2^24 exceeds it
-	if (rho < 3 || rho > SplitRule<TI>::kPrec || cap > kVsMaxLevels) bad = 1;
-	double M = __builtin_bit_cast(double, gmax);
-	int s = 0;
-	bool zero = false;
-	for (s = 0; s < cap && !bad; s++) {
-		const int tau = SplitRule<TI>::tau(M, J);
-		if (rho + tau > SplitRule<TI>::kSigEmax || rho + tau - 1 < SplitRule<TI>::kEmin) {
-			bad = 1;
-			break;
+	__shared__ int planBad;
+	if (threadIdx.x == 0) {
+		const uint16_t *Jl = Js;
+		int bad = badB;
+		const unsigned long long gmax = segMax[BS - 1];  // max |x|
+		auto Fk = [&](int k) -> double {
+			if (K0 > K1 || k < K0) return 0.0;
+			if (k > K1) return __builtin_bit_cast(double, gmax);
+			return __builtin_bit_cast(double, Gb[k + kVsOff]);
+		};
+		// rho <= kPrec keeps every unit an even divisor of the one before
+		// (u_{s+1} <= u_s 2^(rho-1-kPrec)), which the residue chain needs; a binary32 vector longer
+		// than 2^24 exceeds it
+		if (rho < 3 || rho > SplitRule<TI>::kPrec || cap > kVsMaxLevels) bad = 1;
+		double M = __builtin_bit_cast(double, gmax);
+		int s = 0;
+		bool zero = false;
+		for (s = 0; s < cap && !bad; s++) {
+			const int tau = SplitRule<TI>::tau(M, Jl);
+			if (rho + tau > SplitRule<TI>::kSigEmax || rho + tau - 1 < SplitRule<TI>::kEmin) {
+				bad = 1;
+				break;
+			}
+			const int k = rho + tau - SplitRule<TI>::kPrec;
+			plan->k[s] = k;
+			M = Fk(k);
+			if (M == 0.0) {
+				zero = true;
+				break;
+			}
 		}
-		const int k = rho + tau - SplitRule<TI>::kPrec;
-		plan->k[s] = k;
-		M = Fk(k);
-		if (M == 0.0) {
-			zero = true;
-			break;
+		plan->bad = bad;
+		if (!bad) {
+			int count;
+			if (cap <= 1)
+				count = 1;
+			else if (zero)
+				count = (s < 1 ? 1 : s) + 1;
+			else
+				count = cap;
+			if (reproMode == 0) count++;
+			plan->count = count;
+			plan->last = zero ? s : cap - 1;
+			plan->zero = zero ? 1 : 0;
+			info[0] = count;
+			info[1] = zero ? s : cap;
 		}
+		planBad = bad;
 	}
-	plan->bad = bad;
-	if (!bad) {
-		int count;
-		if (cap <= 1)
-			count = 1;
-		else if (zero)
-			count = (s < 1 ? 1 : s) + 1;
-		else
-			count = cap;
-		if (reproMode == 0) count++;
-		plan->count = count;
-		plan->last = zero ? s : cap - 1;
-		plan->zero = zero ? 1 : 0;
-		info[0] = count;
-		info[1] = zero ? s : cap;
-	} else {
-		for (int q = 0; q < cap + 2; q++) maxv[q] = 0ull;  // the fallback reduces into maxv
-	}
+	__syncthreads();
+	// a plan outside the model: the reference loop stores the slices and the count, here when the
+	// consumers read no stored slices otherwise (k_vsplit_write does it when it runs)
+	if (fallback && planBad) vsplit_fallback<BS, TI>(n, x, slices, lds, rhoIn, shift, cap, reproMode, J, info);
 }
 
 // ---- predicted chain: F at the few units the level walk will visit -------------------------------
 // Level s+1's unit exponent is k_{s+1} = rho + tau(F(k_s)) - prec with F(k_s) <= 2^(k_s - 1), so it is
 // at most k_s - d, d = prec - rho + 1, and exactly that whenever F(k_s) > 2^(k_s - 2) (tau rounding
-// aside) -- the case for all but the last level of vectors of varied mantissas.  k_vsplit_pre reduces
-// max |x| and the lowest non-trivial unit exponent min lo; k_vsplit_levels evaluates F at
-// k_s = k_0 - d s for the levels that can be non-zero (plus the zero one); k_vsplit_verify walks the
-// chain with the exact tau rule and keeps the plan when every k_s it meets is the predicted one.
-// Otherwise it sets plan->needScan and the table scan and walk (gated on it) make the plan.  The
-// F values are exact maxima either way, so the plan -- and every slice -- is the same.
-constexpr int kVsPred = 12;      // predicted levels evaluated at most
-constexpr int kVsLoOff = 4096;   // min lo is kept as the maximum of kVsLoOff - lo
-struct VsPre {
-	unsigned long long maxBits;  // max |x| (bits of a non-negative binary64)
-	int loOff;                   // max over non-zero x of kVsLoOff - lo (lo: first unit exponent with a
-	int bad;                     //   non-zero residue); bad: an input outside the model
-	unsigned long long F[kVsPred];
-};
+// aside) -- the case for all but the last level of vectors of varied mantissas.  The reductions
+// max |x| and the lowest non-trivial unit exponent min lo (k_vsplit_pre, or the kernel that produced
+// x) fix k_0 and the last level that can be non-zero; k_vsplit_levels evaluates F at
+// k_s = k_0 - d s for those levels (plus the zero one), and k_vsplit_verify walks the chain with
+// the exact tau rule, keeping the plan when every k_s it meets is the predicted one.  Otherwise it
+// sets plan->needScan and the table scan (gated on it) and its walk make the plan.  The F values are
+// exact maxima either way, so the plan -- and every slice -- is the same.
 
-template <class TI>
-__device__ inline void vs_predict(const VsPre *pre, int rho, int cap, const uint16_t *J, double &M0,
-                                  int &k0, int &d, int &L) {
-	M0 = __builtin_bit_cast(double, (uint64_t)pre->maxBits);
-	k0 = rho + SplitRule<TI>::tau(M0, J) - SplitRule<TI>::kPrec;
-	d = SplitRule<TI>::kPrec - rho + 1;
-	const int minLo = kVsLoOff - pre->loOff;
-	L = (d > 0 && k0 >= minLo) ? (k0 - minLo) / d + 2 : 1;
-	if (L > cap) L = cap;
-	if (L > kVsPred) L = kVsPred;
+// category clock of the device CG: the first workgroup of a category's first kernel charges the
+// time since the previous stamp to the category then open; an iteration past the stop closes it
+__device__ inline void cg_clock(CgDev *S, int it, int cat) {
+	if (S == nullptr || blockIdx.x != 0 || threadIdx.x != 0) return;
+	if (it >= S->stopAt) cat = kClkNone;
+	if (cat == kClkNone && S->clkCat == kClkNone) return;
+	const unsigned long long t = (unsigned long long)wall_clock64();
+	if (S->clkCat != kClkNone) S->clkTot[S->clkCat] += t - S->clkLast;
+	S->clkLast = t;
+	S->clkCat = cat;
 }
 
-template <int BS, class TI>
-__global__ __launch_bounds__(BS) void k_vsplit_pre(int n, const TI *x, VsPre *pre, const int *stopAt, int it) {
-	if (stopAt && it >= *stopAt) return;
+__global__ void k_cg_clock_close(CgDev *S) { cg_clock(S, S->stopAt, kClkNone); }
+
+void launch_cg_clock_close(hipStream_t st, CgDev *S) {
+	hipLaunchKernelGGL(k_cg_clock_close, dim3(1), dim3(64), 0, st, S);
+}
+
+// the reductions of the predicted chain for one element, then for the workgroup (every thread of
+// the workgroup calls vs_stats_flush)
+__device__ inline void vs_stats_acc(double v, double &mx, int &lo, int &bad) {
+	const VsElem q = vs_decompose(v, bad);
+	if (q.m == 0) return;
+	mx = std_max(mx, fabs(v));
+	lo = max(lo, kVsLoOff - q.lo);
+}
+
+// atomicMax that skips the atomic when the slot already holds at least v (the slot only grows
+// while the reductions run), which keeps most workgroups off the shared cache line
+__device__ inline void atomic_max_lazy(unsigned long long *p, unsigned long long v) {
+	if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v) atomicMax(p, v);
+}
+__device__ inline void atomic_max_lazy(int *p, int v) {
+	if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v) atomicMax(p, v);
+}
+
+// the reductions of all slots (every workgroup of the level kernel reads them; with atomic loads in
+// the verify step, after the other workgroups' atomics)
+template <bool ATOMIC>
+__device__ inline void vs_pre_sum(const VsPre *pre, unsigned long long &maxBits, int &loOff, int &bad) {
+	maxBits = 0ull;
+	loOff = 0;
+	bad = 0;
+#pragma unroll
+	for (int q = 0; q < kVsSpread; q++) {
+		const VsSlot &sl = pre->slot[q];
+		if constexpr (ATOMIC) {
+			maxBits = std::max(maxBits, __hip_atomic_load(&sl.maxBits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+			loOff = max(loOff, __hip_atomic_load(&sl.loOff, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+			bad |= __hip_atomic_load(&sl.bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		} else {
+			maxBits = std::max(maxBits, sl.maxBits);
+			loOff = max(loOff, sl.loOff);
+			bad |= sl.bad;
+		}
+	}
+}
+
+template <int BS>
+__device__ inline void vs_stats_flush(double mx, int lo, int bad, VsPre *pre) {
 	__shared__ double redd[BS / 64];
 	__shared__ int redi[BS / 64];
-	double mx = 0.0;
-	int lo = 0, bad = 0;
-	for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
-		const double v = (double)x[i];
-		const VsElem q = vs_decompose(v, bad);
-		if (q.m == 0) continue;
-		mx = std_max(mx, fabs(v));
-		lo = max(lo, kVsLoOff - q.lo);
-	}
 	mx = block_max_d<BS>(mx, redd);
 	for (int o = 32; o > 0; o >>= 1) {
 		lo = max(lo, __shfl_xor(lo, o, 64));
 		bad = max(bad, __shfl_xor(bad, o, 64));
 	}
-	__syncthreads();
 	if (threadIdx.x % 64 == 0) redi[threadIdx.x / 64] = lo | (bad << 30);
 	__syncthreads();
 	if (threadIdx.x == 0) {
@@ -415,102 +478,204 @@ __global__ __launch_bounds__(BS) void k_vsplit_pre(int n, const TI *x, VsPre *pr
 			l = max(l, redi[w] & ((1 << 30) - 1));
 			bd |= redi[w] >> 30;
 		}
-		if (mx > 0.0) atomicMax(&pre->maxBits, (unsigned long long)d2u(mx));
-		if (l > 0) atomicMax(&pre->loOff, l);
-		if (bd) atomicOr(&pre->bad, 1);
+		VsSlot &sl = pre->slot[blockIdx.x % kVsSpread];
+		if (mx > 0.0) atomic_max_lazy(&sl.maxBits, (unsigned long long)d2u(mx));
+		if (l > 0) atomic_max_lazy(&sl.loOff, l);
+		if (bd) atomicOr(&sl.bad, 1);
 	}
 }
+
+template <class TI>
+__device__ inline void vs_predict(double M0, int minLo, int rho, int cap, const uint16_t *J, int &k0,
+                                  int &d, int &L) {
+	k0 = rho + SplitRule<TI>::tau(M0, J) - SplitRule<TI>::kPrec;
+	d = SplitRule<TI>::kPrec - rho + 1;
+	L = (d > 0 && k0 >= minLo) ? (k0 - minLo) / d + 2 : 1;
+	if (L > cap) L = cap;
+	if (L > kVsPred) L = kVsPred;
+}
+
+constexpr int kVsLevBlocks = 1024;  // workgroups of the reduction kernels at most
 
 template <int BS, class TI>
-__global__ __launch_bounds__(BS) void k_vsplit_levels(int n, const TI *x, VsPre *pre, int rho, int cap,
-                                                      const uint16_t *J, const int *stopAt, int it) {
+__global__ __launch_bounds__(BS) void k_vsplit_pre(int n, const TI *x, VsPre *pre, const int *stopAt, int it) {
 	if (stopAt && it >= *stopAt) return;
-	if (pre->bad || pre->maxBits == 0ull) return;  // the verify step sends these to the scan
-	double M0;
-	int k0, d, L;
-	vs_predict<TI>(pre, rho, cap, J, M0, k0, d, L);
-	__shared__ double red[BS / 64];
-	double Fm[kVsPred];
+	double mx = 0.0;
+	int lo = 0, bad = 0;
+	const int64_t stride = (int64_t)gridDim.x * BS;
+	for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += 4 * stride) {
+		double v[4];
 #pragma unroll
-	for (int s = 0; s < kVsPred; s++) Fm[s] = 0.0;
-	for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
-		const double v = (double)x[i];
+		for (int e = 0; e < 4; e++) v[e] = (i + e * stride < n) ? (double)x[i + e * stride] : 0.0;
 #pragma unroll
-		for (int s = 0; s < kVsPred; s++)
-			if (s < L) Fm[s] = std_max(Fm[s], fabs(vs_cmod(v, k0 - d * s)));
+		for (int e = 0; e < 4; e++) vs_stats_acc(v[e], mx, lo, bad);
 	}
-#pragma unroll
-	for (int s = 0; s < kVsPred; s++) {
-		if (s < L) {  // L is uniform: every thread takes the reductions it needs
-			const double f = block_max_d<BS>(Fm[s], red);
-			if (threadIdx.x == 0 && f > 0.0) atomicMax(&pre->F[s], (unsigned long long)d2u(f));
-		}
-	}
+	vs_stats_flush<BS>(mx, lo, bad, pre);
 }
 
-// one thread: the walk over the predicted F values (the same steps as k_vsplit_walk), then the
-// prediction state is cleared for the next split
+// The walk over the predicted F values (the steps of vs_walk), by one thread of k_vsplit_verify.  Anything the prediction does not settle (an input outside the
+// model, a zero vector, a level off the chain) goes to the table path.
 template <class TI>
-__global__ void k_vsplit_verify(VsPre *pre, int rho, int cap, int reproMode, const uint16_t *J,
-                                VsPlan *plan, int *info, unsigned long long *maxv, const int *stopAt,
-                                int it) {
-	if (stopAt && it >= *stopAt) return;
-	if (threadIdx.x != 0) return;
-	double M0;
-	int k0, d, L;
-	vs_predict<TI>(pre, rho, cap, J, M0, k0, d, L);
-	double F[kVsPred];
-	for (int s = 0; s < kVsPred; s++) F[s] = __builtin_bit_cast(double, (uint64_t)pre->F[s]);
-	int bad = pre->bad;
-	pre->maxBits = 0ull;
-	pre->loOff = 0;
-	pre->bad = 0;
-	for (int s = 0; s < kVsPred; s++) pre->F[s] = 0ull;
-	if (bad || M0 == 0.0) {  // outside the model, or all zero: the table path decides as before
+__device__ void vs_verify(unsigned long long mb, int lof, int bad, const unsigned long long *Fb, int rho,
+                          int cap, int reproMode, const uint16_t *J, VsPlan *plan, int *info) {
+	const double M0 = __builtin_bit_cast(double, (uint64_t)mb);
+	if (bad || M0 == 0.0 || rho < 3 || rho > SplitRule<TI>::kPrec || cap > kVsMaxLevels) {
 		plan->needScan = 1;
 		return;
 	}
-	if (rho < 3 || rho > SplitRule<TI>::kPrec || cap > kVsMaxLevels) bad = 1;
+	int k0, d, L;
+	vs_predict<TI>(M0, kVsLoOff - lof, rho, cap, J, k0, d, L);
 	double M = M0;
 	int s = 0;
 	bool zero = false;
-	for (s = 0; s < cap && !bad; s++) {
+	for (s = 0; s < cap; s++) {
 		const int tau = SplitRule<TI>::tau(M, J);
-		if (rho + tau > SplitRule<TI>::kSigEmax || rho + tau - 1 < SplitRule<TI>::kEmin) {
-			bad = 1;
-			break;
-		}
 		const int k = rho + tau - SplitRule<TI>::kPrec;
-		if (s >= L || k != k0 - d * s) {  // off the predicted chain
+		if (rho + tau > SplitRule<TI>::kSigEmax || rho + tau - 1 < SplitRule<TI>::kEmin || s >= L ||
+		    k != k0 - d * s) {  // outside the model, or off the predicted chain
 			plan->needScan = 1;
 			return;
 		}
 		plan->k[s] = k;
-		M = F[s];
+		M = __builtin_bit_cast(double, (uint64_t)Fb[s]);
 		if (M == 0.0) {
 			zero = true;
 			break;
 		}
 	}
+	int count;
+	if (cap <= 1)
+		count = 1;
+	else if (zero)
+		count = (s < 1 ? 1 : s) + 1;
+	else
+		count = cap;
+	if (reproMode == 0) count++;
 	plan->needScan = 0;
-	plan->bad = bad;
-	if (!bad) {
-		int count;
-		if (cap <= 1)
-			count = 1;
-		else if (zero)
-			count = (s < 1 ? 1 : s) + 1;
-		else
-			count = cap;
-		if (reproMode == 0) count++;
-		plan->count = count;
-		plan->last = zero ? s : cap - 1;
-		plan->zero = zero ? 1 : 0;
-		info[0] = count;
-		info[1] = zero ? s : cap;
-	} else {
-		for (int q = 0; q < cap + 2; q++) maxv[q] = 0ull;  // the fallback reduces into maxv
+	plan->bad = 0;
+	plan->count = count;
+	plan->last = zero ? s : cap - 1;
+	plan->zero = zero ? 1 : 0;
+	info[0] = count;
+	info[1] = zero ? s : cap;
+}
+
+template <int BS, class TI>
+__global__ __launch_bounds__(BS) void k_vsplit_levels(int n, const TI *x, VsPre *pre, int rho, int cap,
+                                                      const uint16_t *J, CgDev *S, int it, int clkCat) {
+	cg_clock(S, it, clkCat);
+	if (S && it >= S->stopAt) return;
+	__shared__ double red[BS / 64];
+	unsigned long long mb;
+	int lof, pbad;
+	vs_pre_sum<false>(pre, mb, lof, pbad);
+	if (!pbad && mb != 0ull) {
+		const double M0 = __builtin_bit_cast(double, (uint64_t)mb);
+		int k0, d, L;
+		vs_predict<TI>(M0, kVsLoOff - lof, rho, cap, J, k0, d, L);
+		double Fm[kVsPred];
+#pragma unroll
+		for (int q = 0; q < kVsPred; q++) Fm[q] = 0.0;
+		// four elements per thread and step, their loads in flight together
+		const int64_t stride = (int64_t)gridDim.x * BS;
+		for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += 4 * stride) {
+			double v[4];
+#pragma unroll
+			for (int e = 0; e < 4; e++) v[e] = (i + e * stride < n) ? (double)x[i + e * stride] : 0.0;
+#pragma unroll
+			for (int e = 0; e < 4; e++)
+#pragma unroll
+				for (int q = 0; q < kVsPred; q++)
+					if (q < L) Fm[q] = std_max(Fm[q], fabs(vs_cmod(v[e], k0 - d * q)));
+		}
+#pragma unroll
+		for (int q = 0; q < kVsPred; q++) {
+			if (q < L) {  // L is uniform: every thread takes the reductions it needs
+				const double f = block_max_d<BS>(Fm[q], red);
+				if (threadIdx.x == 0 && f > 0.0)
+					atomic_max_lazy(&pre->slot[blockIdx.x % kVsSpread].F[q], (unsigned long long)d2u(f));
+			}
+		}
 	}
+}
+
+// After k_vsplit_levels, one workgroup: reduces the slots, clears them for the next split and
+// verifies the chain (one thread).  A kernel of its own: a last-workgroup step inside the level
+// kernel would cost every workgroup a device-scope release.
+template <int BS, class TI>
+__global__ __launch_bounds__(BS) void k_vsplit_verify(VsPre *pre, int rho, int cap, int reproMode,
+                                                      const uint16_t *J, VsPlan *plan, int *info,
+                                                      const int *stopAt, int it) {
+	if (stopAt && it >= *stopAt) return;
+	__shared__ unsigned long long Fb[kVsPred], mbS;
+	__shared__ int lofS, badS;
+	const int t = threadIdx.x;
+	if (t < kVsPred) {
+		unsigned long long f = 0ull;
+		for (int q = 0; q < kVsSpread; q++) f = std::max(f, pre->slot[q].F[t]);
+		Fb[t] = f;
+	} else if (t == kVsPred) {
+		vs_pre_sum<false>(pre, mbS, lofS, badS);
+	}
+	__syncthreads();
+	constexpr int kWords = (int)(sizeof(VsSlot) / 8) * kVsSpread;
+	for (int q = t; q < kWords; q += BS) ((unsigned long long *)pre->slot)[q] = 0ull;
+	if (t == 0) vs_verify<TI>(mbS, lofS, badS, Fb, rho, cap, reproMode, J, plan, info);
+}
+
+// Slices of one element from its plan: the reference's loop (src/cpu/ozblas_split.cpp:75-104)
+// with the plan's units, sigma_s = 0.75 * 2^(rho + tau_s) = 1.5 * 2^(k_s + prec - 1), in the input's
+// precision.  Slot last + 1 holds the remainder (0 when the chain ended in zero) and later slots 0:
+// the slots k_vsplit_write stores.  Slots at or above the slice count are for the caller to mask.
+// sigma = 1.5 * 2^(k + prec - 1) from its bits (integer work on the uniform k: scalar registers)
+template <class TI>
+__device__ inline TI vs_sigma(int k);
+template <>
+__device__ inline double vs_sigma<double>(int k) {
+	return __builtin_bit_cast(double, ((uint64_t)(k + 52 + 1023) << 52) | (uint64_t(1) << 51));
+}
+template <>
+__device__ inline float vs_sigma<float>(int k) {
+	return __builtin_bit_cast(float, ((uint32_t)(k + 23 + 127) << 23) | (uint32_t(1) << 22));
+}
+
+template <class TI, int NS>
+struct VsGen {
+	TI sig[NS];
+	int last, zero;
+	__device__ inline void init(const VsPlan *p) {
+		last = p->last;
+		zero = p->zero;
+#pragma unroll
+		for (int q = 0; q < NS; q++) sig[q] = (q <= last) ? vs_sigma<TI>(p->k[q]) : (TI)0;
+	}
+	__device__ inline void gen(TI v, double (&out)[NS]) const {
+		TI r = v;
+#pragma unroll
+		for (int q = 0; q < NS; q++) {
+			if (q <= last) {
+				const TI a = (r + sig[q]) - sig[q];
+				out[q] = (double)a;
+				r = r - a;
+			} else {
+				out[q] = (q == last + 1 && !zero) ? (double)r : 0.0;
+			}
+		}
+	}
+};
+
+// slot q alone (the levels before it replayed): the general paths with more slots than registers
+template <class TI>
+__device__ inline double vs_slot(TI v, const VsPlan *p, int q) {
+	TI r = v;
+	const int last = p->last;
+	for (int t = 0; t <= last; t++) {
+		const TI sg = vs_sigma<TI>(p->k[t]);
+		const TI a = (r + sg) - sg;
+		if (t == q) return (double)a;
+		r = r - a;
+	}
+	return (q == last + 1 && !p->zero) ? (double)r : 0.0;
 }
 
 // the reference loop in one workgroup of BS threads (inputs outside the model above): slice s at slot s, its
@@ -568,9 +733,7 @@ template <class TI>
 __global__ __launch_bounds__(256) void k_vsplit_write(int n, const TI *x, double *slices,
                                                       int64_t lds, const VsPlan *plan, int rho,
                                                       int shift, int cap, int reproMode,
-                                                      const uint16_t *J, int *info,
-                                                      const int *stopAt, int it) {
-	if (stopAt && it >= *stopAt) return;
+                                                      const uint16_t *J, int *info) {
 	if (plan->bad) {
 		if (blockIdx.x == 0) vsplit_fallback<256, TI>(n, x, slices, lds, rho, shift, cap, reproMode, J, info);
 		return;
@@ -592,16 +755,18 @@ __global__ __launch_bounds__(256) void k_vsplit_write(int n, const TI *x, double
 
 size_t vsplit_scratch_bytes() { return 8ull * kVsK + 8ull * kVsBlocks + 64 + sizeof(VsPre) + 64; }
 
-// Splits x (length n) into slices at stride lds (cap + 2 slots); the count goes to info[0] on the
-// device.  scratch: vsplit_scratch_bytes(), zeroed once (the kernels leave it zero again).
+// The plan of x (length n): the predicted chain, then the table scan and walk only when it does not
+// hold; with fallback the walk's workgroup also stores the slices of a plan marked bad (the
+// reference loop).  scratch: vsplit_scratch_bytes(), zeroed once (the kernels leave it zero again).
 template <class TI>
-static hipError_t vsplit(hipStream_t st, int n, const TI *x, double *slices, int64_t lds, int rho,
-                         int shift, int cap, int reproMode, const uint16_t *J,
-                         unsigned long long *maxv, int *info, void *scratch, VsPlan *plan,
-                         const int *stopAt, int it) {
+static hipError_t vsplit_plan(hipStream_t st, int n, const TI *x, double *slices, int64_t lds, int rho,
+                              int shift, int cap, int reproMode, const uint16_t *J, int *info,
+                              void *scratch, VsPre *pre, VsPlan *plan, bool statsDone, CgDev *S, int it,
+                              int clkCat, int fallback) {
 	unsigned long long *Fg = (unsigned long long *)scratch;
 	int2 *range = (int2 *)(Fg + kVsK);
-	VsPre *pre = (VsPre *)((char *)scratch + 8ull * kVsK + 8ull * kVsBlocks + 64);
+	if (pre == nullptr) pre = (VsPre *)((char *)scratch + 8ull * kVsK + 8ull * kVsBlocks + 64);
+	const int *stopAt = S ? &S->stopAt : nullptr;
 	int rhoEff = rho;
 	if (shift > 1) {  // sigma * 2^j acts as rho + j
 		if (shift & (shift - 1)) rhoEff = -1000;  // not a power of two: the fallback splits
@@ -609,38 +774,62 @@ static hipError_t vsplit(hipStream_t st, int n, const TI *x, double *slices, int
 	}
 	int grid = (int)std::min<int64_t>(kVsBlocks, ((int64_t)n + kVsBS - 1) / kVsBS);
 	if (grid < 1) grid = 1;
-	// the predicted chain first; the table scan and walk only when it does not hold
-	hipLaunchKernelGGL((k_vsplit_pre<kVsBS, TI>), dim3(grid), dim3(kVsBS), 0, st, n, x, pre, stopAt, it);
-	hipLaunchKernelGGL((k_vsplit_levels<kVsBS, TI>), dim3(grid), dim3(kVsBS), 0, st, n, x, pre, rhoEff, cap,
-	                   J, stopAt, it);
-	hipLaunchKernelGGL(k_vsplit_verify<TI>, dim3(1), dim3(64), 0, st, pre, rhoEff, cap, reproMode, J, plan,
-	                   info, maxv, stopAt, it);
-	const int *gate = &plan->needScan;
-	hipLaunchKernelGGL((k_vsplit_scan<kVsBS, TI>), dim3(grid), dim3(kVsBS), 0, st, n, x, Fg, range, stopAt,
-	                   it, gate);
-	hipLaunchKernelGGL((k_vsplit_walk<kVsBS, TI>), dim3(1), dim3(kVsBS), 0, st, grid, Fg,
-	                   (const int2 *)range, rhoEff, cap, reproMode, J, plan, info, maxv, stopAt, it, gate);
-	int gw = (int)std::min<int64_t>(2048, ((int64_t)n + 255) / 256);
-	if (gw < 1) gw = 1;
-	hipLaunchKernelGGL(k_vsplit_write<TI>, dim3(gw), dim3(256), 0, st, n, x, slices, lds,
-	                   (const VsPlan *)plan, rho, shift, cap, reproMode, J, info, stopAt, it);
+	int gridL = (int)std::min<int64_t>(kVsLevBlocks, ((int64_t)n + 4 * kVsBS - 1) / (4 * kVsBS));
+	if (gridL < 1) gridL = 1;
+	if (!statsDone)
+		hipLaunchKernelGGL((k_vsplit_pre<kVsBS, TI>), dim3(gridL), dim3(kVsBS), 0, st, n, x, pre, stopAt, it);
+	hipLaunchKernelGGL((k_vsplit_levels<kVsBS, TI>), dim3(gridL), dim3(kVsBS), 0, st, n, x, pre, rhoEff, cap,
+	                   J, S, it, clkCat);
+	hipLaunchKernelGGL((k_vsplit_verify<256, TI>), dim3(1), dim3(256), 0, st, pre, rhoEff, cap, reproMode, J,
+	                   plan, info, stopAt, it);
+	unsigned int *done = (unsigned int *)(range + kVsBlocks);
+	hipLaunchKernelGGL((k_vsplit_scan<kVsBS, TI>), dim3(grid), dim3(kVsBS), 0, st, n, x, Fg, range, done,
+	                   stopAt, it, (const int *)&plan->needScan, rhoEff, cap, reproMode, J, plan, info,
+	                   slices, lds, rho, shift, fallback);
 	return hipGetLastError();
 }
 
-hipError_t launch_vsplit(hipStream_t st, int n, const double *x, double *slices, int64_t lds,
-                         int rho, int shift, int cap, int reproMode, const uint16_t *J,
-                         unsigned long long *maxv, int *info, void *scratch, VsPlan *plan,
-                         const int *stopAt, int it) {
-	return vsplit(st, n, x, slices, lds, rho, shift, cap, reproMode, J, maxv, info, scratch, plan,
-	              stopAt, it);
+// the plan, then the slices (slice_s = cmod(x, u_{s-1}) - cmod(x, u_s), remainder, zero slots)
+template <class TI>
+static hipError_t vsplit(hipStream_t st, int n, const TI *x, double *slices, int64_t lds, int rho,
+                         int shift, int cap, int reproMode, const uint16_t *J, int *info, void *scratch,
+                         VsPlan *plan) {
+	const hipError_t e = vsplit_plan(st, n, x, slices, lds, rho, shift, cap, reproMode, J, info, scratch,
+	                                 nullptr, plan, false, nullptr, 0, kClkNone, 0);
+	if (e != hipSuccess) return e;
+	int gw = (int)std::min<int64_t>(2048, ((int64_t)n + 255) / 256);
+	if (gw < 1) gw = 1;
+	hipLaunchKernelGGL(k_vsplit_write<TI>, dim3(gw), dim3(256), 0, st, n, x, slices, lds,
+	                   (const VsPlan *)plan, rho, shift, cap, reproMode, J, info);
+	return hipGetLastError();
 }
 
-hipError_t launch_vsplit(hipStream_t st, int n, const float *x, double *slices, int64_t lds,
-                         int rho, int shift, int cap, int reproMode, const uint16_t *JF,
-                         unsigned long long *maxv, int *info, void *scratch, VsPlan *plan,
-                         const int *stopAt, int it) {
-	return vsplit(st, n, x, slices, lds, rho, shift, cap, reproMode, JF, maxv, info, scratch, plan,
-	              stopAt, it);
+hipError_t launch_vsplit(hipStream_t st, int n, const double *x, double *slices, int64_t lds, int rho,
+                         int shift, int cap, int reproMode, const uint16_t *J, int *info, void *scratch,
+                         VsPlan *plan) {
+	return vsplit(st, n, x, slices, lds, rho, shift, cap, reproMode, J, info, scratch, plan);
+}
+
+hipError_t launch_vsplit(hipStream_t st, int n, const float *x, double *slices, int64_t lds, int rho,
+                         int shift, int cap, int reproMode, const uint16_t *JF, int *info, void *scratch,
+                         VsPlan *plan) {
+	return vsplit(st, n, x, slices, lds, rho, shift, cap, reproMode, JF, info, scratch, plan);
+}
+
+hipError_t launch_vsplit_plan(hipStream_t st, int n, const double *x, double *slices, int64_t lds,
+                              int rho, int shift, int cap, int reproMode, const uint16_t *J, int *info,
+                              void *scratch, VsPre *pre, VsPlan *plan, bool statsDone, CgDev *S, int it,
+                              int clkCat) {
+	return vsplit_plan(st, n, x, slices, lds, rho, shift, cap, reproMode, J, info, scratch, pre, plan,
+	                   statsDone, S, it, clkCat, 1);
+}
+
+hipError_t launch_vsplit_plan(hipStream_t st, int n, const float *x, double *slices, int64_t lds,
+                              int rho, int shift, int cap, int reproMode, const uint16_t *JF, int *info,
+                              void *scratch, VsPre *pre, VsPlan *plan, bool statsDone, CgDev *S, int it,
+                              int clkCat) {
+	return vsplit_plan(st, n, x, slices, lds, rho, shift, cap, reproMode, JF, info, scratch, pre, plan,
+	                   statsDone, S, it, clkCat, 1);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -650,7 +839,8 @@ hipError_t launch_vsplit(hipStream_t st, int n, const float *x, double *slices, 
 // ---------------------------------------------------------------------------------------------
 constexpr int kDotChunk = 16;
 constexpr int kDotPairs = 64;    // pair slots per workgroup partial of k_dots_reg
-constexpr int kDotMaxWG = 256;  // workgroups of k_dots_reg at most (one wave per SIMD)
+constexpr int kDotMaxWG = 512;  // workgroups of k_dots_reg at most (two waves per SIMD)
+constexpr int kDotRegSum = 36;  // pair dots the completion sums in registers (6 x 6 slices)
 
 template <int BS>
 __global__ __launch_bounds__(BS) void k_slice_dots(int n, const double *xs, int64_t ldx, int nA,
@@ -888,11 +1078,13 @@ void launch_nearsum_small(hipStream_t st, int nvals, double *vals, double alpha,
 // ---------------------------------------------------------------------------------------------
 // T = float: the binary32 instantiation -- each pair sum is the fma1<float> chain
 // (float)fma(a, x, (double)v) of csrmm_n<float>, then NearSum<float> (J = JF2) or Fsum in float
-template <int NAM, int NBM, class T = double>
+// With OTF the slices of x are generated from x and its plan (gen) instead of read from Xs.
+template <int NAM, int NBM, class T = double, class TI = double, bool OTF = false>
 __device__ __forceinline__ T csrmm_row_sum(int r, const double *As, int64_t ldas, int nA,
                                            const int *colind, const int *rowptr, const double *Xs,
                                            int64_t ldxs, int nB, int maxlevel, int fsum,
-                                           const uint16_t *J) {
+                                           const uint16_t *J, const TI *X = nullptr,
+                                           const VsGen<TI, NBM> *gen = nullptr) {
 	const int b = rowptr[r], e = rowptr[r + 1];
 	RegVec<NAM * NBM, T> v;
 #pragma unroll
@@ -911,9 +1103,16 @@ __device__ __forceinline__ T csrmm_row_sum(int r, const double *As, int64_t ldas
 #pragma unroll
 			for (int ia = 0; ia < NAM; ia++)
 				a[q][ia] = (in && ia < nA) ? As[(int64_t)ia * ldas + j0 + q] : 0.0;
+			if constexpr (OTF) {
+				gen->gen(in ? X[c[q]] : (TI)0, x[q]);
 #pragma unroll
-			for (int ib = 0; ib < NBM; ib++)
-				x[q][ib] = (in && ib < nB) ? Xs[(int64_t)ib * ldxs + c[q]] : 0.0;
+				for (int ib = 0; ib < NBM; ib++)
+					if (ib >= nB) x[q][ib] = 0.0;
+			} else {
+#pragma unroll
+				for (int ib = 0; ib < NBM; ib++)
+					x[q][ib] = (in && ib < nB) ? Xs[(int64_t)ib * ldxs + c[q]] : 0.0;
+			}
 		}
 #pragma unroll
 		for (int q = 0; q < CH; q++) {
@@ -1511,15 +1710,16 @@ void launch_scal_axpy(hipStream_t st, int n, float beta, const float *r, float *
 __device__ inline bool cg_skip(const CgDev *S, int it) { return S != nullptr && it >= S->stopAt; }
 
 // q = A p with the pre-split matrix (nA <= NAM) and the device slice count of p; alpha = 1, beta = 0
-// as ozblasRcsrmv(..., 1., ..., 0., q) at src/cpu/ozblas_cg.cpp:82
+// as ozblasRcsrmv(..., 1., ..., 0., q) at src/cpu/ozblas_cg.cpp:82.  p's slices come from p and its
+// plan (or its stored slices when the plan is marked bad); the reductions of q's split accumulate
+// into preY.
 template <int NAM, int NBM>
 __global__ __launch_bounds__(256) void k_csrmm_sum_dev(int m, const double *As, int64_t ldas, int nA,
-                                                       const int *colind, const int *rowptr,
-                                                       const double *Xs, int64_t ldxs,
-                                                       const int *infoX, int fsum, double *y,
-                                                       const uint16_t *J, CgDev *S, int it) {
+                                                       const int *colind, const int *rowptr, VsRef X,
+                                                       int fsum, double *y, const uint16_t *J,
+                                                       CgDev *S, int it, VsPre *preY) {
 	if (cg_skip(S, it)) return;
-	const int nB = infoX[0];
+	const int nB = X.info[0];
 	if (nB > S->maxVecSlices) {
 		if (blockIdx.x == 0 && threadIdx.x == 0) {
 			S->overflow = 1;
@@ -1529,51 +1729,74 @@ __global__ __launch_bounds__(256) void k_csrmm_sum_dev(int m, const double *As, 
 	}
 	if (nB > NBM) return;  // k_csrmm_big_dev computes this product
 	const int r = blockIdx.x * blockDim.x + threadIdx.x;
-	if (r >= m) return;
-	const int maxlevel = (nA - 1) + (nB - 1);
-	const double t = csrmm_row_sum<NAM, NBM>(r, As, ldas, nA, colind, rowptr, Xs, ldxs, nB, maxlevel,
-	                                         fsum, J);
-	y[r] = fma(1.0, t, 0.0 * y[r]);
+	double mx = 0.0;
+	int lo = 0, bad = 0;
+	if (r < m) {
+		const int maxlevel = (nA - 1) + (nB - 1);
+		double t;
+		if (X.plan->bad) {
+			t = csrmm_row_sum<NAM, NBM>(r, As, ldas, nA, colind, rowptr, X.slices, X.lds, nB, maxlevel, fsum, J);
+		} else {
+			VsGen<double, NBM> g;
+			g.init(X.plan);
+			t = csrmm_row_sum<NAM, NBM, double, double, true>(r, As, ldas, nA, colind, rowptr, nullptr, 0, nB,
+			                                                  maxlevel, fsum, J, (const double *)X.x, &g);
+		}
+		const double v = fma(1.0, t, 0.0 * y[r]);
+		y[r] = v;
+		vs_stats_acc(v, mx, lo, bad);
+	}
+	vs_stats_flush<256>(mx, lo, bad, preY);
 }
 
 // The same product when p has more slices than the register kernel holds: each pair sum runs over
 // the row's nonzeros in order (exact) into Cs[q * ldc + r], then NearSum over the row's pairs, or
 // Fsum in (level, ia) order computed on the fly.
 __global__ __launch_bounds__(256) void k_csrmm_big_dev(int m, const double *As, int64_t ldas, int nA,
-                                                       const int *colind, const int *rowptr,
-                                                       const double *Xs, int64_t ldxs,
-                                                       const int *infoX, int nbReg, int fsum,
-                                                       double *y, double *Cs, int64_t ldc,
-                                                       const uint16_t *J, const CgDev *S, int it) {
+                                                       const int *colind, const int *rowptr, VsRef X,
+                                                       int nbReg, int fsum, double *y, double *Cs,
+                                                       int64_t ldc, const uint16_t *J, const CgDev *S,
+                                                       int it, VsPre *preY) {
 	if (cg_skip(S, it)) return;
-	const int nB = infoX[0];
+	const int nB = X.info[0];
 	if (nB <= nbReg || nB > S->maxVecSlices) return;
 	const int r = blockIdx.x * blockDim.x + threadIdx.x;
-	if (r >= m) return;
-	const int b = rowptr[r], e = rowptr[r + 1];
-	auto pair = [&](int ia, int ib) {
-		double acc = 0.0;
-		for (int j = b; j < e; j++)
-			acc = fma(As[(int64_t)ia * ldas + j], Xs[(int64_t)ib * ldxs + colind[j]], acc);
-		return acc;
-	};
-	const int maxlevel = (nA - 1) + (nB - 1);
-	double t;
-	if (fsum) {
-		t = 0.;
-		for (int ik = 0; ik <= maxlevel; ik++)
-			for (int ia = 0; ia < nA; ia++) {
-				const int ib = ik - ia;
-				if (ib >= 0 && ib < nB) t = t + pair(ia, ib);
+	const bool stored = X.plan->bad;
+	const double *Xv = (const double *)X.x;
+	double mx = 0.0;
+	int lo = 0, bad = 0;
+	if (r < m) {
+		const int b = rowptr[r], e = rowptr[r + 1];
+		auto pair = [&](int ia, int ib) {
+			double acc = 0.0;
+			for (int j = b; j < e; j++) {
+				const double xv = stored ? X.slices[(int64_t)ib * X.lds + colind[j]]
+				                         : vs_slot<double>(Xv[colind[j]], X.plan, ib);
+				acc = fma(As[(int64_t)ia * ldas + j], xv, acc);
 			}
-	} else {
-		int cnt = 0;
-		for (int ia = 0; ia < nA; ia++)
-			for (int ib = 0; ib < nB; ib++) Cs[(int64_t)(cnt++) * ldc + r] = pair(ia, ib);
-		StridedVec v{Cs + r, ldc};
-		t = nearsum(cnt, v, J);
+			return acc;
+		};
+		const int maxlevel = (nA - 1) + (nB - 1);
+		double t;
+		if (fsum) {
+			t = 0.;
+			for (int ik = 0; ik <= maxlevel; ik++)
+				for (int ia = 0; ia < nA; ia++) {
+					const int ib = ik - ia;
+					if (ib >= 0 && ib < nB) t = t + pair(ia, ib);
+				}
+		} else {
+			int cnt = 0;
+			for (int ia = 0; ia < nA; ia++)
+				for (int ib = 0; ib < nB; ib++) Cs[(int64_t)(cnt++) * ldc + r] = pair(ia, ib);
+			StridedVec v{Cs + r, ldc};
+			t = nearsum(cnt, v, J);
+		}
+		const double v = fma(1.0, t, 0.0 * y[r]);
+		y[r] = v;
+		vs_stats_acc(v, mx, lo, bad);
 	}
-	y[r] = fma(1.0, t, 0.0 * y[r]);
+	vs_stats_flush<256>(mx, lo, bad, preY);
 }
 
 // Completion of a device DOT in one workgroup, after the product kernel: sums the nA x nB pair
@@ -1582,40 +1805,58 @@ __global__ __launch_bounds__(256) void k_csrmm_big_dev(int m, const double *As, 
 //   1: tmp = t, alpha = dold / tmp                       (src/cpu/ozblas_cg.cpp:83-84)
 //   2: dnew = t, resi = sqrt(dnew), beta = dnew / dold,  (src/cpu/ozblas_cg.cpp:91-94,100-112)
 //      dold = dnew, history and the residual test
-// then clears vals and the split maxima it consumed for the next use.
+// then clears vals for the next use.
 // With f32 (binary32 DOT; mode 0 only) the pair dots round to float first, the sum is the float
 // one (J is then JF2) and C is a float: C[0] = fma1<float>(1, t, 0 * C[0]).
-// The pair dots are the per-workgroup partials part[w * kDotPairs + p] of k_dots_reg's nwg
+// The pair dots are the per-workgroup partials part[p * kDotMaxWG + w] of k_dots_reg's nwg
 // workgroups, summed here by the whole workgroup (exact in any order), or with part == nullptr
 // the sums dots_general accumulated in vals.
-__device__ void dots_finish(int nA, int nB, double *vals, const double *part, int nwg,
-                            unsigned long long *maxvX, unsigned long long *maxvY, int nmax,
-                            const uint16_t *J, int fsum, CgDev *S, int it, int mode, void *C,
-                            int f32) {
-	__shared__ double psum[1][kDotPairs];
+template <int BS>
+__device__ void dots_finish(int nA, int nB, double *vals, const double *part, int nwg, const uint16_t *J,
+                            int fsum, CgDev *S, int it, int mode, void *C, int f32) {
+	static_assert(kDotMaxWG <= 2 * BS, "two partials per thread and pair");
+	__shared__ double psum[kDotPairs];
+	__shared__ double red[BS / 64][16];
 	const int npairs = nA * nB;
 	double *src = vals;
 	if (part) {
-		// entry k = w * npairs + p of the partials, spread over the threads, loads issued in groups
-		// of 8 before their LDS additions
-		if (threadIdx.x < kDotPairs) psum[0][threadIdx.x] = 0.0;
-		__syncthreads();
-		const int total = nwg * npairs;
-		for (int k0 = threadIdx.x; k0 < total; k0 += 8 * (int)blockDim.x) {
-			double v[8];
+		// pairs in chunks of 16: every thread loads its (at most two) partials of each pair of the
+		// chunk, all loads in flight together (pair p's partials are contiguous: pair-major), then a
+		// butterfly over each wave and the waves' sums in LDS
+		const int t = threadIdx.x, w = t / 64, l = t % 64;
+		for (int p0 = 0; p0 < npairs; p0 += 16) {
+			double v[16];
 #pragma unroll
-			for (int u = 0; u < 8; u++) {
-				const int k = k0 + u * (int)blockDim.x;
-				v[u] = (k < total) ? part[(int64_t)(k / npairs) * kDotPairs + k % npairs] : 0.0;
+			for (int q = 0; q < 16; q++) {
+				double a = 0.0, b = 0.0;
+				if (p0 + q < npairs) {
+					const double *row = part + (int64_t)(p0 + q) * kDotMaxWG;
+					a = (t < nwg) ? row[t] : 0.0;
+					b = (t + BS < nwg) ? row[t + BS] : 0.0;
+				}
+				v[q] = a + b;
 			}
 #pragma unroll
-			for (int u = 0; u < 8; u++) {
-				const int k = k0 + u * (int)blockDim.x;
-				if (k < total && v[u] != 0.0) atomicAdd(&psum[0][k % npairs], v[u]);
+			for (int o = 32; o > 0; o >>= 1) {
+				double u[16];
+#pragma unroll
+				for (int q = 0; q < 16; q++) u[q] = __shfl_xor(v[q], o, 64);
+#pragma unroll
+				for (int q = 0; q < 16; q++) v[q] += u[q];
 			}
+			if (l == 0) {
+#pragma unroll
+				for (int q = 0; q < 16; q++) red[w][q] = v[q];
+			}
+			__syncthreads();
+			if (t < 16 && p0 + t < npairs) {
+				double r = red[0][t];
+				for (int k = 1; k < BS / 64; k++) r += red[k][t];
+				psum[p0 + t] = r;
+			}
+			__syncthreads();
 		}
-		__syncthreads();
-		src = psum[0];
+		src = psum;
 	}
 	if (threadIdx.x != 0) return;
 	const int maxlevel = (nA - 1) + (nB - 1);  // DOT forces fastMode = 0 (src/cpu/ozblas_gemm.cpp:25-28)
@@ -1624,14 +1865,20 @@ __device__ void dots_finish(int nA, int nB, double *vals, const double *part, in
 		const float t = dot_pairs_sum_f(npairs, fv, nA, nB, maxlevel, fsum, J);
 		float *Cf = (float *)C;
 		Cf[0] = fma1_f(1.f, t, 0.f * Cf[0]);
-		for (int p = 0; p < npairs; p++) vals[p] = 0.0;
-		for (int q = 0; q < nmax; q++) {
-			maxvX[q] = 0ull;
-			maxvY[q] = 0ull;
-		}
+		for (int q = 0; q < npairs; q++) vals[q] = 0.0;
 		return;
 	}
-	const double t = dot_pairs_sum(npairs, src, nA, nB, maxlevel, fsum, J);
+	double t;
+	if (!fsum && npairs <= kDotRegSum) {
+		// NearSum with the pair dots in registers (its result does not depend on the order of the
+		// terms): no memory round trip per step of its passes
+		RegVec<kDotRegSum> u;
+#pragma unroll
+		for (int q = 0; q < kDotRegSum; q++) u[q] = (q < npairs) ? src[q] : 0.0;
+		t = nearsum(npairs, u, J);
+	} else {
+		t = dot_pairs_sum(npairs, src, nA, nB, maxlevel, fsum, J);
+	}
 	const double r = fma(1.0, t, 0.0 * 0.0);
 	if (mode == 0) {
 		double *Cd = (double *)C;
@@ -1655,21 +1902,22 @@ __device__ void dots_finish(int nA, int nB, double *vals, const double *part, in
 		}
 		if (resi / S->nrmb < S->tol) S->stopAt = it + 1;
 	}
-	for (int p = 0; p < npairs; p++) vals[p] = 0.0;
-	for (int q = 0; q < nmax; q++) {
-		maxvX[q] = 0ull;
-		maxvY[q] = 0ull;
-	}
-
+	for (int q = 0; q < npairs; q++) vals[q] = 0.0;
 }
 
-// The same for any slice counts (pairs in chunks of kDotChunk, slices re-read per chunk, exact
-// partials added atomically to vals); k_dots_reg runs it when the counts exceed its registers.
-template <int BS>
-__device__ void dots_general(int n, int nA, int nB, const double *xs, int64_t ldx, const double *ys,
-                             int64_t ldy, double *vals) {
+// slot q of element i of a split vector: generated from the plan, or read from the stored slices
+template <class TI>
+__device__ inline double vs_ref_slot(const VsRef &V, bool stored, int64_t i, int q) {
+	return stored ? V.slices[(int64_t)q * V.lds + i] : vs_slot<TI>(((const TI *)V.x)[i], V.plan, q);
+}
+
+// The same for any slice counts (pairs in chunks of kDotChunk, exact partials added atomically to
+// vals); k_dots_reg runs it when the counts exceed its registers.
+template <int BS, class TI>
+__device__ void dots_general(int n, int nA, int nB, const VsRef &X, const VsRef &Y, double *vals) {
 	__shared__ double red[BS / 64];
 	const int npairs = nA * nB;
+	const bool sx = X.plan->bad, sy = Y.plan->bad;
 	for (int p0 = 0; p0 < npairs; p0 += kDotChunk) {
 		double acc[kDotChunk];
 #pragma unroll
@@ -1680,7 +1928,7 @@ __device__ void dots_general(int n, int nA, int nB, const double *xs, int64_t ld
 				const int p = p0 + q;
 				if (p < npairs) {
 					const int ia = p % nA, ib = p / nA;
-					acc[q] = fma(xs[(int64_t)ia * ldx + i], ys[(int64_t)ib * ldy + i], acc[q]);
+					acc[q] = fma(vs_ref_slot<TI>(X, sx, i, ia), vs_ref_slot<TI>(Y, sy, i, ib), acc[q]);
 				}
 			}
 		}
@@ -1692,51 +1940,92 @@ __device__ void dots_general(int n, int nA, int nB, const double *xs, int64_t ld
 	}
 }
 
-// DOT of two split vectors with device slice counts, slice counts up to NAM x NBM: every element's
-// slices are read once (two elements per thread and step, their loads issued together) and all
-// pair products accumulate in registers.  Each workgroup stores its pair partials to
-// part[blockIdx.x * kDotPairs + p] -- no atomics on a few shared addresses -- and the last one
-// sums them (integer multiples of the product unit below 2^53: any order is exact).  Larger
-// counts take dots_general.
-template <int BS, int NAM, int NBM>
-__global__ __launch_bounds__(BS) void k_dots_reg(int n, const double *xs, int64_t ldx,
-                                                 const int *infoX, const double *ys, int64_t ldy,
-                                                 const int *infoY, double *vals, double *part,
+// the slots of element i (0 beyond n) of a split vector into registers, masked to the count
+template <class TI, int NS>
+__device__ __forceinline__ void vs_ref_load(const VsRef &V, bool stored, const VsGen<TI, NS> &g, int cnt,
+                                            int64_t i, bool in, double (&out)[NS]) {
+	if (stored) {
+#pragma unroll
+		for (int q = 0; q < NS; q++) out[q] = (in && q < cnt) ? V.slices[(int64_t)q * V.lds + i] : 0.0;
+	} else {
+		g.gen(in ? ((const TI *)V.x)[i] : (TI)0, out);
+#pragma unroll
+		for (int q = 0; q < NS; q++)
+			if (q >= cnt) out[q] = 0.0;
+	}
+}
+
+template <class TI, int NS>
+__device__ __forceinline__ void vs_gen_masked(const VsGen<TI, NS> &g, TI v, int cnt, double (&out)[NS]) {
+	g.gen(v, out);
+#pragma unroll
+	for (int q = 0; q < NS; q++)
+		if (q >= cnt) out[q] = 0.0;
+}
+
+// DOT of two split vectors with device slice counts, slice counts up to NAM x NBM: each element is
+// read once (two elements per thread and step, their loads issued together), its slices generated
+// in registers from the plans, and all pair products accumulate in registers.  Each workgroup
+// stores its pair partials to part[p * kDotMaxWG + blockIdx.x] -- no atomics on a few shared
+// addresses -- and k_dots_fin sums them (integer multiples of the product unit below 2^53: any
+// order is exact).  Larger counts take dots_general.
+// the pair accumulation over this thread's elements; SX / SY: the vector's slices are stored (its
+// plan is marked bad) rather than generated
+template <int BS, int NAM, int NBM, class TI, bool SX, bool SY>
+__device__ __forceinline__ void dots_reg_acc(int n, const VsRef &X, const VsRef &Y, int nA, int nB,
+                                             double (&acc)[NAM * NBM]) {
+	VsGen<TI, NAM> gx;
+	VsGen<TI, NBM> gy;
+	if constexpr (!SX) gx.init(X.plan);
+	if constexpr (!SY) gy.init(Y.plan);
+	const TI *xv = (const TI *)X.x, *yv = (const TI *)Y.x;
+	const int64_t stride = (int64_t)gridDim.x * BS;
+	constexpr int E = (SX || SY || sizeof(TI) == 4) ? 2 : 4;  // elements per thread and step
+	for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += E * stride) {
+		// every element's loads in flight, then one element's slices at a time
+		TI xe[E], ye[E];
+#pragma unroll
+		for (int e = 0; e < E; e++) {
+			const int64_t ie = i + e * stride;
+			xe[e] = (!SX && ie < n) ? xv[ie] : (TI)0;
+			ye[e] = (!SY && ie < n) ? yv[ie] : (TI)0;
+		}
+#pragma unroll
+		for (int e = 0; e < E; e++) {
+			const int64_t ie = i + e * stride;
+			const bool in = ie < n;
+			double xa[NAM], yb[NBM];
+			if constexpr (SX) vs_ref_load<TI, NAM>(X, true, gx, nA, ie, in, xa);
+			else vs_gen_masked<TI, NAM>(gx, xe[e], nA, xa);
+			if constexpr (SY) vs_ref_load<TI, NBM>(Y, true, gy, nB, ie, in, yb);
+			else vs_gen_masked<TI, NBM>(gy, ye[e], nB, yb);
+#pragma unroll
+			for (int a = 0; a < NAM; a++)
+#pragma unroll
+				for (int b = 0; b < NBM; b++) acc[a * NBM + b] = fma(xa[a], yb[b], acc[a * NBM + b]);
+		}
+	}
+}
+
+template <int BS, int NAM, int NBM, class TI>
+__global__ __launch_bounds__(BS) void k_dots_reg(int n, VsRef X, VsRef Y, double *vals, double *part,
                                                  const CgDev *S, int it) {
 	static_assert(NAM * NBM <= kDotPairs && BS % 64 == 0 && BS <= 256, "pair partial layout");
 	if (cg_skip(S, it)) return;
-	const int nA = infoX[0], nB = infoY[0];
+	const int nA = X.info[0], nB = Y.info[0];
 	if (nA > NAM || nB > NBM) {
-		dots_general<BS>(n, nA, nB, xs, ldx, ys, ldy, vals);
+		dots_general<BS, TI>(n, nA, nB, X, Y, vals);
 		return;
 	}
+	const bool sx = X.plan->bad, sy = Y.plan->bad;
 	__shared__ double red[BS / 64][NAM * NBM];
 	double acc[NAM * NBM];
 #pragma unroll
 	for (int q = 0; q < NAM * NBM; q++) acc[q] = 0.0;
-	const int64_t stride = (int64_t)gridDim.x * BS;
-	for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += 2 * stride) {
-		const int64_t i1 = i + stride;
-		const bool two = i1 < n;
-		double xa[2][NAM], yb[2][NBM];
-#pragma unroll
-		for (int a = 0; a < NAM; a++) {
-			xa[0][a] = (a < nA) ? xs[(int64_t)a * ldx + i] : 0.0;
-			xa[1][a] = (a < nA && two) ? xs[(int64_t)a * ldx + i1] : 0.0;
-		}
-#pragma unroll
-		for (int b = 0; b < NBM; b++) {
-			yb[0][b] = (b < nB) ? ys[(int64_t)b * ldy + i] : 0.0;
-			yb[1][b] = (b < nB && two) ? ys[(int64_t)b * ldy + i1] : 0.0;
-		}
-#pragma unroll
-		for (int e = 0; e < 2; e++)
-#pragma unroll
-			for (int a = 0; a < NAM; a++)
-#pragma unroll
-				for (int b = 0; b < NBM; b++)
-					acc[a * NBM + b] = fma(xa[e][a], yb[e][b], acc[a * NBM + b]);
-	}
+	if (!sx && !sy) dots_reg_acc<BS, NAM, NBM, TI, false, false>(n, X, Y, nA, nB, acc);
+	else if (!sx) dots_reg_acc<BS, NAM, NBM, TI, false, true>(n, X, Y, nA, nB, acc);
+	else if (!sy) dots_reg_acc<BS, NAM, NBM, TI, true, false>(n, X, Y, nA, nB, acc);
+	else dots_reg_acc<BS, NAM, NBM, TI, true, true>(n, X, Y, nA, nB, acc);
 	const int w = threadIdx.x / 64, l = threadIdx.x % 64;
 	// wave sums: one butterfly step over 16 pairs at a time (independent lane exchanges in flight
 	// together), skipping the rows a >= nA; absent pairs hold zeros and are not read
@@ -1764,7 +2053,7 @@ __global__ __launch_bounds__(BS) void k_dots_reg(int n, const double *xs, int64_
 		if (a < nA && b < nB) {
 			double v = red[0][q];
 			for (int k = 1; k < BS / 64; k++) v += red[k][q];
-			part[(int64_t)blockIdx.x * kDotPairs + a + nA * b] = v;
+			part[(int64_t)(a + nA * b) * kDotMaxWG + blockIdx.x] = v;  // pair-major
 		}
 	}
 }
@@ -1773,77 +2062,97 @@ __global__ __launch_bounds__(BS) void k_dots_reg(int n, const double *xs, int64_
 // the general path accumulated in vals)
 template <int BS, int NAM, int NBM>
 __global__ __launch_bounds__(BS) void k_dots_fin(int nwg, const int *infoX, const int *infoY,
-                                                 double *vals, const double *part,
-                                                 unsigned long long *maxvX,
-                                                 unsigned long long *maxvY, int nmax,
-                                                 const uint16_t *J, int fsum, CgDev *S, int it,
-                                                 int mode, void *C, int f32) {
+                                                 double *vals, const double *part, const uint16_t *J,
+                                                 int fsum, CgDev *S, int it, int mode, void *C, int f32) {
 	if (cg_skip(S, it)) return;
 	const int nA = infoX[0], nB = infoY[0];
 	const bool reg = nA <= NAM && nB <= NBM;
-	dots_finish(nA, nB, vals, reg ? part : nullptr, nwg, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
+	dots_finish<BS>(nA, nB, vals, reg ? part : nullptr, nwg, J, fsum, S, it, mode, C, f32);
 }
 
-// x = fma(alpha, p, x); r = fma(-alpha, q, r)   (src/cpu/ozblas_cg.cpp:87-88, ozblas_axpy.cpp)
-__global__ void k_axpy2_dev(int n, const CgDev *S, int it, const double *P, double *X,
-                            const double *Q, double *R) {
+// x = fma(alpha, p, x); r = fma(-alpha, q, r)   (src/cpu/ozblas_cg.cpp:87-88, ozblas_axpy.cpp),
+// the first kernel of the AXPY category; r's split reductions into preR
+__global__ __launch_bounds__(256) void k_axpy2_dev(int n, CgDev *S, int it, const double *P, double *X,
+                                                   const double *Q, double *R, VsPre *preR) {
+	cg_clock(S, it, kClkAxpy);
 	if (cg_skip(S, it)) return;
 	const double a = S->alpha;
+	double mx = 0.0;
+	int lo = 0, bad = 0;
 	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
 	     i += (int64_t)gridDim.x * blockDim.x) {
 		X[i] = fma(a, P[i], X[i]);
-		R[i] = fma(-a, Q[i], R[i]);
+		const double r = fma(-a, Q[i], R[i]);
+		R[i] = r;
+		vs_stats_acc(r, mx, lo, bad);
 	}
+	vs_stats_flush<256>(mx, lo, bad, preR);
 }
 
-// p = beta * p, then p = fma(1, r, p)   (src/cpu/ozblas_cg.cpp:97-98)
-__global__ void k_scal_axpy_dev(int n, const CgDev *S, int it, const double *R, double *P) {
+// p = beta * p, then p = fma(1, r, p)   (src/cpu/ozblas_cg.cpp:97-98); p's split reductions into preP
+__global__ __launch_bounds__(256) void k_scal_axpy_dev(int n, CgDev *S, int it, const double *R, double *P,
+                                                       VsPre *preP) {
+	cg_clock(S, it, kClkAxpy);
 	if (cg_skip(S, it)) return;
 	const double b = S->beta;
+	double mx = 0.0;
+	int lo = 0, bad = 0;
 	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-	     i += (int64_t)gridDim.x * blockDim.x)
-		P[i] = fma(1.0, R[i], b * P[i]);
+	     i += (int64_t)gridDim.x * blockDim.x) {
+		const double p = fma(1.0, R[i], b * P[i]);
+		P[i] = p;
+		vs_stats_acc(p, mx, lo, bad);
+	}
+	vs_stats_flush<256>(mx, lo, bad, preP);
 }
 
 void launch_csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
-                          const int *colind, const int *rowptr, const double *Xs, int64_t ldxs,
-                          const int *infoX, int fsum, double *y, double *Cs, int64_t ldc,
-                          const uint16_t *J, CgDev *S, int it) {
+                          const int *colind, const int *rowptr, VsRef X, int fsum, double *y,
+                          double *Cs, int64_t ldc, const uint16_t *J, CgDev *S, int it, VsPre *preY) {
 	constexpr int kNbReg = 8;
 	if (nA <= 2)  // the matrix's count is known here; p's count only on the device
 		hipLaunchKernelGGL((k_csrmm_sum_dev<2, kNbReg>), dim3((m + 255) / 256), dim3(256), 0, st, m, As,
-		                   ldas, nA, colind, rowptr, Xs, ldxs, infoX, fsum, y, J, S, it);
+		                   ldas, nA, colind, rowptr, X, fsum, y, J, S, it, preY);
 	else
 		hipLaunchKernelGGL((k_csrmm_sum_dev<4, kNbReg>), dim3((m + 255) / 256), dim3(256), 0, st, m, As,
-		                   ldas, nA, colind, rowptr, Xs, ldxs, infoX, fsum, y, J, S, it);
+		                   ldas, nA, colind, rowptr, X, fsum, y, J, S, it, preY);
 	hipLaunchKernelGGL(k_csrmm_big_dev, dim3((m + 255) / 256), dim3(256), 0, st, m, As, ldas, nA,
-	                   colind, rowptr, Xs, ldxs, infoX, kNbReg, fsum, y, Cs, ldc, J, S, it);
+	                   colind, rowptr, X, kNbReg, fsum, y, Cs, ldc, J, (const CgDev *)S, it, preY);
 }
 
 size_t dots_part_bytes() { return sizeof(double) * kDotPairs * kDotMaxWG; }
 
-void launch_dots_dev(hipStream_t st, int n, const double *xs, int64_t ldx, const int *infoX,
-                     const double *ys, int64_t ldy, const int *infoY, double *vals, double *part,
-                     unsigned long long *maxvX, unsigned long long *maxvY, int nmax,
+void launch_dots_dev(hipStream_t st, int n, VsRef X, VsRef Y, double *vals, double *part,
                      const uint16_t *J, int fsum, CgDev *S, int it, int mode, void *C, int f32) {
 	constexpr int kRA = 8, kRB = 8;
-	int gx = (n + 511) / 512;  // two elements per thread and step
+	int gx = (n + 1023) / 1024;  // four elements per thread and step
 	if (gx > kDotMaxWG) gx = kDotMaxWG;
 	if (gx < 1) gx = 1;
-	hipLaunchKernelGGL((k_dots_reg<256, kRA, kRB>), dim3(gx), dim3(256), 0, st, n, xs, ldx, infoX, ys,
-	                   ldy, infoY, vals, part, (const CgDev *)S, it);
-	hipLaunchKernelGGL((k_dots_fin<256, kRA, kRB>), dim3(1), dim3(256), 0, st, gx, infoX, infoY, vals,
-	                   (const double *)part, maxvX, maxvY, nmax, J, fsum, S, it, mode, C, f32);
+	if (f32)
+		hipLaunchKernelGGL((k_dots_reg<256, kRA, kRB, float>), dim3(gx), dim3(256), 0, st, n, X, Y, vals,
+		                   part, (const CgDev *)S, it);
+	else
+		hipLaunchKernelGGL((k_dots_reg<256, kRA, kRB, double>), dim3(gx), dim3(256), 0, st, n, X, Y, vals,
+		                   part, (const CgDev *)S, it);
+	hipLaunchKernelGGL((k_dots_fin<256, kRA, kRB>), dim3(1), dim3(256), 0, st, gx, X.info, Y.info, vals,
+	                   (const double *)part, J, fsum, S, it, mode, C, f32);
 }
 
-void launch_axpy2_dev(hipStream_t st, int n, const CgDev *S, int it, const double *P, double *X,
-                      const double *Q, double *R) {
-	hipLaunchKernelGGL(k_axpy2_dev, dim3(grid1(n)), dim3(256), 0, st, n, S, it, P, X, Q, R);
+// the AXPYs that also reduce for the next split: fewer workgroups (each reduces once)
+static int grid_red(int64_t n) {
+	int64_t g = (n + 255) / 256;
+	if (g > 512) g = 512;
+	return g < 1 ? 1 : (int)g;
 }
 
-void launch_scal_axpy_dev(hipStream_t st, int n, const CgDev *S, int it, const double *R,
-                          double *P) {
-	hipLaunchKernelGGL(k_scal_axpy_dev, dim3(grid1(n)), dim3(256), 0, st, n, S, it, R, P);
+void launch_axpy2_dev(hipStream_t st, int n, CgDev *S, int it, const double *P, double *X,
+                      const double *Q, double *R, VsPre *preR) {
+	hipLaunchKernelGGL(k_axpy2_dev, dim3(grid_red(n)), dim3(256), 0, st, n, S, it, P, X, Q, R, preR);
+}
+
+void launch_scal_axpy_dev(hipStream_t st, int n, CgDev *S, int it, const double *R, double *P,
+                          VsPre *preP) {
+	hipLaunchKernelGGL(k_scal_axpy_dev, dim3(grid_red(n)), dim3(256), 0, st, n, S, it, R, P, preP);
 }
 
 }  // namespace oz

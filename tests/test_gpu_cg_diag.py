@@ -114,5 +114,7 @@ def test_cg_time_history_and_totals(oz, oracle, path):
     assert len(secs) == iters + 1
     assert secs[0] > 0 and np.all(np.diff(secs) >= 0) and secs[-1] <= wall
     assert H.t_SplitMat_total > 0
-    if path == "host":
-        assert H.t_SpMV_SpMM_total > 0 and H.t_DOT_NRM2_total > 0 and H.t_AXPY_SCAL_total > 0
+    # the category totals on both paths (the device path times its categories on the GPU clock)
+    totals = (H.t_SpMV_SpMM_total, H.t_DOT_NRM2_total, H.t_AXPY_SCAL_total)
+    assert all(t > 0 for t in totals), totals
+    assert sum(totals) + H.t_SplitMat_total <= wall, (totals, wall)

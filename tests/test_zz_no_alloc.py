@@ -10,7 +10,7 @@ from gpu_util import dev, say, torch_cuda
 
 pytestmark = pytest.mark.gpu
 
-MAX_N = 130688  # two distinct vectors, nSplitMax = 0, reproMode = 1
+MAX_N = 130432  # two distinct vectors, nSplitMax = 0, reproMode = 1
 
 
 @pytest.fixture(scope="module")
