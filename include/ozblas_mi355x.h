@@ -34,7 +34,8 @@
  * multiplied on the FP64 path, which is exact, then rounded once to float as the reference's float
  * slice products.  sliceType is not consulted (the digit forms are binary64 only), and Dot2
  * (splitEpsMode = 2) returns NOT_SUPPORTED, as the reference's float Dot2 GEMM is a stub
- * (src/cpu/ozblas_blas_wrapper.cpp:179-181).  ozblasScg iterates on the synchronous path.
+ * (src/cpu/ozblas_blas_wrapper.cpp:179-181).  ozblasScg iterates device-resident like ozblasDcg,
+ * for search directions of at most 8 slices (more redo the iterations on the synchronous path).
  */
 #ifndef OZBLAS_MI355X_H
 #define OZBLAS_MI355X_H

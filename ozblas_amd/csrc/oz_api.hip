@@ -1409,10 +1409,13 @@ static int32_t cg_host_loop(ozblasHandle_t *oh, int n, int nnz, const double *AS
 }
 
 // Device-resident iterations; *overflow = 1 when the host path must redo them.
+// T = float: the <float,float> solve with binary32 vectors and recurrences; p's slices are capped
+// at the CSR kernel's register bound (more overflow to the synchronous path).
+template <class T>
 static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *AS, const int *colind,
-                              const int *rowptr, double *x, int maxiter, double tol, double nrmb,
-                              double dold, const CgVecs<double> &v, double *hist, int *overflow,
-                              const CgDiag<double> &diag) {
+                              const int *rowptr, T *x, int maxiter, T tol, T nrmb, T dold,
+                              const CgVecs<T> &v, T *hist, int *overflow, const CgDiag<T> &diag) {
+	constexpr bool f32 = sizeof(T) == 4;
 	Backend *b = be(oh);
 	hipStream_t st = b->stream;
 	// times of the second history: an event after every iteration whose residual may be recorded
@@ -1427,7 +1430,8 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 		if (ev0) (void)hipEventDestroy(ev0);
 	};
 	const int cap = split_cap(oh);
-	const int rho = Prec<double>::rho(oh, n);
+	const int rho = Prec<T>::rho(oh, n);
+	const uint16_t *Js = Prec<T>::Jsplit(b), *Jn = Prec<T>::Jsum(b);
 	const int reproCount = repro_slot(oh) ? 0 : 1;
 	const int64_t lds = rup(n, 32);
 	const int fsum = oh->sumMode == 0 ? 1 : 0;
@@ -1441,10 +1445,11 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 	CgDev *S = (CgDev *)walloc(oh, sizeof(CgDev));
 	void *vscr = walloc(oh, vsplit_scratch_bytes());
 	VsPlan *plans = (VsPlan *)walloc(oh, 3 * sizeof(VsPlan));  // P, Q, R
-	VsPre *pres = (VsPre *)walloc(oh, 3 * sizeof(VsPre));
+	VsPre *pres = (VsPre *)walloc(oh, 6 * sizeof(VsPre));  // a pair per vector (split parity)
 	double *part = (double *)walloc(oh, dots_part_bytes());
 	if (!vscr || !plans || !pres || !part) return OZBLAS_STATUS_WORKSPACE;
-	const int maxVec = (oh->cgDeviceSlices > 0 && oh->cgDeviceSlices < cap) ? oh->cgDeviceSlices : cap;
+	int maxVec = (oh->cgDeviceSlices > 0 && oh->cgDeviceSlices < cap) ? oh->cgDeviceSlices : cap;
+	if (f32 && maxVec > 8) maxVec = 8;  // the binary32 product has no per-pair row loop
 	const int64_t ldc = rup(n, 32);
 	double *Cs = (double *)walloc(oh, 8ull * ldc * nA * maxVec);  // pair sums of the general product
 	if (!slP || !slQ || !slR || !sm || !S || !Cs) return OZBLAS_STATUS_WORKSPACE;
@@ -1453,7 +1458,7 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 	double *dhist = (double *)(info + 16);
 	OZ_HIP(hipMemsetAsync(sm, 0, small, st));
 	OZ_HIP(hipMemsetAsync(vscr, 0, vsplit_scratch_bytes(), st));
-	OZ_HIP(hipMemsetAsync(pres, 0, 3 * sizeof(VsPre), st));
+	OZ_HIP(hipMemsetAsync(pres, 0, 6 * sizeof(VsPre), st));
 	CgDev h{};
 	h.nrmb = nrmb;
 	h.dold = dold;
@@ -1467,7 +1472,7 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 	OZ_HIP(hipMemcpyAsync(S, &h, sizeof(CgDev), hipMemcpyHostToDevice, st));
 	int *infoP = info, *infoQ = info + 2, *infoR = info + 4;
 	VsPlan *planP = plans, *planQ = plans + 1, *planR = plans + 2;
-	VsPre *preP = pres, *preQ = pres + 1, *preR = pres + 2;
+	VsPre *preP = pres, *preQ = pres + 2, *preR = pres + 4;
 	const VsRef refP{v.P, planP, slP, lds, infoP}, refQ{v.Q, planQ, slQ, lds, infoQ},
 	    refR{v.R, planR, slR, lds, infoR};
 	if (stamps) {
@@ -1480,18 +1485,19 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 	// kernel of each category opens its clock: SpMV (split of p, q = A p), DOT (split of q or r and
 	// the dot), AXPY/SCAL.
 	for (int it = 0; it < maxiter; it++) {
-		OZ_HIP(launch_vsplit_plan(st, n, v.P, slP, lds, rho, 1, cap, reproCount, b->dTau, infoP, vscr, preP,
-		                          planP, it > 0, S, it, kClkSpmv));
-		launch_csrmm_sum_dev(st, n, AS, rup(b->nnzA, 32), nA, colind, rowptr, refP, fsum, v.Q, Cs, ldc,
-		                     b->dTau, S, it, preQ);
-		OZ_HIP(launch_vsplit_plan(st, n, v.Q, slQ, lds, rho, 1, cap, reproCount, b->dTau, infoQ, vscr, preQ,
-		                          planQ, true, S, it, kClkDot));
-		launch_dots_dev(st, n, refP, refQ, vals, part, b->dTau, fsum, S, it, 1, nullptr);
-		launch_axpy2_dev(st, n, S, it, v.P, x, v.Q, v.R, preR);
-		OZ_HIP(launch_vsplit_plan(st, n, v.R, slR, lds, rho, 1, cap, reproCount, b->dTau, infoR, vscr, preR,
-		                          planR, true, S, it, kClkDot));
-		launch_dots_dev(st, n, refR, refR, vals, part, b->dTau, fsum, S, it, 2, nullptr);
-		launch_scal_axpy_dev(st, n, S, it, v.R, v.P, preP);
+		const int par = it & 1, parNext = (it + 1) & 1;
+		OZ_HIP(launch_vsplit_plan(st, n, v.P, slP, lds, rho, 1, cap, reproCount, Js, infoP, vscr, preP, planP,
+		                          it > 0, S, it, kClkSpmv, par));
+		launch_csrmm_sum_dev(st, n, AS, rup(b->nnzA, 32), nA, colind, rowptr, refP, fsum, v.Q, Cs, ldc, Jn, S, it,
+		                     preQ + par);
+		OZ_HIP(launch_vsplit_plan(st, n, v.Q, slQ, lds, rho, 1, cap, reproCount, Js, infoQ, vscr, preQ, planQ,
+		                          true, S, it, kClkDot, par));
+		launch_dots_dev(st, n, refP, refQ, vals, part, Jn, fsum, S, it, 1, nullptr, f32 ? 1 : 0);
+		launch_axpy2_dev(st, n, S, it, v.P, x, v.Q, v.R, preR + par);
+		OZ_HIP(launch_vsplit_plan(st, n, v.R, slR, lds, rho, 1, cap, reproCount, Js, infoR, vscr, preR, planR,
+		                          true, S, it, kClkDot, par));
+		launch_dots_dev(st, n, refR, refR, vals, part, Jn, fsum, S, it, 2, nullptr, f32 ? 1 : 0);
+		launch_scal_axpy_dev(st, n, S, it, v.R, v.P, preP + parNext);
 		if (stamps && ((it + 1) % oh->verbose == 0 || it + 1 == maxiter || it % 16 == 15)) {
 			hipEvent_t e;
 			OZ_HIP(hipEventCreate(&e));
@@ -1518,8 +1524,12 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 		oh->t_AXPY_SCAL_total += tick * (double)h.clkTot[kClkAxpy];
 	}
 	oh->cg_numiter = (h.stopAt < maxiter) ? h.stopAt : maxiter;
-	if (oh->verbose > 0 && hist && h.vcnt > 1)
-		OZ_HIP(hipMemcpy(hist + 1, dhist + 1, 8ull * (h.vcnt - 1), hipMemcpyDeviceToHost));
+	if (oh->verbose > 0 && hist && h.vcnt > 1) {
+		// the device history holds T values as binary64
+		std::vector<double> hh(h.vcnt - 1);
+		OZ_HIP(hipMemcpy(hh.data(), dhist + 1, 8ull * (h.vcnt - 1), hipMemcpyDeviceToHost));
+		for (int j = 1; j < h.vcnt; j++) hist[j] = (T)hh[j - 1];
+	}
 	if (stamps) {
 		// entry j of the history is the j-th iteration (1-based count) with count % verbose == 0,
 		// or the last one when the tolerance stopped the solve; an iteration past the stop did no
@@ -1540,7 +1550,7 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 	return OZBLAS_STATUS_SUCCESS;
 }
 
-// binary64 iterates device-resident where it can; binary32 iterates on the synchronous path
+// binary64 and binary32 iterate device-resident where they can (cg_device_loop), else synchronously
 template <class T>
 static int32_t cg_entry(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz, const T *A,
                         const int32_t *colind, const int32_t *rowptr, const T *b_, T *x,
@@ -1620,22 +1630,20 @@ static int32_t cg_entry(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz,
 	}
 	{
 		// the true residual of a recorded iteration needs its x: that history runs synchronously
-		bool devicePath = E == 8 && oh->cgDeviceSlices >= 0 && !repro_slot(oh) &&
+		bool devicePath = oh->cgDeviceSlices >= 0 && !repro_slot(oh) &&
 		                  oh->fastMode == 0 && oh->splitEpsMode != 2 && oh->nSplitA_ <= 4 && maxiter > 0 &&
 		                  !diag.trueres;
-		if constexpr (E == 8) {
-			if (devicePath) {
-				wreset(oh, b->keep);
-				s = cg_device_loop(oh, n, oh->nSplitA_, AS, colind, rowptr, x, maxiter, tol, nrmb, dold, v,
-				                   hist, &overflow, diag);
-				wreset(oh, b->keep);
-				if (s == OZBLAS_STATUS_WORKSPACE) {  // nothing ran: iterate on the synchronous path
-					s = OZBLAS_STATUS_SUCCESS;
-					overflow = 1;
-					devicePath = false;
-				}
-				if (s) goto done;
+		if (devicePath) {
+			wreset(oh, b->keep);
+			s = cg_device_loop<T>(oh, n, oh->nSplitA_, AS, colind, rowptr, x, maxiter, tol, nrmb, dold, v, hist,
+			                      &overflow, diag);
+			wreset(oh, b->keep);
+			if (s == OZBLAS_STATUS_WORKSPACE) {  // nothing ran: iterate on the synchronous path
+				s = OZBLAS_STATUS_SUCCESS;
+				overflow = 1;
+				devicePath = false;
 			}
+			if (s) goto done;
 		}
 		oh->cgDevicePath = !devicePath ? 0 : (overflow ? 2 : 1);
 		if (overflow) {  // synchronous path (from the saved set-up state when the device path gave up)

@@ -220,19 +220,21 @@ hipError_t launch_vsplit(hipStream_t st, int n, const float *x, double *slices, 
                          int rho, int shift, int cap, int reproMode, const uint16_t *JF, int *info,
                          void *scratch, VsPlan *plan);
 // The plan only: consumers that take (x, plan) generate the slices themselves; the slices are
-// stored only for a plan marked bad (the reference loop in one workgroup).  pre: the vector's
-// VsPre (zero, or holding the producer's reductions when statsDone).  In a device CG, S gives the
-// iteration bound (its kernels of iteration it >= S->stopAt do nothing) and the category clock
-// that the split's first kernel opens (clkCat).
+// stored only for a plan marked bad (the reference loop in one workgroup).  pre: a pair of VsPre
+// for the vector (zero at first); this split reads pre[parity] (holding the producer's reductions
+// when statsDone) and clears pre[parity ^ 1] for the split after it, so consecutive splits of one
+// vector alternate the parity.  In a device CG, S gives the iteration bound (its kernels of
+// iteration it >= S->stopAt do nothing) and the category clock that the split's first kernel opens
+// (clkCat).
 struct CgDev;
 hipError_t launch_vsplit_plan(hipStream_t st, int n, const double *x, double *slices, int64_t lds,
                               int rho, int shift, int cap, int reproMode, const uint16_t *J, int *info,
                               void *scratch, VsPre *pre, VsPlan *plan, bool statsDone,
-                              CgDev *S = nullptr, int it = 0, int clkCat = -1);
+                              CgDev *S = nullptr, int it = 0, int clkCat = -1, int parity = 0);
 hipError_t launch_vsplit_plan(hipStream_t st, int n, const float *x, double *slices, int64_t lds,
                               int rho, int shift, int cap, int reproMode, const uint16_t *JF, int *info,
                               void *scratch, VsPre *pre, VsPlan *plan, bool statsDone,
-                              CgDev *S = nullptr, int it = 0, int clkCat = -1);
+                              CgDev *S = nullptr, int it = 0, int clkCat = -1, int parity = 0);
 void launch_axpy_host(hipStream_t st, int n, double alpha, const double *x, double *y, int fused);
 void launch_axpy_host(hipStream_t st, int n, float alpha, const float *x, float *y, int fused);
 // t = b - A x by the reference's plain CSRMV (the CG's true residual)
@@ -273,12 +275,16 @@ struct VsRef {
 	int64_t lds;
 	const int *info;  // info[0]: the slice count
 };
-// q = A p on the device path: a register kernel for p with <= 8 slices and a general one (pair
-// sums through Cs, m x nA x maxVecSlices doubles at pitch ldc) for more; the reductions of the
-// predicted split chain of q accumulate into preY
+// q = A p on the device path: registers for p with <= 8 slices, a per-pair row loop for more (pair
+// sums through Cs, m x nA x maxVecSlices doubles at pitch ldc); the reductions of the predicted
+// split chain of q accumulate into preY
 void launch_csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
                           const int *colind, const int *rowptr, VsRef X, int fsum, double *y,
                           double *Cs, int64_t ldc, const uint16_t *J, CgDev *S, int it, VsPre *preY);
+// binary32 p and q (JF2: the binary32 NearSum table); p's slices capped at 8 by maxVecSlices
+void launch_csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
+                          const int *colind, const int *rowptr, VsRef X, int fsum, float *y,
+                          double *Cs, int64_t ldc, const uint16_t *JF2, CgDev *S, int it, VsPre *preY);
 // part: dots_part_bytes() of scratch for the per-workgroup pair partials (no initial value)
 size_t dots_part_bytes();
 // DOT of two split vectors (binary32 inputs with f32), completed by `mode` (0: C[0] = the DOT)
@@ -289,5 +295,8 @@ void launch_axpy2_dev(hipStream_t st, int n, CgDev *S, int it, const double *P, 
                       const double *Q, double *R, VsPre *preR);
 void launch_scal_axpy_dev(hipStream_t st, int n, CgDev *S, int it, const double *R, double *P,
                           VsPre *preP);
+void launch_axpy2_dev(hipStream_t st, int n, CgDev *S, int it, const float *P, float *X, const float *Q,
+                      float *R, VsPre *preR);
+void launch_scal_axpy_dev(hipStream_t st, int n, CgDev *S, int it, const float *R, float *P, VsPre *preP);
 
 }  // namespace oz

@@ -135,15 +135,42 @@ __device__ void vs_walk(int nwg, unsigned long long *Fg, const int2 *range, int 
                         int64_t lds, int rhoIn, int shift, int fallback);
 
 // the table scan; its last workgroup then walks the levels (vs_walk)
+template <class TI>
+__device__ int vs_verify(unsigned long long mb, int lof, int bad, const unsigned long long *Fb, int rho,
+                         int cap, int reproMode, const uint16_t *J, VsPlan *plan, int *info, bool write);
+template <bool ATOMIC>
+__device__ inline void vs_pre_sum(const VsPre *pre, unsigned long long &maxBits, int &loOff, int &bad);
+
+// Every workgroup first reduces the predicted chain's slots and walks it (the same result in each;
+// workgroup 0 writes the plan).  When the chain holds, that is all: no table.  Otherwise the table
+// scan, and its last workgroup walks the levels (vs_walk).
 template <int BS, class TI>
 __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned long long *Fg,
                                                     int2 *range, unsigned int *done, const int *stopAt,
-                                                    int it, const int *gate, int rho, int cap,
+                                                    int it, const VsPre *pre, int rho, int cap,
                                                     int reproMode, const uint16_t *J, VsPlan *plan,
                                                     int *info, double *slices, int64_t lds, int rhoIn,
                                                     int shift, int fallback) {
 	if (stopAt && it >= *stopAt) return;
-	if (gate && *gate == 0) return;  // the predicted chain held: no table needed
+	{
+		__shared__ unsigned long long Fp[kVsPred], mbS;
+		__shared__ int lofS, badS, needS;
+		const int t = threadIdx.x;
+		if (t < kVsPred) {
+			unsigned long long f = 0ull;
+			for (int q = 0; q < kVsSpread; q++) f = std::max(f, pre->slot[q].F[t]);
+			Fp[t] = f;
+		} else if (t == 64) {
+			vs_pre_sum<false>(pre, mbS, lofS, badS);
+		}
+		__syncthreads();
+		if (t == 0) {
+			needS = vs_verify<TI>(mbS, lofS, badS, Fp, rho, cap, reproMode, J, plan, info, blockIdx.x == 0);
+			if (blockIdx.x == 0 && needS) plan->needScan = 1;
+		}
+		__syncthreads();
+		if (!needS) return;  // the predicted chain held: no table needed
+	}
 	__shared__ unsigned long long Fb[kVsK];
 	__shared__ int kminB, kmaxB, badB;
 	__shared__ int red[2][BS / 64];
@@ -516,15 +543,13 @@ __global__ __launch_bounds__(BS) void k_vsplit_pre(int n, const TI *x, VsPre *pr
 // The walk over the predicted F values (the steps of vs_walk), by one thread of k_vsplit_verify.  Anything the prediction does not settle (an input outside the
 // model, a zero vector, a level off the chain) goes to the table path.
 template <class TI>
-__device__ void vs_verify(unsigned long long mb, int lof, int bad, const unsigned long long *Fb, int rho,
-                          int cap, int reproMode, const uint16_t *J, VsPlan *plan, int *info) {
+__device__ int vs_verify(unsigned long long mb, int lof, int bad, const unsigned long long *Fb, int rho,
+                         int cap, int reproMode, const uint16_t *J, VsPlan *plan, int *info, bool write) {
 	const double M0 = __builtin_bit_cast(double, (uint64_t)mb);
-	if (bad || M0 == 0.0 || rho < 3 || rho > SplitRule<TI>::kPrec || cap > kVsMaxLevels) {
-		plan->needScan = 1;
-		return;
-	}
+	if (bad || M0 == 0.0 || rho < 3 || rho > SplitRule<TI>::kPrec || cap > kVsMaxLevels) return 1;
 	int k0, d, L;
 	vs_predict<TI>(M0, kVsLoOff - lof, rho, cap, J, k0, d, L);
+	int ks[kVsPred];
 	double M = M0;
 	int s = 0;
 	bool zero = false;
@@ -532,17 +557,16 @@ __device__ void vs_verify(unsigned long long mb, int lof, int bad, const unsigne
 		const int tau = SplitRule<TI>::tau(M, J);
 		const int k = rho + tau - SplitRule<TI>::kPrec;
 		if (rho + tau > SplitRule<TI>::kSigEmax || rho + tau - 1 < SplitRule<TI>::kEmin || s >= L ||
-		    k != k0 - d * s) {  // outside the model, or off the predicted chain
-			plan->needScan = 1;
-			return;
-		}
-		plan->k[s] = k;
+		    k != k0 - d * s)  // outside the model, or off the predicted chain
+			return 1;
+		ks[s] = k;
 		M = __builtin_bit_cast(double, (uint64_t)Fb[s]);
 		if (M == 0.0) {
 			zero = true;
 			break;
 		}
 	}
+	if (!write) return 0;
 	int count;
 	if (cap <= 1)
 		count = 1;
@@ -551,20 +575,30 @@ __device__ void vs_verify(unsigned long long mb, int lof, int bad, const unsigne
 	else
 		count = cap;
 	if (reproMode == 0) count++;
+	const int last = zero ? s : cap - 1;
+	for (int q = 0; q <= last && q < kVsPred; q++) plan->k[q] = ks[q];
 	plan->needScan = 0;
 	plan->bad = 0;
 	plan->count = count;
-	plan->last = zero ? s : cap - 1;
+	plan->last = last;
 	plan->zero = zero ? 1 : 0;
 	info[0] = count;
 	info[1] = zero ? s : cap;
+	return 0;
 }
 
+constexpr int kVsSlotWords = (int)(sizeof(VsSlot) / 8) * kVsSpread;
+
+// pre: this split's reductions; preNext: the other buffer of the pair, cleared here by workgroup 0
+// for the split after this one (nothing reads or writes it meanwhile)
 template <int BS, class TI>
-__global__ __launch_bounds__(BS) void k_vsplit_levels(int n, const TI *x, VsPre *pre, int rho, int cap,
-                                                      const uint16_t *J, CgDev *S, int it, int clkCat) {
+__global__ __launch_bounds__(BS) void k_vsplit_levels(int n, const TI *x, VsPre *pre, VsPre *preNext, int rho,
+                                                      int cap, const uint16_t *J, CgDev *S, int it,
+                                                      int clkCat) {
 	cg_clock(S, it, clkCat);
 	if (S && it >= S->stopAt) return;
+	if (blockIdx.x == 0)
+		for (int q = threadIdx.x; q < kVsSlotWords; q += BS) ((unsigned long long *)preNext->slot)[q] = 0ull;
 	__shared__ double red[BS / 64];
 	unsigned long long mb;
 	int lof, pbad;
@@ -597,30 +631,6 @@ __global__ __launch_bounds__(BS) void k_vsplit_levels(int n, const TI *x, VsPre 
 			}
 		}
 	}
-}
-
-// After k_vsplit_levels, one workgroup: reduces the slots, clears them for the next split and
-// verifies the chain (one thread).  A kernel of its own: a last-workgroup step inside the level
-// kernel would cost every workgroup a device-scope release.
-template <int BS, class TI>
-__global__ __launch_bounds__(BS) void k_vsplit_verify(VsPre *pre, int rho, int cap, int reproMode,
-                                                      const uint16_t *J, VsPlan *plan, int *info,
-                                                      const int *stopAt, int it) {
-	if (stopAt && it >= *stopAt) return;
-	__shared__ unsigned long long Fb[kVsPred], mbS;
-	__shared__ int lofS, badS;
-	const int t = threadIdx.x;
-	if (t < kVsPred) {
-		unsigned long long f = 0ull;
-		for (int q = 0; q < kVsSpread; q++) f = std::max(f, pre->slot[q].F[t]);
-		Fb[t] = f;
-	} else if (t == kVsPred) {
-		vs_pre_sum<false>(pre, mbS, lofS, badS);
-	}
-	__syncthreads();
-	constexpr int kWords = (int)(sizeof(VsSlot) / 8) * kVsSpread;
-	for (int q = t; q < kWords; q += BS) ((unsigned long long *)pre->slot)[q] = 0ull;
-	if (t == 0) vs_verify<TI>(mbS, lofS, badS, Fb, rho, cap, reproMode, J, plan, info);
 }
 
 // Slices of one element from its plan: the reference's loop (src/cpu/ozblas_split.cpp:75-104)
@@ -753,7 +763,7 @@ __global__ __launch_bounds__(256) void k_vsplit_write(int n, const TI *x, double
 	}
 }
 
-size_t vsplit_scratch_bytes() { return 8ull * kVsK + 8ull * kVsBlocks + 64 + sizeof(VsPre) + 64; }
+size_t vsplit_scratch_bytes() { return 8ull * kVsK + 8ull * kVsBlocks + 64 + 2 * sizeof(VsPre) + 64; }
 
 // The plan of x (length n): the predicted chain, then the table scan and walk only when it does not
 // hold; with fallback the walk's workgroup also stores the slices of a plan marked bad (the
@@ -762,10 +772,11 @@ template <class TI>
 static hipError_t vsplit_plan(hipStream_t st, int n, const TI *x, double *slices, int64_t lds, int rho,
                               int shift, int cap, int reproMode, const uint16_t *J, int *info,
                               void *scratch, VsPre *pre, VsPlan *plan, bool statsDone, CgDev *S, int it,
-                              int clkCat, int fallback) {
+                              int clkCat, int fallback, int parity) {
 	unsigned long long *Fg = (unsigned long long *)scratch;
 	int2 *range = (int2 *)(Fg + kVsK);
 	if (pre == nullptr) pre = (VsPre *)((char *)scratch + 8ull * kVsK + 8ull * kVsBlocks + 64);
+	VsPre *cur = pre + (parity & 1), *next = pre + ((parity & 1) ^ 1);
 	const int *stopAt = S ? &S->stopAt : nullptr;
 	int rhoEff = rho;
 	if (shift > 1) {  // sigma * 2^j acts as rho + j
@@ -777,15 +788,13 @@ static hipError_t vsplit_plan(hipStream_t st, int n, const TI *x, double *slices
 	int gridL = (int)std::min<int64_t>(kVsLevBlocks, ((int64_t)n + 4 * kVsBS - 1) / (4 * kVsBS));
 	if (gridL < 1) gridL = 1;
 	if (!statsDone)
-		hipLaunchKernelGGL((k_vsplit_pre<kVsBS, TI>), dim3(gridL), dim3(kVsBS), 0, st, n, x, pre, stopAt, it);
-	hipLaunchKernelGGL((k_vsplit_levels<kVsBS, TI>), dim3(gridL), dim3(kVsBS), 0, st, n, x, pre, rhoEff, cap,
-	                   J, S, it, clkCat);
-	hipLaunchKernelGGL((k_vsplit_verify<256, TI>), dim3(1), dim3(256), 0, st, pre, rhoEff, cap, reproMode, J,
-	                   plan, info, stopAt, it);
+		hipLaunchKernelGGL((k_vsplit_pre<kVsBS, TI>), dim3(gridL), dim3(kVsBS), 0, st, n, x, cur, stopAt, it);
+	hipLaunchKernelGGL((k_vsplit_levels<kVsBS, TI>), dim3(gridL), dim3(kVsBS), 0, st, n, x, cur, next, rhoEff,
+	                   cap, J, S, it, clkCat);
 	unsigned int *done = (unsigned int *)(range + kVsBlocks);
 	hipLaunchKernelGGL((k_vsplit_scan<kVsBS, TI>), dim3(grid), dim3(kVsBS), 0, st, n, x, Fg, range, done,
-	                   stopAt, it, (const int *)&plan->needScan, rhoEff, cap, reproMode, J, plan, info,
-	                   slices, lds, rho, shift, fallback);
+	                   stopAt, it, (const VsPre *)cur, rhoEff, cap, reproMode, J, plan, info, slices, lds, rho,
+	                   shift, fallback);
 	return hipGetLastError();
 }
 
@@ -795,7 +804,7 @@ static hipError_t vsplit(hipStream_t st, int n, const TI *x, double *slices, int
                          int shift, int cap, int reproMode, const uint16_t *J, int *info, void *scratch,
                          VsPlan *plan) {
 	const hipError_t e = vsplit_plan(st, n, x, slices, lds, rho, shift, cap, reproMode, J, info, scratch,
-	                                 nullptr, plan, false, nullptr, 0, kClkNone, 0);
+	                                 nullptr, plan, false, nullptr, 0, kClkNone, 0, 0);
 	if (e != hipSuccess) return e;
 	int gw = (int)std::min<int64_t>(2048, ((int64_t)n + 255) / 256);
 	if (gw < 1) gw = 1;
@@ -819,17 +828,17 @@ hipError_t launch_vsplit(hipStream_t st, int n, const float *x, double *slices, 
 hipError_t launch_vsplit_plan(hipStream_t st, int n, const double *x, double *slices, int64_t lds,
                               int rho, int shift, int cap, int reproMode, const uint16_t *J, int *info,
                               void *scratch, VsPre *pre, VsPlan *plan, bool statsDone, CgDev *S, int it,
-                              int clkCat) {
+                              int clkCat, int parity) {
 	return vsplit_plan(st, n, x, slices, lds, rho, shift, cap, reproMode, J, info, scratch, pre, plan,
-	                   statsDone, S, it, clkCat, 1);
+	                   statsDone, S, it, clkCat, 1, parity);
 }
 
 hipError_t launch_vsplit_plan(hipStream_t st, int n, const float *x, double *slices, int64_t lds,
                               int rho, int shift, int cap, int reproMode, const uint16_t *JF, int *info,
                               void *scratch, VsPre *pre, VsPlan *plan, bool statsDone, CgDev *S, int it,
-                              int clkCat) {
+                              int clkCat, int parity) {
 	return vsplit_plan(st, n, x, slices, lds, rho, shift, cap, reproMode, JF, info, scratch, pre, plan,
-	                   statsDone, S, it, clkCat, 1);
+	                   statsDone, S, it, clkCat, 1, parity);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -840,7 +849,6 @@ hipError_t launch_vsplit_plan(hipStream_t st, int n, const float *x, double *sli
 constexpr int kDotChunk = 16;
 constexpr int kDotPairs = 64;    // pair slots per workgroup partial of k_dots_reg
 constexpr int kDotMaxWG = 512;  // workgroups of k_dots_reg at most (two waves per SIMD)
-constexpr int kDotRegSum = 36;  // pair dots the completion sums in registers (6 x 6 slices)
 
 template <int BS>
 __global__ __launch_bounds__(BS) void k_slice_dots(int n, const double *xs, int64_t ldx, int nA,
@@ -1709,15 +1717,55 @@ void launch_scal_axpy(hipStream_t st, int n, float beta, const float *r, float *
 // ---------------------------------------------------------------------------------------------
 __device__ inline bool cg_skip(const CgDev *S, int it) { return S != nullptr && it >= S->stopAt; }
 
+// the row sum of the product when p has more slices than the register kernel holds: each pair
+// sum runs over the row's nonzeros in order (exact) into Cs[q * ldc + r], then NearSum over the
+// row's pairs, or Fsum in (level, ia) order computed on the fly
+__device__ inline double csrmm_big_row(int r, const double *As, int64_t ldas, int nA, const int *colind,
+                                       const int *rowptr, const VsRef &X, int nB, int fsum, double *Cs,
+                                       int64_t ldc, const uint16_t *J) {
+	const bool stored = X.plan->bad;
+	const double *Xv = (const double *)X.x;
+	const int b = rowptr[r], e = rowptr[r + 1];
+	auto pair = [&](int ia, int ib) {
+		double acc = 0.0;
+		for (int j = b; j < e; j++) {
+			const double xv = stored ? X.slices[(int64_t)ib * X.lds + colind[j]]
+			                         : vs_slot<double>(Xv[colind[j]], X.plan, ib);
+			acc = fma(As[(int64_t)ia * ldas + j], xv, acc);
+		}
+		return acc;
+	};
+	const int maxlevel = (nA - 1) + (nB - 1);
+	double t;
+	if (fsum) {
+		t = 0.;
+		for (int ik = 0; ik <= maxlevel; ik++)
+			for (int ia = 0; ia < nA; ia++) {
+				const int ib = ik - ia;
+				if (ib >= 0 && ib < nB) t = t + pair(ia, ib);
+			}
+	} else {
+		int cnt = 0;
+		for (int ia = 0; ia < nA; ia++)
+			for (int ib = 0; ib < nB; ib++) Cs[(int64_t)(cnt++) * ldc + r] = pair(ia, ib);
+		StridedVec v{Cs + r, ldc};
+		t = nearsum(cnt, v, J);
+	}
+	return t;
+}
+
 // q = A p with the pre-split matrix (nA <= NAM) and the device slice count of p; alpha = 1, beta = 0
 // as ozblasRcsrmv(..., 1., ..., 0., q) at src/cpu/ozblas_cg.cpp:82.  p's slices come from p and its
-// plan (or its stored slices when the plan is marked bad); the reductions of q's split accumulate
-// into preY.
-template <int NAM, int NBM>
+// plan (or its stored slices when the plan is marked bad); more than NBM of them take the
+// per-pair row loop (csrmm_big_row, pair sums through Cs: m x nA x maxVecSlices doubles at pitch
+// ldc).  The reductions of q's split accumulate into preY.
+// T = float: the <float,float> CG -- p, q binary32, the pair sums and NearSum in float (J = JF2); the
+// host caps p's slices at NBM for it (more overflow to the synchronous path).
+template <int NAM, int NBM, class T>
 __global__ __launch_bounds__(256) void k_csrmm_sum_dev(int m, const double *As, int64_t ldas, int nA,
                                                        const int *colind, const int *rowptr, VsRef X,
-                                                       int fsum, double *y, const uint16_t *J,
-                                                       CgDev *S, int it, VsPre *preY) {
+                                                       int fsum, T *y, double *Cs, int64_t ldc,
+                                                       const uint16_t *J, CgDev *S, int it, VsPre *preY) {
 	if (cg_skip(S, it)) return;
 	const int nB = X.info[0];
 	if (nB > S->maxVecSlices) {
@@ -1727,74 +1775,25 @@ __global__ __launch_bounds__(256) void k_csrmm_sum_dev(int m, const double *As, 
 		}
 		return;
 	}
-	if (nB > NBM) return;  // k_csrmm_big_dev computes this product
 	const int r = blockIdx.x * blockDim.x + threadIdx.x;
 	double mx = 0.0;
 	int lo = 0, bad = 0;
 	if (r < m) {
 		const int maxlevel = (nA - 1) + (nB - 1);
-		double t;
-		if (X.plan->bad) {
-			t = csrmm_row_sum<NAM, NBM>(r, As, ldas, nA, colind, rowptr, X.slices, X.lds, nB, maxlevel, fsum, J);
+		T t = 0;
+		if (nB > NBM) {
+			if constexpr (sizeof(T) == 8) t = csrmm_big_row(r, As, ldas, nA, colind, rowptr, X, nB, fsum, Cs, ldc, J);
+		} else if (X.plan->bad) {
+			t = csrmm_row_sum<NAM, NBM, T>(r, As, ldas, nA, colind, rowptr, X.slices, X.lds, nB, maxlevel, fsum, J);
 		} else {
-			VsGen<double, NBM> g;
+			VsGen<T, NBM> g;
 			g.init(X.plan);
-			t = csrmm_row_sum<NAM, NBM, double, double, true>(r, As, ldas, nA, colind, rowptr, nullptr, 0, nB,
-			                                                  maxlevel, fsum, J, (const double *)X.x, &g);
+			t = csrmm_row_sum<NAM, NBM, T, T, true>(r, As, ldas, nA, colind, rowptr, nullptr, 0, nB, maxlevel, fsum,
+			                                        J, (const T *)X.x, &g);
 		}
-		const double v = fma(1.0, t, 0.0 * y[r]);
+		const T v = fma1_t((T)1, t, (T)0 * y[r]);
 		y[r] = v;
-		vs_stats_acc(v, mx, lo, bad);
-	}
-	vs_stats_flush<256>(mx, lo, bad, preY);
-}
-
-// The same product when p has more slices than the register kernel holds: each pair sum runs over
-// the row's nonzeros in order (exact) into Cs[q * ldc + r], then NearSum over the row's pairs, or
-// Fsum in (level, ia) order computed on the fly.
-__global__ __launch_bounds__(256) void k_csrmm_big_dev(int m, const double *As, int64_t ldas, int nA,
-                                                       const int *colind, const int *rowptr, VsRef X,
-                                                       int nbReg, int fsum, double *y, double *Cs,
-                                                       int64_t ldc, const uint16_t *J, const CgDev *S,
-                                                       int it, VsPre *preY) {
-	if (cg_skip(S, it)) return;
-	const int nB = X.info[0];
-	if (nB <= nbReg || nB > S->maxVecSlices) return;
-	const int r = blockIdx.x * blockDim.x + threadIdx.x;
-	const bool stored = X.plan->bad;
-	const double *Xv = (const double *)X.x;
-	double mx = 0.0;
-	int lo = 0, bad = 0;
-	if (r < m) {
-		const int b = rowptr[r], e = rowptr[r + 1];
-		auto pair = [&](int ia, int ib) {
-			double acc = 0.0;
-			for (int j = b; j < e; j++) {
-				const double xv = stored ? X.slices[(int64_t)ib * X.lds + colind[j]]
-				                         : vs_slot<double>(Xv[colind[j]], X.plan, ib);
-				acc = fma(As[(int64_t)ia * ldas + j], xv, acc);
-			}
-			return acc;
-		};
-		const int maxlevel = (nA - 1) + (nB - 1);
-		double t;
-		if (fsum) {
-			t = 0.;
-			for (int ik = 0; ik <= maxlevel; ik++)
-				for (int ia = 0; ia < nA; ia++) {
-					const int ib = ik - ia;
-					if (ib >= 0 && ib < nB) t = t + pair(ia, ib);
-				}
-		} else {
-			int cnt = 0;
-			for (int ia = 0; ia < nA; ia++)
-				for (int ib = 0; ib < nB; ib++) Cs[(int64_t)(cnt++) * ldc + r] = pair(ia, ib);
-			StridedVec v{Cs + r, ldc};
-			t = nearsum(cnt, v, J);
-		}
-		const double v = fma(1.0, t, 0.0 * y[r]);
-		y[r] = v;
-		vs_stats_acc(v, mx, lo, bad);
+		vs_stats_acc((double)v, mx, lo, bad);
 	}
 	vs_stats_flush<256>(mx, lo, bad, preY);
 }
@@ -1806,8 +1805,8 @@ __global__ __launch_bounds__(256) void k_csrmm_big_dev(int m, const double *As, 
 //   2: dnew = t, resi = sqrt(dnew), beta = dnew / dold,  (src/cpu/ozblas_cg.cpp:91-94,100-112)
 //      dold = dnew, history and the residual test
 // then clears vals for the next use.
-// With f32 (binary32 DOT; mode 0 only) the pair dots round to float first, the sum is the float
-// one (J is then JF2) and C is a float: C[0] = fma1<float>(1, t, 0 * C[0]).
+// With f32 (binary32 DOT) the pair dots round to float first, the sum is the float one (J is then
+// JF2), C is a float (C[0] = fma1<float>(1, t, 0 * C[0])) and modes 1 and 2 run in binary32.
 // The pair dots are the per-workgroup partials part[p * kDotMaxWG + w] of k_dots_reg's nwg
 // workgroups, summed here by the whole workgroup (exact in any order), or with part == nullptr
 // the sums dots_general accumulated in vals.
@@ -1858,27 +1857,54 @@ __device__ void dots_finish(int nA, int nB, double *vals, const double *part, in
 		}
 		src = psum;
 	}
-	if (threadIdx.x != 0) return;
 	const int maxlevel = (nA - 1) + (nB - 1);  // DOT forces fastMode = 0 (src/cpu/ozblas_gemm.cpp:25-28)
+	double tw = 0.0;
+	bool waveDone = false;
+	if (!f32 && !fsum && npairs <= 64) {
+		// NearSum across the lanes of wave 0 when every pair dot is finite
+		if (threadIdx.x >= 64) return;
+		const double pv = ((int)threadIdx.x < npairs) ? src[threadIdx.x] : 0.0;
+		const bool finite = __all(!(fabs(pv) > 1.7976931348623157e308) && pv == pv);
+		if (finite) {
+			tw = nearsum_wave(npairs, pv, J);
+			waveDone = true;
+		}
+	}
+	if (threadIdx.x != 0) return;
 	if (f32) {
 		float *fv = vals_to_float(npairs, src);
 		const float t = dot_pairs_sum_f(npairs, fv, nA, nB, maxlevel, fsum, J);
-		float *Cf = (float *)C;
-		Cf[0] = fma1_f(1.f, t, 0.f * Cf[0]);
+		if (mode == 0) {
+			float *Cf = (float *)C;
+			Cf[0] = fma1_f(1.f, t, 0.f * Cf[0]);
+		} else {
+			// the <float,float> CG's recurrences in binary32 (the CgDev fields hold binary32 values)
+			const float r = fma1_f(1.f, t, 0.f * 0.f);
+			if (mode == 1) {
+				S->tmp = r;
+				S->alpha = (float)S->dold / r;
+			} else {
+				const float dnew = r;
+				const float resi = (float)sqrt((double)dnew);
+				const float nrmb = (float)S->nrmb, tol = (float)S->tol;
+				S->dnew = dnew;
+				S->resi = resi;
+				S->beta = dnew / (float)S->dold;
+				S->dold = dnew;
+				S->numiter = it + 1;
+				if (S->verbose > 0) {
+					if (resi / nrmb < tol || S->numiter % S->verbose == 0) {
+						if (S->hist) S->hist[S->vcnt] = resi / nrmb;
+						S->vcnt = S->vcnt + 1;
+					}
+				}
+				if (resi / nrmb < tol) S->stopAt = it + 1;
+			}
+		}
 		for (int q = 0; q < npairs; q++) vals[q] = 0.0;
 		return;
 	}
-	double t;
-	if (!fsum && npairs <= kDotRegSum) {
-		// NearSum with the pair dots in registers (its result does not depend on the order of the
-		// terms): no memory round trip per step of its passes
-		RegVec<kDotRegSum> u;
-#pragma unroll
-		for (int q = 0; q < kDotRegSum; q++) u[q] = (q < npairs) ? src[q] : 0.0;
-		t = nearsum(npairs, u, J);
-	} else {
-		t = dot_pairs_sum(npairs, src, nA, nB, maxlevel, fsum, J);
-	}
+	const double t = waveDone ? tw : dot_pairs_sum(npairs, src, nA, nB, maxlevel, fsum, J);
 	const double r = fma(1.0, t, 0.0 * 0.0);
 	if (mode == 0) {
 		double *Cd = (double *)C;
@@ -2072,52 +2098,65 @@ __global__ __launch_bounds__(BS) void k_dots_fin(int nwg, const int *infoX, cons
 
 // x = fma(alpha, p, x); r = fma(-alpha, q, r)   (src/cpu/ozblas_cg.cpp:87-88, ozblas_axpy.cpp),
 // the first kernel of the AXPY category; r's split reductions into preR
-__global__ __launch_bounds__(256) void k_axpy2_dev(int n, CgDev *S, int it, const double *P, double *X,
-                                                   const double *Q, double *R, VsPre *preR) {
+// (T = float: fma1<float> with the binary32 alpha)
+template <class T>
+__global__ __launch_bounds__(256) void k_axpy2_dev(int n, CgDev *S, int it, const T *P, T *X, const T *Q, T *R,
+                                                   VsPre *preR) {
 	cg_clock(S, it, kClkAxpy);
 	if (cg_skip(S, it)) return;
-	const double a = S->alpha;
+	const T a = (T)S->alpha;
 	double mx = 0.0;
 	int lo = 0, bad = 0;
 	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
 	     i += (int64_t)gridDim.x * blockDim.x) {
-		X[i] = fma(a, P[i], X[i]);
-		const double r = fma(-a, Q[i], R[i]);
+		X[i] = fma1_t(a, P[i], X[i]);
+		const T r = fma1_t(-a, Q[i], R[i]);
 		R[i] = r;
-		vs_stats_acc(r, mx, lo, bad);
+		vs_stats_acc((double)r, mx, lo, bad);
 	}
 	vs_stats_flush<256>(mx, lo, bad, preR);
 }
 
 // p = beta * p, then p = fma(1, r, p)   (src/cpu/ozblas_cg.cpp:97-98); p's split reductions into preP
-__global__ __launch_bounds__(256) void k_scal_axpy_dev(int n, CgDev *S, int it, const double *R, double *P,
-                                                       VsPre *preP) {
+template <class T>
+__global__ __launch_bounds__(256) void k_scal_axpy_dev(int n, CgDev *S, int it, const T *R, T *P, VsPre *preP) {
 	cg_clock(S, it, kClkAxpy);
 	if (cg_skip(S, it)) return;
-	const double b = S->beta;
+	const T b = (T)S->beta;
 	double mx = 0.0;
 	int lo = 0, bad = 0;
 	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
 	     i += (int64_t)gridDim.x * blockDim.x) {
-		const double p = fma(1.0, R[i], b * P[i]);
+		const T p = fma1_t((T)1, R[i], b * P[i]);
 		P[i] = p;
-		vs_stats_acc(p, mx, lo, bad);
+		vs_stats_acc((double)p, mx, lo, bad);
 	}
 	vs_stats_flush<256>(mx, lo, bad, preP);
+}
+
+template <class T>
+static void csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas, int nA, const int *colind,
+                          const int *rowptr, VsRef X, int fsum, T *y, double *Cs, int64_t ldc, const uint16_t *J,
+                          CgDev *S, int it, VsPre *preY) {
+	constexpr int kNbReg = 8;
+	if (nA <= 2)  // the matrix's count is known here; p's count only on the device
+		hipLaunchKernelGGL((k_csrmm_sum_dev<2, kNbReg, T>), dim3((m + 255) / 256), dim3(256), 0, st, m, As,
+		                   ldas, nA, colind, rowptr, X, fsum, y, Cs, ldc, J, S, it, preY);
+	else
+		hipLaunchKernelGGL((k_csrmm_sum_dev<4, kNbReg, T>), dim3((m + 255) / 256), dim3(256), 0, st, m, As,
+		                   ldas, nA, colind, rowptr, X, fsum, y, Cs, ldc, J, S, it, preY);
 }
 
 void launch_csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
                           const int *colind, const int *rowptr, VsRef X, int fsum, double *y,
                           double *Cs, int64_t ldc, const uint16_t *J, CgDev *S, int it, VsPre *preY) {
-	constexpr int kNbReg = 8;
-	if (nA <= 2)  // the matrix's count is known here; p's count only on the device
-		hipLaunchKernelGGL((k_csrmm_sum_dev<2, kNbReg>), dim3((m + 255) / 256), dim3(256), 0, st, m, As,
-		                   ldas, nA, colind, rowptr, X, fsum, y, J, S, it, preY);
-	else
-		hipLaunchKernelGGL((k_csrmm_sum_dev<4, kNbReg>), dim3((m + 255) / 256), dim3(256), 0, st, m, As,
-		                   ldas, nA, colind, rowptr, X, fsum, y, J, S, it, preY);
-	hipLaunchKernelGGL(k_csrmm_big_dev, dim3((m + 255) / 256), dim3(256), 0, st, m, As, ldas, nA,
-	                   colind, rowptr, X, kNbReg, fsum, y, Cs, ldc, J, (const CgDev *)S, it, preY);
+	csrmm_sum_dev(st, m, As, ldas, nA, colind, rowptr, X, fsum, y, Cs, ldc, J, S, it, preY);
+}
+
+void launch_csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
+                          const int *colind, const int *rowptr, VsRef X, int fsum, float *y,
+                          double *Cs, int64_t ldc, const uint16_t *JF2, CgDev *S, int it, VsPre *preY) {
+	csrmm_sum_dev(st, m, As, ldas, nA, colind, rowptr, X, fsum, y, Cs, ldc, JF2, S, it, preY);
 }
 
 size_t dots_part_bytes() { return sizeof(double) * kDotPairs * kDotMaxWG; }
@@ -2147,12 +2186,21 @@ static int grid_red(int64_t n) {
 
 void launch_axpy2_dev(hipStream_t st, int n, CgDev *S, int it, const double *P, double *X,
                       const double *Q, double *R, VsPre *preR) {
-	hipLaunchKernelGGL(k_axpy2_dev, dim3(grid_red(n)), dim3(256), 0, st, n, S, it, P, X, Q, R, preR);
+	hipLaunchKernelGGL(k_axpy2_dev<double>, dim3(grid_red(n)), dim3(256), 0, st, n, S, it, P, X, Q, R, preR);
+}
+
+void launch_axpy2_dev(hipStream_t st, int n, CgDev *S, int it, const float *P, float *X, const float *Q,
+                      float *R, VsPre *preR) {
+	hipLaunchKernelGGL(k_axpy2_dev<float>, dim3(grid_red(n)), dim3(256), 0, st, n, S, it, P, X, Q, R, preR);
 }
 
 void launch_scal_axpy_dev(hipStream_t st, int n, CgDev *S, int it, const double *R, double *P,
                           VsPre *preP) {
-	hipLaunchKernelGGL(k_scal_axpy_dev, dim3(grid_red(n)), dim3(256), 0, st, n, S, it, R, P, preP);
+	hipLaunchKernelGGL(k_scal_axpy_dev<double>, dim3(grid_red(n)), dim3(256), 0, st, n, S, it, R, P, preP);
+}
+
+void launch_scal_axpy_dev(hipStream_t st, int n, CgDev *S, int it, const float *R, float *P, VsPre *preP) {
+	hipLaunchKernelGGL(k_scal_axpy_dev<float>, dim3(grid_red(n)), dim3(256), 0, st, n, S, it, R, P, preP);
 }
 
 }  // namespace oz

@@ -123,6 +123,88 @@ __device__ inline double nearsum(int n, V &vec, const uint16_t *J) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// The same NearSum over n <= 64 finite values held one per lane of a wave (lane i: vec[i], lanes
+// i >= n: 0), every lane computing the same result.  Each pass of the transform extracts
+// tmp_i = (sigma + vec[i]) - sigma with sigma >= 2^m max|vec|, 2^m >= n + 2: the tmp_i are
+// multiples of ulp(sigma) / 2 and every partial sum of them stays within sigma, so the
+// reference's running sum tau is exact and equals a butterfly sum over the lanes, as mu equals a
+// butterfly maximum (values finite).  The one inexact sum -- of the remainders after a transform --
+// runs in lane order (src/cpu/ozblas_sum.cpp:64-67), each lane's value broadcast in turn.
+// ---------------------------------------------------------------------------------------------
+__device__ inline double ns_wave_max(double v) {
+	for (int o = 32; o > 0; o >>= 1) v = mac_max(v, __shfl_xor(v, o, 64));
+	return v;
+}
+__device__ inline double ns_wave_sum(double v) {
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+	return v;
+}
+
+__device__ inline void ns_transform_wave(int n, double &v, double rho, double &tau1, double &tau2,
+                                         const uint16_t *J) {
+	if (n == 0) {
+		tau1 = rho;
+		tau2 = 0.;
+		return;
+	}
+	const double mu = ns_wave_max(fabs(v));
+	if (mu == 0.) {
+		tau1 = rho;
+		tau2 = 0.;
+		return;
+	}
+	const int m = ceil_log2_int(n + 2);
+	double sigma = scalbn_clamped(1., (int64_t)m + ceil_log2_tab(mu, J));  // mu finite
+	double t = rho;
+	const double tiny = 2.2250738585072014e-308;
+	const double fac = scalbn_clamped(1., 2 * m + 1 - 53);
+	const double shrink = scalbn_clamped(1., m - 53);
+	while (true) {
+		const double tmp = (sigma + v) - sigma;
+		v = v - tmp;
+		const double tau = ns_wave_sum(tmp);
+		tau1 = t + tau;
+		if ((sigma <= tiny) || (fabs(tau1) >= fac * sigma)) {
+			tau2 = tau - (tau1 - t);
+			return;
+		}
+		sigma = shrink * sigma;
+		t = tau1;
+	}
+}
+
+__device__ inline void ns_transform_k_wave(int n, double &v, double rho, double &res, double &r,
+                                           const uint16_t *J) {
+	double tmp = 0., tau1, tau2;
+	ns_transform_wave(n, v, rho, tau1, tau2, J);
+	for (int i = 0; i < n; i++) tmp = tmp + __shfl(v, i, 64);
+	res = tau1 + (tau2 + tmp);
+	r = tau2 - (res - tau1);
+}
+
+// all 64 lanes of the wave call it; v: this lane's value (0 for lanes >= n)
+__device__ inline double nearsum_wave(int n, double v, const uint16_t *J) {
+	double tmp, res, res2, r, r2, mu, delta, delta2;
+	const double eps = 1.1102230246251565e-16;
+	ns_transform_k_wave(n, v, 0., res, r, J);
+	ns_transform_k_wave(n, v, r, delta, r2, J);
+	if (delta == 0) return res;
+	res2 = res + ns_sign(delta) * eps * fabs(res);
+	if (res2 == res) {
+		mu = ns_sign(delta) * eps * fabs(res);
+		res2 = res + 2. * ns_sign(delta) * eps * fabs(res);
+	} else {
+		mu = (res2 - res) / 2.;
+	}
+	if (fabs(delta) < fabs(mu)) return res;
+	if (fabs(delta) > fabs(mu)) return res2;
+	ns_transform_k_wave(n, v, r2, delta2, tmp, J);
+	if (delta2 == 0) return res + mu;
+	if (ns_sign(delta2) == ns_sign(mu)) return res2;
+	return res;
+}
+
+// ---------------------------------------------------------------------------------------------
 // binary32 NearSum: the <float> instantiation of the same functions.  There the element type is
 // float, but unqualified fabs and scalbn and the double literals 2. and /2. promote part of each
 // expression to double before the result is rounded to float on assignment; ceil(std::log2(mu))

@@ -291,6 +291,51 @@ def test_scg_tolerance_stop_bitwise(oz, oracle):
     assert f32_equal(host(dx), xw)
 
 
+@pytest.mark.parametrize("sumMode", [0, 1])
+@pytest.mark.parametrize("path", ["device", "host", "overflow"])
+def test_scg_paths_bitwise(oz, oracle, sumMode, path):
+    """The binary32 CG iterated device-resident, synchronously, and falling back from the device
+    path (a slice bound of 1: the first CSR product gives up) gives the oracle's bits."""
+    g, iters = 12, 60
+    rowptr, colind, val, n, nnz = _laplace_f(oracle, g)
+    oracle.srand48(5)
+    b = np.array([2.0 * oracle.drand48() - 1.0 for _ in range(n)], dtype=np.float32)
+    say(f"scg paths n={n} iters={iters} sumMode={sumMode} path={path}")
+    h = oracle.Handle(1 << 30, sumMode=sumMode)
+    xw = np.zeros(n, dtype=np.float32)
+    want = h.scg(n, nnz, val, colind, rowptr, b, xw, iters, np.float32(0))
+    bound = {"device": 0, "host": -1, "overflow": 1}[path]
+    H = oz.Handle(1 << 28, sumMode=sumMode, cgDeviceSlices=bound)
+    dx = dev(np.zeros(n, dtype=np.float32))
+    got = H.scg(n, nnz, dev(val), dev(colind), dev(rowptr), dev(b), dx, iters, 0.0)
+    assert H.cgDevicePath == {"device": 1, "host": 0, "overflow": 2}[path]
+    assert len(got) == iters + 1
+    assert f32_equal(got.astype(np.float32), want), f32_diff(got, want)
+    assert f32_equal(host(dx), xw), f32_diff(host(dx), xw)
+
+
+@pytest.mark.parametrize("path", ["device", "host"])
+def test_scg_tolerance_stop_paths_bitwise(oz, oracle, path):
+    g, iters = 10, 80
+    rowptr, colind, val, n, nnz = _laplace_f(oracle, g)
+    oracle.srand48(7)
+    b = np.array([2.0 * oracle.drand48() - 1.0 for _ in range(n)], dtype=np.float32)
+    full = oracle.Handle(1 << 30, sumMode=1).scg(n, nnz, val, colind, rowptr, b,
+                                                 np.zeros(n, dtype=np.float32), iters, np.float32(0))
+    tol = np.float32(np.nextafter(full[20], np.float32(1)))
+    h = oracle.Handle(1 << 30, sumMode=1)
+    xw = np.zeros(n, dtype=np.float32)
+    want = h.scg(n, nnz, val, colind, rowptr, b, xw, iters, tol)
+    assert 1 < len(want) - 1 < iters
+    H = oz.Handle(1 << 28, sumMode=1, cgDeviceSlices=0 if path == "device" else -1)
+    dx = dev(np.zeros(n, dtype=np.float32))
+    say(f"scg tol stop path={path} n={n}")
+    got = H.scg(n, nnz, dev(val), dev(colind), dev(rowptr), dev(b), dx, iters, float(tol))
+    assert H.cgDevicePath == (1 if path == "device" else 0)
+    assert f32_equal(got.astype(np.float32), want), f32_diff(got, want)
+    assert f32_equal(host(dx), xw)
+
+
 def test_binary32_dot2_not_supported(oz):
     """The reference's binary32 Dot2 GEMM is a stub; the entry points report NOT_SUPPORTED."""
     H = oz.Handle(1 << 24, splitEpsMode=2)
