@@ -116,6 +116,47 @@ __host__ __device__ inline int ceil_log2_tab(double x, const uint16_t *J) {
 	return (off <= (uint64_t)J[e + kTauBias]) ? e : e + 1;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Wave reductions on the DPP network (no LDS round trips): quad permutes, half-row and row mirrors,
+// then the row broadcasts of lanes 15 and 31; the total lands in lane 63.  Lanes a step does not
+// write take 0, the identity of the sums and of maxima of non-negative values.  Binary64 values
+// move as two 32-bit halves.
+// ---------------------------------------------------------------------------------------------
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ double dpp_f64(double v) {
+	const uint64_t b = __builtin_bit_cast(uint64_t, v);
+	const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROWMASK, 0xF, false);
+	const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROWMASK, 0xF, false);
+	return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double lane_f64(double v, int lane) {
+	const uint64_t b = __builtin_bit_cast(uint64_t, v);
+	const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
+	const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
+	return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+// the sum over the 64 lanes, valid in lane 63 (exact sums: any order gives the same bits)
+__device__ __forceinline__ double wave_sum63_f64(double v) {
+	v += dpp_f64<0xB1>(v);         // quad_perm [1,0,3,2]
+	v += dpp_f64<0x4E>(v);         // quad_perm [2,3,0,1]
+	v += dpp_f64<0x141>(v);        // row_half_mirror
+	v += dpp_f64<0x140>(v);        // row_mirror: every lane holds its row's sum
+	v += dpp_f64<0x142, 0xA>(v);   // row_bcast:15 into rows 1 and 3
+	v += dpp_f64<0x143, 0xC>(v);   // row_bcast:31 into rows 2 and 3
+	return v;
+}
+__device__ __forceinline__ double wave_sum_f64(double v) { return lane_f64(wave_sum63_f64(v), 63); }
+// the maximum of non-negative values over the 64 lanes, in every lane
+__device__ __forceinline__ double wave_max_nonneg_f64(double v) {
+	v = fmax(v, dpp_f64<0xB1>(v));
+	v = fmax(v, dpp_f64<0x4E>(v));
+	v = fmax(v, dpp_f64<0x141>(v));
+	v = fmax(v, dpp_f64<0x140>(v));
+	v = fmax(v, dpp_f64<0x142, 0xA>(v));
+	v = fmax(v, dpp_f64<0x143, 0xC>(v));
+	return lane_f64(v, 63);
+}
+
 // (short)ceil(log2(fabs(max))): 0 for zero, NaN and Inf (x86-64 conversion of +-inf/NaN -> 0)
 __host__ __device__ inline int tau_of(double max, const uint16_t *J) {
 	const double x = fabs(max);

@@ -7,9 +7,11 @@
 
 namespace oz {
 
+// block reductions: the wave step on the DPP network (values non-negative for the maximum; exact
+// sums for the sum, so any order), then the waves through LDS
 template <int BS>
 __device__ inline double block_max_d(double v, double *red) {
-	for (int o = 32; o > 0; o >>= 1) v = std_max(v, __shfl_xor(v, o, 64));
+	v = wave_max_nonneg_f64(v);
 	const int w = threadIdx.x / 64, l = threadIdx.x % 64;
 	__syncthreads();
 	if (l == 0) red[w] = v;
@@ -21,7 +23,7 @@ __device__ inline double block_max_d(double v, double *red) {
 
 template <int BS>
 __device__ inline double block_sum_d(double v, double *red) {
-	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+	v = wave_sum_f64(v);
 	const int w = threadIdx.x / 64, l = threadIdx.x % 64;
 	__syncthreads();
 	if (l == 0) red[w] = v;
@@ -849,6 +851,7 @@ hipError_t launch_vsplit_plan(hipStream_t st, int n, const float *x, double *sli
 constexpr int kDotChunk = 16;
 constexpr int kDotPairs = 64;    // pair slots per workgroup partial of k_dots_reg
 constexpr int kDotMaxWG = 512;  // workgroups of k_dots_reg at most (two waves per SIMD)
+constexpr int kDotFinOne = 36;  // pair counts (6 x 6 slices) whose partials k_dots_fin loads in one round
 
 template <int BS>
 __global__ __launch_bounds__(BS) void k_slice_dots(int n, const double *xs, int64_t ldx, int nA,
@@ -1821,9 +1824,38 @@ __device__ void dots_finish(int nA, int nB, double *vals, const double *part, in
 	if (part) {
 		// pairs in chunks of 16: every thread loads its (at most two) partials of each pair of the
 		// chunk, all loads in flight together (pair p's partials are contiguous: pair-major), then a
-		// butterfly over each wave and the waves' sums in LDS
+		// sum over each wave (DPP) and the waves' sums in LDS
 		const int t = threadIdx.x, w = t / 64, l = t % 64;
-		for (int p0 = 0; p0 < npairs; p0 += 16) {
+		if (npairs <= kDotFinOne) {
+			// one round: every load of every pair in flight together (the partials come from
+			// memory, so a round is a full memory latency)
+			double v[kDotFinOne];
+#pragma unroll
+			for (int q = 0; q < kDotFinOne; q++) {
+				double a = 0.0, b = 0.0;
+				if (q < npairs) {
+					const double *row = part + (int64_t)q * kDotMaxWG;
+					a = (t < nwg) ? row[t] : 0.0;
+					b = (t + BS < nwg) ? row[t + BS] : 0.0;
+				}
+				v[q] = a + b;
+			}
+#pragma unroll
+			for (int q = 0; q < kDotFinOne; q++) v[q] = wave_sum63_f64(v[q]);
+			__shared__ double redOne[BS / 64][kDotFinOne];
+			if (l == 63) {
+#pragma unroll
+				for (int q = 0; q < kDotFinOne; q++) redOne[w][q] = v[q];
+			}
+			__syncthreads();
+			if (t < npairs) {
+				double r = redOne[0][t];
+				for (int k = 1; k < BS / 64; k++) r += redOne[k][t];
+				psum[t] = r;
+			}
+			__syncthreads();
+		}
+		for (int p0 = (npairs <= kDotFinOne) ? npairs : 0; p0 < npairs; p0 += 16) {
 			double v[16];
 #pragma unroll
 			for (int q = 0; q < 16; q++) {
@@ -1836,14 +1868,8 @@ __device__ void dots_finish(int nA, int nB, double *vals, const double *part, in
 				v[q] = a + b;
 			}
 #pragma unroll
-			for (int o = 32; o > 0; o >>= 1) {
-				double u[16];
-#pragma unroll
-				for (int q = 0; q < 16; q++) u[q] = __shfl_xor(v[q], o, 64);
-#pragma unroll
-				for (int q = 0; q < 16; q++) v[q] += u[q];
-			}
-			if (l == 0) {
+			for (int q = 0; q < 16; q++) v[q] = wave_sum63_f64(v[q]);
+			if (l == 63) {
 #pragma unroll
 				for (int q = 0; q < 16; q++) red[w][q] = v[q];
 			}
@@ -2053,22 +2079,16 @@ __global__ __launch_bounds__(BS) void k_dots_reg(int n, VsRef X, VsRef Y, double
 	else if (!sy) dots_reg_acc<BS, NAM, NBM, TI, true, false>(n, X, Y, nA, nB, acc);
 	else dots_reg_acc<BS, NAM, NBM, TI, true, true>(n, X, Y, nA, nB, acc);
 	const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-	// wave sums: one butterfly step over 16 pairs at a time (independent lane exchanges in flight
-	// together), skipping the rows a >= nA; absent pairs hold zeros and are not read
+	// wave sums on the DPP network, 16 pairs at a time (skipping the rows a >= nA; absent pairs hold
+	// zeros and are not read); lane 63 holds the sums
 	constexpr int QC = 16;
 	static_assert((NAM * NBM) % QC == 0, "pair chunks");
 #pragma unroll
 	for (int q0 = 0; q0 < NAM * NBM; q0 += QC) {
 		if (q0 / NBM >= nA) continue;
 #pragma unroll
-		for (int o = 32; o > 0; o >>= 1) {
-			double t[QC];
-#pragma unroll
-			for (int q = 0; q < QC; q++) t[q] = __shfl_xor(acc[q0 + q], o, 64);
-#pragma unroll
-			for (int q = 0; q < QC; q++) acc[q0 + q] += t[q];
-		}
-		if (l == 0) {
+		for (int q = 0; q < QC; q++) acc[q0 + q] = wave_sum63_f64(acc[q0 + q]);
+		if (l == 63) {
 #pragma unroll
 			for (int q = 0; q < QC; q++) red[w][q0 + q] = acc[q0 + q];
 		}

@@ -127,18 +127,12 @@ __device__ inline double nearsum(int n, V &vec, const uint16_t *J) {
 // i >= n: 0), every lane computing the same result.  Each pass of the transform extracts
 // tmp_i = (sigma + vec[i]) - sigma with sigma >= 2^m max|vec|, 2^m >= n + 2: the tmp_i are
 // multiples of ulp(sigma) / 2 and every partial sum of them stays within sigma, so the
-// reference's running sum tau is exact and equals a butterfly sum over the lanes, as mu equals a
-// butterfly maximum (values finite).  The one inexact sum -- of the remainders after a transform --
-// runs in lane order (src/cpu/ozblas_sum.cpp:64-67), each lane's value broadcast in turn.
+// reference's running sum tau is exact and equals a sum over the lanes in any order, as mu equals a
+// maximum over the lanes (values finite).  The one inexact sum -- of the remainders after a transform --
+// runs in lane order (src/cpu/ozblas_sum.cpp:64-67), each lane's value read in turn.
 // ---------------------------------------------------------------------------------------------
-__device__ inline double ns_wave_max(double v) {
-	for (int o = 32; o > 0; o >>= 1) v = mac_max(v, __shfl_xor(v, o, 64));
-	return v;
-}
-__device__ inline double ns_wave_sum(double v) {
-	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-	return v;
-}
+__device__ inline double ns_wave_max(double v) { return wave_max_nonneg_f64(v); }  // of |values|: finite, >= 0
+__device__ inline double ns_wave_sum(double v) { return wave_sum_f64(v); }
 
 __device__ inline void ns_transform_wave(int n, double &v, double rho, double &tau1, double &tau2,
                                          const uint16_t *J) {
@@ -177,7 +171,7 @@ __device__ inline void ns_transform_k_wave(int n, double &v, double rho, double 
                                            const uint16_t *J) {
 	double tmp = 0., tau1, tau2;
 	ns_transform_wave(n, v, rho, tau1, tau2, J);
-	for (int i = 0; i < n; i++) tmp = tmp + __shfl(v, i, 64);
+	for (int i = 0; i < n; i++) tmp = tmp + lane_f64(v, i);
 	res = tau1 + (tau2 + tmp);
 	r = tau2 - (res - tau1);
 }

@@ -132,3 +132,54 @@ def test_remainder_sum_is_order_dependent():
         seq = seq + x
     bfly = (vals[0] + vals[1]) + (vals[2] + vals[3])
     assert seq != bfly
+
+
+# ---- the wave reduction's lane network (wave_sum63_f64 / wave_max_nonneg_f64 in oz_internal.hpp) --
+def dpp(vals, ctrl, row_mask=0xF):
+    """One DPP move of 64 lanes (GFX9 semantics): quad_perm 0x00-0xFF, row_mirror 0x140,
+    row_half_mirror 0x141, row_bcast:15 0x142, row_bcast:31 0x143; lanes of rows outside row_mask,
+    or whose source does not exist, take the old value 0."""
+    out = [0.0] * 64
+    for i in range(64):
+        row = i // 16
+        if not (row_mask >> row) & 1:
+            continue
+        if ctrl < 0x100:
+            src = (i & ~3) + ((ctrl >> (2 * (i & 3))) & 3)
+        elif ctrl == 0x140:
+            src = (i & ~15) + (15 - (i & 15))
+        elif ctrl == 0x141:
+            src = (i & ~7) + (7 - (i & 7))
+        elif ctrl == 0x142:
+            src = 16 * (row - 1) + 15 if row > 0 else None
+        elif ctrl == 0x143:
+            src = 31 if row >= 2 else None
+        out[i] = vals[src] if src is not None else 0.0
+    return out
+
+
+STEPS = [(0xB1, 0xF), (0x4E, 0xF), (0x141, 0xF), (0x140, 0xF), (0x142, 0xA), (0x143, 0xC)]
+
+
+def test_wave_reduction_network_reaches_lane_63():
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        ints = [int(v) for v in rng.integers(-2 ** 40, 2 ** 40, 64)]
+        v = [float(x) for x in ints]  # exact sums in any order
+        for ctrl, mask in STEPS:
+            d = dpp(v, ctrl, mask)
+            v = [a + b for a, b in zip(v, d)]
+        assert v[63] == float(sum(ints))
+        m = [abs(float(x)) for x in rng.integers(0, 2 ** 40, 64)]
+        mx = list(m)
+        for ctrl, mask in STEPS:
+            d = dpp(mx, ctrl, mask)
+            mx = [max(a, b) for a, b in zip(mx, d)]
+        assert mx[63] == max(m)
+    # without the row_bcast:31 step, or with the row masks swapped, lane 63 misses rows
+    for steps in (STEPS[:5], STEPS[:4] + [(0x142, 0xC), (0x143, 0xA)]):
+        v = [1.0] * 64
+        for ctrl, mask in steps:
+            d = dpp(v, ctrl, mask)
+            v = [a + b for a, b in zip(v, d)]
+        assert v[63] != 64.0
