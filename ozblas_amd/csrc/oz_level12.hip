@@ -535,7 +535,11 @@ __global__ __launch_bounds__(BS) void k_vsplit_pre(int n, const TI *x, VsPre *pr
 	for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += 4 * stride) {
 		double v[4];
 #pragma unroll
-		for (int e = 0; e < 4; e++) v[e] = (i + e * stride < n) ? (double)x[i + e * stride] : 0.0;
+		for (int e = 0; e < 4; e++) {
+			const int64_t ie = i + e * stride;
+			const double xl = (double)x[ie < n ? ie : (int64_t)n - 1];
+			v[e] = ie < n ? xl : 0.0;
+		}
 #pragma unroll
 		for (int e = 0; e < 4; e++) vs_stats_acc(v[e], mx, lo, bad);
 	}
@@ -617,7 +621,11 @@ __global__ __launch_bounds__(BS) void k_vsplit_levels(int n, const TI *x, VsPre 
 		for (int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x; i < n; i += 4 * stride) {
 			double v[4];
 #pragma unroll
-			for (int e = 0; e < 4; e++) v[e] = (i + e * stride < n) ? (double)x[i + e * stride] : 0.0;
+			for (int e = 0; e < 4; e++) {
+				const int64_t ie = i + e * stride;
+				const double xl = (double)x[ie < n ? ie : (int64_t)n - 1];
+				v[e] = ie < n ? xl : 0.0;
+			}
 #pragma unroll
 			for (int e = 0; e < 4; e++)
 #pragma unroll
@@ -659,19 +667,20 @@ struct VsGen {
 		last = p->last;
 		zero = p->zero;
 #pragma unroll
-		for (int q = 0; q < NS; q++) sig[q] = (q <= last) ? vs_sigma<TI>(p->k[q]) : (TI)0;
+		for (int q = 0; q < NS; q++) {
+			const int kq = p->k[q];
+			sig[q] = (q <= last) ? vs_sigma<TI>(kq) : (TI)0;
+		}
 	}
+	// Branch-free: with sigma = 0 beyond the last level the step returns the remainder itself at
+	// slot last + 1 (exactly 0 when the chain ended in zero) and 0 after it.
 	__device__ inline void gen(TI v, double (&out)[NS]) const {
 		TI r = v;
 #pragma unroll
 		for (int q = 0; q < NS; q++) {
-			if (q <= last) {
-				const TI a = (r + sig[q]) - sig[q];
-				out[q] = (double)a;
-				r = r - a;
-			} else {
-				out[q] = (q == last + 1 && !zero) ? (double)r : 0.0;
-			}
+			const TI a = (r + sig[q]) - sig[q];
+			out[q] = (double)a;
+			r = r - a;
 		}
 	}
 };
@@ -1101,25 +1110,43 @@ __device__ __forceinline__ T csrmm_row_sum(int r, const double *As, int64_t ldas
 #pragma unroll
 	for (int q = 0; q < NAM * NBM; q++) v[q] = (T)0;
 	// nonzeros in chunks of CH: the chunk's column indices, then its A and x loads, all in flight
-	// together, then the fma chains in row order (the products of one pair stay in sequence)
+	// together, then the fma chains in row order (the products of one pair stay in sequence).  With
+	// OTF every load is unconditional, at an index clamped into the row and the slice counts, and the
+	// unused values are selected away afterwards: a conditional load becomes a branch with a full
+	// memory wait behind it, which serialises the chunk.
 	constexpr int CH = (NAM * NBM <= 16) ? 4 : 2;
 	for (int j0 = b; j0 < e; j0 += CH) {
 		int c[CH];
-#pragma unroll
-		for (int q = 0; q < CH; q++) c[q] = (j0 + q < e) ? colind[j0 + q] : 0;
 		double a[CH][NAM], x[CH][NBM];
+		if constexpr (OTF) {
 #pragma unroll
-		for (int q = 0; q < CH; q++) {
-			const bool in = j0 + q < e;
+			for (int q = 0; q < CH; q++) c[q] = colind[min(j0 + q, e - 1)];
 #pragma unroll
-			for (int ia = 0; ia < NAM; ia++)
-				a[q][ia] = (in && ia < nA) ? As[(int64_t)ia * ldas + j0 + q] : 0.0;
-			if constexpr (OTF) {
-				gen->gen(in ? X[c[q]] : (TI)0, x[q]);
+			for (int q = 0; q < CH; q++) {
+				const bool in = j0 + q < e;
+				const int jj = min(j0 + q, e - 1);
+#pragma unroll
+				for (int ia = 0; ia < NAM; ia++) {
+					const double av = As[(int64_t)min(ia, nA - 1) * ldas + jj];
+					a[q][ia] = (in && ia < nA) ? av : 0.0;
+				}
+				const TI xv = X[c[q]];
+				gen->gen(in ? xv : (TI)0, x[q]);
 #pragma unroll
 				for (int ib = 0; ib < NBM; ib++)
 					if (ib >= nB) x[q][ib] = 0.0;
-			} else {
+			}
+		} else {
+			// (the stored slices: the compiler sinks such loads into their branches whatever the
+			// source form, and this one measured fastest)
+#pragma unroll
+			for (int q = 0; q < CH; q++) c[q] = (j0 + q < e) ? colind[j0 + q] : 0;
+#pragma unroll
+			for (int q = 0; q < CH; q++) {
+				const bool in = j0 + q < e;
+#pragma unroll
+				for (int ia = 0; ia < NAM; ia++)
+					a[q][ia] = (in && ia < nA) ? As[(int64_t)ia * ldas + j0 + q] : 0.0;
 #pragma unroll
 				for (int ib = 0; ib < NBM; ib++)
 					x[q][ib] = (in && ib < nB) ? Xs[(int64_t)ib * ldxs + c[q]] : 0.0;
@@ -1832,13 +1859,11 @@ __device__ void dots_finish(int nA, int nB, double *vals, const double *part, in
 			double v[kDotFinOne];
 #pragma unroll
 			for (int q = 0; q < kDotFinOne; q++) {
-				double a = 0.0, b = 0.0;
-				if (q < npairs) {
-					const double *row = part + (int64_t)q * kDotMaxWG;
-					a = (t < nwg) ? row[t] : 0.0;
-					b = (t + BS < nwg) ? row[t + BS] : 0.0;
-				}
-				v[q] = a + b;
+				// unconditional loads inside the buffer (kDotPairs x kDotMaxWG); the rows q >= npairs
+				// are summed too and never read
+				const double *row = part + (int64_t)q * kDotMaxWG;
+				const double a = row[t], b = row[t + BS];
+				v[q] = (t < nwg ? a : 0.0) + (t + BS < nwg ? b : 0.0);
 			}
 #pragma unroll
 			for (int q = 0; q < kDotFinOne; q++) v[q] = wave_sum63_f64(v[q]);
@@ -1859,13 +1884,9 @@ __device__ void dots_finish(int nA, int nB, double *vals, const double *part, in
 			double v[16];
 #pragma unroll
 			for (int q = 0; q < 16; q++) {
-				double a = 0.0, b = 0.0;
-				if (p0 + q < npairs) {
-					const double *row = part + (int64_t)(p0 + q) * kDotMaxWG;
-					a = (t < nwg) ? row[t] : 0.0;
-					b = (t + BS < nwg) ? row[t + BS] : 0.0;
-				}
-				v[q] = a + b;
+				const double *row = part + (int64_t)min(p0 + q, kDotPairs - 1) * kDotMaxWG;
+				const double a = row[t], b = row[t + BS];
+				v[q] = (t < nwg ? a : 0.0) + (t + BS < nwg ? b : 0.0);
 			}
 #pragma unroll
 			for (int q = 0; q < 16; q++) v[q] = wave_sum63_f64(v[q]);
@@ -1998,7 +2019,10 @@ __device__ __forceinline__ void vs_ref_load(const VsRef &V, bool stored, const V
                                             int64_t i, bool in, double (&out)[NS]) {
 	if (stored) {
 #pragma unroll
-		for (int q = 0; q < NS; q++) out[q] = (in && q < cnt) ? V.slices[(int64_t)q * V.lds + i] : 0.0;
+		for (int q = 0; q < NS; q++) {
+			const double sv = V.slices[(int64_t)min(q, cnt - 1) * V.lds + (in ? i : 0)];
+			out[q] = (in && q < cnt) ? sv : 0.0;
+		}
 	} else {
 		g.gen(in ? ((const TI *)V.x)[i] : (TI)0, out);
 #pragma unroll
@@ -2038,9 +2062,20 @@ __device__ __forceinline__ void dots_reg_acc(int n, const VsRef &X, const VsRef 
 		TI xe[E], ye[E];
 #pragma unroll
 		for (int e = 0; e < E; e++) {
-			const int64_t ie = i + e * stride;
-			xe[e] = (!SX && ie < n) ? xv[ie] : (TI)0;
-			ye[e] = (!SY && ie < n) ? yv[ie] : (TI)0;
+			// unconditional loads at an index clamped into the vector, then the select
+			const int64_t ie = i + e * stride, ic = ie < n ? ie : (int64_t)n - 1;
+			if constexpr (!SX) {
+				const TI xl = xv[ic];
+				xe[e] = ie < n ? xl : (TI)0;
+			} else {
+				xe[e] = 0;
+			}
+			if constexpr (!SY) {
+				const TI yl = yv[ic];
+				ye[e] = ie < n ? yl : (TI)0;
+			} else {
+				ye[e] = 0;
+			}
 		}
 #pragma unroll
 		for (int e = 0; e < E; e++) {
