@@ -60,7 +60,8 @@ def parse():
     return ap.parse_args()
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_digit_gemm_i8_traffic.md")
+TRAFFIC_PROFILES = [os.path.join(ROOT, "profiles", f) for f in
+                    ("k_digit_gemm_i8_residues_traffic.md", "k_digit_gemm_i8_traffic.md")]
 
 
 def measured_traffic(n, phi, form, scheme):
@@ -68,13 +69,14 @@ def measured_traffic(n, phi, form, scheme):
     them (FETCH_SIZE + WRITE_SIZE of that launch, each scaled by the same counter's reading for a
     plain copy of known size).  Counters cannot be read inside this timed run, so the figure comes
     from that summary; None when it is absent or was measured on another workload."""
-    try:
-        text = open(TRAFFIC_PROFILE).read()
-    except OSError:
-        return None
-    for m in re.finditer(r"^traffic: n=(\d+) phi=([0-9.]+) form=(\w+) scheme=(\w+) bytes=(\d+)$", text, re.M):
-        if int(m.group(1)) == n and float(m.group(2)) == phi and m.group(3) == form and m.group(4) == scheme:
-            return int(m.group(5))
+    for path in TRAFFIC_PROFILES:
+        try:
+            text = open(path).read()
+        except OSError:
+            continue
+        for m in re.finditer(r"^traffic: n=(\d+) phi=([0-9.]+) form=(\w+) scheme=(\w+) bytes=(\d+)$", text, re.M):
+            if int(m.group(1)) == n and float(m.group(2)) == phi and m.group(3) == form and m.group(4) == scheme:
+                return int(m.group(5))
     return None
 
 
