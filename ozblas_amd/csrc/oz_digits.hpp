@@ -302,6 +302,17 @@ __device__ __forceinline__ int mod_i32(int v, int p, float invpf) {
 	return r;
 }
 
+// v mod p for |v| <= 2^27 (a product over at most 8192 k of balanced residues) and p >= 25: one
+// binary32 quotient floor(RN(RN(v) RN(1/p))) is within 24/p < 1 of v/p, so v - q p lies in [-p, 2p)
+// and two corrections finish (q p by the 24-bit multiply: |q| < 2^23)
+__device__ __forceinline__ int mod_i32_short(int v, int p, float invpf) {
+	const int q = (int)floorf((float)v * invpf);
+	int r = v - __mul24(q, p);
+	r += (r < 0) ? p : 0;
+	r -= (r >= p) ? p : 0;
+	return r;
+}
+
 // Product residues of the residue form: one byte per element, four consecutive columns of a row
 // in one 32-bit word -- element (i, j) of a plane at (j / 4) * 4 * ldr + 4 * i + j % 4, so a lane's
 // four MFMA result columns are one word and a wave's words are contiguous.
@@ -458,22 +469,29 @@ __device__ __forceinline__ void i8_tile(const I8Tile &g, char *stage0, char *sta
 			const int p = g.rc->p[t];
 			const float invpf = g.rc->invpf[t];
 			uint8_t *Rb = g.R + (int64_t)t * g.rstride;
+			auto store = [&](auto modf) {
 #pragma unroll
-			for (int mi = 0; mi < TM; mi++)
+				for (int mi = 0; mi < TM; mi++)
 #pragma unroll
-				for (int nj = 0; nj < TN; nj++)
+					for (int nj = 0; nj < TN; nj++)
 #pragma unroll
-					for (int r4 = 0; r4 < 4; r4++) {
-						const int i = i0 + wm + 32 * mi + l % 32;
-						const int j = j0 + wn + 32 * nj + 8 * r4 + 4 * (l / 32);
-						uint32_t w = 0;
+						for (int r4 = 0; r4 < 4; r4++) {
+							const int i = i0 + wm + 32 * mi + l % 32;
+							const int j = j0 + wn + 32 * nj + 8 * r4 + 4 * (l / 32);
+							uint32_t w = 0;
 #pragma unroll
-						for (int c = 0; c < 4; c++) {
-							w |= (uint32_t)mod_i32(acc[mi][nj][4 * r4 + c], p, invpf) << (8 * c);
-							acc[mi][nj][4 * r4 + c] = 0;
+							for (int c = 0; c < 4; c++) {
+								w |= (uint32_t)modf(acc[mi][nj][4 * r4 + c], p, invpf) << (8 * c);
+								acc[mi][nj][4 * r4 + c] = 0;
+							}
+							*(uint32_t *)(Rb + res_word_offset(i, j, g.ldt)) = w;
 						}
-						*(uint32_t *)(Rb + res_word_offset(i, j, g.ldt)) = w;
-					}
+			};
+			// |acc| <= 2^14 k: the one-step reduction for chunks of at most 8192 k
+			if (g.k1 - g.k0 <= 8192)
+				store([](int v, int pp, float ip) { return mod_i32_short(v, pp, ip); });
+			else
+				store([](int v, int pp, float ip) { return mod_i32(v, pp, ip); });
 		}
 	};
 	// prologue: stage 0 -> buffer 0, stage 1 -> registers set 1

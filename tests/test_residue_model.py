@@ -210,3 +210,28 @@ def test_epilogue_mod_against_integers(rc):
                 for d in (-1, 0, 1)]
         for v in vs + [x for x in near if abs(x) <= 2 ** 30]:
             assert mod_i32(v, p, c["invpf"][t]) == v % p, (p, v)
+
+
+def mod_i32_short(v, p, invpf):
+    """mod_i32_short restated: one binary32 quotient floor(RN(RN(v) * invpf)), then two corrections."""
+    q = int(np.floor(np.float32(np.float32(v) * invpf)))
+    r = v - q * p
+    assert -p <= r < 2 * p, (v, p, r)  # the bound the two corrections rely on
+    r = r + p if r < 0 else r
+    return r - p if r >= p else r
+
+
+def test_short_chunk_epilogue_mod_against_integers(rc):
+    """acc mod p with the one-step quotient for |acc| <= 2^27 (chunks of at most 8192 k): every
+    modulus (29 the smallest), at the ends of the range, near multiples of p and where the binary32
+    rounding of acc is largest."""
+    c, _ = rc[MAXMOD]
+    rng = random.Random(13)
+    top = 2 ** 27
+    vs = [0, 1, -1, top, -top, top - 1, -top + 1, 2 ** 24 + 1, -(2 ** 24) - 1, 2 ** 26 + 3, -(2 ** 26) - 5]
+    vs += [rng.randrange(-top, top + 1) for _ in range(400)]
+    for t in range(MAXMOD):
+        p = int(c["p"][t])
+        near = [q * p + d for q in (rng.randrange(-top // p, top // p) for _ in range(40)) for d in (-1, 0, 1)]
+        for v in vs + [x for x in near if abs(x) <= top]:
+            assert mod_i32_short(v, p, c["invpf"][t]) == v % p, (p, v)
