@@ -9,12 +9,13 @@
  *   cuozblasCreate / cuozblasDestroy                     (include/cuozblas.h:67-68)
  *   cuozblasRnrm2 / Raxpy / Rdot / Rgemv / Rgemm         (include/cuozblas.h:73-77)
  *   cuozblasRcsrmv / RcsrmvSplitA / Rcg                  (include/cuozblas.h:78-80)
- * at <double,...> (ozblasD*), <float,...> (ozblasS*) and the mixed <double,float,...> (ozblasDS*).
+ * at <double,...> (ozblasD*), <float,...> (ozblasS*) and the mixed <double,float,...> (ozblasDS*)
+ * and <float,double,...> (ozblasSD*).
  *
  * The reference returns 0 and calls exit(1) on an error; these return the OZBLAS_STATUS_* code
  * (0 on success), or exit(1) as the reference does when OZBLAS_SHIM_EXIT_ON_ERROR is defined.
- * Instantiations without a C entry point (<float,double,double> and the binary128 ones) are
- * declared by the primary templates and left undefined, so using one fails at link time.
+ * Instantiations without a C entry point (the binary128 ones) are declared by the primary templates
+ * and left undefined, so using one fails at link time.
  */
 #ifndef CUOZBLAS_MI355X_HPP
 #define CUOZBLAS_MI355X_HPP
@@ -205,6 +206,66 @@ inline int32_t cuozblasRcg<double, float>(cuozblasHandle_t *oh, const char tranA
                                           const int32_t *matArowptr, const double *vecB, double *vecX,
                                           int32_t maxiter, double tol) {
 	return cuozblas_status(ozblasDScg(oh, tranA, dimN, dimNNZ, descrA, matA, matAcolind, matArowptr, vecB,
+	                                  vecX, maxiter, tol),
+	                       "cuozblasRcg");
+}
+
+// ---- the mixed instantiations (binary32 data, binary64-held slices) ------------------------------
+template <>
+inline int32_t cuozblasRnrm2<float, double, double>(cuozblasHandle_t *oh, const int32_t n, const float *devX,
+                                                    const int32_t incx, float *ret) {
+	return cuozblas_status(ozblasSDnrm2(oh, n, devX, incx, ret), "cuozblasRnrm2");
+}
+template <>
+inline int32_t cuozblasRdot<float, double, double>(cuozblasHandle_t *oh, const int32_t n, const float *devA,
+                                                   const int32_t incx, const float *devB,
+                                                   const int32_t incy, float *ret) {
+	return cuozblas_status(ozblasSDdot(oh, n, devA, incx, devB, incy, ret), "cuozblasRdot");
+}
+template <>
+inline int32_t cuozblasRgemv<float, double, double>(cuozblasHandle_t *oh, const char tranA, const int32_t m,
+                                                    const int32_t n, const float alpha, const float *devA,
+                                                    const int32_t lda, const float *devB, const int32_t incx,
+                                                    const float beta, float *devC, const int32_t incy) {
+	return cuozblas_status(ozblasSDgemv(oh, tranA, m, n, alpha, devA, lda, devB, incx, beta, devC, incy),
+	                       "cuozblasRgemv");
+}
+template <>
+inline int32_t cuozblasRgemm<float, double, double>(cuozblasHandle_t *oh, const char tranA, const char tranB,
+                                                    const int32_t m, const int32_t n, const int32_t k,
+                                                    const float alpha, const float *devA, const int32_t lda,
+                                                    const float *devB, const int32_t ldb, const float beta,
+                                                    float *devC, const int32_t ldc) {
+	return cuozblas_status(
+	    ozblasSDgemm(oh, tranA, tranB, m, n, k, alpha, devA, lda, devB, ldb, beta, devC, ldc),
+	    "cuozblasRgemm");
+}
+template <>
+inline int32_t cuozblasRcsrmv<float, double>(cuozblasHandle_t *oh, const char tranA, const int32_t m,
+                                             const int32_t n, const int32_t nnz, const float alpha,
+                                             const cusparseMatDescr_t descrA, const float *devA,
+                                             const int32_t *devAcolind, const int32_t *devArowptr,
+                                             const float *devB, const float beta, float *devC) {
+	return cuozblas_status(ozblasSDcsrmv(oh, tranA, m, n, nnz, alpha, descrA, devA, devAcolind, devArowptr,
+	                                     devB, beta, devC),
+	                       "cuozblasRcsrmv");
+}
+template <>
+inline double *cuozblasRcsrmvSplitA<float, double>(cuozblasHandle_t *oh, const char tranA, const int32_t m,
+                                                   const int32_t n, const int32_t nnz,
+                                                   const cusparseMatDescr_t descrA, const float *devA,
+                                                   const int32_t *devArowptr) {
+	double *p = ozblasSDcsrmvSplitA(oh, tranA, m, n, nnz, descrA, devA, devArowptr);
+	if (!p) cuozblas_status(oh->lastStatus, "cuozblasRcsrmvSplitA");
+	return p;
+}
+template <>
+inline int32_t cuozblasRcg<float, double>(cuozblasHandle_t *oh, const char tranA, const int32_t dimN,
+                                          const int32_t dimNNZ, const cusparseMatDescr_t descrA,
+                                          const float *matA, const int32_t *matAcolind,
+                                          const int32_t *matArowptr, const float *vecB, float *vecX,
+                                          int32_t maxiter, float tol) {
+	return cuozblas_status(ozblasSDcg(oh, tranA, dimN, dimNNZ, descrA, matA, matAcolind, matArowptr, vecB,
 	                                  vecX, maxiter, tol),
 	                       "cuozblasRcg");
 }

@@ -248,6 +248,34 @@ int32_t ozblasDScg(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz, cons
                    const double *A, const int32_t *colind, const int32_t *rowptr, const double *b,
                    double *x, int32_t maxiter, double tol);
 
+/* mixed: binary32 data split into binary64-held slices -- the reference's <float,double,double> /
+ * <float,double> instantiations (src/cpu/ozblas_gemm.cpp:321, ozblas_gemv.cpp:35, ozblas_dot.cpp:45,
+ * ozblas_nrm2.cpp:37, ozblas_csrmv.cpp:187,277, ozblas_cg.cpp:134; templates of
+ * include/cuozblas.h:73-80).  The split runs in binary32 with getRho<float,double>, the slice
+ * products in binary64, the Fsum in binary32.  Bitwise equal to the reference CPU build wherever
+ * its binary64 slice products are exact (inner dimension above 128 in splitEpsMode 0; every CSR
+ * product, whose fma chain the reference fixes); sumMode = 0 only (the mixed NearSum exits in the
+ * reference, src/cpu/ozblas_sum.cpp:275-330); Dot2 returns NOT_SUPPORTED.  ozblasSDcsrmvSplitA
+ * returns the library's slice buffer typed as the reference's double *, to be passed back as A of
+ * ozblasSDcsrmv. */
+int32_t ozblasSDgemm(ozblasHandle_t *oh, char transA, char transB, int32_t m, int32_t n, int32_t k,
+                     float alpha, const float *A, int32_t lda, const float *B, int32_t ldb,
+                     float beta, float *C, int32_t ldc);
+int32_t ozblasSDgemv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, float alpha,
+                     const float *A, int32_t lda, const float *x, int32_t incx, float beta, float *y,
+                     int32_t incy);
+int32_t ozblasSDdot(ozblasHandle_t *oh, int32_t n, const float *x, int32_t incx, const float *y,
+                    int32_t incy, float *ret);
+int32_t ozblasSDnrm2(ozblasHandle_t *oh, int32_t n, const float *x, int32_t incx, float *ret);
+int32_t ozblasSDcsrmv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, int32_t nnz,
+                      float alpha, const void *descrA, const float *A, const int32_t *colind,
+                      const int32_t *rowptr, const float *x, float beta, float *y);
+double *ozblasSDcsrmvSplitA(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, int32_t nnz,
+                            const void *descrA, const float *A, const int32_t *rowptr);
+int32_t ozblasSDcg(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz, const void *descrA,
+                   const float *A, const int32_t *colind, const int32_t *rowptr, const float *b,
+                   float *x, int32_t maxiter, float tol);
+
 /* batched uneven GEMM (no reference counterpart; the reference batches only the slice pairs of one
  * GEMM, src/cuda/cuozblas_gemm.cu:150-190).  Arrays of per-problem sizes and device pointers live
  * in host memory.  Every C[p] is bitwise equal to ozblasDgemm on problem p alone. */

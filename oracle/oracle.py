@@ -162,6 +162,29 @@ def lib():
         L.ozo_ds_cg.restype = C.c_int32
         L.ozo_ds_cg.argtypes = [H, C.c_int32, C.c_int32, _dp, _ip, _ip, _dp, _dp, C.c_int32,
                                 C.c_double]
+        L.ozo_sd_rho.restype = C.c_int32
+        L.ozo_sd_rho.argtypes = [C.c_int32, C.c_int32]
+        L.ozo_sd_split.restype = C.c_int32
+        L.ozo_sd_split.argtypes = [H, C.c_int32, C.c_int32, _fp, C.c_int32, _fp, C.c_int32, _dp,
+                                   C.c_int32, _sp, C.c_int32, _fp]
+        L.ozo_sd_gemm.restype = C.c_int32
+        L.ozo_sd_gemm.argtypes = [H, C.c_char, C.c_char, C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                                  _fp, C.c_int32, _fp, C.c_int32, C.c_float, _fp, C.c_int32]
+        L.ozo_sd_dot.restype = C.c_float
+        L.ozo_sd_dot.argtypes = [H, C.c_int32, _fp, _fp]
+        L.ozo_sd_nrm2.restype = C.c_float
+        L.ozo_sd_nrm2.argtypes = [H, C.c_int32, _fp]
+        L.ozo_sd_gemv.restype = C.c_int32
+        L.ozo_sd_gemv.argtypes = [H, C.c_char, C.c_int32, C.c_int32, C.c_float, _fp, C.c_int32, _fp,
+                                  C.c_float, _fp]
+        L.ozo_sd_csrmv.restype = C.c_int32
+        L.ozo_sd_csrmv.argtypes = [H, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_void_p, _ip,
+                                   _ip, _fp, C.c_float, _fp]
+        L.ozo_sd_csrmv_splitA.restype = C.c_void_p
+        L.ozo_sd_csrmv_splitA.argtypes = [H, C.c_int32, C.c_int32, C.c_int32, _fp, _ip]
+        L.ozo_sd_cg.restype = C.c_int32
+        L.ozo_sd_cg.argtypes = [H, C.c_int32, C.c_int32, _fp, _ip, _ip, _fp, _fp, C.c_int32,
+                                C.c_float]
         L.ozo_laplace3d_nnz.restype = C.c_int64
         L.ozo_laplace3d_nnz.argtypes = [C.c_int32]
         L.ozo_laplace3d.argtypes = [C.c_int32, C.c_double, _ip, _ip, _dp]
@@ -356,9 +379,55 @@ class Handle:
         return hist[: self._h.contents.cg_numiter + 1]
 
 
+    # mixed routines (the <float,double,double> instantiations, ozoracle_sd.c) -----------------
+    def sdgemm(self, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+        rc = lib().ozo_sd_gemm(self._h, transA.encode(), transB.encode(), m, n, k, alpha, A, lda, B,
+                               ldb, beta, C, ldc)
+        if rc != 0:
+            raise RuntimeError(f"ozo_sd_gemm failed ({rc})")
+        return C
+
+    def sddot(self, n, x, y):
+        return np.float32(lib().ozo_sd_dot(self._h, n, x, y))
+
+    def sdnrm2(self, n, x):
+        return np.float32(lib().ozo_sd_nrm2(self._h, n, x))
+
+    def sdgemv(self, trans, m, n, alpha, A, lda, x, beta, y):
+        rc = lib().ozo_sd_gemv(self._h, trans.encode(), m, n, alpha, A, lda, x, beta, y)
+        if rc != 0:
+            raise RuntimeError(f"ozo_sd_gemv failed ({rc})")
+        return y
+
+    def sdcsrmv(self, m, n, nnz, alpha, val, colind, rowptr, x, beta, y):
+        ptr = val if isinstance(val, int) else val.ctypes.data
+        rc = lib().ozo_sd_csrmv(self._h, m, n, nnz, alpha, ptr, colind, rowptr, x, beta, y)
+        if rc != 0:
+            raise RuntimeError(f"ozo_sd_csrmv failed ({rc})")
+        return y
+
+    def sdcsrmv_splitA(self, m, n, nnz, val, rowptr):
+        return lib().ozo_sd_csrmv_splitA(self._h, m, n, nnz, val, rowptr)
+
+    def sdcg(self, n, nnz, val, colind, rowptr, b, x, maxiter, tol, history=True):
+        hist = np.zeros(maxiter + 2, dtype=np.float64)
+        if history:
+            self._h.contents.verbose = 1
+            self._h.contents.cg_verbose1 = hist.ctypes.data
+        rc = lib().ozo_sd_cg(self._h, n, nnz, val, colind, rowptr, b, x, maxiter, tol)
+        if rc != 0:
+            raise RuntimeError(f"ozo_sd_cg failed ({rc})")
+        self._h.contents.cg_verbose1 = None
+        return hist[: self._h.contents.cg_numiter + 1].astype(np.float32)
+
+
 # building blocks -------------------------------------------------------------------------------
 def ds_rho(dim, mode=0):
     return lib().ozo_ds_rho(dim, mode)
+
+
+def sd_rho(dim, mode=0):
+    return lib().ozo_sd_rho(dim, mode)
 
 
 def s_rho(dim, mode=0):

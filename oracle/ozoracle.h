@@ -178,6 +178,28 @@ float *ozo_ds_csrmv_splitA(ozo_handle *h, int32_t m, int32_t n, int32_t nnz, con
 int32_t ozo_ds_cg(ozo_handle *h, int32_t n, int32_t nnz, const double *val, const int32_t *colind,
                   const int32_t *rowptr, const double *b, double *x, int32_t maxiter, double tol);
 
+/* mixed restatement (ozoracle_sd.c): the <float,double,double> / <float,double> instantiations;
+   -3 (or NaN / NULL) where the reference has no such path (NearSum, the plain bypass) and for
+   Dot2 (not restated) */
+int32_t ozo_sd_rho(int32_t dim, int32_t splitEpsMode);
+int32_t ozo_sd_split(ozo_handle *h, int32_t m, int32_t n, const float *in, int32_t ldi, float *out,
+                     int32_t ldo, double *split, int32_t lds, int16_t *exps, int32_t ldse,
+                     float *max);
+int32_t ozo_sd_gemm(ozo_handle *h, char transA, char transB, int32_t m, int32_t n, int32_t k,
+                    float alpha, const float *A, int32_t lda, const float *B, int32_t ldb,
+                    float beta, float *C, int32_t ldc);
+float ozo_sd_dot(ozo_handle *h, int32_t n, const float *x, const float *y);
+float ozo_sd_nrm2(ozo_handle *h, int32_t n, const float *x);
+int32_t ozo_sd_gemv(ozo_handle *h, char trans, int32_t m, int32_t n, float alpha, const float *A,
+                    int32_t lda, const float *x, float beta, float *y);
+int32_t ozo_sd_csrmv(ozo_handle *h, int32_t m, int32_t n, int32_t nnz, float alpha, const float *val,
+                     const int32_t *colind, const int32_t *rowptr, const float *x, float beta,
+                     float *y);
+double *ozo_sd_csrmv_splitA(ozo_handle *h, int32_t m, int32_t n, int32_t nnz, const float *val,
+                            const int32_t *rowptr);
+int32_t ozo_sd_cg(ozo_handle *h, int32_t n, int32_t nnz, const float *val, const int32_t *colind,
+                  const int32_t *rowptr, const float *b, float *x, int32_t maxiter, float tol);
+
 /* generators: testing/testing_common.cpp:664-806 */
 void ozo_srand48(int64_t seed);
 double ozo_drand48(void);

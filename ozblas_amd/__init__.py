@@ -102,6 +102,13 @@ _dsnrm2 = _sig("ozblasDSnrm2", _i, _H, _i, _p, _i, C.POINTER(C.c_double))
 _dscsrmv = _sig("ozblasDScsrmv", _i, _H, _c, _i, _i, _i, _d, _p, _p, _p, _p, _p, _d, _p)
 _dscsrmv_split = _sig("ozblasDScsrmvSplitA", _p, _H, _c, _i, _i, _i, _p, _p, _p)
 _dscg = _sig("ozblasDScg", _i, _H, _c, _i, _i, _p, _p, _p, _p, _p, _p, _i, _d)
+_sdgemm = _sig("ozblasSDgemm", _i, _H, _c, _c, _i, _i, _i, _f, _p, _i, _p, _i, _f, _p, _i)
+_sdgemv = _sig("ozblasSDgemv", _i, _H, _c, _i, _i, _f, _p, _i, _p, _i, _f, _p, _i)
+_sddot = _sig("ozblasSDdot", _i, _H, _i, _p, _i, _p, _i, C.POINTER(C.c_float))
+_sdnrm2 = _sig("ozblasSDnrm2", _i, _H, _i, _p, _i, C.POINTER(C.c_float))
+_sdcsrmv = _sig("ozblasSDcsrmv", _i, _H, _c, _i, _i, _i, _f, _p, _p, _p, _p, _p, _f, _p)
+_sdcsrmv_split = _sig("ozblasSDcsrmvSplitA", _p, _H, _c, _i, _i, _i, _p, _p, _p)
+_sdcg = _sig("ozblasSDcg", _i, _H, _c, _i, _i, _p, _p, _p, _p, _p, _p, _i, _f)
 _version = _sig("ozblasGetVersionString", C.c_char_p)
 _gen_phi = _sig("ozblasGenPhi", None, _i, _i, _i, _p, _d, C.c_int64)
 internal_tau = _sig("ozblas_internal_tau", _i, _d)
@@ -110,6 +117,9 @@ internal_rho_row = _sig("ozblas_internal_rho_row", _i, _i)
 internal_rho_row_mode = _sig("ozblas_internal_rho_row_mode", _i, _i, _i)
 internal_rho_ds = _sig("ozblas_internal_rho_ds", _i, _i, _i)
 internal_rho_row_ds = _sig("ozblas_internal_rho_row_ds", _i, _i, _i)
+internal_rho_sd = _sig("ozblas_internal_rho_sd", _i, _i, _i)
+internal_rho_row_sd = _sig("ozblas_internal_rho_row_sd", _i, _i, _i)
+internal_f32_term = _sig("ozblas_internal_f32_term", C.c_float, _d)
 internal_tau_f = _sig("ozblas_internal_tau_f", _i, C.c_float)
 internal_nearsum_exp_f = _sig("ozblas_internal_nearsum_exp_f", _i, C.c_float)
 internal_rho_f = _sig("ozblas_internal_rho_f", _i, _i, _i)
@@ -348,6 +358,39 @@ class Handle:
 
     def dscg(self, n, nnz, val, colind, rowptr, b, x, maxiter, tol, history=True, second=None):
         return self._cg(_dscg, "ozblasDScg", "float64", n, nnz, val, colind, rowptr, b, x, maxiter,
+                        tol, history, second)
+
+    # ---- mixed: binary32 data, binary64-held slices (<float,double,double>) ------------------
+    def sdgemm(self, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc):
+        self._check("ozblasSDgemm", _sdgemm(self.ref, transA.encode(), transB.encode(), m, n, k,
+                                            alpha, _ptr(A), lda, _ptr(B), ldb, beta, _ptr(Cm), ldc))
+
+    def sdgemv(self, trans, m, n, alpha, A, lda, x, beta, y):
+        self._check("ozblasSDgemv", _sdgemv(self.ref, trans.encode(), m, n, alpha, _ptr(A), lda,
+                                            _ptr(x), 1, beta, _ptr(y), 1))
+
+    def sddot(self, n, x, y) -> float:
+        r = C.c_float(0.0)
+        self._check("ozblasSDdot", _sddot(self.ref, n, _ptr(x), 1, _ptr(y), 1, C.byref(r)))
+        return r.value
+
+    def sdnrm2(self, n, x) -> float:
+        r = C.c_float(0.0)
+        self._check("ozblasSDnrm2", _sdnrm2(self.ref, n, _ptr(x), 1, C.byref(r)))
+        return r.value
+
+    def sdcsrmv(self, m, n, nnz, alpha, val, colind, rowptr, x, beta, y):
+        self._check("ozblasSDcsrmv", _sdcsrmv(self.ref, b"N", m, n, nnz, alpha, None, _ptr(val),
+                                              _ptr(colind), _ptr(rowptr), _ptr(x), beta, _ptr(y)))
+
+    def sdcsrmv_splitA(self, m, n, nnz, val, rowptr):
+        p = _sdcsrmv_split(self.ref, b"N", m, n, nnz, None, _ptr(val), _ptr(rowptr))
+        if not p:
+            raise OzblasError("ozblasSDcsrmvSplitA", self._h.lastStatus)
+        return p
+
+    def sdcg(self, n, nnz, val, colind, rowptr, b, x, maxiter, tol, history=True, second=None):
+        return self._cg(_sdcg, "ozblasSDcg", "float32", n, nnz, val, colind, rowptr, b, x, maxiter,
                         tol, history, second)
 
     def dgemm_batched(self, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc):

@@ -46,10 +46,12 @@ static int32_t sync(ozblasHandle_t *oh) {
 
 // Per-precision rules of the drivers: rho of getRho and the device tau tables (the binary32 split
 // takes tau from (float)log2((double)max), its NearSum from log2f).  Both precisions store slices
-// as doubles and share the FP64 product kernels.  The mixed routines (binary64 data split into
-// binary32 slices, Backend::mixed) differ from binary64 only in rho and in the split loop's slot
-// rule: their slices, multiplied unscaled in FP64, are exactly the reference's
-// scalbn(float product, seA + seB).
+// as doubles and share the FP64 product kernels.  The mixed routines (Backend::mixed) differ from
+// their data type's routines in rho and in the split loop's slot rule: binary64 data split into
+// binary32 slices (<double,float>) give unscaled FP64 products that are exactly the reference's
+// scalbn(float product, seA + seB); binary32 data split into binary64-held slices (<float,double>)
+// give products whose binary32 term (f32_term) is the reference's scalbn1<float>((float)P, seA + seB),
+// and their CSR chains run in binary64.
 template <class T>
 struct Prec;
 template <>
@@ -62,7 +64,9 @@ struct Prec<double> {
 };
 template <>
 struct Prec<float> {
-	static int rho(const ozblasHandle_t *oh, int dim) { return rho_host_f(dim, oh->splitEpsMode); }
+	static int rho(const ozblasHandle_t *oh, int dim) {
+		return be(oh)->mixed ? rho_host_sd(dim, oh->splitEpsMode) : rho_host_f(dim, oh->splitEpsMode);
+	}
 	static const uint16_t *Jsplit(Backend *b) { return b->dTauF; }
 	static const uint16_t *Jsum(Backend *b) { return b->dTauF2; }
 };
@@ -902,12 +906,8 @@ static int32_t csr_split(ozblasHandle_t *oh, int m, int nnz, const T *val, const
 		if (!cs.slices) return OZBLAS_STATUS_WORKSPACE;
 	}
 	OZ_HIP(hipMemsetAsync(b->dCounters, 0, sizeof(int), b->stream));
-	if constexpr (sizeof(T) == 8)
-		launch_split_csr(b->stream, m, val, rowptr, work, cs.slices, cs.lds, oh->splitShift, cap,
-		                 Prec<T>::Jsplit(b), zrow, b->dCounters, oh->splitEpsMode, b->mixed);
-	else
-		launch_split_csr(b->stream, m, val, rowptr, work, cs.slices, cs.lds, oh->splitShift, cap,
-		                 Prec<T>::Jsplit(b), zrow, b->dCounters, oh->splitEpsMode);
+	launch_split_csr(b->stream, m, val, rowptr, work, cs.slices, cs.lds, oh->splitShift, cap,
+	                 Prec<T>::Jsplit(b), zrow, b->dCounters, oh->splitEpsMode, b->mixed);
 	OZ_HIP(hipMemcpyAsync(b->hPinned, b->dCounters, sizeof(int), hipMemcpyDeviceToHost, b->stream));
 	OZ_TRY(sync(oh));
 	const int zmax = ((int32_t *)b->hPinned)[0];
@@ -947,16 +947,24 @@ static int32_t csrmv_core(ozblasHandle_t *oh, int m, int n, int nnz, T alpha, co
 		return OZBLAS_STATUS_SUCCESS;
 	}
 	if (nA <= 4 && nB <= 8) {
-		launch_csrmm_sum(b->stream, m, nnz, as.slices, as.lds, nA, colind, rowptr, xs.slices, xs.lds, nB,
-		                 maxlevel, fsum ? 1 : 0, alpha, beta, y, Prec<T>::Jsum(b));
+		if constexpr (sizeof(T) == 4)
+			launch_csrmm_sum(b->stream, m, nnz, as.slices, as.lds, nA, colind, rowptr, xs.slices, xs.lds, nB,
+			                 maxlevel, fsum ? 1 : 0, alpha, beta, y, Prec<T>::Jsum(b), b->mixed);
+		else  // <double,float>: the binary32 chain of the reference is exact, so the binary64 one
+			launch_csrmm_sum(b->stream, m, nnz, as.slices, as.lds, nA, colind, rowptr, xs.slices, xs.lds, nB,
+			                 maxlevel, fsum ? 1 : 0, alpha, beta, y, Prec<T>::Jsum(b));
 	} else {
 		int ic = 0;
 		for (int ia = 0; ia < imin(maxlevel + 1, nA); ia++) ic += imin(nB, maxlevel + 1 - ia);
 		const int64_t ldcs = rup(m, 8);
 		T *Cs = (T *)walloc(oh, sizeof(T) * ldcs * ((int64_t)nA * nB > ic ? (int64_t)nA * nB : ic));
 		if (!Cs) return OZBLAS_STATUS_WORKSPACE;
-		launch_csrmm_store(b->stream, m, as.slices, as.lds, nA, colind, rowptr, xs.slices, xs.lds, nB,
-		                   maxlevel, Cs, ldcs);
+		if constexpr (sizeof(T) == 4)
+			launch_csrmm_store(b->stream, m, as.slices, as.lds, nA, colind, rowptr, xs.slices, xs.lds, nB,
+			                   maxlevel, Cs, ldcs, b->mixed);
+		else
+			launch_csrmm_store(b->stream, m, as.slices, as.lds, nA, colind, rowptr, xs.slices, xs.lds, nB,
+			                   maxlevel, Cs, ldcs);
 		if (fsum)
 			launch_global_fsum(b->stream, m, 1, nA, nB, maxlevel, 3, Cs, ldcs, ldcs, y, 1, alpha, beta);
 		else
@@ -1488,8 +1496,12 @@ static int32_t cg_device_loop(ozblasHandle_t *oh, int n, int nA, const double *A
 		const int par = it & 1, parNext = (it + 1) & 1;
 		OZ_HIP(launch_vsplit_plan(st, n, v.P, slP, lds, rho, 1, cap, reproCount, Js, infoP, vscr, preP, planP,
 		                          it > 0, S, it, kClkSpmv, par));
-		launch_csrmm_sum_dev(st, n, AS, rup(b->nnzA, 32), nA, colind, rowptr, refP, fsum, v.Q, Cs, ldc, Jn, S, it,
-		                     preQ + par);
+		if constexpr (sizeof(T) == 4)
+			launch_csrmm_sum_dev(st, n, AS, rup(b->nnzA, 32), nA, colind, rowptr, refP, fsum, v.Q, Cs, ldc, Jn, S,
+			                     it, preQ + par, b->mixed);
+		else
+			launch_csrmm_sum_dev(st, n, AS, rup(b->nnzA, 32), nA, colind, rowptr, refP, fsum, v.Q, Cs, ldc, Jn, S,
+			                     it, preQ + par);
 		OZ_HIP(launch_vsplit_plan(st, n, v.Q, slQ, lds, rho, 1, cap, reproCount, Js, infoQ, vscr, preQ, planQ,
 		                          true, S, it, kClkDot, par));
 		launch_dots_dev(st, n, refP, refQ, vals, part, Jn, fsum, S, it, 1, nullptr, f32 ? 1 : 0);
@@ -1775,6 +1787,84 @@ extern "C" int32_t ozblasDScg(ozblasHandle_t *oh, char transA, int32_t n, int32_
                               const void *descrA, const double *A, const int32_t *colind,
                               const int32_t *rowptr, const double *b, double *x, int32_t maxiter,
                               double tol) {
+	(void)descrA;
+	OZ_TRY(ensure_init(oh));
+	MixedScope mx(oh);
+	return cg_entry(oh, transA, n, nnz, A, colind, rowptr, b, x, maxiter, tol);
+}
+
+// =============================================================================================
+// mixed routines: binary32 data, binary64-held slices (the reference's <float,double,double> and
+// <float,double> instantiations).  The split runs in binary32 with getRho<float,double>, the slice
+// products in binary64, and the Fsum in binary32 over the products rounded as the reference's
+// (float) of the scaled product (f32_term).  Fsum only: the reference's mixed NearSum exits; Dot2
+// (splitEpsMode 2) is not provided.
+// =============================================================================================
+extern "C" int32_t ozblasSDgemm(ozblasHandle_t *oh, char transA, char transB, int32_t m, int32_t n,
+                                int32_t k, float alpha, const float *A, int32_t lda, const float *B,
+                                int32_t ldb, float beta, float *C, int32_t ldc) {
+	OZ_TRY(ensure_init(oh));
+	if (m < 0 || n < 0 || k < 0) return oh->lastStatus = OZBLAS_STATUS_INVALID_VALUE;
+	if (m == 0 || n == 0) return oh->lastStatus = OZBLAS_STATUS_SUCCESS;
+	oh->splitShift = 1;  // counterInit
+	if (oh->reproMode == 0 && oh->nSplitMax == 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	if (oh->splitEpsMode == 2 || oh->sumMode != 0) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	if (k == 0) return oh->lastStatus = gemm_k0(oh, m, n, alpha, beta, C, ldc);
+	MixedScope mx(oh);
+	return oh->lastStatus = gemm_plain(oh, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+extern "C" int32_t ozblasSDgemv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, float alpha,
+                                const float *A, int32_t lda, const float *x, int32_t incx, float beta,
+                                float *y, int32_t incy) {
+	OZ_TRY(ensure_init(oh));
+	if (incx != 1 || incy != 1) return oh->lastStatus = OZBLAS_STATUS_NOT_SUPPORTED;
+	if (is_n(transA))  // src/cpu/ozblas_gemv.cpp:23-26
+		return ozblasSDgemm(oh, 'n', 'n', m, 1, n, alpha, A, lda, x, n, beta, y, m);
+	return ozblasSDgemm(oh, 't', 'n', n, 1, m, alpha, A, lda, x, m, beta, y, n);
+}
+
+extern "C" int32_t ozblasSDdot(ozblasHandle_t *oh, int32_t n, const float *x, int32_t incx,
+                               const float *y, int32_t incy, float *ret) {
+	OZ_TRY(ensure_init(oh));
+	MixedScope mx(oh);
+	return dot_entry(oh, n, x, incx, y, incy, ret);
+}
+
+extern "C" int32_t ozblasSDnrm2(ozblasHandle_t *oh, int32_t n, const float *x, int32_t incx,
+                                float *ret) {
+	OZ_TRY(ensure_init(oh));
+	float d = 0.f;
+	const int32_t s = ozblasSDdot(oh, n, x, incx, x, incx, &d);
+	if (s != OZBLAS_STATUS_SUCCESS) return s;
+	*ret = (float)sqrt((double)d);  // the double sqrt, rounded (src/cpu/ozblas_nrm2.cpp:27)
+	return OZBLAS_STATUS_SUCCESS;
+}
+
+extern "C" double *ozblasSDcsrmvSplitA(ozblasHandle_t *oh, char transA, int32_t m, int32_t n,
+                                       int32_t nnz, const void *descrA, const float *A,
+                                       const int32_t *rowptr) {
+	(void)descrA;
+	(void)n;
+	if (ensure_init(oh) != OZBLAS_STATUS_SUCCESS) return nullptr;
+	MixedScope mx(oh);
+	return csrmv_split_a(oh, transA, m, nnz, A, rowptr);
+}
+
+extern "C" int32_t ozblasSDcsrmv(ozblasHandle_t *oh, char transA, int32_t m, int32_t n, int32_t nnz,
+                                 float alpha, const void *descrA, const float *A,
+                                 const int32_t *colind, const int32_t *rowptr, const float *x,
+                                 float beta, float *y) {
+	(void)descrA;
+	OZ_TRY(ensure_init(oh));
+	MixedScope mx(oh);
+	return csrmv_entry(oh, transA, m, n, nnz, alpha, A, colind, rowptr, x, beta, y);
+}
+
+extern "C" int32_t ozblasSDcg(ozblasHandle_t *oh, char transA, int32_t n, int32_t nnz,
+                              const void *descrA, const float *A, const int32_t *colind,
+                              const int32_t *rowptr, const float *b, float *x, int32_t maxiter,
+                              float tol) {
 	(void)descrA;
 	OZ_TRY(ensure_init(oh));
 	MixedScope mx(oh);

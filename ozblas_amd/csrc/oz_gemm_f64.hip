@@ -13,9 +13,10 @@
 // k_gemm_f64<true> : fused Fsum -- the pairs are walked in the reference's (level, ia, ib) order
 //                    and t += product is accumulated in registers, then C = fma(alpha, t, beta*C)
 //                    (src/cpu/ozblas_sum.cpp:354-378), so nothing is materialised.
-// TC = float serves the binary32 instantiation: its slices (stored as doubles) give exact FP64
-// products that round once to float, as the reference's float slice GEMM; the Fsum then adds in
-// float and the epilogue is fma1<float> (the double fma, rounded to float).
+// TC = float serves the binary32 instantiations: their slices (stored as doubles) give exact FP64
+// products that round to float as the reference's float slice GEMM (<float,float>) or its
+// (float) of the binary64 slice GEMM followed by the exponent scaling (<float,double>: f32_term);
+// the Fsum then adds in float and the epilogue is fma1<float> (the double fma, rounded to float).
 #include "oz_kernels.hpp"
 
 namespace oz {
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const dou
 		if (FSUM) {
 			for (int a = 0; a < 4; a++)
 				for (int b = 0; b < 4; b++)
-					for (int r = 0; r < 4; r++) tot[a][b][r] = tot[a][b][r] + (TC)acc[a][b][r];
+					for (int r = 0; r < 4; r++) tot[a][b][r] = tot[a][b][r] + sum_term<TC>(acc[a][b][r]);
 		}
 	}
 	TC *Cp = FSUM ? C : C + cOff[blockIdx.y];
@@ -123,7 +124,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const dou
 					if (FSUM)
 						*c = fma1_t(alpha, tot[mi][nj][r], beta * *c);
 					else
-						*c = (TC)acc[mi][nj][r];
+						*c = sum_term<TC>(acc[mi][nj][r]);
 				}
 			}
 }

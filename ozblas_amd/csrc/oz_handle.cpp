@@ -83,6 +83,19 @@ int rho_host_ds(int dim, int splitEpsMode) {
 	return (int)r;
 }
 
+// getRho<float,double> (src/cpu/ozblas_split.cpp:20-34 with getEpse<float> = 24,
+// getEpse<double> = 53, getEpse2<double> = 106)
+int rho_host_sd(int dim, int splitEpsMode) {
+	double r;
+	switch (splitEpsMode) {
+	case 2: r = ceil((24 - (106 - log2(1. * dim)) / 2)); break;
+	case 9: r = ceil((24 - (53 - log2(2. * sqrt((double)dim))) / 2)); break;
+	default: r = ceil((24 - (53 - log2(1. * dim)) / 2)); break;
+	}
+	if (!(r >= -2147483648.0 && r < 2147483648.0)) return INT32_MIN;
+	return (int)r;
+}
+
 // getRho<float,float> (src/cpu/ozblas_split.cpp:20-34 with getEpse = 24, getEpse2 = 53)
 int rho_host_f(int dim, int splitEpsMode) {
 	double r;
@@ -256,6 +269,9 @@ extern "C" int32_t ozblas_internal_rho_row_f(int32_t dim, int32_t mode) { return
 extern "C" int32_t ozblas_internal_rho_row_mode(int32_t dim, int32_t mode) { return rho_dev(dim, mode); }
 extern "C" int32_t ozblas_internal_rho_ds(int32_t dim, int32_t mode) { return rho_host_ds(dim, mode); }
 extern "C" int32_t ozblas_internal_rho_row_ds(int32_t dim, int32_t mode) { return rho_dev_ds(dim, mode); }
+extern "C" int32_t ozblas_internal_rho_sd(int32_t dim, int32_t mode) { return rho_host_sd(dim, mode); }
+extern "C" float ozblas_internal_f32_term(double p) { return f32_term(p); }
+extern "C" int32_t ozblas_internal_rho_row_sd(int32_t dim, int32_t mode) { return rho_dev_sd(dim, mode); }
 extern "C" void ozblas_internal_tau_batch(int64_t n, const double *x, int32_t *out) {
 	const uint16_t *J = tau_table_host();
 	for (int64_t i = 0; i < n; i++) out[i] = tau_of(x[i], J);

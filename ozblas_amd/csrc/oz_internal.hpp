@@ -325,6 +325,44 @@ __host__ __device__ inline int rho_dev_ds(int dim, int mode) {
 }
 int rho_host_ds(int dim, int splitEpsMode);  // getRho<double,float> with the host libm
 
+// getRho<float,double> (the mixed split of binary32 data into binary64-held slices: epse of TYPE1
+// 24, of TYPE2 53, epse2 of TYPE2 106) from integers: mode 0 ceil((log2(d) - 5) / 2)
+// = ceil((q - 5) / 2); mode 9 ceil(log2(d) / 4 - 2) = ceil(q / 4) - 2; mode 2
+// ceil(log2(d) / 2 - 29) = ceil(q / 2) - 29; q = ceil(log2 d)
+__host__ __device__ inline int rho_dev_sd(int dim, int mode) {
+	if (dim <= 0) return INT32_MIN;
+	const int p = 31 - __builtin_clz((unsigned)dim);
+	const int q = ((dim & (dim - 1)) == 0) ? p : p + 1;
+	if (mode == 9) return (q + 3) / 4 - 2;
+	if (mode == 2) return (q + 1) / 2 - 29;
+	return (q - 5 >= 0) ? (q - 5 + 1) / 2 : -((5 - q) / 2);
+}
+int rho_host_sd(int dim, int splitEpsMode);  // getRho<float,double> with the host libm
+
+// A slice product P (exact in binary64) as the binary32 term of the reference's float sums.  The
+// <float,double> Fsum takes c = (float)P of the product of the SCALED slices and adds
+// scalbn1<float>(c, seA + seB) = (float)scalbn((double)c, seA + seB): the product rounds to 24
+// significant bits first, and only then, if it lies below 2^-126, onto the binary32 subnormal
+// grid.  The slices here are unscaled, so the first rounding is done on the bits of P.  Above
+// 2^-126 (and for every product that is itself a binary32 value, as in the <float,float>
+// instantiation) this is the plain conversion.
+__host__ __device__ inline float f32_term(double p) {
+	if (!(fabs(p) < 0x1p-126)) return (float)p;
+	uint64_t u = __builtin_bit_cast(uint64_t, p);
+	const uint64_t low = u & 0x1FFFFFFFull, half = 0x10000000ull;
+	u -= low;
+	if (low > half || (low == half && ((u >> 29) & 1))) u += 0x20000000ull;
+	return (float)__builtin_bit_cast(double, u);
+}
+// term of a sum in T from a product accumulated in V (V = T, or V = double for T = float)
+template <class T>
+__host__ __device__ inline T sum_term(double p);
+template <>
+__host__ __device__ inline double sum_term<double>(double p) { return p; }
+template <>
+__host__ __device__ inline float sum_term<float>(double p) { return f32_term(p); }
+__host__ __device__ inline float sum_term_f(float p) { return p; }
+
 // host image of the digit statistics of one operand (ExpStats of oz_digits.hpp)
 struct ExpStatsH {
 	int maxS, minLsb, maxE, nonfinite, maxSpan, maxNorm, pad[2];

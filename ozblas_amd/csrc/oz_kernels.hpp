@@ -27,13 +27,14 @@ void launch_split_vec_pass(hipStream_t st, int m, const double *src, double *rem
 void launch_fill(hipStream_t st, double *p, int64_t n, double v);
 void launch_fill(hipStream_t st, float *p, int64_t n, float v);
 void launch_copy(hipStream_t st, const double *src, double *dst, int64_t n);
-// mixed = 1: the rho of the mixed split (binary64 data, binary32 slices)
+// mixed = 1: the rho of the mixed split (binary64 data into binary32 slices, or binary32 data
+// into binary64 slices)
 void launch_split_csr(hipStream_t st, int m, const double *val, const int *rowptr, double *work,
                       double *split, int64_t lds, int shift, int cap, const uint16_t *J, int *zrow,
                       int *zmax, int epsMode, int mixed = 0);
 void launch_split_csr(hipStream_t st, int m, const float *val, const int *rowptr, float *work,
                       double *split, int64_t lds, int shift, int cap, const uint16_t *JF, int *zrow,
-                      int *zmax, int epsMode);
+                      int *zmax, int epsMode, int mixed = 0);
 void launch_zero_slots_csr(hipStream_t st, int m, const int *rowptr, double *split, int64_t lds,
                            const int *zrow, int count);
 void launch_digit_stats(hipStream_t st, int m, int n, const double *in, int64_t ldi, int *colE,
@@ -162,14 +163,15 @@ void launch_csrmm_sum(hipStream_t st, int m, int nnz, const double *As, int64_t 
 void launch_csrmm_store(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
                         const int *colind, const int *rowptr, const double *Xs, int64_t ldxs,
                         int nB, int maxlevel, double *Cs, int64_t ldcs);
-// binary32 (slices stored as doubles; sums, y and Cs in float; JF2 for NearSum<float>)
+// binary32 (slices stored as doubles; sums, y and Cs in float; JF2 for NearSum<float>); mixed = 1:
+// the pair products are binary64 chains (the <float,double> instantiation), not binary32 ones
 void launch_csrmm_sum(hipStream_t st, int m, int nnz, const double *As, int64_t ldas, int nA,
                       const int *colind, const int *rowptr, const double *Xs, int64_t ldxs, int nB,
                       int maxlevel, int fsum, float alpha, float beta, float *y,
-                      const uint16_t *JF2);
+                      const uint16_t *JF2, int mixed);
 void launch_csrmm_store(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
                         const int *colind, const int *rowptr, const double *Xs, int64_t ldxs,
-                        int nB, int maxlevel, float *Cs, int64_t ldcs);
+                        int nB, int maxlevel, float *Cs, int64_t ldcs, int mixed);
 // GEMV of the plain form on k-contiguous slices (slice s of A column i at As + s*sA + i*ldk, of x
 // at Xs + s*sX), fused with the sum (Fsum in level order or NearSum) and the epilogue into y;
 // false when the slice counts exceed its register bound (8 x 8)
@@ -281,10 +283,12 @@ struct VsRef {
 void launch_csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
                           const int *colind, const int *rowptr, VsRef X, int fsum, double *y,
                           double *Cs, int64_t ldc, const uint16_t *J, CgDev *S, int it, VsPre *preY);
-// binary32 p and q (JF2: the binary32 NearSum table); p's slices capped at 8 by maxVecSlices
+// binary32 p and q (JF2: the binary32 NearSum table); p's slices capped at 8 by maxVecSlices;
+// mixed = 1: binary64 pair chains (<float,double>)
 void launch_csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
                           const int *colind, const int *rowptr, VsRef X, int fsum, float *y,
-                          double *Cs, int64_t ldc, const uint16_t *JF2, CgDev *S, int it, VsPre *preY);
+                          double *Cs, int64_t ldc, const uint16_t *JF2, CgDev *S, int it, VsPre *preY,
+                          int mixed = 0);
 // part: dots_part_bytes() of scratch for the per-workgroup pair partials (no initial value)
 size_t dots_part_bytes();
 // DOT of two split vectors (binary32 inputs with f32), completed by `mode` (0: C[0] = the DOT)

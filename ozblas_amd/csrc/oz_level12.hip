@@ -1063,13 +1063,14 @@ __device__ inline float dot_pairs_sum_f(int nvals, float *vals, int nA, int nB, 
 	return nearsum_f(nvals, v, JF2);
 }
 
-// rounds vals[0, nvals) to float in place: float p occupies the bytes of double p / 2, which
-// ascending p has already read
+// rounds vals[0, nvals) to float in place (f32_term: the term of the <float,float> and the
+// <float,double> Fsum alike): float p occupies the bytes of double p / 2, which ascending p has
+// already read
 __device__ inline float *vals_to_float(int nvals, double *vals) {
 	float *f = (float *)vals;
 	for (int p = 0; p < nvals; p++) {
 		const double v = vals[p];
-		f[p] = (float)v;
+		f[p] = f32_term(v);
 	}
 	return f;
 }
@@ -1096,19 +1097,21 @@ void launch_nearsum_small(hipStream_t st, int nvals, double *vals, double alpha,
 // over the pairs with ia + ib <= maxlevel (or Fsum in (level, ia, ib) order), then
 // y = fma(alpha, t, beta * y).
 // ---------------------------------------------------------------------------------------------
-// T = float: the binary32 instantiation -- each pair sum is the fma1<float> chain
-// (float)fma(a, x, (double)v) of csrmm_n<float>, then NearSum<float> (J = JF2) or Fsum in float
+// T = float: the binary32 instantiations -- with V = float each pair sum is the fma1<float> chain
+// (float)fma(a, x, (double)v) of csrmm_n<float> (<float,float>), with V = double the binary64 chain
+// of csrmm_n<double> (<float,double>), whose sum then enters the float sum as f32_term; then
+// NearSum<float> (J = JF2) or Fsum in float.
 // With OTF the slices of x are generated from x and its plan (gen) instead of read from Xs.
-template <int NAM, int NBM, class T = double, class TI = double, bool OTF = false>
+template <int NAM, int NBM, class T = double, class TI = double, bool OTF = false, class V = T>
 __device__ __forceinline__ T csrmm_row_sum(int r, const double *As, int64_t ldas, int nA,
                                            const int *colind, const int *rowptr, const double *Xs,
                                            int64_t ldxs, int nB, int maxlevel, int fsum,
                                            const uint16_t *J, const TI *X = nullptr,
                                            const VsGen<TI, NBM> *gen = nullptr) {
 	const int b = rowptr[r], e = rowptr[r + 1];
-	RegVec<NAM * NBM, T> v;
+	RegVec<NAM * NBM, V> v;
 #pragma unroll
-	for (int q = 0; q < NAM * NBM; q++) v[q] = (T)0;
+	for (int q = 0; q < NAM * NBM; q++) v[q] = (V)0;
 	// nonzeros in chunks of CH: the chunk's column indices, then its A and x loads, all in flight
 	// together, then the fma chains in row order (the products of one pair stay in sequence).  With
 	// OTF every load is unconditional, at an index clamped into the row and the slice counts, and the
@@ -1159,9 +1162,17 @@ __device__ __forceinline__ T csrmm_row_sum(int r, const double *As, int64_t ldas
 			for (int ia = 0; ia < NAM; ia++)
 #pragma unroll
 				for (int ib = 0; ib < NBM; ib++)
-					v[ia * NBM + ib] = (T)fma(a[q][ia], x[q][ib], (double)v[ia * NBM + ib]);
+					v[ia * NBM + ib] = (V)fma(a[q][ia], x[q][ib], (double)v[ia * NBM + ib]);
 		}
 	}
+	// a pair sum as a term of the sum in T (the binary64 chain of <float,double> rounds as the
+	// reference's (float) of the scaled product)
+	auto term = [](V p) -> T {
+		if constexpr (sizeof(V) == sizeof(T))
+			return p;
+		else
+			return sum_term<T>(p);
+	};
 	T t;
 	if (fsum) {
 		t = (T)0;
@@ -1170,7 +1181,7 @@ __device__ __forceinline__ T csrmm_row_sum(int r, const double *As, int64_t ldas
 #pragma unroll
 			for (int ia = 0; ia < NAM; ia++) {
 				const int ib = ik - ia;
-				if (ib >= 0 && ib < NBM && ia < nA && ib < nB && ik <= maxlevel) t = t + v[ia * NBM + ib];
+				if (ib >= 0 && ib < NBM && ia < nA && ib < nB && ik <= maxlevel) t = t + term(v[ia * NBM + ib]);
 			}
 	} else {
 		// NearSum over the products (its result does not depend on the order of the terms):
@@ -1188,7 +1199,7 @@ __device__ __forceinline__ T csrmm_row_sum(int r, const double *As, int64_t ldas
 				const bool ok = ia < nA && ib < nB && ia + ib <= maxlevel;
 #pragma unroll
 				for (int q = 0; q < NAM * NBM; q++)
-					if (ok && q == cnt) u[q] = v[ia * NBM + ib];
+					if (ok && q == cnt) u[q] = term(v[ia * NBM + ib]);
 				cnt += ok ? 1 : 0;
 			}
 		if constexpr (sizeof(T) == 4)
@@ -1199,7 +1210,7 @@ __device__ __forceinline__ T csrmm_row_sum(int r, const double *As, int64_t ldas
 	return t;
 }
 
-template <int NAM, int NBM, class T>
+template <int NAM, int NBM, class T, class V = T>
 __global__ __launch_bounds__(256) void k_csrmm_sum(int m, const double *As, int64_t ldas, int nA,
                                                    const int *colind, const int *rowptr,
                                                    const double *Xs, int64_t ldxs, int nB,
@@ -1207,28 +1218,28 @@ __global__ __launch_bounds__(256) void k_csrmm_sum(int m, const double *As, int6
                                                    const uint16_t *J) {
 	const int r = blockIdx.x * blockDim.x + threadIdx.x;
 	if (r >= m) return;
-	const T t = csrmm_row_sum<NAM, NBM, T>(r, As, ldas, nA, colind, rowptr, Xs, ldxs, nB, maxlevel,
-	                                       fsum, J);
+	const T t = csrmm_row_sum<NAM, NBM, T, double, false, V>(r, As, ldas, nA, colind, rowptr, Xs, ldxs,
+	                                                         nB, maxlevel, fsum, J);
 	y[r] = fma1_t(alpha, t, beta * y[r]);
 }
 
-template <class T>
+template <class T, class V = T>
 static void csrmm_sum(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
                       const int *colind, const int *rowptr, const double *Xs, int64_t ldxs, int nB,
                       int maxlevel, int fsum, T alpha, T beta, T *y, const uint16_t *J) {
 	// register bounds as tight as the counts allow (the per-row sum works on NAM x NBM values)
 	const dim3 g((m + 255) / 256), b(256);
 	if (nA <= 2 && nB <= 4)
-		hipLaunchKernelGGL((k_csrmm_sum<2, 4, T>), g, b, 0, st, m, As, ldas, nA, colind, rowptr, Xs,
+		hipLaunchKernelGGL((k_csrmm_sum<2, 4, T, V>), g, b, 0, st, m, As, ldas, nA, colind, rowptr, Xs,
 		                   ldxs, nB, maxlevel, fsum, alpha, beta, y, J);
 	else if (nA <= 2)
-		hipLaunchKernelGGL((k_csrmm_sum<2, 8, T>), g, b, 0, st, m, As, ldas, nA, colind, rowptr, Xs,
+		hipLaunchKernelGGL((k_csrmm_sum<2, 8, T, V>), g, b, 0, st, m, As, ldas, nA, colind, rowptr, Xs,
 		                   ldxs, nB, maxlevel, fsum, alpha, beta, y, J);
 	else if (nB <= 4)
-		hipLaunchKernelGGL((k_csrmm_sum<4, 4, T>), g, b, 0, st, m, As, ldas, nA, colind, rowptr, Xs,
+		hipLaunchKernelGGL((k_csrmm_sum<4, 4, T, V>), g, b, 0, st, m, As, ldas, nA, colind, rowptr, Xs,
 		                   ldxs, nB, maxlevel, fsum, alpha, beta, y, J);
 	else
-		hipLaunchKernelGGL((k_csrmm_sum<4, 8, T>), g, b, 0, st, m, As, ldas, nA, colind, rowptr, Xs,
+		hipLaunchKernelGGL((k_csrmm_sum<4, 8, T, V>), g, b, 0, st, m, As, ldas, nA, colind, rowptr, Xs,
 		                   ldxs, nB, maxlevel, fsum, alpha, beta, y, J);
 }
 
@@ -1243,9 +1254,14 @@ void launch_csrmm_sum(hipStream_t st, int m, int nnz, const double *As, int64_t 
 void launch_csrmm_sum(hipStream_t st, int m, int nnz, const double *As, int64_t ldas, int nA,
                       const int *colind, const int *rowptr, const double *Xs, int64_t ldxs, int nB,
                       int maxlevel, int fsum, float alpha, float beta, float *y,
-                      const uint16_t *JF2) {
+                      const uint16_t *JF2, int mixed) {
 	(void)nnz;
-	csrmm_sum(st, m, As, ldas, nA, colind, rowptr, Xs, ldxs, nB, maxlevel, fsum, alpha, beta, y, JF2);
+	if (mixed)
+		csrmm_sum<float, double>(st, m, As, ldas, nA, colind, rowptr, Xs, ldxs, nB, maxlevel, fsum, alpha,
+		                         beta, y, JF2);
+	else
+		csrmm_sum<float>(st, m, As, ldas, nA, colind, rowptr, Xs, ldxs, nB, maxlevel, fsum, alpha, beta, y,
+		                 JF2);
 }
 
 // Dot2 CSRMV (csrmm_n_x2, src/cpu/ozblas_blas_wrapper.cpp:262-295, then the global sum over
@@ -1402,7 +1418,7 @@ __global__ __launch_bounds__(256) void k_gemv_slices(int M, int K, const double 
 #pragma unroll
 				for (int a = 0; a < NAM; a++) {
 					const int b = ik - a;
-					if (b >= 0 && b < NBM && a < nA && b < nB) t = t + (T)acc[r][a][b];
+					if (b >= 0 && b < NBM && a < nA && b < nB) t = t + sum_term<T>(acc[r][a][b]);
 				}
 		} else {
 			RegVec<NAM * NBM, T> u;
@@ -1416,7 +1432,7 @@ __global__ __launch_bounds__(256) void k_gemv_slices(int M, int K, const double 
 					const bool ok = a < nA && b < nB;
 #pragma unroll
 					for (int q = 0; q < NAM * NBM; q++)
-						if (ok && q == cnt) u[q] = (T)acc[r][a][b];
+						if (ok && q == cnt) u[q] = sum_term<T>(acc[r][a][b]);
 					cnt += ok ? 1 : 0;
 				}
 			if constexpr (sizeof(T) == 4)
@@ -1620,8 +1636,9 @@ bool launch_split_gemv(hipStream_t st, int K, int M, const double *in, int64_t l
 }
 
 // general case: store the products where blasRcsrmm puts them (column ic of CSplit, ld ldcs);
-// T = float: the fma1<float> chain of csrmm_n<float>
-template <class T>
+// T = float: the fma1<float> chain of csrmm_n<float> (V = float), or the binary64 chain of
+// csrmm_n<double> stored as its float term (V = double, <float,double>)
+template <class T, class V = T>
 __global__ void k_csrmm_store(int m, const double *As, int64_t ldas, int nA, const int *colind,
                               const int *rowptr, const double *Xs, int64_t ldxs, int nB,
                               int maxlevel, T *Cs, int64_t ldcs) {
@@ -1633,10 +1650,13 @@ __global__ void k_csrmm_store(int m, const double *As, int64_t ldas, int nA, con
 	for (int ia = 0; ia < na; ia++) {
 		const int numB = (nB < maxlevel + 1 - ia) ? nB : maxlevel + 1 - ia;
 		for (int ib = 0; ib < numB; ib++) {
-			T tr = (T)0;
+			V tr = (V)0;
 			for (int jj = b; jj < e; jj++)
-				tr = (T)fma(As[(int64_t)ia * ldas + jj], Xs[(int64_t)ib * ldxs + colind[jj]], (double)tr);
-			Cs[(int64_t)(col + ib) * ldcs + r] = tr;
+				tr = (V)fma(As[(int64_t)ia * ldas + jj], Xs[(int64_t)ib * ldxs + colind[jj]], (double)tr);
+			if constexpr (sizeof(V) == sizeof(T))
+				Cs[(int64_t)(col + ib) * ldcs + r] = tr;
+			else
+				Cs[(int64_t)(col + ib) * ldcs + r] = sum_term<T>(tr);
 		}
 		col += numB;
 	}
@@ -1651,9 +1671,13 @@ void launch_csrmm_store(hipStream_t st, int m, const double *As, int64_t ldas, i
 
 void launch_csrmm_store(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
                         const int *colind, const int *rowptr, const double *Xs, int64_t ldxs,
-                        int nB, int maxlevel, float *Cs, int64_t ldcs) {
-	hipLaunchKernelGGL(k_csrmm_store<float>, dim3((m + 255) / 256), dim3(256), 0, st, m, As, ldas, nA,
-	                   colind, rowptr, Xs, ldxs, nB, maxlevel, Cs, ldcs);
+                        int nB, int maxlevel, float *Cs, int64_t ldcs, int mixed) {
+	if (mixed)
+		hipLaunchKernelGGL((k_csrmm_store<float, double>), dim3((m + 255) / 256), dim3(256), 0, st, m, As,
+		                   ldas, nA, colind, rowptr, Xs, ldxs, nB, maxlevel, Cs, ldcs);
+	else
+		hipLaunchKernelGGL(k_csrmm_store<float>, dim3((m + 255) / 256), dim3(256), 0, st, m, As, ldas, nA,
+		                   colind, rowptr, Xs, ldxs, nB, maxlevel, Cs, ldcs);
 }
 
 // ozblasRaxpy with reproMode != 0 (src/cpu/ozblas_axpy.cpp:142-144): y = fma1(alpha, x, y)
@@ -1791,7 +1815,7 @@ __device__ inline double csrmm_big_row(int r, const double *As, int64_t ldas, in
 // ldc).  The reductions of q's split accumulate into preY.
 // T = float: the <float,float> CG -- p, q binary32, the pair sums and NearSum in float (J = JF2); the
 // host caps p's slices at NBM for it (more overflow to the synchronous path).
-template <int NAM, int NBM, class T>
+template <int NAM, int NBM, class T, class V = T>
 __global__ __launch_bounds__(256) void k_csrmm_sum_dev(int m, const double *As, int64_t ldas, int nA,
                                                        const int *colind, const int *rowptr, VsRef X,
                                                        int fsum, T *y, double *Cs, int64_t ldc,
@@ -1814,12 +1838,13 @@ __global__ __launch_bounds__(256) void k_csrmm_sum_dev(int m, const double *As, 
 		if (nB > NBM) {
 			if constexpr (sizeof(T) == 8) t = csrmm_big_row(r, As, ldas, nA, colind, rowptr, X, nB, fsum, Cs, ldc, J);
 		} else if (X.plan->bad) {
-			t = csrmm_row_sum<NAM, NBM, T>(r, As, ldas, nA, colind, rowptr, X.slices, X.lds, nB, maxlevel, fsum, J);
+			t = csrmm_row_sum<NAM, NBM, T, double, false, V>(r, As, ldas, nA, colind, rowptr, X.slices, X.lds, nB,
+			                                                 maxlevel, fsum, J);
 		} else {
 			VsGen<T, NBM> g;
 			g.init(X.plan);
-			t = csrmm_row_sum<NAM, NBM, T, T, true>(r, As, ldas, nA, colind, rowptr, nullptr, 0, nB, maxlevel, fsum,
-			                                        J, (const T *)X.x, &g);
+			t = csrmm_row_sum<NAM, NBM, T, T, true, V>(r, As, ldas, nA, colind, rowptr, nullptr, 0, nB, maxlevel,
+			                                           fsum, J, (const T *)X.x, &g);
 		}
 		const T v = fma1_t((T)1, t, (T)0 * y[r]);
 		y[r] = v;
@@ -2189,16 +2214,16 @@ __global__ __launch_bounds__(256) void k_scal_axpy_dev(int n, CgDev *S, int it, 
 	vs_stats_flush<256>(mx, lo, bad, preP);
 }
 
-template <class T>
+template <class T, class V = T>
 static void csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas, int nA, const int *colind,
                           const int *rowptr, VsRef X, int fsum, T *y, double *Cs, int64_t ldc, const uint16_t *J,
                           CgDev *S, int it, VsPre *preY) {
 	constexpr int kNbReg = 8;
 	if (nA <= 2)  // the matrix's count is known here; p's count only on the device
-		hipLaunchKernelGGL((k_csrmm_sum_dev<2, kNbReg, T>), dim3((m + 255) / 256), dim3(256), 0, st, m, As,
+		hipLaunchKernelGGL((k_csrmm_sum_dev<2, kNbReg, T, V>), dim3((m + 255) / 256), dim3(256), 0, st, m, As,
 		                   ldas, nA, colind, rowptr, X, fsum, y, Cs, ldc, J, S, it, preY);
 	else
-		hipLaunchKernelGGL((k_csrmm_sum_dev<4, kNbReg, T>), dim3((m + 255) / 256), dim3(256), 0, st, m, As,
+		hipLaunchKernelGGL((k_csrmm_sum_dev<4, kNbReg, T, V>), dim3((m + 255) / 256), dim3(256), 0, st, m, As,
 		                   ldas, nA, colind, rowptr, X, fsum, y, Cs, ldc, J, S, it, preY);
 }
 
@@ -2210,8 +2235,12 @@ void launch_csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas,
 
 void launch_csrmm_sum_dev(hipStream_t st, int m, const double *As, int64_t ldas, int nA,
                           const int *colind, const int *rowptr, VsRef X, int fsum, float *y,
-                          double *Cs, int64_t ldc, const uint16_t *JF2, CgDev *S, int it, VsPre *preY) {
-	csrmm_sum_dev(st, m, As, ldas, nA, colind, rowptr, X, fsum, y, Cs, ldc, JF2, S, it, preY);
+                          double *Cs, int64_t ldc, const uint16_t *JF2, CgDev *S, int it, VsPre *preY,
+                          int mixed) {
+	if (mixed)
+		csrmm_sum_dev<float, double>(st, m, As, ldas, nA, colind, rowptr, X, fsum, y, Cs, ldc, JF2, S, it, preY);
+	else
+		csrmm_sum_dev<float>(st, m, As, ldas, nA, colind, rowptr, X, fsum, y, Cs, ldc, JF2, S, it, preY);
 }
 
 size_t dots_part_bytes() { return sizeof(double) * kDotPairs * kDotMaxWG; }

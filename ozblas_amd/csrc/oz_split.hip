@@ -221,8 +221,10 @@ __global__ void k_split_csr(int m, const TI *val, const int *rowptr, TI *work, d
 	const int r = blockIdx.x * blockDim.x + threadIdx.x;
 	if (r >= m) return;
 	const int b = rowptr[r], e = rowptr[r + 1];
-	// the mixed (binary32-slice) split of binary64 data has its own rho, the rest is the same
-	const int rho = mixed ? rho_dev_ds(e - b, epsMode) : SplitRule<TI>::rho_row(e - b, epsMode);
+	// the mixed splits (binary64 data into binary32 slices, binary32 data into binary64 slices)
+	// have their own rho, the rest is the same
+	const int rho = !mixed ? SplitRule<TI>::rho_row(e - b, epsMode)
+	                       : (sizeof(TI) == 8 ? rho_dev_ds(e - b, epsMode) : rho_dev_sd(e - b, epsMode));
 	double mx = 0.0;
 	for (int i = b; i < e; i++) mx = std_max(mx, fabs((double)val[i]));
 	int s = 0;
@@ -423,9 +425,9 @@ void launch_split_csr(hipStream_t st, int m, const double *val, const int *rowpt
 
 void launch_split_csr(hipStream_t st, int m, const float *val, const int *rowptr, float *work,
                       double *split, int64_t lds, int shift, int cap, const uint16_t *JF, int *zrow,
-                      int *zmax, int epsMode) {
+                      int *zmax, int epsMode, int mixed) {
 	hipLaunchKernelGGL(k_split_csr<float>, dim3((m + 255) / 256), dim3(256), 0, st, m, val, rowptr,
-	                   work, split, lds, shift, cap, JF, zrow, zmax, epsMode, 0);
+	                   work, split, lds, shift, cap, JF, zrow, zmax, epsMode, mixed);
 }
 
 void launch_zero_slots_csr(hipStream_t st, int m, const int *rowptr, double *split, int64_t lds,

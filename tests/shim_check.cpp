@@ -212,6 +212,87 @@ static int run_mixed() {
 	return 0;
 }
 
+// the mixed routines: binary32 data, binary64-held slices (templates at <float,double,...>)
+static int run_mixed_sd() {
+	cuozblasHandle_t h1, h2;
+	cuozblasCreate(&h1, 256ull << 20);
+	if (ozblasCreate(&h2, 256ull << 20) != 0) return 2;
+	h1.sumMode = h2.sumMode = 0;  // Fsum: the only sum of the mixed routines
+	const int m = 37, n = 29, k = 153;
+	{
+		const int lda = m + 3, ldb = n + 1, ldc = m + 2;
+		Dev<float> A(rand_vec<float>((size_t)lda * k, 4.0)), B(rand_vec<float>((size_t)ldb * k, 0.5));
+		const std::vector<float> C0 = rand_vec<float>((size_t)ldc * n, 1.0);
+		Dev<float> C1(C0), C2(C0);
+		const int32_t s1 = cuozblasRgemm<float, double, double>(&h1, 'n', 't', m, n, k, 1.5f, A.p, lda, B.p,
+		                                                        ldb, -0.25f, C1.p, ldc);
+		const int32_t s2 = ozblasSDgemm(&h2, 'n', 't', m, n, k, 1.5f, A.p, lda, B.p, ldb, -0.25f, C2.p, ldc);
+		report("SD", "gemm", same(C1.get(), C2.get()) && !same(C1.get(), C0), s1, s2);
+	}
+	{
+		Dev<float> A(rand_vec<float>((size_t)k * n, 2.0)), x(rand_vec<float>(k, 1.0));
+		const std::vector<float> y0 = rand_vec<float>(n, 1.0);
+		Dev<float> y1(y0), y2(y0);
+		const int32_t s1 =
+		    cuozblasRgemv<float, double, double>(&h1, 't', k, n, 0.75f, A.p, k, x.p, 1, 2.0f, y1.p, 1);
+		const int32_t s2 = ozblasSDgemv(&h2, 't', k, n, 0.75f, A.p, k, x.p, 1, 2.0f, y2.p, 1);
+		report("SD", "gemv", same(y1.get(), y2.get()), s1, s2);
+	}
+	{
+		const int len = 1001;
+		Dev<float> x(rand_vec<float>(len, 3.0)), y(rand_vec<float>(len, 1.0));
+		float r1 = 0, r2 = 0;
+		int32_t s1 = cuozblasRdot<float, double, double>(&h1, len, x.p, 1, y.p, 1, &r1);
+		int32_t s2 = ozblasSDdot(&h2, len, x.p, 1, y.p, 1, &r2);
+		report("SD", "dot", std::memcmp(&r1, &r2, 4) == 0, s1, s2);
+		s1 = cuozblasRnrm2<float, double, double>(&h1, len, x.p, 1, &r1);
+		s2 = ozblasSDnrm2(&h2, len, x.p, 1, &r2);
+		report("SD", "nrm2", std::memcmp(&r1, &r2, 4) == 0 && r1 > 0, s1, s2);
+	}
+	{
+		const int nn = 300;
+		std::vector<int> rp(nn + 1), ci;
+		std::vector<float> val;
+		for (int r = 0; r < nn; r++) {
+			rp[r] = (int)ci.size();
+			for (int c = r - 1; c <= r + 1; c++) {
+				if (c < 0 || c >= nn) continue;
+				ci.push_back(c);
+				val.push_back(c == r ? 2.001f : -1.0f);
+			}
+		}
+		rp[nn] = (int)ci.size();
+		const int nnz = (int)val.size();
+		Dev<float> A(val), x(rand_vec<float>(nn, 1.0)), b(rand_vec<float>(nn, 1.0));
+		Dev<int> CI(ci), RP(rp);
+		const std::vector<float> y0 = rand_vec<float>(nn, 1.0);
+		Dev<float> y1(y0), y2(y0);
+		int32_t s1 = cuozblasRcsrmv<float, double>(&h1, 'n', nn, nn, nnz, 1.5f, nullptr, A.p, CI.p, RP.p, x.p,
+		                                           -0.5f, y1.p);
+		int32_t s2 = ozblasSDcsrmv(&h2, 'n', nn, nn, nnz, 1.5f, nullptr, A.p, CI.p, RP.p, x.p, -0.5f, y2.p);
+		report("SD", "csrmv", same(y1.get(), y2.get()), s1, s2);
+		double *sp1 = cuozblasRcsrmvSplitA<float, double>(&h1, 'n', nn, nn, nnz, nullptr, A.p, RP.p);
+		double *sp2 = ozblasSDcsrmvSplitA(&h2, 'n', nn, nn, nnz, nullptr, A.p, RP.p);
+		Dev<float> z1(y0), z2(y0);
+		s1 = sp1 ? cuozblasRcsrmv<float, double>(&h1, 'n', nn, nn, nnz, 1.0f, nullptr, (const float *)sp1,
+		                                         CI.p, RP.p, x.p, 0.0f, z1.p)
+		         : h1.lastStatus;
+		s2 = sp2 ? ozblasSDcsrmv(&h2, 'n', nn, nn, nnz, 1.0f, nullptr, (const float *)sp2, CI.p, RP.p, x.p,
+		                         0.0f, z2.p)
+		         : h2.lastStatus;
+		h1.memMaskSplitA = h2.memMaskSplitA = 0;
+		report("SD", "csrmvSplitA", sp1 && sp2 && same(z1.get(), z2.get()), s1, s2);
+		const std::vector<float> x0(nn, 0.0f);
+		Dev<float> xc1(x0), xc2(x0);
+		s1 = cuozblasRcg<float, double>(&h1, 'n', nn, nnz, nullptr, A.p, CI.p, RP.p, b.p, xc1.p, 25, 0.0f);
+		s2 = ozblasSDcg(&h2, 'n', nn, nnz, nullptr, A.p, CI.p, RP.p, b.p, xc2.p, 25, 0.0f);
+		report("SD", "cg", same(xc1.get(), xc2.get()) && h1.cg_numiter == 25, s1, s2);
+	}
+	cuozblasDestroy(&h1);
+	ozblasDestroy(&h2);
+	return 0;
+}
+
 template <class P>
 static int run() {
 	using T = typename P::T;
@@ -297,7 +378,7 @@ static int run() {
 }
 
 int main() {
-	if (run<CD>() != 0 || run<CS>() != 0 || run_mixed() != 0) return 2;
+	if (run<CD>() != 0 || run<CS>() != 0 || run_mixed() != 0 || run_mixed_sd() != 0) return 2;
 	if (g_fail) {
 		printf("shim check: FAILED\n");
 		return 1;

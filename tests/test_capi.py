@@ -151,6 +151,54 @@ def test_mixed_rho_rules_match_libm(lib, oracle):
             assert ozblas_amd.internal_rho_row_ds(d, mode) == want, (mode, d)
 
 
+def test_f32_term_is_the_scaled_double_rounding(lib):
+    """f32_term(P) of an unscaled product equals the reference's route through the scaled product:
+    c = (float)(P * 2^-e), then (float)scalbn((double)c, e) -- a rounding to 24 bits, then onto the
+    binary32 subnormal grid below 2^-126.  Checked against numpy's float32 conversions with the
+    scale e chosen to bring P into the normal range, including ties of either rounding."""
+    import ozblas_amd
+    rng = np.random.default_rng(11)
+    mant = rng.integers(2 ** 52, 2 ** 53, 3000)
+    # ties and near-ties of the 24-bit rounding and of the subnormal grid
+    mant[:200] = (mant[:200] >> 29 << 29) | (1 << 28)
+    mant[200:400] = (mant[200:400] >> 30 << 30) | (1 << 29)
+    mant[400:600] = (mant[400:600] >> 29 << 29) | (1 << 28) | 1
+    exps = rng.integers(-160, 40, mant.size)
+    exps[:600] = rng.integers(-150, -124, 600)
+    # the double-rounding case: half an ulp of the subnormal grid plus less than half an ulp of
+    # the 24-bit one (a single rounding goes up, the reference's two go to even)
+    for q in range(600, 800):
+        e = int(rng.integers(-148, -127))
+        drop = 53 - (e + 150)  # bits below the subnormal grid
+        top = int(rng.integers(2 ** 52, 2 ** 53)) >> (drop + 1) << (drop + 1)
+        mant[q] = top | (1 << (drop - 1)) | int(rng.integers(1, 2 ** 27))
+        exps[q] = e
+    sign = np.where(rng.random(mant.size) < 0.5, -1.0, 1.0)
+    p = sign * np.ldexp(mant.astype(np.float64), exps - 52)
+    # the constructed cases do differ from a single rounding
+    assert sum(np.float32(v) != np.float32(np.ldexp(np.float64(np.float32(np.ldexp(v, 60))), -60))
+               for v in p[600:800]) > 50
+    for v in list(p) + [0.0, -0.0, 2.0 ** -126, -(2.0 ** -149), 2.0 ** -150, 3 * 2.0 ** -151, np.inf]:
+        e = 0 if v == 0 or not np.isfinite(v) else int(np.floor(np.log2(abs(v))))
+        c = np.float32(np.ldexp(v, -e)) if np.isfinite(v) else np.float32(v)
+        want = np.float32(np.ldexp(np.float64(c), e))
+        got = np.float32(ozblas_amd.internal_f32_term(float(v)))
+        assert got.tobytes() == want.tobytes(), (float(v).hex(), float(got).hex(), float(want).hex())
+
+
+def test_mixed_sd_rho_rules_match_libm(lib, oracle):
+    """getRho<float,double> (binary32 data, binary64-held slices) for splitEpsMode 0, 2 and 9: the
+    host form and the integer form of the CSR rows agree with the oracle's libm evaluation."""
+    import ozblas_amd
+    dims = list(range(0, 5000)) + [2 ** p + o for p in range(12, 31) for o in (-1, 0, 1)]
+    dims += [10 ** 6, 123456789, 2 ** 31 - 1]
+    for mode in (0, 2, 9):
+        for d in dims:
+            want = oracle.sd_rho(d, mode)
+            assert ozblas_amd.internal_rho_sd(d, mode) == want, (mode, d)
+            assert ozblas_amd.internal_rho_row_sd(d, mode) == want, (mode, d)
+
+
 def test_gen_phi_matches_oracle_generator(oracle):
     import ozblas_amd
     for phi, seed in ((1.0, 1), (0.5, 2), (4.0, 7)):
@@ -238,7 +286,7 @@ def test_certified_rounding_from_leading_levels(lib):
 
 def test_cpp_shim_binds_every_entry_point():
     """The cuOzBLAS C++ templates of include/cuozblas_mi355x.hpp compile (the check program built
-    by build()) and resolve to the library's D and S entry points."""
+    by build()) and resolve to the library's D, S, DS and SD entry points."""
     import subprocess
     exe = os.path.join(ROOT, "ozblas_amd", "_build", "shim_check")
     if not os.path.exists(exe):
@@ -246,6 +294,6 @@ def test_cpp_shim_binds_every_entry_point():
     out = subprocess.run(["nm", "-D", "--undefined-only", exe], capture_output=True, text=True,
                          check=True).stdout
     used = set(re.findall(r"\b(ozblas\w+)\b", out))
-    for p in ("D", "S", "DS"):
+    for p in ("D", "S", "DS", "SD"):
         for r in ("dot", "nrm2", "gemv", "gemm", "csrmv", "csrmvSplitA", "cg"):
             assert f"ozblas{p}{r}" in used

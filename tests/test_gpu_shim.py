@@ -22,4 +22,4 @@ def test_cpp_shim_matches_c_entry_points():
     say(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "shim check: all ok" in r.stdout
-    assert r.stdout.count(" ok ") == 23
+    assert r.stdout.count(" ok ") == 30
