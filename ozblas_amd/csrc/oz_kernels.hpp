@@ -90,6 +90,8 @@ constexpr int kResMaxMod = 49;   // 256 and the 48 odd pairwise coprime moduli u
 constexpr int kResMaxLimb = 18;  // 32-bit limbs of M (<= 392 bits) plus headroom
 constexpr int kResMaxShift = 512;  // element shifts below 2^512 (256^a mod p for a < 64)
 constexpr int kResMaxGroups = (kResMaxMod + 2) / 3;
+constexpr int kResChunkBits = 39;  // binary64 chunks of the reconstruction: 49 x 255 x 2^39 < 2^53
+constexpr int kResMaxChunk = (64 * 9 + kResChunkBits - 1) / kResChunkBits;  // up to 9 64-bit words
 constexpr int kResChunk = 65536;   // k per exact int32 product chunk (|acc| <= 2^16 2^14 = 2^30)
 // product residues: plane c * nmod + t holds modulus t over k chunk c
 struct ResConst {
@@ -106,6 +108,9 @@ struct ResConst {
 	double Pg[kResMaxGroups];
 	double invPg[kResMaxGroups];  // 1 / P_g rounded to binary64
 	uint32_t pow2g[kResMaxGroups][kResMaxShift];
+	// M / p and M in kResChunkBits-bit chunks (chunk c holds bits [39 c, 39 c + 39)), as binary64
+	double Mpd[kResMaxMod][kResMaxChunk];
+	double Md[kResMaxChunk];
 };
 // the moduli list and per-count constants (host, built once); log2 of the product of the first n
 const ResConst &res_const_host(int nmod);

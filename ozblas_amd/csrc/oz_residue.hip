@@ -13,7 +13,9 @@
 //     gives X mod p_t;
 //   - the reconstruction: c_t = (X mod p_t) (M/p_t)^-1 mod p_t, X = sum_t c_t M/p_t - K M with
 //     K = round(sum_t c_t / p_t) (exact, since |X| <= M/4 keeps the sum a quarter away from the
-//     rounding boundary), in a multi-word integer, then one RN-even of X 2^(lo_i + lo_j).
+//     rounding boundary), summed exactly in binary64 chunks of 39 bits (c_t < 2^8, so every chunk
+//     sum of up to 49 moduli stays below 2^53), carried into a multi-word integer, then one RN-even
+//     of X 2^(lo_i + lo_j).
 // The number of GEMMs is the number of moduli, about (spanA + spanB + log2 k + 2) / 7.9, against
 // about (spanA / 8) (spanB / 8) / 2 digit-pair GEMMs of the radix-256 digit form.
 #include "oz_residue.hpp"
@@ -58,6 +60,16 @@ static void limb_mul_small(std::vector<uint32_t> &a, uint32_t s) {
 	}
 	if (carry) a.push_back((uint32_t)carry);
 }
+// bits [39 c, 39 c + 39) of a limb integer, as a binary64 (exact)
+static double limb_chunk(const std::vector<uint32_t> &a, int c) {
+	uint64_t v = 0;
+	for (int b = 0; b < kResChunkBits; b++) {
+		const int bit = kResChunkBits * c + b;
+		const size_t q = (size_t)(bit / 32);
+		if (q < a.size() && ((a[q] >> (bit % 32)) & 1u)) v |= uint64_t(1) << b;
+	}
+	return (double)v;
+}
 static uint32_t limb_mod_small(const std::vector<uint32_t> &a, uint32_t p) {
 	uint64_t r = 0;
 	for (size_t q = a.size(); q-- > 0;) r = ((r << 32) | a[q]) % p;
@@ -87,6 +99,7 @@ static void build_res_consts() {
 		c.nl = (bitsNM + 1 + 31) / 32;
 		if (c.nl > kResMaxLimb) c.nl = kResMaxLimb;
 		for (size_t q = 0; q < M.size() && q < (size_t)kResMaxLimb; q++) c.M[q] = M[q];
+		for (int ch = 0; ch < kResMaxChunk; ch++) c.Md[ch] = limb_chunk(M, ch);
 		for (int t = 0; t < nmod; t++) {
 			const int p = ms[t];
 			c.p[t] = p;
@@ -96,6 +109,7 @@ static void build_res_consts() {
 			for (int s = 0; s < nmod; s++)
 				if (s != t) limb_mul_small(Mp, (uint32_t)ms[s]);
 			for (size_t q = 0; q < Mp.size() && q < (size_t)kResMaxLimb; q++) c.Mp[t][q] = Mp[q];
+			for (int ch = 0; ch < kResMaxChunk; ch++) c.Mpd[t][ch] = limb_chunk(Mp, ch);
 			const uint32_t w = limb_mod_small(Mp, (uint32_t)p);
 			int u = 0;
 			for (int v = 1; v < p; v++)
@@ -198,10 +212,14 @@ void launch_res_write(hipStream_t st, int elem, int m, int n, const double *in, 
 }
 
 template <int NW>
-__global__ __launch_bounds__(256) void k_res_recon(int M, int N, const uint8_t *R, int64_t ldr,
-                                                   int64_t rstride, int nchunk, const ResConst *rc,
-                                                   int nmod, const int *rowLo, const int *colLo,
-                                                   double alpha, double beta, double *C, int64_t ldc) {
+// (C written only here, the inputs only read: __restrict__ lets the constants come through the
+// scalar cache)
+__global__ __launch_bounds__(256) void k_res_recon(int M, int N, const uint8_t *__restrict__ R, int64_t ldr,
+                                                   int64_t rstride, int nchunk,
+                                                   const ResConst *__restrict__ rc, int nmod,
+                                                   const int *__restrict__ rowLo, const int *__restrict__ colLo,
+                                                   double alpha, double beta, double *__restrict__ C,
+                                                   int64_t ldc) {
 	const int G = (N + 3) / 4;
 	const int64_t total = (int64_t)M * G;
 	for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;

@@ -8,51 +8,35 @@
 namespace oz {
 
 // ---- device: residues of one element ------------------------------------------------------
-// x = +-f 2^e (f odd, or 0) relative to the column unit 2^lo: x 2^-lo = +-f 2^sh with sh >= 0
+// x 2^-lo = sf 2^sh with sf a signed integer below 2^53 in magnitude and sh >= 0: sh is the exponent
+// of x's last place relative to lo (clamped at 0), sf = x 2^-(lo + sh) exactly.  lo is at most the
+// lowest set bit of every element of the row or column, so sf is an integer either way, and sh is at
+// most the shift of x's lowest set bit (below the operand's span).  A zero x gives sf = 0, sh = 0.
 struct ResElem {
-	double fd;  // f as a binary64 (exact: f < 2^53)
+	double sf;
 	int sh;
-	int neg;
 };
 
 __device__ inline ResElem res_elem(double x, int lo) {
-	ResElem r{0.0, 0, 0};
-	if (x == 0.0) return r;
-	const uint64_t b = d2u(x);
-	const int E = (int)((b >> 52) & 0x7ff);
-	uint64_t f = b & ((uint64_t(1) << 52) - 1);
-	int e0;
-	if (E == 0) {
-		e0 = -1074;
-	} else {
-		f |= (uint64_t(1) << 52);
-		e0 = E - 1075;
-	}
-	const int tz = __builtin_ctzll(f);
-	f >>= tz;
-	r.fd = (double)f;
-	r.sh = e0 + tz - lo;  // in [0, span) by the choice of lo
-	r.neg = (int)(b >> 63);
-	return r;
+	const int Eb = (int)((d2u(x) >> 52) & 0x7ff);  // biased exponent; 0 for zero and subnormals
+	int sh = (Eb > 1 ? Eb : 1) - 1075 - lo;         // the unit of x's last place is 2^(max(Eb,1) - 1075)
+	sh = sh > 0 ? sh : 0;
+	return ResElem{ldexp(x, -(lo + sh)), sh};
 }
 
-// Residues by groups of three moduli.  First r = (+-f mod P) (2^sh mod P) mod P in binary64, P the
-// group's product (< 2^24): f < 2^53 and the product of two values below 2^24 stay exact, the
-// quotient estimates are off by at most one (|error| <= 2 f 2^-53 / P < 1).  Then each member's
-// balanced residue from r < 2^24 in binary32: in [-(p-1)/2, (p-1)/2] for odd p, [-128, 127] for 256.
+// Residues by groups of three moduli: (sf 2^sh) mod P in binary64, P the group's product (< 2^24).
+// sf mod P is only reduced to [-P, 2P) (the floor estimate is off by at most one: |error| <=
+// 2 |sf| 2^-53 / P < 1); times 2^sh mod P that stays exact (|x| < 2^25 2^24), and the second
+// reduction is made canonical, [0, P).  The sign rides on sf, so every member's residue comes out
+// of res_member already signed.
 __device__ __forceinline__ int res_group(const ResElem &e, double P, double invP, const uint32_t *pw2) {
-	if (e.fd == 0.0) return 0;
-	double q = floor(e.fd * invP);
-	double r = fma(-q, P, e.fd);
-	r += (r < 0.0) ? P : 0.0;
-	r -= (r >= P) ? P : 0.0;
-	const double x = r * (double)pw2[e.sh];  // < 2^48: exact
+	double q = floor(e.sf * invP);
+	const double r0 = fma(-q, P, e.sf);              // exact, in [-P, 2P)
+	const double x = r0 * (double)pw2[e.sh];         // exact, |x| < 2^49
 	q = floor(x * invP);
-	r = fma(-q, P, x);
+	double r = fma(-q, P, x);                        // exact, in [-P, 2P)
 	r += (r < 0.0) ? P : 0.0;
 	r -= (r >= P) ? P : 0.0;
-	// the sign: (-v) mod P, so that every member's residue comes out of res_member already signed
-	if (e.neg && r != 0.0) r = P - r;
 	return (int)r;
 }
 
@@ -120,67 +104,94 @@ __device__ inline void res_write_tile_i8(int m, int n, const double *in, int64_t
 	}
 }
 
-// v mod p for 0 <= v < 2^24 (binary32 quotient estimate off by at most one)
+// v mod p for 0 <= v < 2^24 (binary32 quotient estimate off by at most one; q p with a 24-bit
+// multiply: q < 2^24 / p, p <= 256)
 __device__ __forceinline__ int mod_f(int v, int p, float invpf) {
 	const float q = floorf((float)v * invpf);
-	int r = v - (int)q * p;
+	int r = v - (int)__umul24((uint32_t)(int)q, (uint32_t)p);
 	r += (r < 0) ? p : 0;
 	r -= (r >= p) ? p : 0;
 	return r;
 }
 
+// v mod p for 0 <= v < 2^16, p <= 256: the binary32 quotient estimate is exact but where v / p is an
+// integer (its error, below (v / p) 2^-23 < 2^-11, is smaller than the 1/p any other v / p keeps from
+// the next integer), and there it can fall one short: a single correction
+__device__ __forceinline__ int mod_f16(int v, int p, float invpf) {
+	const float q = floorf((float)v * invpf);
+	const int r = v - (int)__umul24((uint32_t)(int)q, (uint32_t)p);
+	return (r >= p) ? r - p : r;
+}
+
 // The four elements (i, 4g .. 4g + 3) of one product-residue word: X of each from its residues
 // (nmod moduli, nchunk k-chunk planes per modulus summed modulo p), then t[e] = RN(X 2^e0[e]).
+// c_t = r_t u_t mod p_t < 2^8; X = sum_t c_t M/p_t - K M is summed in binary64 chunks of 39 bits
+// (every product c_t chunk < 2^47, every chunk sum of up to 49 moduli < 2^53: exact in any order),
+// K = rint(sum_t c_t / p_t), and the chunks minus K M's are carried into NW 64-bit words.  The words
+// of 4 moduli are loaded together (clamped indices; the surplus ones are not used).
 // Rw addresses the word in plane 0; plane b is rstride bytes further.
 template <int NW>
 __device__ inline void res_group_values(const uint8_t *Rw, int64_t rstride, int nchunk,
                                         const ResConst *rc, int nmod, const int *e0, double *t) {
-	constexpr int NL = 2 * NW;
-	int64_t acc[4][NL];
+	constexpr int NC = (64 * NW + kResChunkBits - 1) / kResChunkBits;
+	constexpr int QB = 4;
+	double A[4][NC];
 	double S[4];
 #pragma unroll
 	for (int e = 0; e < 4; e++) {
 		S[e] = 0.0;
 #pragma unroll
-		for (int l = 0; l < NL; l++) acc[e][l] = 0;
+		for (int c = 0; c < NC; c++) A[e][c] = 0.0;
 	}
-	for (int q = 0; q < nmod; q++) {
-		const int p = rc->p[q];
-		const float ipf = rc->invpf[q];
-		const int u = rc->u[q];
-		const double ipd = rc->invpd[q];
-		int rs[4] = {0, 0, 0, 0};
+	for (int q0 = 0; q0 < nmod; q0 += QB) {
+		int rs[QB][4];
+#pragma unroll
+		for (int j = 0; j < QB; j++)
+#pragma unroll
+			for (int e = 0; e < 4; e++) rs[j][e] = 0;
 		for (int c = 0; c < nchunk; c++) {
-			const uint32_t w = *(const uint32_t *)(Rw + (int64_t)(c * nmod + q) * rstride);
+			uint32_t w[QB];
 #pragma unroll
-			for (int e = 0; e < 4; e++) rs[e] += (int)((w >> (8 * e)) & 255);
+			for (int j = 0; j < QB; j++) {
+				const int q = min(q0 + j, nmod - 1);
+				w[j] = *(const uint32_t *)(Rw + (int64_t)(c * nmod + q) * rstride);
+			}
+#pragma unroll
+			for (int j = 0; j < QB; j++)
+#pragma unroll
+				for (int e = 0; e < 4; e++) rs[j][e] += (int)((w[j] >> (8 * e)) & 255);
 		}
-		uint32_t mp[NL];
 #pragma unroll
-		for (int l = 0; l < NL; l++) mp[l] = rc->Mp[q][l];
+		for (int j = 0; j < QB; j++) {
+			const int q = q0 + j;
+			if (q >= nmod) break;
+			const int p = rc->p[q];
+			const float ipf = rc->invpf[q];
+			const uint32_t u = (uint32_t)rc->u[q];
+			const double ipd = rc->invpd[q];
+			double mp[NC];
 #pragma unroll
-		for (int e = 0; e < 4; e++) {
-			const int r = (nchunk > 1) ? mod_f(rs[e], p, ipf) : rs[e];
-			const int ct = mod_f(r * u, p, ipf);  // < 2^16 before the reduction
-			S[e] += (double)ct * ipd;
+			for (int c = 0; c < NC; c++) mp[c] = rc->Mpd[q][c];
 #pragma unroll
-			for (int l = 0; l < NL; l++) acc[e][l] += (int64_t)((uint64_t)(uint32_t)ct * mp[l]);
+			for (int e = 0; e < 4; e++) {
+				const int r = (nchunk > 1) ? mod_f(rs[j][e], p, ipf) : rs[j][e];  // rs <= 255 nchunk < 2^24
+				const double ct = (double)mod_f16((int)__umul24((uint32_t)r, u), p, ipf);
+				S[e] = fma(ct, ipd, S[e]);
+#pragma unroll
+				for (int c = 0; c < NC; c++) A[e][c] = fma(ct, mp[c], A[e][c]);
+			}
 		}
 	}
 #pragma unroll
 	for (int e = 0; e < 4; e++) {
-		const int64_t K = (int64_t)rint(S[e]);
+		const double K = rint(S[e]);
 		BigInt<NW> X;
-		int64_t carry = 0;
 #pragma unroll
-		for (int q = 0; q < NW; q++) {
-			const int64_t v0 = acc[e][2 * q] - K * (int64_t)rc->M[2 * q] + carry;
-			const uint64_t w0 = (uint32_t)v0;
-			carry = v0 >> 32;
-			const int64_t v1 = acc[e][2 * q + 1] - K * (int64_t)rc->M[2 * q + 1] + carry;
-			const uint64_t w1 = (uint32_t)v1;
-			carry = v1 >> 32;
-			X.w[q] = w0 | (w1 << 32);
+		for (int q = 0; q < NW; q++) X.w[q] = 0;
+#pragma unroll
+		for (int c = 0; c < NC; c++) {
+			const double d = fma(-K, rc->Md[c], A[e][c]);  // exact: |K M_c| < 2^45, |d| < 2^53
+			big_add_shifted<NW>(X, (int64_t)d, kResChunkBits * c);
 		}
 		t[e] = big_finish<NW>(X, e0[e]);
 	}

@@ -5,9 +5,11 @@ integers:
   - the element residues (res_group: (+-f mod P) (2^sh mod P) mod P in binary64 for the product P of
     three moduli; res_member: the balanced residue of that value modulo each member in binary32),
     restated in IEEE arithmetic (numpy), against (+-f 2^sh) mod p;
-  - the reconstruction (res_elem_value: c_t = r_t u_t mod p_t, K = rint(sum c_t / p_t) in
-    binary64, X = sum c_t M/p_t - K M), restated the same way, against X itself for X at 0, +-1,
-    +-M/4 and random values in between: the claim that |X| <= M/4 makes K exact.
+  - the element decomposition (res_elem: x 2^-lo = sf 2^sh) against exact rationals;
+  - the reconstruction (res_group_values: c_t = r_t u_t mod p_t, K = rint(sum c_t / p_t) in
+    binary64, X = sum c_t M/p_t - K M in binary64 chunks of 39 bits, each sum checked exact),
+    restated the same way, against X itself for X at 0, +-1, +-M/4 and random values in between:
+    the claim that |X| <= M/4 makes K exact.
 A plausible mistake (a wrong inverse, a dropped limb, K rounded from the wrong side, a residue
 taken modulo the wrong power of two) fails one of them.  No GPU call is made."""
 import ctypes
@@ -22,13 +24,16 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "ozblas_amd", "libozblas_mi355x.so")
 MAXMOD, MAXLIMB, MAXSH = 49, 18, 512
 NGROUP = (MAXMOD + 2) // 3
+CHUNK = 39
+MAXCHUNK = (64 * 9 + CHUNK - 1) // CHUNK
 
 
 def _dtype():
     return np.dtype([("nmod", "<i4"), ("nl", "<i4"), ("p", "<i4", MAXMOD), ("invpf", "<f4", MAXMOD),
                      ("invpd", "<f8", MAXMOD), ("u", "<i4", MAXMOD), ("Mp", "<u4", (MAXMOD, MAXLIMB)),
                      ("M", "<u4", MAXLIMB), ("pow256", "u1", (MAXMOD, MAXSH // 8)),
-                     ("Pg", "<f8", NGROUP), ("invPg", "<f8", NGROUP), ("pow2g", "<u4", (NGROUP, MAXSH))],
+                     ("Pg", "<f8", NGROUP), ("invPg", "<f8", NGROUP), ("pow2g", "<u4", (NGROUP, MAXSH)),
+                     ("Mpd", "<f8", (MAXMOD, MAXCHUNK)), ("Md", "<f8", MAXCHUNK)],
                     align=True)
 
 
@@ -72,33 +77,45 @@ def test_moduli_and_constants(rc):
         # X and sum_t c_t M/p_t < nmod M fit the carried limbs with a sign bit
         assert 32 * int(c["nl"]) >= (nmod * M).bit_length() + 1
         assert [int(w) for w in c["M"]] == limbs(M, MAXLIMB)
+        assert [int(v) for v in c["Md"]] == [(M >> (CHUNK * q)) & ((1 << CHUNK) - 1) for q in range(MAXCHUNK)]
         for t in range(nmod):
             p = ps[t]
             assert int(c["p"][t]) == p
             assert [int(w) for w in c["Mp"][t]] == limbs(M // p, MAXLIMB)
+            assert [int(v) for v in c["Mpd"][t]] == [((M // p) >> (CHUNK * q)) & ((1 << CHUNK) - 1)
+                                                     for q in range(MAXCHUNK)]
             assert (int(c["u"][t]) * (M // p)) % p == 1
             assert c["invpd"][t] == np.float64(1.0) / np.float64(p)
             assert c["invpf"][t] == np.float32(1.0) / np.float32(p)
             assert [int(v) for v in c["pow256"][t]] == [pow(256, a, p) for a in range(MAXSH // 8)]
 
 
-def res_group(fd, sh, neg, P, invP, pw2):
-    """res_group restated in binary64 (numpy rounds each operation like the device; the fma is exact)."""
-    if fd == 0:
-        return 0
-    fd = np.float64(fd)
-    q = np.floor(fd * invP)
-    r = np.float64(fd - q * P)  # fma(-q, P, f): exact, q P < 2^53
-    r = r + P if r < 0 else r
-    r = r - P if r >= P else r
-    x = np.float64(r * np.float64(pw2[sh]))
-    assert x < 2.0 ** 53
+def fma_d(a, b, c):
+    from fractions import Fraction as Fr
+    return float(Fr(float(a)) * Fr(float(b)) + Fr(float(c)))
+
+
+def res_elem(x, lo):
+    """res_elem restated: x 2^-lo = sf 2^sh, sh the exponent of x's last place relative to lo."""
+    eb = (np.float64(x).view(np.uint64) >> np.uint64(52)) & np.uint64(0x7FF)
+    sh = max(max(int(eb), 1) - 1075 - lo, 0)
+    return float(np.ldexp(np.float64(x), -(lo + sh))), sh
+
+
+def res_group(sf, sh, P, invP, pw2):
+    """res_group restated in binary64 (numpy rounds each operation like the device; the fmas are
+    exact): sf mod P left in [-P, 2P), times 2^sh mod P, then reduced to [0, P)."""
+    sf = np.float64(sf)
+    q = np.floor(sf * invP)
+    r0 = fma_d(-q, P, sf)  # one rounding of an integer below 2^25: exact
+    assert -P <= r0 < 2 * P and r0 == int(sf) - int(q) * int(P)
+    x = np.float64(r0 * float(pw2[sh]))
+    assert abs(float(x)) < 2.0 ** 49 and float(x) == r0 * float(pw2[sh])
     q = np.floor(x * invP)
-    r = np.float64(float(x) - float(q) * float(P))  # exact: |x - q P| < 2 P
+    r = fma_d(-q, P, x)
+    assert -P <= r < 2 * P and r == int(x) - int(q) * int(P)
     r = r + P if r < 0 else r
     r = r - P if r >= P else r
-    if neg and r != 0:
-        r = P - r
     return int(r)
 
 
@@ -140,7 +157,7 @@ def test_element_residues_against_integers(rc):
         for f, sh in cases:
             for neg in (0, 1):
                 v = (-1 if neg else 1) * f * pow(2, sh, int(P))
-                rx = res_group(f, sh, neg, P, c["invPg"][g], c["pow2g"][g])
+                rx = res_group((-1 if neg else 1) * f, sh, P, c["invPg"][g], c["pow2g"][g])
                 assert rx == v % int(P), (P, f, sh, neg)
                 for t in range(3 * g, min(3 * g + 3, MAXMOD)):
                     p = int(c["p"][t])
@@ -150,19 +167,50 @@ def test_element_residues_against_integers(rc):
                     assert -128 <= got <= 127
 
 
+def test_element_decomposition_against_integers():
+    """res_elem: x 2^-lo == sf 2^sh exactly, sf an integer below 2^53, 0 <= sh <= the shift of x's
+    lowest set bit -- for normal, subnormal and zero x and every lo up to x's lowest set bit."""
+    rng = np.random.default_rng(17)
+    xs = [0.0, -0.0, 1.0, -3.0, 2.0 ** -1074, -(2.0 ** -1074) * 12345, 2.0 ** -1022, 1.5 * 2.0 ** 1000]
+    xs += list(rng.standard_normal(200) * np.exp2(rng.integers(-1060, 1000, 200)))
+    for x in xs:
+        if x == 0.0:
+            lows = [0, -1074, 5]
+        else:
+            m, e = math.frexp(abs(x))  # x = m 2^e
+            mi = int(m * 2 ** 53)
+            low = e - 53 + ((mi & -mi).bit_length() - 1)  # exponent of the lowest set bit
+            lows = [low, low - 1, low - 60, max(low - 500, -1074), -1074]
+        for lo in lows:
+            sf, sh = res_elem(x, lo)
+            assert sf == int(sf) and abs(sf) < 2.0 ** 53 and sh >= 0
+            from fractions import Fraction as Fr
+            assert Fr(x) * Fr(2) ** (-lo) == Fr(sf) * Fr(2) ** sh, (x, lo)
+            if x != 0.0:
+                assert sh <= low - lo
+
+
 def reconstruct(c, nmod, X):
-    """res_elem_value restated: residues of X -> X, with K from the binary64 sum."""
+    """res_group_values restated: residues of X -> c_t, the binary64 chunk sums of c_t M/p_t (each
+    checked exact), K = rint of the fma sum of c_t / p_t, the chunks minus K M's, carried."""
     ps = [int(v) for v in c["p"][:nmod]]
-    M = math.prod(ps)
-    S = np.float64(0.0)
-    acc = 0
+    S = 0.0
+    A = [0.0] * MAXCHUNK
     for t in range(nmod):
         r = X % ps[t]  # the GEMM stores X mod p in [0, p)
         ct = (r * int(c["u"][t])) % ps[t]
-        S = np.float64(S + np.float64(ct) * c["invpd"][t])
-        acc += ct * (M // ps[t])
-    K = int(np.rint(S))
-    return acc - K * M
+        assert ct < 256
+        S = fma_d(ct, c["invpd"][t], S)
+        for q in range(MAXCHUNK):
+            A[q] = fma_d(ct, c["Mpd"][t][q], A[q])
+            assert A[q] == int(A[q]) and A[q] < 2.0 ** 53
+    K = float(np.rint(S))
+    out = 0
+    for q in range(MAXCHUNK):
+        d = fma_d(-K, c["Md"][q], A[q])
+        assert d == int(d) and abs(d) < 2.0 ** 53
+        out += int(d) << (CHUNK * q)
+    return out
 
 
 @pytest.mark.parametrize("nmod", [2, 5, 20, 21, 34, MAXMOD])
@@ -235,3 +283,37 @@ def test_short_chunk_epilogue_mod_against_integers(rc):
         near = [q * p + d for q in (rng.randrange(-top // p, top // p) for _ in range(40)) for d in (-1, 0, 1)]
         for v in vs + [x for x in near if abs(x) <= top]:
             assert mod_i32_short(v, p, c["invpf"][t]) == v % p, (p, v)
+
+
+def mod_f16(v, p, invpf):
+    """mod_f16 restated: floor(RN(RN(v) * invpf)) in binary32, then one correction downwards."""
+    q = int(np.floor(np.float32(np.float32(v) * invpf)))
+    r = v - q * p
+    assert 0 <= r <= p, (v, p, r)  # the bound the single correction relies on
+    return r - p if r >= p else r
+
+
+def mod_f(v, p, invpf):
+    """mod_f restated (0 <= v < 2^24): the same quotient, corrected either way."""
+    q = int(np.floor(np.float32(np.float32(v) * invpf)))
+    r = v - q * p
+    assert -p <= r < 2 * p, (v, p, r)
+    r = r + p if r < 0 else r
+    return r - p if r >= p else r
+
+
+def test_reconstruction_mods_against_integers(rc):
+    """c_t = r u mod p with the single-correction quotient for every v < 2^16 that is a multiple of p
+    or next to one (where the binary32 quotient can fall short), and random v; the chunk-sum
+    reduction for v < 2^24."""
+    c, _ = rc[MAXMOD]
+    rng = random.Random(19)
+    for t in range(MAXMOD):
+        p, ipf = int(c["p"][t]), c["invpf"][t]
+        vs = [q * p + d for q in range(0, (1 << 16) // p + 1) for d in (0, 1, p - 1)]
+        vs += [rng.randrange(0, 1 << 16) for _ in range(300)]
+        for v in vs:
+            if 0 <= v < (1 << 16):
+                assert mod_f16(v, p, ipf) == v % p, (p, v)
+        for v in [rng.randrange(0, 1 << 24) for _ in range(300)] + [(1 << 24) - 1, 255 * 32768]:
+            assert mod_f(v, p, ipf) == v % p, (p, v)
