@@ -726,9 +726,9 @@ static int32_t gemm_residue(ozblasHandle_t *oh, char transA, char transB, int m,
 	OZ_HIP(hipMemcpyAsync(dBins, bins.data(), sizeof(LevelBin) * bins.size(), hipMemcpyHostToDevice, st));
 	OZ_HIP(hipMemcpyAsync(dOrder, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, st));
 	OZ_HIP(hipEventRecord(b->ev[1], st));
-	launch_res_write(st, elem, k, m, ain, ldai, ra.colLo, dRc, nmod, ra.st.maxSpan, ra.planes, ra.ldp, ra.pstride);
+	launch_res_write(st, elem, k, m, ain, ldai, ra.colLo, dRc, nmod, ra.st.maxSpan, 1, ra.planes, ra.ldp, ra.pstride);
 	OZ_HIP(hipEventRecord(b->ev[2], st));
-	launch_res_write(st, elem, k, n, bin, ldbi, rb.colLo, dRc, nmod, rb.st.maxSpan, rb.planes, rb.ldp, rb.pstride);
+	launch_res_write(st, elem, k, n, bin, ldbi, rb.colLo, dRc, nmod, rb.st.maxSpan, 0, rb.planes, rb.ldp, rb.pstride);
 	OZ_HIP(hipEventRecord(b->ev[3], st));
 	launch_digit_gemm(st, elem, m, n, Kpad, ra.planes, ra.ldp, ra.pstride, rb.planes, rb.ldp,
 	                  rb.pstride, dPairs, dBins, nbins, dOrder, nullptr, ldt, tstride, nullptr, 0, dRc, Rp);

@@ -55,7 +55,9 @@ struct GWrite {  // tiled INT8 residue planes of one operand
 	int8_t *planes;
 	int64_t ldp, pstride;
 	int nmod, kts;
-	int shLimit, pad;  // 2^sh mod P tables staged for sh below the planned span
+	int shLimit;  // 2^sh mod P tables staged for sh below the planned span
+	int scaleU;   // op(A): residues times u_t, from the constants of the problem's moduli count
+	const ResConst *rcw;
 };
 struct GProb {  // one problem of the grouped MFMA launch
 	const int8_t *A, *B;
@@ -188,7 +190,8 @@ __global__ __launch_bounds__(256) void k_write_grouped(const GWrite *g, const in
 	__shared__ ResLds T;
 	const int o = owner(start, n, blockIdx.x);
 	const GWrite w = g[o];
-	res_lds_load(T, rc, w.nmod, w.shLimit);
+	(void)rc;
+	res_lds_load(T, w.rcw, w.nmod, w.shLimit, w.scaleU);
 	__syncthreads();
 	const int lb = blockIdx.x - start[o];
 	res_write_tile_i8(w.m, w.n, w.in, w.ldi, w.colLo, w.nmod, T, w.planes, w.ldp, w.pstride, lb % w.kts,
@@ -666,7 +669,8 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 				w.nmod = h.nmod;
 				w.kts = (int)(h.ldp / 64);
 				w.shLimit = std::max(1, p->hSt[o].maxSpan);
-				w.pad = 0;
+				w.scaleU = side == 0 ? 1 : 0;
+				w.rcw = (const ResConst *)(M2 + rcOff[h.nmod]);
 				writeStart.push_back(wb);
 				wb += w.kts * (int)(rup64(w.n, 256) / 16);
 				hWrite.push_back(w);

@@ -108,6 +108,10 @@ struct ResConst {
 	double Pg[kResMaxGroups];
 	double invPg[kResMaxGroups];  // 1 / P_g rounded to binary64
 	uint32_t pow2g[kResMaxGroups][kResMaxShift];
+	// (2^sh U_g) mod P_g with U_g = u_t mod p_t for the group's moduli t < nmod (1 beyond): the
+	// residues of op(A) written with it come out multiplied by u_t, so the product modulo p_t is
+	// already c_t = X u_t mod p_t
+	uint32_t pow2gU[kResMaxGroups][kResMaxShift];
 	// M / p and M in kResChunkBits-bit chunks (chunk c holds bits [39 c, 39 c + 39)), as binary64
 	double Mpd[kResMaxMod][kResMaxChunk];
 	double Md[kResMaxChunk];
@@ -123,9 +127,11 @@ void launch_digit_gemm(hipStream_t st, int elem, int M, int N, int Kpad, const v
                        const int *tileList = nullptr, int ntiles = 0, const ResConst *rc = nullptr,
                        uint8_t *R = nullptr);
 // residue planes of one operand (k-contiguous columns; INT8 tiled layout for elem 1, else
-// 2-byte k-contiguous planes), element x of column j as the integer x 2^-colLo[j] modulo each p
+// 2-byte k-contiguous planes), element x of column j as the integer x 2^-colLo[j] modulo each p,
+// times u_t = (M/p_t)^-1 mod p_t with scaleU (op(A): the products are then the c_t of the
+// reconstruction)
 void launch_res_write(hipStream_t st, int elem, int m, int n, const double *in, int64_t ldi,
-                      const int *colLo, const ResConst *rc, int nmod, int shLimit, void *planes,
+                      const int *colLo, const ResConst *rc, int nmod, int shLimit, int scaleU, void *planes,
                       int64_t ldp, int64_t pstride);
 // C = fma(alpha, RN(X 2^(rowLo[i] + colLo[j])), beta C), X from the residue planes (nchunk k-chunk
 // planes per modulus, summed modulo p)

@@ -10,8 +10,9 @@
 //   - each operand element is reduced modulo every p_t to a balanced residue in [-128, 127]
 //     (exact in int8, fp16 and bf16);
 //   - one INT8 GEMM per modulus (int32 accumulation is exact below 2^31: k <= 2^17 per chunk)
-//     gives X mod p_t;
-//   - the reconstruction: c_t = (X mod p_t) (M/p_t)^-1 mod p_t, X = sum_t c_t M/p_t - K M with
+//     gives X mod p_t -- or directly c_t = X u_t mod p_t, u_t = (M/p_t)^-1 mod p_t, since op(A)'s
+//     residues are written multiplied by u_t (through the group tables, at no cost);
+//   - the reconstruction: X = sum_t c_t M/p_t - K M with
 //     K = round(sum_t c_t / p_t) (exact, since |X| <= M/4 keeps the sum a quarter away from the
 //     rounding boundary), summed exactly in binary64 chunks of 39 bits (c_t < 2^8, so every chunk
 //     sum of up to 49 moduli stays below 2^53), carried into a multi-word integer, then one RN-even
@@ -131,9 +132,24 @@ static void build_res_consts() {
 			for (int t = 3 * gi; t < 3 * gi + 3 && t < avail; t++) P *= (uint64_t)ms[t];
 			c.Pg[gi] = (double)P;
 			c.invPg[gi] = 1.0 / (double)P;
+			// U = u_t mod p_t for the members t < nmod and 1 mod the others: the Chinese remainder
+			// sum over the members of v_t (P/p_t) ((P/p_t)^-1 mod p_t)
+			uint64_t U = 0;
+			for (int t = 3 * gi; t < 3 * gi + 3 && t < avail; t++) {
+				const uint64_t pt = (uint64_t)ms[t], Q = P / pt;
+				const uint64_t vt = (t < nmod) ? (uint64_t)c.u[t] : 1u;
+				uint64_t inv = 0;
+				for (uint64_t v = 1; v < pt; v++)
+					if ((Q % pt) * v % pt == 1) {
+						inv = v;
+						break;
+					}
+				U = (U + vt * Q % P * inv) % P;  // each factor below 2^24: no overflow
+			}
 			uint64_t pw2 = 1 % P;
 			for (int sh = 0; sh < kResMaxShift; sh++) {
 				c.pow2g[gi][sh] = (uint32_t)pw2;
+				c.pow2gU[gi][sh] = (uint32_t)(pw2 * U % P);
 				pw2 = (pw2 * 2) % P;
 			}
 		}
@@ -156,10 +172,10 @@ double res_log2_product(int nmod) {
 // 64 k-bytes of every modulus plane per step (4 bytes per thread), looping over k tiles.
 __global__ __launch_bounds__(256) void k_res_write_i8t(int m, int n, const double *in, int64_t ldi,
                                                        const int *colLo, const ResConst *rc, int nmod,
-                                                       int shLimit, int8_t *planes, int64_t ldp,
-                                                       int64_t pstride) {
+                                                       int shLimit, int scaleU, int8_t *planes,
+                                                       int64_t ldp, int64_t pstride) {
 	__shared__ ResLds T;
-	res_lds_load(T, rc, nmod, shLimit);
+	res_lds_load(T, rc, nmod, shLimit, scaleU);
 	__syncthreads();
 	const int ktiles = (int)(ldp / 64);
 	for (int kt = blockIdx.x; kt < ktiles; kt += gridDim.x)
@@ -171,9 +187,10 @@ __global__ __launch_bounds__(256) void k_res_write_i8t(int m, int n, const doubl
 template <typename T>
 __global__ __launch_bounds__(256) void k_res_write_h(int m, int n, const double *in, int64_t ldi,
                                                      const int *colLo, const ResConst *rc, int nmod,
-                                                     int shLimit, T *planes, int64_t ldp, int64_t pstride) {
+                                                     int shLimit, int scaleU, T *planes, int64_t ldp,
+                                                     int64_t pstride) {
 	__shared__ ResLds L;
-	res_lds_load(L, rc, nmod, shLimit);
+	res_lds_load(L, rc, nmod, shLimit, scaleU);
 	__syncthreads();
 	const int ng = (nmod + 2) / 3;
 	for (int j = blockIdx.x; j < n; j += gridDim.x) {
@@ -192,23 +209,23 @@ __global__ __launch_bounds__(256) void k_res_write_h(int m, int n, const double 
 }
 
 void launch_res_write(hipStream_t st, int elem, int m, int n, const double *in, int64_t ldi,
-                      const int *colLo, const ResConst *rc, int nmod, int shLimit, void *planes,
+                      const int *colLo, const ResConst *rc, int nmod, int shLimit, int scaleU, void *planes,
                       int64_t ldp, int64_t pstride) {
 	if (shLimit < 1) shLimit = 1;
 	if (elem == 1) {  // tiled layout; ldp % 128 == 0 and pstride = ldp * (n rounded up to 256)
 		const int ktiles = (int)(ldp / 64);
 		const int gx = ktiles < 16 ? ktiles : 16;
 		hipLaunchKernelGGL(k_res_write_i8t, dim3((unsigned)gx, (unsigned)(pstride / ldp / 16)), dim3(256), 0,
-		                   st, m, n, in, ldi, colLo, rc, nmod, shLimit, (int8_t *)planes, ldp, pstride);
+		                   st, m, n, in, ldi, colLo, rc, nmod, shLimit, scaleU, (int8_t *)planes, ldp, pstride);
 		return;
 	}
 	const int g = n < 2048 ? n : 2048;
 	if (elem == 2)
 		hipLaunchKernelGGL(k_res_write_h<_Float16>, dim3(g), dim3(256), 0, st, m, n, in, ldi, colLo, rc, nmod,
-		                   shLimit, (_Float16 *)planes, ldp, pstride);
+		                   shLimit, scaleU, (_Float16 *)planes, ldp, pstride);
 	else
 		hipLaunchKernelGGL(k_res_write_h<__bf16>, dim3(g), dim3(256), 0, st, m, n, in, ldi, colLo, rc, nmod,
-		                   shLimit, (__bf16 *)planes, ldp, pstride);
+		                   shLimit, scaleU, (__bf16 *)planes, ldp, pstride);
 }
 
 template <int NW>

@@ -61,9 +61,11 @@ struct ResLds {
 	float ipf[kResMaxMod];
 };
 
-__device__ inline void res_lds_load(ResLds &T, const ResConst *rc, int nmod, int shLimit) {
+// scaleU: the tables of op(A), whose residues carry the factors u_t (ResConst::pow2gU)
+__device__ inline void res_lds_load(ResLds &T, const ResConst *rc, int nmod, int shLimit, int scaleU) {
 	const int ng = (nmod + 2) / 3;
-	for (int q = threadIdx.x; q < ng * shLimit; q += blockDim.x) T.pw2[q / shLimit][q % shLimit] = rc->pow2g[q / shLimit][q % shLimit];
+	const uint32_t(*src)[kResMaxShift] = scaleU ? rc->pow2gU : rc->pow2g;
+	for (int q = threadIdx.x; q < ng * shLimit; q += blockDim.x) T.pw2[q / shLimit][q % shLimit] = src[q / shLimit][q % shLimit];
 	for (int q = threadIdx.x; q < ng; q += blockDim.x) {
 		T.Pg[q] = rc->Pg[q];
 		T.invPg[q] = rc->invPg[q];
@@ -114,18 +116,10 @@ __device__ __forceinline__ int mod_f(int v, int p, float invpf) {
 	return r;
 }
 
-// v mod p for 0 <= v < 2^16, p <= 256: the binary32 quotient estimate is exact but where v / p is an
-// integer (its error, below (v / p) 2^-23 < 2^-11, is smaller than the 1/p any other v / p keeps from
-// the next integer), and there it can fall one short: a single correction
-__device__ __forceinline__ int mod_f16(int v, int p, float invpf) {
-	const float q = floorf((float)v * invpf);
-	const int r = v - (int)__umul24((uint32_t)(int)q, (uint32_t)p);
-	return (r >= p) ? r - p : r;
-}
-
 // The four elements (i, 4g .. 4g + 3) of one product-residue word: X of each from its residues
 // (nmod moduli, nchunk k-chunk planes per modulus summed modulo p), then t[e] = RN(X 2^e0[e]).
-// c_t = r_t u_t mod p_t < 2^8; X = sum_t c_t M/p_t - K M is summed in binary64 chunks of 39 bits
+// The planes hold c_t = X u_t mod p_t < 2^8 (op(A) was written with the factors u_t);
+// X = sum_t c_t M/p_t - K M is summed in binary64 chunks of 39 bits
 // (every product c_t chunk < 2^47, every chunk sum of up to 49 moduli < 2^53: exact in any order),
 // K = rint(sum_t c_t / p_t), and the chunks minus K M's are carried into NW 64-bit words.  The words
 // of 4 moduli are loaded together (clamped indices; the surplus ones are not used).
@@ -167,15 +161,13 @@ __device__ inline void res_group_values(const uint8_t *Rw, int64_t rstride, int 
 			if (q >= nmod) break;
 			const int p = rc->p[q];
 			const float ipf = rc->invpf[q];
-			const uint32_t u = (uint32_t)rc->u[q];
 			const double ipd = rc->invpd[q];
 			double mp[NC];
 #pragma unroll
 			for (int c = 0; c < NC; c++) mp[c] = rc->Mpd[q][c];
 #pragma unroll
 			for (int e = 0; e < 4; e++) {
-				const int r = (nchunk > 1) ? mod_f(rs[j][e], p, ipf) : rs[j][e];  // rs <= 255 nchunk < 2^24
-				const double ct = (double)mod_f16((int)__umul24((uint32_t)r, u), p, ipf);
+				const double ct = (double)((nchunk > 1) ? mod_f(rs[j][e], p, ipf) : rs[j][e]);  // rs < 2^24
 				S[e] = fma(ct, ipd, S[e]);
 #pragma unroll
 				for (int c = 0; c < NC; c++) A[e][c] = fma(ct, mp[c], A[e][c]);

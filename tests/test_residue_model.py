@@ -6,7 +6,8 @@ integers:
     three moduli; res_member: the balanced residue of that value modulo each member in binary32),
     restated in IEEE arithmetic (numpy), against (+-f 2^sh) mod p;
   - the element decomposition (res_elem: x 2^-lo = sf 2^sh) against exact rationals;
-  - the reconstruction (res_group_values: c_t = r_t u_t mod p_t, K = rint(sum c_t / p_t) in
+  - the op(A) group tables with the factors u_t = (M/p_t)^-1 mod p_t folded in (pow2gU);
+  - the reconstruction (res_group_values: c_t = X u_t mod p_t, K = rint(sum c_t / p_t) in
     binary64, X = sum c_t M/p_t - K M in binary64 chunks of 39 bits, each sum checked exact),
     restated the same way, against X itself for X at 0, +-1, +-M/4 and random values in between:
     the claim that |X| <= M/4 makes K exact.
@@ -33,6 +34,7 @@ def _dtype():
                      ("invpd", "<f8", MAXMOD), ("u", "<i4", MAXMOD), ("Mp", "<u4", (MAXMOD, MAXLIMB)),
                      ("M", "<u4", MAXLIMB), ("pow256", "u1", (MAXMOD, MAXSH // 8)),
                      ("Pg", "<f8", NGROUP), ("invPg", "<f8", NGROUP), ("pow2g", "<u4", (NGROUP, MAXSH)),
+                     ("pow2gU", "<u4", (NGROUP, MAXSH)),
                      ("Mpd", "<f8", (MAXMOD, MAXCHUNK)), ("Md", "<f8", MAXCHUNK)],
                     align=True)
 
@@ -143,6 +145,17 @@ def test_group_tables(rc):
         P = math.prod(ps[3 * g: 3 * g + 3])
         assert P < 2 ** 24 and c["Pg"][g] == P and c["invPg"][g] == np.float64(1.0) / np.float64(P)
         assert [int(v) for v in c["pow2g"][g]] == [pow(2, sh, P) for sh in range(MAXSH)]
+    # the op(A) tables: 2^sh u_t modulo each member p_t < nmod, 2^sh modulo the members beyond
+    for nmod in (1, 2, 3, 7, 20, 23, 34, MAXMOD):
+        c, _ = rc[nmod]
+        for g in range(NGROUP):
+            P = math.prod(ps[3 * g: 3 * g + 3])
+            for sh in (0, 1, 7, 100, MAXSH - 1):
+                v = int(c["pow2gU"][g][sh])
+                assert v < P
+                for t in range(3 * g, min(3 * g + 3, MAXMOD)):
+                    ut = int(c["u"][t]) if t < nmod else 1
+                    assert v % ps[t] == (pow(2, sh, ps[t]) * ut) % ps[t], (nmod, g, t, sh)
 
 
 def test_element_residues_against_integers(rc):
@@ -191,14 +204,13 @@ def test_element_decomposition_against_integers():
 
 
 def reconstruct(c, nmod, X):
-    """res_group_values restated: residues of X -> c_t, the binary64 chunk sums of c_t M/p_t (each
+    """res_group_values restated: c_t = X u_t mod p_t, the binary64 chunk sums of c_t M/p_t (each
     checked exact), K = rint of the fma sum of c_t / p_t, the chunks minus K M's, carried."""
     ps = [int(v) for v in c["p"][:nmod]]
     S = 0.0
     A = [0.0] * MAXCHUNK
     for t in range(nmod):
-        r = X % ps[t]  # the GEMM stores X mod p in [0, p)
-        ct = (r * int(c["u"][t])) % ps[t]
+        ct = (X * int(c["u"][t])) % ps[t]  # the GEMM of op(A)'s u-scaled residues stores X u mod p
         assert ct < 256
         S = fma_d(ct, c["invpd"][t], S)
         for q in range(MAXCHUNK):
@@ -285,14 +297,6 @@ def test_short_chunk_epilogue_mod_against_integers(rc):
             assert mod_i32_short(v, p, c["invpf"][t]) == v % p, (p, v)
 
 
-def mod_f16(v, p, invpf):
-    """mod_f16 restated: floor(RN(RN(v) * invpf)) in binary32, then one correction downwards."""
-    q = int(np.floor(np.float32(np.float32(v) * invpf)))
-    r = v - q * p
-    assert 0 <= r <= p, (v, p, r)  # the bound the single correction relies on
-    return r - p if r >= p else r
-
-
 def mod_f(v, p, invpf):
     """mod_f restated (0 <= v < 2^24): the same quotient, corrected either way."""
     q = int(np.floor(np.float32(np.float32(v) * invpf)))
@@ -302,18 +306,15 @@ def mod_f(v, p, invpf):
     return r - p if r >= p else r
 
 
-def test_reconstruction_mods_against_integers(rc):
-    """c_t = r u mod p with the single-correction quotient for every v < 2^16 that is a multiple of p
-    or next to one (where the binary32 quotient can fall short), and random v; the chunk-sum
-    reduction for v < 2^24."""
+def test_reconstruction_chunk_sum_mod_against_integers(rc):
+    """The reduction of the k-chunk sums (up to 255 x 32768 < 2^24) modulo every p, random and near
+    multiples of p."""
     c, _ = rc[MAXMOD]
     rng = random.Random(19)
     for t in range(MAXMOD):
         p, ipf = int(c["p"][t]), c["invpf"][t]
-        vs = [q * p + d for q in range(0, (1 << 16) // p + 1) for d in (0, 1, p - 1)]
-        vs += [rng.randrange(0, 1 << 16) for _ in range(300)]
+        vs = [rng.randrange(0, 1 << 24) for _ in range(300)] + [(1 << 24) - 1, 255 * 32768]
+        vs += [q * p + d for q in (rng.randrange(0, (1 << 24) // p) for _ in range(50)) for d in (-1, 0, 1)]
         for v in vs:
-            if 0 <= v < (1 << 16):
-                assert mod_f16(v, p, ipf) == v % p, (p, v)
-        for v in [rng.randrange(0, 1 << 24) for _ in range(300)] + [(1 << 24) - 1, 255 * 32768]:
-            assert mod_f(v, p, ipf) == v % p, (p, v)
+            if 0 <= v < (1 << 24):
+                assert mod_f(v, p, ipf) == v % p, (p, v)
