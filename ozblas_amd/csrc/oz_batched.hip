@@ -139,30 +139,29 @@ __global__ __launch_bounds__(256) void k_transpose_grouped(const GTrans *g, cons
 	}
 }
 
-// kStatCols consecutive columns (of all operands, in order) per workgroup; the statistics of each
-// operand's columns are folded into its counters once per workgroup
-constexpr int kStatCols = 8;
+// kStatCols consecutive columns (of all operands, in order) per workgroup, kStatCols / 4 per wave;
+// each wave folds the statistics of each operand's columns into its counters once
+constexpr int kStatCols = 16;
 __global__ __launch_bounds__(256) void k_stats_grouped(const GStat *g, const int *start, int n,
                                                        int total) {
-	__shared__ int red[4];
-	__shared__ double redd[4];
-	const int c0 = blockIdx.x * kStatCols, c1 = min(c0 + kStatCols, total);
+	constexpr int per = kStatCols / 4;
+	const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+	const int c0 = blockIdx.x * kStatCols + w * per, c1 = min(c0 + per, total);
 	int cur = -1;
 	ColAcc acc;
 	GStat s{};
 	for (int c = c0; c < c1; c++) {
 		const int o = owner(start, n, c);
 		if (o != cur) {
-			if (cur >= 0 && threadIdx.x == 0) col_acc_flush(acc, s.st);
+			if (cur >= 0 && l == 0) col_acc_flush(acc, s.st);
 			acc = ColAcc();
 			cur = o;
 			s = g[o];
 		}
 		const int j = c - start[o];
-		digit_col_stats_acc<256>(s.in + (int64_t)j * s.ldi, s.m, s.colE + j, s.colS + j, acc, red, redd,
-		                         s.colLo + j);
+		digit_col_stats_wave(s.in + (int64_t)j * s.ldi, s.m, s.colE + j, s.colS + j, acc, s.colLo + j);
 	}
-	if (cur >= 0 && threadIdx.x == 0) col_acc_flush(acc, s.st);
+	if (cur >= 0 && l == 0) col_acc_flush(acc, s.st);
 }
 
 // flags a problem whose operands no longer satisfy what its plan was built for
@@ -184,18 +183,17 @@ __global__ void k_check_grouped(const GCheck *c, int n, const ExpStats *st, int 
 	}
 }
 
-// one workgroup per (operand, tile): the tables of the operand's moduli, staged up to its span
-__global__ __launch_bounds__(256) void k_write_grouped(const GWrite *g, const int *start, int n,
-                                                       const ResConst *rc) {
+// one workgroup per (operand, 16-column block), over every k tile of the block: the tables of the
+// operand's moduli, staged up to its span, serve the whole block
+__global__ __launch_bounds__(256) void k_write_grouped(const GWrite *g, const int *start, int n) {
 	__shared__ ResLds T;
 	const int o = owner(start, n, blockIdx.x);
 	const GWrite w = g[o];
-	(void)rc;
 	res_lds_load(T, w.rcw, w.nmod, w.shLimit, w.scaleU);
 	__syncthreads();
-	const int lb = blockIdx.x - start[o];
-	res_write_tile_i8(w.m, w.n, w.in, w.ldi, w.colLo, w.nmod, T, w.planes, w.ldp, w.pstride, lb % w.kts,
-	                  lb / w.kts);
+	const int jb = blockIdx.x - start[o];
+	for (int kt = 0; kt < w.kts; kt++)
+		res_write_tile_i8(w.m, w.n, w.in, w.ldi, w.colLo, w.nmod, T, w.planes, w.ldp, w.pstride, kt, jb);
 }
 
 // Persistent grouped MFMA launch: every workgroup takes the next (problem, bin, tile) item from
@@ -350,7 +348,7 @@ static int32_t launch_execute(BatchPlan *p, hipStream_t st) {
 		                   p->nCheck, p->dSt, p->dFlag);
 	if (p->nWriteBlocks > 0)
 		hipLaunchKernelGGL(k_write_grouped, dim3(p->nWriteBlocks), dim3(256), 0, st, p->dWrite,
-		                   p->dWriteStart, p->nWrite, p->dRcAll);
+		                   p->dWriteStart, p->nWrite);
 	const int c = p->tcfg;
 	if (p->nItems[c] > 0) {
 		if (c == 0)
@@ -672,7 +670,7 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 				w.scaleU = side == 0 ? 1 : 0;
 				w.rcw = (const ResConst *)(M2 + rcOff[h.nmod]);
 				writeStart.push_back(wb);
-				wb += w.kts * (int)(rup64(w.n, 256) / 16);
+				wb += (int)(rup64(w.n, 256) / 16);
 				hWrite.push_back(w);
 			}
 			for (int c = 0; c < 2; c++) {
@@ -784,7 +782,7 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 	// ---- autotune: digit planes once, then one grouped launch per tile shape -------------------
 	if (p->nWriteBlocks > 0)
 		hipLaunchKernelGGL(k_write_grouped, dim3(p->nWriteBlocks), dim3(256), 0, st, p->dWrite,
-		                   p->dWriteStart, p->nWrite, p->dRcAll);
+		                   p->dWriteStart, p->nWrite);
 	p->tcfg = 0;
 	if (p->nItems[0] > 0) {
 		for (int c = 0; c < 2; c++) {

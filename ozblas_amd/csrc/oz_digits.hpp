@@ -155,6 +155,73 @@ __device__ inline void digit_col_stats_acc(const double *x, int m, int *colE, in
 	}
 }
 
+// The same statistics of one column computed by one wave, with no barrier (the batched launch's
+// columns are at most a few thousand long): max, lowest set bit and the norm's sum of squares reduced
+// across the wave; lane 0 adds the column to `acc`.  The results are those of digit_col_stats_acc
+// (the sum of squares only in another order, inside the bound's rounding margin).
+__device__ inline int wave_max_i32(int v) {
+	for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+	return v;
+}
+__device__ inline void digit_col_stats_wave(const double *x, int m, int *colE, int *colS, ColAcc &acc,
+                                            int *colLo) {
+	const int l = threadIdx.x % 64;
+	int nlo = INT32_MIN, bad = 0;
+	double mx = 0.0;
+	for (int i0 = l; i0 < m; i0 += 8 * 64) {
+		double v[8];
+#pragma unroll
+		for (int u = 0; u < 8; u++) v[u] = (i0 + u * 64 < m) ? x[i0 + u * 64] : 0.0;
+#pragma unroll
+		for (int u = 0; u < 8; u++) {
+			if (!isfinite(v[u])) {
+				bad = 1;
+				continue;
+			}
+			if (v[u] == 0.0) continue;
+			mx = std_max(mx, fabs(v[u]));
+			nlo = max(nlo, -lsb_exp(v[u]));
+		}
+	}
+	mx = wave_max_nonneg_f64(mx);
+	nlo = wave_max_i32(nlo);
+	bad = wave_max_i32(bad);
+	double ss = 0.0;
+	int hiN = 0;
+	if (colLo && mx > 0.0 && !bad) {
+		frexp(mx, &hiN);
+		for (int i = l; i < m; i += 64) {
+			const double v = ldexp(x[i], -hiN);
+			ss += v * v;
+		}
+		ss = wave_sum_f64(ss);
+	}
+	if (l == 0) {
+		if (colLo && mx > 0.0 && !bad) {
+			const double up = sqrt((ss * (1.0 + m * 0x1p-52) + m * 0x1p-1000)) * (1.0 + 0x1p-50);
+			int eN;
+			frexp(up, &eN);
+			acc.maxNorm = max(acc.maxNorm, eN + hiN + nlo);
+		}
+		int E = 0, S = 0;
+		if (mx > 0.0) {
+			int hi;
+			frexp(mx, &hi);
+			E = (mx <= ldexp(127.0, hi - 7)) ? hi + 1 : hi + 2;
+			const int lo = -nlo;
+			S = (E - lo + 7) / 8;
+			acc.minLsb = min(acc.minLsb, lo);
+			acc.maxSpan = max(acc.maxSpan, hi - lo);
+		}
+		*colE = E;
+		*colS = S;
+		if (colLo) *colLo = (mx > 0.0) ? -nlo : 0;
+		acc.maxS = max(acc.maxS, S);
+		acc.maxE = max(acc.maxE, E);
+		if (bad) acc.bad = 1;
+	}
+}
+
 // the same for one column with its reductions folded into *st at once
 template <int BS>
 __device__ inline void digit_col_stats(const double *x, int m, int *colE, int *colS, ExpStats *st,
