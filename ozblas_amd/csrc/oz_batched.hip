@@ -240,16 +240,22 @@ __global__ __launch_bounds__(256, 1) void k_gemm_grouped(const GProb *P, const G
 	}
 }
 
+// kRecGroups column groups of 4 per thread (the problem lookup and its table read once for them)
+constexpr int kRecGroups = 4;
 template <int NW>
 __global__ __launch_bounds__(256) void k_recon_grouped(const GRec *g, const int *start, int n,
                                                        const int *flag) {
 	if (*flag) return;  // C stays untouched when the plan check failed
 	const int o = owner(start, n, blockIdx.x);
 	const GRec r = g[o];
-	const int64_t e = (int64_t)(blockIdx.x - start[o]) * 256 + threadIdx.x;
-	if (e >= (int64_t)r.M * ((r.N + 3) / 4)) return;
-	res_recon_group<NW>(r.M, r.N, r.R, r.ldt, r.tstride, r.nchunk, r.rc, r.nmod, r.rowLo, r.colLo, r.alpha,
-	                    r.beta, r.C, r.ldc, (int)(e % r.M), (int)(e / r.M));
+	const int64_t total = (int64_t)r.M * ((r.N + 3) / 4);
+	const int64_t e0 = (int64_t)(blockIdx.x - start[o]) * 256 * kRecGroups + threadIdx.x;
+	for (int u = 0; u < kRecGroups; u++) {
+		const int64_t e = e0 + (int64_t)u * 256;
+		if (e >= total) break;
+		res_recon_group<NW>(r.M, r.N, r.R, r.ldt, r.tstride, r.nchunk, r.rc, r.nmod, r.rowLo, r.colLo,
+		                    r.alpha, r.beta, r.C, r.ldc, (int)(e % r.M), (int)(e / r.M));
+	}
 }
 
 // ---- host plan -------------------------------------------------------------------------------
@@ -299,7 +305,9 @@ struct BatchPlan {
 	GRec *dRec = nullptr;
 	int *dRecStart = nullptr;
 	GCheck *dCheck = nullptr;
-	int NW = 3;
+	// the reconstruction by word count (NW 3, 4, 6, 9): the problems of class c are GRec entries
+	// [recFirst[c], recFirst[c] + recN[c]), recBlocks[c] workgroups, block starts relative to the class
+	int recFirst[4] = {0, 0, 0, 0}, recN[4] = {0, 0, 0, 0}, recBlocks[4] = {0, 0, 0, 0};
 	int tcfg = 0;  // 0: 128 x 128 tiles, 1: 256 x 256 tiles
 	int grid[2] = {0, 0};
 	float tuneMs[2] = {0.f, 0.f};
@@ -358,19 +366,20 @@ static int32_t launch_execute(BatchPlan *p, hipStream_t st) {
 			hipLaunchKernelGGL((k_gemm_grouped<4, 4>), dim3(p->grid[1]), dim3(256), 0, st, p->dProb[1],
 			                   p->dItems[1], p->nItems[1], p->dCounter, p->dFlag, p->dRcAll);
 	}
-	if (p->nRecBlocks > 0) {
-		if (p->NW <= 3)
-			hipLaunchKernelGGL(k_recon_grouped<3>, dim3(p->nRecBlocks), dim3(256), 0, st, p->dRec,
-			                   p->dRecStart, p->nRec, p->dFlag);
-		else if (p->NW <= 4)
-			hipLaunchKernelGGL(k_recon_grouped<4>, dim3(p->nRecBlocks), dim3(256), 0, st, p->dRec,
-			                   p->dRecStart, p->nRec, p->dFlag);
-		else if (p->NW <= 6)
-			hipLaunchKernelGGL(k_recon_grouped<6>, dim3(p->nRecBlocks), dim3(256), 0, st, p->dRec,
-			                   p->dRecStart, p->nRec, p->dFlag);
+	// one launch per word-count class: the registers of the widest class do not bound the others
+	for (int cl = 0; cl < 4; cl++) {
+		if (p->recBlocks[cl] == 0) continue;
+		const GRec *g = p->dRec + p->recFirst[cl];
+		const int *gs = p->dRecStart + p->recFirst[cl];
+		const dim3 grid((unsigned)p->recBlocks[cl]);
+		if (cl == 0)
+			hipLaunchKernelGGL(k_recon_grouped<3>, grid, dim3(256), 0, st, g, gs, p->recN[cl], p->dFlag);
+		else if (cl == 1)
+			hipLaunchKernelGGL(k_recon_grouped<4>, grid, dim3(256), 0, st, g, gs, p->recN[cl], p->dFlag);
+		else if (cl == 2)
+			hipLaunchKernelGGL(k_recon_grouped<6>, grid, dim3(256), 0, st, g, gs, p->recN[cl], p->dFlag);
 		else
-			hipLaunchKernelGGL(k_recon_grouped<9>, dim3(p->nRecBlocks), dim3(256), 0, st, p->dRec,
-			                   p->dRecStart, p->nRec, p->dFlag);
+			hipLaunchKernelGGL(k_recon_grouped<9>, grid, dim3(256), 0, st, g, gs, p->recN[cl], p->dFlag);
 	}
 	OZ_HIP(hipGetLastError());
 	return OZBLAS_STATUS_SUCCESS;
@@ -559,7 +568,6 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 	std::vector<LevelBin> hBins;  // the bins of both tile shapes
 	std::vector<int2> hPairs;
 	for (int t = 0; t < kResMaxMod; t++) hPairs.push_back(make_int2(t, t));  // product t: plane t x plane t
-	int maxNl = 2;
 	std::vector<int> usedNmod;
 	for (ProbH &h : probs) {
 		const int q = h.q;
@@ -604,11 +612,9 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 			h.R = L2.take((uint64_t)h.tstride * h.nmod * h.nchunk);
 			p->totalPairs += h.nmod;
 			p->ops += 2.0 * m[q] * (double)n[q] * k[q] * h.nmod;
-			maxNl = std::max(maxNl, res_const_host(h.nmod).nl);
 			if (std::find(usedNmod.begin(), usedNmod.end(), h.nmod) == usedNmod.end()) usedNmod.push_back(h.nmod);
 		}
 	}
-	p->NW = (maxNl + 1) / 2;
 	// constants: the full moduli table, then one per moduli count in use
 	const uint64_t rcAllOff = L2.take(sizeof(ResConst));
 	std::vector<uint64_t> rcOff(kResMaxMod + 1, 0);
@@ -617,7 +623,7 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 	std::vector<GWrite> hWrite;
 	std::vector<int> writeStart;
 	std::vector<GRec> hRec;
-	std::vector<int> recStart;
+	std::vector<int> recStart, recClass;
 	std::vector<GCheck> hCheck;
 	std::vector<GProb> hProb[2];
 	std::vector<GItem> hItems[2];
@@ -716,8 +722,8 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 		r.N = n[q];
 		r.nmod = h.nmod;
 		r.nchunk = h.nchunk;
-		recStart.push_back(rb);
-		rb += (int)(((int64_t)m[q] * ((n[q] + 3) / 4) + 255) / 256);
+		const int nw = (h.nmod > 0) ? (res_const_host(h.nmod).nl + 1) / 2 : 3;
+		recClass.push_back(nw <= 3 ? 0 : nw <= 4 ? 1 : nw <= 6 ? 2 : 3);
 		hRec.push_back(r);
 		GCheck ck;
 		ck.opA = 2 * q;
@@ -750,6 +756,24 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 	p->nWriteBlocks = wb;
 	p->dWrite = (GWrite *)(M2 + writeOff);
 	p->dWriteStart = (int *)(M2 + writeStartOff);
+	{  // the reconstruction tables in class order, block starts relative to each class
+		std::vector<GRec> byClass;
+		for (int cl = 0; cl < 4; cl++) {
+			p->recFirst[cl] = (int)byClass.size();
+			int blocks = 0;
+			for (size_t i = 0; i < hRec.size(); i++) {
+				if (recClass[i] != cl) continue;
+				recStart.push_back(blocks);
+				blocks += (int)(((int64_t)hRec[i].M * ((hRec[i].N + 3) / 4) + 256 * kRecGroups - 1) /
+				                (256 * kRecGroups));
+				byClass.push_back(hRec[i]);
+			}
+			p->recN[cl] = (int)byClass.size() - p->recFirst[cl];
+			p->recBlocks[cl] = blocks;
+			rb += blocks;
+		}
+		hRec.swap(byClass);
+	}
 	p->nRec = (int)hRec.size();
 	p->nRecBlocks = rb;
 	p->dRec = (GRec *)(M2 + recOff);

@@ -356,28 +356,15 @@ struct I8Tile {
 	const ResConst *rc;
 };
 
-// v mod p in [0, p) for |v| <= 2^30 and 3 <= p <= 256 (256 included: 1/256 is exact), in binary32: a first quotient estimate
-// leaves |r| <= 129 p (the estimate's error is below |v| 3 2^-24 / p + 1/2), which the second
-// step (exact below 2^24) reduces to [0, p) up to one correction.
-__device__ __forceinline__ int mod_i32(int v, int p, float invpf) {
-	const int q1 = (int)rintf((float)v * invpf);
-	int r = v - q1 * p;
-	const int q2 = (int)floorf((float)r * invpf);
-	r -= q2 * p;
-	r += (r < 0) ? p : 0;
-	r -= (r >= p) ? p : 0;
-	return r;
-}
-
-// v mod p for |v| <= 2^27 (a product over at most 8192 k of balanced residues) and p >= 25: one
-// binary32 quotient floor(RN(RN(v) RN(1/p))) is within 24/p < 1 of v/p, so v - q p lies in [-p, 2p)
-// and two corrections finish (q p by the 24-bit multiply: |q| < 2^23)
-__device__ __forceinline__ int mod_i32_short(int v, int p, float invpf) {
-	const int q = (int)floorf((float)v * invpf);
-	int r = v - __mul24(q, p);
-	r += (r < 0) ? p : 0;
-	r -= (r >= p) ? p : 0;
-	return r;
+// v mod p in [0, p) for |v| <= 2^30 and 3 <= p <= 256, in binary64 without a correction step:
+// q = floor(fma(v, RN(1/p), 2^-10)) is floor(v / p) exactly -- the fma's error, below
+// |v/p| 2^-53 + 2^-23 < 2^-22, and the offset 2^-10 stay inside the gap of at least 1/p >= 2^-8 that
+// separates a non-integer v/p from the integer above it, and the offset lifts an integer v/p off the
+// one below -- so r = v - q p, exact in one fma, is the canonical residue.
+__device__ __forceinline__ int mod_f64(int v, double p, double invp) {
+	const double vd = (double)v;
+	const double q = floor(fma(vd, invp, 0x1p-10));
+	return (int)fma(-q, p, vd);
 }
 
 // Product residues of the residue form: one byte per element, four consecutive columns of a row
@@ -533,32 +520,26 @@ __device__ __forceinline__ void i8_tile(const I8Tile &g, char *stage0, char *sta
 			if (--left > 0) return;
 			left = kst;
 			const int t = g.s0 + mq++;
-			const int p = g.rc->p[t];
-			const float invpf = g.rc->invpf[t];
+			const double p = (double)g.rc->p[t];
+			const double invp = g.rc->invpd[t];
 			uint8_t *Rb = g.R + (int64_t)t * g.rstride;
-			auto store = [&](auto modf) {
+			// |acc| <= 2^14 k <= 2^30 (chunks of at most 65536 k)
 #pragma unroll
-				for (int mi = 0; mi < TM; mi++)
+			for (int mi = 0; mi < TM; mi++)
 #pragma unroll
-					for (int nj = 0; nj < TN; nj++)
+				for (int nj = 0; nj < TN; nj++)
 #pragma unroll
-						for (int r4 = 0; r4 < 4; r4++) {
-							const int i = i0 + wm + 32 * mi + l % 32;
-							const int j = j0 + wn + 32 * nj + 8 * r4 + 4 * (l / 32);
-							uint32_t w = 0;
+					for (int r4 = 0; r4 < 4; r4++) {
+						const int i = i0 + wm + 32 * mi + l % 32;
+						const int j = j0 + wn + 32 * nj + 8 * r4 + 4 * (l / 32);
+						uint32_t w = 0;
 #pragma unroll
-							for (int c = 0; c < 4; c++) {
-								w |= (uint32_t)modf(acc[mi][nj][4 * r4 + c], p, invpf) << (8 * c);
-								acc[mi][nj][4 * r4 + c] = 0;
-							}
-							*(uint32_t *)(Rb + res_word_offset(i, j, g.ldt)) = w;
+						for (int c = 0; c < 4; c++) {
+							w |= (uint32_t)mod_f64(acc[mi][nj][4 * r4 + c], p, invp) << (8 * c);
+							acc[mi][nj][4 * r4 + c] = 0;
 						}
-			};
-			// |acc| <= 2^14 k: the one-step reduction for chunks of at most 8192 k
-			if (g.k1 - g.k0 <= 8192)
-				store([](int v, int pp, float ip) { return mod_i32_short(v, pp, ip); });
-			else
-				store([](int v, int pp, float ip) { return mod_i32(v, pp, ip); });
+						*(uint32_t *)(Rb + res_word_offset(i, j, g.ldt)) = w;
+					}
 		}
 	};
 	// prologue: stage 0 -> buffer 0, stage 1 -> registers set 1

@@ -182,8 +182,8 @@ __global__ __launch_bounds__(256) void k_digit_gemm_h(int M, int N, const char *
 	// D[p][q]: p = B side (column j) = 4*(l/16) + r, q = A side (row i) = l%16
 	if constexpr (RES) {  // residue form: bin = (modulus bin.level, k chunk), acc mod p, 4 columns a word
 		uint8_t *Rb = R + (int64_t)binIdx * tstride;
-		const int p = rc->p[bin.level];
-		const float invpf = rc->invpf[bin.level];
+		const double p = (double)rc->p[bin.level];
+		const double invp = rc->invpd[bin.level];
 		for (int mi = 0; mi < 4; mi++)
 			for (int nj = 0; nj < 4; nj++) {
 				const int i = i0 + wm + 16 * mi + l % 16;
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void k_digit_gemm_h(int M, int N, const char *
 				uint32_t w = 0;
 				for (int r = 0; r < 4; r++) {
 					const int32_t v = iacc[mi][nj][r] + (int32_t)facc[mi][nj][r];
-					w |= (uint32_t)mod_i32(v, p, invpf) << (8 * r);
+					w |= (uint32_t)mod_f64(v, p, invp) << (8 * r);
 				}
 				if (i < M && j < N) *(uint32_t *)(Rb + res_word_offset(i, j, ldt)) = w;
 			}

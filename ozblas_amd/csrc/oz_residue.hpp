@@ -82,6 +82,11 @@ __device__ inline void res_write_tile_i8(int m, int n, const double *in, int64_t
                                          int nmod, const ResLds &T, int8_t *planes, int64_t ldp,
                                          int64_t pstride, int kt, int jb) {
 	const int t = threadIdx.x;
+	if (jb * 16 >= n || kt * 64 >= m) {  // a tile of padding only: zero residues, nothing to compute
+		int8_t *dst = planes + (int64_t)jb * 16 * ldp + (int64_t)kt * 1024 + 4 * t;
+		for (int s = 0; s < nmod; s++) *(uint32_t *)(dst + (int64_t)s * pstride) = 0u;
+		return;
+	}
 	const int j = jb * 16 + t / 16;
 	const int lo = (j < n) ? colLo[j] : 0;
 	const double *x = in + (int64_t)(j < n ? j : 0) * ldi;

@@ -11,6 +11,7 @@ integers:
     binary64, X = sum c_t M/p_t - K M in binary64 chunks of 39 bits, each sum checked exact),
     restated the same way, against X itself for X at 0, +-1, +-M/4 and random values in between:
     the claim that |X| <= M/4 makes K exact.
+  - the MFMA epilogues' acc mod p (an exact binary64 quotient, no correction) for |acc| <= 2^30.
 A plausible mistake (a wrong inverse, a dropped limb, K rounded from the wrong side, a residue
 taken modulo the wrong power of two) fails one of them.  No GPU call is made."""
 import ctypes
@@ -246,55 +247,33 @@ def test_reconstruction_is_exact_within_quarter(rc, nmod):
             assert rc[n][1] >= need and (n == 1 or rc[n - 1][1] < need + 1e-6)
 
 
-def mod_i32(v, p, invpf):
-    """mod_i32 of the MFMA epilogues restated in binary32 (numpy) and int32 arithmetic."""
-    q1 = int(np.rint(np.float32(np.float32(v) * invpf)))
-    r = v - q1 * p
-    assert -(2 ** 31) <= q1 * p < 2 ** 31
-    q2 = int(np.floor(np.float32(np.float32(r) * invpf)))
-    r -= q2 * p
-    r = r + p if r < 0 else r
-    return r - p if r >= p else r
+def mod_f64(v, p, invpd):
+    """mod_f64 of the MFMA epilogues restated: q = floor(RN(v invp + 2^-10)) (one rounding, through
+    exact rationals), r = v - q p (exact); checked to be the canonical residue without a correction."""
+    from fractions import Fraction as Fr
+    t = float(Fr(int(v)) * Fr(float(invpd)) + Fr(1, 1024))  # fma: one rounding of the exact value
+    q = math.floor(t)
+    r = v - q * p
+    assert 0 <= r < p, (v, p, r)
+    return r
 
 
 def test_epilogue_mod_against_integers(rc):
     """acc mod p in the MFMA epilogues for |acc| <= 2^30 (65,536 k x 2^14 per chunk), every modulus
-    (256 included), at the ends of the range and near multiples of p."""
+    (256 included): the ends of the range, multiples of p and their neighbours (where the quotient's
+    rounding matters), and random values."""
     c, _ = rc[MAXMOD]
     rng = random.Random(11)
-    vs = [0, 1, -1, 2 ** 30, -(2 ** 30), 2 ** 30 - 1, -(2 ** 30) + 1, 2 ** 24, 2 ** 24 + 1, -(2 ** 24) - 3]
-    vs += [rng.randrange(-(2 ** 30), 2 ** 30 + 1) for _ in range(300)]
+    top = 2 ** 30
+    vs = [0, 1, -1, top, -top, top - 1, -top + 1, 2 ** 24, 2 ** 24 + 1, -(2 ** 24) - 3, 2 ** 27, -(2 ** 27)]
+    vs += [rng.randrange(-top, top + 1) for _ in range(300)]
     for t in range(MAXMOD):
         p = int(c["p"][t])
-        near = [q * p + d for q in (rng.randrange(-(2 ** 30) // p, 2 ** 30 // p) for _ in range(20))
-                for d in (-1, 0, 1)]
-        for v in vs + [x for x in near if abs(x) <= 2 ** 30]:
-            assert mod_i32(v, p, c["invpf"][t]) == v % p, (p, v)
-
-
-def mod_i32_short(v, p, invpf):
-    """mod_i32_short restated: one binary32 quotient floor(RN(RN(v) * invpf)), then two corrections."""
-    q = int(np.floor(np.float32(np.float32(v) * invpf)))
-    r = v - q * p
-    assert -p <= r < 2 * p, (v, p, r)  # the bound the two corrections rely on
-    r = r + p if r < 0 else r
-    return r - p if r >= p else r
-
-
-def test_short_chunk_epilogue_mod_against_integers(rc):
-    """acc mod p with the one-step quotient for |acc| <= 2^27 (chunks of at most 8192 k): every
-    modulus (29 the smallest), at the ends of the range, near multiples of p and where the binary32
-    rounding of acc is largest."""
-    c, _ = rc[MAXMOD]
-    rng = random.Random(13)
-    top = 2 ** 27
-    vs = [0, 1, -1, top, -top, top - 1, -top + 1, 2 ** 24 + 1, -(2 ** 24) - 1, 2 ** 26 + 3, -(2 ** 26) - 5]
-    vs += [rng.randrange(-top, top + 1) for _ in range(400)]
-    for t in range(MAXMOD):
-        p = int(c["p"][t])
-        near = [q * p + d for q in (rng.randrange(-top // p, top // p) for _ in range(40)) for d in (-1, 0, 1)]
+        assert c["invpd"][t] == np.float64(1.0) / np.float64(p)
+        near = [q * p + d for q in (rng.randrange(-top // p, top // p) for _ in range(60)) for d in (-1, 0, 1)]
+        near += [q * p + d for q in (top // p, -(top // p), 1, -1) for d in (-1, 0, 1)]
         for v in vs + [x for x in near if abs(x) <= top]:
-            assert mod_i32_short(v, p, c["invpf"][t]) == v % p, (p, v)
+            assert mod_f64(v, p, c["invpd"][t]) == v % p, (p, v)
 
 
 def mod_f(v, p, invpf):
