@@ -199,10 +199,10 @@ __global__ __launch_bounds__(256) void k_res_write_h(int m, int n, const double 
 		for (int i = threadIdx.x; i < ldp; i += blockDim.x) {
 			const ResElem e = res_elem(i < m ? x[i] : 0.0, lo);
 			for (int gi = 0; gi < ng; gi++) {
-				const int rx = res_group(e, L.Pg[gi], L.invPg[gi], L.pw2[gi]);
+				const double rx = res_group(e, L.Pg[gi], L.invPg[gi], L.pw2[gi]);
 				const int s1 = min(3 * gi + 3, nmod);
 				for (int s = 3 * gi; s < s1; s++)
-					planes[(int64_t)s * pstride + (int64_t)j * ldp + i] = digit_cast<T>(res_member(rx, L.p[s], L.ipf[s]));
+					planes[(int64_t)s * pstride + (int64_t)j * ldp + i] = digit_cast<T>(res_member(rx, L.pd[s], L.ipd[s]));
 			}
 		}
 	}

@@ -24,32 +24,26 @@ __device__ inline ResElem res_elem(double x, int lo) {
 	return ResElem{ldexp(x, -(lo + sh)), sh};
 }
 
-// Residues by groups of three moduli: (sf 2^sh) mod P in binary64, P the group's product (< 2^24).
-// sf mod P is only reduced to [-P, 2P) (the floor estimate is off by at most one: |error| <=
-// 2 |sf| 2^-53 / P < 1); times 2^sh mod P that stays exact (|x| < 2^25 2^24), and the second
-// reduction is made canonical, [0, P).  The sign rides on sf, so every member's residue comes out
-// of res_member already signed.
-__device__ __forceinline__ int res_group(const ResElem &e, double P, double invP, const uint32_t *pw2) {
+// Residues by groups of three moduli: (sf 2^sh) mod P in binary64, P the group's product (< 2^24),
+// left in [-P, 2P) (an exact integer congruent to the element modulo P): each floor estimate is off
+// by at most one (|error| <= 2 |v| 2^-53 / P < 1), the first remainder times 2^sh mod P stays exact
+// (|x| < 2^25 2^24), and res_member takes any such representative.  The sign rides on sf.
+__device__ __forceinline__ double res_group(const ResElem &e, double P, double invP, const uint32_t *pw2) {
 	double q = floor(e.sf * invP);
 	const double r0 = fma(-q, P, e.sf);              // exact, in [-P, 2P)
 	const double x = r0 * (double)pw2[e.sh];         // exact, |x| < 2^49
 	q = floor(x * invP);
-	double r = fma(-q, P, x);                        // exact, in [-P, 2P)
-	r += (r < 0.0) ? P : 0.0;
-	r -= (r >= P) ? P : 0.0;
-	return (int)r;
+	return fma(-q, P, x);                            // exact, in [-P, 2P)
 }
 
-// rx - p rint(rx / p) is the balanced residue up to one correction either way (the binary32 quotient
-// estimate is off by at most one near a half-integer); q p with a 24-bit multiply (q < 2^20, p < 2^9)
-__device__ __forceinline__ int res_member(int rx, int p, float invpf) {
-	const int q = (int)rintf((float)rx * invpf);  // rx < 2^24 is exact
-	int r = rx - (int)__umul24((uint32_t)q, (uint32_t)p);
-	const int hi = (p & 1) ? (p >> 1) : (p >> 1) - 1;  // [-(p-1)/2, (p-1)/2] odd p, [-128, 127] for 256
-	const int lo = -(p >> 1);
-	r -= (r > hi) ? p : 0;
-	r += (r < lo) ? p : 0;
-	return r;
+// The balanced residue of an integer rx (|rx| < 2^25) modulo p: q = floor(rx / p + 1/2) exactly, as
+// floor(fma(rx, RN(1/p), 1/2)) -- its error, below 2^-28, cannot cross an integer: rx / p + 1/2 is
+// an integer only for p = 256, where 1/p is exact, and otherwise keeps 1/(2p) >= 2^-9 from one --
+// so r = rx - q p (exact in one fma) lies in [-p/2, p/2): [-(p-1)/2, (p-1)/2] for odd p, [-128, 127]
+// for 256.
+__device__ __forceinline__ int res_member(double rx, double p, double invp) {
+	const double q = floor(fma(rx, invp, 0.5));
+	return (int)fma(-q, p, rx);
 }
 
 // the per-group and per-modulus tables of the residue writers, staged in LDS by the whole workgroup
@@ -57,8 +51,7 @@ __device__ __forceinline__ int res_member(int rx, int p, float invpf) {
 struct ResLds {
 	uint32_t pw2[kResMaxGroups][kResMaxShift];
 	double Pg[kResMaxGroups], invPg[kResMaxGroups];
-	int p[kResMaxMod];
-	float ipf[kResMaxMod];
+	double pd[kResMaxMod], ipd[kResMaxMod];  // p and RN(1/p)
 };
 
 // scaleU: the tables of op(A), whose residues carry the factors u_t (ResConst::pow2gU)
@@ -71,8 +64,8 @@ __device__ inline void res_lds_load(ResLds &T, const ResConst *rc, int nmod, int
 		T.invPg[q] = rc->invPg[q];
 	}
 	for (int q = threadIdx.x; q < nmod; q += blockDim.x) {
-		T.p[q] = rc->p[q];
-		T.ipf[q] = rc->invpf[q];
+		T.pd[q] = (double)rc->p[q];
+		T.ipd[q] = rc->invpd[q];
 	}
 }
 
@@ -97,7 +90,7 @@ __device__ inline void res_write_tile_i8(int m, int n, const double *in, int64_t
 	int8_t *dst = planes + (int64_t)jb * 16 * ldp + (int64_t)kt * 1024 + 4 * t;
 	const int ng = (nmod + 2) / 3;
 	for (int gi = 0; gi < ng; gi++) {
-		int rx[4];
+		double rx[4];
 #pragma unroll
 		for (int u = 0; u < 4; u++) rx[u] = res_group(e[u], T.Pg[gi], T.invPg[gi], T.pw2[gi]);
 		const int s1 = min(3 * gi + 3, nmod);
@@ -105,7 +98,7 @@ __device__ inline void res_write_tile_i8(int m, int n, const double *in, int64_t
 			uint32_t word = 0;
 #pragma unroll
 			for (int u = 0; u < 4; u++)
-				word |= (uint32_t)(uint8_t)(int8_t)res_member(rx[u], T.p[s], T.ipf[s]) << (8 * u);
+				word |= (uint32_t)(uint8_t)(int8_t)res_member(rx[u], T.pd[s], T.ipd[s]) << (8 * u);
 			*(uint32_t *)(dst + (int64_t)s * pstride) = word;
 		}
 	}

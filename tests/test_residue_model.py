@@ -3,7 +3,8 @@ integers:
   - the host constants the kernels read (moduli, (M/p)^-1 mod p, the limbs of M and M/p, 256^a
     mod p), taken from the library itself, against Python integers;
   - the element residues (res_group: (+-f mod P) (2^sh mod P) mod P in binary64 for the product P of
-    three moduli; res_member: the balanced residue of that value modulo each member in binary32),
+    three moduli, left in [-P, 2P); res_member: the balanced residue of that value modulo each member
+    from one exact binary64 quotient),
     restated in IEEE arithmetic (numpy), against (+-f 2^sh) mod p;
   - the element decomposition (res_elem: x 2^-lo = sf 2^sh) against exact rationals;
   - the op(A) group tables with the factors u_t = (M/p_t)^-1 mod p_t folded in (pow2gU);
@@ -106,8 +107,8 @@ def res_elem(x, lo):
 
 
 def res_group(sf, sh, P, invP, pw2):
-    """res_group restated in binary64 (numpy rounds each operation like the device; the fmas are
-    exact): sf mod P left in [-P, 2P), times 2^sh mod P, then reduced to [0, P)."""
+    """res_group restated in binary64 (numpy rounds each product like the device; the fmas are exact):
+    sf mod P left in [-P, 2P), times 2^sh mod P, reduced again to [-P, 2P)."""
     sf = np.float64(sf)
     q = np.floor(sf * invP)
     r0 = fma_d(-q, P, sf)  # one rounding of an integer below 2^25: exact
@@ -117,20 +118,14 @@ def res_group(sf, sh, P, invP, pw2):
     q = np.floor(x * invP)
     r = fma_d(-q, P, x)
     assert -P <= r < 2 * P and r == int(x) - int(q) * int(P)
-    r = r + P if r < 0 else r
-    r = r - P if r >= P else r
     return int(r)
 
 
-def res_member(rx, p, invpf):
-    """res_member restated in binary32 / int32 (rint: nearest, ties to even, as the device)."""
-    q = int(np.rint(np.float32(np.float32(rx) * invpf)))
-    r = rx - q * p
-    hi = p >> 1 if p & 1 else (p >> 1) - 1
-    lo = -(p >> 1)
-    r = r - p if r > hi else r
-    r = r + p if r < lo else r
-    return r
+def res_member(rx, p, invpd):
+    """res_member restated: q = floor(RN(rx invp + 1/2)) through exact rationals, r = rx - q p."""
+    assert abs(rx) < 2 ** 25
+    q = math.floor(fma_d(rx, invpd, 0.5))
+    return rx - q * p
 
 
 def balanced(v, p):
@@ -172,13 +167,29 @@ def test_element_residues_against_integers(rc):
             for neg in (0, 1):
                 v = (-1 if neg else 1) * f * pow(2, sh, int(P))
                 rx = res_group((-1 if neg else 1) * f, sh, P, c["invPg"][g], c["pow2g"][g])
-                assert rx == v % int(P), (P, f, sh, neg)
+                assert rx % int(P) == v % int(P), (P, f, sh, neg)
                 for t in range(3 * g, min(3 * g + 3, MAXMOD)):
                     p = int(c["p"][t])
-                    got = res_member(rx, p, c["invpf"][t])
+                    got = res_member(rx, p, c["invpd"][t])
                     want = balanced((-1 if neg else 1) * f * 2 ** sh, p)
                     assert got == want, (p, f, sh, neg)
                     assert -128 <= got <= 127
+
+
+def test_member_residue_at_the_half_points(rc):
+    """res_member at the representatives next to the balanced range's ends (rx = k p +- (p-1)/2,
+    +-(p+1)/2, and the ties k 256 + 128 of the even modulus), over |rx| < 2^25."""
+    c, _ = rc[MAXMOD]
+    rng = random.Random(23)
+    for t in range(MAXMOD):
+        p = int(c["p"][t])
+        ks = [0, 1, -1, (2 ** 25 - p) // p, -((2 ** 25 - p) // p)] + [rng.randrange(-(2 ** 25) // p + 1, (2 ** 25) // p - 1) for _ in range(40)]
+        for k in ks:
+            for d in (-(p + 1) // 2, -(p // 2), -(p - 1) // 2, 0, (p - 1) // 2, p // 2, (p + 1) // 2):
+                rx = k * p + d
+                if abs(rx) >= 2 ** 25:
+                    continue
+                assert res_member(rx, p, c["invpd"][t]) == balanced(rx, p), (p, rx)
 
 
 def test_element_decomposition_against_integers():
