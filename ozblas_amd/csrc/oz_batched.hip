@@ -89,6 +89,17 @@ struct GCheck {  // plan assumptions of one problem, re-checked on every executi
 	int shA, shB;    // spans the residue writer's tables were staged for
 };
 
+// start[] staged in LDS by the whole workgroup (one round of independent loads, instead of the
+// chain of dependent global loads a search would issue), for the statistics, which look up every
+// column; tables longer than the stage are searched where they are
+constexpr int kOwnerStage = 1024;
+__device__ inline const int *stage_starts(const int *start, int n, int *sh) {
+	if (n > kOwnerStage) return start;
+	for (int i = threadIdx.x; i < n; i += blockDim.x) sh[i] = start[i];
+	__syncthreads();
+	return sh;
+}
+
 // largest o < n with start[o] <= b (start[] ascending, start[0] = 0)
 __device__ inline int owner(const int *start, int n, int b) {
 	int lo = 0, hi = n - 1;
@@ -145,6 +156,8 @@ constexpr int kStatCols = 16;
 __global__ __launch_bounds__(256) void k_stats_grouped(const GStat *g, const int *start, int n,
                                                        int total) {
 	constexpr int per = kStatCols / 4;
+	__shared__ int sStart[kOwnerStage];
+	start = stage_starts(start, n, sStart);
 	const int w = threadIdx.x / 64, l = threadIdx.x % 64;
 	const int c0 = blockIdx.x * kStatCols + w * per, c1 = min(c0 + per, total);
 	int cur = -1;
@@ -248,13 +261,20 @@ __global__ __launch_bounds__(256) void k_recon_grouped(const GRec *g, const int 
 	if (*flag) return;  // C stays untouched when the plan check failed
 	const int o = owner(start, n, blockIdx.x);
 	const GRec r = g[o];
-	const int64_t total = (int64_t)r.M * ((r.N + 3) / 4);
-	const int64_t e0 = (int64_t)(blockIdx.x - start[o]) * 256 * kRecGroups + threadIdx.x;
-	for (int u = 0; u < kRecGroups; u++) {
-		const int64_t e = e0 + (int64_t)u * 256;
-		if (e >= total) break;
+	// item e = (row e % M, column group e / M): one division, then rows carried (the plan keeps
+	// M * G below 2^31 - 2^20)
+	const int total = r.M * ((r.N + 3) / 4);
+	const int e0 = (blockIdx.x - start[o]) * 256 * kRecGroups + (int)threadIdx.x;
+	if (e0 >= total) return;
+	int i = e0 % r.M, gq = e0 / r.M;
+	for (int e = e0; e < total && e < e0 + 256 * kRecGroups; e += 256) {
 		res_recon_group<NW>(r.M, r.N, r.R, r.ldt, r.tstride, r.nchunk, r.rc, r.nmod, r.rowLo, r.colLo,
-		                    r.alpha, r.beta, r.C, r.ldc, (int)(e % r.M), (int)(e / r.M));
+		                    r.alpha, r.beta, r.C, r.ldc, i, gq);
+		i += 256;
+		while (i >= r.M) {  // at most 256 / M + 1 steps
+			i -= r.M;
+			gq++;
+		}
 	}
 }
 
@@ -536,7 +556,8 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 	std::vector<ProbH> probs;
 	for (int q : live) {
 		const ExpStatsH &sa = p->hSt[2 * q], &sb = p->hSt[2 * q + 1];
-		if (!(k[q] > 0 && fast_valid(oh, sa, sb, k[q]))) {
+		// (the grouped reconstruction indexes a problem's outputs in 32 bits)
+		if (!(k[q] > 0 && fast_valid(oh, sa, sb, k[q])) || (int64_t)m[q] * ((n[q] + 3) / 4) >= (1LL << 31) - (1 << 20)) {
 			p->fallback.push_back(q);
 			continue;
 		}

@@ -237,12 +237,32 @@ __global__ __launch_bounds__(256) void k_res_recon(int M, int N, const uint8_t *
                                                    const int *__restrict__ rowLo, const int *__restrict__ colLo,
                                                    double alpha, double beta, double *__restrict__ C,
                                                    int64_t ldc) {
+	// item e = (row e % M, column group e / M): one division for the first item, then the grid
+	// stride split the same way once, and the row carried by compare-and-subtract (M * G < 2^31)
 	const int G = (N + 3) / 4;
-	const int64_t total = (int64_t)M * G;
-	for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-	     e += (int64_t)gridDim.x * blockDim.x)
-		res_recon_group<NW>(M, N, R, ldr, rstride, nchunk, rc, nmod, rowLo, colLo, alpha, beta, C, ldc,
-		                    (int)(e % M), (int)(e / M));
+	if ((int64_t)M * G > (int64_t)INT32_MAX - (int64_t)gridDim.x * blockDim.x) {  // 64-bit items
+		const int64_t tot = (int64_t)M * G;
+		for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+		     e += (int64_t)gridDim.x * blockDim.x)
+			res_recon_group<NW>(M, N, R, ldr, rstride, nchunk, rc, nmod, rowLo, colLo, alpha, beta, C, ldc,
+			                    (int)(e % M), (int)(e / M));
+		return;
+	}
+	const int total = M * G;
+	const int stride = (int)(gridDim.x * blockDim.x);
+	const int e0 = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (e0 >= total) return;
+	int i = e0 % M, g = e0 / M;
+	const int si = stride % M, sg = stride / M;
+	for (int e = e0; e < total; e += stride) {
+		res_recon_group<NW>(M, N, R, ldr, rstride, nchunk, rc, nmod, rowLo, colLo, alpha, beta, C, ldc, i, g);
+		i += si;
+		g += sg;
+		if (i >= M) {
+			i -= M;
+			g++;
+		}
+	}
 }
 
 void launch_res_recon(hipStream_t st, int M, int N, const uint8_t *R, int64_t ldr, int64_t rstride,

@@ -123,7 +123,7 @@ __device__ __forceinline__ int mod_f(int v, int p, float invpf) {
 // of 4 moduli are loaded together (clamped indices; the surplus ones are not used).
 // Rw addresses the word in plane 0; plane b is rstride bytes further.
 template <int NW>
-__device__ inline void res_group_values(const uint8_t *Rw, int64_t rstride, int nchunk,
+__device__ __forceinline__ void res_group_values(const uint8_t *Rw, int64_t rstride, int nchunk,
                                         const ResConst *rc, int nmod, const int *e0, double *t) {
 	constexpr int NC = (64 * NW + kResChunkBits - 1) / kResChunkBits;
 	constexpr int QB = 4;
@@ -190,7 +190,7 @@ __device__ inline void res_group_values(const uint8_t *Rw, int64_t rstride, int 
 // one thread per (row i, column group g) of an M x N product: C(i, 4g + e) = fma(alpha, RN(X 2^(lo_i +
 // lo_j)), beta C) for the columns of the group inside N
 template <int NW>
-__device__ inline void res_recon_group(int M, int N, const uint8_t *R, int64_t ldr, int64_t rstride,
+__device__ __forceinline__ void res_recon_group(int M, int N, const uint8_t *R, int64_t ldr, int64_t rstride,
                                        int nchunk, const ResConst *rc, int nmod, const int *rowLo,
                                        const int *colLo, double alpha, double beta, double *C,
                                        int64_t ldc, int i, int g) {
