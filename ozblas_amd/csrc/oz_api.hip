@@ -773,20 +773,22 @@ struct VecSplit {
 // otherwise it stays on the device in vs.info for kernels that read it there.  With planOnly the
 // slices are stored only for a plan outside the residue model: the consumer generates them from
 // (x, plan) (VsRef).
+// zScratch / zInfo: a zeroed scratch (vsplit_scratch_bytes()) and info (16 bytes) the caller
+// cleared together with its other buffers (one fill in place of several)
 template <class T>
 static int32_t vec_split(ozblasHandle_t *oh, int n, const T *x, VecSplit &vs, bool readCount = true,
-                         bool planOnly = false) {
+                         bool planOnly = false, void *zScratch = nullptr, int *zInfo = nullptr) {
 	Backend *b = be(oh);
 	const int cap = split_cap(oh);
 	const int rho = Prec<T>::rho(oh, n);
 	vs.x = x;
 	vs.lds = rup(n, 32);
 	vs.slices = (double *)walloc(oh, 8ull * vs.lds * (cap + 2));
-	vs.info = (int *)walloc(oh, 16);
-	void *scratch = walloc(oh, vsplit_scratch_bytes());
+	vs.info = zInfo ? zInfo : (int *)walloc(oh, 16);
+	void *scratch = zScratch ? zScratch : walloc(oh, vsplit_scratch_bytes());
 	vs.plan = (VsPlan *)walloc(oh, sizeof(VsPlan));
 	if (!vs.slices || !vs.info || !scratch || !vs.plan) return OZBLAS_STATUS_WORKSPACE;
-	OZ_HIP(hipMemsetAsync(scratch, 0, vsplit_scratch_bytes(), b->stream));
+	if (!zScratch) OZ_HIP(hipMemsetAsync(scratch, 0, vsplit_scratch_bytes(), b->stream));
 	if (planOnly)
 		OZ_HIP(launch_vsplit_plan(b->stream, n, x, vs.slices, vs.lds, rho, oh->splitShift, cap,
 		                          repro_slot(oh) ? 0 : 1, Prec<T>::Jsplit(b), vs.info, scratch, nullptr,
@@ -843,38 +845,45 @@ static int32_t dot_impl(ozblasHandle_t *oh, int n, const T *x, const T *y, T *re
 	oh->splitShift = 1;  // counterInit (src/cpu/ozblas.cpp:8)
 	const bool dot2 = (oh->splitEpsMode == 2);
 	VecSplit xs, ys;
-	OZ_TRY(vec_split(oh, n, x, xs, dot2, !dot2));
-	if (y == x) {
-		ys = xs;
+	if (dot2) {
+		OZ_TRY(vec_split(oh, n, x, xs, true, false));
+		if (y == x)
+			ys = xs;
+		else
+			OZ_TRY(vec_split(oh, n, y, ys, true, false));
+		T *dres = (T *)walloc(oh, 8);
+		if (!dres) return OZBLAS_STATUS_WORKSPACE;
+		OZ_HIP(hipMemsetAsync(dres, 0, 8, b->stream));
+		if constexpr (!f32) OZ_TRY(dot_core(oh, n, xs, ys, dres));
+		OZ_HIP(hipMemcpyAsync(b->hPinned, dres, sizeof(T), hipMemcpyDeviceToHost, b->stream));
+		OZ_TRY(sync(oh));
+		std::memcpy(ret, b->hPinned, sizeof(T));
 	} else {
-		OZ_TRY(vec_split(oh, n, y, ys, dot2, !dot2));
-	}
-	T *dres = (T *)walloc(oh, 8);
-	if (!dres) return OZBLAS_STATUS_WORKSPACE;
-	OZ_HIP(hipMemsetAsync(dres, 0, 8, b->stream));
-	if constexpr (!f32) {
-		if (dot2) OZ_TRY(dot_core(oh, n, xs, ys, dres));
-	}
-	if (!dot2) {
-		// slice counts stay on the device: one launch for the pair dots and their sum
+		// one zeroed block [scratch x | scratch y | pair sums | info x | info y | result] (one fill),
+		// the slice counts stay on the device, one launch for the pair dots and their sum, and the
+		// counts and the result come back in one copy
 		const int cap = split_cap(oh);
-		double *vals = (double *)walloc(oh, 8ull * (cap + 1) * (cap + 1));
+		const size_t sb = (size_t)rup((int64_t)vsplit_scratch_bytes(), 256);
+		const size_t vb = (size_t)rup(8ll * (cap + 1) * (cap + 1), 256);
+		char *Z = (char *)walloc(oh, 2 * sb + vb + 48);
 		double *part = (double *)walloc(oh, dots_part_bytes());
-		if (!vals || !part) return OZBLAS_STATUS_WORKSPACE;
-		OZ_HIP(hipMemsetAsync(vals, 0, 8ull * (cap + 1) * (cap + 1), b->stream));
-		launch_dots_dev(b->stream, n, xs.ref(), ys.ref(), vals, part, Prec<T>::Jsum(b),
+		if (!Z || !part) return OZBLAS_STATUS_WORKSPACE;
+		int *R = (int *)(Z + 2 * sb + vb);  // [info x (4 ints) | info y (4 ints) | result (8 bytes)]
+		T *dres = (T *)(R + 8);
+		OZ_HIP(hipMemsetAsync(Z, 0, 2 * sb + vb + 48, b->stream));
+		OZ_TRY(vec_split(oh, n, x, xs, false, true, Z, R));
+		if (y == x)
+			ys = xs;
+		else
+			OZ_TRY(vec_split(oh, n, y, ys, false, true, Z + sb, R + 4));
+		launch_dots_dev(b->stream, n, xs.ref(), ys.ref(), (double *)(Z + 2 * sb), part, Prec<T>::Jsum(b),
 		                oh->sumMode == 0 ? 1 : 0, nullptr, 0, 0, dres, f32 ? 1 : 0);
-	}
-	OZ_HIP(hipMemcpyAsync(b->hPinned, dres, sizeof(T), hipMemcpyDeviceToHost, b->stream));
-	if (!dot2) {
-		OZ_HIP(hipMemcpyAsync((int32_t *)b->hPinned + 2, xs.info, 4, hipMemcpyDeviceToHost, b->stream));
-		OZ_HIP(hipMemcpyAsync((int32_t *)b->hPinned + 3, ys.info, 4, hipMemcpyDeviceToHost, b->stream));
-	}
-	OZ_TRY(sync(oh));
-	std::memcpy(ret, b->hPinned, sizeof(T));
-	if (!dot2) {
-		xs.count = ((int32_t *)b->hPinned)[2];
-		ys.count = ((int32_t *)b->hPinned)[3];
+		OZ_HIP(hipMemcpyAsync(b->hPinned, R, 48, hipMemcpyDeviceToHost, b->stream));
+		OZ_TRY(sync(oh));
+		const int32_t *hr = (const int32_t *)b->hPinned;
+		std::memcpy(ret, hr + 8, sizeof(T));
+		xs.count = hr[0];
+		ys.count = (y == x) ? hr[0] : hr[4];
 	}
 	oh->nSplitA = (float)xs.count;
 	oh->nSplitB = (float)ys.count;
@@ -1102,19 +1111,26 @@ static uint64_t dot_workspace_bytes(const ozblasHandle_t *oh, int n, bool same) 
 	const int cap = split_cap(oh);
 	const uint64_t lds = (uint64_t)rup(n, 32);
 	uint64_t t = 0;
-	for (int v = 0; v < (same ? 1 : 2); v++) {
-		t = bump(t, 8ull * lds * (cap + 2));  // slices (stored for a plan outside the model)
-		t = bump(t, 16);                      // count
-		t = bump(t, vsplit_scratch_bytes());
-		t = bump(t, sizeof(VsPlan));
-	}
-	t = bump(t, 8);  // result
-	if (oh->splitEpsMode == 2) {
+	if (oh->splitEpsMode == 2) {  // dot_impl's Dot2 order: per vector, then result and products
+		for (int v = 0; v < (same ? 1 : 2); v++) {
+			t = bump(t, 8ull * lds * (cap + 2));  // slices
+			t = bump(t, 16);                      // count
+			t = bump(t, vsplit_scratch_bytes());
+			t = bump(t, sizeof(VsPlan));
+		}
+		t = bump(t, 8);  // result
 		t = bump(t, 16ull * (cap + 2) * (cap + 2));
 		t = bump(t, dot2_partials_bytes(cap + 2, cap + 2));
-	} else {
-		t = bump(t, 8ull * (cap + 1) * (cap + 1));
-		t = bump(t, dots_part_bytes());
+		return t;
+	}
+	// the zeroed block (two scratches, pair sums, counts and result), the partials, per vector
+	const uint64_t sb = (uint64_t)rup((int64_t)vsplit_scratch_bytes(), 256);
+	const uint64_t vb = (uint64_t)rup(8ll * (cap + 1) * (cap + 1), 256);
+	t = bump(t, 2 * sb + vb + 48);
+	t = bump(t, dots_part_bytes());
+	for (int v = 0; v < (same ? 1 : 2); v++) {
+		t = bump(t, 8ull * lds * (cap + 2));  // slices (stored for a plan outside the model)
+		t = bump(t, sizeof(VsPlan));
 	}
 	return t;
 }
