@@ -871,11 +871,28 @@ static int32_t dot_impl(ozblasHandle_t *oh, int n, const T *x, const T *y, T *re
 		int *R = (int *)(Z + 2 * sb + vb);  // [info x (4 ints) | info y (4 ints) | result (8 bytes)]
 		T *dres = (T *)(R + 8);
 		OZ_HIP(hipMemsetAsync(Z, 0, 2 * sb + vb + 48, b->stream));
-		OZ_TRY(vec_split(oh, n, x, xs, false, true, Z, R));
-		if (y == x)
+		if (y == x) {
+			OZ_TRY(vec_split(oh, n, x, xs, false, true, Z, R));
 			ys = xs;
-		else
-			OZ_TRY(vec_split(oh, n, y, ys, false, true, Z + sb, R + 4));
+		} else {  // both plans by the same three launches
+			VecSplit *v[2] = {&xs, &ys};
+			const T *src[2] = {x, y};
+			double *sl[2];
+			int *inf[2];
+			void *scr[2];
+			VsPlan *pl[2];
+			for (int q = 0; q < 2; q++) {
+				v[q]->x = src[q];
+				v[q]->lds = rup(n, 32);
+				v[q]->slices = sl[q] = (double *)walloc(oh, 8ull * v[q]->lds * (cap + 2));
+				v[q]->plan = pl[q] = (VsPlan *)walloc(oh, sizeof(VsPlan));
+				v[q]->info = inf[q] = R + 4 * q;
+				scr[q] = Z + q * sb;
+				if (!sl[q] || !pl[q]) return OZBLAS_STATUS_WORKSPACE;
+			}
+			OZ_HIP(launch_vsplit_plan2(b->stream, n, x, y, Prec<T>::rho(oh, n), oh->splitShift, cap,
+			                           repro_slot(oh) ? 0 : 1, Prec<T>::Jsplit(b), sl, xs.lds, inf, scr, pl));
+		}
 		launch_dots_dev(b->stream, n, xs.ref(), ys.ref(), (double *)(Z + 2 * sb), part, Prec<T>::Jsum(b),
 		                oh->sumMode == 0 ? 1 : 0, nullptr, 0, 0, dres, f32 ? 1 : 0);
 		OZ_HIP(hipMemcpyAsync(b->hPinned, R, 48, hipMemcpyDeviceToHost, b->stream));

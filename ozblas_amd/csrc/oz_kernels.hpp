@@ -248,6 +248,14 @@ hipError_t launch_vsplit_plan(hipStream_t st, int n, const float *x, double *sli
                               int rho, int shift, int cap, int reproMode, const uint16_t *JF, int *info,
                               void *scratch, VsPre *pre, VsPlan *plan, bool statsDone,
                               CgDev *S = nullptr, int it = 0, int clkCat = -1, int parity = 0);
+// the plans of two vectors of one length (the DOT's x and y; each with its own slices, info,
+// zeroed scratch and plan), by the same three launches
+hipError_t launch_vsplit_plan2(hipStream_t st, int n, const double *x, const double *y, int rho, int shift,
+                               int cap, int reproMode, const uint16_t *J, double *const slices[2], int64_t lds,
+                               int *const info[2], void *const scratch[2], VsPlan *const plan[2]);
+hipError_t launch_vsplit_plan2(hipStream_t st, int n, const float *x, const float *y, int rho, int shift,
+                               int cap, int reproMode, const uint16_t *JF, double *const slices[2], int64_t lds,
+                               int *const info[2], void *const scratch[2], VsPlan *const plan[2]);
 void launch_axpy_host(hipStream_t st, int n, double alpha, const double *x, double *y, int fused);
 void launch_axpy_host(hipStream_t st, int n, float alpha, const float *x, float *y, int fused);
 // t = b - A x by the reference's plain CSRMV (the CG's true residual)

@@ -143,6 +143,20 @@ __device__ int vs_verify(unsigned long long mb, int lof, int bad, const unsigned
 template <bool ATOMIC>
 __device__ inline void vs_pre_sum(const VsPre *pre, unsigned long long &maxBits, int &loOff, int &bad);
 
+// The second vector of a launch that splits two vectors of one length together (the rows
+// blockIdx.y == 1 of the grid): its data and its own reductions, scratch, plan and slices.
+template <class TI>
+struct VsSecond {
+	const TI *x;
+	VsPre *pre, *preNext;
+	unsigned long long *Fg;
+	int2 *range;
+	unsigned int *done;
+	VsPlan *plan;
+	int *info;
+	double *slices;
+};
+
 // Every workgroup first reduces the predicted chain's slots and walks it (the same result in each;
 // workgroup 0 writes the plan).  When the chain holds, that is all: no table.  Otherwise the table
 // scan, and its last workgroup walks the levels (vs_walk).
@@ -152,8 +166,18 @@ __global__ __launch_bounds__(BS) void k_vsplit_scan(int n, const TI *x, unsigned
                                                     int it, const VsPre *pre, int rho, int cap,
                                                     int reproMode, const uint16_t *J, VsPlan *plan,
                                                     int *info, double *slices, int64_t lds, int rhoIn,
-                                                    int shift, int fallback) {
+                                                    int shift, int fallback, VsSecond<TI> v2) {
 	if (stopAt && it >= *stopAt) return;
+	if (blockIdx.y == 1) {
+		x = v2.x;
+		Fg = v2.Fg;
+		range = v2.range;
+		done = v2.done;
+		pre = v2.pre;
+		plan = v2.plan;
+		info = v2.info;
+		slices = v2.slices;
+	}
 	{
 		__shared__ unsigned long long Fp[kVsPred], mbS;
 		__shared__ int lofS, badS, needS;
@@ -527,8 +551,13 @@ __device__ inline void vs_predict(double M0, int minLo, int rho, int cap, const 
 constexpr int kVsLevBlocks = 1024;  // workgroups of the reduction kernels at most
 
 template <int BS, class TI>
-__global__ __launch_bounds__(BS) void k_vsplit_pre(int n, const TI *x, VsPre *pre, const int *stopAt, int it) {
+__global__ __launch_bounds__(BS) void k_vsplit_pre(int n, const TI *x, VsPre *pre, const int *stopAt, int it,
+                                                   VsSecond<TI> v2) {
 	if (stopAt && it >= *stopAt) return;
+	if (blockIdx.y == 1) {
+		x = v2.x;
+		pre = v2.pre;
+	}
 	double mx = 0.0;
 	int lo = 0, bad = 0;
 	const int64_t stride = (int64_t)gridDim.x * BS;
@@ -600,9 +629,14 @@ constexpr int kVsSlotWords = (int)(sizeof(VsSlot) / 8) * kVsSpread;
 template <int BS, class TI>
 __global__ __launch_bounds__(BS) void k_vsplit_levels(int n, const TI *x, VsPre *pre, VsPre *preNext, int rho,
                                                       int cap, const uint16_t *J, CgDev *S, int it,
-                                                      int clkCat) {
+                                                      int clkCat, VsSecond<TI> v2) {
 	cg_clock(S, it, clkCat);
 	if (S && it >= S->stopAt) return;
+	if (blockIdx.y == 1) {
+		x = v2.x;
+		pre = v2.pre;
+		preNext = v2.preNext;
+	}
 	if (blockIdx.x == 0)
 		for (int q = threadIdx.x; q < kVsSlotWords; q += BS) ((unsigned long long *)preNext->slot)[q] = 0ull;
 	__shared__ double red[BS / 64];
@@ -779,15 +813,35 @@ size_t vsplit_scratch_bytes() { return 8ull * kVsK + 8ull * kVsBlocks + 64 + 2 *
 // The plan of x (length n): the predicted chain, then the table scan and walk only when it does not
 // hold; with fallback the walk's workgroup also stores the slices of a plan marked bad (the
 // reference loop).  scratch: vsplit_scratch_bytes(), zeroed once (the kernels leave it zero again).
+// the pointers of a vector's split in its scratch (Fg, range, the done counter, the VsPre pair)
+template <class TI>
+static VsSecond<TI> vs_second(const TI *x, void *scratch, VsPlan *plan, int *info, double *slices) {
+	VsSecond<TI> v{};
+	v.x = x;
+	v.Fg = (unsigned long long *)scratch;
+	v.range = (int2 *)(v.Fg + kVsK);
+	v.done = (unsigned int *)(v.range + kVsBlocks);
+	v.pre = (VsPre *)((char *)scratch + 8ull * kVsK + 8ull * kVsBlocks + 64);
+	v.preNext = v.pre + 1;
+	v.plan = plan;
+	v.info = info;
+	v.slices = slices;
+	return v;
+}
+
+// With y (the DOT's second vector, its own scratch, plan, info and slices in y2) both vectors are
+// split by the same three launches (grid rows 0 and 1).
 template <class TI>
 static hipError_t vsplit_plan(hipStream_t st, int n, const TI *x, double *slices, int64_t lds, int rho,
                               int shift, int cap, int reproMode, const uint16_t *J, int *info,
                               void *scratch, VsPre *pre, VsPlan *plan, bool statsDone, CgDev *S, int it,
-                              int clkCat, int fallback, int parity) {
+                              int clkCat, int fallback, int parity, const VsSecond<TI> *y2 = nullptr) {
 	unsigned long long *Fg = (unsigned long long *)scratch;
 	int2 *range = (int2 *)(Fg + kVsK);
 	if (pre == nullptr) pre = (VsPre *)((char *)scratch + 8ull * kVsK + 8ull * kVsBlocks + 64);
 	VsPre *cur = pre + (parity & 1), *next = pre + ((parity & 1) ^ 1);
+	const VsSecond<TI> v2 = y2 ? *y2 : VsSecond<TI>{};
+	const unsigned rows = y2 ? 2u : 1u;
 	const int *stopAt = S ? &S->stopAt : nullptr;
 	int rhoEff = rho;
 	if (shift > 1) {  // sigma * 2^j acts as rho + j
@@ -799,13 +853,14 @@ static hipError_t vsplit_plan(hipStream_t st, int n, const TI *x, double *slices
 	int gridL = (int)std::min<int64_t>(kVsLevBlocks, ((int64_t)n + 4 * kVsBS - 1) / (4 * kVsBS));
 	if (gridL < 1) gridL = 1;
 	if (!statsDone)
-		hipLaunchKernelGGL((k_vsplit_pre<kVsBS, TI>), dim3(gridL), dim3(kVsBS), 0, st, n, x, cur, stopAt, it);
-	hipLaunchKernelGGL((k_vsplit_levels<kVsBS, TI>), dim3(gridL), dim3(kVsBS), 0, st, n, x, cur, next, rhoEff,
-	                   cap, J, S, it, clkCat);
+		hipLaunchKernelGGL((k_vsplit_pre<kVsBS, TI>), dim3(gridL, rows), dim3(kVsBS), 0, st, n, x, cur, stopAt, it,
+		                   v2);
+	hipLaunchKernelGGL((k_vsplit_levels<kVsBS, TI>), dim3(gridL, rows), dim3(kVsBS), 0, st, n, x, cur, next,
+	                   rhoEff, cap, J, S, it, clkCat, v2);
 	unsigned int *done = (unsigned int *)(range + kVsBlocks);
-	hipLaunchKernelGGL((k_vsplit_scan<kVsBS, TI>), dim3(grid), dim3(kVsBS), 0, st, n, x, Fg, range, done,
+	hipLaunchKernelGGL((k_vsplit_scan<kVsBS, TI>), dim3(grid, rows), dim3(kVsBS), 0, st, n, x, Fg, range, done,
 	                   stopAt, it, (const VsPre *)cur, rhoEff, cap, reproMode, J, plan, info, slices, lds, rho,
-	                   shift, fallback);
+	                   shift, fallback, v2);
 	return hipGetLastError();
 }
 
@@ -834,6 +889,27 @@ hipError_t launch_vsplit(hipStream_t st, int n, const float *x, double *slices, 
                          int shift, int cap, int reproMode, const uint16_t *JF, int *info, void *scratch,
                          VsPlan *plan) {
 	return vsplit(st, n, x, slices, lds, rho, shift, cap, reproMode, JF, info, scratch, plan);
+}
+
+template <class TI>
+static hipError_t vsplit_plan2(hipStream_t st, int n, const TI *x, const TI *y, int rho, int shift, int cap,
+                               int reproMode, const uint16_t *J, double *const slices[2], int64_t lds,
+                               int *const info[2], void *const scratch[2], VsPlan *const plan[2]) {
+	const VsSecond<TI> v2 = vs_second<TI>(y, scratch[1], plan[1], info[1], slices[1]);
+	return vsplit_plan(st, n, x, slices[0], lds, rho, shift, cap, reproMode, J, info[0], scratch[0], nullptr,
+	                   plan[0], false, nullptr, 0, kClkNone, 1, 0, &v2);
+}
+
+hipError_t launch_vsplit_plan2(hipStream_t st, int n, const double *x, const double *y, int rho, int shift,
+                               int cap, int reproMode, const uint16_t *J, double *const slices[2], int64_t lds,
+                               int *const info[2], void *const scratch[2], VsPlan *const plan[2]) {
+	return vsplit_plan2(st, n, x, y, rho, shift, cap, reproMode, J, slices, lds, info, scratch, plan);
+}
+
+hipError_t launch_vsplit_plan2(hipStream_t st, int n, const float *x, const float *y, int rho, int shift,
+                               int cap, int reproMode, const uint16_t *JF, double *const slices[2], int64_t lds,
+                               int *const info[2], void *const scratch[2], VsPlan *const plan[2]) {
+	return vsplit_plan2(st, n, x, y, rho, shift, cap, reproMode, JF, slices, lds, info, scratch, plan);
 }
 
 hipError_t launch_vsplit_plan(hipStream_t st, int n, const double *x, double *slices, int64_t lds,
