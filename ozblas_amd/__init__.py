@@ -218,8 +218,10 @@ class Handle:
         if getattr(self, "_h", None) is not None and self._h.initialized:
             _destroy(self.ref)
 
-    def __del__(self):
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # bound at definition: the module's globals (sys among them) may already be None when the
+        # interpreter tears the module down
+        if _finalizing():
             return
         try:
             self.close()

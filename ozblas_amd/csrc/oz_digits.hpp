@@ -77,8 +77,7 @@ __device__ inline void col_acc_flush(const ColAcc &a, ExpStats *st) {
 // Digit statistics of one k-contiguous column x[0..m) (whole workgroup of BS threads):
 // the column scale E (|x| <= 127 * 2^(E-8) for every element) and digit count S (every element is
 // a multiple of 2^(E - 8S)); thread 0 adds the column to `acc`.  Column j of the operand is
-// x = sum_{s<S} d_s 2^(E - 8(s+1)) with d_s in [-128, 127].  Eight loads per thread are issued
-// before they are used.
+// x = sum_{s<S} d_s 2^(E - 8(s+1)) with d_s in [-128, 127].
 template <int BS>
 __device__ inline double block_sum(double v, double *red) {
 	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -91,40 +90,92 @@ __device__ inline double block_sum(double v, double *red) {
 	return r;
 }
 
+// elementwise part of the column statistics: running max |x|, max of -(lowest set bit), non-finite
+__device__ __forceinline__ void col_stat_elem(double v, double &mx, int &nlo, int &bad) {
+	if (!isfinite(v)) {
+		bad = 1;
+		return;
+	}
+	if (v == 0.0) return;
+	mx = std_max(mx, fabs(v));
+	nlo = max(nlo, -lsb_exp(v));
+}
+
+// the three workgroup reductions of the column statistics behind one barrier pair (red holds
+// 2 BS / 64 ints, redd BS / 64 doubles); every thread gets the results
+template <int BS>
+__device__ inline void block_max3(double &mx, int &nlo, int &bad, int *red, double *redd) {
+	for (int o = 32; o > 0; o >>= 1) {
+		mx = std_max(mx, __shfl_xor(mx, o, 64));
+		nlo = max(nlo, __shfl_xor(nlo, o, 64));
+		bad = max(bad, __shfl_xor(bad, o, 64));
+	}
+	const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+	__syncthreads();
+	if (l == 0) {
+		redd[w] = mx;
+		red[w] = nlo;
+		red[BS / 64 + w] = bad;
+	}
+	__syncthreads();
+	mx = redd[0];
+	nlo = red[0];
+	bad = red[BS / 64];
+	for (int i = 1; i < BS / 64; i++) {
+		mx = std_max(mx, redd[i]);
+		nlo = max(nlo, red[i]);
+		bad = max(bad, red[BS / 64 + i]);
+	}
+}
+
+// Columns of up to kColReg * BS elements are loaded once, all loads issued before the first use, and
+// kept in registers for the norm pass; longer ones are read in batches of 8 per thread and read
+// again (from L2) for the norm.  `red` holds 2 BS / 64 ints.
+constexpr int kColReg = 32;
 template <int BS>
 __device__ inline void digit_col_stats_acc(const double *x, int m, int *colE, int *colS, ColAcc &acc,
                                            int *red, double *redd, int *colLo = nullptr) {
 	int nlo = INT32_MIN, bad = 0;  // nlo = -(min lsb)
 	double mx = 0.0;
-	for (int i0 = threadIdx.x; i0 < m; i0 += 8 * BS) {
-		double v[8];
+	const bool inreg = m <= kColReg * BS;
+	double v[kColReg];
+	if (inreg) {
 #pragma unroll
-		for (int u = 0; u < 8; u++) v[u] = (i0 + u * BS < m) ? x[i0 + u * BS] : 0.0;
+		for (int u = 0; u < kColReg; u++) {
+			const int i = threadIdx.x + u * BS;
+			v[u] = (i < m) ? x[i] : 0.0;
+		}
 #pragma unroll
-		for (int u = 0; u < 8; u++) {
-			if (!isfinite(v[u])) {
-				bad = 1;
-				continue;
-			}
-			if (v[u] == 0.0) continue;
-			mx = std_max(mx, fabs(v[u]));
-			nlo = max(nlo, -lsb_exp(v[u]));
+		for (int u = 0; u < kColReg; u++) col_stat_elem(v[u], mx, nlo, bad);
+	} else {
+		for (int i0 = threadIdx.x; i0 < m; i0 += 8 * BS) {
+			double b[8];
+#pragma unroll
+			for (int u = 0; u < 8; u++) b[u] = (i0 + u * BS < m) ? x[i0 + u * BS] : 0.0;
+#pragma unroll
+			for (int u = 0; u < 8; u++) col_stat_elem(b[u], mx, nlo, bad);
 		}
 	}
-	mx = block_max<BS>(mx, redd);
-	nlo = block_max_i<BS>(nlo, red);
-	bad = block_max_i<BS>(bad, red);
+	block_max3<BS>(mx, nlo, bad, red, redd);
 	// residue form: an upper bound on log2 of the Euclidean norm of the column's integers x 2^-lo
 	// (Cauchy-Schwarz bounds |X| by the product of two such norms).  The squares are summed as
-	// (x 2^-hi)^2 <= 1, in a second pass over the column (in L2); terms that underflow are covered by
-	// the m 2^-1000 added, the rounding of the m-term sum by the factor 1 + m 2^-52.
+	// (x 2^-hi)^2 <= 1, in a second pass over the column (registers or L2); terms that underflow are
+	// covered by the m 2^-1000 added, the rounding of the m-term sum by the factor 1 + m 2^-52.
 	double ss = 0.0;
 	int hiN = 0;
 	if (colLo && mx > 0.0 && !bad) {
 		frexp(mx, &hiN);
-		for (int i = threadIdx.x; i < m; i += BS) {
-			const double v = ldexp(x[i], -hiN);
-			ss += v * v;
+		if (inreg) {  // the same order as the loop below; the zeros past m add nothing
+#pragma unroll
+			for (int u = 0; u < kColReg; u++) {
+				const double t = ldexp(v[u], -hiN);
+				ss += t * t;
+			}
+		} else {
+			for (int i = threadIdx.x; i < m; i += BS) {
+				const double t = ldexp(x[i], -hiN);
+				ss += t * t;
+			}
 		}
 		ss = block_sum<BS>(ss, redd);
 	}
@@ -168,19 +219,21 @@ __device__ inline void digit_col_stats_wave(const double *x, int m, int *colE, i
 	const int l = threadIdx.x % 64;
 	int nlo = INT32_MIN, bad = 0;
 	double mx = 0.0;
-	for (int i0 = l; i0 < m; i0 += 8 * 64) {
-		double v[8];
+	constexpr int RW = 16;  // columns of up to RW * 64 elements stay in registers for the norm pass
+	const bool inreg = m <= RW * 64;
+	double v[RW];
+	if (inreg) {
 #pragma unroll
-		for (int u = 0; u < 8; u++) v[u] = (i0 + u * 64 < m) ? x[i0 + u * 64] : 0.0;
+		for (int u = 0; u < RW; u++) v[u] = (l + u * 64 < m) ? x[l + u * 64] : 0.0;
 #pragma unroll
-		for (int u = 0; u < 8; u++) {
-			if (!isfinite(v[u])) {
-				bad = 1;
-				continue;
-			}
-			if (v[u] == 0.0) continue;
-			mx = std_max(mx, fabs(v[u]));
-			nlo = max(nlo, -lsb_exp(v[u]));
+		for (int u = 0; u < RW; u++) col_stat_elem(v[u], mx, nlo, bad);
+	} else {
+		for (int i0 = l; i0 < m; i0 += 8 * 64) {
+			double b[8];
+#pragma unroll
+			for (int u = 0; u < 8; u++) b[u] = (i0 + u * 64 < m) ? x[i0 + u * 64] : 0.0;
+#pragma unroll
+			for (int u = 0; u < 8; u++) col_stat_elem(b[u], mx, nlo, bad);
 		}
 	}
 	mx = wave_max_nonneg_f64(mx);
@@ -190,9 +243,17 @@ __device__ inline void digit_col_stats_wave(const double *x, int m, int *colE, i
 	int hiN = 0;
 	if (colLo && mx > 0.0 && !bad) {
 		frexp(mx, &hiN);
-		for (int i = l; i < m; i += 64) {
-			const double v = ldexp(x[i], -hiN);
-			ss += v * v;
+		if (inreg) {
+#pragma unroll
+			for (int u = 0; u < RW; u++) {
+				const double t = ldexp(v[u], -hiN);
+				ss += t * t;
+			}
+		} else {
+			for (int i = l; i < m; i += 64) {
+				const double t = ldexp(x[i], -hiN);
+				ss += t * t;
+			}
 		}
 		ss = wave_sum_f64(ss);
 	}

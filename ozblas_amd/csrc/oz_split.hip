@@ -273,7 +273,7 @@ __global__ void k_zero_slots_csr(int m, const int *rowptr, double *split, int64_
 template <int BS>
 __global__ __launch_bounds__(BS) void k_digit_stats(int m, int n, const double *in, int64_t ldi,
                                                     int *colE, int *colS, ExpStats *st, int *colLo) {
-	__shared__ int red[BS / 64];
+	__shared__ int red[2 * BS / 64];
 	__shared__ double redd[BS / 64];
 	ColAcc acc;
 	for (int j = blockIdx.x; j < n; j += gridDim.x)
@@ -439,7 +439,7 @@ void launch_zero_slots_csr(hipStream_t st, int m, const int *rowptr, double *spl
 
 void launch_digit_stats(hipStream_t st, int m, int n, const double *in, int64_t ldi, int *colE,
                         int *colS, void *stats, int *colLo) {
-	const int grid = n < 1024 ? n : 1024;  // columns per workgroup share its few counter atomics
+	const int grid = n < 2048 ? n : 2048;  // columns per workgroup share its few counter atomics
 	hipLaunchKernelGGL(k_digit_stats<256>, dim3(grid), dim3(256), 0, st, m, n, in, ldi, colE, colS,
 	                   (ExpStats *)stats, colLo);
 }
