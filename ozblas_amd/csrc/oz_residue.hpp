@@ -122,11 +122,34 @@ __device__ __forceinline__ int mod_f(int v, int p, float invpf) {
 // K = rint(sum_t c_t / p_t), and the chunks minus K M's are carried into NW 64-bit words.  The words
 // of 4 moduli are loaded together (clamped indices; the surplus ones are not used).
 // Rw addresses the word in plane 0; plane b is rstride bytes further.
+// an integer-valued binary64 with |d| < 2^51 as int64: d + 1.5 2^52 is exact and its fraction
+// field holds d + 2^51
+__device__ __forceinline__ int64_t f64_to_i64_exact(double d) {
+	return (int64_t)(d2u(d + 0x1.8p52) - 0x4338000000000000ULL);
+}
+
+// a += v 2^sh (two's complement) when the bits of a at and above sh are all zero: no carries
+template <int NW>
+__device__ __forceinline__ void big_or_shifted(BigInt<NW> &a, int64_t v, int sh) {
+	const int wi = sh >> 6, b = sh & 63;
+	const uint64_t ext = (v < 0) ? ~0ULL : 0ULL;
+	const uint64_t lo = (uint64_t)v << b;
+	const uint64_t hi = b ? (uint64_t)(v >> (64 - b)) : ext;
+#pragma unroll
+	for (int q = 0; q < NW; q++) {
+		if (q == wi)
+			a.w[q] |= lo;
+		else if (q == wi + 1)
+			a.w[q] |= hi;
+		else if (q > wi + 1)
+			a.w[q] |= ext;
+	}
+}
+
 template <int NW>
 __device__ __forceinline__ void res_group_values(const uint8_t *Rw, int64_t rstride, int nchunk,
                                         const ResConst *rc, int nmod, const int *e0, double *t) {
 	constexpr int NC = (64 * NW + kResChunkBits - 1) / kResChunkBits;
-	constexpr int QB = 4;
 	double A[4][NC];
 	double S[4];
 #pragma unroll
@@ -135,54 +158,99 @@ __device__ __forceinline__ void res_group_values(const uint8_t *Rw, int64_t rstr
 #pragma unroll
 		for (int c = 0; c < NC; c++) A[e][c] = 0.0;
 	}
-	for (int q0 = 0; q0 < nmod; q0 += QB) {
-		int rs[QB][4];
+	// modulus q's residues ct[e] of the four elements enter the sums
+	auto absorb = [&](int q, const double (&ct)[4]) {
+		const double ipd = rc->invpd[q];
+		double mp[NC];
 #pragma unroll
-		for (int j = 0; j < QB; j++)
+		for (int c = 0; c < NC; c++) mp[c] = rc->Mpd[q][c];
 #pragma unroll
-			for (int e = 0; e < 4; e++) rs[j][e] = 0;
-		for (int c = 0; c < nchunk; c++) {
-			uint32_t w[QB];
+		for (int e = 0; e < 4; e++) {
+			S[e] = fma(ct[e], ipd, S[e]);
+#pragma unroll
+			for (int c = 0; c < NC; c++) A[e][c] = fma(ct[e], mp[c], A[e][c]);
+		}
+	};
+	if (nchunk == 1) {
+		// one k chunk: the words of QB moduli are loaded while the previous QB are absorbed, so
+		// 2 QB loads per thread are in flight
+		constexpr int QB = 8;
+		uint32_t w[QB];
+		auto load = [&](uint32_t(&d)[QB], int q0) {
+#pragma unroll
+			for (int j = 0; j < QB; j++) d[j] = *(const uint32_t *)(Rw + (int64_t)min(q0 + j, nmod - 1) * rstride);
+		};
+		load(w, 0);
+		for (int q0 = 0; q0 < nmod; q0 += QB) {
+			uint32_t wn[QB];
+			load(wn, min(q0 + QB, nmod - 1));
 #pragma unroll
 			for (int j = 0; j < QB; j++) {
-				const int q = min(q0 + j, nmod - 1);
-				w[j] = *(const uint32_t *)(Rw + (int64_t)(c * nmod + q) * rstride);
+				if (q0 + j >= nmod) break;
+				double ct[4];
+#pragma unroll
+				for (int e = 0; e < 4; e++) ct[e] = (double)((w[j] >> (8 * e)) & 255);
+				absorb(q0 + j, ct);
 			}
+#pragma unroll
+			for (int j = 0; j < QB; j++) w[j] = wn[j];
+		}
+	} else {
+		// several k chunks: the chunks' residues of a modulus are summed (below 2^24) and reduced
+		constexpr int QB = 4;
+		for (int q0 = 0; q0 < nmod; q0 += QB) {
+			int rs[QB][4];
 #pragma unroll
 			for (int j = 0; j < QB; j++)
 #pragma unroll
-				for (int e = 0; e < 4; e++) rs[j][e] += (int)((w[j] >> (8 * e)) & 255);
-		}
+				for (int e = 0; e < 4; e++) rs[j][e] = 0;
+			for (int c = 0; c < nchunk; c++) {
+				uint32_t w[QB];
 #pragma unroll
-		for (int j = 0; j < QB; j++) {
-			const int q = q0 + j;
-			if (q >= nmod) break;
-			const int p = rc->p[q];
-			const float ipf = rc->invpf[q];
-			const double ipd = rc->invpd[q];
-			double mp[NC];
+				for (int j = 0; j < QB; j++) {
+					const int q = min(q0 + j, nmod - 1);
+					w[j] = *(const uint32_t *)(Rw + (int64_t)(c * nmod + q) * rstride);
+				}
 #pragma unroll
-			for (int c = 0; c < NC; c++) mp[c] = rc->Mpd[q][c];
+				for (int j = 0; j < QB; j++)
 #pragma unroll
-			for (int e = 0; e < 4; e++) {
-				const double ct = (double)((nchunk > 1) ? mod_f(rs[j][e], p, ipf) : rs[j][e]);  // rs < 2^24
-				S[e] = fma(ct, ipd, S[e]);
+					for (int e = 0; e < 4; e++) rs[j][e] += (int)((w[j] >> (8 * e)) & 255);
+			}
 #pragma unroll
-				for (int c = 0; c < NC; c++) A[e][c] = fma(ct, mp[c], A[e][c]);
+			for (int j = 0; j < QB; j++) {
+				const int q = q0 + j;
+				if (q >= nmod) break;
+				const int p = rc->p[q];
+				const float ipf = rc->invpf[q];
+				double ct[4];
+#pragma unroll
+				for (int e = 0; e < 4; e++) ct[e] = (double)mod_f(rs[j][e], p, ipf);  // rs < 2^24
+				absorb(q, ct);
 			}
 		}
 	}
 #pragma unroll
 	for (int e = 0; e < 4; e++) {
 		const double K = rint(S[e]);
+		// chunk c of X is d_c = A_c - K M_c, exact: 0 <= A_c < 49 * 255 * 2^39 < 2^52.61 and
+		// |K M_c| < 2^45.  The carries run in binary64: hi = floor(d_c 2^-39) (at most 2^13.61 in
+		// magnitude) moves to d_(c+1), which stays below 2^53, and leaves d_c in [0, 2^39); every step
+		// is exact.  The top chunk carries the sign, |d_top| < 2^51 because |X| < 2^(64 NW - 1).
+		double d[NC];
+#pragma unroll
+		for (int c = 0; c < NC; c++) d[c] = fma(-K, rc->Md[c], A[e][c]);
+#pragma unroll
+		for (int c = 0; c + 1 < NC; c++) {
+			const double hi = floor(d[c] * 0x1p-39);
+			d[c] = fma(-hi, 0x1p39, d[c]);
+			d[c + 1] += hi;
+		}
+		// the chunks occupy disjoint bit ranges: X is their OR
 		BigInt<NW> X;
 #pragma unroll
 		for (int q = 0; q < NW; q++) X.w[q] = 0;
 #pragma unroll
-		for (int c = 0; c < NC; c++) {
-			const double d = fma(-K, rc->Md[c], A[e][c]);  // exact: |K M_c| < 2^45, |d| < 2^53
-			big_add_shifted<NW>(X, (int64_t)d, kResChunkBits * c);
-		}
+		for (int c = 0; c < NC; c++) big_or_shifted<NW>(X, f64_to_i64_exact(d[c]), kResChunkBits * c);
 		t[e] = big_finish<NW>(X, e0[e]);
 	}
 }

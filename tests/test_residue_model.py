@@ -217,7 +217,8 @@ def test_element_decomposition_against_integers():
 
 def reconstruct(c, nmod, X):
     """res_group_values restated: c_t = X u_t mod p_t, the binary64 chunk sums of c_t M/p_t (each
-    checked exact), K = rint of the fma sum of c_t / p_t, the chunks minus K M's, carried."""
+    checked exact), K = rint of the fma sum of c_t / p_t, the chunks minus K M's, carried in binary64
+    (each carry checked exact) so that all chunks but the top one lie in [0, 2^39)."""
     ps = [int(v) for v in c["p"][:nmod]]
     S = 0.0
     A = [0.0] * MAXCHUNK
@@ -229,11 +230,24 @@ def reconstruct(c, nmod, X):
             A[q] = fma_d(ct, c["Mpd"][t][q], A[q])
             assert A[q] == int(A[q]) and A[q] < 2.0 ** 53
     K = float(np.rint(S))
+    nc = MAXCHUNK
+    while nc > 1 and c["Md"][nc - 1] == 0.0 and all(c["Mpd"][t][nc - 1] == 0.0 for t in range(nmod)):
+        nc -= 1
+    nc += 1  # one chunk above M's: it takes the sign and the last carry
+    nc = min(nc, MAXCHUNK)
+    d = [fma_d(-K, c["Md"][q], A[q]) for q in range(nc)]
+    for q in range(nc - 1):  # the carries in binary64, each step checked exact
+        assert d[q] == int(d[q]) and abs(d[q]) < 2.0 ** 53
+        hi = math.floor(d[q] * 2.0 ** -CHUNK)
+        r = fma_d(-hi, 2.0 ** CHUNK, d[q])
+        assert 0 <= r < 2.0 ** CHUNK and r == d[q] - hi * 2.0 ** CHUNK
+        nxt = d[q + 1] + hi
+        assert nxt == int(d[q + 1]) + hi and abs(nxt) < 2.0 ** 53
+        d[q], d[q + 1] = r, nxt
+    assert abs(d[nc - 1]) < 2.0 ** 51  # the top chunk's integer conversion
     out = 0
-    for q in range(MAXCHUNK):
-        d = fma_d(-K, c["Md"][q], A[q])
-        assert d == int(d) and abs(d) < 2.0 ** 53
-        out += int(d) << (CHUNK * q)
+    for q in range(nc):  # disjoint bit ranges: the sum is the OR of the chunks
+        out += int(d[q]) << (CHUNK * q)
     return out
 
 
