@@ -114,14 +114,6 @@ __device__ __forceinline__ int mod_f(int v, int p, float invpf) {
 	return r;
 }
 
-// The four elements (i, 4g .. 4g + 3) of one product-residue word: X of each from its residues
-// (nmod moduli, nchunk k-chunk planes per modulus summed modulo p), then t[e] = RN(X 2^e0[e]).
-// The planes hold c_t = X u_t mod p_t < 2^8 (op(A) was written with the factors u_t);
-// X = sum_t c_t M/p_t - K M is summed in binary64 chunks of 39 bits
-// (every product c_t chunk < 2^47, every chunk sum of up to 49 moduli < 2^53: exact in any order),
-// K = rint(sum_t c_t / p_t), and the chunks minus K M's are carried into NW 64-bit words.  The words
-// of 4 moduli are loaded together (clamped indices; the surplus ones are not used).
-// Rw addresses the word in plane 0; plane b is rstride bytes further.
 // an integer-valued binary64 with |d| < 2^51 as int64: d + 1.5 2^52 is exact and its fraction
 // field holds d + 2^51
 __device__ __forceinline__ int64_t f64_to_i64_exact(double d) {
@@ -146,6 +138,14 @@ __device__ __forceinline__ void big_or_shifted(BigInt<NW> &a, int64_t v, int sh)
 	}
 }
 
+// The four elements (i, 4g .. 4g + 3) of one product-residue word: X of each from its residues
+// (nmod moduli, nchunk k-chunk planes per modulus summed modulo p), then t[e] = RN(X 2^e0[e]).
+// The planes hold c_t = X u_t mod p_t < 2^8 (op(A) was written with the factors u_t);
+// X = sum_t c_t M/p_t - K M is summed in binary64 chunks of 39 bits
+// (every product c_t chunk < 2^47, every chunk sum of up to 49 moduli < 2^53: exact in any order),
+// K = rint(sum_t c_t / p_t), and the chunks minus K M's are carried into NW 64-bit words.  The words
+// of a block of moduli are loaded together (clamped indices; the surplus ones are not used).
+// Rw addresses the word in plane 0; plane b is rstride bytes further.
 template <int NW>
 __device__ __forceinline__ void res_group_values(const uint8_t *Rw, int64_t rstride, int nchunk,
                                         const ResConst *rc, int nmod, const int *e0, double *t) {
