@@ -71,6 +71,31 @@ def test_dot_split_edge_cases_bitwise(oz, oracle, sumMode):
             assert (H.nSplitA, H.nSplitB) == (h.nSplitA, h.nSplitB), name
 
 
+def test_dot_sequence_after_inputs_outside_model_same_handle(oz, oracle):
+    """One handle splits vectors outside the residue model (an element near 2^1023, alone or among
+    thousands, so that one workgroup of several is outside it) between ordinary ones: every DOT and
+    its slice counts equal the oracle's, so no workgroup table of an earlier split leaks into a later
+    one."""
+    rng = np.random.default_rng(41)
+    big = rng.uniform(-1, 1, 5000)
+    big[2500] = 1.25 * 2.0 ** 1022
+    wide = rng.standard_normal(4000) * np.exp2(rng.integers(-200, 200, 4000).astype(float))
+    seq = [np.array([1.5 * 2.0 ** 1022, -3.0, 1.0, 2.0 ** 1000]), rng.uniform(-1, 1, 3000), big, wide,
+           big, rng.uniform(-1, 1, 70000)]
+    for sumMode in (0, 1):
+        H = oz.Handle(1 << 28, sumMode=sumMode)
+        for q, x in enumerate(seq):
+            n = len(x)
+            y = np.random.default_rng(n + q).uniform(-1, 1, n)
+            h = oracle.Handle(1 << 30, sumMode=sumMode)
+            want = h.dot(n, x, y)
+            say(f"ddot sequence item {q} n={n} sumMode={sumMode}")
+            got = H.ddot(n, dev(x), dev(y))
+            same = np.float64(got).tobytes() == np.float64(want).tobytes()
+            assert same or (np.isnan(got) and np.isnan(want)), (q, got, want)  # the split overflows
+            assert (H.nSplitA, H.nSplitB) == (h.nSplitA, h.nSplitB), q
+
+
 @pytest.mark.parametrize("nSplitMax", [0, 3])
 def test_dot_split_range_over_512_binades_bitwise(oz, oracle, nSplitMax):
     """A vector whose exponents span about 600 binades: the split's residue table then holds more
