@@ -375,8 +375,8 @@ static int32_t launch_execute(BatchPlan *p, hipStream_t st) {
 		hipLaunchKernelGGL(k_check_grouped, dim3((p->nCheck + 255) / 256), dim3(256), 0, st, p->dCheck,
 		                   p->nCheck, p->dSt, p->dFlag);
 	if (p->nWriteBlocks > 0)
-		hipLaunchKernelGGL(k_write_grouped, dim3(p->nWriteBlocks), dim3(256), 0, st, p->dWrite,
-		                   p->dWriteStart, p->nWrite);
+		hipLaunchKernelGGL(k_write_grouped, dim3(p->nWriteBlocks), dim3(256),
+		                   res_lds_bytes(kResMaxMod, kResMaxShift), st, p->dWrite, p->dWriteStart, p->nWrite);
 	const int c = p->tcfg;
 	if (p->nItems[c] > 0) {
 		if (c == 0)
@@ -826,8 +826,8 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 	}
 	// ---- autotune: digit planes once, then one grouped launch per tile shape -------------------
 	if (p->nWriteBlocks > 0)
-		hipLaunchKernelGGL(k_write_grouped, dim3(p->nWriteBlocks), dim3(256), 0, st, p->dWrite,
-		                   p->dWriteStart, p->nWrite);
+		hipLaunchKernelGGL(k_write_grouped, dim3(p->nWriteBlocks), dim3(256),
+		                   res_lds_bytes(kResMaxMod, kResMaxShift), st, p->dWrite, p->dWriteStart, p->nWrite);
 	p->tcfg = 0;
 	if (p->nItems[0] > 0) {
 		for (int c = 0; c < 2; c++) {

@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void k_res_write_h(int m, int n, const double 
 		for (int i = threadIdx.x; i < ldp; i += blockDim.x) {
 			const ResElem e = res_elem(i < m ? x[i] : 0.0, lo);
 			for (int gi = 0; gi < ng; gi++) {
-				const double rx = res_group(e, L.Pg[gi], L.invPg[gi], L.pw2[gi]);
+				const double rx = res_group(e, L.Pg[gi], L.invPg[gi], resPw2 + gi * shLimit);
 				const int s1 = min(3 * gi + 3, nmod);
 				for (int s = 3 * gi; s < s1; s++)
 					planes[(int64_t)s * pstride + (int64_t)j * ldp + i] = digit_cast<T>(res_member(rx, L.pd[s], L.ipd[s]));
@@ -215,17 +215,18 @@ void launch_res_write(hipStream_t st, int elem, int m, int n, const double *in, 
 	if (elem == 1) {  // tiled layout; ldp % 128 == 0 and pstride = ldp * (n rounded up to 256)
 		const int ktiles = (int)(ldp / 64);
 		const int gx = ktiles < 16 ? ktiles : 16;
-		hipLaunchKernelGGL(k_res_write_i8t, dim3((unsigned)gx, (unsigned)(pstride / ldp / 16)), dim3(256), 0,
-		                   st, m, n, in, ldi, colLo, rc, nmod, shLimit, scaleU, (int8_t *)planes, ldp, pstride);
+		hipLaunchKernelGGL(k_res_write_i8t, dim3((unsigned)gx, (unsigned)(pstride / ldp / 16)), dim3(256),
+		                   res_lds_bytes(nmod, shLimit), st, m, n, in, ldi, colLo, rc, nmod, shLimit, scaleU,
+		                   (int8_t *)planes, ldp, pstride);
 		return;
 	}
 	const int g = n < 2048 ? n : 2048;
 	if (elem == 2)
-		hipLaunchKernelGGL(k_res_write_h<_Float16>, dim3(g), dim3(256), 0, st, m, n, in, ldi, colLo, rc, nmod,
-		                   shLimit, scaleU, (_Float16 *)planes, ldp, pstride);
+		hipLaunchKernelGGL(k_res_write_h<_Float16>, dim3(g), dim3(256), res_lds_bytes(nmod, shLimit), st, m, n,
+		                   in, ldi, colLo, rc, nmod, shLimit, scaleU, (_Float16 *)planes, ldp, pstride);
 	else
-		hipLaunchKernelGGL(k_res_write_h<__bf16>, dim3(g), dim3(256), 0, st, m, n, in, ldi, colLo, rc, nmod,
-		                   shLimit, scaleU, (__bf16 *)planes, ldp, pstride);
+		hipLaunchKernelGGL(k_res_write_h<__bf16>, dim3(g), dim3(256), res_lds_bytes(nmod, shLimit), st, m, n,
+		                   in, ldi, colLo, rc, nmod, shLimit, scaleU, (__bf16 *)planes, ldp, pstride);
 }
 
 template <int NW>

@@ -49,16 +49,24 @@ __device__ __forceinline__ int res_member(double rx, double p, double invp) {
 // the per-group and per-modulus tables of the residue writers, staged in LDS by the whole workgroup
 // (2^sh mod P_g only for sh below the operand's span)
 struct ResLds {
-	uint32_t pw2[kResMaxGroups][kResMaxShift];
 	double Pg[kResMaxGroups], invPg[kResMaxGroups];
 	double pd[kResMaxMod], ipd[kResMaxMod];  // p and RN(1/p)
+	int shs;  // row length of the shift table
 };
+// the shift table 2^sh mod P_g (sh below the operand's span, `shs` per group row) in the kernel's
+// dynamic LDS: res_lds_bytes(nmod, shLimit) of it, so that a narrow span leaves room for more
+// workgroups per CU
+extern __shared__ uint32_t resPw2[];
+__host__ __device__ inline size_t res_lds_bytes(int nmod, int shLimit) {
+	return sizeof(uint32_t) * (size_t)((nmod + 2) / 3) * (size_t)shLimit;
+}
 
 // scaleU: the tables of op(A), whose residues carry the factors u_t (ResConst::pow2gU)
 __device__ inline void res_lds_load(ResLds &T, const ResConst *rc, int nmod, int shLimit, int scaleU) {
 	const int ng = (nmod + 2) / 3;
 	const uint32_t(*src)[kResMaxShift] = scaleU ? rc->pow2gU : rc->pow2g;
-	for (int q = threadIdx.x; q < ng * shLimit; q += blockDim.x) T.pw2[q / shLimit][q % shLimit] = src[q / shLimit][q % shLimit];
+	for (int q = threadIdx.x; q < ng * shLimit; q += blockDim.x) resPw2[q] = src[q / shLimit][q % shLimit];
+	if (threadIdx.x == 0) T.shs = shLimit;
 	for (int q = threadIdx.x; q < ng; q += blockDim.x) {
 		T.Pg[q] = rc->Pg[q];
 		T.invPg[q] = rc->invPg[q];
@@ -89,10 +97,11 @@ __device__ inline void res_write_tile_i8(int m, int n, const double *in, int64_t
 	for (int u = 0; u < 4; u++) e[u] = res_elem((j < n && q0 + u < m) ? x[q0 + u] : 0.0, lo);
 	int8_t *dst = planes + (int64_t)jb * 16 * ldp + (int64_t)kt * 1024 + 4 * t;
 	const int ng = (nmod + 2) / 3;
+	const int shs = T.shs;
 	for (int gi = 0; gi < ng; gi++) {
 		double rx[4];
 #pragma unroll
-		for (int u = 0; u < 4; u++) rx[u] = res_group(e[u], T.Pg[gi], T.invPg[gi], T.pw2[gi]);
+		for (int u = 0; u < 4; u++) rx[u] = res_group(e[u], T.Pg[gi], T.invPg[gi], resPw2 + gi * shs);
 		const int s1 = min(3 * gi + 3, nmod);
 		for (int s = 3 * gi; s < s1; s++) {
 			uint32_t word = 0;
