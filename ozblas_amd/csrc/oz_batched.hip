@@ -317,6 +317,7 @@ struct BatchPlan {
 	std::vector<int> fast;      // problems in the grouped launch
 	std::vector<int> fallback;  // problems computed by ozblasDgemm one by one
 	int nWrite = 0, nWriteBlocks = 0, nRec = 0, nRecBlocks = 0, nCheck = 0;
+	size_t writeLds = 0;  // dynamic LDS of k_write_grouped: the largest operand's shift table
 	GWrite *dWrite = nullptr;
 	int *dWriteStart = nullptr;
 	GProb *dProb[2] = {nullptr, nullptr};
@@ -375,8 +376,8 @@ static int32_t launch_execute(BatchPlan *p, hipStream_t st) {
 		hipLaunchKernelGGL(k_check_grouped, dim3((p->nCheck + 255) / 256), dim3(256), 0, st, p->dCheck,
 		                   p->nCheck, p->dSt, p->dFlag);
 	if (p->nWriteBlocks > 0)
-		hipLaunchKernelGGL(k_write_grouped, dim3(p->nWriteBlocks), dim3(256),
-		                   res_lds_bytes(kResMaxMod, kResMaxShift), st, p->dWrite, p->dWriteStart, p->nWrite);
+		hipLaunchKernelGGL(k_write_grouped, dim3(p->nWriteBlocks), dim3(256), p->writeLds, st, p->dWrite,
+		                   p->dWriteStart, p->nWrite);
 	const int c = p->tcfg;
 	if (p->nItems[c] > 0) {
 		if (c == 0)
@@ -642,6 +643,7 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 	for (int nm : usedNmod) rcOff[nm] = L2.take(sizeof(ResConst));
 	// tables
 	std::vector<GWrite> hWrite;
+	size_t writeLds = 0;
 	std::vector<int> writeStart;
 	std::vector<GRec> hRec;
 	std::vector<int> recStart, recClass;
@@ -694,6 +696,7 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 				w.nmod = h.nmod;
 				w.kts = (int)(h.ldp / 64);
 				w.shLimit = std::max(1, p->hSt[o].maxSpan);
+				writeLds = std::max(writeLds, res_lds_bytes(w.nmod, w.shLimit));
 				w.scaleU = side == 0 ? 1 : 0;
 				w.rcw = (const ResConst *)(M2 + rcOff[h.nmod]);
 				writeStart.push_back(wb);
@@ -775,6 +778,7 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 	}
 	p->nWrite = (int)hWrite.size();
 	p->nWriteBlocks = wb;
+	p->writeLds = writeLds;
 	p->dWrite = (GWrite *)(M2 + writeOff);
 	p->dWriteStart = (int *)(M2 + writeStartOff);
 	{  // the reconstruction tables in class order, block starts relative to each class
@@ -826,8 +830,8 @@ static int32_t plan_build(ozblasHandle_t *oh, BatchPlan *p, int batch, const cha
 	}
 	// ---- autotune: digit planes once, then one grouped launch per tile shape -------------------
 	if (p->nWriteBlocks > 0)
-		hipLaunchKernelGGL(k_write_grouped, dim3(p->nWriteBlocks), dim3(256),
-		                   res_lds_bytes(kResMaxMod, kResMaxShift), st, p->dWrite, p->dWriteStart, p->nWrite);
+		hipLaunchKernelGGL(k_write_grouped, dim3(p->nWriteBlocks), dim3(256), p->writeLds, st, p->dWrite,
+		                   p->dWriteStart, p->nWrite);
 	p->tcfg = 0;
 	if (p->nItems[0] > 0) {
 		for (int c = 0; c < 2; c++) {
